@@ -1,0 +1,780 @@
+// C-ABI entry points (include/pp_hip.h) and the multigrid driver.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "pp_kernels.h"
+#include "pp_setup.h"
+
+using namespace pp;
+
+#define HIPCHK(x)                                                                 \
+  do {                                                                            \
+    hipError_t e_ = (x);                                                          \
+    if (e_ != hipSuccess) {                                                       \
+      set_error("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return PP_HIP_ERROR;                                                        \
+    }                                                                             \
+  } while (0)
+
+namespace {
+
+// ---------------------------------------------------------------------
+// launch geometry
+// ---------------------------------------------------------------------
+struct CellLaunch {
+  dim3 grid, block;
+};
+CellLaunch cell_launch(int cx, int cy, int cz) {
+  int bx = std::min(256, ((cx + 63) / 64) * 64);
+  int by = 256 / bx;
+  CellLaunch c;
+  c.block = dim3(bx, by, 1);
+  c.grid = dim3((cx + bx - 1) / bx, (cy + by - 1) / by, cz);
+  return c;
+}
+
+template <typename T>
+LevelArgs<T> largs(const Level &L);
+template <>
+LevelArgs<float> largs<float>(const Level &L) {
+  LevelArgs<float> a;
+  a.cx = L.cx; a.cy = L.cy; a.cz = L.cz; a.sx = L.sx; a.sxy = L.sxy; a.fl = L.flags;
+  a.cmx = L.cm32[0]; a.cpx = L.cp32[0]; a.cmy = L.cm32[1]; a.cpy = L.cp32[1]; a.cmz = L.cm32[2]; a.cpz = L.cp32[2];
+  a.wx = L.w32[0]; a.wy = L.w32[1]; a.wz = L.w32[2];
+  return a;
+}
+template <>
+LevelArgs<double> largs<double>(const Level &L) {
+  LevelArgs<double> a;
+  a.cx = L.cx; a.cy = L.cy; a.cz = L.cz; a.sx = L.sx; a.sxy = L.sxy; a.fl = L.flags;
+  a.cmx = L.cm64[0]; a.cpx = L.cp64[0]; a.cmy = L.cm64[1]; a.cpy = L.cp64[1]; a.cmz = L.cm64[2]; a.cpz = L.cp64[2];
+  a.wx = L.w64[0]; a.wy = L.w64[1]; a.wz = L.w64[2];
+  return a;
+}
+template <typename T>
+T *lv_e(Level &L);
+template <>
+float *lv_e<float>(Level &L) { return L.e32; }
+template <>
+double *lv_e<double>(Level &L) { return L.e64; }
+template <typename T>
+T *lv_rhs(Level &L);
+template <>
+float *lv_rhs<float>(Level &L) { return L.rhs32; }
+template <>
+double *lv_rhs<double>(Level &L) { return L.rhs64; }
+template <typename T>
+T *lv_res(Level &L);
+template <>
+float *lv_res<float>(Level &L) { return L.res32; }
+template <>
+double *lv_res<double>(Level &L) { return L.res64; }
+template <typename T>
+ProlongDir<T> pdir(const Level &L, int d);
+template <>
+ProlongDir<float> pdir<float>(const Level &L, int d) { return {L.par_d[d], L.oth_d[d], L.wpar32[d]}; }
+template <>
+ProlongDir<double> pdir<double>(const Level &L, int d) { return {L.par_d[d], L.oth_d[d], L.wpar64[d]}; }
+template <typename T>
+const T *ghost_coef(const GhostTables &g);
+template <>
+const float *ghost_coef<float>(const GhostTables &g) { return g.coef32; }
+template <>
+const double *ghost_coef<double>(const GhostTables &g) { return g.coef64; }
+template <typename T>
+T *ghost_buf(GhostTables &g);
+template <>
+float *ghost_buf<float>(GhostTables &g) { return g.buf32; }
+template <>
+double *ghost_buf<double>(GhostTables &g) { return g.buf64; }
+
+// algorithmic bytes per cell (see pp_kernels.h)
+double bytes_line(int s) { return 3.0 * s + 1.0; }
+
+// ---------------------------------------------------------------------
+// line passes
+// ---------------------------------------------------------------------
+template <typename T>
+void launch_xline(pp_ctx *c, Level &L, int color, T omega) {
+  LevelArgs<T> a = largs<T>(L);
+  int nyh = (L.cy + 1) / 2;
+  int waves = nyh * L.cz;
+  dim3 grid((waves + 3) / 4), block(256);
+  T *e = lv_e<T>(L);
+  const T *rhs = lv_rhs<T>(L);
+  int S = (L.cx + 63) / 64;
+  hipStream_t st = c->stream;
+  switch (S) {
+    case 1: k_xline<T, 1><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
+    case 2: k_xline<T, 2><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
+    case 3: k_xline<T, 3><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
+    case 4: k_xline<T, 4><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
+    case 5: k_xline<T, 5><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
+    case 6: k_xline<T, 6><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
+    case 7:
+    case 8: k_xline<T, 8><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
+    default: break;  // rejected at pp_create
+  }
+}
+
+// segment length LS and segments P for a line of n cells
+// FP32: up to 16 segments of up to 16 cells (64 x P threads); FP64 keeps
+// at most 12 cells per thread in registers and uses up to 32 segments
+// (32 x P threads) for long lines.
+void yz_shape(int n, int &LS, int &P, int &TX, int elem = 4) {
+  static const int opts[] = {1, 2, 4, 8, 12, 16};
+  int maxls = elem == 8 ? 12 : 16;
+  LS = maxls;
+  for (int o : opts) {
+    if (o > maxls) break;
+    if ((n + o - 1) / o <= 16) { LS = o; break; }
+  }
+  if ((n + LS - 1) / LS > 16 && elem == 8) {
+    for (int o : opts) {
+      if (o > maxls) break;
+      if ((n + o - 1) / o <= 32) { LS = o; break; }
+    }
+  }
+  P = (n + LS - 1) / LS;
+  TX = P <= 16 ? 64 : 32;
+}
+
+template <typename T, int DIR>
+void launch_yzline(pp_ctx *c, Level &L, int color, T omega) {
+  LevelArgs<T> a = largs<T>(L);
+  int n = DIR == 1 ? L.cy : L.cz;
+  int planes = DIR == 1 ? L.cz : L.cy;
+  int LS, P, TX;
+  yz_shape(n, LS, P, TX, (int)sizeof(T));
+  int half = (L.cx + 1) / 2;
+  dim3 grid((half + TX - 1) / TX, planes), block(TX, P);
+  size_t sh = sizeof(T) * 7 * TX * P;
+  T *e = lv_e<T>(L);
+  const T *rhs = lv_rhs<T>(L);
+  hipStream_t st = c->stream;
+  switch (LS) {
+    case 1: k_yzline<T, 1, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
+    case 2: k_yzline<T, 2, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
+    case 4: k_yzline<T, 4, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
+    case 8: k_yzline<T, 8, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
+    case 12: k_yzline<T, 12, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
+    default: k_yzline<T, 16, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
+  }
+}
+
+// one Jacobi sweep of the homogeneous ghost rows on a level-0 correction
+template <typename T>
+void ghost_sweep_level0(pp_ctx *c, T *e) {
+  GhostTables &g = c->gh;
+  if (g.n == 0) return;
+  dim3 grid((g.n + 255) / 256), block(256);
+  k_ghost_gather<T, T><<<grid, block, 0, c->stream>>>(g.n, g.s_al, ghost_coef<T>(g), nullptr, e, ghost_buf<T>(g));
+  k_ghost_scatter<T><<<grid, block, 0, c->stream>>>(g.n, g.g_al, ghost_buf<T>(g), e);
+}
+
+template <typename T>
+void smooth(pp_ctx *c, int l, T omega) {
+  Level &L = c->lv[l];
+  if (l == 0) ghost_sweep_level0<T>(c, lv_e<T>(L));
+  for (int col = 0; col < 2; col++) launch_xline<T>(c, L, col, omega);
+  for (int col = 0; col < 2; col++) launch_yzline<T, 1>(c, L, col, omega);
+  for (int col = 0; col < 2; col++) launch_yzline<T, 2>(c, L, col, omega);
+  double cells = (double)L.cx * L.cy * L.cz;
+  c->bytes_acc += 3.0 * cells * bytes_line(sizeof(T));
+}
+
+template <typename T>
+void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
+  Level &L = c->lv[l];
+  hipStream_t st = c->stream;
+  double cells = (double)L.cx * L.cy * L.cz;
+  if (l == (int)c->lv.size() - 1) {
+    LevelArgs<T> a = largs<T>(L);
+    int n = c->cs.n;
+    k_coarse_solve<T><<<1, 512, sizeof(double) * n, st>>>(a, n, c->cs.G, lv_rhs<T>(L), lv_e<T>(L));
+    c->bytes_acc += 8.0 * n * n;
+    return;
+  }
+  T om = (T)o->omega;
+  for (int s = 0; s < o->nu_pre; s++) smooth<T>(c, l, om);
+  LevelArgs<T> a = largs<T>(L);
+  CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
+  k_residual<T><<<cl.grid, cl.block, 0, st>>>(a, lv_e<T>(L), lv_rhs<T>(L), lv_res<T>(L));
+  c->bytes_acc += cells * (3.0 * sizeof(T) + 1.0);
+  Level &C = c->lv[l + 1];
+  LevelArgs<T> ac = largs<T>(C);
+  CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
+  k_restrict<T><<<cc.grid, cc.block, 0, st>>>(a, ac, C.cf[0], C.cf[1], C.cf[2], lv_res<T>(L), lv_rhs<T>(C),
+                                               lv_e<T>(C));
+  c->bytes_acc += cells * (sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz;
+  vcycle<T>(c, l + 1, o);
+  k_prolong<T><<<cl.grid, cl.block, 0, st>>>(a, ac, pdir<T>(L, 0), pdir<T>(L, 1), pdir<T>(L, 2), lv_e<T>(C),
+                                              lv_e<T>(L));
+  c->bytes_acc += cells * (2.0 * sizeof(T) + 1.0) + (sizeof(T) + 1.0) * C.cx * C.cy * C.cz;
+  for (int s = 0; s < o->nu_post; s++) smooth<T>(c, l, om);
+}
+
+// ---------------------------------------------------------------------
+// FP64 helpers on the reference layout
+// ---------------------------------------------------------------------
+CellLaunch ref_launch(const pp_ctx *c) { return cell_launch(c->nxc, c->nyc, c->nzc); }
+
+int reduce_summax(pp_ctx *c, int nblk) {
+  k_reduce_summax<<<1, 256, 0, c->stream>>>(c->d_part, nblk, c->d_res);
+  HIPCHK(hipMemcpyAsync(c->h_res, c->d_res, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PP_OK;
+}
+
+// exact ghost rows + outer ghost layer on an FP64 reference-layout field
+void ghost_exact_ref(pp_ctx *c, double *p, bool with_bnd) {
+  GhostTables &g = c->gh;
+  hipStream_t st = c->stream;
+  if (g.n > 0) {
+    dim3 grid((g.n + 255) / 256), block(256);
+    for (int s = 0; s < g.kexact; s++) {
+      k_ghost_gather<double, double><<<grid, block, 0, st>>>(g.n, g.s_ref, g.coef64, with_bnd ? g.bnd : nullptr, p,
+                                                            g.buf64);
+      k_ghost_scatter<double><<<grid, block, 0, st>>>(g.n, g.g_ref, g.buf64, p);
+    }
+  }
+  k_outer_ghost_x<<<dim3((c->nyc + 2 + 255) / 256, c->nzc + 2), 256, 0, st>>>(c->ref, p);
+  k_outer_ghost_y<<<dim3((c->nxc + 2 + 255) / 256, c->nzc + 2), 256, 0, st>>>(c->ref, p);
+  k_outer_ghost_z<<<dim3((c->nxc + 2 + 255) / 256, c->nyc + 2), 256, 0, st>>>(c->ref, p);
+}
+
+// fluid sums: out[0..4] = {sum w*b, sum w, sum b*b, sum V*p, sum V}
+int fluid_sums(pp_ctx *c, const double *w, const double *b, const double *p) {
+  CellLaunch cl = ref_launch(c);
+  long nblk = (long)cl.grid.x * cl.grid.y * cl.grid.z;
+  k_fluid_sums<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), w, b, p, c->d_part);
+  k_reduce_sums<<<1, 256, 0, c->stream>>>(c->d_part, (int)nblk, 5, c->d_res);
+  return PP_OK;
+}
+
+int mean_remove_ref(pp_ctx *c, double *p, double *mean_out) {
+  CellLaunch cl = ref_launch(c);
+  fluid_sums(c, nullptr, nullptr, p);
+  if (!c->h_fresh.empty()) {
+    HIPCHK(hipMemcpyAsync(c->d_tmp, p, sizeof(double) * c->nref, hipMemcpyDeviceToDevice, c->stream));
+  }
+  k_mean_sub<<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), p, c->d_res);
+  if (!c->h_fresh.empty()) {
+    // fresh cells read the mean-removed snapshot
+    HIPCHK(hipMemcpyAsync(c->d_tmp, p, sizeof(double) * c->nref, hipMemcpyDeviceToDevice, c->stream));
+    k_fresh_fill<<<cl.grid, cl.block, 0, c->stream>>>(c->ref, c->d_iblank_ref, c->d_fresh, c->d_tmp, p);
+  }
+  if (mean_out) {
+    HIPCHK(hipMemcpyAsync(c->h_res, c->d_res, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *mean_out = c->h_res[3] / c->h_res[4];
+  }
+  return PP_OK;
+}
+
+bool opts_ok(const pp_solve_opts *o) {
+  return o && (o->precision == PP_PREC_FP64 || o->precision == PP_PREC_MIXED) && o->max_outer >= 0 &&
+         o->max_outer <= 10000 && o->cycles_per_outer >= 1 && o->nu_pre >= 0 && o->nu_post >= 0 &&
+         o->omega > 0.0 && o->omega < 2.0;
+}
+
+// ---------------------------------------------------------------------
+// the solve (FP64 iterative refinement around T-precision V-cycles)
+// ---------------------------------------------------------------------
+template <typename T>
+int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, const pp_solve_opts *o,
+               pp_solve_stats *st) {
+  hipStream_t s = c->stream;
+  Level &L0 = c->lv[0];
+  LevelArgs<double> a0 = largs<double>(L0);
+  CellLaunch cl = ref_launch(c);
+  long nblk = (long)cl.grid.x * cl.grid.y * cl.grid.z;
+  pp_solve_stats S;
+  memset(&S, 0, sizeof(S));
+  c->bytes_acc = 0.0;
+  hipEvent_t ev0, ev1;
+  HIPCHK(hipEventCreate(&ev0));
+  HIPCHK(hipEventCreate(&ev1));
+  HIPCHK(hipEventRecord(ev0, s));
+  double cells = (double)c->nxc * c->nyc * c->nzc;
+  // ghost Neumann data: bnd = (c_self + probeLength) * uNormal
+  GhostTables &g = c->gh;
+  bool have_bnd = bc_src != nullptr && g.n > 0;
+  if (g.n > 0) {
+    if (have_bnd) {
+      // bnd[n] = bself[n] * bc_src[n]
+      std::vector<double> hb(g.n), hs(g.n);
+      HIPCHK(hipMemcpy(hs.data(), bc_src, sizeof(double) * g.n, hipMemcpyDeviceToHost));
+      for (int q = 0; q < g.n; q++) hb[q] = g.h_bself[q] * hs[q];
+      HIPCHK(hipMemcpyAsync(g.bnd, hb.data(), sizeof(double) * g.n, hipMemcpyHostToDevice, s));
+    }
+  }
+  // b = rhs on fluid cells
+  k_mask_rhs<<<cl.grid, cl.block, 0, s>>>(c->ref, a0, rhs, c->d_b);
+  // compatibility projection: b_f -= (w.b + wg.bnd) / sum_f w
+  if (o->project_rhs) {
+    fluid_sums(c, c->d_w, c->d_b, nullptr);
+    HIPCHK(hipMemcpyAsync(c->h_res, c->d_res, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double num = c->h_res[0], den = c->h_res[1];
+    if (have_bnd) {
+      // ghost-row part of w^T b
+      std::vector<double> hw(g.n), hb(g.n);
+      for (int q = 0; q < g.n; q++) {
+        double wv;
+        HIPCHK(hipMemcpy(&wv, c->d_w + g.h_g_ref[q], sizeof(double), hipMemcpyDeviceToHost));
+        hw[q] = wv;
+      }
+      HIPCHK(hipMemcpy(hb.data(), g.bnd, sizeof(double) * g.n, hipMemcpyDeviceToHost));
+      for (int q = 0; q < g.n; q++) num += hw[q] * hb[q];
+    }
+    double shift = num / den;
+    S.rhs_shift = shift;
+    c->h_res[0] = shift;
+    c->h_res[1] = 1.0;
+    HIPCHK(hipMemcpyAsync(c->d_res + 8, c->h_res, 2 * sizeof(double), hipMemcpyHostToDevice, s));
+    k_fluid_shift<<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_b, c->d_res + 8, c->d_res + 9, -1.0);
+  }
+  fluid_sums(c, nullptr, c->d_b, nullptr);
+  HIPCHK(hipMemcpyAsync(c->h_res, c->d_res, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double bnorm = std::sqrt(c->h_res[2]);
+  S.b_l2 = bnorm;
+  if (!std::isfinite(bnorm)) {
+    set_error("non-finite right-hand side");
+    S.status = PP_NONFINITE;
+    if (st) *st = S;
+    return PP_NONFINITE;
+  }
+  // initial guess: subtracting_average_pressure (CORE_POISSON.F90:535), then
+  // ghost rows and outer layer (GCM_correct_pressure semantics)
+  mean_remove_ref(c, p, nullptr);
+  ghost_exact_ref(c, p, have_bnd);
+  double bden = bnorm > 0.0 ? bnorm : 1.0;
+  int status = PP_NOT_CONVERGED;
+  int outer = 0;
+  for (;;) {
+    // r = b - A p -> level-0 rhs (T), e0 = 0, norms
+    k_outer_residual<T, 256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, p, c->d_b, lv_rhs<T>(L0), lv_e<T>(L0),
+                                                          c->d_part);
+    c->bytes_acc += cells * (17.0 + 2.0 * sizeof(T));
+    int rc = reduce_summax(c, (int)nblk);
+    if (rc) return rc;
+    double rl2 = std::sqrt(c->h_res[0]);
+    double rinf = c->h_res[1];
+    long argr = (long)c->h_res[2];
+    if (!std::isfinite(rl2)) {
+      set_error("non-finite residual after %d outer iterations", outer);
+      status = PP_NONFINITE;
+      break;
+    }
+    S.rel_l2 = rl2 / bden;
+    S.abs_linf = rinf;
+    if (rinf > 0.0 && argr >= 0) {
+      long q = argr;
+      S.loc[0] = (int)(q % c->ref.ex);
+      S.loc[1] = (int)((q / c->ref.ex) % (c->nyc + 3));
+      S.loc[2] = (int)(q / c->ref.exy);
+    } else {
+      S.loc[0] = S.loc[1] = S.loc[2] = 0;
+    }
+    if (outer > 0 && outer <= 64) S.history[outer - 1] = S.rel_l2;
+    bool conv = (o->rel_tol > 0.0 && rl2 <= o->rel_tol * bden) || (o->abs_linf_tol > 0.0 && rinf <= o->abs_linf_tol) ||
+                bnorm == 0.0;
+    if (conv) { status = PP_OK; break; }
+    if (outer >= o->max_outer) break;
+    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
+      vcycle<T>(c, 0, o);
+      S.vcycles++;
+    }
+    // p += e on fluid cells, ghost rows exact
+    k_update<T><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, p, lv_e<T>(L0));
+    c->bytes_acc += cells * (17.0 + sizeof(T));
+    ghost_exact_ref(c, p, have_bnd);
+    outer++;
+  }
+  if (o->mean_remove) {
+    double mean = 0.0;
+    mean_remove_ref(c, p, &mean);
+    S.mean_removed = mean;
+    ghost_exact_ref(c, p, have_bnd);
+  }
+  HIPCHK(hipEventRecord(ev1, s));
+  HIPCHK(hipEventSynchronize(ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+  hipEventDestroy(ev0);
+  hipEventDestroy(ev1);
+  HIPCHK(hipGetLastError());
+  S.ms = ms;
+  S.bytes = c->bytes_acc;
+  S.outer_iterations = outer;
+  S.status = status;
+  if (st) *st = S;
+  return status;
+}
+
+}  // namespace
+
+// ======================================================================
+// C-ABI
+// ======================================================================
+extern "C" {
+
+void pp_default_opts(pp_solve_opts *o) {
+  if (!o) return;
+  o->precision = PP_PREC_MIXED;
+  o->rel_tol = 1e-8;
+  o->abs_linf_tol = 0.0;
+  o->max_outer = 100;
+  o->cycles_per_outer = 1;
+  o->nu_pre = 2;
+  o->nu_post = 2;
+  o->omega = 1.0;
+  o->mean_remove = 1;
+  o->project_rhs = 1;
+  o->use_graph = 0;
+}
+
+const char *pp_last_error(void) { return pp::last_error(); }
+
+int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
+  if (!out || !g) { set_error("pp_create: null argument"); return PP_BAD_ARG; }
+  *out = nullptr;
+  if (g->nxc < 4 || g->nyc < 4 || g->nzc < 4 || !g->x || !g->y || !g->z) {
+    set_error("pp_create: grid must have at least 4 cells per direction and node arrays");
+    return PP_BAD_ARG;
+  }
+  if (g->nxc > 512) {
+    set_error("pp_create: nxc=%d exceeds the 512 cells per x-line the wavefront line solver holds", g->nxc);
+    return PP_BAD_ARG;
+  }
+  for (int f = 0; f < 6; f++)
+    if (g->bc[f] != PP_BC_NEUMANN) {
+      set_error("pp_create: face %d has pressure BC %d; only PP_BC_NEUMANN (2) is implemented", f, g->bc[f]);
+      return PP_BAD_ARG;
+    }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("pp_create: no HIP device available");
+    return PP_HIP_ERROR;
+  }
+  if (g->device < 0 || g->device >= ndev) { set_error("pp_create: bad device %d", g->device); return PP_BAD_ARG; }
+  pp_ctx *c = new pp_ctx();
+  c->device = g->device;
+  HIPCHK(hipSetDevice(c->device));
+  c->nxc = g->nxc; c->nyc = g->nyc; c->nzc = g->nzc;
+  c->ref.nxc = g->nxc; c->ref.nyc = g->nyc; c->ref.nzc = g->nzc;
+  c->ref.ex = g->nxc + 3;
+  c->ref.exy = (long)(g->nxc + 3) * (g->nyc + 3);
+  c->nref = c->ref.exy * (g->nzc + 3);
+  const double *xs[3] = {g->x, g->y, g->z};
+  int ns[3] = {g->nxc, g->nyc, g->nzc};
+  for (int d = 0; d < 3; d++) {
+    for (int q = 1; q <= ns[d]; q++)
+      if (!(xs[d][q] > xs[d][q - 1])) {
+        set_error("pp_create: node coordinates of direction %d are not increasing at %d", d, q);
+        delete c;
+        return PP_BAD_ARG;
+      }
+    make_grid_dir(ns[d] + 1, xs[d], c->hx[d], c->rxc[d], c->rdx[d], c->rdxinv[d], c->rdxc[d], c->rdxcinv[d]);
+  }
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  size_t nb = sizeof(double) * c->nref;
+  HIPCHK(hipMalloc(&c->d_p, nb));
+  HIPCHK(hipMalloc(&c->d_b, nb));
+  HIPCHK(hipMalloc(&c->d_w, nb));
+  HIPCHK(hipMalloc(&c->d_bwork, nb));
+  HIPCHK(hipMalloc(&c->d_tmp, nb));
+  HIPCHK(hipMemset(c->d_b, 0, nb));
+  HIPCHK(hipMemset(c->d_w, 0, nb));
+  CellLaunch cl = cell_launch(c->nxc, c->nyc, c->nzc);
+  c->nblk_red = (int)((long)cl.grid.x * cl.grid.y * cl.grid.z);
+  HIPCHK(hipMalloc(&c->d_part, sizeof(double) * 5 * (size_t)c->nblk_red + 64));
+  HIPCHK(hipMalloc(&c->d_res, sizeof(double) * 16));
+  HIPCHK(hipHostMalloc(&c->h_res, sizeof(double) * 16));
+  *out = c;
+  return PP_OK;
+}
+
+int32_t pp_destroy(pp_ctx *c) {
+  if (!c) return PP_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_levels(c);
+  free_ghosts(c);
+  hipFree(c->d_p); hipFree(c->d_b); hipFree(c->d_w); hipFree(c->d_bwork); hipFree(c->d_tmp);
+  hipFree(c->d_fresh); hipFree(c->d_iblank_ref);
+  hipFree(c->d_part); hipFree(c->d_res);
+  if (c->h_res) hipHostFree(c->h_res);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return PP_OK;
+}
+
+int32_t pp_set_geometry(pp_ctx *c, const pp_geometry *geo) {
+  if (!c || !geo || !geo->iblank) { set_error("pp_set_geometry: null argument"); return PP_BAD_ARG; }
+  if (geo->n_ghost < 0 || (geo->n_ghost > 0 && (!geo->i_ghost || !geo->j_ghost || !geo->k_ghost || !geo->i_cell ||
+                                                 !geo->j_cell || !geo->k_cell || !geo->coeff || !geo->probe_length))) {
+    set_error("pp_set_geometry: ghost arrays missing");
+    return PP_BAD_ARG;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  c->have_geom = false;
+  c->h_iblank.assign(geo->iblank, geo->iblank + c->nref);
+  for (long r = 0; r < c->nref; r++)
+    if (c->h_iblank[r] != 0 && c->h_iblank[r] != 1) {
+      set_error("pp_set_geometry: iblank must be 0 or 1 (entry %ld = %d)", r, (int)c->h_iblank[r]);
+      return PP_BAD_GEOMETRY;
+    }
+  // the outer layer carries no geometry
+  for (int k = 0; k < c->nzc + 3; k++)
+    for (int j = 0; j < c->nyc + 3; j++)
+      for (int i = 0; i < c->nxc + 3; i++)
+        if (i == 0 || j == 0 || k == 0 || i > c->nxc || j > c->nyc || k > c->nzc)
+          c->h_iblank[(long)i + c->ref.ex * j + c->ref.exy * k] = 0;
+  derive_ghost_mark(c);
+  long nmark = 0;
+  for (long r = 0; r < c->nref; r++) nmark += c->h_ghost[r];
+  if (nmark != geo->n_ghost) {
+    set_error("pp_set_geometry: iblank has %ld ghost cells, the ghost list has %d", nmark, geo->n_ghost);
+    return PP_BAD_GEOMETRY;
+  }
+  free_ghosts(c);
+  int rc = build_levels(c);
+  if (rc) return rc;
+  build_flags(c);
+  rc = upload_levels(c);
+  if (rc) return rc;
+  rc = build_ghosts(c, geo);
+  if (rc) return rc;
+  rc = build_coarse(c);
+  if (rc) return rc;
+  // fresh cells
+  c->h_fresh.clear();
+  hipFree(c->d_fresh);
+  c->d_fresh = nullptr;
+  if (geo->fresh_cell) {
+    bool any = false;
+    for (long r = 0; r < c->nref; r++) any |= geo->fresh_cell[r] == 1;
+    if (any) {
+      c->h_fresh.assign(geo->fresh_cell, geo->fresh_cell + c->nref);
+      HIPCHK(hipMalloc(&c->d_fresh, c->nref));
+      HIPCHK(hipMemcpy(c->d_fresh, c->h_fresh.data(), c->nref, hipMemcpyHostToDevice));
+    }
+  }
+  hipFree(c->d_iblank_ref);
+  HIPCHK(hipMalloc(&c->d_iblank_ref, c->nref));
+  HIPCHK(hipMemcpy(c->d_iblank_ref, c->h_iblank.data(), c->nref, hipMemcpyHostToDevice));
+  // left null vector: fluid volumes (refined by the adjoint solve)
+  {
+    std::vector<double> w(c->nref, 0.0);
+    for (int k = 0; k < c->nzc; k++)
+      for (int j = 0; j < c->nyc; j++)
+        for (int i = 0; i < c->nxc; i++) {
+          long r = ridx(c->ref, i, j, k);
+          if (!c->h_iblank[r]) w[r] = c->rdx[0][i + 1] * c->rdx[1][j + 1] * c->rdx[2][k + 1];
+        }
+    HIPCHK(hipMemcpy(c->d_w, w.data(), sizeof(double) * c->nref, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipDeviceSynchronize());
+  c->have_geom = true;
+  return PP_OK;
+}
+
+int32_t pp_solve_device(pp_ctx *c, double *d_p, const double *d_rhs, const double *d_bc, const pp_solve_opts *o,
+                        pp_solve_stats *st) {
+  if (!c || !d_p || !d_rhs) { set_error("pp_solve_device: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_solve_device: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  pp_solve_opts def;
+  if (!o) { pp_default_opts(&def); o = &def; }
+  if (!opts_ok(o)) { set_error("pp_solve_device: invalid options"); return PP_BAD_ARG; }
+  HIPCHK(hipSetDevice(c->device));
+  if (o->precision == PP_PREC_MIXED) return solve_impl<float>(c, d_p, d_rhs, d_bc, o, st);
+  return solve_impl<double>(c, d_p, d_rhs, d_bc, o, st);
+}
+
+int32_t pp_solve(pp_ctx *c, double *p, const double *rhs, const double *bc, const pp_solve_opts *o,
+                 pp_solve_stats *st) {
+  if (!c || !p || !rhs) { set_error("pp_solve: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_solve: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  size_t nb = sizeof(double) * c->nref;
+  HIPCHK(hipMemcpy(c->d_p, p, nb, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_bwork, rhs, nb, hipMemcpyHostToDevice));
+  double *dbc = nullptr;
+  if (bc && c->gh.n > 0) {
+    HIPCHK(hipMalloc(&dbc, sizeof(double) * c->gh.n));
+    HIPCHK(hipMemcpy(dbc, bc, sizeof(double) * c->gh.n, hipMemcpyHostToDevice));
+  }
+  int rc = pp_solve_device(c, c->d_p, c->d_bwork, dbc, o, st);
+  if (dbc) hipFree(dbc);
+  if (rc == PP_OK || rc == PP_NOT_CONVERGED) HIPCHK(hipMemcpy(p, c->d_p, nb, hipMemcpyDeviceToHost));
+  return rc;
+}
+
+int32_t pp_calc_residual(pp_ctx *c, const double *var, const double *r, double *res, double *resm, int32_t *loc,
+                         double *l2) {
+  if (!c || !var || !r) { set_error("pp_calc_residual: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_calc_residual: no geometry"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  size_t nb = sizeof(double) * c->nref;
+  HIPCHK(hipMemcpy(c->d_p, var, nb, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_bwork, r, nb, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(c->d_tmp, 0, nb));
+  CellLaunch cl = ref_launch(c);
+  long nblk = (long)cl.grid.x * cl.grid.y * cl.grid.z;
+  k_ref_residual<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), c->d_p, c->d_bwork,
+                                                           c->d_tmp, c->d_part);
+  int rc = reduce_summax(c, (int)nblk);
+  if (rc) return rc;
+  if (res) HIPCHK(hipMemcpy(res, c->d_tmp, nb, hipMemcpyDeviceToHost));
+  double mx = c->h_res[1];
+  if (resm) *resm = mx;
+  if (l2) *l2 = std::sqrt(c->h_res[0]);
+  if (loc) {
+    long q = (long)c->h_res[2];
+    if (mx > 0.0 && q >= 0) {
+      loc[0] = (int)(q % c->ref.ex);
+      loc[1] = (int)((q / c->ref.ex) % (c->nyc + 3));
+      loc[2] = (int)(q / c->ref.exy);
+    } else {
+      loc[0] = loc[1] = loc[2] = 0;
+    }
+  }
+  return PP_OK;
+}
+
+int32_t pp_ghost_rows(pp_ctx *c, double *p, const double *bc) {
+  if (!c || !p) { set_error("pp_ghost_rows: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_ghost_rows: no geometry"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  size_t nb = sizeof(double) * c->nref;
+  HIPCHK(hipMemcpy(c->d_p, p, nb, hipMemcpyHostToDevice));
+  GhostTables &g = c->gh;
+  bool have = bc && g.n > 0;
+  if (have) {
+    std::vector<double> hb(g.n);
+    for (int q = 0; q < g.n; q++) hb[q] = g.h_bself[q] * bc[q];
+    HIPCHK(hipMemcpy(g.bnd, hb.data(), sizeof(double) * g.n, hipMemcpyHostToDevice));
+  }
+  ghost_exact_ref(c, c->d_p, have);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(p, c->d_p, nb, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+int32_t pp_mean_remove(pp_ctx *c, double *p, double *mean) {
+  if (!c || !p) { set_error("pp_mean_remove: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_mean_remove: no geometry"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  size_t nb = sizeof(double) * c->nref;
+  HIPCHK(hipMemcpy(c->d_p, p, nb, hipMemcpyHostToDevice));
+  double m = 0.0;
+  int rc = mean_remove_ref(c, c->d_p, &m);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(p, c->d_p, nb, hipMemcpyDeviceToHost));
+  if (mean) *mean = m;
+  return PP_OK;
+}
+
+int32_t pp_level_info(pp_ctx *c, int32_t *n_levels, int32_t *dims) {
+  if (!c || !n_levels) { set_error("pp_level_info: null argument"); return PP_BAD_ARG; }
+  *n_levels = (int32_t)c->lv.size();
+  if (dims)
+    for (size_t l = 0; l < c->lv.size(); l++) {
+      dims[3 * l] = c->lv[l].cx; dims[3 * l + 1] = c->lv[l].cy; dims[3 * l + 2] = c->lv[l].cz;
+    }
+  return PP_OK;
+}
+
+int32_t pp_null_vector(pp_ctx *c, double *w) {
+  if (!c || !w) { set_error("pp_null_vector: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_null_vector: no geometry"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpy(w, c->d_w, sizeof(double) * c->nref, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+int32_t pp_tdma_batched(pp_ctx *c, const double *a, const double *b, const double *cc, const double *r, double *u,
+                        int32_t n, int32_t nl, int32_t method) {
+  if (!c || !a || !b || !cc || !r || !u || n < 1 || nl < 1) { set_error("pp_tdma_batched: bad argument"); return PP_BAD_ARG; }
+  if (method == 0 && n > 512) { set_error("pp_tdma_batched: wavefront PCR holds at most 512 rows"); return PP_BAD_ARG; }
+  if (method == 1 && n > 512) { set_error("pp_tdma_batched: partitioned Thomas holds at most 512 rows"); return PP_BAD_ARG; }
+  if (method != 0 && method != 1) { set_error("pp_tdma_batched: method must be 0 or 1"); return PP_BAD_ARG; }
+  HIPCHK(hipSetDevice(c->device));
+  size_t nb = sizeof(double) * (size_t)n * nl;
+  double *d[5];
+  for (int q = 0; q < 5; q++) HIPCHK(hipMalloc(&d[q], nb));
+  std::vector<double> tmp;
+  const double *src[4] = {a, b, cc, r};
+  for (int q = 0; q < 4; q++) {
+    if (method == 1) {
+      // interleave: element m of line l at m*nl + l
+      tmp.resize((size_t)n * nl);
+      for (int l = 0; l < nl; l++)
+        for (int m = 0; m < n; m++) tmp[(size_t)m * nl + l] = src[q][(size_t)l * n + m];
+      HIPCHK(hipMemcpy(d[q], tmp.data(), nb, hipMemcpyHostToDevice));
+    } else {
+      HIPCHK(hipMemcpy(d[q], src[q], nb, hipMemcpyHostToDevice));
+    }
+  }
+  hipStream_t s = c->stream;
+  if (method == 0) {
+    int S = (n + 63) / 64;
+    dim3 grid((nl + 3) / 4), block(256);
+    switch (S) {
+      case 1: k_tdma_wave<1><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+      case 2: k_tdma_wave<2><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+      case 3: k_tdma_wave<3><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+      case 4: k_tdma_wave<4><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+      case 5: k_tdma_wave<5><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+      case 6: k_tdma_wave<6><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+      default: k_tdma_wave<8><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+    }
+  } else {
+    int LS, P, TX;
+    yz_shape(n, LS, P, TX, 8);
+    dim3 grid((nl + TX - 1) / TX), block(TX, P);
+    size_t sh = sizeof(double) * 7 * TX * P;
+    switch (LS) {
+      case 1: k_tdma_seg<1><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+      case 2: k_tdma_seg<2><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+      case 4: k_tdma_seg<4><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+      case 8: k_tdma_seg<8><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+      case 12: k_tdma_seg<12><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+      default: k_tdma_seg<16><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  if (method == 1) {
+    tmp.resize((size_t)n * nl);
+    HIPCHK(hipMemcpy(tmp.data(), d[4], nb, hipMemcpyDeviceToHost));
+    for (int l = 0; l < nl; l++)
+      for (int m = 0; m < n; m++) u[(size_t)l * n + m] = tmp[(size_t)m * nl + l];
+  } else {
+    HIPCHK(hipMemcpy(u, d[4], nb, hipMemcpyDeviceToHost));
+  }
+  for (int q = 0; q < 5; q++) hipFree(d[q]);
+  return PP_OK;
+}
+
+int32_t pp_divergence(pp_ctx *c, const double *fu, const double *fv, const double *fw, double dtinv, double *nlu0,
+                      double *sum) {
+  (void)c; (void)fu; (void)fv; (void)fw; (void)dtinv; (void)nlu0; (void)sum;
+  set_error("pp_divergence: not implemented yet");
+  return PP_BAD_ARG;
+}
+
+int32_t pp_adjoint(pp_ctx *c, const double *w, double *lambda, double *dJ_dg, const pp_solve_opts *o,
+                   pp_solve_stats *st) {
+  (void)c; (void)w; (void)lambda; (void)dJ_dg; (void)o; (void)st;
+  set_error("pp_adjoint: not implemented yet");
+  return PP_BAD_ARG;
+}
+
+}  // extern "C"

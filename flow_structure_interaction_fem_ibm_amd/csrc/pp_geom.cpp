@@ -1,0 +1,449 @@
+// Body geometry -> the GCM inputs of the pressure solve (iblank, ghost list,
+// image-point cells, Neumann interpolation coefficients), host C++.
+//
+// Replaces set_boundary for canonical solid bodies (BOUNDARY_SET.F90:8-51):
+//   iblank        set_iblank_canonical_body_fast (BOUNDARY_IBLANK_FAST.F90:2-501)
+//                 or, in analytic mode, "cell centre inside the body"
+//   cleanup       find_isolatedIblank(1) (BOUNDARY_SET.F90:440-499),
+//                 fill_cell (SSM_SET_BC.F90:1425-1511)
+//   ghosts        GCM_set_internal_boundary (GCM_SET_BOUNDARY.F90:2-304)
+//   coefficients  GCM_Calc_VanMatrixDN (GCM_SET_BOUNDARY.F90:307-494)
+// The triangulated mode lives in pp_geom_tri.cpp.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "pp_geom.h"
+
+namespace ppg {
+
+void parallel_for(long n, const std::function<void(long, long)> &fn) {
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 4096 || nt == 1) { fn(0, n); return; }
+  std::vector<std::thread> th;
+  long chunk = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; t++) {
+    long a = t * chunk, b = std::min(n, a + chunk);
+    if (a >= b) break;
+    th.emplace_back([&, a, b] { fn(a, b); });
+  }
+  for (auto &t : th) t.join();
+}
+
+Grid::Grid(int nxc_, int nyc_, int nzc_, const double *xn, const double *yn, const double *zn)
+    : nxc(nxc_), nyc(nyc_), nzc(nzc_) {
+  const double *src[3] = {xn, yn, zn};
+  int ns[3] = {nxc, nyc, nzc};
+  for (int d = 0; d < 3; d++) {
+    int n = ns[d] + 1;  // reference node count
+    auto &x = X[d];
+    x.assign(n + 2, 0.0);
+    XC[d].assign(n + 2, 0.0);
+    DX[d].assign(n + 2, 0.0);
+    for (int i = 1; i <= n; i++) x[i] = src[d][i - 1];
+    x[0] = -x[2];
+    x[n + 1] = x[n] + x[n] - x[n - 1];
+    for (int i = 0; i <= n; i++) {
+      XC[d][i] = 0.5 * (x[i + 1] + x[i]);
+      DX[d][i] = x[i + 1] - x[i];
+    }
+    XC[d][n + 1] = x[n + 1];
+  }
+  ex = nxc + 3;
+  exy = (long)ex * (nyc + 3);
+  size = exy * (nzc + 3);
+}
+
+// ---------------------------------------------------------------------
+// analytic bodies
+// ---------------------------------------------------------------------
+static inline void to_body(const Body &b, const double *p, double *q) {
+  double ca = std::cos(b.alpha_deg * M_PI / 180.0), sa = std::sin(b.alpha_deg * M_PI / 180.0);
+  double dx = p[0] - b.c[0], dy = p[1] - b.c[1], dz = p[2] - b.c[2];
+  q[0] = ca * dx + sa * dy;
+  q[1] = -sa * dx + ca * dy;
+  q[2] = dz;
+}
+static inline void from_body_vec(const Body &b, const double *q, double *v) {
+  double ca = std::cos(b.alpha_deg * M_PI / 180.0), sa = std::sin(b.alpha_deg * M_PI / 180.0);
+  v[0] = ca * q[0] - sa * q[1];
+  v[1] = sa * q[0] + ca * q[1];
+  v[2] = q[2];
+}
+
+bool inside_analytic(const Body &b, const double *p) {
+  double q[3];
+  to_body(b, p, q);
+  double s = 0.0;
+  for (int d = 0; d < 3; d++) s += (q[d] / b.r[d]) * (q[d] / b.r[d]);
+  return s <= 1.0;
+}
+
+// Closest point on the ellipsoid surface to p (body intercept) and the
+// inward unit normal there.  Distance-minimising point x = a^2 q/(a^2+t)
+// with t the root of sum (a q/(a^2+t))^2 = 1 (bisection, then Newton).
+void closest_point_analytic(const Body &b, const double *p, double *bi, double *nin) {
+  double q[3];
+  to_body(b, p, q);
+  const double *a = b.r;
+  double x[3];
+  if (a[0] == a[1] && a[1] == a[2]) {
+    double r = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    if (r == 0.0) { x[0] = a[0]; x[1] = 0; x[2] = 0; }
+    else for (int d = 0; d < 3; d++) x[d] = q[d] * a[0] / r;
+  } else {
+    auto f = [&](double t) {
+      double s = 0.0;
+      for (int d = 0; d < 3; d++) {
+        double v = a[d] * q[d] / (a[d] * a[d] + t);
+        s += v * v;
+      }
+      return s - 1.0;
+    };
+    double amin2 = std::min(a[0] * a[0], std::min(a[1] * a[1], a[2] * a[2]));
+    double lo = -amin2 + 1e-300, hi = 1.0;
+    // bracket: f decreasing on (-amin^2, inf)
+    double qn = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    double amax = std::max(a[0], std::max(a[1], a[2]));
+    hi = std::max(1.0, amax * qn);
+    while (f(hi) > 0) hi *= 2.0;
+    // move lo up until f(lo) > 0 (it tends to +inf as t -> -amin^2 unless the
+    // matching q component is 0)
+    double l = -amin2;
+    double step = amin2;
+    lo = l + step;
+    for (int it = 0; it < 200 && f(lo) <= 0; it++) { step *= 0.5; lo = l + step; }
+    double t = 0.5 * (lo + hi);
+    for (int it = 0; it < 200; it++) {
+      double fv = f(t);
+      if (fv > 0) lo = t; else hi = t;
+      double dfv = 0.0;
+      for (int d = 0; d < 3; d++) {
+        double den = a[d] * a[d] + t;
+        dfv += -2.0 * a[d] * a[d] * q[d] * q[d] / (den * den * den);
+      }
+      double tn = t - fv / dfv;
+      if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+      if (std::fabs(tn - t) <= 1e-15 * std::max(1.0, std::fabs(t))) { t = tn; break; }
+      t = tn;
+    }
+    for (int d = 0; d < 3; d++) x[d] = a[d] * a[d] * q[d] / (a[d] * a[d] + t);
+  }
+  double g[3] = {-x[0] / (a[0] * a[0]), -x[1] / (a[1] * a[1]), -x[2] / (a[2] * a[2])};
+  double gn = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  for (int d = 0; d < 3; d++) g[d] /= gn;
+  double w[3];
+  from_body_vec(b, x, w);
+  for (int d = 0; d < 3; d++) bi[d] = b.c[d] + w[d];
+  from_body_vec(b, g, nin);
+}
+
+// ---------------------------------------------------------------------
+// shared pipeline
+// ---------------------------------------------------------------------
+
+// find_isolatedIblank(1) (BOUNDARY_SET.F90:440-499): a blank cell with no
+// other blank cell in its clamped 3x3x3 neighbourhood is cleared (in place,
+// in loop order).
+void isolated_iblank(const Grid &G, std::vector<int8_t> &ib) {
+  for (int k = 1; k <= G.nzc; k++)
+    for (int j = 1; j <= G.nyc; j++)
+      for (int i = 1; i <= G.nxc; i++) {
+        if (ib[G.r(i, j, k)] != 1) continue;
+        int s = 0;
+        for (int kk = std::max(1, k - 1); kk <= std::min(G.nzc, k + 1); kk++)
+          for (int jj = std::max(1, j - 1); jj <= std::min(G.nyc, j + 1); jj++)
+            for (int ii = std::max(1, i - 1); ii <= std::min(G.nxc, i + 1); ii++) s += ib[G.r(ii, jj, kk)];
+        if (s <= 1) ib[G.r(i, j, k)] = 0;
+      }
+}
+
+// fill_cell (SSM_SET_BC.F90:1425-1511) with the outer-face flags of
+// set_outer_iup: a fluid cell on an outer face whose inward neighbour
+// along that axis is blank becomes blank (sequential sweeps in x, y, z).
+void fill_cell(const Grid &G, std::vector<int8_t> &ib, std::vector<int8_t> &fresh) {
+  auto sweep = [&](int axis) {
+    for (int k = 1; k <= G.nzc; k++)
+      for (int j = 1; j <= G.nyc; j++)
+        for (int i = 1; i <= G.nxc; i++) {
+          long r = G.r(i, j, k);
+          if (ib[r] != 0) continue;
+          long st = axis == 0 ? 1 : axis == 1 ? G.ex : G.exy;
+          int c = axis == 0 ? i : axis == 1 ? j : k;
+          int n = axis == 0 ? G.nxc : axis == 1 ? G.nyc : G.nzc;
+          bool up = (c == n), um = (c == 1);  // iup / ium of set_outer_iup
+          if ((ib[r - st] == 1 && up) || (ib[r + st] == 1 && um)) {
+            ib[r] = 1;
+            if (fresh[r] == 1) fresh[r] = 0;
+          }
+        }
+  };
+  sweep(0);
+  sweep(1);
+  sweep(2);
+}
+
+// GCM ghost marking and list (GCM_SET_BOUNDARY.F90:49-118)
+void mark_ghosts(const Grid &G, const std::vector<int8_t> &ib, std::vector<int8_t> &gm, std::vector<int> &list) {
+  gm.assign(G.size, 0);
+  list.clear();
+  int nx = G.nxc + 1, ny = G.nyc + 1, nz = G.nzc + 1;
+  for (int k = 1; k <= nz - 1; k++)
+    for (int j = 1; j <= ny - 1; j++)
+      for (int i = 1; i <= nx - 1; i++) {
+        long r = G.r(i, j, k);
+        if (ib[r] != 1) continue;
+        int iM = std::max(i - 1, 1), iP = std::min(i + 1, nx - 1);
+        int jM = std::max(j - 1, 1), jP = std::min(j + 1, ny - 1);
+        int kM = std::max(k - 1, 1), kP = std::min(k + 1, nz - 1);
+        if (ib[G.r(iM, j, k)] == 0 || ib[G.r(iP, j, k)] == 0 || ib[G.r(i, jM, k)] == 0 ||
+            ib[G.r(i, jP, k)] == 0 || ib[G.r(i, j, kM)] == 0 || ib[G.r(i, j, kP)] == 0) {
+          gm[r] = 1;
+          list.push_back(i);
+          list.push_back(j);
+          list.push_back(k);
+        }
+      }
+}
+
+static const int incI[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+static const int incJ[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+static const int incK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+
+// Solve M^T c = beta for the 8x8 Neumann Vandermonde matrix M (partial
+// pivoting); c are the coefficients of GCM_Calc_VanMatrixDN, i.e. row beta of
+// the inverse applied to the basis at the image point.
+static bool solve8T(const double M[8][8], const double beta[8], double c[8]) {
+  double A[8][9];
+  for (int i = 0; i < 8; i++) {
+    for (int j = 0; j < 8; j++) A[i][j] = M[j][i];
+    A[i][8] = beta[i];
+  }
+  for (int k = 0; k < 8; k++) {
+    int p = k;
+    for (int i = k + 1; i < 8; i++)
+      if (std::fabs(A[i][k]) > std::fabs(A[p][k])) p = i;
+    if (A[p][k] == 0.0) return false;
+    if (p != k)
+      for (int j = 0; j < 9; j++) std::swap(A[k][j], A[p][j]);
+    for (int i = k + 1; i < 8; i++) {
+      double f = A[i][k] / A[k][k];
+      for (int j = k; j < 9; j++) A[i][j] -= f * A[k][j];
+    }
+  }
+  for (int i = 7; i >= 0; i--) {
+    double s = A[i][8];
+    for (int j = i + 1; j < 8; j++) s -= A[i][j] * c[j];
+    c[i] = s / A[i][i];
+  }
+  return true;
+}
+
+// Image point, its interpolation cell and the Neumann coefficients of one
+// ghost (GCM_SET_BOUNDARY.F90:141-257, 399-491).  Returns 0, 1 (image point
+// cell not found) or 2 (intercept longer than twice the cell diagonal).
+int ghost_stencil(const Grid &G, int iG, int jG, int kG, const double *bi, const double *nrm, double probe_norm,
+                  GhostOut &o) {
+  double xg = G.XC[0][iG], yg = G.XC[1][jG], zg = G.XC[2][kG];
+  double ds = std::sqrt((xg - bi[0]) * (xg - bi[0]) + (yg - bi[1]) * (yg - bi[1]) + (zg - bi[2]) * (zg - bi[2]));
+  double diag = std::sqrt(G.dxc(0, iG) * G.dxc(0, iG) + G.dxc(1, jG) * G.dxc(1, jG) + G.dxc(2, kG) * G.dxc(2, kG));
+  if (ds > diag && ds / diag > 2.0) return 2;
+  double sx = (bi[0] - xg) / ds, sy = (bi[1] - yg) / ds, sz = (bi[2] - zg) / ds;
+  double probe = ds * probe_norm;
+  double ip[3] = {xg + sx * probe, yg + sy * probe, zg + sz * probe};
+  int idx[3] = {iG, jG, kG};
+  int cell[3] = {-1, -1, -1};
+  int nn[3] = {G.nxc + 1, G.nyc + 1, G.nzc + 1};
+  for (int d = 0; d < 3; d++) {
+    int range = (int)std::lround(probe / G.DX[d][idx[d]]) + 1;
+    int lo = std::max(idx[d] - range, 0), hi = std::min(idx[d] + range, nn[d]);
+    for (int i = lo; i <= hi; i++)
+      if (G.XC[d][i] <= ip[d] && G.XC[d][i + 1] > ip[d]) cell[d] = i;
+  }
+  if (cell[0] < 0 || cell[1] < 0 || cell[2] < 0) return 1;
+  double M[8][8];
+  for (int r = 0; r < 8; r++) {
+    int i = cell[0] + incI[r], j = cell[1] + incJ[r], k = cell[2] + incK[r];
+    double x1 = G.XC[0][i], x2 = G.XC[1][j], x3 = G.XC[2][k];
+    if (i == iG && j == jG && k == kG) {
+      x1 = bi[0]; x2 = bi[1]; x3 = bi[2];
+      double n1 = nrm[0], n2 = nrm[1], n3 = nrm[2];
+      M[r][0] = n1 * x2 * x3 + n2 * x1 * x3 + n3 * x1 * x2;
+      M[r][1] = n1 * x2 + n2 * x1;
+      M[r][2] = n1 * x3 + n3 * x1;
+      M[r][3] = n2 * x3 + n3 * x2;
+      M[r][4] = n1; M[r][5] = n2; M[r][6] = n3; M[r][7] = 0.0;
+    } else {
+      M[r][0] = x1 * x2 * x3; M[r][1] = x1 * x2; M[r][2] = x1 * x3; M[r][3] = x2 * x3;
+      M[r][4] = x1; M[r][5] = x2; M[r][6] = x3; M[r][7] = 1.0;
+    }
+  }
+  double beta[8] = {ip[0] * ip[1] * ip[2], ip[0] * ip[1], ip[0] * ip[2], ip[1] * ip[2], ip[0], ip[1], ip[2], 1.0};
+  if (!solve8T(M, beta, o.coeff)) return 3;
+  o.ic = cell[0]; o.jc = cell[1]; o.kc = cell[2];
+  o.probe = probe;
+  for (int d = 0; d < 3; d++) { o.bi[d] = bi[d]; o.ip[d] = ip[d]; o.nrm[d] = nrm[d]; }
+  return 0;
+}
+
+}  // namespace ppg
+
+// ======================================================================
+// C-ABI
+// ======================================================================
+using namespace ppg;
+
+struct pp_geom {
+  std::vector<int8_t> iblank, fresh, ghost;
+  std::vector<int> list;       // ig, jg, kg triples (1-based)
+  std::vector<GhostOut> gout;
+  int nblank = 0, nfresh = 0;
+};
+
+extern "C" {
+
+int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const pp_body_desc *bodies, int32_t mode,
+                      double probe_norm, const int8_t *iblank_prev) {
+  if (!out || !g || nbody < 0 || nbody > 256 || (nbody > 0 && !bodies) || probe_norm <= 0.0) {
+    pp::set_error("pp_geom_build: bad argument");
+    return PP_BAD_ARG;
+  }
+  *out = nullptr;
+  if (mode != PP_GEOM_ANALYTIC && mode != PP_GEOM_TRIANGULATED) {
+    pp::set_error("pp_geom_build: unknown mode %d", mode);
+    return PP_BAD_ARG;
+  }
+  Grid G(g->nxc, g->nyc, g->nzc, g->x, g->y, g->z);
+  std::vector<Body> bs(nbody);
+  for (int b = 0; b < nbody; b++) {
+    const pp_body_desc &d = bodies[b];
+    if (d.shape != PP_BODY_ELLIPSOID || d.semi[0] <= 0 || d.semi[1] <= 0 || d.semi[2] <= 0) {
+      pp::set_error("pp_geom_build: body %d is not a valid ellipsoid", b);
+      return PP_BAD_ARG;
+    }
+    for (int q = 0; q < 3; q++) { bs[b].c[q] = d.center[q]; bs[b].r[q] = d.semi[q]; }
+    bs[b].alpha_deg = d.alpha_deg;
+    bs[b].n_theta = d.n_theta;
+  }
+  pp_geom *o = new pp_geom();
+  o->iblank.assign(G.size, 0);
+  o->fresh.assign(G.size, 0);
+  std::vector<int> bodynum(G.size, -1);
+  int rc = 0;
+  TriBodies tri;
+  if (mode == PP_GEOM_ANALYTIC) {
+    parallel_for((long)G.nzc, [&](long k0, long k1) {
+      for (long k = k0 + 1; k <= k1; k++)
+        for (int j = 1; j <= G.nyc; j++)
+          for (int i = 1; i <= G.nxc; i++) {
+            double p[3] = {G.XC[0][i], G.XC[1][j], G.XC[2][k]};
+            for (int b = 0; b < nbody; b++)
+              if (inside_analytic(bs[b], p)) {
+                o->iblank[G.r(i, j, (int)k)] = 1;
+                bodynum[G.r(i, j, (int)k)] = b;
+                break;
+              }
+          }
+    });
+  } else {
+    rc = tri_build(G, bs, tri);
+    if (rc == 0) rc = tri_iblank(G, tri, o->iblank, bodynum);
+    if (rc) {
+      delete o;
+      pp::set_error("pp_geom_build: triangulated iblank failed (%d)", rc);
+      return PP_BAD_GEOMETRY;
+    }
+  }
+  // fresh cells: fluid now, blank before (BOUNDARY_IBLANK_FAST.F90:451-457)
+  if (iblank_prev)
+    for (int k = 1; k <= G.nzc; k++)
+      for (int j = 1; j <= G.nyc; j++)
+        for (int i = 1; i <= G.nxc; i++) {
+          long r = G.r(i, j, k);
+          if (o->iblank[r] == 0 && iblank_prev[r] == 1) o->fresh[r] = 1;
+        }
+  isolated_iblank(G, o->iblank);
+  fill_cell(G, o->iblank, o->fresh);
+  mark_ghosts(G, o->iblank, o->ghost, o->list);
+  int ng = (int)o->list.size() / 3;
+  o->gout.resize(ng);
+  std::vector<int> err(ng, 0);
+  parallel_for(ng, [&](long a, long b) {
+    for (long q = a; q < b; q++) {
+      int iG = o->list[3 * q], jG = o->list[3 * q + 1], kG = o->list[3 * q + 2];
+      double bi[3], nrm[3];
+      double p[3] = {G.XC[0][iG], G.XC[1][jG], G.XC[2][kG]};
+      if (mode == PP_GEOM_ANALYTIC) {
+        int b0 = bodynum[G.r(iG, jG, kG)];
+        if (b0 < 0) b0 = 0;
+        // closest body surface among all bodies
+        double best = 1e300;
+        for (int bb = 0; bb < nbody; bb++) {
+          double tb[3], tn[3];
+          closest_point_analytic(bs[bb], p, tb, tn);
+          double d2 = (tb[0] - p[0]) * (tb[0] - p[0]) + (tb[1] - p[1]) * (tb[1] - p[1]) + (tb[2] - p[2]) * (tb[2] - p[2]);
+          if (d2 < best || (bb == b0 && d2 == best)) {
+            best = d2;
+            for (int d = 0; d < 3; d++) { bi[d] = tb[d]; nrm[d] = tn[d]; }
+          }
+        }
+      } else {
+        tri_intercept(G, tri, bodynum[G.r(iG, jG, kG)], iG, jG, kG, bi, nrm, o->gout[q].elem);
+      }
+      err[q] = ghost_stencil(G, iG, jG, kG, bi, nrm, probe_norm, o->gout[q]);
+    }
+  });
+  for (int q = 0; q < ng; q++)
+    if (err[q]) {
+      pp::set_error("pp_geom_build: ghost %d at (%d,%d,%d): %s", q, o->list[3 * q], o->list[3 * q + 1],
+                    o->list[3 * q + 2],
+                    err[q] == 1 ? "image point cell not found" : err[q] == 2 ? "intercept too long" : "singular Vandermonde");
+      delete o;
+      return PP_BAD_GEOMETRY;
+    }
+  for (long r = 0; r < G.size; r++) { o->nblank += o->iblank[r]; o->nfresh += o->fresh[r]; }
+  *out = o;
+  return PP_OK;
+}
+
+int32_t pp_geom_counts(const pp_geom *o, int32_t *n_ghost, int32_t *n_blank, int32_t *n_fresh) {
+  if (!o) { pp::set_error("pp_geom_counts: null handle"); return PP_BAD_ARG; }
+  if (n_ghost) *n_ghost = (int32_t)o->gout.size();
+  if (n_blank) *n_blank = o->nblank;
+  if (n_fresh) *n_fresh = o->nfresh;
+  return PP_OK;
+}
+
+int32_t pp_geom_get(const pp_geom *o, int8_t *iblank, int8_t *fresh, int32_t *ig, int32_t *jg, int32_t *kg,
+                    int32_t *ic, int32_t *jc, int32_t *kc, double *coeff, double *probe, double *bi, double *ip,
+                    double *nrm) {
+  if (!o) { pp::set_error("pp_geom_get: null handle"); return PP_BAD_ARG; }
+  if (iblank) memcpy(iblank, o->iblank.data(), o->iblank.size());
+  if (fresh) memcpy(fresh, o->fresh.data(), o->fresh.size());
+  for (size_t q = 0; q < o->gout.size(); q++) {
+    const GhostOut &g = o->gout[q];
+    if (ig) ig[q] = o->list[3 * q];
+    if (jg) jg[q] = o->list[3 * q + 1];
+    if (kg) kg[q] = o->list[3 * q + 2];
+    if (ic) ic[q] = g.ic;
+    if (jc) jc[q] = g.jc;
+    if (kc) kc[q] = g.kc;
+    if (coeff) memcpy(coeff + 8 * q, g.coeff, 8 * sizeof(double));
+    if (probe) probe[q] = g.probe;
+    for (int d = 0; d < 3; d++) {
+      if (bi) bi[3 * q + d] = g.bi[d];
+      if (ip) ip[3 * q + d] = g.ip[d];
+      if (nrm) nrm[3 * q + d] = g.nrm[d];
+    }
+  }
+  return PP_OK;
+}
+
+int32_t pp_geom_destroy(pp_geom *o) {
+  delete o;
+  return PP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// Internal declarations of the host geometry module.
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <vector>
+
+#include "pp_hip.h"
+
+namespace pp {
+void set_error(const char *fmt, ...);
+}
+
+namespace ppg {
+
+void parallel_for(long n, const std::function<void(long, long)> &fn);
+
+// Reference grid arrays x(0:n+1), xc(0:n+1), dx(0:n+1) per direction
+// (make_grid, CORE_GRID.F90:29-151) and the reference field indexing.
+struct Grid {
+  int nxc, nyc, nzc;
+  std::vector<double> X[3], XC[3], DX[3];
+  long ex, exy, size;
+  Grid(int nxc, int nyc, int nzc, const double *xn, const double *yn, const double *zn);
+  long r(int i, int j, int k) const { return (long)i + ex * j + exy * k; }
+  double dxc(int d, int i) const { return XC[d][i] - XC[d][i - 1]; }
+};
+
+struct Body {
+  double c[3], r[3], alpha_deg;
+  int n_theta;
+};
+
+struct GhostOut {
+  int ic = -1, jc = -1, kc = -1;
+  int elem = 0;
+  double coeff[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double probe = 0, bi[3] = {0, 0, 0}, ip[3] = {0, 0, 0}, nrm[3] = {0, 0, 0};
+};
+
+// triangulated bodies (pp_geom_tri.cpp)
+struct TriBody {
+  int nm = 0, ne = 0;
+  std::vector<double> xm, ym, zm;            // markers (0-based)
+  std::vector<int> tri;                      // 3 per element, 1-based marker ids
+  std::vector<double> cx, cy, cz, nx, ny, nz, area;
+  std::vector<std::vector<int>> vert_elems;  // elements sharing each marker
+};
+struct TriBodies {
+  std::vector<TriBody> b;
+};
+int tri_build(const Grid &G, const std::vector<Body> &bodies, TriBodies &T);
+int tri_iblank(const Grid &G, const TriBodies &T, std::vector<int8_t> &iblank, std::vector<int> &bodynum);
+void tri_intercept(const Grid &G, const TriBodies &T, int body, int iG, int jG, int kG, double *bi, double *nrm,
+                   int &elem);
+
+bool inside_analytic(const Body &b, const double *p);
+void closest_point_analytic(const Body &b, const double *p, double *bi, double *nin);
+void isolated_iblank(const Grid &G, std::vector<int8_t> &ib);
+void fill_cell(const Grid &G, std::vector<int8_t> &ib, std::vector<int8_t> &fresh);
+void mark_ghosts(const Grid &G, const std::vector<int8_t> &ib, std::vector<int8_t> &gm, std::vector<int> &list);
+int ghost_stencil(const Grid &G, int iG, int jG, int kG, const double *bi, const double *nrm, double probe_norm,
+                  GhostOut &o);
+
+}  // namespace ppg

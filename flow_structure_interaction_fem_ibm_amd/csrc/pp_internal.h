@@ -1,0 +1,148 @@
+// Internal declarations of the MI355X pressure-Poisson library.
+//
+// Device data layout (one level of the hierarchy, FP32 or FP64):
+//   cell (i,j,k), 0-based, i in [-1,cx], j in [-1,cy], k in [-1,cz]
+//   (one halo cell each side) lives at
+//       OX + i + sx*(j+1) + sxy*(k+1)
+//   with OX = 4 and the row pitch sx a multiple of 16 elements, so the
+//   interior of every row starts 16-byte aligned and every row starts
+//   64-byte aligned.  One flag byte per cell uses the same offsets:
+//       bit0 blank (iblank=1), bit1 ghost (ghostCellMark=1),
+//       bit2..7 face closed (Neumann) on the i-,i+,j-,j+,k-,k+ face.
+// The FP64 solution and RHS stay in the caller's reference layout
+// ((0:nx+1,0:ny+1,0:nz+1), i fastest) so no layout copy is needed at the
+// boundary.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pp_hip.h"
+
+namespace pp {
+
+constexpr int OX = 4;
+constexpr uint8_t FL_BLANK = 1, FL_GHOST = 2, FL_IM = 4, FL_IP = 8,
+                  FL_JM = 16, FL_JP = 32, FL_KM = 64, FL_KP = 128;
+
+__host__ __device__ inline long lidx(int sx, long sxy, int i, int j, int k) {
+  return (long)OX + i + (long)sx * (j + 1) + sxy * (k + 1);
+}
+
+// Per-level device view passed by value to kernels.
+template <typename T>
+struct LevelArgs {
+  int cx, cy, cz;
+  int sx;
+  long sxy;
+  const uint8_t *fl;
+  const T *cmx, *cpx, *cmy, *cpy, *cmz, *cpz;  // face coefficients per cell
+  const T *wx, *wy, *wz;                        // cell widths
+};
+
+// Prolongation map of one direction of a fine level onto its coarse parent.
+template <typename T>
+struct ProlongDir {
+  const int *par, *oth;   // parent and second coarse neighbour per fine index
+  const T *wpar;          // weight of the parent
+};
+
+// Reference-layout view of the finest level (FP64 fields of the caller).
+struct RefArgs {
+  int nxc, nyc, nzc;
+  long ex, exy;           // extents nxc+3, (nxc+3)*(nyc+3)
+};
+__host__ __device__ inline long ridx(const RefArgs &r, int i, int j, int k) {
+  // 0-based interior cell (i,j,k) = reference (i+1,j+1,k+1)
+  return (long)(i + 1) + r.ex * (j + 1) + r.exy * (k + 1);
+}
+
+struct Level {
+  int cx = 0, cy = 0, cz = 0, sx = 0, sy = 0;
+  long sxy = 0, total = 0;
+  int cf[3] = {1, 1, 1};  // coarsening factor of this level w.r.t. the previous
+  std::vector<double> node[3], xc[3], w[3], cm[3], cp[3];
+  // maps from this (fine) level onto level+1
+  std::vector<int> par[3], oth[3];
+  std::vector<double> wpar[3];
+  std::vector<uint8_t> hflags;
+  // device
+  uint8_t *flags = nullptr;
+  double *cm64[3] = {}, *cp64[3] = {}, *w64[3] = {}, *wpar64[3] = {};
+  float *cm32[3] = {}, *cp32[3] = {}, *w32[3] = {}, *wpar32[3] = {};
+  int *par_d[3] = {}, *oth_d[3] = {};
+  float *e32 = nullptr, *rhs32 = nullptr, *res32 = nullptr;
+  double *e64 = nullptr, *rhs64 = nullptr, *res64 = nullptr;
+};
+
+struct GhostTables {
+  int n = 0;
+  int kexact = 1;                 // Jacobi sweeps that solve the ghost rows
+  // forward gather (device)
+  long *g_ref = nullptr, *g_al = nullptr;       // ghost cell offsets
+  long *s_ref = nullptr, *s_al = nullptr;       // 8 stencil offsets per ghost
+  double *coef64 = nullptr;                     // 8 per ghost, self entry 0
+  float *coef32 = nullptr;
+  double *bself = nullptr;                      // c_self + probeLength
+  double *bnd = nullptr;                        // bself * uNormal (per solve)
+  double *buf64 = nullptr;
+  float *buf32 = nullptr;
+  // adjoint: fluid neighbours of each ghost and their face conductances
+  long *nb_ref = nullptr, *nb_al = nullptr;     // 6 per ghost (-1: none)
+  double *nb_w = nullptr;
+  // transpose of the stencil: touched cells and the ghosts reading them
+  int nt = 0;
+  long *t_ref = nullptr, *t_al = nullptr;
+  int *t_ptr = nullptr, *t_ghost = nullptr;
+  double *t_coef = nullptr;
+  int *g_tidx = nullptr;                        // touched index of a ghost cell or -1
+  // host copies
+  std::vector<long> h_g_ref, h_g_al, h_s_ref, h_s_al, h_nb_ref, h_nb_al, h_t_ref, h_t_al;
+  std::vector<double> h_coef, h_bself, h_nb_w, h_t_coef;
+  std::vector<int> h_t_ptr, h_t_ghost, h_g_tidx;
+  std::vector<int> h_ig, h_jg, h_kg;            // 1-based reference indices
+  int solid_in_stencil = 0;                     // stencil entries on solid non-ghost cells
+};
+
+struct CoarseSolve {
+  int n = 0;                      // cells of the coarsest level
+  double *G = nullptr;            // n*n, G[j*n+i] = d e_i / d r_j
+};
+
+}  // namespace pp
+
+struct pp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nxc = 0, nyc = 0, nzc = 0;
+  pp::RefArgs ref{};
+  long nref = 0;                       // reference field size
+  std::vector<double> hx[3];           // reference node coords x(0..n+1)
+  std::vector<double> rxc[3], rdx[3], rdxinv[3], rdxc[3], rdxcinv[3];
+  std::vector<pp::Level> lv;
+  bool have_geom = false;
+  std::vector<int8_t> h_iblank, h_ghost, h_fresh;  // reference layout
+  pp::GhostTables gh;
+  pp::CoarseSolve cs;
+  // FP64 work arrays in reference layout
+  double *d_p = nullptr, *d_b = nullptr, *d_w = nullptr, *d_bwork = nullptr;
+  double *d_tmp = nullptr;
+  int8_t *d_fresh = nullptr;
+  int8_t *d_iblank_ref = nullptr;
+  // reductions
+  double *d_part = nullptr;            // partial sums
+  double *d_res = nullptr;             // final reduction results
+  double *h_res = nullptr;             // pinned host mirror
+  int nblk_red = 0;
+  // graph cache
+  hipGraphExec_t gexec = nullptr;
+  int graph_key = -1;
+  double bytes_acc = 0;
+};
+
+namespace pp {
+void set_error(const char *fmt, ...);
+}

@@ -1,0 +1,19 @@
+// Host-side set-up routines shared by the API translation unit.
+#pragma once
+
+#include "pp_internal.h"
+
+namespace pp {
+const char *last_error();
+void make_grid_dir(int n, const double *xn, std::vector<double> &x, std::vector<double> &xc,
+                   std::vector<double> &dx, std::vector<double> &dxinv, std::vector<double> &dxc,
+                   std::vector<double> &dxcinv);
+int build_levels(pp_ctx *c);
+void build_flags(pp_ctx *c);
+int upload_levels(pp_ctx *c);
+void free_levels(pp_ctx *c);
+void free_ghosts(pp_ctx *c);
+void derive_ghost_mark(pp_ctx *c);
+int build_ghosts(pp_ctx *c, const pp_geometry *g);
+int build_coarse(pp_ctx *c);
+}  // namespace pp

@@ -1,0 +1,501 @@
+// Host-side set-up of a context: grid metrics, level hierarchy, flags,
+// ghost tables and the coarsest-level pseudo-inverse.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <random>
+
+#include "pp_internal.h"
+#include "pp_setup.h"
+
+namespace pp {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+const char *last_error() { return g_err.c_str(); }
+
+// make_grid (CORE_GRID.F90:29-151) for one direction.  xn = x(1..n),
+// outputs indexed 0..n+1 like the reference arrays.
+void make_grid_dir(int n, const double *xn, std::vector<double> &x, std::vector<double> &xc,
+                   std::vector<double> &dx, std::vector<double> &dxinv, std::vector<double> &dxc,
+                   std::vector<double> &dxcinv) {
+  x.assign(n + 2, 0.0); xc.assign(n + 2, 0.0); dx.assign(n + 2, 0.0);
+  dxinv.assign(n + 2, 0.0); dxc.assign(n + 2, 0.0); dxcinv.assign(n + 2, 0.0);
+  for (int i = 1; i <= n; i++) x[i] = xn[i - 1];
+  x[0] = -x[2];
+  x[n + 1] = x[n] + x[n] - x[n - 1];
+  for (int i = 0; i <= n; i++) {
+    xc[i] = 0.5 * (x[i + 1] + x[i]);
+    dx[i] = x[i + 1] - x[i];
+    dxinv[i] = 1.0 / dx[i];
+  }
+  xc[n + 1] = x[n + 1];
+  for (int i = 1; i <= n; i++) {
+    dxc[i] = xc[i] - xc[i - 1];
+    dxcinv[i] = 1.0 / dxc[i];
+  }
+}
+
+// Metrics of one direction of a level from its nodes X[0..n] (0-based,
+// X[q] = x(q+1) of the reference): cell widths, centres and the two face
+// coefficients of every cell, dxcinv(face)*dxinv(cell), formed in the same
+// order as the reference (x(0) = -x(2), x(n+1) = 2x(n)-x(n-1); MG_initial
+// CORE_MG.F90:186-205 builds the coarse levels the same way).
+static void level_metrics(const std::vector<double> &X, std::vector<double> &w, std::vector<double> &xc,
+                          std::vector<double> &cm, std::vector<double> &cp) {
+  int n = (int)X.size() - 1;
+  std::vector<double> x(n + 3);  // reference indexing 0..n+2 (x(q) = X[q-1])
+  for (int q = 1; q <= n + 1; q++) x[q] = X[q - 1];
+  x[0] = -x[2];
+  x[n + 2] = x[n + 1] + x[n + 1] - x[n];
+  std::vector<double> rxc(n + 3), rdx(n + 3), rdxinv(n + 3), rdxcinv(n + 3);
+  for (int i = 0; i <= n + 1; i++) {
+    rxc[i] = 0.5 * (x[i + 1] + x[i]);
+    rdx[i] = x[i + 1] - x[i];
+    rdxinv[i] = 1.0 / rdx[i];
+  }
+  for (int i = 1; i <= n + 1; i++) rdxcinv[i] = 1.0 / (rxc[i] - rxc[i - 1]);
+  w.assign(n, 0.0); xc.assign(n, 0.0); cm.assign(n, 0.0); cp.assign(n, 0.0);
+  for (int c = 0; c < n; c++) {
+    int i = c + 1;
+    w[c] = rdx[i];
+    xc[c] = rxc[i];
+    cm[c] = rdxcinv[i] * rdxinv[i];
+    cp[c] = rdxcinv[i + 1] * rdxinv[i];
+  }
+}
+
+template <typename T>
+static T *dev_upload(const std::vector<T> &h) {
+  T *d = nullptr;
+  if (h.empty()) return nullptr;
+  if (hipMalloc(&d, sizeof(T) * h.size()) != hipSuccess) return nullptr;
+  hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+  return d;
+}
+template <typename T>
+static std::vector<float> to_f32(const std::vector<T> &h) {
+  std::vector<float> r(h.size());
+  for (size_t i = 0; i < h.size(); i++) r[i] = (float)h[i];
+  return r;
+}
+
+void free_levels(pp_ctx *c) {
+  for (auto &L : c->lv) {
+    hipFree(L.flags);
+    for (int d = 0; d < 3; d++) {
+      hipFree(L.cm64[d]); hipFree(L.cp64[d]); hipFree(L.w64[d]); hipFree(L.wpar64[d]);
+      hipFree(L.cm32[d]); hipFree(L.cp32[d]); hipFree(L.w32[d]); hipFree(L.wpar32[d]);
+      hipFree(L.par_d[d]); hipFree(L.oth_d[d]);
+    }
+    hipFree(L.e32); hipFree(L.rhs32); hipFree(L.res32);
+    hipFree(L.e64); hipFree(L.rhs64); hipFree(L.res64);
+  }
+  c->lv.clear();
+}
+
+void free_ghosts(pp_ctx *c) {
+  GhostTables &g = c->gh;
+  void *ptrs[] = {g.g_ref, g.g_al, g.s_ref, g.s_al, g.coef64, g.coef32, g.bself, g.bnd, g.buf64, g.buf32,
+                  g.nb_ref, g.nb_al, g.nb_w, g.t_ref, g.t_al, g.t_ptr, g.t_ghost, g.t_coef, g.g_tidx};
+  for (void *p : ptrs) hipFree(p);
+  g = GhostTables();
+  hipFree(c->cs.G);
+  c->cs = CoarseSolve();
+}
+
+// Level hierarchy: level 0 is the grid; a direction is halved while its
+// cell count is even and at least 4; coarsening stops when the level has at
+// most kCoarseMax cells or no direction can be halved.
+static constexpr long kCoarseMax = 512;
+static constexpr int kMaxLevels = 12;
+
+int build_levels(pp_ctx *c) {
+  free_levels(c);
+  std::vector<Level> lv;
+  Level L0;
+  L0.cx = c->nxc; L0.cy = c->nyc; L0.cz = c->nzc;
+  for (int d = 0; d < 3; d++) {
+    int n = d == 0 ? c->nxc : d == 1 ? c->nyc : c->nzc;
+    L0.node[d].assign(c->hx[d].begin() + 1, c->hx[d].begin() + n + 2);  // x(1..n+1)
+  }
+  lv.push_back(L0);
+  while ((int)lv.size() < kMaxLevels) {
+    Level &F = lv.back();
+    long cells = (long)F.cx * F.cy * F.cz;
+    if (cells <= kCoarseMax) break;
+    int ncell[3] = {F.cx, F.cy, F.cz};
+    int cf[3];
+    bool any = false;
+    for (int d = 0; d < 3; d++) {
+      cf[d] = (ncell[d] % 2 == 0 && ncell[d] >= 4) ? 2 : 1;
+      any |= cf[d] == 2;
+    }
+    if (!any) break;
+    Level C;
+    C.cx = F.cx / cf[0]; C.cy = F.cy / cf[1]; C.cz = F.cz / cf[2];
+    for (int d = 0; d < 3; d++) {
+      C.cf[d] = cf[d];
+      if (cf[d] == 1) {
+        C.node[d] = F.node[d];
+      } else {
+        int nc = ncell[d] / 2;
+        C.node[d].resize(nc + 1);
+        for (int q = 0; q <= nc; q++) C.node[d][q] = F.node[d][2 * q];
+      }
+    }
+    lv.push_back(C);
+  }
+  for (auto &L : lv) {
+    L.sx = ((L.cx + 8 + 15) / 16) * 16;
+    L.sy = L.cy + 2;
+    L.sxy = (long)L.sx * L.sy;
+    L.total = L.sxy * (L.cz + 2) + 64;
+    for (int d = 0; d < 3; d++) level_metrics(L.node[d], L.w[d], L.xc[d], L.cm[d], L.cp[d]);
+  }
+  // prolongation maps fine l -> coarse l+1
+  for (size_t l = 0; l + 1 < lv.size(); l++) {
+    Level &F = lv[l], &C = lv[l + 1];
+    for (int d = 0; d < 3; d++) {
+      int nf = (int)F.w[d].size(), nc = (int)C.w[d].size();
+      F.par[d].assign(nf, 0); F.oth[d].assign(nf, 0); F.wpar[d].assign(nf, 1.0);
+      for (int i = 0; i < nf; i++) {
+        if (C.cf[d] == 1) { F.par[d][i] = i; F.oth[d][i] = i; F.wpar[d][i] = 1.0; continue; }
+        int P = i / 2;
+        int O = (i % 2 == 0) ? P - 1 : P + 1;
+        F.par[d][i] = P;
+        if (O < 0 || O >= nc) { F.oth[d][i] = P; F.wpar[d][i] = 1.0; continue; }
+        F.oth[d][i] = O;
+        F.wpar[d][i] = (F.xc[d][i] - C.xc[d][O]) / (C.xc[d][P] - C.xc[d][O]);
+      }
+    }
+  }
+  c->lv = std::move(lv);
+  return PP_OK;
+}
+
+// Flags of every level.  Level 0: blank/ghost from the geometry, closed
+// faces on the outer boundary only (set_outer_iup, BOUNDARY_SET.F90:54-92;
+// under GCM the fine internal faces stay open, so fluid cells couple to the
+// ghost values).  Coarse levels: blank if every child is blank, no ghosts,
+// a face closed where a fluid cell meets a blank neighbour
+// (MG_Prepare, CORE_MG.F90:1736-1781) or the outer boundary.
+void build_flags(pp_ctx *c) {
+  for (size_t l = 0; l < c->lv.size(); l++) {
+    Level &L = c->lv[l];
+    L.hflags.assign(L.total, (uint8_t)(FL_BLANK));
+    std::vector<uint8_t> blank((size_t)L.cx * L.cy * L.cz, 0), ghost((size_t)L.cx * L.cy * L.cz, 0);
+    auto cidx = [&](int i, int j, int k) { return (size_t)i + (size_t)L.cx * (j + (size_t)L.cy * k); };
+    if (l == 0) {
+      for (int k = 0; k < L.cz; k++)
+        for (int j = 0; j < L.cy; j++)
+          for (int i = 0; i < L.cx; i++) {
+            long r = ridx(c->ref, i, j, k);
+            blank[cidx(i, j, k)] = c->h_iblank[r] ? 1 : 0;
+            ghost[cidx(i, j, k)] = c->h_ghost[r] ? 1 : 0;
+          }
+    } else {
+      Level &F = c->lv[l - 1];
+      const std::vector<uint8_t> &fh = F.hflags;
+      for (int k = 0; k < L.cz; k++)
+        for (int j = 0; j < L.cy; j++)
+          for (int i = 0; i < L.cx; i++) {
+            int all = 1;
+            for (int c3 = 0; c3 < L.cf[2]; c3++)
+              for (int b = 0; b < L.cf[1]; b++)
+                for (int a = 0; a < L.cf[0]; a++) {
+                  long q = lidx(F.sx, F.sxy, i * L.cf[0] + a, j * L.cf[1] + b, k * L.cf[2] + c3);
+                  all &= (fh[q] & FL_BLANK) ? 1 : 0;
+                }
+            blank[cidx(i, j, k)] = (uint8_t)all;
+          }
+    }
+    for (int k = 0; k < L.cz; k++)
+      for (int j = 0; j < L.cy; j++)
+        for (int i = 0; i < L.cx; i++) {
+          uint8_t f = 0;
+          bool bl = blank[cidx(i, j, k)];
+          if (bl) f |= FL_BLANK;
+          if (ghost[cidx(i, j, k)]) f |= FL_GHOST;
+          if (i == 0) f |= FL_IM;
+          if (i == L.cx - 1) f |= FL_IP;
+          if (j == 0) f |= FL_JM;
+          if (j == L.cy - 1) f |= FL_JP;
+          if (k == 0) f |= FL_KM;
+          if (k == L.cz - 1) f |= FL_KP;
+          if (l > 0 && !bl) {
+            if (i > 0 && blank[cidx(i - 1, j, k)]) f |= FL_IM;
+            if (i < L.cx - 1 && blank[cidx(i + 1, j, k)]) f |= FL_IP;
+            if (j > 0 && blank[cidx(i, j - 1, k)]) f |= FL_JM;
+            if (j < L.cy - 1 && blank[cidx(i, j + 1, k)]) f |= FL_JP;
+            if (k > 0 && blank[cidx(i, j, k - 1)]) f |= FL_KM;
+            if (k < L.cz - 1 && blank[cidx(i, j, k + 1)]) f |= FL_KP;
+          }
+          L.hflags[lidx(L.sx, L.sxy, i, j, k)] = f;
+        }
+  }
+}
+
+int upload_levels(pp_ctx *c) {
+  for (size_t l = 0; l < c->lv.size(); l++) {
+    Level &L = c->lv[l];
+    L.flags = dev_upload(L.hflags);
+    for (int d = 0; d < 3; d++) {
+      L.cm64[d] = dev_upload(L.cm[d]); L.cp64[d] = dev_upload(L.cp[d]); L.w64[d] = dev_upload(L.w[d]);
+      L.cm32[d] = dev_upload(to_f32(L.cm[d])); L.cp32[d] = dev_upload(to_f32(L.cp[d]));
+      L.w32[d] = dev_upload(to_f32(L.w[d]));
+      if (l + 1 < c->lv.size()) {
+        L.wpar64[d] = dev_upload(L.wpar[d]); L.wpar32[d] = dev_upload(to_f32(L.wpar[d]));
+        L.par_d[d] = dev_upload(L.par[d]); L.oth_d[d] = dev_upload(L.oth[d]);
+      }
+    }
+    size_t n = L.total;
+    if (hipMalloc(&L.e32, n * 4) || hipMalloc(&L.rhs32, n * 4) || hipMalloc(&L.res32, n * 4) ||
+        hipMalloc(&L.e64, n * 8) || hipMalloc(&L.rhs64, n * 8) || hipMalloc(&L.res64, n * 8)) {
+      set_error("hipMalloc failed for level %zu (%zu cells)", l, n);
+      return PP_HIP_ERROR;
+    }
+    hipMemset(L.e32, 0, n * 4); hipMemset(L.rhs32, 0, n * 4); hipMemset(L.res32, 0, n * 4);
+    hipMemset(L.e64, 0, n * 8); hipMemset(L.rhs64, 0, n * 8); hipMemset(L.res64, 0, n * 8);
+    if (!L.flags) { set_error("hipMalloc failed for level flags"); return PP_HIP_ERROR; }
+  }
+  return PP_OK;
+}
+
+// Ghost marking exactly as GCM_set_internal_boundary (GCM_SET_BOUNDARY.F90:49-73):
+// a blank cell with a fluid 6-neighbour, neighbours clamped to [1,n-1].
+void derive_ghost_mark(pp_ctx *c) {
+  const auto &ib = c->h_iblank;
+  c->h_ghost.assign(c->nref, 0);
+  int nx = c->nxc + 1, ny = c->nyc + 1, nz = c->nzc + 1;
+  auto R = [&](int i, int j, int k) { return (long)i + c->ref.ex * j + c->ref.exy * k; };
+  for (int k = 1; k <= nz - 1; k++)
+    for (int j = 1; j <= ny - 1; j++)
+      for (int i = 1; i <= nx - 1; i++) {
+        int iM = std::max(i - 1, 1), iP = std::min(i + 1, nx - 1);
+        int jM = std::max(j - 1, 1), jP = std::min(j + 1, ny - 1);
+        int kM = std::max(k - 1, 1), kP = std::min(k + 1, nz - 1);
+        int b = ib[R(i, j, k)];
+        if (b == 1 && (ib[R(iM, j, k)] == 0 || ib[R(iP, j, k)] == 0 || ib[R(i, jM, k)] == 0 ||
+                       ib[R(i, jP, k)] == 0 || ib[R(i, j, kM)] == 0 || ib[R(i, j, kP)] == 0))
+          c->h_ghost[R(i, j, k)] = 1;
+      }
+}
+
+static const int incI[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+static const int incJ[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+static const int incK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+
+int build_ghosts(pp_ctx *c, const pp_geometry *g) {
+  GhostTables &G = c->gh;
+  Level &L0 = c->lv[0];
+  int n = g->n_ghost;
+  G.n = n;
+  G.h_ig.resize(n); G.h_jg.resize(n); G.h_kg.resize(n);
+  G.h_g_ref.resize(n); G.h_g_al.resize(n);
+  G.h_s_ref.resize(8L * n); G.h_s_al.resize(8L * n); G.h_coef.resize(8L * n);
+  G.h_bself.resize(n);
+  G.solid_in_stencil = 0;
+  auto inb = [&](int i, int j, int k) {
+    return i >= 1 && i <= c->nxc && j >= 1 && j <= c->nyc && k >= 1 && k <= c->nzc;
+  };
+  for (int q = 0; q < n; q++) {
+    int iG = g->i_ghost[q], jG = g->j_ghost[q], kG = g->k_ghost[q];
+    if (!inb(iG, jG, kG)) { set_error("ghost %d index (%d,%d,%d) outside the grid", q, iG, jG, kG); return PP_BAD_GEOMETRY; }
+    long rg = ridx(c->ref, iG - 1, jG - 1, kG - 1);
+    if (c->h_ghost[rg] != 1) {
+      set_error("ghost %d at (%d,%d,%d) is not a ghost cell of iblank", q, iG, jG, kG);
+      return PP_BAD_GEOMETRY;
+    }
+    G.h_ig[q] = iG; G.h_jg[q] = jG; G.h_kg[q] = kG;
+    G.h_g_ref[q] = rg;
+    G.h_g_al[q] = lidx(L0.sx, L0.sxy, iG - 1, jG - 1, kG - 1);
+    double cself = 0.0;
+    for (int r = 0; r < 8; r++) {
+      int ii = g->i_cell[q] + incI[r], jj = g->j_cell[q] + incJ[r], kk = g->k_cell[q] + incK[r];
+      double cf = g->coeff[8L * q + r];
+      if (!std::isfinite(cf)) { set_error("ghost %d coefficient %d is not finite", q, r); return PP_BAD_GEOMETRY; }
+      if (ii < 0 || jj < 0 || kk < 0 || ii > c->nxc + 1 || jj > c->nyc + 1 || kk > c->nzc + 1) {
+        set_error("ghost %d stencil cell (%d,%d,%d) outside the grid", q, ii, jj, kk);
+        return PP_BAD_GEOMETRY;
+      }
+      if (ii == iG && jj == jG && kk == kG) {
+        cself += cf;
+        G.h_coef[8L * q + r] = 0.0;
+        G.h_s_ref[8L * q + r] = rg;
+        G.h_s_al[8L * q + r] = G.h_g_al[q];
+        continue;
+      }
+      // outer ghost layer holds the Neumann copy of the adjacent interior cell
+      int ci = std::min(std::max(ii, 1), c->nxc), cj = std::min(std::max(jj, 1), c->nyc),
+          ck = std::min(std::max(kk, 1), c->nzc);
+      long rs = ridx(c->ref, ci - 1, cj - 1, ck - 1);
+      if (c->h_iblank[rs] == 1 && c->h_ghost[rs] != 1) G.solid_in_stencil++;
+      G.h_coef[8L * q + r] = cf;
+      G.h_s_ref[8L * q + r] = rs;
+      G.h_s_al[8L * q + r] = lidx(L0.sx, L0.sxy, ci - 1, cj - 1, ck - 1);
+    }
+    G.h_bself[q] = cself + g->probe_length[q];
+  }
+  // Jacobi sweeps that solve the ghost-ghost coupling to round-off
+  // (the reference iterates Gauss-Seidel to restol_ad).
+  {
+    std::vector<double> p(c->nref, 0.0), nw(n);
+    std::mt19937_64 rng(12345);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (long r = 0; r < c->nref; r++) p[r] = c->h_iblank[r] ? 0.0 : U(rng);
+    for (int q = 0; q < n; q++) p[G.h_g_ref[q]] = 0.0;
+    int it = 0;
+    double dmax = 1.0;
+    while (it < 200 && dmax > 1e-15) {
+      dmax = 0.0;
+      for (int q = 0; q < n; q++) {
+        double s = 0.0;
+        for (int r = 0; r < 8; r++) s += G.h_coef[8L * q + r] * p[G.h_s_ref[8L * q + r]];
+        nw[q] = s;
+      }
+      for (int q = 0; q < n; q++) {
+        dmax = std::max(dmax, std::fabs(nw[q] - p[G.h_g_ref[q]]));
+        p[G.h_g_ref[q]] = nw[q];
+      }
+      it++;
+    }
+    if (dmax > 1e-15) {
+      set_error("ghost rows do not converge under Jacobi iteration (change %g after %d sweeps)", dmax, it);
+      return PP_BAD_GEOMETRY;
+    }
+    G.kexact = std::max(it, 1);
+  }
+  G.g_ref = dev_upload(G.h_g_ref); G.g_al = dev_upload(G.h_g_al);
+  G.s_ref = dev_upload(G.h_s_ref); G.s_al = dev_upload(G.h_s_al);
+  G.coef64 = dev_upload(G.h_coef); G.coef32 = dev_upload(to_f32(G.h_coef));
+  G.bself = dev_upload(G.h_bself);
+  if (n > 0) {
+    hipMalloc(&G.bnd, sizeof(double) * n);
+    hipMemset(G.bnd, 0, sizeof(double) * n);
+    hipMalloc(&G.buf64, sizeof(double) * n);
+    hipMalloc(&G.buf32, sizeof(float) * n);
+  }
+  return PP_OK;
+}
+
+// Dense LU with partial pivoting (in place), then solve for one rhs.
+static bool lu_factor(std::vector<double> &A, int n, std::vector<int> &piv) {
+  piv.resize(n);
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    double mx = std::fabs(A[(long)k * n + k]);
+    for (int i = k + 1; i < n; i++)
+      if (std::fabs(A[(long)i * n + k]) > mx) { mx = std::fabs(A[(long)i * n + k]); p = i; }
+    if (mx == 0.0) return false;
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < n; j++) std::swap(A[(long)k * n + j], A[(long)p * n + j]);
+    double inv = 1.0 / A[(long)k * n + k];
+    for (int i = k + 1; i < n; i++) {
+      double f = A[(long)i * n + k] * inv;
+      A[(long)i * n + k] = f;
+      if (f != 0.0)
+        for (int j = k + 1; j < n; j++) A[(long)i * n + j] -= f * A[(long)k * n + j];
+    }
+  }
+  return true;
+}
+// lu_factor swaps whole rows (multipliers included), so all interchanges
+// are applied to b first, then the unit-lower and upper solves.
+static void lu_solve(const std::vector<double> &A, int n, const std::vector<int> &piv, std::vector<double> &b) {
+  for (int k = 0; k < n; k++)
+    if (piv[k] != k) std::swap(b[k], b[piv[k]]);
+  for (int k = 0; k < n; k++) {
+    double bk = b[k];
+    if (bk != 0.0)
+      for (int i = k + 1; i < n; i++) b[i] -= A[(long)i * n + k] * bk;
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double s = b[i];
+    for (int j = i + 1; j < n; j++) s -= A[(long)i * n + j] * b[j];
+    b[i] = s / A[(long)i * n + i];
+  }
+}
+
+// Coarsest level: G maps the coarse rhs to the correction that solves the
+// coarse equations after removing the volume-weighted mean of the rhs on
+// every connected fluid component, with zero volume-weighted mean of the
+// correction (bordered system, one constraint per component).
+int build_coarse(pp_ctx *c) {
+  Level &L = c->lv.back();
+  int n = L.cx * L.cy * L.cz;
+  auto cid = [&](int i, int j, int k) { return i + L.cx * (j + L.cy * k); };
+  std::vector<int> comp(n, -1);
+  std::vector<char> fluid(n, 0);
+  for (int k = 0; k < L.cz; k++)
+    for (int j = 0; j < L.cy; j++)
+      for (int i = 0; i < L.cx; i++) fluid[cid(i, j, k)] = !(L.hflags[lidx(L.sx, L.sxy, i, j, k)] & FL_BLANK);
+  int ncomp = 0;
+  std::vector<int> stack;
+  for (int s = 0; s < n; s++) {
+    if (!fluid[s] || comp[s] >= 0) continue;
+    comp[s] = ncomp;
+    stack.push_back(s);
+    while (!stack.empty()) {
+      int q = stack.back();
+      stack.pop_back();
+      int i = q % L.cx, t = q / L.cx, j = t % L.cy, k = t / L.cy;
+      uint8_t f = L.hflags[lidx(L.sx, L.sxy, i, j, k)];
+      int nb[6][4] = {{i - 1, j, k, FL_IM}, {i + 1, j, k, FL_IP}, {i, j - 1, k, FL_JM},
+                      {i, j + 1, k, FL_JP}, {i, j, k - 1, FL_KM}, {i, j, k + 1, FL_KP}};
+      for (auto &e : nb) {
+        if (f & e[3]) continue;
+        int r = cid(e[0], e[1], e[2]);
+        if (fluid[r] && comp[r] < 0) { comp[r] = ncomp; stack.push_back(r); }
+      }
+    }
+    ncomp++;
+  }
+  int m = n + ncomp;
+  std::vector<double> A((long)m * m, 0.0);
+  for (int q = 0; q < n; q++) {
+    if (!fluid[q]) { A[(long)q * m + q] = 1.0; continue; }
+    int i = q % L.cx, t = q / L.cx, j = t % L.cy, k = t / L.cy;
+    uint8_t f = L.hflags[lidx(L.sx, L.sxy, i, j, k)];
+    double amx = (f & FL_IM) ? 0 : L.cm[0][i], apx = (f & FL_IP) ? 0 : L.cp[0][i];
+    double amy = (f & FL_JM) ? 0 : L.cm[1][j], apy = (f & FL_JP) ? 0 : L.cp[1][j];
+    double amz = (f & FL_KM) ? 0 : L.cm[2][k], apz = (f & FL_KP) ? 0 : L.cp[2][k];
+    A[(long)q * m + q] = -(amx + apx) - (amy + apy) - (amz + apz);
+    if (amx != 0) A[(long)q * m + cid(i - 1, j, k)] += amx;
+    if (apx != 0) A[(long)q * m + cid(i + 1, j, k)] += apx;
+    if (amy != 0) A[(long)q * m + cid(i, j - 1, k)] += amy;
+    if (apy != 0) A[(long)q * m + cid(i, j + 1, k)] += apy;
+    if (amz != 0) A[(long)q * m + cid(i, j, k - 1)] += amz;
+    if (apz != 0) A[(long)q * m + cid(i, j, k + 1)] += apz;
+    A[(long)q * m + n + comp[q]] = 1.0;
+    double vol = L.w[0][i] * L.w[1][j] * L.w[2][k];
+    A[(long)(n + comp[q]) * m + q] = vol;
+  }
+  std::vector<int> piv;
+  if (!lu_factor(A, m, piv)) { set_error("coarsest-level bordered matrix is singular"); return PP_BAD_GEOMETRY; }
+  std::vector<double> Gm((long)n * n, 0.0), rhs(m);
+  for (int p = 0; p < n; p++) {
+    if (!fluid[p]) continue;
+    std::fill(rhs.begin(), rhs.end(), 0.0);
+    rhs[p] = 1.0;
+    lu_solve(A, m, piv, rhs);
+    for (int q = 0; q < n; q++) Gm[(long)p * n + q] = fluid[q] ? rhs[q] : 0.0;
+  }
+  hipFree(c->cs.G);
+  c->cs.n = n;
+  c->cs.G = dev_upload(Gm);
+  return c->cs.G ? PP_OK : PP_HIP_ERROR;
+}
+
+}  // namespace pp

@@ -1,0 +1,376 @@
+"""ctypes binding of the pressure-Poisson C-ABI (include/pp_hip.h).
+
+Mirrors the reference's pressure path (IBM/CORE_POISSON.F90 solve_poisson,
+calc_residual, subtracting_average_pressure, GCM_correct_pressure;
+IBM/CORE_TDMA.F90 tdma) on numpy arrays in the reference layout
+(0:nx+1, 0:ny+1, 0:nz+1), i fastest, i.e. numpy shape (nzc+3, nyc+3, nxc+3).
+Device-resident solves take torch tensors on a ROCm device.
+
+The library is loaded from this package directory; nothing falls back to a
+CPU path: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpp_hip.so")
+
+PP_OK = 0
+PP_NOT_CONVERGED = 1
+PP_BAD_ARG = 2
+PP_BAD_GEOMETRY = 3
+PP_HIP_ERROR = 4
+PP_NONFINITE = 5
+PP_NO_GEOMETRY = 6
+PP_BC_NEUMANN = 2
+PP_PREC_FP64 = 0
+PP_PREC_MIXED = 1
+
+_STATUS = {0: "PP_OK", 1: "PP_NOT_CONVERGED", 2: "PP_BAD_ARG", 3: "PP_BAD_GEOMETRY",
+           4: "PP_HIP_ERROR", 5: "PP_NONFINITE", 6: "PP_NO_GEOMETRY"}
+
+
+class PPError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("nxc", C.c_int32), ("nyc", C.c_int32), ("nzc", C.c_int32),
+                ("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)),
+                ("z", C.POINTER(C.c_double)), ("bc", C.c_int32 * 6), ("device", C.c_int32)]
+
+
+class Geometry(C.Structure):
+    _fields_ = [("iblank", C.POINTER(C.c_int8)), ("fresh_cell", C.POINTER(C.c_int8)),
+                ("n_ghost", C.c_int32),
+                ("i_ghost", C.POINTER(C.c_int32)), ("j_ghost", C.POINTER(C.c_int32)),
+                ("k_ghost", C.POINTER(C.c_int32)),
+                ("i_cell", C.POINTER(C.c_int32)), ("j_cell", C.POINTER(C.c_int32)),
+                ("k_cell", C.POINTER(C.c_int32)),
+                ("coeff", C.POINTER(C.c_double)), ("probe_length", C.POINTER(C.c_double))]
+
+
+class BodyDesc(C.Structure):
+    _fields_ = [("shape", C.c_int32), ("center", C.c_double * 3), ("semi", C.c_double * 3),
+                ("alpha_deg", C.c_double), ("n_theta", C.c_int32)]
+
+
+PP_BODY_ELLIPSOID = 3
+PP_GEOM_ANALYTIC = 0
+PP_GEOM_TRIANGULATED = 1
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("rel_tol", C.c_double), ("abs_linf_tol", C.c_double),
+                ("max_outer", C.c_int32), ("cycles_per_outer", C.c_int32),
+                ("nu_pre", C.c_int32), ("nu_post", C.c_int32), ("omega", C.c_double),
+                ("mean_remove", C.c_int32), ("project_rhs", C.c_int32), ("use_graph", C.c_int32)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("status", C.c_int32), ("outer_iterations", C.c_int32), ("vcycles", C.c_int32),
+                ("rel_l2", C.c_double), ("abs_linf", C.c_double), ("loc", C.c_int32 * 3),
+                ("b_l2", C.c_double), ("rhs_shift", C.c_double), ("mean_removed", C.c_double),
+                ("ms", C.c_double), ("bytes", C.c_double), ("history", C.c_double * 64)]
+
+    def as_dict(self):
+        n = min(self.outer_iterations, 64)
+        return {"status": _STATUS.get(self.status, self.status),
+                "outer_iterations": self.outer_iterations, "vcycles": self.vcycles,
+                "rel_l2": self.rel_l2, "abs_linf": self.abs_linf, "loc": list(self.loc),
+                "b_l2": self.b_l2, "rhs_shift": self.rhs_shift,
+                "mean_removed": self.mean_removed, "ms": self.ms, "bytes": self.bytes,
+                "history": [self.history[i] for i in range(n)]}
+
+
+_lib = None
+
+
+def lib():
+    """Load libpp_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    L.pp_last_error.restype = C.c_char_p
+    L.pp_default_opts.argtypes = [C.POINTER(SolveOpts)]
+    L.pp_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(GridDesc)]
+    L.pp_destroy.argtypes = [P]
+    L.pp_set_geometry.argtypes = [P, C.POINTER(Geometry)]
+    L.pp_solve.argtypes = [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
+    L.pp_solve_device.argtypes = [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
+    L.pp_adjoint.argtypes = [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
+    L.pp_calc_residual.argtypes = [P, P, P, P, dp, C.POINTER(C.c_int32), dp]
+    L.pp_ghost_rows.argtypes = [P, P, P]
+    L.pp_mean_remove.argtypes = [P, P, dp]
+    L.pp_divergence.argtypes = [P, P, P, P, C.c_double, P, dp]
+    L.pp_tdma_batched.argtypes = [P, P, P, P, P, P, C.c_int32, C.c_int32, C.c_int32]
+    L.pp_null_vector.argtypes = [P, P]
+    L.pp_level_info.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.pp_geom_build.argtypes = [C.POINTER(C.c_void_p), C.POINTER(GridDesc), C.c_int32,
+                                C.POINTER(BodyDesc), C.c_int32, C.c_double, P]
+    L.pp_geom_counts.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.pp_geom_get.argtypes = [P] * 14
+    L.pp_geom_destroy.argtypes = [P]
+    for name in ("pp_create", "pp_destroy", "pp_set_geometry", "pp_solve", "pp_solve_device",
+                 "pp_adjoint", "pp_calc_residual", "pp_ghost_rows", "pp_mean_remove",
+                 "pp_divergence", "pp_tdma_batched", "pp_null_vector", "pp_level_info",
+                 "pp_geom_build", "pp_geom_counts", "pp_geom_get", "pp_geom_destroy"):
+        getattr(L, name).restype = C.c_int32
+    _lib = L
+    return L
+
+
+EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_set_geometry",
+            "pp_solve", "pp_solve_device", "pp_adjoint", "pp_calc_residual", "pp_ghost_rows",
+            "pp_mean_remove", "pp_divergence", "pp_tdma_batched", "pp_null_vector",
+            "pp_level_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get", "pp_geom_destroy")
+
+
+def _check(rc, allow=(PP_OK,)):
+    if rc not in allow:
+        raise PPError(rc, lib().pp_last_error().decode())
+    return rc
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def default_opts(**kw) -> SolveOpts:
+    o = SolveOpts()
+    lib().pp_default_opts(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
+
+
+@dataclass
+class GhostSet:
+    """The GCM ghost-cell geometry of AMODULES.F90:507-579 (1-based indices)."""
+    ig: np.ndarray
+    jg: np.ndarray
+    kg: np.ndarray
+    ic: np.ndarray
+    jc: np.ndarray
+    kc: np.ndarray
+    coeff: np.ndarray      # (n, 8): coeffGCMN(:, n)
+    probe: np.ndarray      # (n,)
+
+    @property
+    def n(self):
+        return int(self.ig.shape[0])
+
+
+class PressureSolver:
+    """One context of the MI355X pressure solve (pp_create ... pp_destroy)."""
+
+    def __init__(self, xn, yn, zn, device=0):
+        L = lib()
+        self.xn = np.ascontiguousarray(xn, dtype=np.float64)
+        self.yn = np.ascontiguousarray(yn, dtype=np.float64)
+        self.zn = np.ascontiguousarray(zn, dtype=np.float64)
+        self.nxc, self.nyc, self.nzc = len(self.xn) - 1, len(self.yn) - 1, len(self.zn) - 1
+        g = GridDesc()
+        g.nxc, g.nyc, g.nzc = self.nxc, self.nyc, self.nzc
+        g.x, g.y, g.z = _ptr(self.xn, C.c_double), _ptr(self.yn, C.c_double), _ptr(self.zn, C.c_double)
+        for f in range(6):
+            g.bc[f] = PP_BC_NEUMANN
+        g.device = device
+        h = C.c_void_p()
+        _check(L.pp_create(C.byref(h), C.byref(g)))
+        self.h = h
+        self.shape = (self.nzc + 3, self.nyc + 3, self.nxc + 3)
+        self.ghosts = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_geometry(self, iblank, ghosts: GhostSet, fresh=None):
+        ib = np.ascontiguousarray(iblank, dtype=np.int8)
+        assert ib.shape == self.shape
+        gs = ghosts
+        arrs = [np.ascontiguousarray(a, dtype=np.int32) for a in (gs.ig, gs.jg, gs.kg, gs.ic, gs.jc, gs.kc)]
+        coeff = np.ascontiguousarray(gs.coeff, dtype=np.float64).reshape(-1)
+        probe = np.ascontiguousarray(gs.probe, dtype=np.float64)
+        geo = Geometry()
+        geo.iblank = _ptr(ib, C.c_int8)
+        fr = None
+        if fresh is not None:
+            fr = np.ascontiguousarray(fresh, dtype=np.int8)
+            geo.fresh_cell = _ptr(fr, C.c_int8)
+        geo.n_ghost = gs.n
+        geo.i_ghost, geo.j_ghost, geo.k_ghost = (_ptr(a, C.c_int32) for a in arrs[:3])
+        geo.i_cell, geo.j_cell, geo.k_cell = (_ptr(a, C.c_int32) for a in arrs[3:])
+        geo.coeff = _ptr(coeff, C.c_double)
+        geo.probe_length = _ptr(probe, C.c_double)
+        _check(lib().pp_set_geometry(self.h, C.byref(geo)))
+        self.ghosts = gs
+        self._keep = (ib, arrs, coeff, probe, fr)
+
+    def solve(self, p, rhs, bc_src=None, **kw):
+        """solve_poisson on host arrays; p is updated in place.  Returns stats."""
+        o = default_opts(**kw)
+        st = SolveStats()
+        assert p.dtype == np.float64 and p.flags.c_contiguous and p.shape == self.shape
+        r = np.ascontiguousarray(rhs, dtype=np.float64)
+        bc = None if bc_src is None else np.ascontiguousarray(bc_src, dtype=np.float64)
+        rc = lib().pp_solve(self.h, p.ctypes.data, r.ctypes.data,
+                            None if bc is None else bc.ctypes.data, C.byref(o), C.byref(st))
+        _check(rc, (PP_OK, PP_NOT_CONVERGED))
+        return st.as_dict()
+
+    def solve_device(self, p, rhs, bc_src=None, **kw):
+        """Device-resident solve on torch float64 tensors in the reference layout."""
+        o = default_opts(**kw)
+        st = SolveStats()
+        rc = lib().pp_solve_device(self.h, p.data_ptr(), rhs.data_ptr(),
+                                   None if bc_src is None else bc_src.data_ptr(),
+                                   C.byref(o), C.byref(st))
+        _check(rc, (PP_OK, PP_NOT_CONVERGED))
+        return st.as_dict()
+
+    def adjoint(self, w, bc_grad=False, **kw):
+        o = default_opts(**kw)
+        st = SolveStats()
+        wv = np.ascontiguousarray(w, dtype=np.float64)
+        lam = np.zeros(self.shape)
+        dg = np.zeros(max(self.ghosts.n if self.ghosts else 0, 1)) if bc_grad else None
+        rc = lib().pp_adjoint(self.h, wv.ctypes.data, lam.ctypes.data,
+                              None if dg is None else dg.ctypes.data, C.byref(o), C.byref(st))
+        _check(rc, (PP_OK, PP_NOT_CONVERGED))
+        if bc_grad:
+            return lam, dg[: self.ghosts.n], st.as_dict()
+        return lam, st.as_dict()
+
+    def calc_residual(self, var, r):
+        var = np.ascontiguousarray(var, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        res = np.zeros(self.shape)
+        resm = C.c_double()
+        l2 = C.c_double()
+        loc = (C.c_int32 * 3)()
+        _check(lib().pp_calc_residual(self.h, var.ctypes.data, r.ctypes.data, res.ctypes.data,
+                                      C.byref(resm), loc, C.byref(l2)))
+        return res, resm.value, tuple(loc), l2.value
+
+    def ghost_rows(self, p, bc_src=None):
+        assert p.dtype == np.float64 and p.flags.c_contiguous
+        bc = None if bc_src is None else np.ascontiguousarray(bc_src, dtype=np.float64)
+        _check(lib().pp_ghost_rows(self.h, p.ctypes.data, None if bc is None else bc.ctypes.data))
+        return p
+
+    def mean_remove(self, p):
+        m = C.c_double()
+        _check(lib().pp_mean_remove(self.h, p.ctypes.data, C.byref(m)))
+        return m.value
+
+    def null_vector(self):
+        w = np.zeros(self.shape)
+        _check(lib().pp_null_vector(self.h, w.ctypes.data))
+        return w
+
+    def level_info(self):
+        n = C.c_int32()
+        dims = (C.c_int32 * 48)()
+        _check(lib().pp_level_info(self.h, C.byref(n), dims))
+        return [(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]) for l in range(n.value)]
+
+    def tdma_batched(self, a, b, c, r, method):
+        a, b, c, r = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, r))
+        nl, n = a.shape
+        u = np.zeros_like(a)
+        _check(lib().pp_tdma_batched(self.h, a.ctypes.data, b.ctypes.data, c.ctypes.data,
+                                     r.ctypes.data, u.ctypes.data, n, nl, method))
+        return u
+
+
+@dataclass
+class Ellipsoid:
+    """Canonical ELLIPSOID body (MAIN.F90:1140-1168): centre, semi-axes
+    (radiusx, radiusy, radiusz), rotation alpha about z in degrees and the
+    marker count per latitude ring n_theta."""
+    center: tuple
+    semi: tuple
+    alpha_deg: float = 0.0
+    n_theta: int = 64
+
+
+def _grid_desc(xn, yn, zn, device=0):
+    g = GridDesc()
+    g.nxc, g.nyc, g.nzc = len(xn) - 1, len(yn) - 1, len(zn) - 1
+    g.x, g.y, g.z = _ptr(xn, C.c_double), _ptr(yn, C.c_double), _ptr(zn, C.c_double)
+    for f in range(6):
+        g.bc[f] = PP_BC_NEUMANN
+    g.device = device
+    return g
+
+
+def build_geometry(xn, yn, zn, bodies, mode=PP_GEOM_ANALYTIC, probe_norm=2.0, iblank_prev=None):
+    """set_boundary for canonical ellipsoids through pp_geom_build.
+
+    Returns (iblank, fresh, GhostSet, extras) with iblank/fresh in the
+    reference layout and extras = dict(bi=..., ip=..., nrm=...)."""
+    L = lib()
+    xn, yn, zn = (np.ascontiguousarray(a, dtype=np.float64) for a in (xn, yn, zn))
+    g = _grid_desc(xn, yn, zn)
+    arr = (BodyDesc * max(len(bodies), 1))()
+    for q, b in enumerate(bodies):
+        arr[q].shape = PP_BODY_ELLIPSOID
+        for d in range(3):
+            arr[q].center[d] = b.center[d]
+            arr[q].semi[d] = b.semi[d]
+        arr[q].alpha_deg = b.alpha_deg
+        arr[q].n_theta = b.n_theta
+    prev = None if iblank_prev is None else np.ascontiguousarray(iblank_prev, dtype=np.int8)
+    h = C.c_void_p()
+    _check(L.pp_geom_build(C.byref(h), C.byref(g), len(bodies), arr, mode, probe_norm,
+                           None if prev is None else prev.ctypes.data))
+    try:
+        ng, nb, nf = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(L.pp_geom_counts(h, C.byref(ng), C.byref(nb), C.byref(nf)))
+        n = ng.value
+        shape = (len(zn) + 2, len(yn) + 2, len(xn) + 2)
+        ib = np.zeros(shape, np.int8)
+        fr = np.zeros(shape, np.int8)
+        ints = [np.zeros(max(n, 1), np.int32) for _ in range(6)]
+        coeff = np.zeros((max(n, 1), 8))
+        probe = np.zeros(max(n, 1))
+        bi, ip, nrm = (np.zeros((max(n, 1), 3)) for _ in range(3))
+        _check(L.pp_geom_get(h, ib.ctypes.data, fr.ctypes.data, *[a.ctypes.data for a in ints],
+                             coeff.ctypes.data, probe.ctypes.data, bi.ctypes.data, ip.ctypes.data,
+                             nrm.ctypes.data))
+    finally:
+        L.pp_geom_destroy(h)
+    gs = GhostSet(*[a[:n] for a in ints], coeff[:n], probe[:n])
+    return ib, fr, gs, {"bi": bi[:n], "ip": ip[:n], "nrm": nrm[:n]}
+
+
+def tanh_nodes(n, length, beta):
+    """Nodes clustered at the domain centre: x(xi) = L/2 [1 + artanh((2xi-1) tanh beta)/beta],
+    xi = j/n (beta = 0: uniform)."""
+    xi = np.arange(n + 1) / n
+    if beta == 0:
+        return xi * length
+    return 0.5 * length * (1.0 + np.arctanh((2.0 * xi - 1.0) * np.tanh(beta)) / beta)

@@ -1,0 +1,210 @@
+/*
+ * pp_hip.h -- C-ABI of the MI355X pressure-Poisson solve.
+ *
+ * Drop-in replacement for the pressure-Poisson path of the reference
+ * ghost-cell immersed-boundary solver (VICAR3D):
+ *
+ *   SUBROUTINE solve_poisson()            IBM/CORE_POISSON.F90:493-896
+ *     MG_Solver(pPrime,nlu0,MG_itsolv,MG_Residual,compTime)
+ *                                          IBM/CORE_MG.F90:281-334
+ *     GCM_correct_pressure                 IBM/CORE_POISSON.F90:1794-1823
+ *     calc_residual                        IBM/CORE_POISSON.F90:1111-1222
+ *
+ * Plain pointers and sizes only.  3-D fields are passed in the reference
+ * layout: (0:nx+1, 0:ny+1, 0:nz+1) with i fastest and nx = nxc+1 node
+ * counts, i.e. (nxc+3)*(nyc+3)*(nzc+3) entries, linear index
+ * i + (nxc+3)*(j + (nyc+3)*k) with interior cells i = 1..nxc.
+ * Flag arrays are int8_t (INTEGER(1)); values are double (REAL(8)).
+ *
+ * Every entry point returns an int32_t status (PP_OK = 0) and never aborts;
+ * pp_last_error() returns the text of the calling thread's last error.
+ * One host thread per context; calls are synchronous on return.
+ */
+#ifndef PP_HIP_H
+#define PP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PP_OK 0
+#define PP_NOT_CONVERGED 1   /* iteration limit reached (reference prints and continues) */
+#define PP_BAD_ARG 2
+#define PP_BAD_GEOMETRY 3    /* e.g. an image point outside the grid */
+#define PP_HIP_ERROR 4
+#define PP_NONFINITE 5       /* NaN/Inf in the RHS or the residual */
+#define PP_NO_GEOMETRY 6     /* solve called before pp_set_geometry */
+
+/* outer pressure boundary condition per face (pbcx1..pbcz2,
+ * AMODULES.F90:187-333); only PP_BC_NEUMANN is implemented */
+#define PP_BC_NEUMANN 2
+
+/* precision of the multigrid cycles */
+#define PP_PREC_FP64 0       /* FP64 V-cycles (the GPU FP64 multigrid) */
+#define PP_PREC_MIXED 1      /* FP32 V-cycles inside FP64 iterative refinement */
+
+typedef struct pp_ctx pp_ctx;
+
+/* Grid: make_grid (CORE_GRID.F90:29-151) inputs.  x/y/z hold the node
+ * coordinates x(1..nxc+1) (uniform or read from xgrid.dat in the reference);
+ * the library derives x(0), x(nx+1), centres, widths and inverses exactly as
+ * make_grid does. */
+typedef struct pp_grid_desc {
+  int32_t nxc, nyc, nzc;
+  const double *x, *y, *z;
+  int32_t bc[6];             /* pbcx1, pbcx2, pbcy1, pbcy2, pbcz1, pbcz2 */
+  int32_t device;            /* HIP device ordinal */
+} pp_grid_desc;
+
+/* Geometry: the outputs of set_boundary (BOUNDARY_SET.F90:8-51) that the
+ * pressure path reads (AMODULES.F90:378-436, 507-579).  Host pointers in the
+ * reference layout / 1-based reference indices; not retained. */
+typedef struct pp_geometry {
+  const int8_t *iblank;          /* required */
+  const int8_t *fresh_cell;      /* may be NULL */
+  int32_t n_ghost;               /* nGhost */
+  const int32_t *i_ghost, *j_ghost, *k_ghost;   /* iGhost/jGhost/kGhost */
+  const int32_t *i_cell, *j_cell, *k_cell;      /* iCellIndex/... (IP cell) */
+  const double *coeff;           /* coeffGCMN(8, nGhost), column-major */
+  const double *probe_length;    /* probeLength(nGhost) */
+} pp_geometry;
+
+/* Solve options (input.dat knobs, MAIN.F90:476-489, plus the build's). */
+typedef struct pp_solve_opts {
+  int32_t precision;         /* PP_PREC_FP64 or PP_PREC_MIXED */
+  double rel_tol;            /* stop when ||b-Ap||_2/||b||_2 <= rel_tol (if > 0) */
+  double abs_linf_tol;       /* restol_Poisson: stop when max|b-Ap| <= tol (if > 0) */
+  int32_t max_outer;         /* iterMax_Poisson: outer iterations */
+  int32_t cycles_per_outer;  /* V-cycles per outer iteration */
+  int32_t nu_pre, nu_post;   /* smoothing steps before/after coarse correction */
+  double omega;              /* line-SOR weight */
+  int32_t mean_remove;       /* subtract the fluid volume mean on return (0/1) */
+  int32_t project_rhs;       /* compatibility projection of the RHS (0/1) */
+  int32_t use_graph;         /* replay the outer iteration as a hipGraph (0/1) */
+} pp_solve_opts;
+
+typedef struct pp_solve_stats {
+  int32_t status;
+  int32_t outer_iterations;
+  int32_t vcycles;
+  double rel_l2;             /* final ||b-Ap||_2/||b||_2 over fluid cells */
+  double abs_linf;           /* final max|b-Ap| (calc_residual's maxres) */
+  int32_t loc[3];            /* argmax, reference indices */
+  double b_l2;               /* ||b||_2 after projection */
+  double rhs_shift;          /* constant removed from the fluid RHS */
+  double mean_removed;       /* fluid volume mean removed from p */
+  double ms;                 /* device time of the solve (HIP events) */
+  double bytes;              /* algorithmic bytes of the launched kernels */
+  double history[64];        /* rel_l2 after each outer iteration */
+} pp_solve_stats;
+
+/* Library defaults for pp_solve_opts. */
+void pp_default_opts(pp_solve_opts *o);
+const char *pp_last_error(void);
+
+/* Create a context on one GPU for a grid (replaces MG_initial,
+ * CORE_MG.F90:31-278, and the grid half of init_simulation). */
+int32_t pp_create(pp_ctx **ctx, const pp_grid_desc *g);
+int32_t pp_destroy(pp_ctx *ctx);
+
+/* Upload a geometry (call after every set_boundary: MAIN.F90:1549,
+ * BOUNDARY_MOVE.F90:210).  Derives ghostCellMark and the outer-face flags
+ * (BOUNDARY_SET.F90:54-92, GCM_SET_BOUNDARY.F90:49-118) from iblank and
+ * checks them against the ghost list; builds the level hierarchy, ghost
+ * tables and the left null vector used by the compatibility projection. */
+int32_t pp_set_geometry(pp_ctx *ctx, const pp_geometry *geo);
+
+/* solve_poisson (CORE_POISSON.F90:493-676): p_inout = pPrime (initial guess
+ * in, solution out), rhs = nlu0, bc_src = Neumann datum per ghost
+ * (uNormal, GCM_GHOSTCELL_VEL_UPDATE_UNSTRUC.F90:331-343) or NULL for 0.
+ * Host pointers, reference layout.  Returns PP_OK or PP_NOT_CONVERGED (the
+ * solution is still written, as the reference continues). */
+int32_t pp_solve(pp_ctx *ctx, double *p_inout, const double *rhs,
+                 const double *bc_src, const pp_solve_opts *o,
+                 pp_solve_stats *st);
+/* Same with device pointers (reference layout, already in HBM). */
+int32_t pp_solve_device(pp_ctx *ctx, double *d_p, const double *d_rhs,
+                        const double *d_bc_src, const pp_solve_opts *o,
+                        pp_solve_stats *st);
+
+/* Adjoint of the zero-mean solve.  For J = sum(w * p) with p = pp_solve's
+ * output (projected RHS, zero fluid mean), returns dJ/d(rhs) in lambda
+ * (reference layout) and dJ/d(bc_src) in dJ_dg (n_ghost, may be NULL).
+ * No counterpart in the reference (it has no adjoint). */
+int32_t pp_adjoint(pp_ctx *ctx, const double *w, double *lambda,
+                   double *dJ_dg, const pp_solve_opts *o, pp_solve_stats *st);
+
+/* ---- single routines of the reference path (parity checks, callers that
+ *      keep their own loop) ------------------------------------------- */
+
+/* calc_residual (CORE_POISSON.F90:1111-1222) with nlw0 = 0: res (may be
+ * NULL) = masked r - A var; *resm = max|res|, loc = argmax, *l2 = ||res||_2. */
+int32_t pp_calc_residual(pp_ctx *ctx, const double *var, const double *r,
+                         double *res, double *resm, int32_t *loc, double *l2);
+/* GCM_correct_pressure's ghost part: set_outer_ghost_pres (Neumann,
+ * CORE_SET_OUTER_GHOST.F90:66-213) + GCM_p_set_bc_internal solved exactly
+ * (GCM_GHOSTCELL_VEL_UPDATE_UNSTRUC.F90:250-384); bc_src may be NULL. */
+int32_t pp_ghost_rows(pp_ctx *ctx, double *p, const double *bc_src);
+/* subtracting_average_pressure (CORE_POISSON.F90:419-490). */
+int32_t pp_mean_remove(pp_ctx *ctx, double *p, double *mean);
+/* rhs_poisson, non-cut-cell branch (CORE_POISSON.F90:281-288):
+ * nlu0 = dtinv*div(face velocity)*(1-iblank); *sum = sum(nlu0*dV). */
+int32_t pp_divergence(pp_ctx *ctx, const double *face_u, const double *face_v,
+                      const double *face_w, double dtinv, double *nlu0,
+                      double *sum);
+/* tdma (CORE_TDMA.F90:10-34) on a batch of independent systems, through
+ * the device line solvers: n_lines systems of length n stored line after
+ * line (a: sub-, b: main, c: super-diagonal, r: rhs, u: solution).
+ * method 0 = wavefront PCR (lines along the contiguous index),
+ * method 1 = partitioned Thomas (lines across it). FP64. */
+int32_t pp_tdma_batched(pp_ctx *ctx, const double *a, const double *b,
+                        const double *c, const double *r, double *u,
+                        int32_t n, int32_t n_lines, int32_t method);
+/* Left null vector of the ghost-cell operator (A^T w = 0), normalised so
+ * that sum over fluid cells of w equals the fluid volume; reference layout,
+ * fluid and ghost entries.  No counterpart in the reference. */
+int32_t pp_null_vector(pp_ctx *ctx, double *w);
+/* Level hierarchy: *n_levels and (cx,cy,cz) per level into dims[3*l]. */
+int32_t pp_level_info(pp_ctx *ctx, int32_t *n_levels, int32_t *dims);
+
+/* ---- geometry: body description -> pp_geometry inputs ---------------
+ * Replaces set_boundary for canonical solid bodies (BOUNDARY_SET.F90:8-51):
+ * iblank (set_iblank_canonical_body_fast, BOUNDARY_IBLANK_FAST.F90:2-501,
+ * or the analytic cell-centre-inside rule), find_isolatedIblank(1)
+ * (BOUNDARY_SET.F90:440-499), fill_cell (SSM_SET_BC.F90:1425-1511), ghost
+ * marking, body intercepts, image points and Neumann coefficients
+ * (GCM_set_internal_boundary, GCM_SET_BOUNDARY.F90:2-494).  Host only. */
+#define PP_BODY_ELLIPSOID 3       /* canonical_body_type ELLIPSOID */
+#define PP_GEOM_ANALYTIC 0        /* exact ellipsoid: centre-inside iblank, closest-point intercepts */
+#define PP_GEOM_TRIANGULATED 1    /* the reference's triangulated surface and heuristics */
+
+typedef struct pp_body_desc {
+  int32_t shape;             /* PP_BODY_ELLIPSOID */
+  double center[3];          /* xcent, ycent, zcent */
+  double semi[3];            /* radiusx, radiusy, radiusz */
+  double alpha_deg;          /* rotation about z (BOUNDARY_MARKER_INIT.F90:127-142) */
+  int32_t n_theta;           /* markers per latitude ring (MAIN.F90:1140-1168) */
+} pp_body_desc;
+
+typedef struct pp_geom pp_geom;
+/* iblank_prev (reference layout, may be NULL) marks fresh cells. */
+int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t n_body,
+                      const pp_body_desc *bodies, int32_t mode,
+                      double probe_length_normalized, const int8_t *iblank_prev);
+int32_t pp_geom_counts(const pp_geom *geo, int32_t *n_ghost, int32_t *n_blank,
+                       int32_t *n_fresh);
+/* Any output pointer may be NULL.  coeff is coeffGCMN(8,n) column-major;
+ * bi/ip/nrm are 3 per ghost (body intercept, image point, inward normal). */
+int32_t pp_geom_get(const pp_geom *geo, int8_t *iblank, int8_t *fresh,
+                    int32_t *ig, int32_t *jg, int32_t *kg, int32_t *ic,
+                    int32_t *jc, int32_t *kc, double *coeff, double *probe,
+                    double *bi, double *ip, double *nrm);
+int32_t pp_geom_destroy(pp_geom *geo);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

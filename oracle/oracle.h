@@ -1,0 +1,143 @@
+/*
+ * oracle.h -- CPU restatement of the reference pressure-Poisson path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it, as the checker and as the timed CPU baseline.
+ *
+ * Every routine restates one routine of the reference Fortran (cited as
+ * file:line under IBM/) in plain C, in the reference's own index space:
+ *
+ *   - 3-D fields are (0:nx+1, 0:ny+1, 0:nz+1) with i fastest, where
+ *     nx = nxc+1 is the reference's node count (MAIN.F90:519-521);
+ *     linear index = i + (nx+2)*(j + (ny+2)*k).
+ *   - Flags (iblank, ghostCellMark, ium..kup, fresh_cell) are int8_t
+ *     (INTEGER(1), AMODULES.F90:388-399).
+ *   - All arithmetic is double (CGREAL=8, AMODULES.F90:6).
+ *
+ * Parity pins for each part are listed in the header of the .c file that
+ * implements it.  Functions that no golden vector, closed form, invariant or
+ * brute-force check pins are marked "parity unpinned" there.
+ */
+#ifndef FSI_ORACLE_H
+#define FSI_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_ctx or_ctx;
+
+/* ---- grid (CORE_GRID.F90:29-151) ------------------------------------ */
+/* xn/yn/zn: node coordinates x(1..nx) (nx = nxc+1 values each). */
+or_ctx *or_create(int nxc, int nyc, int nzc, const double *xn,
+                  const double *yn, const double *zn);
+void or_destroy(or_ctx *c);
+/* dir 0/1/2; out holds 6 arrays of (n+2): x, xc, dx, dxinv, dxc, dxcinv */
+void or_get_metrics(const or_ctx *c, int dir, double *out);
+long or_field_size(const or_ctx *c); /* (nx+2)(ny+2)(nz+2) */
+
+/* ---- flags and ghost marking ---------------------------------------- */
+/* iblank in reference layout.  Sets outer-face flags (BOUNDARY_SET.F90:54-92),
+ * ghostCellMark and the i-fastest ghost list (GCM_SET_BOUNDARY.F90:49-118). */
+int or_set_iblank(or_ctx *c, const int8_t *iblank);
+int or_nghost(const or_ctx *c);
+void or_get_flags(const or_ctx *c, int8_t *ghost, int8_t *ium, int8_t *iup,
+                  int8_t *jum, int8_t *jup, int8_t *kum, int8_t *kup);
+void or_get_ghost_list(const or_ctx *c, int32_t *ig, int32_t *jg, int32_t *kg);
+/* IP cell indices (reference indexing), coeffGCMN(8,n) column-major and
+ * probeLength(n) -- the output of the geometry pipeline. */
+void or_set_ghost_stencils(or_ctx *c, const int32_t *ic, const int32_t *jc,
+                           const int32_t *kc, const double *coeff,
+                           const double *probe);
+void or_set_fresh(or_ctx *c, const int8_t *fresh);
+
+/* ---- single-level operators ----------------------------------------- */
+/* calc_residual (CORE_POISSON.F90:1111-1222) with nlw0 == 0.  Writes the
+ * masked residual field into res (may be NULL), returns max|res| (resm) and
+ * its argmax (loc, reference indices) and the L2 norm. */
+double or_calc_residual(const or_ctx *c, const double *var, const double *r,
+                        double *res, int32_t *loc, double *l2);
+/* subtracting_average_pressure (CORE_POISSON.F90:419-490) */
+double or_mean_remove(const or_ctx *c, double *p);
+/* set_outer_ghost_pres for all-Neumann outer faces
+ * (CORE_SET_OUTER_GHOST.F90:66-213) */
+void or_outer_ghost(const or_ctx *c, double *p);
+/* GCM_p_set_bc_internal (GCM_GHOSTCELL_VEL_UPDATE_UNSTRUC.F90:250-384).
+ * g (uNormal per ghost) may be NULL (viscous flow: 0).  Returns sweeps. */
+int or_ghost_rows(const or_ctx *c, double *p, const double *g, int itermax,
+                  double restol);
+/* rhs_poisson non-cut-cell branch (CORE_POISSON.F90:281-288) */
+double or_divergence(const or_ctx *c, const double *fu, const double *fv,
+                     const double *fw, double dtinv, double *nlu0);
+/* tdma (CORE_TDMA.F90:10-34); arrays indexed 0..n2 as in the reference */
+void or_tdma(const double *a, const double *b, const double *c,
+             const double *r, double *u, int n1, int n2);
+
+/* ---- reference multigrid (CORE_MG.F90) ------------------------------ */
+typedef struct or_mg_opts {
+  int iter_max;        /* iterMax_Poisson */
+  double restol;       /* restol_Poisson: absolute L-inf stop (reference) */
+  double rel_tol;      /* >0: also stop when ||r||2/||b||2 <= rel_tol */
+  int iter_finest;     /* iterFinest */
+  int iter_coarsest;   /* iterCoarsest */
+  double omega;        /* SOR weight */
+  int itermax_ad;      /* iterMax_ad for ghost rows */
+  double restol_ad;    /* restol_ad for ghost rows */
+  int mean_remove;     /* call subtracting_average_pressure at the start */
+} or_mg_opts;
+
+typedef struct or_mg_stats {
+  int iterations;
+  double linf;         /* final abs L-inf residual */
+  double l2;           /* final L2 residual */
+  double b_l2;         /* ||rhs||2 over fluid cells */
+  int32_t loc[3];
+  int levels[3];       /* mgLevels_X/Y/Z */
+} or_mg_stats;
+
+/* solve_poisson MG branch (CORE_POISSON.F90:493-676); p in/out, rhs = nlu0 */
+int or_mg_solve(or_ctx *c, double *p, const double *rhs,
+                const or_mg_opts *o, or_mg_stats *s);
+/* One MG_Solver call (CORE_MG.F90:281-334) followed by GCM_correct_pressure */
+void or_mg_cycle(or_ctx *c, double *p, const double *rhs, const or_mg_opts *o);
+
+/* ---- FP64 operator applies for checks ------------------------------- */
+/* y = A p over the fluid rows (the calc_residual operator without rhs),
+ * ghost rows y_g = p_g - sum_r c_r p_r; other cells 0. */
+void or_apply_A(const or_ctx *c, const double *p, double *y);
+/* y = A^T x for the same operator (fluid + ghost unknowns). */
+void or_apply_AT(const or_ctx *c, const double *x, double *y);
+
+/* ---- geometry (oracle/or_geom.c) ------------------------------------ */
+/* Canonical ellipsoid surface triangulation (BOUNDARY_MARKER_INIT.F90:115-205)
+ * and triangle normals/centroids (BOUNDARY_SET_ARCLENGTH_NORM.F90:69-167),
+ * normals oriented inward.  Returns the marker / element counts; arrays are
+ * owned by the oracle and read back with or_geom_get_surface. */
+int or_geom_ellipsoid(or_ctx *c, int n_theta, double xc0, double yc0,
+                      double zc0, double ra, double rb, double rc,
+                      double alpha_deg);
+void or_geom_clear_bodies(or_ctx *c);
+int or_geom_nbodies(const or_ctx *c);
+void or_geom_surface_sizes(const or_ctx *c, int body, int *nmarker,
+                           int *nelem);
+void or_geom_get_surface(const or_ctx *c, int body, double *xyz, int32_t *tri,
+                         double *cent, double *norm);
+/* set_internal_boundary for canonical solid bodies (BOUNDARY_SET.F90:267-410):
+ * iblank by set_iblank_canonical_body_fast, find_isolatedIblank(1), fill_cell,
+ * ghost marking, body intercepts, image points and Neumann coefficients.
+ * iblank_prev may be NULL (first step: no fresh cells).  Returns 0 or an
+ * error code (1: image point search failed, 2: intercept too long). */
+int or_geom_set_boundary(or_ctx *c, const int8_t *iblank_prev,
+                         double probe_length_normalized);
+void or_geom_get_iblank(const or_ctx *c, int8_t *iblank, int8_t *fresh);
+void or_geom_get_ghost_geometry(const or_ctx *c, int32_t *ic, int32_t *jc,
+                                int32_t *kc, double *coeff, double *probe,
+                                double *bi, double *ip, int32_t *elem);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,174 @@
+"""ctypes wrapper of the oracle (oracle/_build/liboracle.so): the CPU
+restatement of the reference pressure path used as the checker."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+
+class MGOpts(C.Structure):
+    _fields_ = [("iter_max", C.c_int), ("restol", C.c_double), ("rel_tol", C.c_double),
+                ("iter_finest", C.c_int), ("iter_coarsest", C.c_int), ("omega", C.c_double),
+                ("itermax_ad", C.c_int), ("restol_ad", C.c_double), ("mean_remove", C.c_int)]
+
+
+class MGStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("linf", C.c_double), ("l2", C.c_double),
+                ("b_l2", C.c_double), ("loc", C.c_int32 * 3), ("levels", C.c_int * 3)]
+
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} missing: run make -C oracle")
+        L = C.CDLL(ORACLE_SO)
+        P = C.c_void_p
+        L.or_create.restype = P
+        L.or_create.argtypes = [C.c_int] * 3 + [P] * 3
+        L.or_destroy.argtypes = [P]
+        L.or_field_size.restype = C.c_long
+        L.or_field_size.argtypes = [P]
+        L.or_get_metrics.argtypes = [P, C.c_int, P]
+        L.or_set_iblank.argtypes = [P, P]
+        L.or_nghost.argtypes = [P]
+        L.or_get_flags.argtypes = [P] * 8
+        L.or_get_ghost_list.argtypes = [P] * 4
+        L.or_set_ghost_stencils.argtypes = [P] * 6
+        L.or_set_fresh.argtypes = [P, P]
+        L.or_calc_residual.restype = C.c_double
+        L.or_calc_residual.argtypes = [P, P, P, P, P, P]
+        L.or_mean_remove.restype = C.c_double
+        L.or_mean_remove.argtypes = [P, P]
+        L.or_outer_ghost.argtypes = [P, P]
+        L.or_ghost_rows.argtypes = [P, P, P, C.c_int, C.c_double]
+        L.or_divergence.restype = C.c_double
+        L.or_divergence.argtypes = [P, P, P, P, C.c_double, P]
+        L.or_tdma.argtypes = [P] * 5 + [C.c_int, C.c_int]
+        L.or_mg_solve.argtypes = [P, P, P, C.POINTER(MGOpts), C.POINTER(MGStats)]
+        L.or_mg_cycle.argtypes = [P, P, P, C.POINTER(MGOpts)]
+        L.or_apply_A.argtypes = [P, P, P]
+        L.or_apply_AT.argtypes = [P, P, P]
+        _L = L
+    return _L
+
+
+def mg_opts(iter_max=200, restol=0.0, rel_tol=1e-12, iter_finest=2, iter_coarsest=4, omega=1.0,
+            itermax_ad=500, restol_ad=1e-15, mean_remove=1):
+    return MGOpts(iter_max, restol, rel_tol, iter_finest, iter_coarsest, omega, itermax_ad,
+                  restol_ad, mean_remove)
+
+
+class Oracle:
+    def __init__(self, xn, yn, zn):
+        self.xn, self.yn, self.zn = (np.ascontiguousarray(a, dtype=np.float64) for a in (xn, yn, zn))
+        self.nxc, self.nyc, self.nzc = len(xn) - 1, len(yn) - 1, len(zn) - 1
+        self.h = lib().or_create(self.nxc, self.nyc, self.nzc, self.xn.ctypes.data,
+                                 self.yn.ctypes.data, self.zn.ctypes.data)
+        assert self.h
+        self.shape = (self.nzc + 3, self.nyc + 3, self.nxc + 3)
+
+    def __del__(self):
+        try:
+            lib().or_destroy(self.h)
+        except Exception:
+            pass
+
+    def metrics(self, d):
+        n = (self.nxc, self.nyc, self.nzc)[d] + 1
+        out = np.zeros(6 * (n + 2))
+        lib().or_get_metrics(self.h, d, out.ctypes.data)
+        return out.reshape(6, n + 2)  # x, xc, dx, dxinv, dxc, dxcinv
+
+    def set_iblank(self, ib):
+        ib = np.ascontiguousarray(ib, dtype=np.int8)
+        return lib().or_set_iblank(self.h, ib.ctypes.data)
+
+    def flags(self):
+        arrs = [np.zeros(self.shape, np.int8) for _ in range(7)]
+        lib().or_get_flags(self.h, *[a.ctypes.data for a in arrs])
+        return dict(zip(("ghost", "ium", "iup", "jum", "jup", "kum", "kup"), arrs))
+
+    def ghost_list(self):
+        n = lib().or_nghost(self.h)
+        a = [np.zeros(max(n, 1), np.int32) for _ in range(3)]
+        lib().or_get_ghost_list(self.h, *[x.ctypes.data for x in a])
+        return [x[:n] for x in a]
+
+    def set_stencils(self, gs):
+        arrs = [np.ascontiguousarray(a, dtype=np.int32) for a in (gs.ic, gs.jc, gs.kc)]
+        co = np.ascontiguousarray(gs.coeff, dtype=np.float64)
+        pr = np.ascontiguousarray(gs.probe, dtype=np.float64)
+        lib().or_set_ghost_stencils(self.h, *[a.ctypes.data for a in arrs], co.ctypes.data, pr.ctypes.data)
+        self._keep = (arrs, co, pr)
+
+    def set_fresh(self, fr):
+        fr = np.ascontiguousarray(fr, dtype=np.int8)
+        lib().or_set_fresh(self.h, fr.ctypes.data)
+
+    def calc_residual(self, var, r):
+        var = np.ascontiguousarray(var, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        res = np.zeros(self.shape)
+        loc = np.zeros(3, np.int32)
+        l2 = C.c_double()
+        m = lib().or_calc_residual(self.h, var.ctypes.data, r.ctypes.data, res.ctypes.data,
+                                   loc.ctypes.data, C.addressof(l2))
+        return res, m, tuple(int(v) for v in loc), l2.value
+
+    def mean_remove(self, p):
+        return lib().or_mean_remove(self.h, p.ctypes.data)
+
+    def outer_ghost(self, p):
+        lib().or_outer_ghost(self.h, p.ctypes.data)
+
+    def ghost_rows(self, p, g=None, itermax=1000, restol=1e-15):
+        gg = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
+        return lib().or_ghost_rows(self.h, p.ctypes.data, None if gg is None else gg.ctypes.data,
+                                   itermax, restol)
+
+    def divergence(self, fu, fv, fw, dtinv=1.0):
+        out = np.zeros(self.shape)
+        s = lib().or_divergence(self.h, fu.ctypes.data, fv.ctypes.data, fw.ctypes.data, dtinv, out.ctypes.data)
+        return out, s
+
+    def apply_A(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        y = np.zeros(self.shape)
+        lib().or_apply_A(self.h, p.ctypes.data, y.ctypes.data)
+        return y
+
+    def apply_AT(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.shape)
+        lib().or_apply_AT(self.h, x.ctypes.data, y.ctypes.data)
+        return y
+
+    def mg_solve(self, p, rhs, **kw):
+        o = mg_opts(**kw)
+        s = MGStats()
+        rc = lib().or_mg_solve(self.h, p.ctypes.data, np.ascontiguousarray(rhs).ctypes.data,
+                               C.byref(o), C.byref(s))
+        return rc, {"iterations": s.iterations, "linf": s.linf, "l2": s.l2, "b_l2": s.b_l2,
+                    "loc": tuple(s.loc), "levels": tuple(s.levels)}
+
+    def mg_cycle(self, p, rhs, **kw):
+        o = mg_opts(**kw)
+        lib().or_mg_cycle(self.h, p.ctypes.data, np.ascontiguousarray(rhs).ctypes.data, C.byref(o))
+
+
+def tdma(a, b, c, r):
+    """Run the oracle's tdma on one system given as 0-based arrays of length n."""
+    n = len(b)
+    A = np.zeros(n + 1); B = np.ones(n + 1); Cc = np.zeros(n + 1); R = np.zeros(n + 1); U = np.zeros(n + 1)
+    A[1:] = a; B[1:] = b; Cc[1:] = c; R[1:] = r
+    lib().or_tdma(A.ctypes.data, B.ctypes.data, Cc.ctypes.data, R.ctypes.data, U.ctypes.data, 1, n)
+    return U[1:]
