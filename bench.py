@@ -1,0 +1,208 @@
+"""Benchmark: pressure-Poisson multigrid solves per second on one MI355X.
+
+Workload (256x192x192 stretched grid, the headline configuration):
+  domain [0,4]x[0,3]x[0,3]; x uniform, y and z tanh-clustered at the centre
+  (beta = 2); one prolate ellipsoid, semi-axes (0.5,0.25,0.25), centre
+  (1.5,1.5,1.5), rotated 20 deg about z; RHS = discrete divergence
+  (rhs_poisson, dt = 1) of seeded face velocities N(0,1) times
+  sin(pi x/4) sin(pi y/3) sin(pi z/3).  One step = one solve from p = 0 to
+  ||b - A p||_2 / ||b||_2 <= 1e-8 (after the compatibility projection).
+
+value = FP32 V-cycles inside FP64 iterative refinement, solves/s, inputs
+resident in HBM.  The same line also reports the FP64 multigrid's solve
+rate, the refinement solve's error against the FP64 solve, the roofline
+of the dominant kernel (HIP events around its launches) and the
+reference multigrid restated in C (the oracle) timed on one host core.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from flow_structure_interaction_fem_ibm_amd import pp  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MG Poisson solve/s @1e-8 resid (256x192x192): FP32+refine vs FP64, HBM GB/s"
+
+
+def workload(nx=256, ny=192, nz=192, seed=2):
+    xn = np.linspace(0.0, 4.0, nx + 1)
+    yn = pp.tanh_nodes(ny, 3.0, 2.0)
+    zn = pp.tanh_nodes(nz, 3.0, 2.0)
+    body = pp.Ellipsoid((1.5, 1.5, 1.5), (0.5, 0.25, 0.25), 20.0, 96)
+    ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, [body], mode=pp.PP_GEOM_ANALYTIC)
+    rhs = divergence_rhs(xn, yn, zn, ib, seed)
+    return xn, yn, zn, ib, gs, rhs
+
+
+def divergence_rhs(xn, yn, zn, ib, seed):
+    """rhs_poisson (CORE_POISSON.F90:281-288) with dt = 1 of face velocities
+    N(0,1) * sin(pi x/4) sin(pi y/3) sin(pi z/3) sampled at face centres."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nx, ny, nz = len(xn) - 1, len(yn) - 1, len(zn) - 1
+    xc, yc, zc = (0.5 * (a[1:] + a[:-1]) for a in (xn, yn, zn))
+    env = lambda x, y, z: np.sin(np.pi * x / 4.0) * np.sin(np.pi * y / 3.0) * np.sin(np.pi * z / 3.0)
+    # face_u at x nodes (nx+1), y/z centres, etc.
+    u = rng.standard_normal((nz, ny, nx + 1)) * env(xn[None, None, :], yc[None, :, None], zc[:, None, None])
+    v = rng.standard_normal((nz, ny + 1, nx)) * env(xc[None, None, :], yn[None, :, None], zc[:, None, None])
+    w = rng.standard_normal((nz + 1, ny, nx)) * env(xc[None, None, :], yc[None, :, None], zn[:, None, None])
+    div = ((u[:, :, 1:] - u[:, :, :-1]) / np.diff(xn)[None, None, :]
+           + (v[:, 1:, :] - v[:, :-1, :]) / np.diff(yn)[None, :, None]
+           + (w[1:, :, :] - w[:-1, :, :]) / np.diff(zn)[:, None, None])
+    rhs = np.zeros((nz + 3, ny + 3, nx + 3))
+    rhs[1:-2, 1:-2, 1:-2] = div
+    return rhs * (1 - ib)
+
+
+def cpu_baseline(xn, yn, zn, ib, gs, rhs_projected, n_iter_sample=2):
+    """The oracle's restatement of the reference MG (CORE_MG.F90 MG_Solver +
+    GCM_correct_pressure) on the same grid, geometry and projected RHS: time
+    n_iter_sample outer iterations on one core; the iterations to 1e-8 are
+    the first iteration's reduction followed by the later iterations'
+    per-iteration factor."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle  # noqa: E402
+    o = Oracle(xn, yn, zn)
+    o.set_iblank(ib)
+    o.set_stencils(gs)
+    p = np.zeros(ib.shape)
+    hist = [o.calc_residual(p, rhs_projected)[3]]
+    t0 = time.perf_counter()
+    for _ in range(n_iter_sample):
+        o.mg_cycle(p, rhs_projected)
+        hist.append(o.calc_residual(p, rhs_projected)[3])
+    dt = (time.perf_counter() - t0) / n_iter_sample
+    first = hist[1] / hist[0]
+    factor = (hist[-1] / hist[1]) ** (1.0 / (n_iter_sample - 1))
+    n_needed = None
+    if 0 < factor < 1:
+        n_needed = 1 + max(0, int(np.ceil(np.log(1e-8 / first) / np.log(factor))))
+    solve_s = dt * n_needed if n_needed else None
+    return {"value": (1.0 / solve_s) if solve_s else None, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": (f"{n_iter_sample} outer iteration(s) of the restated reference MG_Solver "
+                       f"(x,y,z semi-coarsened V-cycles, line SOR) on the full 256x192x192 problem: "
+                       f"{dt:.2f} s/iteration, first-iteration reduction {first:.3g}, then "
+                       f"{factor:.3f}/iteration, "
+                       f"{n_needed} iterations extrapolated to 1e-8"),
+            "s_per_iteration": dt, "factor_per_iteration": factor, "iterations_to_1e-8": n_needed}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--nu", type=int, default=2)
+    ap.add_argument("--cycles", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the solve is a single-GPU "
+              f"kernel path; multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(2)
+    t_setup = time.perf_counter()
+    xn, yn, zn, ib, gs, rhs = workload()
+    solver = pp.PressureSolver(xn, yn, zn)
+    solver.set_geometry(ib, gs)
+    t_setup = time.perf_counter() - t_setup
+    dev = torch.device("cuda:0")
+    d_rhs = torch.from_numpy(rhs).to(dev)
+    d_p = torch.zeros(solver.shape, dtype=torch.float64, device=dev)
+    common = dict(rel_tol=args.tol, max_outer=200, nu_pre=args.nu, nu_post=args.nu,
+                  cycles_per_outer=args.cycles)
+
+    def run(prec, steps, warm, profile=False):
+        for _ in range(warm):
+            d_p.zero_()
+            solver.solve_device(d_p, d_rhs, precision=prec, **common)
+        torch.cuda.synchronize()
+        stats = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            d_p.zero_()
+            stats.append(solver.solve_device(d_p, d_rhs, precision=prec, profile=int(profile), **common))
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, stats
+
+    ms_mixed, st_mixed = run(pp.PP_PREC_MIXED, args.steps, args.warmup)
+    p_mixed = d_p.cpu().numpy()
+    ms_fp64, st_fp64 = run(pp.PP_PREC_FP64, max(2, args.steps // 2), 1)
+    p_fp64 = d_p.cpu().numpy()
+    # one profiled solve of each precision: per-kernel-class HIP events
+    _, prof_mixed = run(pp.PP_PREC_MIXED, 1, 0, profile=True)
+    _, prof_fp64 = run(pp.PP_PREC_FP64, 1, 0, profile=True)
+    for s in st_mixed + st_fp64:
+        if s["status"] != "PP_OK":
+            print(json.dumps({"error": "solve did not converge", "stats": s}), file=sys.stderr)
+            sys.exit(1)
+    fl = np.zeros(ib.shape, bool)
+    fl[1:-2, 1:-2, 1:-2] = cases_fluid = (ib[1:-2, 1:-2, 1:-2] == 0)
+    err = float(np.linalg.norm((p_mixed - p_fp64)[fl]) / np.linalg.norm(p_fp64[fl]))
+    km = prof_mixed[0]["kernels"]
+    dom = max(km, key=lambda k: km[k]["ms"])
+    d = km[dom]
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    cells = int(np.prod(ib[1:-2, 1:-2, 1:-2].shape))
+    s0 = st_mixed[-1]
+    out = {
+        "metric": METRIC,
+        "value": 1.0 / ms_mixed,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 V-cycles + f64 refinement",
+        "data": "synthetic (seeded divergence RHS, analytic ellipsoid geometry)",
+        "config": {"workload": "256x192x192 tanh-stretched (beta=2 in y,z), rotated prolate ellipsoid, "
+                               "divergence RHS, solve to 1e-8 relative residual",
+                   "cells": cells, "ghosts": gs.n, "levels": solver.level_info(),
+                   "nu_pre": args.nu, "nu_post": args.nu, "cycles_per_outer": args.cycles},
+        "fp64": {"solves_per_s": 1.0 / ms_fp64, "ms_per_solve": ms_fp64 * 1e3,
+                 "outer_iterations": st_fp64[-1]["outer_iterations"], "rel_l2": st_fp64[-1]["rel_l2"]},
+        "speedup_vs_fp64": ms_fp64 / ms_mixed,
+        "mixed": {"outer_iterations": s0["outer_iterations"], "vcycles": s0["vcycles"], "rel_l2": s0["rel_l2"],
+                  "abs_linf": s0["abs_linf"], "device_ms": s0["ms"], "algorithmic_GB": s0["bytes"] / 1e9,
+                  "solve_GBs": s0["bytes"] / 1e9 / (s0["ms"] * 1e-3)},
+        "error_vs_fp64_solve": err,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "launches": d["launches"], "avg_us": 1e3 * d["ms"] / max(d["launches"], 1),
+                     "bytes_per_launch": d["bytes"] / max(d["launches"], 1)},
+        "kernels_mixed": {k: {"ms": v["ms"], "GBs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else None,
+                              "launches": v["launches"], "ms_l0": v["ms_l0"],
+                              "GBs_l0": (v["bytes_l0"] / (v["ms_l0"] * 1e-3) / 1e9) if v["ms_l0"] > 0 else None}
+                          for k, v in km.items()},
+        "kernels_fp64": {k: {"ms": v["ms"], "launches": v["launches"]} for k, v in prof_fp64[0]["kernels"].items()},
+        "setup_s": t_setup,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        # the same projected RHS the GPU solved
+        w = solver.null_vector()
+        bp = rhs.copy()
+        bp[fl] -= (w[fl] * rhs[fl]).sum() / w[fl].sum()
+        out["cpu_baseline"] = cpu_baseline(xn, yn, zn, ib, gs, bp)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

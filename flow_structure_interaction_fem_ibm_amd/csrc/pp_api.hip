@@ -35,6 +35,57 @@ CellLaunch cell_launch(int cx, int cy, int cz) {
   return c;
 }
 
+// Byte accounting and opt-in timing of one launch: every launch site calls
+// kbegin(...) before and kend(...) after the launch.
+void kbegin(pp_ctx *c, int cls, int level, double bytes) {
+  c->bytes_acc += bytes;
+  c->kbytes[cls] += bytes;
+  c->klaunch[cls]++;
+  if (level == 0) { c->kbytes_l0[cls] += bytes; c->klaunch_l0[cls]++; }
+  if (!c->prof_on) return;
+  size_t q = 2 * c->prof_rec.size();
+  while (c->prof_ev.size() < q + 2) {
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    c->prof_ev.push_back(e);
+  }
+  (void)hipEventRecord(c->prof_ev[q], c->stream);
+  c->prof_rec.push_back({cls, level, bytes});
+}
+void kend(pp_ctx *c) {
+  if (!c->prof_on) return;
+  (void)hipEventRecord(c->prof_ev[2 * c->prof_rec.size() - 1], c->stream);
+}
+void kreset(pp_ctx *c, bool profile) {
+  c->bytes_acc = 0.0;
+  for (int q = 0; q < PP_K_CLASSES; q++) {
+    c->kbytes[q] = c->kbytes_l0[q] = 0.0;
+    c->klaunch[q] = c->klaunch_l0[q] = 0;
+  }
+  c->prof_on = profile;
+  c->prof_rec.clear();
+}
+// after the stream has been synchronised
+void kcollect(pp_ctx *c, pp_solve_stats &S) {
+  for (int q = 0; q < PP_K_CLASSES; q++) {
+    S.kern_bytes[q] = c->kbytes[q];
+    S.kern_bytes_l0[q] = c->kbytes_l0[q];
+    S.kern_launches[q] = c->klaunch[q];
+    S.kern_launches_l0[q] = c->klaunch_l0[q];
+    S.kern_ms[q] = S.kern_ms_l0[q] = 0.0;
+  }
+  if (c->prof_on) {
+    for (size_t r = 0; r < c->prof_rec.size(); r++) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, c->prof_ev[2 * r], c->prof_ev[2 * r + 1]);
+      S.kern_ms[c->prof_rec[r].cls] += ms;
+      if (c->prof_rec[r].level == 0) S.kern_ms_l0[c->prof_rec[r].cls] += ms;
+    }
+  }
+  c->prof_on = false;
+  c->prof_rec.clear();
+}
+
 template <typename T>
 LevelArgs<T> largs(const Level &L);
 template <>
@@ -97,7 +148,7 @@ double bytes_line(int s) { return 3.0 * s + 1.0; }
 // line passes
 // ---------------------------------------------------------------------
 template <typename T>
-void launch_xline(pp_ctx *c, Level &L, int color, T omega) {
+void launch_xline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
   LevelArgs<T> a = largs<T>(L);
   int nyh = (L.cy + 1) / 2;
   int waves = nyh * L.cz;
@@ -106,6 +157,7 @@ void launch_xline(pp_ctx *c, Level &L, int color, T omega) {
   const T *rhs = lv_rhs<T>(L);
   int S = (L.cx + 63) / 64;
   hipStream_t st = c->stream;
+  kbegin(c, PP_K_XLINE, lvl, 0.5 * (double)L.cx * L.cy * L.cz * bytes_line(sizeof(T)));
   switch (S) {
     case 1: k_xline<T, 1><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
     case 2: k_xline<T, 2><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
@@ -117,6 +169,7 @@ void launch_xline(pp_ctx *c, Level &L, int color, T omega) {
     case 8: k_xline<T, 8><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
     default: break;  // rejected at pp_create
   }
+  kend(c);
 }
 
 // segment length LS and segments P for a line of n cells
@@ -142,7 +195,7 @@ void yz_shape(int n, int &LS, int &P, int &TX, int elem = 4) {
 }
 
 template <typename T, int DIR>
-void launch_yzline(pp_ctx *c, Level &L, int color, T omega) {
+void launch_yzline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
   LevelArgs<T> a = largs<T>(L);
   int n = DIR == 1 ? L.cy : L.cz;
   int planes = DIR == 1 ? L.cz : L.cy;
@@ -154,6 +207,7 @@ void launch_yzline(pp_ctx *c, Level &L, int color, T omega) {
   T *e = lv_e<T>(L);
   const T *rhs = lv_rhs<T>(L);
   hipStream_t st = c->stream;
+  kbegin(c, DIR == 1 ? PP_K_YLINE : PP_K_ZLINE, lvl, 0.5 * (double)L.cx * L.cy * L.cz * bytes_line(sizeof(T)));
   switch (LS) {
     case 1: k_yzline<T, 1, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
     case 2: k_yzline<T, 2, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
@@ -162,6 +216,7 @@ void launch_yzline(pp_ctx *c, Level &L, int color, T omega) {
     case 12: k_yzline<T, 12, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
     default: k_yzline<T, 16, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
   }
+  kend(c);
 }
 
 // one Jacobi sweep of the homogeneous ghost rows on a level-0 correction
@@ -170,19 +225,73 @@ void ghost_sweep_level0(pp_ctx *c, T *e) {
   GhostTables &g = c->gh;
   if (g.n == 0) return;
   dim3 grid((g.n + 255) / 256), block(256);
+  kbegin(c, PP_K_GHOST, 0, g.n * (8.0 * (8 + 2 * sizeof(T)) + sizeof(T)));
   k_ghost_gather<T, T><<<grid, block, 0, c->stream>>>(g.n, g.s_al, ghost_coef<T>(g), nullptr, e, ghost_buf<T>(g));
+  kend(c);
+  kbegin(c, PP_K_GHOST, 0, g.n * (8.0 + 2 * sizeof(T)));
   k_ghost_scatter<T><<<grid, block, 0, c->stream>>>(g.n, g.g_al, ghost_buf<T>(g), e);
+  kend(c);
+}
+
+template <typename T>
+struct AdjPtrs;
+template <>
+struct AdjPtrs<float> {
+  static float *lam(GhostTables &g, int b) { return g.lam32[b]; }
+  static float *rhsg(GhostTables &g) { return g.rhsg32; }
+  static float *tbase(GhostTables &g) { return g.tbase32; }
+  static const float *nbw(GhostTables &g) { return g.nb_w32; }
+  static const float *tcoef(GhostTables &g) { return g.t_coef32; }
+  static const float *tvinv(GhostTables &g) { return g.t_vinv32; }
+};
+template <>
+struct AdjPtrs<double> {
+  static double *lam(GhostTables &g, int b) { return g.lam64[b]; }
+  static double *rhsg(GhostTables &g) { return g.rhsg64; }
+  static double *tbase(GhostTables &g) { return g.tbase64; }
+  static const double *nbw(GhostTables &g) { return g.nb_w64; }
+  static const double *tcoef(GhostTables &g) { return g.t_coef64; }
+  static const double *tvinv(GhostTables &g) { return g.t_vinv64; }
+};
+
+// Transposed ghost rows on the level-0 correction mu (homogeneous: the
+// ghost sources belong to the outer residual): lambda_g from the fluid
+// neighbours of each ghost, a few Jacobi sweeps of the transposed
+// ghost-ghost coupling (warm-started across smoothing steps), then the
+// touched fluid cells' rhs = base + (sum c lambda)/V.
+int g_adj_cur = 0;
+template <typename T>
+void adj_ghost_level0(pp_ctx *c, T *e, T *rhs) {
+  GhostTables &g = c->gh;
+  if (g.n == 0) return;
+  hipStream_t st = c->stream;
+  dim3 grid((g.n + 255) / 256), block(256);
+  kbegin(c, PP_K_GHOST, 0, g.n * 6.0 * (8 + 2 * sizeof(T)) + (double)g.nt * (24 + 2 * sizeof(T)));
+  k_adj_ghost_rhs<T><<<grid, block, 0, st>>>(g.n, g.nb_al, AdjPtrs<T>::nbw(g), nullptr, e, AdjPtrs<T>::rhsg(g));
+  int sweeps = std::min(g.kexact, 2);
+  for (int q = 0; q < sweeps; q++) {
+    k_adj_ghost_jacobi<T><<<grid, block, 0, st>>>(g.n, g.g_tidx, g.t_ptr, g.t_ghost, AdjPtrs<T>::tcoef(g),
+                                                  AdjPtrs<T>::rhsg(g), AdjPtrs<T>::lam(g, g_adj_cur),
+                                                  AdjPtrs<T>::lam(g, 1 - g_adj_cur));
+    g_adj_cur ^= 1;
+  }
+  if (g.nt > 0)
+    k_adj_touch<T><<<dim3((g.nt + 255) / 256), block, 0, st>>>(g.nt, g.t_al, g.t_ptr, g.t_ghost, AdjPtrs<T>::tcoef(g),
+                                                               AdjPtrs<T>::tvinv(g), AdjPtrs<T>::lam(g, g_adj_cur),
+                                                               AdjPtrs<T>::tbase(g), rhs);
+  kend(c);
 }
 
 template <typename T>
 void smooth(pp_ctx *c, int l, T omega) {
   Level &L = c->lv[l];
-  if (l == 0) ghost_sweep_level0<T>(c, lv_e<T>(L));
-  for (int col = 0; col < 2; col++) launch_xline<T>(c, L, col, omega);
-  for (int col = 0; col < 2; col++) launch_yzline<T, 1>(c, L, col, omega);
-  for (int col = 0; col < 2; col++) launch_yzline<T, 2>(c, L, col, omega);
-  double cells = (double)L.cx * L.cy * L.cz;
-  c->bytes_acc += 3.0 * cells * bytes_line(sizeof(T));
+  if (l == 0) {
+    if (c->adjoint_mode) adj_ghost_level0<T>(c, lv_e<T>(L), lv_rhs<T>(L));
+    else ghost_sweep_level0<T>(c, lv_e<T>(L));
+  }
+  for (int col = 0; col < 2; col++) launch_xline<T>(c, L, l, col, omega);
+  for (int col = 0; col < 2; col++) launch_yzline<T, 1>(c, L, l, col, omega);
+  for (int col = 0; col < 2; col++) launch_yzline<T, 2>(c, L, l, col, omega);
 }
 
 template <typename T>
@@ -193,26 +302,30 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   if (l == (int)c->lv.size() - 1) {
     LevelArgs<T> a = largs<T>(L);
     int n = c->cs.n;
+    kbegin(c, PP_K_COARSE, l, 8.0 * n * n + 2.0 * sizeof(T) * n);
     k_coarse_solve<T><<<1, 512, sizeof(double) * n, st>>>(a, n, c->cs.G, lv_rhs<T>(L), lv_e<T>(L));
-    c->bytes_acc += 8.0 * n * n;
+    kend(c);
     return;
   }
   T om = (T)o->omega;
   for (int s = 0; s < o->nu_pre; s++) smooth<T>(c, l, om);
   LevelArgs<T> a = largs<T>(L);
   CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
+  kbegin(c, PP_K_RESIDUAL, l, cells * (3.0 * sizeof(T) + 1.0));
   k_residual<T><<<cl.grid, cl.block, 0, st>>>(a, lv_e<T>(L), lv_rhs<T>(L), lv_res<T>(L));
-  c->bytes_acc += cells * (3.0 * sizeof(T) + 1.0);
+  kend(c);
   Level &C = c->lv[l + 1];
   LevelArgs<T> ac = largs<T>(C);
   CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
+  kbegin(c, PP_K_RESTRICT, l, cells * (sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz);
   k_restrict<T><<<cc.grid, cc.block, 0, st>>>(a, ac, C.cf[0], C.cf[1], C.cf[2], lv_res<T>(L), lv_rhs<T>(C),
                                                lv_e<T>(C));
-  c->bytes_acc += cells * (sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz;
+  kend(c);
   vcycle<T>(c, l + 1, o);
+  kbegin(c, PP_K_PROLONG, l, cells * (2.0 * sizeof(T) + 1.0) + (sizeof(T) + 1.0) * C.cx * C.cy * C.cz);
   k_prolong<T><<<cl.grid, cl.block, 0, st>>>(a, ac, pdir<T>(L, 0), pdir<T>(L, 1), pdir<T>(L, 2), lv_e<T>(C),
                                               lv_e<T>(L));
-  c->bytes_acc += cells * (2.0 * sizeof(T) + 1.0) + (sizeof(T) + 1.0) * C.cx * C.cy * C.cz;
+  kend(c);
   for (int s = 0; s < o->nu_post; s++) smooth<T>(c, l, om);
 }
 
@@ -235,9 +348,13 @@ void ghost_exact_ref(pp_ctx *c, double *p, bool with_bnd) {
   if (g.n > 0) {
     dim3 grid((g.n + 255) / 256), block(256);
     for (int s = 0; s < g.kexact; s++) {
+      kbegin(c, PP_K_GHOST, 0, g.n * (8.0 * 24 + 8));
       k_ghost_gather<double, double><<<grid, block, 0, st>>>(g.n, g.s_ref, g.coef64, with_bnd ? g.bnd : nullptr, p,
                                                             g.buf64);
+      kend(c);
+      kbegin(c, PP_K_GHOST, 0, g.n * 24.0);
       k_ghost_scatter<double><<<grid, block, 0, st>>>(g.n, g.g_ref, g.buf64, p);
+      kend(c);
     }
   }
   k_outer_ghost_x<<<dim3((c->nyc + 2 + 255) / 256, c->nzc + 2), 256, 0, st>>>(c->ref, p);
@@ -293,7 +410,7 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
   long nblk = (long)cl.grid.x * cl.grid.y * cl.grid.z;
   pp_solve_stats S;
   memset(&S, 0, sizeof(S));
-  c->bytes_acc = 0.0;
+  kreset(c, o->profile != 0);
   hipEvent_t ev0, ev1;
   HIPCHK(hipEventCreate(&ev0));
   HIPCHK(hipEventCreate(&ev1));
@@ -357,9 +474,10 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
   int outer = 0;
   for (;;) {
     // r = b - A p -> level-0 rhs (T), e0 = 0, norms
+    kbegin(c, PP_K_OUTER_RESIDUAL, 0, cells * (17.0 + 2.0 * sizeof(T)));
     k_outer_residual<T, 256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, p, c->d_b, lv_rhs<T>(L0), lv_e<T>(L0),
                                                           c->d_part);
-    c->bytes_acc += cells * (17.0 + 2.0 * sizeof(T));
+    kend(c);
     int rc = reduce_summax(c, (int)nblk);
     if (rc) return rc;
     double rl2 = std::sqrt(c->h_res[0]);
@@ -390,8 +508,9 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
       S.vcycles++;
     }
     // p += e on fluid cells, ghost rows exact
+    kbegin(c, PP_K_UPDATE, 0, cells * (17.0 + sizeof(T)));
     k_update<T><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, p, lv_e<T>(L0));
-    c->bytes_acc += cells * (17.0 + sizeof(T));
+    kend(c);
     ghost_exact_ref(c, p, have_bnd);
     outer++;
   }
@@ -412,8 +531,167 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
   S.bytes = c->bytes_acc;
   S.outer_iterations = outer;
   S.status = status;
+  kcollect(c, S);
   if (st) *st = S;
   return status;
+}
+
+// ---------------------------------------------------------------------
+// the transposed solve A^T lambda = s in mu = lambda/V on fluid cells,
+// lambda_g on ghosts: FP64 refinement around T-precision V-cycles.
+// d_mu (reference layout) holds the initial guess and the result (ghost
+// and solid entries stay 0); d_sv = s_f/V_f; sg = s_g (device, may be
+// null).  ref_norm < 0: converge relative to the first residual.
+// On return the exact lambda_g of the final mu is in gh.lam64[*lam_idx].
+// ---------------------------------------------------------------------
+template <typename T>
+int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref_norm, pp_solve_stats *st,
+                 int *lam_idx) {
+  hipStream_t s = c->stream;
+  Level &L0 = c->lv[0];
+  LevelArgs<double> a0 = largs<double>(L0);
+  CellLaunch cl = ref_launch(c);
+  long nblk = (long)cl.grid.x * cl.grid.y * cl.grid.z;
+  GhostTables &g = c->gh;
+  pp_solve_stats S;
+  memset(&S, 0, sizeof(S));
+  kreset(c, o->profile != 0);
+  double cells = (double)c->nxc * c->nyc * c->nzc;
+  hipEvent_t ev0, ev1;
+  HIPCHK(hipEventCreate(&ev0));
+  HIPCHK(hipEventCreate(&ev1));
+  HIPCHK(hipEventRecord(ev0, s));
+  HIPCHK(hipMemcpyAsync(c->d_bwork, c->d_sv, sizeof(double) * c->nref, hipMemcpyDeviceToDevice, s));
+  dim3 gg((g.n + 255) / 256), gt((g.nt + 255) / 256), blk(256);
+  if (g.nt > 0) k_gather_idx<double><<<gt, blk, 0, s>>>(g.nt, g.t_ref, c->d_sv, g.otbase64);
+  int cur = 0;
+  int status = PP_NOT_CONVERGED;
+  int outer = 0;
+  for (;;) {
+    if (g.n > 0) {
+      k_adj_ghost_rhs<double><<<gg, blk, 0, s>>>(g.n, g.nb_ref, g.nb_w64, sg, c->d_mu, g.orhsg64);
+      k_fill<double><<<gg, blk, 0, s>>>(g.olam64[0], g.n, 0.0);
+      cur = 0;
+      for (int q = 0; q < g.kexact; q++) {
+        k_adj_ghost_jacobi<double><<<gg, blk, 0, s>>>(g.n, g.g_tidx, g.t_ptr, g.t_ghost, g.t_coef64, g.orhsg64,
+                                                     g.olam64[cur], g.olam64[1 - cur]);
+        cur ^= 1;
+      }
+      if (g.nt > 0)
+        k_adj_touch<double><<<gt, blk, 0, s>>>(g.nt, g.t_ref, g.t_ptr, g.t_ghost, g.t_coef64, g.t_vinv64,
+                                               g.olam64[cur], g.otbase64, c->d_bwork);
+    }
+    kbegin(c, PP_K_OUTER_RESIDUAL, 0, cells * (17.0 + 2.0 * sizeof(T)));
+    k_outer_residual<T, 256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, c->d_bwork, lv_rhs<T>(L0),
+                                                          lv_e<T>(L0), c->d_part);
+    kend(c);
+    int rc = reduce_summax(c, (int)nblk);
+    if (rc) return rc;
+    double rl2 = std::sqrt(c->h_res[0]);
+    if (!std::isfinite(rl2)) {
+      set_error("non-finite adjoint residual after %d outer iterations", outer);
+      status = PP_NONFINITE;
+      break;
+    }
+    if (ref_norm < 0.0) ref_norm = rl2 > 0.0 ? rl2 : 1.0;
+    double rel = rl2 / (ref_norm > 0.0 ? ref_norm : 1.0);
+    S.rel_l2 = rel;
+    S.abs_linf = c->h_res[1];
+    if (outer > 0 && outer <= 64) S.history[outer - 1] = rel;
+    if ((o->rel_tol > 0.0 && rel <= o->rel_tol) || rl2 == 0.0) { status = PP_OK; break; }
+    if (outer >= o->max_outer) break;
+    if (g.nt > 0) k_gather_idx<T><<<gt, blk, 0, s>>>(g.nt, g.t_al, lv_rhs<T>(L0), AdjPtrs<T>::tbase(g));
+    if (g.n > 0) {
+      k_fill<T><<<gg, blk, 0, s>>>(AdjPtrs<T>::lam(g, 0), g.n, T(0));
+      g_adj_cur = 0;
+    }
+    c->adjoint_mode = true;
+    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
+      vcycle<T>(c, 0, o);
+      S.vcycles++;
+    }
+    c->adjoint_mode = false;
+    kbegin(c, PP_K_UPDATE, 0, cells * (17.0 + sizeof(T)));
+    k_update<T><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, lv_e<T>(L0));
+    kend(c);
+    outer++;
+  }
+  HIPCHK(hipEventRecord(ev1, s));
+  HIPCHK(hipEventSynchronize(ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  HIPCHK(hipGetLastError());
+  S.ms = ms;
+  S.bytes = c->bytes_acc;
+  S.outer_iterations = outer;
+  S.status = status;
+  S.b_l2 = ref_norm;
+  kcollect(c, S);
+  if (st) *st = S;
+  if (lam_idx) *lam_idx = cur;
+  return status;
+}
+
+double fluid_volume(const pp_ctx *c, int i, int j, int k) {  // 0-based interior cell
+  return c->rdx[0][i + 1] * c->rdx[1][j + 1] * c->rdx[2][k + 1];
+}
+
+// Left null vector of the ghost-cell operator: the transposed iteration
+// with s = 0 started from mu = 1 (the fluid volumes, exact without ghost
+// rows) converges to w; normalised so sum_f w_f = sum_f V_f.
+int compute_null_vector(pp_ctx *c) {
+  std::vector<double> w(c->nref, 0.0), mu(c->nref, 0.0);
+  double vsum = 0.0;
+  for (int k = 0; k < c->nzc; k++)
+    for (int j = 0; j < c->nyc; j++)
+      for (int i = 0; i < c->nxc; i++) {
+        long r = ridx(c->ref, i, j, k);
+        if (!c->h_iblank[r]) {
+          w[r] = fluid_volume(c, i, j, k);
+          mu[r] = 1.0;
+          vsum += w[r];
+        }
+      }
+  if (c->gh.n > 0) {
+    HIPCHK(hipMemcpy(c->d_mu, mu.data(), sizeof(double) * c->nref, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->d_sv, 0, sizeof(double) * c->nref));
+    pp_solve_opts o;
+    pp_default_opts(&o);
+    o.precision = PP_PREC_MIXED;
+    o.rel_tol = 1e-13;
+    o.max_outer = 60;
+    pp_solve_stats st;
+    int lam = 0;
+    int rc = adjoint_impl<float>(c, nullptr, &o, -1.0, &st, &lam);
+    if (rc != PP_OK && rc != PP_NOT_CONVERGED) return rc;
+    if (st.rel_l2 > 1e-10) {
+      set_error("left null vector did not converge (relative residual %g after %d iterations)", st.rel_l2,
+                st.outer_iterations);
+      return PP_BAD_GEOMETRY;
+    }
+    HIPCHK(hipMemcpy(mu.data(), c->d_mu, sizeof(double) * c->nref, hipMemcpyDeviceToHost));
+    std::vector<double> lg(c->gh.n);
+    HIPCHK(hipMemcpy(lg.data(), c->gh.olam64[lam], sizeof(double) * c->gh.n, hipMemcpyDeviceToHost));
+    double wsum = 0.0;
+    for (int k = 0; k < c->nzc; k++)
+      for (int j = 0; j < c->nyc; j++)
+        for (int i = 0; i < c->nxc; i++) {
+          long r = ridx(c->ref, i, j, k);
+          if (!c->h_iblank[r]) {
+            w[r] = fluid_volume(c, i, j, k) * mu[r];
+            wsum += w[r];
+          }
+        }
+    double scale = vsum / wsum;
+    for (long r = 0; r < c->nref; r++) w[r] *= scale;
+    for (int q = 0; q < c->gh.n; q++) w[c->gh.h_g_ref[q]] = lg[q] * scale;
+    c->null_iters = st.outer_iterations;
+    c->null_res = st.rel_l2;
+  }
+  HIPCHK(hipMemcpy(c->d_w, w.data(), sizeof(double) * c->nref, hipMemcpyHostToDevice));
+  return PP_OK;
 }
 
 }  // namespace
@@ -436,6 +714,7 @@ void pp_default_opts(pp_solve_opts *o) {
   o->mean_remove = 1;
   o->project_rhs = 1;
   o->use_graph = 0;
+  o->profile = 0;
 }
 
 const char *pp_last_error(void) { return pp::last_error(); }
@@ -488,6 +767,8 @@ int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
   HIPCHK(hipMalloc(&c->d_w, nb));
   HIPCHK(hipMalloc(&c->d_bwork, nb));
   HIPCHK(hipMalloc(&c->d_tmp, nb));
+  HIPCHK(hipMalloc(&c->d_mu, nb));
+  HIPCHK(hipMalloc(&c->d_sv, nb));
   HIPCHK(hipMemset(c->d_b, 0, nb));
   HIPCHK(hipMemset(c->d_w, 0, nb));
   CellLaunch cl = cell_launch(c->nxc, c->nyc, c->nzc);
@@ -506,7 +787,7 @@ int32_t pp_destroy(pp_ctx *c) {
   free_levels(c);
   free_ghosts(c);
   hipFree(c->d_p); hipFree(c->d_b); hipFree(c->d_w); hipFree(c->d_bwork); hipFree(c->d_tmp);
-  hipFree(c->d_fresh); hipFree(c->d_iblank_ref);
+  hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_mu); hipFree(c->d_sv);
   hipFree(c->d_part); hipFree(c->d_res);
   if (c->h_res) hipHostFree(c->h_res);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -569,17 +850,9 @@ int32_t pp_set_geometry(pp_ctx *c, const pp_geometry *geo) {
   hipFree(c->d_iblank_ref);
   HIPCHK(hipMalloc(&c->d_iblank_ref, c->nref));
   HIPCHK(hipMemcpy(c->d_iblank_ref, c->h_iblank.data(), c->nref, hipMemcpyHostToDevice));
-  // left null vector: fluid volumes (refined by the adjoint solve)
-  {
-    std::vector<double> w(c->nref, 0.0);
-    for (int k = 0; k < c->nzc; k++)
-      for (int j = 0; j < c->nyc; j++)
-        for (int i = 0; i < c->nxc; i++) {
-          long r = ridx(c->ref, i, j, k);
-          if (!c->h_iblank[r]) w[r] = c->rdx[0][i + 1] * c->rdx[1][j + 1] * c->rdx[2][k + 1];
-        }
-    HIPCHK(hipMemcpy(c->d_w, w.data(), sizeof(double) * c->nref, hipMemcpyHostToDevice));
-  }
+  c->have_geom = true;
+  rc = compute_null_vector(c);
+  if (rc) { c->have_geom = false; return rc; }
   HIPCHK(hipDeviceSynchronize());
   c->have_geom = true;
   return PP_OK;
@@ -770,11 +1043,75 @@ int32_t pp_divergence(pp_ctx *c, const double *fu, const double *fv, const doubl
   return PP_BAD_ARG;
 }
 
-int32_t pp_adjoint(pp_ctx *c, const double *w, double *lambda, double *dJ_dg, const pp_solve_opts *o,
+int32_t pp_adjoint(pp_ctx *c, const double *wobj, double *lambda, double *dJ_dg, const pp_solve_opts *o,
                    pp_solve_stats *st) {
-  (void)c; (void)w; (void)lambda; (void)dJ_dg; (void)o; (void)st;
-  set_error("pp_adjoint: not implemented yet");
-  return PP_BAD_ARG;
+  if (!c || !wobj || !lambda) { set_error("pp_adjoint: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_adjoint: no geometry"); return PP_NO_GEOMETRY; }
+  pp_solve_opts def;
+  if (!o) { pp_default_opts(&def); o = &def; }
+  if (!opts_ok(o)) { set_error("pp_adjoint: invalid options"); return PP_BAD_ARG; }
+  HIPCHK(hipSetDevice(c->device));
+  GhostTables &g = c->gh;
+  // J = wobj . M p with M the fluid-volume mean removal: s = M^T wobj
+  double S = 0.0, vsum = 0.0;
+  for (int k = 0; k < c->nzc; k++)
+    for (int j = 0; j < c->nyc; j++)
+      for (int i = 0; i < c->nxc; i++) {
+        long r = ridx(c->ref, i, j, k);
+        if (!c->h_iblank[r]) { S += wobj[r]; vsum += fluid_volume(c, i, j, k); }
+      }
+  std::vector<double> sg(g.n > 0 ? g.n : 1, 0.0);
+  for (int q = 0; q < g.n; q++) { sg[q] = wobj[g.h_g_ref[q]]; S += sg[q]; }
+  std::vector<double> sv(c->nref, 0.0), mu(c->nref, 0.0);
+  double ref = 0.0;
+  for (int k = 0; k < c->nzc; k++)
+    for (int j = 0; j < c->nyc; j++)
+      for (int i = 0; i < c->nxc; i++) {
+        long r = ridx(c->ref, i, j, k);
+        if (c->h_iblank[r]) continue;
+        double V = fluid_volume(c, i, j, k);
+        sv[r] = (wobj[r] - V * S / vsum) / V;
+        ref += sv[r] * sv[r];
+      }
+  ref = std::sqrt(ref);
+  HIPCHK(hipMemcpy(c->d_sv, sv.data(), sizeof(double) * c->nref, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_mu, mu.data(), sizeof(double) * c->nref, hipMemcpyHostToDevice));
+  double *dsg = nullptr;
+  if (g.n > 0) {
+    HIPCHK(hipMalloc(&dsg, sizeof(double) * g.n));
+    HIPCHK(hipMemcpy(dsg, sg.data(), sizeof(double) * g.n, hipMemcpyHostToDevice));
+  }
+  int lam = 0;
+  int rc = o->precision == PP_PREC_MIXED ? adjoint_impl<float>(c, dsg, o, ref > 0 ? ref : 1.0, st, &lam)
+                                         : adjoint_impl<double>(c, dsg, o, ref > 0 ? ref : 1.0, st, &lam);
+  if (dsg) hipFree(dsg);
+  if (rc != PP_OK && rc != PP_NOT_CONVERGED) return rc;
+  HIPCHK(hipMemcpy(mu.data(), c->d_mu, sizeof(double) * c->nref, hipMemcpyDeviceToHost));
+  std::vector<double> lg(g.n > 0 ? g.n : 1, 0.0), w(c->nref);
+  if (g.n > 0) HIPCHK(hipMemcpy(lg.data(), g.olam64[lam], sizeof(double) * g.n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(w.data(), c->d_w, sizeof(double) * c->nref, hipMemcpyDeviceToHost));
+  // dJ/db = Pi^T lambda with Pi the compatibility projection
+  double lsum = 0.0, wsum = 0.0;
+  for (int k = 0; k < c->nzc; k++)
+    for (int j = 0; j < c->nyc; j++)
+      for (int i = 0; i < c->nxc; i++) {
+        long r = ridx(c->ref, i, j, k);
+        if (c->h_iblank[r]) continue;
+        mu[r] *= fluid_volume(c, i, j, k);   // lambda_f
+        lsum += mu[r];
+        wsum += w[r];
+      }
+  double zeta = lsum / wsum;
+  for (long r = 0; r < c->nref; r++) lambda[r] = 0.0;
+  for (int k = 0; k < c->nzc; k++)
+    for (int j = 0; j < c->nyc; j++)
+      for (int i = 0; i < c->nxc; i++) {
+        long r = ridx(c->ref, i, j, k);
+        if (!c->h_iblank[r]) lambda[r] = mu[r] - zeta * w[r];
+      }
+  if (dJ_dg)
+    for (int q = 0; q < g.n; q++) dJ_dg[q] = g.h_bself[q] * (lg[q] - zeta * w[g.h_g_ref[q]]);
+  return rc;
 }
 
 }  // extern "C"

@@ -91,17 +91,26 @@ struct GhostTables {
   double *buf64 = nullptr;
   float *buf32 = nullptr;
   // adjoint: fluid neighbours of each ghost and their face conductances
+  // (coefficient of the ghost in the neighbour's row times its volume)
   long *nb_ref = nullptr, *nb_al = nullptr;     // 6 per ghost (-1: none)
-  double *nb_w = nullptr;
-  // transpose of the stencil: touched cells and the ghosts reading them
+  double *nb_w64 = nullptr;
+  float *nb_w32 = nullptr;
+  // transpose of the stencil: touched cells (fluid or ghost) and the
+  // ghosts whose stencils read them, CSR by touched cell
   int nt = 0;
   long *t_ref = nullptr, *t_al = nullptr;
   int *t_ptr = nullptr, *t_ghost = nullptr;
-  double *t_coef = nullptr;
+  double *t_coef64 = nullptr, *t_vinv64 = nullptr;
+  float *t_coef32 = nullptr, *t_vinv32 = nullptr;  // vinv = 1/V (fluid), 0 (ghost)
   int *g_tidx = nullptr;                        // touched index of a ghost cell or -1
+  // adjoint work buffers (inner V-cycle, per precision) and the FP64
+  // outer loop's own copies
+  double *lam64[2] = {nullptr, nullptr}, *rhsg64 = nullptr, *tbase64 = nullptr;
+  double *olam64[2] = {nullptr, nullptr}, *orhsg64 = nullptr, *otbase64 = nullptr;
+  float *lam32[2] = {nullptr, nullptr}, *rhsg32 = nullptr, *tbase32 = nullptr;
   // host copies
   std::vector<long> h_g_ref, h_g_al, h_s_ref, h_s_al, h_nb_ref, h_nb_al, h_t_ref, h_t_al;
-  std::vector<double> h_coef, h_bself, h_nb_w, h_t_coef;
+  std::vector<double> h_coef, h_bself, h_nb_w, h_t_coef, h_t_vinv;
   std::vector<int> h_t_ptr, h_t_ghost, h_g_tidx;
   std::vector<int> h_ig, h_jg, h_kg;            // 1-based reference indices
   int solid_in_stencil = 0;                     // stencil entries on solid non-ghost cells
@@ -137,10 +146,25 @@ struct pp_ctx {
   double *d_res = nullptr;             // final reduction results
   double *h_res = nullptr;             // pinned host mirror
   int nblk_red = 0;
+  bool adjoint_mode = false;
+  int null_iters = 0;
+  double null_res = 0.0;           // level-0 smoother uses the transposed ghost rows
+  double *d_mu = nullptr;              // adjoint unknown lambda/V (reference layout)
+  double *d_sv = nullptr;              // adjoint rhs s/V (reference layout)
   // graph cache
   hipGraphExec_t gexec = nullptr;
   int graph_key = -1;
   double bytes_acc = 0;
+  // opt-in per-launch timing (pp_solve_opts.profile)
+  bool prof_on = false;
+  std::vector<hipEvent_t> prof_ev;       // pool, 2 per launch
+  struct ProfRec {
+    int cls, level;
+    double bytes;
+  };
+  std::vector<ProfRec> prof_rec;
+  double kbytes[PP_K_CLASSES] = {}, kbytes_l0[PP_K_CLASSES] = {};
+  int klaunch[PP_K_CLASSES] = {}, klaunch_l0[PP_K_CLASSES] = {};
 };
 
 namespace pp {

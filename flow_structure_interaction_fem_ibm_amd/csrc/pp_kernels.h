@@ -329,6 +329,68 @@ __global__ void __launch_bounds__(256) k_fresh_fill(RefArgs R, const int8_t *__r
 // ghost rows: p_g = sum_r c_r p_r + bnd_g (GCM_p_set_bc_internal), Jacobi
 // with a buffer so the result does not depend on thread order.
 // ---------------------------------------------------------------------
+// adjoint (transposed) ghost rows.  With lambda_f = V_f mu_f on fluid cells
+// the transposed fluid block is the same Laplacian acting on mu; the ghost
+// columns give
+//   lambda_g = s_g - sum_{fluid f adj g} (a_fg V_f) mu_f + sum_g' c_g'g lambda_g'
+// and the fluid rows pick up t_f = sum_g c_gf lambda_g as a source:
+//   L mu = (s_f + t_f) / V_f.
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_ghost_rhs(int n, const long *__restrict__ nb_idx,
+                                                       const T *__restrict__ nb_w, const double *__restrict__ sg,
+                                                       const T *__restrict__ mu, T *__restrict__ rhsg) {
+  int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= n) return;
+  T s = sg ? (T)sg[g] : T(0);
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    long q = nb_idx[6 * g + m];
+    if (q >= 0) s -= nb_w[6 * g + m] * mu[q];
+  }
+  rhsg[g] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_ghost_jacobi(int n, const int *__restrict__ g_tidx,
+                                                          const int *__restrict__ t_ptr,
+                                                          const int *__restrict__ t_ghost,
+                                                          const T *__restrict__ t_coef, const T *__restrict__ rhsg,
+                                                          const T *__restrict__ lam_old, T *__restrict__ lam_new) {
+  int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= n) return;
+  T s = rhsg[g];
+  int t = g_tidx[g];
+  if (t >= 0)
+    for (int q = t_ptr[t]; q < t_ptr[t + 1]; q++) s += t_coef[q] * lam_old[t_ghost[q]];
+  lam_new[g] = s;
+}
+
+// target[t_idx[t]] = base[t] + vinv[t] * sum c lambda  for touched fluid cells
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_touch(int nt, const long *__restrict__ t_idx,
+                                                   const int *__restrict__ t_ptr, const int *__restrict__ t_ghost,
+                                                   const T *__restrict__ t_coef, const T *__restrict__ vinv,
+                                                   const T *__restrict__ lam, const T *__restrict__ base,
+                                                   T *__restrict__ target) {
+  int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nt) return;
+  T vi = vinv[t];
+  if (vi == T(0)) return;  // ghost cell: handled by the Jacobi rows
+  T s = T(0);
+  for (int q = t_ptr[t]; q < t_ptr[t + 1]; q++) s += t_coef[q] * lam[t_ghost[q]];
+  target[t_idx[t]] = base[t] + vi * s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather_idx(int n, const long *__restrict__ idx, const T *__restrict__ src,
+                                                    T *__restrict__ dst) {
+  int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  dst[t] = src[idx[t]];
+}
+
+// ---------------------------------------------------------------------
 template <typename T, typename TC>
 __global__ void __launch_bounds__(256) k_ghost_gather(int n, const long *__restrict__ sidx, const TC *__restrict__ coef,
                                                       const double *__restrict__ bnd, const T *__restrict__ p,

@@ -107,7 +107,10 @@ void free_levels(pp_ctx *c) {
 void free_ghosts(pp_ctx *c) {
   GhostTables &g = c->gh;
   void *ptrs[] = {g.g_ref, g.g_al, g.s_ref, g.s_al, g.coef64, g.coef32, g.bself, g.bnd, g.buf64, g.buf32,
-                  g.nb_ref, g.nb_al, g.nb_w, g.t_ref, g.t_al, g.t_ptr, g.t_ghost, g.t_coef, g.g_tidx};
+                  g.nb_ref, g.nb_al, g.nb_w64, g.nb_w32, g.t_ref, g.t_al, g.t_ptr, g.t_ghost, g.t_coef64,
+                  g.t_coef32, g.t_vinv64, g.t_vinv32, g.g_tidx, g.lam64[0], g.lam64[1], g.rhsg64,
+                  g.tbase64, g.lam32[0], g.lam32[1], g.rhsg32, g.tbase32, g.olam64[0], g.olam64[1],
+                  g.orhsg64, g.otbase64};
   for (void *p : ptrs) hipFree(p);
   g = GhostTables();
   hipFree(c->cs.G);
@@ -376,7 +379,94 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     }
     G.kexact = std::max(it, 1);
   }
+  // ---- adjoint tables ----
+  // fluid 6-neighbours of each ghost with the face conductance a*V of the
+  // neighbour's row entry for the ghost (fine-level coefficients
+  // dxcinv(face)*dxinv(cell) as in calc_residual)
+  {
+    auto V = [&](int i, int j, int k) { return c->rdx[0][i] * c->rdx[1][j] * c->rdx[2][k]; };
+    G.h_nb_ref.assign(6L * n, -1); G.h_nb_al.assign(6L * n, -1); G.h_nb_w.assign(6L * n, 0.0);
+    for (int q = 0; q < n; q++) {
+      int gi = G.h_ig[q], gj = G.h_jg[q], gk = G.h_kg[q];   // 1-based
+      int nbs[6][3] = {{gi - 1, gj, gk}, {gi + 1, gj, gk}, {gi, gj - 1, gk},
+                       {gi, gj + 1, gk}, {gi, gj, gk - 1}, {gi, gj, gk + 1}};
+      for (int m = 0; m < 6; m++) {
+        int i = nbs[m][0], j = nbs[m][1], k = nbs[m][2];
+        if (!inb(i, j, k)) continue;
+        long r = ridx(c->ref, i - 1, j - 1, k - 1);
+        if (c->h_iblank[r]) continue;
+        // the neighbour sits on side m of the ghost; the ghost is on the
+        // opposite side of the neighbour
+        double coef;
+        switch (m) {
+          case 0: coef = c->rdxcinv[0][i + 1] * c->rdxinv[0][i]; break;   // ghost is i+1 of f
+          case 1: coef = c->rdxcinv[0][i] * c->rdxinv[0][i]; break;       // ghost is i-1 of f
+          case 2: coef = c->rdxcinv[1][j + 1] * c->rdxinv[1][j]; break;
+          case 3: coef = c->rdxcinv[1][j] * c->rdxinv[1][j]; break;
+          case 4: coef = c->rdxcinv[2][k + 1] * c->rdxinv[2][k]; break;
+          default: coef = c->rdxcinv[2][k] * c->rdxinv[2][k]; break;
+        }
+        G.h_nb_ref[6L * q + m] = r;
+        G.h_nb_al[6L * q + m] = lidx(L0.sx, L0.sxy, i - 1, j - 1, k - 1);
+        G.h_nb_w[6L * q + m] = coef * V(i, j, k);
+      }
+    }
+    // touched cells: CSR of (ghost, coefficient) by stencil cell
+    std::vector<std::pair<long, std::pair<int, double>>> ent;
+    for (int q = 0; q < n; q++)
+      for (int r = 0; r < 8; r++) {
+        double cf = G.h_coef[8L * q + r];
+        long rs = G.h_s_ref[8L * q + r];
+        if (cf == 0.0 || rs == G.h_g_ref[q]) continue;
+        if (c->h_iblank[rs] == 1 && c->h_ghost[rs] != 1) continue;  // solid cells are not unknowns
+        ent.push_back({rs, {q, cf}});
+      }
+    std::stable_sort(ent.begin(), ent.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    G.h_t_ref.clear(); G.h_t_al.clear(); G.h_t_ptr.assign(1, 0); G.h_t_ghost.clear(); G.h_t_coef.clear();
+    G.h_t_vinv.clear();
+    std::vector<long> gpos(c->nref, -1);
+    for (int q = 0; q < n; q++) gpos[G.h_g_ref[q]] = q;
+    G.h_g_tidx.assign(n, -1);
+    for (size_t e = 0; e < ent.size(); e++) {
+      long rs = ent[e].first;
+      if (e == 0 || rs != ent[e - 1].first) {
+        if (e > 0) G.h_t_ptr.push_back((int)G.h_t_ghost.size());
+        int t = (int)G.h_t_ref.size();
+        G.h_t_ref.push_back(rs);
+        long z = rs / c->ref.exy, rem = rs % c->ref.exy, y = rem / c->ref.ex, x = rem % c->ref.ex;
+        G.h_t_al.push_back(lidx(L0.sx, L0.sxy, (int)x - 1, (int)y - 1, (int)z - 1));
+        bool fl = c->h_iblank[rs] == 0;
+        G.h_t_vinv.push_back(fl ? 1.0 / V((int)x, (int)y, (int)z) : 0.0);
+        if (gpos[rs] >= 0) G.h_g_tidx[gpos[rs]] = t;
+      }
+      G.h_t_ghost.push_back(ent[e].second.first);
+      G.h_t_coef.push_back(ent[e].second.second);
+    }
+    if (!ent.empty()) G.h_t_ptr.push_back((int)G.h_t_ghost.size());
+    G.nt = (int)G.h_t_ref.size();
+  }
   G.g_ref = dev_upload(G.h_g_ref); G.g_al = dev_upload(G.h_g_al);
+  G.nb_ref = dev_upload(G.h_nb_ref); G.nb_al = dev_upload(G.h_nb_al);
+  G.nb_w64 = dev_upload(G.h_nb_w); G.nb_w32 = dev_upload(to_f32(G.h_nb_w));
+  G.t_ref = dev_upload(G.h_t_ref); G.t_al = dev_upload(G.h_t_al); G.t_ptr = dev_upload(G.h_t_ptr);
+  G.t_ghost = dev_upload(G.h_t_ghost); G.t_coef64 = dev_upload(G.h_t_coef);
+  G.t_coef32 = dev_upload(to_f32(G.h_t_coef)); G.t_vinv64 = dev_upload(G.h_t_vinv);
+  G.t_vinv32 = dev_upload(to_f32(G.h_t_vinv)); G.g_tidx = dev_upload(G.h_g_tidx);
+  if (n > 0) {
+    for (int b = 0; b < 2; b++) {
+      hipMalloc(&G.lam64[b], sizeof(double) * n);
+      hipMalloc(&G.olam64[b], sizeof(double) * n);
+      hipMalloc(&G.lam32[b], sizeof(float) * n);
+    }
+    hipMalloc(&G.rhsg64, sizeof(double) * n);
+    hipMalloc(&G.orhsg64, sizeof(double) * n);
+    hipMalloc(&G.rhsg32, sizeof(float) * n);
+  }
+  if (G.nt > 0) {
+    hipMalloc(&G.tbase64, sizeof(double) * G.nt);
+    hipMalloc(&G.otbase64, sizeof(double) * G.nt);
+    hipMalloc(&G.tbase32, sizeof(float) * G.nt);
+  }
   G.s_ref = dev_upload(G.h_s_ref); G.s_al = dev_upload(G.h_s_al);
   G.coef64 = dev_upload(G.h_coef); G.coef32 = dev_upload(to_f32(G.h_coef));
   G.bself = dev_upload(G.h_bself);

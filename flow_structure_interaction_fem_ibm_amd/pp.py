@@ -71,14 +71,22 @@ class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("rel_tol", C.c_double), ("abs_linf_tol", C.c_double),
                 ("max_outer", C.c_int32), ("cycles_per_outer", C.c_int32),
                 ("nu_pre", C.c_int32), ("nu_post", C.c_int32), ("omega", C.c_double),
-                ("mean_remove", C.c_int32), ("project_rhs", C.c_int32), ("use_graph", C.c_int32)]
+                ("mean_remove", C.c_int32), ("project_rhs", C.c_int32), ("use_graph", C.c_int32),
+                ("profile", C.c_int32)]
+
+
+KERNEL_CLASSES = ("outer_residual", "update", "xline", "yline", "zline", "residual", "restrict",
+                  "prolong", "ghost", "coarse", "other")
 
 
 class SolveStats(C.Structure):
     _fields_ = [("status", C.c_int32), ("outer_iterations", C.c_int32), ("vcycles", C.c_int32),
                 ("rel_l2", C.c_double), ("abs_linf", C.c_double), ("loc", C.c_int32 * 3),
                 ("b_l2", C.c_double), ("rhs_shift", C.c_double), ("mean_removed", C.c_double),
-                ("ms", C.c_double), ("bytes", C.c_double), ("history", C.c_double * 64)]
+                ("ms", C.c_double), ("bytes", C.c_double), ("history", C.c_double * 64),
+                ("kern_ms", C.c_double * 11), ("kern_bytes", C.c_double * 11),
+                ("kern_launches", C.c_int32 * 11), ("kern_ms_l0", C.c_double * 11),
+                ("kern_bytes_l0", C.c_double * 11), ("kern_launches_l0", C.c_int32 * 11)]
 
     def as_dict(self):
         n = min(self.outer_iterations, 64)
@@ -87,7 +95,12 @@ class SolveStats(C.Structure):
                 "rel_l2": self.rel_l2, "abs_linf": self.abs_linf, "loc": list(self.loc),
                 "b_l2": self.b_l2, "rhs_shift": self.rhs_shift,
                 "mean_removed": self.mean_removed, "ms": self.ms, "bytes": self.bytes,
-                "history": [self.history[i] for i in range(n)]}
+                "history": [self.history[i] for i in range(n)],
+                "kernels": {name: {"ms": self.kern_ms[q], "bytes": self.kern_bytes[q],
+                                   "launches": self.kern_launches[q], "ms_l0": self.kern_ms_l0[q],
+                                   "bytes_l0": self.kern_bytes_l0[q],
+                                   "launches_l0": self.kern_launches_l0[q]}
+                            for q, name in enumerate(KERNEL_CLASSES)}}
 
 
 _lib = None

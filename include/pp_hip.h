@@ -84,7 +84,22 @@ typedef struct pp_solve_opts {
   int32_t mean_remove;       /* subtract the fluid volume mean on return (0/1) */
   int32_t project_rhs;       /* compatibility projection of the RHS (0/1) */
   int32_t use_graph;         /* replay the outer iteration as a hipGraph (0/1) */
+  int32_t profile;           /* time every kernel launch with HIP events (0/1) */
 } pp_solve_opts;
+
+/* kernel classes of pp_solve_stats.kern_* (profile = 1) */
+#define PP_K_OUTER_RESIDUAL 0  /* FP64 b - A p on the reference layout */
+#define PP_K_UPDATE 1          /* FP64 p += e */
+#define PP_K_XLINE 2           /* x-line zebra pass (wavefront PCR) */
+#define PP_K_YLINE 3           /* y-line zebra pass (partitioned Thomas) */
+#define PP_K_ZLINE 4           /* z-line zebra pass (partitioned Thomas) */
+#define PP_K_RESIDUAL 5        /* level residual */
+#define PP_K_RESTRICT 6
+#define PP_K_PROLONG 7
+#define PP_K_GHOST 8           /* ghost-row gathers/scatters */
+#define PP_K_COARSE 9          /* coarsest-level dense solve */
+#define PP_K_OTHER 10
+#define PP_K_CLASSES 11
 
 typedef struct pp_solve_stats {
   int32_t status;
@@ -99,6 +114,15 @@ typedef struct pp_solve_stats {
   double ms;                 /* device time of the solve (HIP events) */
   double bytes;              /* algorithmic bytes of the launched kernels */
   double history[64];        /* rel_l2 after each outer iteration */
+  /* profile = 1: per kernel class, summed device time of the launches on
+   * the finest level (level 0) and over all levels, algorithmic bytes and
+   * launch counts */
+  double kern_ms[PP_K_CLASSES];
+  double kern_bytes[PP_K_CLASSES];
+  int32_t kern_launches[PP_K_CLASSES];
+  double kern_ms_l0[PP_K_CLASSES];
+  double kern_bytes_l0[PP_K_CLASSES];
+  int32_t kern_launches_l0[PP_K_CLASSES];
 } pp_solve_stats;
 
 /* Library defaults for pp_solve_opts. */

@@ -157,15 +157,17 @@ def test_apply_transpose_is_adjoint():
 
 
 def left_null_vector(A, unknown, V):
-    """w with A^T w = 0 on the unknowns, normalised like the fluid volumes."""
+    """w with A^T w = 0 on the unknowns, normalised so that the sum of w over
+    the fluid cells equals the fluid volume (V: volumes, zero on blank cells)."""
     idx = np.flatnonzero(unknown)
     As = A[idx][:, idx].tocsc()
     n = len(idx)
     import scipy.sparse as sp
-    # bordered system [A^T e; v^T 0] [w; s] = [0; sum v]
+    # bordered system [A^T e; f^T 0] [w; s] = [0; sum V_f], f = fluid indicator
     v = V.ravel()[idx]
+    f = (v > 0).astype(float)
     e = np.ones(n)
-    M = sp.bmat([[As.T, sp.csc_matrix(e[:, None])], [sp.csc_matrix(v[None, :]), None]]).tocsc()
+    M = sp.bmat([[As.T, sp.csc_matrix(e[:, None])], [sp.csc_matrix(f[None, :]), None]]).tocsc()
     rhs = np.zeros(n + 1)
     rhs[-1] = v.sum()
     sol = spla.spsolve(M, rhs)
