@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--nu", type=int, default=2)
     ap.add_argument("--cycles", type=int, default=1)
+    ap.add_argument("--krylov", type=int, default=4,
+                    help="flexible GMRES restart length around the V-cycles (0: plain refinement)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the solve is a single-GPU "
@@ -124,7 +126,7 @@ def main():
     d_rhs = torch.from_numpy(rhs).to(dev)
     d_p = torch.zeros(solver.shape, dtype=torch.float64, device=dev)
     common = dict(rel_tol=args.tol, max_outer=200, nu_pre=args.nu, nu_post=args.nu,
-                  cycles_per_outer=args.cycles)
+                  cycles_per_outer=args.cycles, krylov=args.krylov)
 
     def run(prec, steps, warm, profile=False):
         for _ in range(warm):
@@ -175,13 +177,15 @@ def main():
         "config": {"workload": "256x192x192 tanh-stretched (beta=2 in y,z), rotated prolate ellipsoid, "
                                "divergence RHS, solve to 1e-8 relative residual",
                    "cells": cells, "ghosts": gs.n, "levels": solver.level_info(),
-                   "nu_pre": args.nu, "nu_post": args.nu, "cycles_per_outer": args.cycles},
+                   "nu_pre": args.nu, "nu_post": args.nu, "cycles_per_outer": args.cycles,
+                   "krylov_restart": args.krylov},
         "fp64": {"solves_per_s": 1.0 / ms_fp64, "ms_per_solve": ms_fp64 * 1e3,
                  "outer_iterations": st_fp64[-1]["outer_iterations"], "rel_l2": st_fp64[-1]["rel_l2"]},
         "speedup_vs_fp64": ms_fp64 / ms_mixed,
         "mixed": {"outer_iterations": s0["outer_iterations"], "vcycles": s0["vcycles"], "rel_l2": s0["rel_l2"],
                   "abs_linf": s0["abs_linf"], "device_ms": s0["ms"], "algorithmic_GB": s0["bytes"] / 1e9,
-                  "solve_GBs": s0["bytes"] / 1e9 / (s0["ms"] * 1e-3)},
+                  "solve_GBs": s0["bytes"] / 1e9 / (s0["ms"] * 1e-3),
+                  "history": s0["history"]},
         "error_vs_fp64_solve": err,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

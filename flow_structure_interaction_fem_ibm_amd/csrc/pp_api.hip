@@ -27,7 +27,8 @@ struct CellLaunch {
   dim3 grid, block;
 };
 CellLaunch cell_launch(int cx, int cy, int cz) {
-  int bx = std::min(256, ((cx + 63) / 64) * 64);
+  // 256 threads per block always (the block reductions assume 4 waves)
+  int bx = cx <= 64 ? 64 : (cx <= 128 ? 128 : 256);
   int by = 256 / bx;
   CellLaunch c;
   c.block = dim3(bx, by, 1);
@@ -394,11 +395,141 @@ int mean_remove_ref(pp_ctx *c, double *p, double *mean_out) {
 bool opts_ok(const pp_solve_opts *o) {
   return o && (o->precision == PP_PREC_FP64 || o->precision == PP_PREC_MIXED) && o->max_outer >= 0 &&
          o->max_outer <= 10000 && o->cycles_per_outer >= 1 && o->nu_pre >= 0 && o->nu_post >= 0 &&
-         o->omega > 0.0 && o->omega < 2.0;
+         o->omega > 0.0 && o->omega < 2.0 && o->krylov >= 0 && o->krylov <= KRY_MAX;
 }
 
 // ---------------------------------------------------------------------
-// the solve (FP64 iterative refinement around T-precision V-cycles)
+// flexible GMRES(m) on the fluid cells (FP64, reference layout)
+// ---------------------------------------------------------------------
+int ensure_krylov(pp_ctx *c, int m) {
+  if (c->kry_m >= m) return PP_OK;
+  for (double *q : c->d_kry) hipFree(q);
+  c->d_kry.clear();
+  c->kry_m = 0;
+  size_t nb = sizeof(double) * c->nref;
+  for (int q = 0; q < 2 * m + 1; q++) {
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, nb));
+    c->d_kry.push_back(d);
+    HIPCHK(hipMemset(d, 0, nb));
+  }
+  c->kry_m = m;
+  return PP_OK;
+}
+
+// sum of nq per-block partials -> c->h_res[0..nq)
+int reduce_sums_host(pp_ctx *c, int nq) {
+  k_reduce_sums<<<1, 256, 0, c->stream>>>(c->d_part, c->nblk_red, nq, c->d_res);
+  HIPCHK(hipMemcpyAsync(c->h_res, c->d_res, sizeof(double) * nq, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PP_OK;
+}
+
+// One restart cycle.  On entry d_kry[0] holds r = b - A p (fluid cells) and
+// the level-0 rhs holds r in T with e0 = 0; beta = |r|.  Each step applies
+// the V-cycle(s) to the current basis vector (right preconditioning), keeps
+// the preconditioned vector z_j (flexible variant: the V-cycle in FP32 is
+// not exactly linear), orthogonalises A z_j by classical Gram-Schmidt and
+// updates the Givens QR of the Hessenberg matrix.  Stops at m steps, at the
+// iteration limit or when the least-squares residual meets the tolerance;
+// then p += Z y and the ghost rows are made exact.
+template <typename T>
+int krylov_cycle(pp_ctx *c, const pp_solve_opts *o, double *p, bool have_bnd, double beta, double bden,
+                 int &outer, pp_solve_stats &S) {
+  hipStream_t s = c->stream;
+  Level &L0 = c->lv[0];
+  LevelArgs<double> a0 = largs<double>(L0);
+  CellLaunch cl = ref_launch(c);
+  double cells = (double)c->nxc * c->nyc * c->nzc;
+  const int m = o->krylov;
+  double *U[KRY_MAX + 1], *Z[KRY_MAX];
+  for (int q = 0; q <= m; q++) U[q] = c->d_kry[q];
+  for (int q = 0; q < m; q++) Z[q] = c->d_kry[c->kry_m + 1 + q];
+  std::vector<double> H((m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), gv(m + 1, 0.0), sig(m + 1, 0.0);
+  auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
+  sig[0] = 1.0 / beta;
+  gv[0] = beta;
+  int k = 0;
+  for (int j = 0; j < m && outer < o->max_outer; j++) {
+    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
+      vcycle<T>(c, 0, o);
+      S.vcycles++;
+    }
+    kbegin(c, PP_K_KRYLOV, 0, cells * (sizeof(T) + 1.0 + 8.0));
+    k_kry_z<T><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, lv_e<T>(L0), Z[j]);
+    kend(c);
+    ghost_exact_ref(c, Z[j], false);
+    KryVecs V{};
+    V.n = j + 1;
+    for (int q = 0; q <= j; q++) V.u[q] = U[q];
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1)));
+    k_kry_apply<256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, Z[j], U[j + 1], V, c->d_part);
+    kend(c);
+    int rc = reduce_sums_host(c, j + 1);
+    if (rc) return rc;
+    for (int q = 0; q <= j; q++) {
+      double d = c->h_res[q];
+      h(q, j) = sig[j] * sig[q] * d;
+      V.c[q] = sig[q] * sig[q] * d;
+    }
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1) + 2.0 * sizeof(T)));
+    k_kry_orth<T, 256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, U[j + 1], V, lv_rhs<T>(L0), lv_e<T>(L0), c->d_part);
+    kend(c);
+    rc = reduce_sums_host(c, 1);
+    if (rc) return rc;
+    double hn = std::sqrt(c->h_res[0]);
+    h(j + 1, j) = sig[j] * hn;
+    sig[j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
+    for (int i = 0; i < j; i++) {
+      double t = cs[i] * h(i, j) + sn[i] * h(i + 1, j);
+      h(i + 1, j) = -sn[i] * h(i, j) + cs[i] * h(i + 1, j);
+      h(i, j) = t;
+    }
+    double rho = std::hypot(h(j, j), h(j + 1, j));
+    if (!std::isfinite(rho)) {
+      set_error("non-finite Krylov coefficient after %d iterations", outer);
+      return PP_NONFINITE;
+    }
+    if (rho == 0.0) break;  // A z_j adds nothing: keep the first j columns
+    cs[j] = h(j, j) / rho;
+    sn[j] = h(j + 1, j) / rho;
+    h(j, j) = rho;
+    h(j + 1, j) = 0.0;
+    gv[j + 1] = -sn[j] * gv[j];
+    gv[j] = cs[j] * gv[j];
+    k = j + 1;
+    outer++;
+    double est = std::fabs(gv[j + 1]);
+    if (outer <= 64) S.history[outer - 1] = est / bden;
+    bool done = (o->rel_tol > 0.0 && est <= o->rel_tol * bden) || (o->abs_linf_tol > 0.0 && est <= o->abs_linf_tol);
+    if (done || hn == 0.0) break;
+  }
+  if (k == 0) {
+    set_error("Krylov breakdown: the preconditioned residual has no component in the range");
+    return PP_NOT_CONVERGED;
+  }
+  std::vector<double> y(k);
+  for (int i = k - 1; i >= 0; i--) {
+    double t = gv[i];
+    for (int q = i + 1; q < k; q++) t -= h(i, q) * y[q];
+    y[i] = t / h(i, i);
+  }
+  KryVecs V{};
+  V.n = k;
+  for (int q = 0; q < k; q++) {
+    V.u[q] = Z[q];
+    V.c[q] = y[q] * sig[q];
+  }
+  kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * k));
+  k_kry_update<<<cl.grid, cl.block, 0, s>>>(c->ref, a0, p, V);
+  kend(c);
+  ghost_exact_ref(c, p, have_bnd);
+  return PP_OK;
+}
+
+// ---------------------------------------------------------------------
+// the solve (FP64 iterative refinement or flexible GMRES around
+// T-precision V-cycles)
 // ---------------------------------------------------------------------
 template <typename T>
 int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, const pp_solve_opts *o,
@@ -416,6 +547,11 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
   HIPCHK(hipEventCreate(&ev1));
   HIPCHK(hipEventRecord(ev0, s));
   double cells = (double)c->nxc * c->nyc * c->nzc;
+  const int m = o->krylov;
+  if (m > 0) {
+    int rc = ensure_krylov(c, m);
+    if (rc) return rc;
+  }
   // ghost Neumann data: bnd = (c_self + probeLength) * uNormal
   GhostTables &g = c->gh;
   bool have_bnd = bc_src != nullptr && g.n > 0;
@@ -476,7 +612,7 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
     // r = b - A p -> level-0 rhs (T), e0 = 0, norms
     kbegin(c, PP_K_OUTER_RESIDUAL, 0, cells * (17.0 + 2.0 * sizeof(T)));
     k_outer_residual<T, 256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, p, c->d_b, lv_rhs<T>(L0), lv_e<T>(L0),
-                                                          c->d_part);
+                                                          c->d_part, m > 0 ? c->d_kry[0] : nullptr);
     kend(c);
     int rc = reduce_summax(c, (int)nblk);
     if (rc) return rc;
@@ -503,6 +639,12 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
                 bnorm == 0.0;
     if (conv) { status = PP_OK; break; }
     if (outer >= o->max_outer) break;
+    if (m > 0) {
+      int krc = krylov_cycle<T>(c, o, p, have_bnd, rl2, bden, outer, S);
+      if (krc == PP_NOT_CONVERGED || krc == PP_NONFINITE) { status = krc; break; }
+      if (krc) return krc;
+      continue;
+    }
     for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
       vcycle<T>(c, 0, o);
       S.vcycles++;
@@ -583,7 +725,7 @@ int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref
     }
     kbegin(c, PP_K_OUTER_RESIDUAL, 0, cells * (17.0 + 2.0 * sizeof(T)));
     k_outer_residual<T, 256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, c->d_bwork, lv_rhs<T>(L0),
-                                                          lv_e<T>(L0), c->d_part);
+                                                          lv_e<T>(L0), c->d_part, nullptr);
     kend(c);
     int rc = reduce_summax(c, (int)nblk);
     if (rc) return rc;
@@ -715,6 +857,7 @@ void pp_default_opts(pp_solve_opts *o) {
   o->project_rhs = 1;
   o->use_graph = 0;
   o->profile = 0;
+  o->krylov = 4;
 }
 
 const char *pp_last_error(void) { return pp::last_error(); }
@@ -773,9 +916,10 @@ int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
   HIPCHK(hipMemset(c->d_w, 0, nb));
   CellLaunch cl = cell_launch(c->nxc, c->nyc, c->nzc);
   c->nblk_red = (int)((long)cl.grid.x * cl.grid.y * cl.grid.z);
-  HIPCHK(hipMalloc(&c->d_part, sizeof(double) * 5 * (size_t)c->nblk_red + 64));
-  HIPCHK(hipMalloc(&c->d_res, sizeof(double) * 16));
-  HIPCHK(hipHostMalloc(&c->h_res, sizeof(double) * 16));
+  // partial sums: up to KRY_MAX+1 per block (Krylov dots), 5 (fluid sums)
+  HIPCHK(hipMalloc(&c->d_part, sizeof(double) * (KRY_MAX + 1) * (size_t)c->nblk_red + 64));
+  HIPCHK(hipMalloc(&c->d_res, sizeof(double) * 32));
+  HIPCHK(hipHostMalloc(&c->h_res, sizeof(double) * 32));
   *out = c;
   return PP_OK;
 }
@@ -789,6 +933,7 @@ int32_t pp_destroy(pp_ctx *c) {
   hipFree(c->d_p); hipFree(c->d_b); hipFree(c->d_w); hipFree(c->d_bwork); hipFree(c->d_tmp);
   hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_mu); hipFree(c->d_sv);
   hipFree(c->d_part); hipFree(c->d_res);
+  for (double *q : c->d_kry) hipFree(q);
   if (c->h_res) hipHostFree(c->h_res);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   if (c->stream) hipStreamDestroy(c->stream);

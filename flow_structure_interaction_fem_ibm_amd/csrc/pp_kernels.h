@@ -155,7 +155,8 @@ __global__ void k_fill(T *__restrict__ p, long n, T v) {
 template <typename TE, int NT>
 __global__ void __launch_bounds__(NT) k_outer_residual(RefArgs R, LevelArgs<double> L, const double *__restrict__ p,
                                                        const double *__restrict__ b, TE *__restrict__ rhs0,
-                                                       TE *__restrict__ e0, double *__restrict__ part) {
+                                                       TE *__restrict__ e0, double *__restrict__ part,
+                                                       double *__restrict__ rout) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
@@ -172,6 +173,7 @@ __global__ void __launch_bounds__(NT) k_outer_residual(RefArgs R, LevelArgs<doub
     }
     rhs0[a] = (TE)res;
     e0[a] = TE(0);
+    if (rout) rout[r] = res;
     v.sum = res * res;
     v.mx = fabs(res);
     v.arg = (long)r;
@@ -807,6 +809,105 @@ __global__ void __launch_bounds__(1024) k_coarse_solve(LevelArgs<T> L, int n, co
     int i = q % L.cx, t = q / L.cx, j = t % L.cy, k = t / L.cy;
     e[lidx(L.sx, L.sxy, i, j, k)] = (T)s;
   }
+}
+
+// ---------------------------------------------------------------------
+// Flexible GMRES on the reference layout (FP64 vectors u_q, zero on every
+// non-fluid cell).  The basis is kept unnormalised (v_q = sigma_q u_q, the
+// sigmas live on the host) so that no separate scaling pass is needed.
+// ---------------------------------------------------------------------
+constexpr int KRY_MAX = 16;
+struct KryVecs {
+  const double *u[KRY_MAX + 1];
+  double c[KRY_MAX + 1];
+  int n;
+};
+
+// z = e0 on fluid cells, 0 on blank cells (ghost rows are filled afterwards)
+template <typename TE>
+__global__ void __launch_bounds__(256) k_kry_z(RefArgs R, LevelArgs<double> L, const TE *__restrict__ e0,
+                                               double *__restrict__ z) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= R.nxc || j >= R.nyc) return;
+  long a = lidx(L.sx, L.sxy, i, j, k);
+  z[ridx(R, i, j, k)] = (L.fl[a] & FL_BLANK) ? 0.0 : (double)e0[a];
+}
+
+// w = A z on fluid cells (0 elsewhere) with z's ghost rows and outer layer
+// already exact; per-block partial dots <w, u_q>, q < V.n, at part[blk*V.n+q]
+template <int NT>
+__global__ void __launch_bounds__(NT) k_kry_apply(RefArgs R, LevelArgs<double> L, const double *__restrict__ z,
+                                                  double *__restrict__ w, KryVecs V, double *__restrict__ part) {
+  __shared__ double s[NT / 64];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  bool in = i < R.nxc && j < R.nyc;
+  double wv = 0.0;
+  long r = 0;
+  if (in) {
+    long a = lidx(L.sx, L.sxy, i, j, k);
+    r = ridx(R, i, j, k);
+    uint8_t f = L.fl[a];
+    if (!(f & FL_BLANK)) {
+      Row<double> rw = make_row(L, f, i, j, k);
+      wv = z[r] * rw.diag + z[r - 1] * rw.amx + z[r + 1] * rw.apx + z[r - R.ex] * rw.amy + z[r + R.ex] * rw.apy +
+           z[r - R.exy] * rw.amz + z[r + R.exy] * rw.apz;
+    }
+    w[r] = wv;
+  }
+  long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+#pragma unroll
+  for (int q = 0; q < KRY_MAX + 1; q++) {
+    if (q < V.n) {
+      double d = block_sum<NT>(in ? wv * V.u[q][r] : 0.0, s);
+      if (threadIdx.x == 0 && threadIdx.y == 0) part[blk * V.n + q] = d;
+    }
+  }
+}
+
+// u = w - sum_q c_q u_q on fluid cells (in place in w), level-0 rhs = u,
+// e0 = 0, per-block partial |u|^2 at part[blk]
+template <typename TE, int NT>
+__global__ void __launch_bounds__(NT) k_kry_orth(RefArgs R, LevelArgs<double> L, double *__restrict__ w, KryVecs V,
+                                                 TE *__restrict__ rhs0, TE *__restrict__ e0,
+                                                 double *__restrict__ part) {
+  __shared__ double s[NT / 64];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  double u = 0.0;
+  if (i < R.nxc && j < R.nyc) {
+    long a = lidx(L.sx, L.sxy, i, j, k);
+    long r = ridx(R, i, j, k);
+    if (!(L.fl[a] & FL_BLANK)) {
+      u = w[r];
+#pragma unroll
+      for (int q = 0; q < KRY_MAX + 1; q++)
+        if (q < V.n) u -= V.c[q] * V.u[q][r];
+      w[r] = u;
+    }
+    rhs0[a] = (TE)u;
+    e0[a] = TE(0);
+  }
+  double d = block_sum<NT>(u * u, s);
+  long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[blk] = d;
+}
+
+// p += sum_q c_q u_q on fluid cells
+__global__ void __launch_bounds__(256) k_kry_update(RefArgs R, LevelArgs<double> L, double *__restrict__ p,
+                                                    KryVecs V) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= R.nxc || j >= R.nyc) return;
+  long a = lidx(L.sx, L.sxy, i, j, k);
+  if (L.fl[a] & FL_BLANK) return;
+  long r = ridx(R, i, j, k);
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < KRY_MAX + 1; q++)
+    if (q < V.n) acc += V.c[q] * V.u[q][r];
+  p[r] = p[r] + acc;
 }
 
 // ---------------------------------------------------------------------

@@ -72,11 +72,11 @@ class SolveOpts(C.Structure):
                 ("max_outer", C.c_int32), ("cycles_per_outer", C.c_int32),
                 ("nu_pre", C.c_int32), ("nu_post", C.c_int32), ("omega", C.c_double),
                 ("mean_remove", C.c_int32), ("project_rhs", C.c_int32), ("use_graph", C.c_int32),
-                ("profile", C.c_int32)]
+                ("profile", C.c_int32), ("krylov", C.c_int32)]
 
 
 KERNEL_CLASSES = ("outer_residual", "update", "xline", "yline", "zline", "residual", "restrict",
-                  "prolong", "ghost", "coarse", "other")
+                  "prolong", "ghost", "coarse", "other", "krylov")
 
 
 class SolveStats(C.Structure):
@@ -84,9 +84,9 @@ class SolveStats(C.Structure):
                 ("rel_l2", C.c_double), ("abs_linf", C.c_double), ("loc", C.c_int32 * 3),
                 ("b_l2", C.c_double), ("rhs_shift", C.c_double), ("mean_removed", C.c_double),
                 ("ms", C.c_double), ("bytes", C.c_double), ("history", C.c_double * 64),
-                ("kern_ms", C.c_double * 11), ("kern_bytes", C.c_double * 11),
-                ("kern_launches", C.c_int32 * 11), ("kern_ms_l0", C.c_double * 11),
-                ("kern_bytes_l0", C.c_double * 11), ("kern_launches_l0", C.c_int32 * 11)]
+                ("kern_ms", C.c_double * 12), ("kern_bytes", C.c_double * 12),
+                ("kern_launches", C.c_int32 * 12), ("kern_ms_l0", C.c_double * 12),
+                ("kern_bytes_l0", C.c_double * 12), ("kern_launches_l0", C.c_int32 * 12)]
 
     def as_dict(self):
         n = min(self.outer_iterations, 64)

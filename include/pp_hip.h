@@ -85,6 +85,10 @@ typedef struct pp_solve_opts {
   int32_t project_rhs;       /* compatibility projection of the RHS (0/1) */
   int32_t use_graph;         /* replay the outer iteration as a hipGraph (0/1) */
   int32_t profile;           /* time every kernel launch with HIP events (0/1) */
+  int32_t krylov;            /* 0: FP64 iterative refinement around the V-cycles;
+                                m in 1..16: FP64 flexible GMRES(m) with the
+                                V-cycles as right preconditioner (pp_solve only;
+                                pp_adjoint always refines) */
 } pp_solve_opts;
 
 /* kernel classes of pp_solve_stats.kern_* (profile = 1) */
@@ -99,7 +103,8 @@ typedef struct pp_solve_opts {
 #define PP_K_GHOST 8           /* ghost-row gathers/scatters */
 #define PP_K_COARSE 9          /* coarsest-level dense solve */
 #define PP_K_OTHER 10
-#define PP_K_CLASSES 11
+#define PP_K_KRYLOV 11         /* flexible-GMRES apply / orthogonalise / update */
+#define PP_K_CLASSES 12
 
 typedef struct pp_solve_stats {
   int32_t status;
