@@ -22,6 +22,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-value", "-pthread",
 ]
+# rocSOLVER factors the coarsest-level bordered matrix at set-up
+HIP_LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def _run(cmd, cwd=None, verbose=False):
@@ -40,7 +42,7 @@ def build_hip(verbose=False):
     out = os.path.join(HERE, "libpp_hip.so")
     srcs = [os.path.join(HERE, s) for s in HIP_SOURCES if os.path.exists(os.path.join(HERE, s))]
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, f"-I{ROOT}/include", "-o", tmp, *srcs], verbose=verbose)
+    _run([HIPCC, *HIP_FLAGS, f"-I{ROOT}/include", "-o", tmp, *srcs, *HIP_LIBS], verbose=verbose)
     os.replace(tmp, out)
     return out
 

@@ -4,6 +4,8 @@
 #include <cstdio>
 #include <cstring>
 
+#include <rocblas/rocblas.h>
+
 #include "pp_kernels.h"
 #include "pp_setup.h"
 
@@ -130,17 +132,11 @@ ProlongDir<float> pdir<float>(const Level &L, int d) { return {L.par_d[d], L.oth
 template <>
 ProlongDir<double> pdir<double>(const Level &L, int d) { return {L.par_d[d], L.oth_d[d], L.wpar64[d]}; }
 template <typename T>
-const T *ghost_coef(const GhostTables &g);
+const T *ghost_cw(const GhostTables &g);
 template <>
-const float *ghost_coef<float>(const GhostTables &g) { return g.coef32; }
+const float *ghost_cw<float>(const GhostTables &g) { return g.c_w32; }
 template <>
-const double *ghost_coef<double>(const GhostTables &g) { return g.coef64; }
-template <typename T>
-T *ghost_buf(GhostTables &g);
-template <>
-float *ghost_buf<float>(GhostTables &g) { return g.buf32; }
-template <>
-double *ghost_buf<double>(GhostTables &g) { return g.buf64; }
+const double *ghost_cw<double>(const GhostTables &g) { return g.c_w64; }
 
 // algorithmic bytes per cell (see pp_kernels.h)
 double bytes_line(int s) { return 3.0 * s + 1.0; }
@@ -148,28 +144,56 @@ double bytes_line(int s) { return 3.0 * s + 1.0; }
 // ---------------------------------------------------------------------
 // line passes
 // ---------------------------------------------------------------------
+// lane-group width G and rows per lane S for a line of n <= 512 cells
+// (FP64 holds at most 8 rows per lane in registers)
+void line_shape(int n, int &G, int &S, int elem) {
+  if (elem == 8) G = n <= 256 ? 32 : 64;
+  else G = 32;
+  int need = (n + G - 1) / G;
+  static const int opts[] = {1, 2, 3, 4, 6, 8, 12, 16};
+  S = 16;
+  for (int o : opts)
+    if (o >= need) { S = o; break; }
+}
+
+template <typename Fn>
+void with_line_shape(int G, int S, Fn &&fn) {
+  using std::integral_constant;
+#define PP_CASE_S(GG)                                                   \
+  switch (S) {                                                          \
+    case 1: fn(integral_constant<int, GG>{}, integral_constant<int, 1>{}); break;   \
+    case 2: fn(integral_constant<int, GG>{}, integral_constant<int, 2>{}); break;   \
+    case 3: fn(integral_constant<int, GG>{}, integral_constant<int, 3>{}); break;   \
+    case 4: fn(integral_constant<int, GG>{}, integral_constant<int, 4>{}); break;   \
+    case 6: fn(integral_constant<int, GG>{}, integral_constant<int, 6>{}); break;   \
+    case 8: fn(integral_constant<int, GG>{}, integral_constant<int, 8>{}); break;   \
+    case 12: fn(integral_constant<int, GG>{}, integral_constant<int, 12>{}); break; \
+    default: fn(integral_constant<int, GG>{}, integral_constant<int, 16>{}); break; \
+  }
+  if (G == 16) {
+    PP_CASE_S(16)
+  } else if (G == 32) {
+    PP_CASE_S(32)
+  } else {
+    PP_CASE_S(64)
+  }
+#undef PP_CASE_S
+}
+
 template <typename T>
 void launch_xline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
   LevelArgs<T> a = largs<T>(L);
-  int nyh = (L.cy + 1) / 2;
-  int waves = nyh * L.cz;
-  dim3 grid((waves + 3) / 4), block(256);
+  int G, S;
+  line_shape(L.cx, G, S, (int)sizeof(T));
+  long lines = (long)((L.cy + 1) / 2) * L.cz;
+  dim3 grid((unsigned)((lines * G + 255) / 256)), block(256);
   T *e = lv_e<T>(L);
   const T *rhs = lv_rhs<T>(L);
-  int S = (L.cx + 63) / 64;
   hipStream_t st = c->stream;
   kbegin(c, PP_K_XLINE, lvl, 0.5 * (double)L.cx * L.cy * L.cz * bytes_line(sizeof(T)));
-  switch (S) {
-    case 1: k_xline<T, 1><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
-    case 2: k_xline<T, 2><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
-    case 3: k_xline<T, 3><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
-    case 4: k_xline<T, 4><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
-    case 5: k_xline<T, 5><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
-    case 6: k_xline<T, 6><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
-    case 7:
-    case 8: k_xline<T, 8><<<grid, block, 0, st>>>(a, rhs, e, color, omega); break;
-    default: break;  // rejected at pp_create
-  }
+  with_line_shape(G, S, [&](auto g, auto sz) {
+    k_xline<T, decltype(g)::value, decltype(sz)::value><<<grid, block, 0, st>>>(a, rhs, e, color, omega);
+  });
   kend(c);
 }
 
@@ -178,21 +202,17 @@ void launch_xline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
 // at most 12 cells per thread in registers and uses up to 32 segments
 // (32 x P threads) for long lines.
 void yz_shape(int n, int &LS, int &P, int &TX, int elem = 4) {
-  static const int opts[] = {1, 2, 4, 8, 12, 16};
-  int maxls = elem == 8 ? 12 : 16;
+  // at most 8 cells per thread (FP64 register budget; FP32 measured no
+  // faster with 12 or 16), at most 16 segments while that suffices
+  static const int opts[] = {1, 2, 4, 8};
+  int maxls = 8;
   LS = maxls;
   for (int o : opts) {
-    if (o > maxls) break;
     if ((n + o - 1) / o <= 16) { LS = o; break; }
   }
-  if ((n + LS - 1) / LS > 16 && elem == 8) {
-    for (int o : opts) {
-      if (o > maxls) break;
-      if ((n + o - 1) / o <= 32) { LS = o; break; }
-    }
-  }
   P = (n + LS - 1) / LS;
-  TX = P <= 16 ? 64 : 32;
+  TX = P <= 16 ? 64 : (P <= 32 ? 32 : 16);
+  (void)elem;
 }
 
 template <typename T, int DIR>
@@ -213,24 +233,20 @@ void launch_yzline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
     case 1: k_yzline<T, 1, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
     case 2: k_yzline<T, 2, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
     case 4: k_yzline<T, 4, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
-    case 8: k_yzline<T, 8, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
-    case 12: k_yzline<T, 12, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
-    default: k_yzline<T, 16, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
+    default: k_yzline<T, 8, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
   }
   kend(c);
 }
 
-// one Jacobi sweep of the homogeneous ghost rows on a level-0 correction
+// solved homogeneous ghost rows on a level-0 correction (one gather)
 template <typename T>
 void ghost_sweep_level0(pp_ctx *c, T *e) {
   GhostTables &g = c->gh;
   if (g.n == 0) return;
-  dim3 grid((g.n + 255) / 256), block(256);
-  kbegin(c, PP_K_GHOST, 0, g.n * (8.0 * (8 + 2 * sizeof(T)) + sizeof(T)));
-  k_ghost_gather<T, T><<<grid, block, 0, c->stream>>>(g.n, g.s_al, ghost_coef<T>(g), nullptr, e, ghost_buf<T>(g));
-  kend(c);
-  kbegin(c, PP_K_GHOST, 0, g.n * (8.0 + 2 * sizeof(T)));
-  k_ghost_scatter<T><<<grid, block, 0, c->stream>>>(g.n, g.g_al, ghost_buf<T>(g), e);
+  dim3 grid((g.n + 31) / 32), block(256);
+  double nnz = (double)g.h_c_ref.size();
+  kbegin(c, PP_K_GHOST, 0, g.n * (12.0 + sizeof(T)) + nnz * (8.0 + 2.0 * sizeof(T)));
+  k_ghost_composite<T, T><<<grid, block, 0, c->stream>>>(g.n, g.g_al, g.c_ptr, g.c_al, ghost_cw<T>(g), nullptr, e);
   kend(c);
 }
 
@@ -302,9 +318,11 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   double cells = (double)L.cx * L.cy * L.cz;
   if (l == (int)c->lv.size() - 1) {
     LevelArgs<T> a = largs<T>(L);
-    int n = c->cs.n;
-    kbegin(c, PP_K_COARSE, l, 8.0 * n * n + 2.0 * sizeof(T) * n);
-    k_coarse_solve<T><<<1, 512, sizeof(double) * n, st>>>(a, n, c->cs.G, lv_rhs<T>(L), lv_e<T>(L));
+    const CoarseSolve &cs = c->cs;
+    int n = cs.n;
+    kbegin(c, PP_K_COARSE, l, 8.0 * ((double)n * n + 2.0 * (double)n * cs.nch) + 2.0 * sizeof(T) * n);
+    k_coarse_mv<T><<<dim3((n + 255) / 256, cs.nch), 256, 0, st>>>(a, n, cs.pchunk, cs.G, lv_rhs<T>(L), cs.part);
+    k_coarse_sum<T><<<(n + 255) / 256, 256, 0, st>>>(a, n, cs.nch, cs.part, lv_e<T>(L));
     kend(c);
     return;
   }
@@ -330,13 +348,68 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   for (int s = 0; s < o->nu_post; s++) smooth<T>(c, l, om);
 }
 
+// The preconditioner: cycles_per_outer V-cycles on the level-0 rhs (e0 = 0
+// on entry).  With use_graph the launches of one application are captured
+// once into a hipGraph (per precision and cycle shape; dropped when the
+// geometry changes) and replayed.
+template <typename T>
+int precondition(pp_ctx *c, const pp_solve_opts *o, pp_solve_stats &S) {
+  bool graph = o->use_graph && !c->prof_on && !c->adjoint_mode;
+  if (!graph) {
+    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) vcycle<T>(c, 0, o);
+    S.vcycles += o->cycles_per_outer;
+    return PP_OK;
+  }
+  long key = (long)sizeof(T) | ((long)o->nu_pre << 4) | ((long)o->nu_post << 12) |
+             ((long)o->cycles_per_outer << 20) | ((long)std::lround(o->omega * 1e6) << 28);
+  if (!c->gexec || c->graph_key != key) {
+    if (c->gexec) {
+      HIPCHK(hipGraphExecDestroy(c->gexec));
+      c->gexec = nullptr;
+    }
+    double b0 = c->bytes_acc;
+    hipGraph_t gr = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) vcycle<T>(c, 0, o);
+    HIPCHK(hipStreamEndCapture(c->stream, &gr));
+    hipError_t ie = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
+    hipGraphDestroy(gr);
+    if (ie != hipSuccess) {
+      c->gexec = nullptr;
+      set_error("hipGraphInstantiate failed: %s", hipGetErrorString(ie));
+      return PP_HIP_ERROR;
+    }
+    c->graph_key = key;
+    c->graph_bytes = c->bytes_acc - b0;
+  } else {
+    c->bytes_acc += c->graph_bytes;
+  }
+  HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  S.vcycles += o->cycles_per_outer;
+  return PP_OK;
+}
+
 // ---------------------------------------------------------------------
 // FP64 helpers on the reference layout
 // ---------------------------------------------------------------------
 CellLaunch ref_launch(const pp_ctx *c) { return cell_launch(c->nxc, c->nyc, c->nzc); }
 
+// two-stage reductions of per-workgroup partials (kRedStage workgroups,
+// then one): fixed order, so deterministic
+constexpr int kRedStage = 256;
+void launch_reduce_summax(pp_ctx *c, int nblk) {
+  int chunk = (nblk + kRedStage - 1) / kRedStage;
+  k_reduce_summax_stage<<<kRedStage, 256, 0, c->stream>>>(c->d_part, nblk, chunk, c->d_part2);
+  k_reduce_summax<<<1, 256, 0, c->stream>>>(c->d_part2, kRedStage, c->d_res);
+}
+void launch_reduce_sums(pp_ctx *c, int nblk, int nq) {
+  int chunk = (nblk + kRedStage - 1) / kRedStage;
+  k_reduce_sums_stage<<<kRedStage, 256, 0, c->stream>>>(c->d_part, nblk, nq, chunk, c->d_part2);
+  k_reduce_sums<<<1, 256, 0, c->stream>>>(c->d_part2, kRedStage, nq, c->d_res);
+}
+
 int reduce_summax(pp_ctx *c, int nblk) {
-  k_reduce_summax<<<1, 256, 0, c->stream>>>(c->d_part, nblk, c->d_res);
+  launch_reduce_summax(c, nblk);
   HIPCHK(hipMemcpyAsync(c->h_res, c->d_res, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return PP_OK;
@@ -347,20 +420,38 @@ void ghost_exact_ref(pp_ctx *c, double *p, bool with_bnd) {
   GhostTables &g = c->gh;
   hipStream_t st = c->stream;
   if (g.n > 0) {
-    dim3 grid((g.n + 255) / 256), block(256);
-    for (int s = 0; s < g.kexact; s++) {
-      kbegin(c, PP_K_GHOST, 0, g.n * (8.0 * 24 + 8));
-      k_ghost_gather<double, double><<<grid, block, 0, st>>>(g.n, g.s_ref, g.coef64, with_bnd ? g.bnd : nullptr, p,
-                                                            g.buf64);
-      kend(c);
-      kbegin(c, PP_K_GHOST, 0, g.n * 24.0);
-      k_ghost_scatter<double><<<grid, block, 0, st>>>(g.n, g.g_ref, g.buf64, p);
-      kend(c);
-    }
+    dim3 grid((g.n + 31) / 32), block(256);
+    double nnz = (double)g.h_c_ref.size();
+    kbegin(c, PP_K_GHOST, 0, g.n * (12.0 + 8.0 + (with_bnd ? 8.0 : 0.0)) + nnz * 24.0);
+    k_ghost_composite<double, double><<<grid, block, 0, st>>>(g.n, g.g_ref, g.c_ptr, g.c_ref, g.c_w64,
+                                                             with_bnd ? g.bnd_eff : nullptr, p);
+    kend(c);
   }
   k_outer_ghost_x<<<dim3((c->nyc + 2 + 255) / 256, c->nzc + 2), 256, 0, st>>>(c->ref, p);
   k_outer_ghost_y<<<dim3((c->nxc + 2 + 255) / 256, c->nzc + 2), 256, 0, st>>>(c->ref, p);
   k_outer_ghost_z<<<dim3((c->nxc + 2 + 255) / 256, c->nyc + 2), 256, 0, st>>>(c->ref, p);
+}
+
+// Neumann data of the ghost rows from bc_src (host values): bnd = bself *
+// bc_src for the compatibility projection and bnd_eff = (I - C_gg)^-1 bnd
+// (kexact Jacobi sweeps, as the composite rows) for the solved ghost values
+int set_ghost_bnd(pp_ctx *c, const std::vector<double> &bc) {
+  GhostTables &g = c->gh;
+  std::vector<double> hb(g.n), x(g.n), y(g.n);
+  for (int q = 0; q < g.n; q++) hb[q] = g.h_bself[q] * bc[q];
+  x = hb;
+  for (int t = 1; t < g.kexact && !g.h_gg_idx.empty(); t++) {
+    for (int q = 0; q < g.n; q++) {
+      double s = hb[q];
+      for (int e = g.h_gg_ptr[q]; e < g.h_gg_ptr[q + 1]; e++) s += g.h_gg_w[e] * x[g.h_gg_idx[e]];
+      y[q] = s;
+    }
+    x.swap(y);
+  }
+  HIPCHK(hipMemcpyAsync(g.bnd, hb.data(), sizeof(double) * g.n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(g.bnd_eff, x.data(), sizeof(double) * g.n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PP_OK;
 }
 
 // fluid sums: out[0..4] = {sum w*b, sum w, sum b*b, sum V*p, sum V}
@@ -368,7 +459,7 @@ int fluid_sums(pp_ctx *c, const double *w, const double *b, const double *p) {
   CellLaunch cl = ref_launch(c);
   long nblk = (long)cl.grid.x * cl.grid.y * cl.grid.z;
   k_fluid_sums<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), w, b, p, c->d_part);
-  k_reduce_sums<<<1, 256, 0, c->stream>>>(c->d_part, (int)nblk, 5, c->d_res);
+  launch_reduce_sums(c, (int)nblk, 5);
   return PP_OK;
 }
 
@@ -419,7 +510,7 @@ int ensure_krylov(pp_ctx *c, int m) {
 
 // sum of nq per-block partials -> c->h_res[0..nq)
 int reduce_sums_host(pp_ctx *c, int nq) {
-  k_reduce_sums<<<1, 256, 0, c->stream>>>(c->d_part, c->nblk_red, nq, c->d_res);
+  launch_reduce_sums(c, c->nblk_red, nq);
   HIPCHK(hipMemcpyAsync(c->h_res, c->d_res, sizeof(double) * nq, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return PP_OK;
@@ -451,10 +542,8 @@ int krylov_cycle(pp_ctx *c, const pp_solve_opts *o, double *p, bool have_bnd, do
   gv[0] = beta;
   int k = 0;
   for (int j = 0; j < m && outer < o->max_outer; j++) {
-    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
-      vcycle<T>(c, 0, o);
-      S.vcycles++;
-    }
+    int rc = precondition<T>(c, o, S);
+    if (rc) return rc;
     kbegin(c, PP_K_KRYLOV, 0, cells * (sizeof(T) + 1.0 + 8.0));
     k_kry_z<T><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, lv_e<T>(L0), Z[j]);
     kend(c);
@@ -465,7 +554,7 @@ int krylov_cycle(pp_ctx *c, const pp_solve_opts *o, double *p, bool have_bnd, do
     kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1)));
     k_kry_apply<256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, Z[j], U[j + 1], V, c->d_part);
     kend(c);
-    int rc = reduce_sums_host(c, j + 1);
+    rc = reduce_sums_host(c, j + 1);
     if (rc) return rc;
     for (int q = 0; q <= j; q++) {
       double d = c->h_res[q];
@@ -557,11 +646,10 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
   bool have_bnd = bc_src != nullptr && g.n > 0;
   if (g.n > 0) {
     if (have_bnd) {
-      // bnd[n] = bself[n] * bc_src[n]
-      std::vector<double> hb(g.n), hs(g.n);
+      std::vector<double> hs(g.n);
       HIPCHK(hipMemcpy(hs.data(), bc_src, sizeof(double) * g.n, hipMemcpyDeviceToHost));
-      for (int q = 0; q < g.n; q++) hb[q] = g.h_bself[q] * hs[q];
-      HIPCHK(hipMemcpyAsync(g.bnd, hb.data(), sizeof(double) * g.n, hipMemcpyHostToDevice, s));
+      int rc = set_ghost_bnd(c, hs);
+      if (rc) return rc;
     }
   }
   // b = rhs on fluid cells
@@ -645,10 +733,8 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
       if (krc) return krc;
       continue;
     }
-    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
-      vcycle<T>(c, 0, o);
-      S.vcycles++;
-    }
+    int prc = precondition<T>(c, o, S);
+    if (prc) return prc;
     // p += e on fluid cells, ghost rows exact
     kbegin(c, PP_K_UPDATE, 0, cells * (17.0 + sizeof(T)));
     k_update<T><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, p, lv_e<T>(L0));
@@ -857,7 +943,7 @@ void pp_default_opts(pp_solve_opts *o) {
   o->project_rhs = 1;
   o->use_graph = 0;
   o->profile = 0;
-  o->krylov = 4;
+  o->krylov = 8;
 }
 
 const char *pp_last_error(void) { return pp::last_error(); }
@@ -869,8 +955,9 @@ int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
     set_error("pp_create: grid must have at least 4 cells per direction and node arrays");
     return PP_BAD_ARG;
   }
-  if (g->nxc > 512) {
-    set_error("pp_create: nxc=%d exceeds the 512 cells per x-line the wavefront line solver holds", g->nxc);
+  if (g->nxc > 512 || g->nyc > 512 || g->nzc > 512) {
+    set_error("pp_create: %dx%dx%d cells; the line solvers hold at most 512 cells per line", g->nxc, g->nyc,
+              g->nzc);
     return PP_BAD_ARG;
   }
   for (int f = 0; f < 6; f++)
@@ -918,6 +1005,7 @@ int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
   c->nblk_red = (int)((long)cl.grid.x * cl.grid.y * cl.grid.z);
   // partial sums: up to KRY_MAX+1 per block (Krylov dots), 5 (fluid sums)
   HIPCHK(hipMalloc(&c->d_part, sizeof(double) * (KRY_MAX + 1) * (size_t)c->nblk_red + 64));
+  HIPCHK(hipMalloc(&c->d_part2, sizeof(double) * (KRY_MAX + 1) * kRedStage));
   HIPCHK(hipMalloc(&c->d_res, sizeof(double) * 32));
   HIPCHK(hipHostMalloc(&c->h_res, sizeof(double) * 32));
   *out = c;
@@ -932,10 +1020,11 @@ int32_t pp_destroy(pp_ctx *c) {
   free_ghosts(c);
   hipFree(c->d_p); hipFree(c->d_b); hipFree(c->d_w); hipFree(c->d_bwork); hipFree(c->d_tmp);
   hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_mu); hipFree(c->d_sv);
-  hipFree(c->d_part); hipFree(c->d_res);
+  hipFree(c->d_part); hipFree(c->d_part2); hipFree(c->d_res);
   for (double *q : c->d_kry) hipFree(q);
   if (c->h_res) hipHostFree(c->h_res);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->blas) rocblas_destroy_handle((rocblas_handle)c->blas);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return PP_OK;
@@ -950,6 +1039,11 @@ int32_t pp_set_geometry(pp_ctx *c, const pp_geometry *geo) {
   }
   HIPCHK(hipSetDevice(c->device));
   c->have_geom = false;
+  if (c->gexec) {  // captured launches point at the old hierarchy
+    HIPCHK(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+    c->graph_key = -1;
+  }
   c->h_iblank.assign(geo->iblank, geo->iblank + c->nref);
   for (long r = 0; r < c->nref; r++)
     if (c->h_iblank[r] != 0 && c->h_iblank[r] != 1) {
@@ -1075,9 +1169,8 @@ int32_t pp_ghost_rows(pp_ctx *c, double *p, const double *bc) {
   GhostTables &g = c->gh;
   bool have = bc && g.n > 0;
   if (have) {
-    std::vector<double> hb(g.n);
-    for (int q = 0; q < g.n; q++) hb[q] = g.h_bself[q] * bc[q];
-    HIPCHK(hipMemcpy(g.bnd, hb.data(), sizeof(double) * g.n, hipMemcpyHostToDevice));
+    int rc = set_ghost_bnd(c, std::vector<double>(bc, bc + g.n));
+    if (rc) return rc;
   }
   ghost_exact_ref(c, c->d_p, have);
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1142,17 +1235,12 @@ int32_t pp_tdma_batched(pp_ctx *c, const double *a, const double *b, const doubl
   }
   hipStream_t s = c->stream;
   if (method == 0) {
-    int S = (n + 63) / 64;
-    dim3 grid((nl + 3) / 4), block(256);
-    switch (S) {
-      case 1: k_tdma_wave<1><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
-      case 2: k_tdma_wave<2><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
-      case 3: k_tdma_wave<3><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
-      case 4: k_tdma_wave<4><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
-      case 5: k_tdma_wave<5><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
-      case 6: k_tdma_wave<6><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
-      default: k_tdma_wave<8><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
-    }
+    int G, S;
+    line_shape(n, G, S, 8);
+    dim3 grid((unsigned)(((long)nl * G + 255) / 256)), block(256);
+    with_line_shape(G, S, [&](auto g, auto sz) {
+      k_tdma_wave<decltype(g)::value, decltype(sz)::value><<<grid, block, 0, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]);
+    });
   } else {
     int LS, P, TX;
     yz_shape(n, LS, P, TX, 8);
@@ -1162,9 +1250,7 @@ int32_t pp_tdma_batched(pp_ctx *c, const double *a, const double *b, const doubl
       case 1: k_tdma_seg<1><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
       case 2: k_tdma_seg<2><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
       case 4: k_tdma_seg<4><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
-      case 8: k_tdma_seg<8><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
-      case 12: k_tdma_seg<12><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
-      default: k_tdma_seg<16><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
+      default: k_tdma_seg<8><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
     }
   }
   HIPCHK(hipGetLastError());

@@ -83,13 +83,18 @@ struct GhostTables {
   int kexact = 1;                 // Jacobi sweeps that solve the ghost rows
   // forward gather (device)
   long *g_ref = nullptr, *g_al = nullptr;       // ghost cell offsets
-  long *s_ref = nullptr, *s_al = nullptr;       // 8 stencil offsets per ghost
-  double *coef64 = nullptr;                     // 8 per ghost, self entry 0
-  float *coef32 = nullptr;
   double *bself = nullptr;                      // c_self + probeLength
   double *bnd = nullptr;                        // bself * uNormal (per solve)
-  double *buf64 = nullptr;
-  float *buf32 = nullptr;
+  // composite ghost rows: the solved ghost value as a combination of fluid
+  // values, g = sum_q w_q p[f_q] + bnd_eff (ghost-ghost coupling folded in)
+  int *c_ptr = nullptr;                         // CSR by ghost (n+1)
+  long *c_ref = nullptr, *c_al = nullptr;       // fluid cell offsets
+  double *c_w64 = nullptr;
+  float *c_w32 = nullptr;
+  double *bnd_eff = nullptr;                    // (I - C_gg)^-1 bnd (per solve)
+  std::vector<int> h_c_ptr, h_gg_ptr, h_gg_idx;  // h_gg: ghost-ghost coupling CSR
+  std::vector<long> h_c_ref, h_c_al;
+  std::vector<double> h_c_w, h_gg_w;
   // adjoint: fluid neighbours of each ghost and their face conductances
   // (coefficient of the ghost in the neighbour's row times its volume)
   long *nb_ref = nullptr, *nb_al = nullptr;     // 6 per ghost (-1: none)
@@ -118,7 +123,9 @@ struct GhostTables {
 
 struct CoarseSolve {
   int n = 0;                      // cells of the coarsest level
-  double *G = nullptr;            // n*n, G[j*n+i] = d e_i / d r_j
+  double *G = nullptr;            // n*n, G[p*n+q] = d e_q / d r_p
+  int pchunk = 64, nch = 0;       // rhs cells per partial product, partials per cell
+  double *part = nullptr;         // nch*n partial products
 };
 
 }  // namespace pp
@@ -136,6 +143,7 @@ struct pp_ctx {
   std::vector<int8_t> h_iblank, h_ghost, h_fresh;  // reference layout
   pp::GhostTables gh;
   pp::CoarseSolve cs;
+  void *blas = nullptr;                // rocblas_handle of the coarse factorisation
   // FP64 work arrays in reference layout
   double *d_p = nullptr, *d_b = nullptr, *d_w = nullptr, *d_bwork = nullptr;
   double *d_tmp = nullptr;
@@ -143,6 +151,7 @@ struct pp_ctx {
   int8_t *d_iblank_ref = nullptr;
   // reductions
   double *d_part = nullptr;            // partial sums
+  double *d_part2 = nullptr;           // first-stage results of the two-stage reductions
   double *d_res = nullptr;             // final reduction results
   double *h_res = nullptr;             // pinned host mirror
   int nblk_red = 0;
@@ -154,8 +163,9 @@ struct pp_ctx {
   double *d_mu = nullptr;              // adjoint unknown lambda/V (reference layout)
   double *d_sv = nullptr;              // adjoint rhs s/V (reference layout)
   // graph cache
-  hipGraphExec_t gexec = nullptr;
-  int graph_key = -1;
+  hipGraphExec_t gexec = nullptr;        // captured preconditioner (use_graph)
+  long graph_key = -1;
+  double graph_bytes = 0.0;
   double bytes_acc = 0;
   // opt-in per-launch timing (pp_solve_opts.profile)
   bool prof_on = false;

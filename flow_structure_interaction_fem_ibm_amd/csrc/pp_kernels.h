@@ -11,6 +11,8 @@
 //   FP64 update         8+s+1+8  (p, e, flags in; p out)
 #pragma once
 
+#include <type_traits>
+
 #include "pp_internal.h"
 
 namespace pp {
@@ -104,6 +106,38 @@ __global__ void k_reduce_summax(const double *__restrict__ part, int nblk, doubl
   // per-thread partials combined in thread order -> deterministic
   v = block_summax<256>(v);
   if (threadIdx.x == 0) { out[0] = v.sum; out[1] = v.mx; out[2] = (double)v.arg; }
+}
+
+// first stage of a two-stage reduction of partials: workgroup b combines
+// partials [b*chunk, (b+1)*chunk) into out[3*b..3*b+2]
+__global__ void k_reduce_summax_stage(const double *__restrict__ part, int nblk, int chunk, double *__restrict__ out) {
+  SumMax v{0.0, -1.0, -1};
+  int b0 = blockIdx.x * chunk, b1 = min(nblk, b0 + chunk);
+  for (int b = b0 + threadIdx.x; b < b1; b += 256) {
+    SumMax o{part[3 * b], part[3 * b + 1], (long)part[3 * b + 2]};
+    v = summax_combine(v, o);
+  }
+  v = block_summax<256>(v);
+  if (threadIdx.x == 0) { out[3 * blockIdx.x] = v.sum; out[3 * blockIdx.x + 1] = v.mx; out[3 * blockIdx.x + 2] = (double)v.arg; }
+}
+
+// first stage for nq independent sums: out[b*nq + q]
+__global__ void k_reduce_sums_stage(const double *__restrict__ part, int nblk, int nq, int chunk,
+                                    double *__restrict__ out) {
+  __shared__ double s[256];
+  int b0 = blockIdx.x * chunk, b1 = min(nblk, b0 + chunk);
+  for (int q = 0; q < nq; q++) {
+    double v = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += 256) v += part[(long)b * nq + q];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 128; d >= 1; d >>= 1) {
+      if (threadIdx.x < d) s[threadIdx.x] += s[threadIdx.x + d];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(long)blockIdx.x * nq + q] = s[0];
+    __syncthreads();
+  }
 }
 
 // multiple independent sums: part[blk*nq + q]
@@ -393,24 +427,29 @@ __global__ void __launch_bounds__(256) k_gather_idx(int n, const long *__restric
 }
 
 // ---------------------------------------------------------------------
-template <typename T, typename TC>
-__global__ void __launch_bounds__(256) k_ghost_gather(int n, const long *__restrict__ sidx, const TC *__restrict__ coef,
-                                                      const double *__restrict__ bnd, const T *__restrict__ p,
-                                                      T *__restrict__ buf) {
-  int g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= n) return;
-  TC s = bnd ? (TC)bnd[g] : TC(0);
-#pragma unroll
-  for (int r = 0; r < 8; r++) s += coef[8 * g + r] * (TC)p[sidx[8 * g + r]];
-  buf[g] = (T)s;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_ghost_scatter(int n, const long *__restrict__ gidx, const T *__restrict__ buf,
-                                                       T *__restrict__ p) {
-  int g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= n) return;
-  p[gidx[g]] = buf[g];
+// solved ghost rows in one gather: p[g] = sum_q w_q p[f_q] (+ beff[g]);
+// reads fluid cells only, writes ghost cells only.  8 lanes per ghost
+// split its row (fixed split and reduction order: deterministic).
+template <typename T, typename TW>
+__global__ void __launch_bounds__(256) k_ghost_composite(int n, const long *__restrict__ gidx,
+                                                         const int *__restrict__ ptr, const long *__restrict__ fidx,
+                                                         const TW *__restrict__ w, const double *__restrict__ beff,
+                                                         T *__restrict__ p) {
+  int t = blockIdx.x * 256 + threadIdx.x;
+  int g = t >> 3, sub = t & 7;
+  bool ok = g < n;
+  TW s = TW(0);
+  if (ok) {
+    int q1 = ptr[g + 1];
+    for (int q = ptr[g] + sub; q < q1; q += 8) s += w[q] * (TW)p[fidx[q]];
+  }
+  s += __shfl_down(s, 4, 8);
+  s += __shfl_down(s, 2, 8);
+  s += __shfl_down(s, 1, 8);
+  if (ok && sub == 0) {
+    if (beff) s += (TW)beff[g];
+    p[gidx[g]] = (T)s;
+  }
 }
 
 // outer ghost layer, Neumann copy (set_outer_ghost_pres,
@@ -539,136 +578,256 @@ __global__ void __launch_bounds__(256) k_prolong(LevelArgs<T> F, LevelArgs<T> C,
 }
 
 // ---------------------------------------------------------------------
-// x-line zebra pass (lines along the contiguous index): one wavefront per
-// line (j,k) with (j+k)%2 == color.  Lane l owns S consecutive cells; each
-// lane eliminates its segment down to one reduced unknown (its last cell),
-// the 64 reduced equations are solved by parallel cyclic reduction across
-// the wavefront with lane shuffles, and each lane back-substitutes.
+// reciprocals: v_rcp_f32 (1 ulp) for FP32, v_rcp_f64 refined by two
+// Newton steps for FP64 (the line solves are smoothers inside an FP64
+// outer iteration; every division of their recurrences goes through here)
 // ---------------------------------------------------------------------
-template <typename T>
-struct Tri {
-  T a, b, c, d;
+__device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double fast_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// ---------------------------------------------------------------------
+// lane exchange inside a group of G consecutive lanes of a wavefront
+// (G = 16: DPP row shifts, one VALU op; otherwise ds_bpermute).
+//   from_lower<d>(v, edge): value of lane l-d of the group, `edge` for the
+//   first d lanes; from_upper<d>: lane l+d, `edge` for the last d lanes.
+// ---------------------------------------------------------------------
+template <int CTRL>
+__device__ inline float dpp_move(float v, float edge) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ inline double dpp_move(double v, double edge) {
+  long long vb = __double_as_longlong(v), eb = __double_as_longlong(edge);
+  int lo = __builtin_amdgcn_update_dpp((int)(eb & 0xffffffff), (int)(vb & 0xffffffff), CTRL, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp((int)(eb >> 32), (int)(vb >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int G, int D, typename T>
+__device__ inline T from_lower(T v, T edge) {
+  if constexpr (G == 16) {
+    return dpp_move<0x110 + D>(v, edge);  // row_shr:D
+  } else {
+    T r = __shfl_up(v, D, G);
+    return ((threadIdx.x & (G - 1)) < D) ? edge : r;
+  }
+}
+template <int G, int D, typename T>
+__device__ inline T from_upper(T v, T edge) {
+  if constexpr (G == 16) {
+    return dpp_move<0x100 + D>(v, edge);  // row_shl:D
+  } else {
+    T r = __shfl_down(v, D, G);
+    return ((threadIdx.x & (G - 1)) + D >= G) ? edge : r;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Tridiagonal line solve held by a group of G lanes, S consecutive rows
+// per lane (row q = lane_in_group*S + s):  a_q x_{q-1} + b_q x_q + c_q x_{q+1} = d_q.
+// Each lane eliminates its segment while the rows arrive (push), which
+// expresses x_m = D_m + F_m XL + G_m XS through the previous segment's last
+// unknown XL and its own last unknown XS; the G reduced equations are
+// solved by parallel cyclic reduction across the group, then every lane
+// back-substitutes.  Rows past the end of the line are identity rows.
+// ---------------------------------------------------------------------
+template <typename T, int S>
+struct SegElim {
+  T cp[S], dp[S], fp[S];
+  __device__ inline void push(int m, T a, T b, T c, T d) {
+    if (m == 0) {
+      T inv = fast_rcp(b);
+      cp[0] = c * inv;
+      dp[0] = d * inv;
+      fp[0] = -a * inv;
+    } else {
+      T inv = fast_rcp(b - a * cp[m - 1]);
+      cp[m] = c * inv;
+      dp[m] = (d - a * dp[m - 1]) * inv;
+      fp[m] = -a * fp[m - 1] * inv;
+    }
+  }
+  // after the last push: back recurrences in place (dp -> D, fp -> F,
+  // cp -> G with x_m = D_m + F_m XL + G_m XS); returns the last row's
+  // forward coefficients
+  __device__ inline void back(T &cl, T &dl, T &fl) {
+    cl = cp[S - 1]; dl = dp[S - 1]; fl = fp[S - 1];
+    T Dn = T(0), Fn = T(0), Gn = T(1);
+#pragma unroll
+    for (int m = S - 2; m >= 0; m--) {
+      T c = cp[m];
+      Dn = dp[m] - c * Dn;
+      Fn = fp[m] - c * Fn;
+      Gn = -c * Gn;
+      dp[m] = Dn; fp[m] = Fn; cp[m] = Gn;
+    }
+    dp[S - 1] = T(0); fp[S - 1] = T(0); cp[S - 1] = T(1);
+  }
+  // x_m into dp[m] from the neighbouring reduced unknowns
+  __device__ inline void combine(T XL, T XS) {
+#pragma unroll
+    for (int m = 0; m < S - 1; m++) dp[m] = dp[m] + fp[m] * XL + cp[m] * XS;
+    dp[S - 1] = XS;
+  }
+  // the whole line inside one group of G lanes: on return dp[m] holds x_m
+  template <int G>
+  __device__ inline void solve() {
+    T cl, dl, fl;
+    back(cl, dl, fl);
+    T D0n = from_upper<G, 1>(dp[0], T(0)), F0n = from_upper<G, 1>(fp[0], T(0)),
+      G0n = from_upper<G, 1>(cp[0], T(0));
+    T A = -fl, B = T(1) + cl * F0n, C = cl * G0n, R = dl - cl * D0n;
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+      T Am, Bm, Cm, Rm, Ap, Bp, Cp, Rp;
+      switch (d) {
+#define PP_PCR_STEP(DD)                                                                         \
+  case DD:                                                                                      \
+    Am = from_lower<G, DD>(A, T(0)); Bm = from_lower<G, DD>(B, T(1));                           \
+    Cm = from_lower<G, DD>(C, T(0)); Rm = from_lower<G, DD>(R, T(0));                           \
+    Ap = from_upper<G, DD>(A, T(0)); Bp = from_upper<G, DD>(B, T(1));                           \
+    Cp = from_upper<G, DD>(C, T(0)); Rp = from_upper<G, DD>(R, T(0));                           \
+    break;
+        PP_PCR_STEP(1) PP_PCR_STEP(2) PP_PCR_STEP(4) PP_PCR_STEP(8) PP_PCR_STEP(16) PP_PCR_STEP(32)
+#undef PP_PCR_STEP
+        default: Am = Cm = Rm = Ap = Cp = Rp = T(0); Bm = Bp = T(1); break;
+      }
+      T k1 = A * fast_rcp(Bm), k2 = C * fast_rcp(Bp);
+      T nA = -Am * k1, nC = -Cp * k2;
+      T nB = B - Cm * k1 - Ap * k2, nR = R - Rm * k1 - Rp * k2;
+      A = nA; B = nB; C = nC; R = nR;
+    }
+    T X = R * fast_rcp(B);
+    combine(from_lower<G, 1>(X, T(0)), X);
+  }
 };
 
-// Local elimination of a segment of S rows (x_m in terms of the previous
-// segment's last unknown XL and this segment's last unknown XS):
-//   x_m = D_m + F_m XL + G_m XS   (m < S-1),   x_{S-1} = XS.
-// Returns the reduced equation pieces (row S-1 after forward elimination).
+// chunk loads/stores of S consecutive elements (vectorised when the chunk
+// is whole and 16-byte aligned)
 template <typename T, int S>
-__device__ inline void seg_eliminate(const T (&a)[S], const T (&b)[S], const T (&c)[S], const T (&d)[S], T (&D)[S],
-                                     T (&F)[S], T (&G)[S], T &cl, T &dl, T &fl) {
-  T cp[S], dp[S], fp[S];
-  {
-    T inv = T(1) / b[0];
-    cp[0] = c[0] * inv;
-    dp[0] = d[0] * inv;
-    fp[0] = -a[0] * inv;
-  }
+__device__ inline void load_chunk(const T *__restrict__ p, int nvalid, T (&v)[S]) {
+  constexpr int VE = 16 / sizeof(T);
+  if (S % VE == 0 && nvalid == S) {
+    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
 #pragma unroll
-  for (int m = 1; m < S; m++) {
-    T inv = T(1) / (b[m] - a[m] * cp[m - 1]);
-    cp[m] = c[m] * inv;
-    dp[m] = (d[m] - a[m] * dp[m - 1]) * inv;
-    fp[m] = -a[m] * fp[m - 1] * inv;
-  }
-  D[S - 1] = T(0);
-  F[S - 1] = T(0);
-  G[S - 1] = T(1);
+    for (int q = 0; q < S / VE; q++) {
+      V w = reinterpret_cast<const V *>(p)[q];
+      if constexpr (sizeof(T) == 4) {
+        v[4 * q] = w.x; v[4 * q + 1] = w.y; v[4 * q + 2] = w.z; v[4 * q + 3] = w.w;
+      } else {
+        v[2 * q] = w.x; v[2 * q + 1] = w.y;
+      }
+    }
+  } else {
 #pragma unroll
-  for (int m = S - 2; m >= 0; m--) {
-    D[m] = dp[m] - cp[m] * D[m + 1];
-    F[m] = fp[m] - cp[m] * F[m + 1];
-    G[m] = -cp[m] * G[m + 1];
+    for (int s = 0; s < S; s++) v[s] = s < nvalid ? p[s] : T(0);
   }
-  cl = cp[S - 1];
-  dl = dp[S - 1];
-  fl = fp[S - 1];
+}
+template <typename T, int S>
+__device__ inline void store_chunk(T *__restrict__ p, int nvalid, const T (&v)[S]) {
+  constexpr int VE = 16 / sizeof(T);
+  if (S % VE == 0 && nvalid == S) {
+    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+#pragma unroll
+    for (int q = 0; q < S / VE; q++) {
+      V w;
+      if constexpr (sizeof(T) == 4) {
+        w.x = v[4 * q]; w.y = v[4 * q + 1]; w.z = v[4 * q + 2]; w.w = v[4 * q + 3];
+      } else {
+        w.x = v[2 * q]; w.y = v[2 * q + 1];
+      }
+      reinterpret_cast<V *>(p)[q] = w;
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; s++)
+      if (s < nvalid) p[s] = v[s];
+  }
+}
+template <int S>
+__device__ inline void load_flags(const uint8_t *__restrict__ p, int nvalid, uint8_t (&f)[S]) {
+  if (S % 4 == 0 && nvalid == S) {
+#pragma unroll
+    for (int q = 0; q < S / 4; q++) {
+      uint32_t w = reinterpret_cast<const uint32_t *>(p)[q];
+      f[4 * q] = w & 0xff; f[4 * q + 1] = (w >> 8) & 0xff; f[4 * q + 2] = (w >> 16) & 0xff; f[4 * q + 3] = w >> 24;
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; s++) f[s] = s < nvalid ? p[s] : uint8_t(FL_BLANK);
+  }
 }
 
-// PCR over the 64 lanes of a wavefront for one reduced equation per lane:
-// A X_{l-1} + B X_l + C X_{l+1} = R.  Returns X_l.
-template <typename T>
-__device__ inline T wave_pcr(T A, T B, T C, T R) {
-  int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    T Am = shfl_up_d(A, d), Bm = shfl_up_d(B, d), Cm = shfl_up_d(C, d), Rm = shfl_up_d(R, d);
-    T Ap = shfl_down_d(A, d), Bp = shfl_down_d(B, d), Cp = shfl_down_d(C, d), Rp = shfl_down_d(R, d);
-    if (lane < d) { Am = T(0); Bm = T(1); Cm = T(0); Rm = T(0); }
-    if (lane + d >= 64) { Ap = T(0); Bp = T(1); Cp = T(0); Rp = T(0); }
-    T k1 = A / Bm, k2 = C / Bp;
-    T nA = -Am * k1;
-    T nC = -Cp * k2;
-    T nB = B - Cm * k1 - Ap * k2;
-    T nR = R - Rm * k1 - Rp * k2;
-    A = nA; B = nB; C = nC; R = nR;
-  }
-  return R / B;
+// Workgroup index remapped so that each XCD (workgroups are dispatched to
+// the 8 XCDs round-robin) runs one contiguous range of the logical order:
+// neighbouring lines then share an L2.
+__device__ inline int xcd_block(int b, int nb) {
+  int q = nb >> 3, rem = nb & 7, x = b & 7;
+  return x * q + (x < rem ? x : rem) + (b >> 3);
 }
 
-// Solve one tridiagonal line held as S rows per lane across a wavefront.
-template <typename T, int S>
-__device__ inline void wave_line_solve(const T (&a)[S], const T (&b)[S], const T (&c)[S], const T (&d)[S],
-                                       T (&x)[S]) {
-  T D[S], F[S], G[S], cl, dl, fl;
-  seg_eliminate<T, S>(a, b, c, d, D, F, G, cl, dl, fl);
-  T D0n = shfl_down_d(D[0], 1), F0n = shfl_down_d(F[0], 1), G0n = shfl_down_d(G[0], 1);
-  int lane = threadIdx.x & 63;
-  if (lane == 63) { D0n = T(0); F0n = T(0); G0n = T(0); }
-  T A = -fl;
-  T B = T(1) + cl * F0n;
-  T C = cl * G0n;
-  T R = dl - cl * D0n;
-  T X = wave_pcr<T>(A, B, C, R);
-  T XL = shfl_up_d(X, 1);
-  if (lane == 0) XL = T(0);
-#pragma unroll
-  for (int m = 0; m < S - 1; m++) x[m] = D[m] + F[m] * XL + G[m] * X;
-  x[S - 1] = X;
-}
-
-template <typename T, int S>
+// ---------------------------------------------------------------------
+// x-line zebra pass (lines along the contiguous index): lines (j,k) with
+// (j+k)%2 == color; a group of G lanes holds one line, S cells per lane.
+// ---------------------------------------------------------------------
+template <typename T, int G, int S>
 __global__ void __launch_bounds__(256) k_xline(LevelArgs<T> L, const T *__restrict__ rhs, T *__restrict__ e,
                                                int color, T omega) {
-  int lane = threadIdx.x & 63;
-  int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-  int nyh = (L.cy + 1) >> 1;
-  int k = wave / nyh;
-  int jj = wave - k * nyh;
-  if (k >= L.cz) return;  // uniform per wavefront
-  int j = 2 * jj + ((k + color) & 1);
-  if (j >= L.cy) return;  // uniform per wavefront
-  long row = lidx(L.sx, L.sxy, 0, j, k);
-  T a[S], b[S], c[S], d[S], eo[S], x[S];
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int line = (blk * 256 + (int)threadIdx.x) / G;
+  const int sub = threadIdx.x & (G - 1);
+  const int nyh = (L.cy + 1) >> 1;
+  const int k = line / nyh;
+  const int j = 2 * (line - k * nyh) + ((k + color) & 1);
+  const bool valid = k < L.cz && j < L.cy;
+  const int i0 = sub * S;
+  const int nv = valid ? max(0, min(S, L.cx - i0)) : 0;
+  const long base = valid ? lidx(L.sx, L.sxy, i0, j, k) : 0;
+  T eo[S], ejm[S], ejp[S], ekm[S], ekp[S], rv[S], cmx[S], cpx[S];
+  uint8_t fl[S];
+  load_chunk<T, S>(e + base, nv, eo);
+  load_chunk<T, S>(e + base - L.sx, nv, ejm);
+  load_chunk<T, S>(e + base + L.sx, nv, ejp);
+  load_chunk<T, S>(e + base - L.sxy, nv, ekm);
+  load_chunk<T, S>(e + base + L.sxy, nv, ekp);
+  load_chunk<T, S>(rhs + base, nv, rv);
+  load_chunk<T, S>(L.cmx + i0, nv, cmx);
+  load_chunk<T, S>(L.cpx + i0, nv, cpx);
+  load_flags<S>(L.fl + base, nv, fl);
+  T my = valid ? L.cmy[j] : T(0), py = valid ? L.cpy[j] : T(0);
+  T mz = valid ? L.cmz[k] : T(0), pz = valid ? L.cpz[k] : T(0);
+  SegElim<T, S> se;
 #pragma unroll
   for (int s = 0; s < S; s++) {
-    int i = lane * S + s;
-    a[s] = T(0); b[s] = T(1); c[s] = T(0); d[s] = T(0); eo[s] = T(0);
-    if (i < L.cx) {
-      long id = row + i;
-      uint8_t f = L.fl[id];
-      T ev = e[id];
-      eo[s] = ev;
+    uint8_t f = fl[s];
+    T a = T(0), b = T(1), c = T(0), d = T(0);
+    if (s < nv) {
       if (f & FL_BLANK) {
-        d[s] = (f & FL_GHOST) ? ev : T(0);
+        d = (f & FL_GHOST) ? eo[s] : T(0);
       } else {
-        Row<T> w = make_row(L, f, i, j, k);
-        T r = rhs[id] - e[id - L.sx] * w.amy - e[id + L.sx] * w.apy - e[id - L.sxy] * w.amz -
-              e[id + L.sxy] * w.apz;
-        if (tabs(w.diag) < T(1.0e-12)) {
-          d[s] = ev;
+        T amx = (f & FL_IM) ? T(0) : cmx[s], apx = (f & FL_IP) ? T(0) : cpx[s];
+        T amy = (f & FL_JM) ? T(0) : my, apy = (f & FL_JP) ? T(0) : py;
+        T amz = (f & FL_KM) ? T(0) : mz, apz = (f & FL_KP) ? T(0) : pz;
+        T diag = (-(amx + apx) + -(amy + apy)) + -(amz + apz);
+        T r = rv[s] - ejm[s] * amy - ejp[s] * apy - ekm[s] * amz - ekp[s] * apz;
+        if (tabs(diag) < T(1.0e-12)) {
+          d = eo[s];
         } else {
-          a[s] = w.amx; b[s] = w.diag; c[s] = w.apx; d[s] = r;
+          a = amx; b = diag; c = apx; d = r;
         }
       }
     }
+    se.push(s, a, b, c, d);
   }
-  wave_line_solve<T, S>(a, b, c, d, x);
+  se.template solve<G>();
 #pragma unroll
-  for (int s = 0; s < S; s++) {
-    int i = lane * S + s;
-    if (i < L.cx) e[row + i] = (T(1) - omega) * eo[s] + omega * x[s];
-  }
+  for (int s = 0; s < S; s++) eo[s] = (T(1) - omega) * eo[s] + omega * se.dp[s];
+  store_chunk<T, S>(e + base, nv, eo);
 }
 
 // ---------------------------------------------------------------------
@@ -690,63 +849,63 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
   T *sh = reinterpret_cast<T *>(smem_raw);
   const int P = blockDim.y;
   const int tx = threadIdx.x, seg = threadIdx.y;
-  const int plane = blockIdx.y;
-  const int i = 2 * (blockIdx.x * blockDim.x + tx) + ((plane + color) & 1);
-  int j, k;
-  int n;
-  long stride;
-  if (DIR == 1) {
-    k = plane;
-    j = 0;
-    n = Lv.cy;
-    stride = Lv.sx;
-  } else {
-    j = plane;
-    k = 0;
-    n = Lv.cz;
-    stride = Lv.sxy;
-  }
-  const bool plane_ok = (DIR == 1) ? (k < Lv.cz) : (j < Lv.cy);
-  const bool ok = plane_ok && i < Lv.cx;
-  T a[LS], b[LS], c[LS], d[LS];
+  const int lb = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int tile = lb % gridDim.x, plane = lb / gridDim.x;
+  const int i = 2 * (tile * blockDim.x + tx) + ((plane + color) & 1);
+  const int n = DIR == 1 ? Lv.cy : Lv.cz;
+  const bool ok = ((DIR == 1) ? (plane < Lv.cz) : (plane < Lv.cy)) && i < Lv.cx;
   const int m0 = seg * LS;
+  // coefficients constant along the line
+  T mx = T(0), px = T(0), mo = T(0), po = T(0);
+  if (ok) {
+    mx = Lv.cmx[i]; px = Lv.cpx[i];
+    mo = DIR == 1 ? Lv.cmz[plane] : Lv.cmy[plane];
+    po = DIR == 1 ? Lv.cpz[plane] : Lv.cpy[plane];
+  }
+  const long ostride = DIR == 1 ? Lv.sxy : (long)Lv.sx;   // the other transverse direction
+  const long lstride = DIR == 1 ? (long)Lv.sx : Lv.sxy;   // along the line
+  const uint8_t FO_M = DIR == 1 ? FL_KM : FL_JM, FO_P = DIR == 1 ? FL_KP : FL_JP;
+  const uint8_t FL_M = DIR == 1 ? FL_JM : FL_KM, FL_P = DIR == 1 ? FL_JP : FL_KP;
+  const T *cml = DIR == 1 ? Lv.cmy : Lv.cmz;
+  const T *cpl = DIR == 1 ? Lv.cpy : Lv.cpz;
+  long id0 = 0;
+  if (ok) id0 = DIR == 1 ? lidx(Lv.sx, Lv.sxy, i, m0, plane) : lidx(Lv.sx, Lv.sxy, i, plane, m0);
+  T eo[LS];
+  SegElim<T, LS> se;
 #pragma unroll
   for (int m = 0; m < LS; m++) {
-    a[m] = T(0); b[m] = T(1); c[m] = T(0); d[m] = T(0);
+    T a = T(0), b = T(1), c = T(0), d = T(0);
+    eo[m] = T(0);
     int q = m0 + m;
     if (ok && q < n) {
-      int jj = (DIR == 1) ? q : j;
-      int kk = (DIR == 1) ? k : q;
-      long id = lidx(Lv.sx, Lv.sxy, i, jj, kk);
+      long id = id0 + m * lstride;
       uint8_t f = Lv.fl[id];
       T ev = e[id];
+      eo[m] = ev;
       if (f & FL_BLANK) {
-        d[m] = (f & FL_GHOST) ? ev : T(0);
+        d = (f & FL_GHOST) ? ev : T(0);
       } else {
-        Row<T> w = make_row(Lv, f, i, jj, kk);
-        T r;
-        T am, ap;
-        if (DIR == 1) {
-          r = rhs[id] - e[id - 1] * w.amx - e[id + 1] * w.apx - e[id - Lv.sxy] * w.amz - e[id + Lv.sxy] * w.apz;
-          am = w.amy; ap = w.apy;
+        T amx = (f & FL_IM) ? T(0) : mx, apx = (f & FL_IP) ? T(0) : px;
+        T amo = (f & FO_M) ? T(0) : mo, apo = (f & FO_P) ? T(0) : po;
+        T am = (f & FL_M) ? T(0) : cml[q], ap = (f & FL_P) ? T(0) : cpl[q];
+        // (acx + acy + acz) in the reference's order
+        T diag = DIR == 1 ? (-(amx + apx) + -(am + ap)) + -(amo + apo) : (-(amx + apx) + -(amo + apo)) + -(am + ap);
+        T r = rhs[id] - e[id - 1] * amx - e[id + 1] * apx - e[id - ostride] * amo - e[id + ostride] * apo;
+        if (tabs(diag) < T(1.0e-12)) {
+          d = ev;
         } else {
-          r = rhs[id] - e[id - Lv.sx] * w.amy - e[id + Lv.sx] * w.apy - e[id - 1] * w.amx - e[id + 1] * w.apx;
-          am = w.amz; ap = w.apz;
-        }
-        if (tabs(w.diag) < T(1.0e-12)) {
-          d[m] = ev;
-        } else {
-          a[m] = am; b[m] = w.diag; c[m] = ap; d[m] = r;
+          a = am; b = diag; c = ap; d = r;
         }
       }
     }
+    se.push(m, a, b, c, d);
   }
-  T D[LS], F[LS], G[LS], cl, dl, fl;
-  seg_eliminate<T, LS>(a, b, c, d, D, F, G, cl, dl, fl);
+  T cl, dl, fl;
+  se.back(cl, dl, fl);
   // LDS layout: [seg][field][tx]; fields: cl, dl, fl, D0, F0, G0, X
   auto S_ = [&](int s, int fld) -> T & { return sh[((long)s * 7 + fld) * blockDim.x + tx]; };
   S_(seg, 0) = cl; S_(seg, 1) = dl; S_(seg, 2) = fl;
-  S_(seg, 3) = D[0]; S_(seg, 4) = F[0]; S_(seg, 5) = G[0];
+  S_(seg, 3) = se.dp[0]; S_(seg, 4) = se.fp[0]; S_(seg, 5) = se.cp[0];
   __syncthreads();
   if (seg == 0) {
     // Thomas on the P reduced equations of this line (column tx)
@@ -757,7 +916,7 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
       T D0n = T(0), F0n = T(0), G0n = T(0);
       if (s + 1 < P) { D0n = S_(s + 1, 3); F0n = S_(s + 1, 4); G0n = S_(s + 1, 5); }
       T A = -fls, B = T(1) + cls * F0n, C = cls * G0n, R = dls - cls * D0n;
-      T inv = T(1) / (B - A * cprev);
+      T inv = fast_rcp(B - A * cprev);
       cprev = C * inv;
       dprev = (R - A * dprev) * inv;
       S_(s, 6) = cprev;   // temporarily c'
@@ -774,41 +933,48 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
   T XS = S_(seg, 6);
   T XL = seg > 0 ? S_(seg - 1, 6) : T(0);
   if (!ok) return;
+  se.combine(XL, XS);
 #pragma unroll
   for (int m = 0; m < LS; m++) {
-    int q = m0 + m;
-    if (q < n) {
-      T x = (m < LS - 1) ? (D[m] + F[m] * XL + G[m] * XS) : XS;
-      int jj = (DIR == 1) ? q : j;
-      int kk = (DIR == 1) ? k : q;
-      long id = lidx(Lv.sx, Lv.sxy, i, jj, kk);
-      e[id] = (T(1) - omega) * e[id] + omega * x;
-    }
+    if (m0 + m < n) e[id0 + m * lstride] = (T(1) - omega) * eo[m] + omega * se.dp[m];
   }
-  (void)stride;
 }
 
 // ---------------------------------------------------------------------
-// coarsest level: e = G * rhs (dense pseudo-inverse of the coarsest
-// operator with the fluid-volume mean constraint, computed in FP64 on the
-// host).  One workgroup.
+// coarsest level (FP64 in both precisions): e = G r with the dense
+// solution operator G (bordered LU on the GPU at set-up: the coarse equations with the
+// fluid-volume mean constraint per connected component), G[p*n+q] =
+// response of cell q to a unit rhs at cell p.  Stage 1: workgroup (qb, pc)
+// forms the partial products of rhs cells [pc*pchunk, (pc+1)*pchunk) for
+// 256 consecutive q (coalesced rows of G); stage 2 adds the partials in
+// chunk order (deterministic) and writes the correction.
 // ---------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(1024) k_coarse_solve(LevelArgs<T> L, int n, const double *__restrict__ G,
-                                                       const T *__restrict__ rhs, T *__restrict__ e) {
-  extern __shared__ unsigned char smem_raw[];
-  double *r = reinterpret_cast<double *>(smem_raw);
-  for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    int i = q % L.cx, t = q / L.cx, j = t % L.cy, k = t / L.cy;
-    r[q] = (double)rhs[lidx(L.sx, L.sxy, i, j, k)];
+__global__ void __launch_bounds__(256) k_coarse_mv(LevelArgs<T> L, int n, int pchunk, const double *__restrict__ G,
+                                                   const T *__restrict__ rhs, double *__restrict__ part) {
+  __shared__ double r[256];
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  const int p0 = blockIdx.y * pchunk, p1 = min(n, p0 + pchunk);
+  for (int p = p0 + threadIdx.x; p < p1; p += 256) {
+    int i = p % L.cx, t = p / L.cx, j = t % L.cy, k = t / L.cy;
+    r[p - p0] = (double)rhs[lidx(L.sx, L.sxy, i, j, k)];
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    double s = 0.0;
-    for (int p = 0; p < n; p++) s += G[(long)p * n + q] * r[p];
-    int i = q % L.cx, t = q / L.cx, j = t % L.cy, k = t / L.cy;
-    e[lidx(L.sx, L.sxy, i, j, k)] = (T)s;
-  }
+  if (q >= n) return;
+  double s = 0.0;
+  for (int p = p0; p < p1; p++) s += G[(long)p * n + q] * r[p - p0];
+  part[(long)blockIdx.y * n + q] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_coarse_sum(LevelArgs<T> L, int n, int nch, const double *__restrict__ part,
+                                                    T *__restrict__ e) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= n) return;
+  double s = 0.0;
+  for (int c = 0; c < nch; c++) s += part[(long)c * n + q];
+  int i = q % L.cx, t = q / L.cx, j = t % L.cy, k = t / L.cy;
+  e[lidx(L.sx, L.sxy, i, j, k)] = (T)s;
 }
 
 // ---------------------------------------------------------------------
@@ -934,31 +1100,34 @@ __global__ void __launch_bounds__(256) k_level_to_ref(RefArgs R, LevelArgs<T> L,
 // ---------------------------------------------------------------------
 // batched tridiagonal solves through the line-solver cores (tdma parity)
 // ---------------------------------------------------------------------
-template <int S>
+template <int G, int S>
 __global__ void __launch_bounds__(256) k_tdma_wave(int n, int nlines, const double *__restrict__ A,
                                                    const double *__restrict__ B, const double *__restrict__ C,
                                                    const double *__restrict__ Rr, double *__restrict__ U) {
-  int lane = threadIdx.x & 63;
-  int line = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (line >= nlines) return;
-  long base = (long)line * n;
-  double a[S], b[S], c[S], d[S], x[S];
+  // one group of G lanes per line; lanes of a group past nlines solve
+  // identity rows (every lane takes part in the group exchanges)
+  const int line = (blockIdx.x * 256 + (int)threadIdx.x) / G;
+  const int sub = threadIdx.x & (G - 1);
+  const bool ok = line < nlines;
+  const long base = ok ? (long)line * n : 0;
+  SegElim<double, S> se;
 #pragma unroll
   for (int s = 0; s < S; s++) {
-    int q = lane * S + s;
-    a[s] = 0; b[s] = 1; c[s] = 0; d[s] = 0;
-    if (q < n) {
-      a[s] = q > 0 ? A[base + q] : 0.0;
-      b[s] = B[base + q];
-      c[s] = q < n - 1 ? C[base + q] : 0.0;
-      d[s] = Rr[base + q];
+    int q = sub * S + s;
+    double a = 0, b = 1, c = 0, d = 0;
+    if (ok && q < n) {
+      a = q > 0 ? A[base + q] : 0.0;
+      b = B[base + q];
+      c = q < n - 1 ? C[base + q] : 0.0;
+      d = Rr[base + q];
     }
+    se.push(s, a, b, c, d);
   }
-  wave_line_solve<double, S>(a, b, c, d, x);
+  se.template solve<G>();
 #pragma unroll
   for (int s = 0; s < S; s++) {
-    int q = lane * S + s;
-    if (q < n) U[base + q] = x[s];
+    int q = sub * S + s;
+    if (ok && q < n) U[base + q] = se.dp[s];
   }
 }
 
@@ -972,27 +1141,28 @@ __global__ void __launch_bounds__(1024) k_tdma_seg(int n, int nlines, const doub
   const int tx = threadIdx.x, seg = threadIdx.y;
   const int line = blockIdx.x * blockDim.x + tx;
   const bool ok = line < nlines;
-  double a[LS], b[LS], c[LS], d[LS];
   const int m0 = seg * LS;
+  SegElim<double, LS> se;
 #pragma unroll
   for (int m = 0; m < LS; m++) {
     int q = m0 + m;
-    a[m] = 0; b[m] = 1; c[m] = 0; d[m] = 0;
+    double a = 0, b = 1, c = 0, d = 0;
     if (ok && q < n) {
       // lines interleaved so that consecutive threads read consecutive
       // addresses: element q of line l at q*nlines + l
       long id = (long)q * nlines + line;
-      a[m] = q > 0 ? A[id] : 0.0;
-      b[m] = B[id];
-      c[m] = q < n - 1 ? C[id] : 0.0;
-      d[m] = Rr[id];
+      a = q > 0 ? A[id] : 0.0;
+      b = B[id];
+      c = q < n - 1 ? C[id] : 0.0;
+      d = Rr[id];
     }
+    se.push(m, a, b, c, d);
   }
-  double D[LS], F[LS], G[LS], cl, dl, fl;
-  seg_eliminate<double, LS>(a, b, c, d, D, F, G, cl, dl, fl);
+  double cl, dl, fl;
+  se.back(cl, dl, fl);
   auto S_ = [&](int s, int fld) -> double & { return sh[((long)s * 7 + fld) * blockDim.x + tx]; };
   S_(seg, 0) = cl; S_(seg, 1) = dl; S_(seg, 2) = fl;
-  S_(seg, 3) = D[0]; S_(seg, 4) = F[0]; S_(seg, 5) = G[0];
+  S_(seg, 3) = se.dp[0]; S_(seg, 4) = se.fp[0]; S_(seg, 5) = se.cp[0];
   __syncthreads();
   if (seg == 0) {
     double cprev = 0, dprev = 0;
@@ -1001,7 +1171,7 @@ __global__ void __launch_bounds__(1024) k_tdma_seg(int n, int nlines, const doub
       double D0n = 0, F0n = 0, G0n = 0;
       if (s + 1 < P) { D0n = S_(s + 1, 3); F0n = S_(s + 1, 4); G0n = S_(s + 1, 5); }
       double Aa = -fls, Bb = 1.0 + cls * F0n, Cc = cls * G0n, Rv = dls - cls * D0n;
-      double inv = 1.0 / (Bb - Aa * cprev);
+      double inv = fast_rcp(Bb - Aa * cprev);
       cprev = Cc * inv;
       dprev = (Rv - Aa * dprev) * inv;
       S_(s, 6) = cprev;
@@ -1018,10 +1188,11 @@ __global__ void __launch_bounds__(1024) k_tdma_seg(int n, int nlines, const doub
   double XS = S_(seg, 6);
   double XL = seg > 0 ? S_(seg - 1, 6) : 0.0;
   if (!ok) return;
+  se.combine(XL, XS);
 #pragma unroll
   for (int m = 0; m < LS; m++) {
     int q = m0 + m;
-    if (q < n) U[(long)q * nlines + line] = (m < LS - 1) ? (D[m] + F[m] * XL + G[m] * XS) : XS;
+    if (q < n) U[(long)q * nlines + line] = se.dp[m];
   }
 }
 

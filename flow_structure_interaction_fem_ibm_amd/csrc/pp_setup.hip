@@ -7,8 +7,12 @@
 #include <cstring>
 #include <random>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "pp_internal.h"
 #include "pp_setup.h"
+
 
 namespace pp {
 
@@ -106,21 +110,22 @@ void free_levels(pp_ctx *c) {
 
 void free_ghosts(pp_ctx *c) {
   GhostTables &g = c->gh;
-  void *ptrs[] = {g.g_ref, g.g_al, g.s_ref, g.s_al, g.coef64, g.coef32, g.bself, g.bnd, g.buf64, g.buf32,
+  void *ptrs[] = {g.g_ref, g.g_al, g.bself, g.bnd,
+                  g.c_ptr, g.c_ref, g.c_al, g.c_w64, g.c_w32, g.bnd_eff,
                   g.nb_ref, g.nb_al, g.nb_w64, g.nb_w32, g.t_ref, g.t_al, g.t_ptr, g.t_ghost, g.t_coef64,
                   g.t_coef32, g.t_vinv64, g.t_vinv32, g.g_tidx, g.lam64[0], g.lam64[1], g.rhsg64,
                   g.tbase64, g.lam32[0], g.lam32[1], g.rhsg32, g.tbase32, g.olam64[0], g.olam64[1],
                   g.orhsg64, g.otbase64};
   for (void *p : ptrs) hipFree(p);
   g = GhostTables();
-  hipFree(c->cs.G);
+  hipFree(c->cs.G); hipFree(c->cs.part);
   c->cs = CoarseSolve();
 }
 
 // Level hierarchy: level 0 is the grid; a direction is halved while its
 // cell count is even and at least 4; coarsening stops when the level has at
 // most kCoarseMax cells or no direction can be halved.
-static constexpr long kCoarseMax = 512;
+static constexpr long kCoarseMax = 4096;
 static constexpr int kMaxLevels = 12;
 
 int build_levels(pp_ctx *c) {
@@ -379,6 +384,65 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     }
     G.kexact = std::max(it, 1);
   }
+  // ---- composite ghost rows ----
+  // Row g of M = sum_{t < kexact} C_gg^t C_gf (the kexact Jacobi sweeps
+  // above from zero ghost values, expanded once): every solved ghost value
+  // becomes one gather over fluid cells.  Stencil entries on solid
+  // non-ghost cells read 0 (blank non-ghost cells hold 0 throughout).
+  {
+    std::vector<int> gpos(c->nref, -1);
+    for (int q = 0; q < n; q++) gpos[G.h_g_ref[q]] = q;
+    using Row = std::vector<std::pair<long, double>>;
+    auto merge = [](Row &r) {
+      std::sort(r.begin(), r.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+      Row o;
+      for (auto &e : r) {
+        if (!o.empty() && o.back().first == e.first) o.back().second += e.second;
+        else o.push_back(e);
+      }
+      r.swap(o);
+    };
+    std::vector<Row> cgf(n);
+    G.h_gg_ptr.assign(1, 0); G.h_gg_idx.clear(); G.h_gg_w.clear();
+    for (int q = 0; q < n; q++) {
+      for (int r = 0; r < 8; r++) {
+        double cf = G.h_coef[8L * q + r];
+        long rs = G.h_s_ref[8L * q + r];
+        if (cf == 0.0 || rs == G.h_g_ref[q]) continue;
+        if (gpos[rs] >= 0) {
+          G.h_gg_idx.push_back(gpos[rs]);
+          G.h_gg_w.push_back(cf);
+        } else if (c->h_iblank[rs] == 0) {
+          cgf[q].push_back({rs, cf});
+        }
+      }
+      merge(cgf[q]);
+      G.h_gg_ptr.push_back((int)G.h_gg_idx.size());
+    }
+    std::vector<Row> cur = cgf, nxt(n);
+    bool coupled = !G.h_gg_idx.empty();
+    for (int t = 1; coupled && t < G.kexact; t++) {
+      for (int q = 0; q < n; q++) {
+        Row r = cgf[q];
+        for (int e = G.h_gg_ptr[q]; e < G.h_gg_ptr[q + 1]; e++)
+          for (auto &x : cur[G.h_gg_idx[e]]) r.push_back({x.first, G.h_gg_w[e] * x.second});
+        merge(r);
+        nxt[q].swap(r);
+      }
+      cur.swap(nxt);
+    }
+    G.h_c_ptr.assign(1, 0); G.h_c_ref.clear(); G.h_c_al.clear(); G.h_c_w.clear();
+    for (int q = 0; q < n; q++) {
+      for (auto &x : cur[q]) {
+        long rs = x.first;
+        long z = rs / c->ref.exy, rem = rs % c->ref.exy, y = rem / c->ref.ex, xx = rem % c->ref.ex;
+        G.h_c_ref.push_back(rs);
+        G.h_c_al.push_back(lidx(L0.sx, L0.sxy, (int)xx - 1, (int)y - 1, (int)z - 1));
+        G.h_c_w.push_back(x.second);
+      }
+      G.h_c_ptr.push_back((int)G.h_c_ref.size());
+    }
+  }
   // ---- adjoint tables ----
   // fluid 6-neighbours of each ghost with the face conductance a*V of the
   // neighbour's row entry for the ghost (fine-level coefficients
@@ -446,6 +510,8 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     G.nt = (int)G.h_t_ref.size();
   }
   G.g_ref = dev_upload(G.h_g_ref); G.g_al = dev_upload(G.h_g_al);
+  G.c_ptr = dev_upload(G.h_c_ptr); G.c_ref = dev_upload(G.h_c_ref); G.c_al = dev_upload(G.h_c_al);
+  G.c_w64 = dev_upload(G.h_c_w); G.c_w32 = dev_upload(to_f32(G.h_c_w));
   G.nb_ref = dev_upload(G.h_nb_ref); G.nb_al = dev_upload(G.h_nb_al);
   G.nb_w64 = dev_upload(G.h_nb_w); G.nb_w32 = dev_upload(to_f32(G.h_nb_w));
   G.t_ref = dev_upload(G.h_t_ref); G.t_al = dev_upload(G.h_t_al); G.t_ptr = dev_upload(G.h_t_ptr);
@@ -467,55 +533,14 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     hipMalloc(&G.otbase64, sizeof(double) * G.nt);
     hipMalloc(&G.tbase32, sizeof(float) * G.nt);
   }
-  G.s_ref = dev_upload(G.h_s_ref); G.s_al = dev_upload(G.h_s_al);
-  G.coef64 = dev_upload(G.h_coef); G.coef32 = dev_upload(to_f32(G.h_coef));
   G.bself = dev_upload(G.h_bself);
   if (n > 0) {
     hipMalloc(&G.bnd, sizeof(double) * n);
     hipMemset(G.bnd, 0, sizeof(double) * n);
-    hipMalloc(&G.buf64, sizeof(double) * n);
-    hipMalloc(&G.buf32, sizeof(float) * n);
+    hipMalloc(&G.bnd_eff, sizeof(double) * n);
+    hipMemset(G.bnd_eff, 0, sizeof(double) * n);
   }
   return PP_OK;
-}
-
-// Dense LU with partial pivoting (in place), then solve for one rhs.
-static bool lu_factor(std::vector<double> &A, int n, std::vector<int> &piv) {
-  piv.resize(n);
-  for (int k = 0; k < n; k++) {
-    int p = k;
-    double mx = std::fabs(A[(long)k * n + k]);
-    for (int i = k + 1; i < n; i++)
-      if (std::fabs(A[(long)i * n + k]) > mx) { mx = std::fabs(A[(long)i * n + k]); p = i; }
-    if (mx == 0.0) return false;
-    piv[k] = p;
-    if (p != k)
-      for (int j = 0; j < n; j++) std::swap(A[(long)k * n + j], A[(long)p * n + j]);
-    double inv = 1.0 / A[(long)k * n + k];
-    for (int i = k + 1; i < n; i++) {
-      double f = A[(long)i * n + k] * inv;
-      A[(long)i * n + k] = f;
-      if (f != 0.0)
-        for (int j = k + 1; j < n; j++) A[(long)i * n + j] -= f * A[(long)k * n + j];
-    }
-  }
-  return true;
-}
-// lu_factor swaps whole rows (multipliers included), so all interchanges
-// are applied to b first, then the unit-lower and upper solves.
-static void lu_solve(const std::vector<double> &A, int n, const std::vector<int> &piv, std::vector<double> &b) {
-  for (int k = 0; k < n; k++)
-    if (piv[k] != k) std::swap(b[k], b[piv[k]]);
-  for (int k = 0; k < n; k++) {
-    double bk = b[k];
-    if (bk != 0.0)
-      for (int i = k + 1; i < n; i++) b[i] -= A[(long)i * n + k] * bk;
-  }
-  for (int i = n - 1; i >= 0; i--) {
-    double s = b[i];
-    for (int j = i + 1; j < n; j++) s -= A[(long)i * n + j] * b[j];
-    b[i] = s / A[(long)i * n + i];
-  }
 }
 
 // Coarsest level: G maps the coarse rhs to the correction that solves the
@@ -572,20 +597,70 @@ int build_coarse(pp_ctx *c) {
     double vol = L.w[0][i] * L.w[1][j] * L.w[2][k];
     A[(long)(n + comp[q]) * m + q] = vol;
   }
-  std::vector<int> piv;
-  if (!lu_factor(A, m, piv)) { set_error("coarsest-level bordered matrix is singular"); return PP_BAD_GEOMETRY; }
-  std::vector<double> Gm((long)n * n, 0.0), rhs(m);
-  for (int p = 0; p < n; p++) {
-    if (!fluid[p]) continue;
-    std::fill(rhs.begin(), rhs.end(), 0.0);
-    rhs[p] = 1.0;
-    lu_solve(A, m, piv, rhs);
-    for (int q = 0; q < n; q++) Gm[(long)p * n + q] = fluid[q] ? rhs[q] : 0.0;
+  // Explicit solution operator on the GPU: LU (rocSOLVER getrf) of the
+  // bordered matrix and solves for the unit right-hand sides of the fluid
+  // cells (getrs).  A is assembled row-major, i.e. column-major A^T, hence
+  // the transposed solve.  Column p of the solution holds the response of
+  // every cell q to a unit rhs at p: G[p*n + q].
+  if (!c->blas) {
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success) {
+      set_error("rocblas_create_handle failed");
+      return PP_HIP_ERROR;
+    }
+    c->blas = h;
   }
-  hipFree(c->cs.G);
-  c->cs.n = n;
-  c->cs.G = dev_upload(Gm);
-  return c->cs.G ? PP_OK : PP_HIP_ERROR;
+  rocblas_handle h = (rocblas_handle)c->blas;
+  rocblas_set_stream(h, c->stream);
+  std::vector<double> B((long)m * n, 0.0);
+  for (int p = 0; p < n; p++)
+    if (fluid[p]) B[(long)p * m + p] = 1.0;
+  double *dA = dev_upload(A), *dB = dev_upload(B);
+  rocblas_int *dpiv = nullptr, *dinfo = nullptr;
+  hipMalloc(&dpiv, sizeof(rocblas_int) * m);
+  hipMalloc(&dinfo, sizeof(rocblas_int));
+  int rc = PP_OK;
+  if (!dA || !dB || !dpiv || !dinfo) {
+    set_error("coarse solve: device allocation failed");
+    rc = PP_HIP_ERROR;
+  } else if (rocsolver_dgetrf(h, m, m, dA, m, dpiv, dinfo) != rocblas_status_success) {
+    set_error("rocsolver_dgetrf failed");
+    rc = PP_HIP_ERROR;
+  } else {
+    rocblas_int info = 0;
+    hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, c->stream);
+    hipStreamSynchronize(c->stream);
+    if (info != 0) {
+      set_error("coarsest-level bordered matrix is singular (getrf info %d)", (int)info);
+      rc = PP_BAD_GEOMETRY;
+    } else if (rocsolver_dgetrs(h, rocblas_operation_transpose, m, n, dA, m, dpiv, dB, m) !=
+               rocblas_status_success) {
+      set_error("rocsolver_dgetrs failed");
+      rc = PP_HIP_ERROR;
+    }
+  }
+  if (rc == PP_OK) {
+    hipFree(c->cs.G); hipFree(c->cs.part);
+    c->cs = CoarseSolve();
+    c->cs.n = n;
+    c->cs.pchunk = 64;
+    c->cs.nch = (n + c->cs.pchunk - 1) / c->cs.pchunk;
+    hipMalloc(&c->cs.G, sizeof(double) * (size_t)n * n);
+    hipMalloc(&c->cs.part, sizeof(double) * (size_t)n * c->cs.nch);
+    if (!c->cs.G || !c->cs.part) {
+      set_error("coarse solve: device allocation failed");
+      rc = PP_HIP_ERROR;
+    } else {
+      hipMemcpy2DAsync(c->cs.G, sizeof(double) * n, dB, sizeof(double) * m, sizeof(double) * n, n,
+                       hipMemcpyDeviceToDevice, c->stream);
+      if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        set_error("coarse solve: copy failed");
+        rc = PP_HIP_ERROR;
+      }
+    }
+  }
+  hipFree(dA); hipFree(dB); hipFree(dpiv); hipFree(dinfo);
+  return rc;
 }
 
 }  // namespace pp

@@ -135,7 +135,8 @@ void pp_default_opts(pp_solve_opts *o);
 const char *pp_last_error(void);
 
 /* Create a context on one GPU for a grid (replaces MG_initial,
- * CORE_MG.F90:31-278, and the grid half of init_simulation). */
+ * CORE_MG.F90:31-278, and the grid half of init_simulation).  4 to 512
+ * cells per direction (PP_BAD_ARG otherwise). */
 int32_t pp_create(pp_ctx **ctx, const pp_grid_desc *g);
 int32_t pp_destroy(pp_ctx *ctx);
 
