@@ -224,7 +224,7 @@ void launch_yzline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
   yz_shape(n, LS, P, TX, (int)sizeof(T));
   int half = (L.cx + 1) / 2;
   dim3 grid((half + TX - 1) / TX, planes), block(TX, P);
-  size_t sh = sizeof(T) * 7 * TX * P;
+  size_t sh = sizeof(T) * 8 * TX * P;
   T *e = lv_e<T>(L);
   const T *rhs = lv_rhs<T>(L);
   hipStream_t st = c->stream;
@@ -1245,7 +1245,7 @@ int32_t pp_tdma_batched(pp_ctx *c, const double *a, const double *b, const doubl
     int LS, P, TX;
     yz_shape(n, LS, P, TX, 8);
     dim3 grid((nl + TX - 1) / TX), block(TX, P);
-    size_t sh = sizeof(double) * 7 * TX * P;
+    size_t sh = sizeof(double) * 8 * TX * P;
     switch (LS) {
       case 1: k_tdma_seg<1><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;
       case 2: k_tdma_seg<2><<<grid, block, sh, s>>>(n, nl, d[0], d[1], d[2], d[3], d[4]); break;

@@ -830,6 +830,50 @@ __global__ void __launch_bounds__(256) k_xline(LevelArgs<T> L, const T *__restri
   store_chunk<T, S>(e + base, nv, eo);
 }
 
+// Reduced system of a line split into P segments held by the P threads
+// (tx, 0..P-1) of a block column: back recurrences of each segment, then
+// parallel cyclic reduction of the P equations (unknown = a segment's last
+// cell) with one equation per thread, double-buffered in LDS
+// [buf][seg][field][tx] (8*P*blockDim.x elements).  Every thread of the
+// block must call it (it synchronises the block).  Returns the unknowns of
+// the previous segment (XL, 0 for the first) and of this segment (XS).
+template <typename T, int LS>
+__device__ inline void seg_reduced_solve(T *sh, int P, int tx, int seg, SegElim<T, LS> &se, T &XL, T &XS) {
+  T cl, dl, fl;
+  se.back(cl, dl, fl);
+  const int TXb = blockDim.x;
+  T *bufA = sh, *bufB = sh + 4 * TXb * P;
+  auto E_ = [&](T *buf, int s, int fld) -> T & { return buf[((long)s * 4 + fld) * TXb + tx]; };
+  E_(bufB, seg, 0) = se.dp[0]; E_(bufB, seg, 1) = se.fp[0]; E_(bufB, seg, 2) = se.cp[0];
+  __syncthreads();
+  T D0n = T(0), F0n = T(0), G0n = T(0);
+  if (seg + 1 < P) { D0n = E_(bufB, seg + 1, 0); F0n = E_(bufB, seg + 1, 1); G0n = E_(bufB, seg + 1, 2); }
+  T A = -fl, B = T(1) + cl * F0n, C = cl * G0n, R = dl - cl * D0n;
+  T *cur = bufA, *nxt = bufB;
+  E_(cur, seg, 0) = A; E_(cur, seg, 1) = B; E_(cur, seg, 2) = C; E_(cur, seg, 3) = R;
+  __syncthreads();
+  for (int d = 1; d < P; d <<= 1) {
+    T Am = T(0), Bm = T(1), Cm = T(0), Rm = T(0), Ap = T(0), Bp = T(1), Cp = T(0), Rp = T(0);
+    if (seg - d >= 0) {
+      Am = E_(cur, seg - d, 0); Bm = E_(cur, seg - d, 1); Cm = E_(cur, seg - d, 2); Rm = E_(cur, seg - d, 3);
+    }
+    if (seg + d < P) {
+      Ap = E_(cur, seg + d, 0); Bp = E_(cur, seg + d, 1); Cp = E_(cur, seg + d, 2); Rp = E_(cur, seg + d, 3);
+    }
+    T k1 = A * fast_rcp(Bm), k2 = C * fast_rcp(Bp);
+    T nA = -Am * k1, nC = -Cp * k2;
+    T nB = B - Cm * k1 - Ap * k2, nR = R - Rm * k1 - Rp * k2;
+    A = nA; B = nB; C = nC; R = nR;
+    E_(nxt, seg, 0) = A; E_(nxt, seg, 1) = B; E_(nxt, seg, 2) = C; E_(nxt, seg, 3) = R;
+    __syncthreads();
+    T *t = cur; cur = nxt; nxt = t;
+  }
+  XS = R * fast_rcp(B);
+  E_(nxt, seg, 0) = XS;
+  __syncthreads();
+  XL = seg > 0 ? E_(nxt, seg - 1, 0) : T(0);
+}
+
 // ---------------------------------------------------------------------
 // y- and z-line zebra passes (lines across the contiguous index): lines
 // (i, plane) with (i + plane) % 2 == colour, plane = k for DIR = 1 (lines
@@ -838,8 +882,8 @@ __global__ void __launch_bounds__(256) k_xline(LevelArgs<T> L, const T *__restri
 // block of TX x P threads covers TX lines of one colour (every other i of
 // a 2*TX-wide tile) and P segments of LS cells of each line.  Each thread
 // eliminates its segment in registers, the P reduced equations of a line
-// are solved by one thread (Thomas) from LDS, and every thread
-// back-substitutes.  The tile's loads cover 2*TX consecutive i, half of
+// are solved by parallel cyclic reduction in LDS (one equation per thread),
+// and every thread back-substitutes.  The tile's loads cover 2*TX consecutive i, half of
 // them own cells and half the neighbour lines read by the stencil.
 // ---------------------------------------------------------------------
 template <typename T, int LS, int DIR>
@@ -900,38 +944,8 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
     }
     se.push(m, a, b, c, d);
   }
-  T cl, dl, fl;
-  se.back(cl, dl, fl);
-  // LDS layout: [seg][field][tx]; fields: cl, dl, fl, D0, F0, G0, X
-  auto S_ = [&](int s, int fld) -> T & { return sh[((long)s * 7 + fld) * blockDim.x + tx]; };
-  S_(seg, 0) = cl; S_(seg, 1) = dl; S_(seg, 2) = fl;
-  S_(seg, 3) = se.dp[0]; S_(seg, 4) = se.fp[0]; S_(seg, 5) = se.cp[0];
-  __syncthreads();
-  if (seg == 0) {
-    // Thomas on the P reduced equations of this line (column tx)
-    // A_s X_{s-1} + B_s X_s + C_s X_{s+1} = R_s
-    T cprev = T(0), dprev = T(0);
-    for (int s = 0; s < P; s++) {
-      T cls = S_(s, 0), dls = S_(s, 1), fls = S_(s, 2);
-      T D0n = T(0), F0n = T(0), G0n = T(0);
-      if (s + 1 < P) { D0n = S_(s + 1, 3); F0n = S_(s + 1, 4); G0n = S_(s + 1, 5); }
-      T A = -fls, B = T(1) + cls * F0n, C = cls * G0n, R = dls - cls * D0n;
-      T inv = fast_rcp(B - A * cprev);
-      cprev = C * inv;
-      dprev = (R - A * dprev) * inv;
-      S_(s, 6) = cprev;   // temporarily c'
-      S_(s, 3) = dprev;   // d' (D0 of s no longer needed by s-1: already read)
-    }
-    T X = S_(P - 1, 3);
-    S_(P - 1, 6) = X;
-    for (int s = P - 2; s >= 0; s--) {
-      X = S_(s, 3) - S_(s, 6) * X;
-      S_(s, 6) = X;
-    }
-  }
-  __syncthreads();
-  T XS = S_(seg, 6);
-  T XL = seg > 0 ? S_(seg - 1, 6) : T(0);
+  T XL, XS;
+  seg_reduced_solve<T, LS>(sh, P, tx, seg, se, XL, XS);
   if (!ok) return;
   se.combine(XL, XS);
 #pragma unroll
@@ -1158,35 +1172,8 @@ __global__ void __launch_bounds__(1024) k_tdma_seg(int n, int nlines, const doub
     }
     se.push(m, a, b, c, d);
   }
-  double cl, dl, fl;
-  se.back(cl, dl, fl);
-  auto S_ = [&](int s, int fld) -> double & { return sh[((long)s * 7 + fld) * blockDim.x + tx]; };
-  S_(seg, 0) = cl; S_(seg, 1) = dl; S_(seg, 2) = fl;
-  S_(seg, 3) = se.dp[0]; S_(seg, 4) = se.fp[0]; S_(seg, 5) = se.cp[0];
-  __syncthreads();
-  if (seg == 0) {
-    double cprev = 0, dprev = 0;
-    for (int s = 0; s < P; s++) {
-      double cls = S_(s, 0), dls = S_(s, 1), fls = S_(s, 2);
-      double D0n = 0, F0n = 0, G0n = 0;
-      if (s + 1 < P) { D0n = S_(s + 1, 3); F0n = S_(s + 1, 4); G0n = S_(s + 1, 5); }
-      double Aa = -fls, Bb = 1.0 + cls * F0n, Cc = cls * G0n, Rv = dls - cls * D0n;
-      double inv = fast_rcp(Bb - Aa * cprev);
-      cprev = Cc * inv;
-      dprev = (Rv - Aa * dprev) * inv;
-      S_(s, 6) = cprev;
-      S_(s, 3) = dprev;
-    }
-    double X = S_(P - 1, 3);
-    S_(P - 1, 6) = X;
-    for (int s = P - 2; s >= 0; s--) {
-      X = S_(s, 3) - S_(s, 6) * X;
-      S_(s, 6) = X;
-    }
-  }
-  __syncthreads();
-  double XS = S_(seg, 6);
-  double XL = seg > 0 ? S_(seg - 1, 6) : 0.0;
+  double XL, XS;
+  seg_reduced_solve<double, LS>(sh, P, tx, seg, se, XL, XS);
   if (!ok) return;
   se.combine(XL, XS);
 #pragma unroll
