@@ -991,6 +991,13 @@ int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
     make_grid_dir(ns[d] + 1, xs[d], c->hx[d], c->rxc[d], c->rdx[d], c->rdxinv[d], c->rdxc[d], c->rdxcinv[d]);
   }
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (int d = 0; d < 3; d++) {
+    size_t nd = sizeof(double) * c->rdx[d].size();
+    HIPCHK(hipMalloc(&c->d_rdx[d], nd));
+    HIPCHK(hipMalloc(&c->d_rdxinv[d], nd));
+    HIPCHK(hipMemcpy(c->d_rdx[d], c->rdx[d].data(), nd, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_rdxinv[d], c->rdxinv[d].data(), nd, hipMemcpyHostToDevice));
+  }
   size_t nb = sizeof(double) * c->nref;
   HIPCHK(hipMalloc(&c->d_p, nb));
   HIPCHK(hipMalloc(&c->d_b, nb));
@@ -1022,6 +1029,7 @@ int32_t pp_destroy(pp_ctx *c) {
   hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_mu); hipFree(c->d_sv);
   hipFree(c->d_part); hipFree(c->d_part2); hipFree(c->d_res);
   for (double *q : c->d_kry) hipFree(q);
+  for (int d = 0; d < 3; d++) { hipFree(c->d_rdx[d]); hipFree(c->d_rdxinv[d]); }
   if (c->h_res) hipHostFree(c->h_res);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   if (c->blas) rocblas_destroy_handle((rocblas_handle)c->blas);
@@ -1269,9 +1277,33 @@ int32_t pp_tdma_batched(pp_ctx *c, const double *a, const double *b, const doubl
 
 int32_t pp_divergence(pp_ctx *c, const double *fu, const double *fv, const double *fw, double dtinv, double *nlu0,
                       double *sum) {
-  (void)c; (void)fu; (void)fv; (void)fw; (void)dtinv; (void)nlu0; (void)sum;
-  set_error("pp_divergence: not implemented yet");
-  return PP_BAD_ARG;
+  if (!c || !fu || !fv || !fw || !nlu0) { set_error("pp_divergence: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_divergence: no geometry (iblank)"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  size_t nb = sizeof(double) * c->nref;
+  double *d[4] = {nullptr, nullptr, nullptr, nullptr};
+  const double *src[3] = {fu, fv, fw};
+  int rc = PP_OK;
+  for (int q = 0; q < 4 && rc == PP_OK; q++)
+    if (hipMalloc(&d[q], nb) != hipSuccess) { set_error("pp_divergence: hipMalloc failed"); rc = PP_HIP_ERROR; }
+  if (rc == PP_OK) {
+    for (int q = 0; q < 3; q++) hipMemcpyAsync(d[q], src[q], nb, hipMemcpyHostToDevice, c->stream);
+    hipMemsetAsync(d[3], 0, nb, c->stream);
+    CellLaunch cl = ref_launch(c);
+    k_divergence<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, c->d_rdx[0], c->d_rdx[1], c->d_rdx[2],
+                                                           c->d_rdxinv[0], c->d_rdxinv[1], c->d_rdxinv[2],
+                                                           c->d_iblank_ref, d[0], d[1], d[2], dtinv, d[3],
+                                                           c->d_part);
+    launch_reduce_sums(c, c->nblk_red, 1);
+    hipMemcpyAsync(c->h_res, c->d_res, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(nlu0, d[3], nb, hipMemcpyDeviceToHost, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) { set_error("pp_divergence: %s", hipGetErrorString(e)); rc = PP_HIP_ERROR; }
+    else if (sum) *sum = c->h_res[0];
+  }
+  for (int q = 0; q < 4; q++) hipFree(d[q]);
+  return rc;
 }
 
 int32_t pp_adjoint(pp_ctx *c, const double *wobj, double *lambda, double *dJ_dg, const pp_solve_opts *o,

@@ -144,6 +144,7 @@ struct pp_ctx {
   pp::GhostTables gh;
   pp::CoarseSolve cs;
   void *blas = nullptr;                // rocblas_handle of the coarse factorisation
+  double *d_rdx[3] = {}, *d_rdxinv[3] = {};  // make_grid dx, dxinv (0:n+1) per direction
   // FP64 work arrays in reference layout
   double *d_p = nullptr, *d_b = nullptr, *d_w = nullptr, *d_bwork = nullptr;
   double *d_tmp = nullptr;

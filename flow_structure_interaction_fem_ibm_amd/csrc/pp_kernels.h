@@ -255,6 +255,36 @@ __global__ void __launch_bounds__(NT) k_ref_residual(RefArgs R, LevelArgs<double
   }
 }
 
+// rhs_poisson, non-cut-cell branch (CORE_POISSON.F90:281-288), reference
+// layout, same operation order as the reference: nlu0 = dtinv*((u_{i+1}-u_i)
+// dxinv + (v_{j+1}-v_j) dyinv + (w_{k+1}-w_k) dzinv)*(1-iblank); per-block
+// partial sums of nlu0*dx*dy*dz at part[blk]
+template <int NT>
+__global__ void __launch_bounds__(NT) k_divergence(RefArgs R, const double *__restrict__ dx, const double *__restrict__ dy,
+                                                   const double *__restrict__ dz, const double *__restrict__ dxinv,
+                                                   const double *__restrict__ dyinv, const double *__restrict__ dzinv,
+                                                   const int8_t *__restrict__ ib, const double *__restrict__ fu,
+                                                   const double *__restrict__ fv, const double *__restrict__ fw,
+                                                   double dtinv, double *__restrict__ out, double *__restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ double s[NT / 64];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  double vol = 0.0;
+  if (i < R.nxc && j < R.nyc) {
+    long id = ridx(R, i, j, k);
+    int I = i + 1, J = j + 1, K = k + 1;
+    double v = dtinv * ((fu[id + 1] - fu[id]) * dxinv[I] + (fv[id + R.ex] - fv[id]) * dyinv[J] +
+                        (fw[id + R.exy] - fw[id]) * dzinv[K]);
+    v = v * (double)(1 - ib[id]);
+    out[id] = v;
+    vol = v * dx[I] * dy[J] * dz[K];
+  }
+  double t = block_sum<NT>(vol, s);
+  long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[blk] = t;
+}
+
 // p += e on fluid cells (refinement update, FP64 reference layout)
 template <typename TE>
 __global__ void __launch_bounds__(256) k_update(RefArgs R, LevelArgs<double> L, double *__restrict__ p,

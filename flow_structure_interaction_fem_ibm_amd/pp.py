@@ -294,6 +294,17 @@ class PressureSolver:
         _check(lib().pp_ghost_rows(self.h, p.ctypes.data, None if bc is None else bc.ctypes.data))
         return p
 
+    def divergence(self, face_u, face_v, face_w, dtinv=1.0):
+        """rhs_poisson (non-cut-cell): returns (nlu0, sum(nlu0*dV))."""
+        f = [np.ascontiguousarray(a, dtype=np.float64) for a in (face_u, face_v, face_w)]
+        for a in f:
+            assert a.shape == self.shape
+        out = np.zeros(self.shape)
+        s = C.c_double()
+        _check(lib().pp_divergence(self.h, f[0].ctypes.data, f[1].ctypes.data, f[2].ctypes.data, float(dtinv),
+                                   out.ctypes.data, C.byref(s)))
+        return out, s.value
+
     def mean_remove(self, p):
         m = C.c_double()
         _check(lib().pp_mean_remove(self.h, p.ctypes.data, C.byref(m)))

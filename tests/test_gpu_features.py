@@ -1,0 +1,128 @@
+"""GPU: rhs_poisson divergence against the oracle (bit-exact) and closed
+forms; the flexible-GMRES outer iteration against plain refinement; the
+hipGraph-replayed preconditioner against eager launches over several
+replays and a geometry change."""
+import numpy as np
+import pytest
+
+import cases
+from flow_structure_interaction_fem_ibm_amd import pp
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def fluid_mask(ib):
+    fl = np.zeros(ib.shape, bool)
+    fl[1:-2, 1:-2, 1:-2] = ib[1:-2, 1:-2, 1:-2] == 0
+    return fl
+
+
+def test_divergence_matches_oracle_bitwise():
+    announce("divergence 20x18x16 stretched sphere vs oracle")
+    xn, yn, zn, ib, gs, _ = cases.sphere_case(20, r=0.2, stretch=1.5, ny=18, nz=16)
+    s = pp.PressureSolver(xn, yn, zn)
+    s.set_geometry(ib, gs)
+    rng = np.random.default_rng(7)
+    fu, fv, fw = (rng.normal(size=ib.shape) for _ in range(3))
+    out, tot = s.divergence(fu, fv, fw, dtinv=3.0)
+    o = Oracle(xn, yn, zn)
+    o.set_iblank(ib)
+    ref, rtot = o.divergence(fu, fv, fw, 3.0)
+    assert np.array_equal(out, ref)
+    assert tot == pytest.approx(rtot, rel=1e-12, abs=1e-12 * np.abs(ref).sum())
+
+
+def test_divergence_closed_forms():
+    announce("divergence closed forms 16^3 sphere")
+    xn, yn, zn, ib, gs, _ = cases.sphere_case(16, r=0.25, stretch=1.2)
+    s = pp.PressureSolver(xn, yn, zn)
+    s.set_geometry(ib, gs)
+    # constant face velocities: zero divergence everywhere
+    c = np.full(ib.shape, 1.75)
+    out, tot = s.divergence(c, c, c)
+    assert np.all(out == 0.0) and tot == 0.0
+    # linear u = a x, v = b y, w = c z at the face nodes: div = a + b + c on
+    # fluid cells, 0 on blank cells
+    x = cases.metrics(xn)[0]
+    y = cases.metrics(yn)[0]
+    z = cases.metrics(zn)[0]
+    a, b, cc = 2.0, -0.5, 1.25
+    fu = np.broadcast_to(a * x[None, None, :], ib.shape).copy()
+    fv = np.broadcast_to(b * y[None, :, None], ib.shape).copy()
+    fw = np.broadcast_to(cc * z[:, None, None], ib.shape).copy()
+    out, _ = s.divergence(fu, fv, fw, dtinv=2.0)
+    fl = fluid_mask(ib)
+    assert np.allclose(out[fl], 2.0 * (a + b + cc), rtol=0, atol=1e-12)
+    solid = np.zeros(ib.shape, bool)
+    solid[1:-2, 1:-2, 1:-2] = ib[1:-2, 1:-2, 1:-2] == 1
+    assert np.all(out[solid] == 0.0)
+
+
+@pytest.mark.parametrize("prec", [pp.PP_PREC_FP64, pp.PP_PREC_MIXED])
+def test_krylov_and_refinement_reach_the_same_solution(prec):
+    announce(f"FGMRES(8) vs refinement 32x28x24 stretched sphere prec={prec}")
+    xn, yn, zn, ib, gs, _ = cases.sphere_case(32, r=0.2, stretch=1.5, ny=28, nz=24)
+    s = pp.PressureSolver(xn, yn, zn)
+    s.set_geometry(ib, gs)
+    b = cases.random_field(ib.shape, 5, ib)
+    out = {}
+    for m in (0, 8):
+        p = np.zeros(ib.shape)
+        st = s.solve(p, b, precision=prec, rel_tol=1e-12, max_outer=200, krylov=m)
+        assert st["status"] == "PP_OK", (m, st)
+        out[m] = (p, st)
+    fl = fluid_mask(ib)
+    err = np.linalg.norm((out[0][0] - out[8][0])[fl]) / np.linalg.norm(out[0][0][fl])
+    assert err < 1e-10, err
+    # the Krylov iteration does not need more preconditioner applications
+    assert out[8][1]["vcycles"] <= out[0][1]["vcycles"]
+
+
+def test_graph_replays_match_eager_launches():
+    announce("hipGraph preconditioner: 4 replayed solves + geometry change vs eager")
+    xn, yn, zn, ib, gs, _ = cases.sphere_case(24, r=0.2, stretch=1.3)
+    s = pp.PressureSolver(xn, yn, zn)
+    s.set_geometry(ib, gs)
+    fl = fluid_mask(ib)
+    for q in range(4):
+        b = cases.random_field(ib.shape, 100 + q, ib)
+        p_eager = np.zeros(ib.shape)
+        st_e = s.solve(p_eager, b, rel_tol=1e-11, max_outer=100, use_graph=0)
+        p_graph = np.zeros(ib.shape)
+        st_g = s.solve(p_graph, b, rel_tol=1e-11, max_outer=100, use_graph=1)
+        assert st_e["status"] == "PP_OK" and st_g["status"] == "PP_OK"
+        assert st_g["outer_iterations"] == st_e["outer_iterations"]
+        err = np.linalg.norm((p_eager - p_graph)[fl]) / np.linalg.norm(p_eager[fl])
+        assert err < 1e-13, (q, err)
+    # a new geometry drops the captured graph
+    xn2, yn2, zn2, ib2, gs2, _ = cases.sphere_case(24, r=0.26, c=(0.45, 0.5, 0.55), stretch=1.3)
+    s.set_geometry(ib2, gs2)
+    fl2 = fluid_mask(ib2)
+    b = cases.random_field(ib2.shape, 200, ib2)
+    p_e = np.zeros(ib2.shape)
+    p_g = np.zeros(ib2.shape)
+    assert s.solve(p_e, b, rel_tol=1e-11, use_graph=0)["status"] == "PP_OK"
+    assert s.solve(p_g, b, rel_tol=1e-11, use_graph=1)["status"] == "PP_OK"
+    assert np.linalg.norm((p_e - p_g)[fl2]) / np.linalg.norm(p_e[fl2]) < 1e-13
+
+
+def test_coarsest_level_is_solved_directly():
+    announce("hierarchy of the 256x192x192-shaped 64x48x48 grid and the coarse solve")
+    xn = np.linspace(0, 4, 65)
+    yn = pp.tanh_nodes(48, 3.0, 2.0)
+    zn = pp.tanh_nodes(48, 3.0, 2.0)
+    ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, [pp.Ellipsoid((1.5, 1.5, 1.5), (0.5, 0.25, 0.25), 20.0, 48)])
+    s = pp.PressureSolver(xn, yn, zn)
+    s.set_geometry(ib, gs)
+    lv = s.level_info()
+    assert lv[0] == (64, 48, 48)
+    assert np.prod(lv[-1]) <= 4096 < np.prod(lv[-2])
+    b = cases.random_field(ib.shape, 9, ib)
+    p = np.zeros(ib.shape)
+    st = s.solve(p, b, rel_tol=1e-10, max_outer=60)
+    assert st["status"] == "PP_OK", st
