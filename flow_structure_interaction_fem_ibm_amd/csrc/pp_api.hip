@@ -1200,6 +1200,65 @@ int32_t pp_mean_remove(pp_ctx *c, double *p, double *mean) {
   return PP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+template <typename T>
+int time_kernel(pp_ctx *c, int cls, int level, int reps, double *avg_us, double *bytes) {
+  Level &L = c->lv[level];
+  hipStream_t st = c->stream;
+  kreset(c, false);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
+  LevelArgs<T> a = largs<T>(L);
+  HIPCHK(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; r++) {
+    switch (cls) {
+      case PP_K_XLINE: launch_xline<T>(c, L, level, r & 1, T(1)); break;
+      case PP_K_YLINE: launch_yzline<T, 1>(c, L, level, r & 1, T(1)); break;
+      case PP_K_ZLINE: launch_yzline<T, 2>(c, L, level, r & 1, T(1)); break;
+      default:
+        kbegin(c, PP_K_RESIDUAL, level, (double)L.cx * L.cy * L.cz * (3.0 * sizeof(T) + 1.0));
+        k_residual<T><<<cl.grid, cl.block, 0, st>>>(a, lv_e<T>(L), lv_rhs<T>(L), lv_res<T>(L));
+        kend(c);
+        break;
+    }
+  }
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipGetLastError());
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *avg_us = 1e3 * (double)ms / reps;
+  *bytes = c->bytes_acc / reps;
+  return PP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int32_t pp_time_kernel(pp_ctx *c, int32_t cls, int32_t level, int32_t precision, int32_t reps, double *avg_us,
+                       double *bytes) {
+  if (!c || !avg_us || !bytes) { set_error("pp_time_kernel: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_time_kernel: no geometry"); return PP_NO_GEOMETRY; }
+  if (cls != PP_K_XLINE && cls != PP_K_YLINE && cls != PP_K_ZLINE && cls != PP_K_RESIDUAL) {
+    set_error("pp_time_kernel: kernel class %d is not timed separately", cls);
+    return PP_BAD_ARG;
+  }
+  if (level < 0 || level >= (int)c->lv.size() - 1 || reps < 1 || reps > 100000 ||
+      (precision != PP_PREC_FP64 && precision != PP_PREC_MIXED)) {
+    set_error("pp_time_kernel: bad level %d, reps %d or precision %d", level, reps, precision);
+    return PP_BAD_ARG;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  return precision == PP_PREC_MIXED ? time_kernel<float>(c, cls, level, reps, avg_us, bytes)
+                                    : time_kernel<double>(c, cls, level, reps, avg_us, bytes);
+}
+
 int32_t pp_level_info(pp_ctx *c, int32_t *n_levels, int32_t *dims) {
   if (!c || !n_levels) { set_error("pp_level_info: null argument"); return PP_BAD_ARG; }
   *n_levels = (int32_t)c->lv.size();

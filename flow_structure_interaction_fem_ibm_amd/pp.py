@@ -131,6 +131,7 @@ def lib():
     L.pp_tdma_batched.argtypes = [P, P, P, P, P, P, C.c_int32, C.c_int32, C.c_int32]
     L.pp_null_vector.argtypes = [P, P]
     L.pp_level_info.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.pp_time_kernel.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp]
     L.pp_geom_build.argtypes = [C.POINTER(C.c_void_p), C.POINTER(GridDesc), C.c_int32,
                                 C.POINTER(BodyDesc), C.c_int32, C.c_double, P]
     L.pp_geom_counts.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -138,7 +139,7 @@ def lib():
     L.pp_geom_destroy.argtypes = [P]
     for name in ("pp_create", "pp_destroy", "pp_set_geometry", "pp_solve", "pp_solve_device",
                  "pp_adjoint", "pp_calc_residual", "pp_ghost_rows", "pp_mean_remove",
-                 "pp_divergence", "pp_tdma_batched", "pp_null_vector", "pp_level_info",
+                 "pp_divergence", "pp_tdma_batched", "pp_null_vector", "pp_level_info", "pp_time_kernel",
                  "pp_geom_build", "pp_geom_counts", "pp_geom_get", "pp_geom_destroy"):
         getattr(L, name).restype = C.c_int32
     _lib = L
@@ -148,7 +149,8 @@ def lib():
 EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_set_geometry",
             "pp_solve", "pp_solve_device", "pp_adjoint", "pp_calc_residual", "pp_ghost_rows",
             "pp_mean_remove", "pp_divergence", "pp_tdma_batched", "pp_null_vector",
-            "pp_level_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get", "pp_geom_destroy")
+            "pp_level_info", "pp_time_kernel", "pp_geom_build", "pp_geom_counts", "pp_geom_get",
+            "pp_geom_destroy")
 
 
 def _check(rc, allow=(PP_OK,)):
@@ -320,6 +322,14 @@ class PressureSolver:
         dims = (C.c_int32 * 48)()
         _check(lib().pp_level_info(self.h, C.byref(n), dims))
         return [(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]) for l in range(n.value)]
+
+    def time_kernel(self, kernel_class, level=0, precision=PP_PREC_MIXED, reps=50):
+        """(avg_us, bytes_per_launch) of reps back-to-back launches of one
+        kernel class ("xline", "yline", "zline", "residual") on a level."""
+        cls = KERNEL_CLASSES.index(kernel_class)
+        us, b = C.c_double(), C.c_double()
+        _check(lib().pp_time_kernel(self.h, cls, level, precision, reps, C.byref(us), C.byref(b)))
+        return us.value, b.value
 
     def tdma_batched(self, a, b, c, r, method):
         a, b, c, r = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, r))

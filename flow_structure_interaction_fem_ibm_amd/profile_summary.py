@@ -1,0 +1,81 @@
+"""Summaries of rocprofv3 CSV output (kernel trace, counter collection) as
+short markdown, grouped by kernel and grid size (one grid size per
+multigrid level).
+
+  python -m flow_structure_interaction_fem_ibm_amd.profile_summary trace <kernel_trace.csv> [--last-solve]
+  python -m flow_structure_interaction_fem_ibm_amd.profile_summary pmc <counter_collection.csv> [...more csv]
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import json
+
+
+def _short(name):
+    n = name.split("(")[0].replace("void ", "")
+    return n
+
+
+def trace(path, last_solve=False, top=30):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    if last_solve:
+        starts = [i for i, r in enumerate(rows) if "k_mask_rhs" in r["Kernel_Name"]]
+        if starts:
+            rows = rows[starts[-1]:]
+    agg = collections.defaultdict(lambda: [0, 0.0])
+    for r in rows:
+        key = (_short(r["Kernel_Name"]), f'{r["Grid_Size_X"]}x{r["Grid_Size_Y"]}x{r["Grid_Size_Z"]}',
+               f'{r["Workgroup_Size_X"]}x{r["Workgroup_Size_Y"]}')
+        agg[key][0] += 1
+        agg[key][1] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+    busy = sum(v[1] for v in agg.values())
+    wall = (int(rows[-1]["End_Timestamp"]) - int(rows[0]["Start_Timestamp"])) * 1e-3 if rows else 0.0
+    out = [f"kernels: {len(rows)}, summed kernel time {busy / 1e3:.3f} ms, first start to last end {wall / 1e3:.3f} ms",
+           "", "| kernel | grid (threads) | block | calls | total us | avg us |", "|---|---|---|---|---|---|"]
+    for (k, g, b), (n, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+        out.append(f"| `{k}` | {g} | {b} | {n} | {t:.1f} | {t / n:.2f} |")
+    return "\n".join(out)
+
+
+def pmc(paths):
+    agg = collections.defaultdict(lambda: collections.defaultdict(float))
+    cnt = collections.defaultdict(collections.Counter)
+    for p in paths:
+        for r in csv.DictReader(open(p)):
+            key = (_short(r["Kernel_Name"]), r.get("Grid_Size", ""))
+            agg[key][r["Counter_Name"]] += float(r["Counter_Value"])
+            cnt[key][r["Counter_Name"]] += 1
+    names = sorted({c for v in agg.values() for c in v})
+    out = ["| kernel | grid (threads) | dispatches | " + " | ".join(f"{c} / dispatch" for c in names) + " |",
+           "|---|---|---|" + "---|" * len(names)]
+    res = {}
+    for key, v in sorted(agg.items(), key=lambda kv: -max(kv[1].values())):
+        n = max(cnt[key].values())
+        per = {c: v[c] / max(cnt[key][c], 1) for c in names if c in v}
+        res[f"{key[0]}|{key[1]}"] = per
+        out.append(f"| `{key[0]}` | {key[1]} | {n} | " + " | ".join(
+            f"{per[c]:.4g}" if c in per else "-" for c in names) + " |")
+    return "\n".join(out), res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("kind", choices=("trace", "pmc"))
+    ap.add_argument("paths", nargs="+")
+    ap.add_argument("--last-solve", action="store_true")
+    ap.add_argument("--json", default=None, help="pmc: also write the per-dispatch figures as JSON")
+    args = ap.parse_args()
+    if args.kind == "trace":
+        print(trace(args.paths[0], args.last_solve))
+    else:
+        text, res = pmc(args.paths)
+        print(text)
+        if args.json:
+            json.dump(res, open(args.json, "w"), indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -199,6 +199,16 @@ int32_t pp_tdma_batched(pp_ctx *ctx, const double *a, const double *b,
 int32_t pp_null_vector(pp_ctx *ctx, double *w);
 /* Level hierarchy: *n_levels and (cx,cy,cz) per level into dims[3*l]. */
 int32_t pp_level_info(pp_ctx *ctx, int32_t *n_levels, int32_t *dims);
+/* Kernel timing for roofline evidence (no counterpart in the reference):
+ * reps back-to-back launches of one multigrid kernel class (PP_K_XLINE,
+ * PP_K_YLINE, PP_K_ZLINE, PP_K_RESIDUAL) on one level in the given
+ * precision, bracketed by two HIP events on the context's stream;
+ * *avg_us = time per launch, *bytes = algorithmic bytes per launch.
+ * Overwrites that level's correction/residual work arrays (no effect on
+ * later solves, which start from a fresh correction). */
+int32_t pp_time_kernel(pp_ctx *ctx, int32_t kernel_class, int32_t level,
+                       int32_t precision, int32_t reps, double *avg_us,
+                       double *bytes);
 
 /* ---- geometry: body description -> pp_geometry inputs ---------------
  * Replaces set_boundary for canonical solid bodies (BOUNDARY_SET.F90:8-51):

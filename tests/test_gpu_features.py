@@ -126,3 +126,25 @@ def test_coarsest_level_is_solved_directly():
     p = np.zeros(ib.shape)
     st = s.solve(p, b, rel_tol=1e-10, max_outer=60)
     assert st["status"] == "PP_OK", st
+
+
+def test_time_kernel_reports_launch_time_and_bytes():
+    announce("pp_time_kernel on a 64x48x48 level 0")
+    xn = np.linspace(0, 4, 65)
+    yn = pp.tanh_nodes(48, 3.0, 2.0)
+    zn = pp.tanh_nodes(48, 3.0, 2.0)
+    ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, [pp.Ellipsoid((1.5, 1.5, 1.5), (0.5, 0.25, 0.25), 20.0, 48)])
+    s = pp.PressureSolver(xn, yn, zn)
+    s.set_geometry(ib, gs)
+    cells = 64 * 48 * 48
+    for cls, per_cell in (("xline", 0.5 * 13), ("yline", 0.5 * 13), ("zline", 0.5 * 13), ("residual", 13)):
+        us, b = s.time_kernel(cls, 0, pp.PP_PREC_MIXED, 10)
+        assert us > 0 and b == pytest.approx(cells * per_cell)
+    us, b = s.time_kernel("yline", 0, pp.PP_PREC_FP64, 4)
+    assert b == pytest.approx(cells * 0.5 * 25)
+    with pytest.raises(pp.PPError):
+        s.time_kernel("restrict", 0)
+    # a solve afterwards is unaffected
+    bb = cases.random_field(ib.shape, 3, ib)
+    p = np.zeros(ib.shape)
+    assert s.solve(p, bb, rel_tol=1e-10)["status"] == "PP_OK"
