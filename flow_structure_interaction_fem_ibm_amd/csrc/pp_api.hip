@@ -320,9 +320,8 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
     LevelArgs<T> a = largs<T>(L);
     const CoarseSolve &cs = c->cs;
     int n = cs.n;
-    kbegin(c, PP_K_COARSE, l, 8.0 * ((double)n * n + 2.0 * (double)n * cs.nch) + 2.0 * sizeof(T) * n);
-    k_coarse_mv<T><<<dim3((n + 255) / 256, cs.nch), 256, 0, st>>>(a, n, cs.pchunk, cs.G, lv_rhs<T>(L), cs.part);
-    k_coarse_sum<T><<<(n + 255) / 256, 256, 0, st>>>(a, n, cs.nch, cs.part, lv_e<T>(L));
+    kbegin(c, PP_K_COARSE, l, 8.0 * (double)n * n + 2.0 * sizeof(T) * n);
+    k_coarse_mv<T><<<(n + 3) / 4, 256, sizeof(double) * n, st>>>(n, cs.cell, cs.G, lv_rhs<T>(L), lv_e<T>(L));
     kend(c);
     return;
   }

@@ -122,10 +122,9 @@ struct GhostTables {
 };
 
 struct CoarseSolve {
-  int n = 0;                      // cells of the coarsest level
-  double *G = nullptr;            // n*n, G[p*n+q] = d e_q / d r_p
-  int pchunk = 64, nch = 0;       // rhs cells per partial product, partials per cell
-  double *part = nullptr;         // nch*n partial products
+  int n = 0;                      // cells of the coarsest level (at most 4096)
+  double *G = nullptr;            // n*n by output cell: G[q*n+p] = d e_q / d r_p
+  int *cell = nullptr;            // level-layout offset of coarse cell q (i fastest)
 };
 
 }  // namespace pp

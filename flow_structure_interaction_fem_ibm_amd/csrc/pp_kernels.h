@@ -986,39 +986,28 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
 
 // ---------------------------------------------------------------------
 // coarsest level (FP64 in both precisions): e = G r with the dense
-// solution operator G (bordered LU on the GPU at set-up: the coarse equations with the
-// fluid-volume mean constraint per connected component), G[p*n+q] =
-// response of cell q to a unit rhs at cell p.  Stage 1: workgroup (qb, pc)
-// forms the partial products of rhs cells [pc*pchunk, (pc+1)*pchunk) for
-// 256 consecutive q (coalesced rows of G); stage 2 adds the partials in
-// chunk order (deterministic) and writes the correction.
+// solution operator G of the coarsest level (bordered LU on the GPU at
+// set-up: the coarse equations with the fluid-volume mean constraint per
+// connected component), stored by output cell: GT[q*n + p] = response of
+// cell q to a unit rhs at cell p; cell[q] = offset of coarse cell q (i
+// fastest) in the level layout.  One wavefront per output cell streams
+// its row of GT (coalesced) against the rhs staged in LDS, then reduces
+// across the wavefront in a fixed order.
 // ---------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_coarse_mv(LevelArgs<T> L, int n, int pchunk, const double *__restrict__ G,
-                                                   const T *__restrict__ rhs, double *__restrict__ part) {
-  __shared__ double r[256];
-  const int q = blockIdx.x * 256 + threadIdx.x;
-  const int p0 = blockIdx.y * pchunk, p1 = min(n, p0 + pchunk);
-  for (int p = p0 + threadIdx.x; p < p1; p += 256) {
-    int i = p % L.cx, t = p / L.cx, j = t % L.cy, k = t / L.cy;
-    r[p - p0] = (double)rhs[lidx(L.sx, L.sxy, i, j, k)];
-  }
+__global__ void __launch_bounds__(256) k_coarse_mv(int n, const int *__restrict__ cell, const double *__restrict__ GT,
+                                                   const T *__restrict__ rhs, T *__restrict__ e) {
+  extern __shared__ unsigned char smem_raw[];
+  double *r = reinterpret_cast<double *>(smem_raw);
+  for (int p = threadIdx.x; p < n; p += 256) r[p] = (double)rhs[cell[p]];
   __syncthreads();
-  if (q >= n) return;
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (q >= n) return;  // uniform per wavefront
+  const double *row = GT + (long)q * n;
   double s = 0.0;
-  for (int p = p0; p < p1; p++) s += G[(long)p * n + q] * r[p - p0];
-  part[(long)blockIdx.y * n + q] = s;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_coarse_sum(LevelArgs<T> L, int n, int nch, const double *__restrict__ part,
-                                                    T *__restrict__ e) {
-  const int q = blockIdx.x * 256 + threadIdx.x;
-  if (q >= n) return;
-  double s = 0.0;
-  for (int c = 0; c < nch; c++) s += part[(long)c * n + q];
-  int i = q % L.cx, t = q / L.cx, j = t % L.cy, k = t / L.cy;
-  e[lidx(L.sx, L.sxy, i, j, k)] = (T)s;
+  for (int p = lane; p < n; p += 64) s += row[p] * r[p];
+  for (int d = 32; d >= 1; d >>= 1) s += __shfl_down(s, d, 64);
+  if (lane == 0) e[cell[q]] = (T)s;
 }
 
 // ---------------------------------------------------------------------

@@ -118,7 +118,7 @@ void free_ghosts(pp_ctx *c) {
                   g.orhsg64, g.otbase64};
   for (void *p : ptrs) hipFree(p);
   g = GhostTables();
-  hipFree(c->cs.G); hipFree(c->cs.part);
+  hipFree(c->cs.G); hipFree(c->cs.cell);
   c->cs = CoarseSolve();
 }
 
@@ -640,21 +640,23 @@ int build_coarse(pp_ctx *c) {
     }
   }
   if (rc == PP_OK) {
-    hipFree(c->cs.G); hipFree(c->cs.part);
-    c->cs = CoarseSolve();
-    c->cs.n = n;
-    c->cs.pchunk = 64;
-    c->cs.nch = (n + c->cs.pchunk - 1) / c->cs.pchunk;
-    hipMalloc(&c->cs.G, sizeof(double) * (size_t)n * n);
-    hipMalloc(&c->cs.part, sizeof(double) * (size_t)n * c->cs.nch);
-    if (!c->cs.G || !c->cs.part) {
-      set_error("coarse solve: device allocation failed");
+    // solution column p (response to a unit rhs at p) -> G[q*n + p]
+    std::vector<double> X((long)m * n), Gt((long)n * n);
+    if (hipMemcpy(X.data(), dB, sizeof(double) * X.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+      set_error("coarse solve: copy failed");
       rc = PP_HIP_ERROR;
     } else {
-      hipMemcpy2DAsync(c->cs.G, sizeof(double) * n, dB, sizeof(double) * m, sizeof(double) * n, n,
-                       hipMemcpyDeviceToDevice, c->stream);
-      if (hipStreamSynchronize(c->stream) != hipSuccess) {
-        set_error("coarse solve: copy failed");
+      for (int p = 0; p < n; p++)
+        for (int q = 0; q < n; q++) Gt[(long)q * n + p] = X[(long)p * m + q];
+      hipFree(c->cs.G); hipFree(c->cs.cell);
+      c->cs = CoarseSolve();
+      c->cs.n = n;
+      c->cs.G = dev_upload(Gt);
+      std::vector<int> off(n);
+      for (int q = 0; q < n; q++) off[q] = (int)lidx(L.sx, L.sxy, q % L.cx, (q / L.cx) % L.cy, q / (L.cx * L.cy));
+      c->cs.cell = dev_upload(off);
+      if (!c->cs.G || !c->cs.cell) {
+        set_error("coarse solve: device allocation failed");
         rc = PP_HIP_ERROR;
       }
     }

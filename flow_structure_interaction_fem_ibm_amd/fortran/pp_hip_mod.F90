@@ -1,0 +1,191 @@
+! pp_hip_mod.F90 -- ISO_C_BINDING interface of the MI355X pressure-Poisson
+! library (include/pp_hip.h) for the Fortran flow solver.
+!
+! The derived types mirror the C structs field by field (BIND(C) layout);
+! every routine returns the library's int32 status (PP_OK = 0).  Arrays are
+! passed as C_LOC of the solver's module arrays in their own layout
+! (0:nx+1, 0:ny+1, 0:nz+1), so no copies are made on the Fortran side.
+!
+! Replaces, as a drop-in behind solve_poisson's pp_solver_type switch
+! (CORE_POISSON.F90:566), the MG branch CORE_POISSON.F90:638-676.
+module pp_hip_mod
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int32_t), parameter, public :: PP_OK = 0, PP_NOT_CONVERGED = 1, PP_BAD_ARG = 2, &
+       PP_BAD_GEOMETRY = 3, PP_HIP_ERROR = 4, PP_NONFINITE = 5, PP_NO_GEOMETRY = 6
+  integer(c_int32_t), parameter, public :: PP_BC_NEUMANN = 2
+  integer(c_int32_t), parameter, public :: PP_PREC_FP64 = 0, PP_PREC_MIXED = 1
+  integer, parameter, public :: PP_K_CLASSES = 12
+
+  type, bind(c), public :: pp_grid_desc
+     integer(c_int32_t) :: nxc, nyc, nzc
+     type(c_ptr) :: x, y, z            ! node coordinates x(1..nxc+1)
+     integer(c_int32_t) :: bc(6)       ! pbcx1, pbcx2, pbcy1, pbcy2, pbcz1, pbcz2
+     integer(c_int32_t) :: device
+  end type pp_grid_desc
+
+  type, bind(c), public :: pp_geometry
+     type(c_ptr) :: iblank             ! INTEGER(1) (0:nx+1,0:ny+1,0:nz+1)
+     type(c_ptr) :: fresh_cell         ! INTEGER(1) or C_NULL_PTR
+     integer(c_int32_t) :: n_ghost
+     type(c_ptr) :: i_ghost, j_ghost, k_ghost   ! INTEGER(4) (nGhost)
+     type(c_ptr) :: i_cell, j_cell, k_cell      ! iCellIndex ... (nGhost)
+     type(c_ptr) :: coeff              ! coeffGCMN(8, nGhost) REAL(8)
+     type(c_ptr) :: probe_length       ! probeLength(nGhost) REAL(8)
+  end type pp_geometry
+
+  type, bind(c), public :: pp_solve_opts
+     integer(c_int32_t) :: precision
+     real(c_double) :: rel_tol
+     real(c_double) :: abs_linf_tol    ! restol_Poisson
+     integer(c_int32_t) :: max_outer   ! iterMax_Poisson
+     integer(c_int32_t) :: cycles_per_outer
+     integer(c_int32_t) :: nu_pre, nu_post
+     real(c_double) :: omega
+     integer(c_int32_t) :: mean_remove, project_rhs, use_graph, profile, krylov
+  end type pp_solve_opts
+
+  type, bind(c), public :: pp_solve_stats
+     integer(c_int32_t) :: status, outer_iterations, vcycles
+     real(c_double) :: rel_l2, abs_linf
+     integer(c_int32_t) :: loc(3)
+     real(c_double) :: b_l2, rhs_shift, mean_removed, ms, bytes
+     real(c_double) :: history(64)
+     real(c_double) :: kern_ms(PP_K_CLASSES), kern_bytes(PP_K_CLASSES)
+     integer(c_int32_t) :: kern_launches(PP_K_CLASSES)
+     real(c_double) :: kern_ms_l0(PP_K_CLASSES), kern_bytes_l0(PP_K_CLASSES)
+     integer(c_int32_t) :: kern_launches_l0(PP_K_CLASSES)
+  end type pp_solve_stats
+
+  public :: pp_default_opts, pp_last_error, pp_create, pp_destroy, pp_set_geometry, pp_solve, &
+       pp_solve_device, pp_adjoint, pp_calc_residual, pp_ghost_rows, pp_mean_remove, pp_divergence, &
+       pp_null_vector, pp_level_info, pp_error_text
+
+  interface
+     subroutine pp_default_opts(o) bind(c, name="pp_default_opts")
+       import :: pp_solve_opts
+       type(pp_solve_opts), intent(out) :: o
+     end subroutine pp_default_opts
+
+     function pp_last_error() bind(c, name="pp_last_error") result(p)
+       import :: c_ptr
+       type(c_ptr) :: p
+     end function pp_last_error
+
+     function pp_create(ctx, g) bind(c, name="pp_create") result(rc)
+       import :: c_ptr, c_int32_t, pp_grid_desc
+       type(c_ptr), intent(out) :: ctx
+       type(pp_grid_desc), intent(in) :: g
+       integer(c_int32_t) :: rc
+     end function pp_create
+
+     function pp_destroy(ctx) bind(c, name="pp_destroy") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t) :: rc
+     end function pp_destroy
+
+     function pp_set_geometry(ctx, geo) bind(c, name="pp_set_geometry") result(rc)
+       import :: c_ptr, c_int32_t, pp_geometry
+       type(c_ptr), value :: ctx
+       type(pp_geometry), intent(in) :: geo
+       integer(c_int32_t) :: rc
+     end function pp_set_geometry
+
+     function pp_solve(ctx, p_inout, rhs, bc_src, o, st) bind(c, name="pp_solve") result(rc)
+       import :: c_ptr, c_int32_t, pp_solve_opts, pp_solve_stats
+       type(c_ptr), value :: ctx, p_inout, rhs, bc_src
+       type(pp_solve_opts), intent(in) :: o
+       type(pp_solve_stats), intent(out) :: st
+       integer(c_int32_t) :: rc
+     end function pp_solve
+
+     function pp_solve_device(ctx, d_p, d_rhs, d_bc, o, st) bind(c, name="pp_solve_device") result(rc)
+       import :: c_ptr, c_int32_t, pp_solve_opts, pp_solve_stats
+       type(c_ptr), value :: ctx, d_p, d_rhs, d_bc
+       type(pp_solve_opts), intent(in) :: o
+       type(pp_solve_stats), intent(out) :: st
+       integer(c_int32_t) :: rc
+     end function pp_solve_device
+
+     function pp_adjoint(ctx, w, lambda, dj_dg, o, st) bind(c, name="pp_adjoint") result(rc)
+       import :: c_ptr, c_int32_t, pp_solve_opts, pp_solve_stats
+       type(c_ptr), value :: ctx, w, lambda, dj_dg
+       type(pp_solve_opts), intent(in) :: o
+       type(pp_solve_stats), intent(out) :: st
+       integer(c_int32_t) :: rc
+     end function pp_adjoint
+
+     function pp_calc_residual(ctx, var, r, res, resm, loc, l2) bind(c, name="pp_calc_residual") result(rc)
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, var, r, res
+       real(c_double), intent(out) :: resm, l2
+       integer(c_int32_t), intent(out) :: loc(3)
+       integer(c_int32_t) :: rc
+     end function pp_calc_residual
+
+     function pp_ghost_rows(ctx, p, bc_src) bind(c, name="pp_ghost_rows") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx, p, bc_src
+       integer(c_int32_t) :: rc
+     end function pp_ghost_rows
+
+     function pp_mean_remove(ctx, p, mean) bind(c, name="pp_mean_remove") result(rc)
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, p
+       real(c_double), intent(out) :: mean
+       integer(c_int32_t) :: rc
+     end function pp_mean_remove
+
+     function pp_divergence(ctx, fu, fv, fw, dtinv, nlu0, sum) bind(c, name="pp_divergence") result(rc)
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, fu, fv, fw, nlu0
+       real(c_double), value :: dtinv
+       real(c_double), intent(out) :: sum
+       integer(c_int32_t) :: rc
+     end function pp_divergence
+
+     function pp_null_vector(ctx, w) bind(c, name="pp_null_vector") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx, w
+       integer(c_int32_t) :: rc
+     end function pp_null_vector
+
+     function pp_level_info(ctx, n_levels, dims) bind(c, name="pp_level_info") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: n_levels
+       integer(c_int32_t), intent(out) :: dims(*)
+       integer(c_int32_t) :: rc
+     end function pp_level_info
+  end interface
+
+contains
+
+  ! pp_last_error() as a Fortran string (for the solver's PRINT statements)
+  function pp_error_text() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: n
+    interface
+       function c_strlen(str) bind(c, name="strlen") result(n)
+         import :: c_ptr, c_size_t
+         type(c_ptr), value :: str
+         integer(c_size_t) :: n
+       end function c_strlen
+    end interface
+    p = pp_last_error()
+    if (.not. c_associated(p)) then
+       msg = ""
+       return
+    end if
+    n = int(c_strlen(p))
+    call c_f_pointer(p, s, [n])
+    allocate(character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function pp_error_text
+
+end module pp_hip_mod
