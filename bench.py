@@ -107,6 +107,7 @@ def main():
     ap.add_argument("--nu", type=int, default=2)
     ap.add_argument("--cycles", type=int, default=1)
     ap.add_argument("--graph", action="store_true", help="replay the V-cycles as a captured hipGraph")
+    ap.add_argument("--nu-coarse", type=int, default=1, help="smoothing steps on the coarse levels (0: --nu)")
     ap.add_argument("--krylov", type=int, default=8,
                     help="flexible GMRES restart length around the V-cycles (0: plain refinement)")
     args = ap.parse_args()
@@ -127,7 +128,7 @@ def main():
     d_rhs = torch.from_numpy(rhs).to(dev)
     d_p = torch.zeros(solver.shape, dtype=torch.float64, device=dev)
     common = dict(rel_tol=args.tol, max_outer=200, nu_pre=args.nu, nu_post=args.nu,
-                  cycles_per_outer=args.cycles, krylov=args.krylov, use_graph=1 if args.graph else 0)
+                  cycles_per_outer=args.cycles, krylov=args.krylov, use_graph=1 if args.graph else 0, nu_coarse=args.nu_coarse)
 
     def run(prec, steps, warm, profile=False):
         for _ in range(warm):
@@ -213,7 +214,7 @@ def main():
                                "divergence RHS, solve to 1e-8 relative residual",
                    "cells": cells, "ghosts": gs.n, "levels": solver.level_info(),
                    "nu_pre": args.nu, "nu_post": args.nu, "cycles_per_outer": args.cycles,
-                   "krylov_restart": args.krylov, "hip_graph": bool(args.graph)},
+                   "krylov_restart": args.krylov, "hip_graph": bool(args.graph), "nu_coarse": args.nu_coarse},
         "fp64": {"solves_per_s": 1.0 / ms_fp64, "ms_per_solve": ms_fp64 * 1e3,
                  "outer_iterations": st_fp64[-1]["outer_iterations"], "rel_l2": st_fp64[-1]["rel_l2"]},
         "speedup_vs_fp64": ms_fp64 / ms_mixed,

@@ -326,7 +326,9 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
     return;
   }
   T om = (T)o->omega;
-  for (int s = 0; s < o->nu_pre; s++) smooth<T>(c, l, om);
+  const int npre = (l > 0 && o->nu_coarse > 0) ? o->nu_coarse : o->nu_pre;
+  const int npost = (l > 0 && o->nu_coarse > 0) ? o->nu_coarse : o->nu_post;
+  for (int s = 0; s < npre; s++) smooth<T>(c, l, om);
   LevelArgs<T> a = largs<T>(L);
   CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
   kbegin(c, PP_K_RESIDUAL, l, cells * (3.0 * sizeof(T) + 1.0));
@@ -344,7 +346,7 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   k_prolong<T><<<cl.grid, cl.block, 0, st>>>(a, ac, pdir<T>(L, 0), pdir<T>(L, 1), pdir<T>(L, 2), lv_e<T>(C),
                                               lv_e<T>(L));
   kend(c);
-  for (int s = 0; s < o->nu_post; s++) smooth<T>(c, l, om);
+  for (int s = 0; s < npost; s++) smooth<T>(c, l, om);
 }
 
 // The preconditioner: cycles_per_outer V-cycles on the level-0 rhs (e0 = 0
@@ -359,7 +361,7 @@ int precondition(pp_ctx *c, const pp_solve_opts *o, pp_solve_stats &S) {
     S.vcycles += o->cycles_per_outer;
     return PP_OK;
   }
-  long key = (long)sizeof(T) | ((long)o->nu_pre << 4) | ((long)o->nu_post << 12) |
+  long key = (long)sizeof(T) | ((long)o->nu_pre << 4) | ((long)o->nu_post << 12) | ((long)std::max(o->nu_coarse, 0) << 8) |
              ((long)o->cycles_per_outer << 20) | ((long)std::lround(o->omega * 1e6) << 28);
   if (!c->gexec || c->graph_key != key) {
     if (c->gexec) {
@@ -943,6 +945,7 @@ void pp_default_opts(pp_solve_opts *o) {
   o->use_graph = 0;
   o->profile = 0;
   o->krylov = 8;
+  o->nu_coarse = 1;
 }
 
 const char *pp_last_error(void) { return pp::last_error(); }

@@ -44,7 +44,7 @@ module pp_hip_mod
      integer(c_int32_t) :: cycles_per_outer
      integer(c_int32_t) :: nu_pre, nu_post
      real(c_double) :: omega
-     integer(c_int32_t) :: mean_remove, project_rhs, use_graph, profile, krylov
+     integer(c_int32_t) :: mean_remove, project_rhs, use_graph, profile, krylov, nu_coarse
   end type pp_solve_opts
 
   type, bind(c), public :: pp_solve_stats

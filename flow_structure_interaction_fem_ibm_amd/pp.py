@@ -72,7 +72,8 @@ class SolveOpts(C.Structure):
                 ("max_outer", C.c_int32), ("cycles_per_outer", C.c_int32),
                 ("nu_pre", C.c_int32), ("nu_post", C.c_int32), ("omega", C.c_double),
                 ("mean_remove", C.c_int32), ("project_rhs", C.c_int32), ("use_graph", C.c_int32),
-                ("profile", C.c_int32), ("krylov", C.c_int32)]
+                ("profile", C.c_int32), ("krylov", C.c_int32),
+                ("nu_coarse", C.c_int32)]
 
 
 KERNEL_CLASSES = ("outer_residual", "update", "xline", "yline", "zline", "residual", "restrict",

@@ -89,6 +89,9 @@ typedef struct pp_solve_opts {
                                 m in 1..16: FP64 flexible GMRES(m) with the
                                 V-cycles as right preconditioner (pp_solve only;
                                 pp_adjoint always refines) */
+  int32_t nu_coarse;         /* smoothing steps before and after the coarse
+                                correction on levels below the finest
+                                (<= 0: nu_pre / nu_post everywhere) */
 } pp_solve_opts;
 
 /* kernel classes of pp_solve_stats.kern_* (profile = 1) */
