@@ -343,8 +343,12 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   kend(c);
   vcycle<T>(c, l + 1, o);
   kbegin(c, PP_K_PROLONG, l, cells * (2.0 * sizeof(T) + 1.0) + (sizeof(T) + 1.0) * C.cx * C.cy * C.cz);
-  k_prolong<T><<<cl.grid, cl.block, 0, st>>>(a, ac, pdir<T>(L, 0), pdir<T>(L, 1), pdir<T>(L, 2), lv_e<T>(C),
-                                              lv_e<T>(L));
+  if (C.cf[0] == 2 && C.cf[1] == 2 && C.cf[2] == 2 && L.cx == 2 * C.cx && L.cy == 2 * C.cy && L.cz == 2 * C.cz)
+    k_prolong2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, pdir<T>(L, 0).wpar, pdir<T>(L, 1).wpar, pdir<T>(L, 2).wpar,
+                                                 lv_e<T>(C), lv_e<T>(L));
+  else
+    k_prolong<T><<<cl.grid, cl.block, 0, st>>>(a, ac, pdir<T>(L, 0), pdir<T>(L, 1), pdir<T>(L, 2), lv_e<T>(C),
+                                                lv_e<T>(L));
   kend(c);
   for (int s = 0; s < npost; s++) smooth<T>(c, l, om);
 }

@@ -607,6 +607,70 @@ __global__ void __launch_bounds__(256) k_prolong(LevelArgs<T> F, LevelArgs<T> C,
   if (den > T(0)) e_f[id] = e_f[id] + num / den;
 }
 
+// Prolongation + correction when every direction is coarsened by 2: one
+// thread per coarse cell (I,J,K) updates its 2x2x2 children, reading the
+// 3x3x3 coarse neighbourhood once (the children's interpolation partners
+// are I-1 / I+1 etc.; a partner outside the level carries weight 0).
+// Same weights, masks and renormalisation as k_prolong.
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong2(LevelArgs<T> F, LevelArgs<T> C, const T *__restrict__ wx,
+                                                  const T *__restrict__ wy, const T *__restrict__ wz,
+                                                  const T *__restrict__ e_c, T *__restrict__ e_f) {
+  using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
+  if (I >= C.cx || J >= C.cy) return;
+  T ev[3][3][3];
+  bool fl[3][3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        long q = lidx(C.sx, C.sxy, I + a - 1, J + b - 1, K + c - 1);
+        ev[c][b][a] = e_c[q];
+        fl[c][b][a] = !(C.fl[q] & FL_BLANK);
+      }
+  const T wxp[2] = {wx[2 * I], wx[2 * I + 1]};
+  const T wyp[2] = {wy[2 * J], wy[2 * J + 1]};
+  const T wzp[2] = {wz[2 * K], wz[2 * K + 1]};
+#pragma unroll
+  for (int kc = 0; kc < 2; kc++)
+#pragma unroll
+    for (int jc = 0; jc < 2; jc++) {
+      long id = lidx(F.sx, F.sxy, 2 * I, 2 * J + jc, 2 * K + kc);
+      V2 ef = *reinterpret_cast<const V2 *>(e_f + id);
+      uint16_t f2 = *reinterpret_cast<const uint16_t *>(F.fl + id);
+      T out[2] = {ef.x, ef.y};
+#pragma unroll
+      for (int ic = 0; ic < 2; ic++) {
+        if ((f2 >> (8 * ic)) & FL_BLANK) continue;
+        // partners: parent (index 1) and the neighbour on the child's side
+        const int oa = ic ? 2 : 0, ob = jc ? 2 : 0, oc = kc ? 2 : 0;
+        const T wi[2] = {wxp[ic], T(1) - wxp[ic]}, wj[2] = {wyp[jc], T(1) - wyp[jc]},
+                wk[2] = {wzp[kc], T(1) - wzp[kc]};
+        const int ai[2] = {1, oa}, bj[2] = {1, ob}, ck[2] = {1, oc};
+        T num = T(0), den = T(0);
+#pragma unroll
+        for (int z = 0; z < 2; z++)
+#pragma unroll
+          for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int x = 0; x < 2; x++) {
+              T w = wi[x] * wj[y] * wk[z];
+              if (w == T(0) || !fl[ck[z]][bj[y]][ai[x]]) continue;
+              num += w * ev[ck[z]][bj[y]][ai[x]];
+              den += w;
+            }
+        if (den > T(0)) out[ic] = out[ic] + num / den;
+      }
+      ef.x = out[0];
+      ef.y = out[1];
+      *reinterpret_cast<V2 *>(e_f + id) = ef;
+    }
+}
+
 // ---------------------------------------------------------------------
 // reciprocals: v_rcp_f32 (1 ulp) for FP32, v_rcp_f64 refined by two
 // Newton steps for FP64 (the line solves are smoothers inside an FP64

@@ -641,8 +641,10 @@ int build_coarse(pp_ctx *c) {
   }
   if (rc == PP_OK) {
     // solution column p (response to a unit rhs at p) -> G[q*n + p]
+    // the solves run on the context's (non-blocking) stream: copy on it
     std::vector<double> X((long)m * n), Gt((long)n * n);
-    if (hipMemcpy(X.data(), dB, sizeof(double) * X.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpyAsync(X.data(), dB, sizeof(double) * X.size(), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
       set_error("coarse solve: copy failed");
       rc = PP_HIP_ERROR;
     } else {
