@@ -225,7 +225,7 @@ def main():
         "error_vs_fp64_solve": err,
         "error_vs_fp64_converged": rel(p_mixed, p_ref),
         "error_fp64_solve_vs_fp64_converged": rel(p_fp64, p_ref),
-        "precision_error_at_1e-12": rel(p_mt, p_ref),
+        "mixed_vs_fp64_both_at_1e-12": rel(p_mt, p_ref),
         "converged_refs": {"fp64_rel_l2": st_ref["rel_l2"], "fp64_outer": st_ref["outer_iterations"],
                            "mixed_rel_l2": st_mt["rel_l2"], "mixed_outer": st_mt["outer_iterations"]},
         "roofline": {"bound": "hbm", "kernel": f"{dom} line pass, finest level" if dom.endswith("line") else dom,

@@ -513,6 +513,22 @@ int ensure_krylov(pp_ctx *c, int m) {
   return PP_OK;
 }
 
+int ensure_krylov_level(pp_ctx *c, int m) {
+  if (c->kryl_m >= m) return PP_OK;
+  for (float *q : c->d_kryl) hipFree(q);
+  c->d_kryl.clear();
+  c->kryl_m = 0;
+  size_t nb = sizeof(float) * (size_t)c->lv[0].total;
+  for (int q = 0; q < 2 * m + 1; q++) {
+    float *d = nullptr;
+    HIPCHK(hipMalloc(&d, nb));
+    c->d_kryl.push_back(d);
+    HIPCHK(hipMemset(d, 0, nb));
+  }
+  c->kryl_m = m;
+  return PP_OK;
+}
+
 // sum of nq per-block partials -> c->h_res[0..nq)
 int reduce_sums_host(pp_ctx *c, int nq) {
   launch_reduce_sums(c, c->nblk_red, nq);
@@ -529,18 +545,86 @@ int reduce_sums_host(pp_ctx *c, int nq) {
 // updates the Givens QR of the Hessenberg matrix.  Stops at m steps, at the
 // iteration limit or when the least-squares residual meets the tolerance;
 // then p += Z y and the ghost rows are made exact.
+// Krylov vector storage: FP64 in the reference layout (FP64 V-cycles) or
+// FP32 in the level-0 layout (FP32 V-cycles).
+template <typename T>
+struct KryStore;
+template <>
+struct KryStore<double> {
+  using V = KryVecs;
+  using E = double;
+  static E *vec(pp_ctx *c, int q) { return c->d_kry[q]; }
+  static E *zvec(pp_ctx *c, int q) { return c->d_kry[c->kry_m + 1 + q]; }
+  // z_j from the V-cycle's correction (ghost rows solved in FP64), w = A z_j
+  static void apply(pp_ctx *c, Level &L0, CellLaunch cl, double cells, int j, E *z, E *w, const V &vv) {
+    LevelArgs<double> a0 = largs<double>(L0);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (8.0 + 1.0 + 8.0));
+    k_kry_z<double><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, a0, lv_e<double>(L0), z);
+    kend(c);
+    ghost_exact_ref(c, z, false);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1)));
+    k_kry_apply<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, a0, z, w, vv, c->d_part);
+    kend(c);
+  }
+  static void orth(pp_ctx *c, Level &L0, CellLaunch cl, double cells, int j, E *w, const V &vv) {
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1) + 16.0));
+    k_kry_orth<double, 256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(L0), w, vv,
+                                                                  lv_rhs<double>(L0), lv_e<double>(L0), c->d_part);
+    kend(c);
+  }
+  static void update(pp_ctx *c, Level &L0, CellLaunch cl, double cells, double *p, const V &vv) {
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * vv.n));
+    k_kry_update<<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(L0), p, vv);
+    kend(c);
+  }
+};
+template <>
+struct KryStore<float> {
+  using V = KryVecsF;
+  using E = float;
+  static E *vec(pp_ctx *c, int q) { return c->d_kryl[q]; }
+  static E *zvec(pp_ctx *c, int q) { return c->d_kryl[c->kryl_m + 1 + q]; }
+  static void apply(pp_ctx *c, Level &L0, CellLaunch cl, double cells, int j, E *z, E *w, const V &vv) {
+    ghost_sweep_level0<float>(c, lv_e<float>(L0));
+    kbegin(c, PP_K_KRYLOV, 0, cells * (4.0 + 1.0 + 4.0 + 4.0 + 4.0 * (j + 1)));
+    k_kryl_apply<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), lv_e<float>(L0), z, w, vv,
+                                                           c->d_part);
+    kend(c);
+  }
+  static void orth(pp_ctx *c, Level &L0, CellLaunch cl, double cells, int j, E *w, const V &vv) {
+    kbegin(c, PP_K_KRYLOV, 0, cells * (1.0 + 8.0 + 4.0 * (j + 1) + 8.0));
+    k_kryl_orth<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), w, vv, lv_rhs<float>(L0),
+                                                          lv_e<float>(L0), c->d_part);
+    kend(c);
+  }
+  static void update(pp_ctx *c, Level &L0, CellLaunch cl, double cells, double *p, const V &vv) {
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 4.0 * vv.n));
+    k_kryl_update<<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(L0), p, vv);
+    kend(c);
+  }
+};
+
+// One restart cycle.  On entry the first basis vector holds r = b - A p
+// (fluid cells) and the level-0 rhs holds r in T with e0 = 0; beta = |r|.
+// Each step applies the V-cycle(s) to the current basis vector (right
+// preconditioning), keeps the preconditioned vector z_j (flexible variant:
+// the V-cycle in FP32 is not exactly linear), orthogonalises A z_j by
+// classical Gram-Schmidt and updates the Givens QR of the Hessenberg
+// matrix.  Stops at m steps, at the iteration limit or when the
+// least-squares residual meets the tolerance; then p += Z y and the ghost
+// rows are made exact.
 template <typename T>
 int krylov_cycle(pp_ctx *c, const pp_solve_opts *o, double *p, bool have_bnd, double beta, double bden,
                  int &outer, pp_solve_stats &S) {
-  hipStream_t s = c->stream;
+  using St = KryStore<T>;
+  using E = typename St::E;
   Level &L0 = c->lv[0];
-  LevelArgs<double> a0 = largs<double>(L0);
   CellLaunch cl = ref_launch(c);
   double cells = (double)c->nxc * c->nyc * c->nzc;
   const int m = o->krylov;
-  double *U[KRY_MAX + 1], *Z[KRY_MAX];
-  for (int q = 0; q <= m; q++) U[q] = c->d_kry[q];
-  for (int q = 0; q < m; q++) Z[q] = c->d_kry[c->kry_m + 1 + q];
+  E *U[KRY_MAX + 1], *Z[KRY_MAX];
+  for (int q = 0; q <= m; q++) U[q] = St::vec(c, q);
+  for (int q = 0; q < m; q++) Z[q] = St::zvec(c, q);
   std::vector<double> H((m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), gv(m + 1, 0.0), sig(m + 1, 0.0);
   auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
   sig[0] = 1.0 / beta;
@@ -549,16 +633,10 @@ int krylov_cycle(pp_ctx *c, const pp_solve_opts *o, double *p, bool have_bnd, do
   for (int j = 0; j < m && outer < o->max_outer; j++) {
     int rc = precondition<T>(c, o, S);
     if (rc) return rc;
-    kbegin(c, PP_K_KRYLOV, 0, cells * (sizeof(T) + 1.0 + 8.0));
-    k_kry_z<T><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, lv_e<T>(L0), Z[j]);
-    kend(c);
-    ghost_exact_ref(c, Z[j], false);
-    KryVecs V{};
+    typename St::V V{};
     V.n = j + 1;
     for (int q = 0; q <= j; q++) V.u[q] = U[q];
-    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1)));
-    k_kry_apply<256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, Z[j], U[j + 1], V, c->d_part);
-    kend(c);
+    St::apply(c, L0, cl, cells, j, Z[j], U[j + 1], V);
     rc = reduce_sums_host(c, j + 1);
     if (rc) return rc;
     for (int q = 0; q <= j; q++) {
@@ -566,9 +644,7 @@ int krylov_cycle(pp_ctx *c, const pp_solve_opts *o, double *p, bool have_bnd, do
       h(q, j) = sig[j] * sig[q] * d;
       V.c[q] = sig[q] * sig[q] * d;
     }
-    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1) + 2.0 * sizeof(T)));
-    k_kry_orth<T, 256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, U[j + 1], V, lv_rhs<T>(L0), lv_e<T>(L0), c->d_part);
-    kend(c);
+    St::orth(c, L0, cl, cells, j, U[j + 1], V);
     rc = reduce_sums_host(c, 1);
     if (rc) return rc;
     double hn = std::sqrt(c->h_res[0]);
@@ -608,15 +684,13 @@ int krylov_cycle(pp_ctx *c, const pp_solve_opts *o, double *p, bool have_bnd, do
     for (int q = i + 1; q < k; q++) t -= h(i, q) * y[q];
     y[i] = t / h(i, i);
   }
-  KryVecs V{};
+  typename St::V V{};
   V.n = k;
   for (int q = 0; q < k; q++) {
     V.u[q] = Z[q];
     V.c[q] = y[q] * sig[q];
   }
-  kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * k));
-  k_kry_update<<<cl.grid, cl.block, 0, s>>>(c->ref, a0, p, V);
-  kend(c);
+  St::update(c, L0, cl, cells, p, V);
   ghost_exact_ref(c, p, have_bnd);
   return PP_OK;
 }
@@ -642,8 +716,13 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
   HIPCHK(hipEventRecord(ev0, s));
   double cells = (double)c->nxc * c->nyc * c->nzc;
   const int m = o->krylov;
-  if (m > 0) {
+  const bool kry64 = m > 0 && sizeof(T) == 8, kry32 = m > 0 && sizeof(T) == 4;
+  if (kry64) {
     int rc = ensure_krylov(c, m);
+    if (rc) return rc;
+  }
+  if (kry32) {
+    int rc = ensure_krylov_level(c, m);
     if (rc) return rc;
   }
   // ghost Neumann data: bnd = (c_self + probeLength) * uNormal
@@ -705,7 +784,8 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
     // r = b - A p -> level-0 rhs (T), e0 = 0, norms
     kbegin(c, PP_K_OUTER_RESIDUAL, 0, cells * (17.0 + 2.0 * sizeof(T)));
     k_outer_residual<T, 256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, p, c->d_b, lv_rhs<T>(L0), lv_e<T>(L0),
-                                                          c->d_part, m > 0 ? c->d_kry[0] : nullptr);
+                                                          c->d_part, kry64 ? c->d_kry[0] : nullptr,
+                                                          kry32 ? (T *)c->d_kryl[0] : (T *)nullptr);
     kend(c);
     int rc = reduce_summax(c, (int)nblk);
     if (rc) return rc;
@@ -816,7 +896,7 @@ int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref
     }
     kbegin(c, PP_K_OUTER_RESIDUAL, 0, cells * (17.0 + 2.0 * sizeof(T)));
     k_outer_residual<T, 256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, c->d_bwork, lv_rhs<T>(L0),
-                                                          lv_e<T>(L0), c->d_part, nullptr);
+                                                          lv_e<T>(L0), c->d_part, nullptr, (T *)nullptr);
     kend(c);
     int rc = reduce_summax(c, (int)nblk);
     if (rc) return rc;
@@ -1035,6 +1115,7 @@ int32_t pp_destroy(pp_ctx *c) {
   hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_mu); hipFree(c->d_sv);
   hipFree(c->d_part); hipFree(c->d_part2); hipFree(c->d_res);
   for (double *q : c->d_kry) hipFree(q);
+  for (float *q : c->d_kryl) hipFree(q);
   for (int d = 0; d < 3; d++) { hipFree(c->d_rdx[d]); hipFree(c->d_rdxinv[d]); }
   if (c->h_res) hipHostFree(c->h_res);
   if (c->gexec) hipGraphExecDestroy(c->gexec);

@@ -158,8 +158,10 @@ struct pp_ctx {
   bool adjoint_mode = false;
   int null_iters = 0;
   double null_res = 0.0;           // level-0 smoother uses the transposed ghost rows
-  std::vector<double *> d_kry;         // flexible GMRES: m+1 basis + m preconditioned vectors
+  std::vector<double *> d_kry;         // flexible GMRES (FP64 path): m+1 basis + m preconditioned vectors
   int kry_m = 0;                       // restart length d_kry is sized for
+  std::vector<float *> d_kryl;         // the same for the mixed path: FP32, level-0 layout
+  int kryl_m = 0;
   double *d_mu = nullptr;              // adjoint unknown lambda/V (reference layout)
   double *d_sv = nullptr;              // adjoint rhs s/V (reference layout)
   // graph cache

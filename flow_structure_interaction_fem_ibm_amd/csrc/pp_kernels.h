@@ -190,7 +190,7 @@ template <typename TE, int NT>
 __global__ void __launch_bounds__(NT) k_outer_residual(RefArgs R, LevelArgs<double> L, const double *__restrict__ p,
                                                        const double *__restrict__ b, TE *__restrict__ rhs0,
                                                        TE *__restrict__ e0, double *__restrict__ part,
-                                                       double *__restrict__ rout) {
+                                                       double *__restrict__ rout, TE *__restrict__ u0) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
@@ -208,6 +208,7 @@ __global__ void __launch_bounds__(NT) k_outer_residual(RefArgs R, LevelArgs<doub
     rhs0[a] = (TE)res;
     e0[a] = TE(0);
     if (rout) rout[r] = res;
+    if (u0) u0[a] = (TE)res;
     v.sum = res * res;
     v.mx = fabs(res);
     v.arg = (long)r;
@@ -1170,6 +1171,102 @@ __global__ void __launch_bounds__(256) k_kry_update(RefArgs R, LevelArgs<double>
 #pragma unroll
   for (int q = 0; q < KRY_MAX + 1; q++)
     if (q < V.n) acc += V.c[q] * V.u[q][r];
+  p[r] = p[r] + acc;
+}
+
+// ---------------------------------------------------------------------
+// Flexible GMRES for the mixed-precision path: basis and preconditioned
+// vectors stored in FP32 in the level-0 layout (half the bytes of the FP64
+// reference-layout vectors); products, dots and coefficients in FP64 on the
+// stored values, so the Arnoldi relation holds for the vectors as stored.
+// The outer FP64 residual at every restart keeps the solve exact to FP64.
+// ---------------------------------------------------------------------
+struct KryVecsF {
+  const float *u[KRY_MAX + 1];
+  double c[KRY_MAX + 1];
+  int n;
+};
+
+// z = e0 (the V-cycle's correction, ghost rows solved), w = A z on fluid
+// cells (FP64 coefficients), partial dots <w, u_q>
+template <int NT>
+__global__ void __launch_bounds__(NT) k_kryl_apply(LevelArgs<double> L, const float *__restrict__ e0,
+                                                   float *__restrict__ z, float *__restrict__ w, KryVecsF V,
+                                                   double *__restrict__ part) {
+  __shared__ double s[NT / 64];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  bool in = i < L.cx && j < L.cy;
+  double wv = 0.0;
+  long a = 0;
+  if (in) {
+    a = lidx(L.sx, L.sxy, i, j, k);
+    float zc = e0[a];
+    z[a] = zc;
+    uint8_t f = L.fl[a];
+    if (!(f & FL_BLANK)) {
+      Row<double> rw = make_row(L, f, i, j, k);
+      wv = (double)zc * rw.diag + (double)e0[a - 1] * rw.amx + (double)e0[a + 1] * rw.apx +
+           (double)e0[a - L.sx] * rw.amy + (double)e0[a + L.sx] * rw.apy + (double)e0[a - L.sxy] * rw.amz +
+           (double)e0[a + L.sxy] * rw.apz;
+    }
+    float wf = (float)wv;
+    w[a] = wf;
+    wv = (double)wf;
+  }
+  long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+#pragma unroll
+  for (int q = 0; q < KRY_MAX + 1; q++) {
+    if (q < V.n) {
+      double d = block_sum<NT>(in ? wv * (double)V.u[q][a] : 0.0, s);
+      if (threadIdx.x == 0 && threadIdx.y == 0) part[blk * V.n + q] = d;
+    }
+  }
+}
+
+// u = w - sum_q c_q u_q on fluid cells (in place in w), level-0 rhs = u,
+// e0 = 0, partial |u|^2 of the stored values
+template <int NT>
+__global__ void __launch_bounds__(NT) k_kryl_orth(LevelArgs<double> L, float *__restrict__ w, KryVecsF V,
+                                                  float *__restrict__ rhs0, float *__restrict__ e0,
+                                                  double *__restrict__ part) {
+  __shared__ double s[NT / 64];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  double u2 = 0.0;
+  if (i < L.cx && j < L.cy) {
+    long a = lidx(L.sx, L.sxy, i, j, k);
+    float uf = 0.0f;
+    if (!(L.fl[a] & FL_BLANK)) {
+      double u = (double)w[a];
+#pragma unroll
+      for (int q = 0; q < KRY_MAX + 1; q++)
+        if (q < V.n) u -= V.c[q] * (double)V.u[q][a];
+      uf = (float)u;
+      w[a] = uf;
+    }
+    rhs0[a] = uf;
+    e0[a] = 0.0f;
+    u2 = (double)uf * (double)uf;
+  }
+  double d = block_sum<NT>(u2, s);
+  long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[blk] = d;
+}
+
+// p (FP64, reference layout) += sum_q c_q z_q (FP32, level layout) on fluid cells
+__global__ void __launch_bounds__(256) k_kryl_update(RefArgs R, LevelArgs<double> L, double *__restrict__ p,
+                                                     KryVecsF V) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= R.nxc || j >= R.nyc) return;
+  long a = lidx(L.sx, L.sxy, i, j, k);
+  if (L.fl[a] & FL_BLANK) return;
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < KRY_MAX + 1; q++)
+    if (q < V.n) acc += V.c[q] * (double)V.u[q][a];
+  long r = ridx(R, i, j, k);
   p[r] = p[r] + acc;
 }
 
