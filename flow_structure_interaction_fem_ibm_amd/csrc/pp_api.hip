@@ -311,6 +311,19 @@ void smooth(pp_ctx *c, int l, T omega) {
   for (int col = 0; col < 2; col++) launch_yzline<T, 2>(c, L, l, col, omega);
 }
 
+// level residual res = rhs - L e (four cells per thread when rows allow)
+template <typename T>
+void launch_residual(pp_ctx *c, Level &L) {
+  LevelArgs<T> a = largs<T>(L);
+  if (L.cx % 4 == 0) {
+    CellLaunch q = cell_launch(L.cx / 4, L.cy, L.cz);
+    k_residual4<T><<<q.grid, q.block, 0, c->stream>>>(a, lv_e<T>(L), lv_rhs<T>(L), lv_res<T>(L));
+  } else {
+    CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
+    k_residual<T><<<cl.grid, cl.block, 0, c->stream>>>(a, lv_e<T>(L), lv_rhs<T>(L), lv_res<T>(L));
+  }
+}
+
 template <typename T>
 void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   Level &L = c->lv[l];
@@ -332,18 +345,23 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   LevelArgs<T> a = largs<T>(L);
   CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
   kbegin(c, PP_K_RESIDUAL, l, cells * (3.0 * sizeof(T) + 1.0));
-  k_residual<T><<<cl.grid, cl.block, 0, st>>>(a, lv_e<T>(L), lv_rhs<T>(L), lv_res<T>(L));
+  launch_residual<T>(c, L);
   kend(c);
   Level &C = c->lv[l + 1];
   LevelArgs<T> ac = largs<T>(C);
   CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
   kbegin(c, PP_K_RESTRICT, l, cells * (sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz);
-  k_restrict<T><<<cc.grid, cc.block, 0, st>>>(a, ac, C.cf[0], C.cf[1], C.cf[2], lv_res<T>(L), lv_rhs<T>(C),
-                                               lv_e<T>(C));
+  const bool by2 = C.cf[0] == 2 && C.cf[1] == 2 && C.cf[2] == 2 && L.cx == 2 * C.cx && L.cy == 2 * C.cy &&
+                   L.cz == 2 * C.cz;
+  if (by2)
+    k_restrict2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, lv_res<T>(L), lv_rhs<T>(C), lv_e<T>(C));
+  else
+    k_restrict<T><<<cc.grid, cc.block, 0, st>>>(a, ac, C.cf[0], C.cf[1], C.cf[2], lv_res<T>(L), lv_rhs<T>(C),
+                                                 lv_e<T>(C));
   kend(c);
   vcycle<T>(c, l + 1, o);
   kbegin(c, PP_K_PROLONG, l, cells * (2.0 * sizeof(T) + 1.0) + (sizeof(T) + 1.0) * C.cx * C.cy * C.cz);
-  if (C.cf[0] == 2 && C.cf[1] == 2 && C.cf[2] == 2 && L.cx == 2 * C.cx && L.cy == 2 * C.cy && L.cz == 2 * C.cz)
+  if (by2)
     k_prolong2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, pdir<T>(L, 0).wpar, pdir<T>(L, 1).wpar, pdir<T>(L, 2).wpar,
                                                  lv_e<T>(C), lv_e<T>(L));
   else
@@ -1308,7 +1326,7 @@ int time_kernel(pp_ctx *c, int cls, int level, int reps, double *avg_us, double 
       case PP_K_ZLINE: launch_yzline<T, 2>(c, L, level, r & 1, T(1)); break;
       default:
         kbegin(c, PP_K_RESIDUAL, level, (double)L.cx * L.cy * L.cz * (3.0 * sizeof(T) + 1.0));
-        k_residual<T><<<cl.grid, cl.block, 0, st>>>(a, lv_e<T>(L), lv_rhs<T>(L), lv_res<T>(L));
+        launch_residual<T>(c, L);
         kend(c);
         break;
     }

@@ -28,6 +28,62 @@ __device__ inline float shfl_up_d(float v, int d) { return __shfl_up(v, d, 64); 
 __device__ inline double shfl_down_d(double v, int d) { return __shfl_down(v, d, 64); }
 __device__ inline float shfl_down_d(float v, int d) { return __shfl_down(v, d, 64); }
 
+// chunk loads/stores of S consecutive elements (vectorised when the chunk
+// is whole and 16-byte aligned)
+template <typename T, int S>
+__device__ inline void load_chunk(const T *__restrict__ p, int nvalid, T (&v)[S]) {
+  constexpr int VE = 16 / sizeof(T);
+  if (S % VE == 0 && nvalid == S) {
+    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+#pragma unroll
+    for (int q = 0; q < S / VE; q++) {
+      V w = reinterpret_cast<const V *>(p)[q];
+      if constexpr (sizeof(T) == 4) {
+        v[4 * q] = w.x; v[4 * q + 1] = w.y; v[4 * q + 2] = w.z; v[4 * q + 3] = w.w;
+      } else {
+        v[2 * q] = w.x; v[2 * q + 1] = w.y;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; s++) v[s] = s < nvalid ? p[s] : T(0);
+  }
+}
+template <typename T, int S>
+__device__ inline void store_chunk(T *__restrict__ p, int nvalid, const T (&v)[S]) {
+  constexpr int VE = 16 / sizeof(T);
+  if (S % VE == 0 && nvalid == S) {
+    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+#pragma unroll
+    for (int q = 0; q < S / VE; q++) {
+      V w;
+      if constexpr (sizeof(T) == 4) {
+        w.x = v[4 * q]; w.y = v[4 * q + 1]; w.z = v[4 * q + 2]; w.w = v[4 * q + 3];
+      } else {
+        w.x = v[2 * q]; w.y = v[2 * q + 1];
+      }
+      reinterpret_cast<V *>(p)[q] = w;
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; s++)
+      if (s < nvalid) p[s] = v[s];
+  }
+}
+template <int S>
+__device__ inline void load_flags(const uint8_t *__restrict__ p, int nvalid, uint8_t (&f)[S]) {
+  if (S % 4 == 0 && nvalid == S) {
+#pragma unroll
+    for (int q = 0; q < S / 4; q++) {
+      uint32_t w = reinterpret_cast<const uint32_t *>(p)[q];
+      f[4 * q] = w & 0xff; f[4 * q + 1] = (w >> 8) & 0xff; f[4 * q + 2] = (w >> 16) & 0xff; f[4 * q + 3] = w >> 24;
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; s++) f[s] = s < nvalid ? p[s] : uint8_t(FL_BLANK);
+  }
+}
+
 // One row of the level operator for cell id with flag f.  Fluid rows are
 // the stretched 7-point Laplacian (MG_itsolv CORE_MG.F90:1945-1968); blank
 // rows are identity holding the ghost value (rhs = ghostMark*var,
@@ -536,6 +592,45 @@ __global__ void __launch_bounds__(256) k_residual(LevelArgs<T> L, const T *__res
   res[a] = v;
 }
 
+// The same residual, four consecutive cells per thread (vector loads and
+// stores; rows whose interior starts 16-byte aligned and cx % 4 == 0).
+template <typename T>
+__global__ void __launch_bounds__(256) k_residual4(LevelArgs<T> L, const T *__restrict__ e, const T *__restrict__ rhs,
+                                                   T *__restrict__ res) {
+  const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i0 >= L.cx || j >= L.cy) return;
+  const long a = lidx(L.sx, L.sxy, i0, j, k);
+  T ec[4], ejm[4], ejp[4], ekm[4], ekp[4], rv[4], cmx[4], cpx[4], out[4];
+  uint8_t fl[4];
+  load_chunk<T, 4>(e + a, 4, ec);
+  load_chunk<T, 4>(e + a - L.sx, 4, ejm);
+  load_chunk<T, 4>(e + a + L.sx, 4, ejp);
+  load_chunk<T, 4>(e + a - L.sxy, 4, ekm);
+  load_chunk<T, 4>(e + a + L.sxy, 4, ekp);
+  load_chunk<T, 4>(rhs + a, 4, rv);
+  load_chunk<T, 4>(L.cmx + i0, 4, cmx);
+  load_chunk<T, 4>(L.cpx + i0, 4, cpx);
+  load_flags<4>(L.fl + a, 4, fl);
+  const T el = e[a - 1], er = e[a + 4];
+  const T my = L.cmy[j], py = L.cpy[j], mz = L.cmz[k], pz = L.cpz[k];
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    uint8_t f = fl[s];
+    T v = T(0);
+    if (!(f & FL_BLANK)) {
+      T amx = (f & FL_IM) ? T(0) : cmx[s], apx = (f & FL_IP) ? T(0) : cpx[s];
+      T amy = (f & FL_JM) ? T(0) : my, apy = (f & FL_JP) ? T(0) : py;
+      T amz = (f & FL_KM) ? T(0) : mz, apz = (f & FL_KP) ? T(0) : pz;
+      T diag = (-(amx + apx) + -(amy + apy)) + -(amz + apz);
+      T left = s == 0 ? el : ec[s - 1], right = s == 3 ? er : ec[s + 1];
+      v = rv[s] - ec[s] * diag - left * amx - right * apx - ejm[s] * amy - ejp[s] * apy - ekm[s] * amz - ekp[s] * apz;
+    }
+    out[s] = v;
+  }
+  store_chunk<T, 4>(res + a, 4, out);
+}
+
 // ---------------------------------------------------------------------
 // restriction: coarse rhs = fluid-volume-weighted mean of the children's
 // residuals (the reference's masked averaging, MG_Restrict_X/Y/Z
@@ -567,6 +662,44 @@ __global__ void __launch_bounds__(256) k_restrict(LevelArgs<T> F, LevelArgs<T> C
     }
   }
   long cid = lidx(C.sx, C.sxy, I, J, K);
+  rhs_c[cid] = den > T(0) ? num / den : T(0);
+  e_c[cid] = T(0);
+}
+
+// The same restriction when every direction is coarsened by 2: the two
+// x-children of a row are one vector load (same summation order).
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict2(LevelArgs<T> F, LevelArgs<T> C, const T *__restrict__ res,
+                                                   T *__restrict__ rhs_c, T *__restrict__ e_c) {
+  using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
+  if (I >= C.cx || J >= C.cy) return;
+  const V2 wx = *reinterpret_cast<const V2 *>(F.wx + 2 * I);
+  T num = T(0), den = T(0);
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const int k = 2 * K + c;
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int j = 2 * J + b;
+      const T wyz = F.wy[j] * F.wz[k];
+      const long id = lidx(F.sx, F.sxy, 2 * I, j, k);
+      const V2 r = *reinterpret_cast<const V2 *>(res + id);
+      const uint16_t f2 = *reinterpret_cast<const uint16_t *>(F.fl + id);
+      if (!(f2 & FL_BLANK)) {
+        T v = wx.x * wyz;
+        num += v * r.x;
+        den += v;
+      }
+      if (!((f2 >> 8) & FL_BLANK)) {
+        T v = wx.y * wyz;
+        num += v * r.y;
+        den += v;
+      }
+    }
+  }
+  const long cid = lidx(C.sx, C.sxy, I, J, K);
   rhs_c[cid] = den > T(0) ? num / den : T(0);
   e_c[cid] = T(0);
 }
@@ -801,62 +934,6 @@ struct SegElim {
     combine(from_lower<G, 1>(X, T(0)), X);
   }
 };
-
-// chunk loads/stores of S consecutive elements (vectorised when the chunk
-// is whole and 16-byte aligned)
-template <typename T, int S>
-__device__ inline void load_chunk(const T *__restrict__ p, int nvalid, T (&v)[S]) {
-  constexpr int VE = 16 / sizeof(T);
-  if (S % VE == 0 && nvalid == S) {
-    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
-#pragma unroll
-    for (int q = 0; q < S / VE; q++) {
-      V w = reinterpret_cast<const V *>(p)[q];
-      if constexpr (sizeof(T) == 4) {
-        v[4 * q] = w.x; v[4 * q + 1] = w.y; v[4 * q + 2] = w.z; v[4 * q + 3] = w.w;
-      } else {
-        v[2 * q] = w.x; v[2 * q + 1] = w.y;
-      }
-    }
-  } else {
-#pragma unroll
-    for (int s = 0; s < S; s++) v[s] = s < nvalid ? p[s] : T(0);
-  }
-}
-template <typename T, int S>
-__device__ inline void store_chunk(T *__restrict__ p, int nvalid, const T (&v)[S]) {
-  constexpr int VE = 16 / sizeof(T);
-  if (S % VE == 0 && nvalid == S) {
-    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
-#pragma unroll
-    for (int q = 0; q < S / VE; q++) {
-      V w;
-      if constexpr (sizeof(T) == 4) {
-        w.x = v[4 * q]; w.y = v[4 * q + 1]; w.z = v[4 * q + 2]; w.w = v[4 * q + 3];
-      } else {
-        w.x = v[2 * q]; w.y = v[2 * q + 1];
-      }
-      reinterpret_cast<V *>(p)[q] = w;
-    }
-  } else {
-#pragma unroll
-    for (int s = 0; s < S; s++)
-      if (s < nvalid) p[s] = v[s];
-  }
-}
-template <int S>
-__device__ inline void load_flags(const uint8_t *__restrict__ p, int nvalid, uint8_t (&f)[S]) {
-  if (S % 4 == 0 && nvalid == S) {
-#pragma unroll
-    for (int q = 0; q < S / 4; q++) {
-      uint32_t w = reinterpret_cast<const uint32_t *>(p)[q];
-      f[4 * q] = w & 0xff; f[4 * q + 1] = (w >> 8) & 0xff; f[4 * q + 2] = (w >> 16) & 0xff; f[4 * q + 3] = w >> 24;
-    }
-  } else {
-#pragma unroll
-    for (int s = 0; s < S; s++) f[s] = s < nvalid ? p[s] : uint8_t(FL_BLANK);
-  }
-}
 
 // Workgroup index remapped so that each XCD (workgroups are dispatched to
 // the 8 XCDs round-robin) runs one contiguous range of the logical order:
