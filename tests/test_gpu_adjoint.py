@@ -119,7 +119,11 @@ class PoissonSolve(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_p):
-        lam, dg, st = ctx.solver.adjoint(grad_p.numpy(), bc_grad=True, **ctx.opts)
+        # the transposed solve's FP64 residual floors near 1e-12 relative on
+        # 96^3 (its right-hand side scales with 1/V); 1e-11 is far below
+        # what the 1e-6 gradient checks resolve
+        opts = dict(ctx.opts, rel_tol=max(ctx.opts.get("rel_tol", 1e-11), 1e-11))
+        lam, dg, st = ctx.solver.adjoint(grad_p.numpy(), bc_grad=True, **opts)
         assert st["status"] == "PP_OK", st
         return torch.from_numpy(lam), torch.from_numpy(dg.copy()), None, None
 
@@ -139,8 +143,12 @@ def objective_weights(ib, gs, seed):
 
 @pytest.mark.parametrize("prec", [pp.PP_PREC_FP64, pp.PP_PREC_MIXED])
 def test_adjoint_matches_central_differences(prec):
-    announce(f"adjoint vs finite differences 32^3 sphere prec={prec}")
-    n = 32
+    """Config 4: 96^3 uniform grid on [0,1]^3, sphere r = 0.2 at the centre,
+    seeded N(0,1) RHS, J = sum w p over ghost-adjacent fluid cells with
+    w ~ U[0,1]; 16 seeded directions in (RHS, ghost Neumann data), central
+    differences with h = 1e-6."""
+    n = 96
+    announce(f"adjoint vs finite differences {n}^3 sphere prec={prec}")
     xn = cases.uniform(n)
     ib, fr, gs, ex = pp.build_geometry(xn, xn, xn, [pp.Ellipsoid((0.5, 0.5, 0.5), (0.2, 0.2, 0.2))])
     s = pp.PressureSolver(xn, xn, xn)
@@ -176,9 +184,12 @@ def test_adjoint_matches_central_differences(prec):
         assert abs(fd - an) <= 1e-6 * max(abs(an), 1.0), (d, fd, an)
 
 
-def test_adjoint_matches_sparse_lu_gradient():
-    announce("adjoint vs sparse-LU gradient 16^3 sphere")
-    n = 16
+@pytest.mark.parametrize("prec", [pp.PP_PREC_FP64, pp.PP_PREC_MIXED])
+def test_adjoint_matches_sparse_lu_gradient(prec):
+    """Config 4's 24^3 copy: the gradient against a direct FP64 LU of the
+    bordered operator."""
+    announce(f"adjoint vs sparse-LU gradient 24^3 sphere prec={prec}")
+    n = 24
     xn = cases.uniform(n)
     ib, fr, gs, ex = pp.build_geometry(xn, xn, xn, [pp.Ellipsoid((0.5, 0.5, 0.5), (0.2, 0.2, 0.2))])
     s = pp.PressureSolver(xn, xn, xn)
@@ -186,7 +197,7 @@ def test_adjoint_matches_sparse_lu_gradient():
     A, unknown = cases.assemble(xn, xn, xn, ib, gs)
     V = cases.volumes(xn, xn, xn)
     wobj = objective_weights(ib, gs, 7)
-    lam, dg, st = s.adjoint(wobj, bc_grad=True, precision=pp.PP_PREC_FP64, rel_tol=1e-13, max_outer=100)
+    lam, dg, st = s.adjoint(wobj, bc_grad=True, precision=prec, rel_tol=1e-13, max_outer=100)
     assert st["status"] == "PP_OK", st
     # reference: J(b) = wobj . M A^+ Pi b, gradient = columns of the solution
     # operator applied to unit vectors, through one sparse LU
