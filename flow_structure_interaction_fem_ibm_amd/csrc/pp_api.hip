@@ -1,5 +1,6 @@
 // C-ABI entry points (include/pp_hip.h) and the multigrid driver.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -974,6 +975,7 @@ double fluid_volume(const pp_ctx *c, int i, int j, int k) {  // 0-based interior
 // rows) converges to w; normalised so sum_f w_f = sum_f V_f.
 int compute_null_vector(pp_ctx *c) {
   std::vector<double> w(c->nref, 0.0), mu(c->nref, 0.0);
+  const bool warm = (long)c->h_mu_prev.size() == c->nref;
   double vsum = 0.0;
   for (int k = 0; k < c->nzc; k++)
     for (int j = 0; j < c->nyc; j++)
@@ -981,7 +983,9 @@ int compute_null_vector(pp_ctx *c) {
         long r = ridx(c->ref, i, j, k);
         if (!c->h_iblank[r]) {
           w[r] = fluid_volume(c, i, j, k);
-          mu[r] = 1.0;
+          // start from the previous geometry's null vector where it exists
+          // (moving bodies), from the fluid indicator elsewhere
+          mu[r] = (warm && c->h_mu_prev[r] != 0.0) ? c->h_mu_prev[r] : 1.0;
           vsum += w[r];
         }
       }
@@ -991,11 +995,13 @@ int compute_null_vector(pp_ctx *c) {
     pp_solve_opts o;
     pp_default_opts(&o);
     o.precision = PP_PREC_MIXED;
-    o.rel_tol = 1e-13;
-    o.max_outer = 60;
+    o.rel_tol = 1e-11;
+    o.max_outer = 200;
     pp_solve_stats st;
     int lam = 0;
-    int rc = adjoint_impl<float>(c, nullptr, &o, -1.0, &st, &lam);
+    // a warm start converges relative to the cold start's initial residual
+    // (of the previous geometry), not to its own much smaller one
+    int rc = adjoint_impl<float>(c, nullptr, &o, warm ? c->null_ref : -1.0, &st, &lam);
     if (rc != PP_OK && rc != PP_NOT_CONVERGED) return rc;
     if (st.rel_l2 > 1e-10) {
       set_error("left null vector did not converge (relative residual %g after %d iterations)", st.rel_l2,
@@ -1003,6 +1009,8 @@ int compute_null_vector(pp_ctx *c) {
       return PP_BAD_GEOMETRY;
     }
     HIPCHK(hipMemcpy(mu.data(), c->d_mu, sizeof(double) * c->nref, hipMemcpyDeviceToHost));
+    c->h_mu_prev = mu;
+    if (!warm) c->null_ref = st.b_l2;
     std::vector<double> lg(c->gh.n);
     HIPCHK(hipMemcpy(lg.data(), c->gh.olam64[lam], sizeof(double) * c->gh.n, hipMemcpyDeviceToHost));
     double wsum = 0.0;
@@ -1177,15 +1185,27 @@ int32_t pp_set_geometry(pp_ctx *c, const pp_geometry *geo) {
     return PP_BAD_GEOMETRY;
   }
   free_ghosts(c);
-  int rc = build_levels(c);
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+  auto t0 = now();
+  int rc = PP_OK;
+  if (c->lv.empty()) {  // the hierarchy depends on the grid only
+    rc = build_levels(c);
+    if (rc) return rc;
+    build_flags(c);
+    rc = upload_levels(c);
+  } else {
+    build_flags(c);
+    rc = upload_flags(c);
+  }
   if (rc) return rc;
-  build_flags(c);
-  rc = upload_levels(c);
-  if (rc) return rc;
+  auto t1 = now();
   rc = build_ghosts(c, geo);
   if (rc) return rc;
+  auto t2 = now();
   rc = build_coarse(c);
   if (rc) return rc;
+  auto t3 = now();
   // fresh cells
   c->h_fresh.clear();
   hipFree(c->d_fresh);
@@ -1203,10 +1223,26 @@ int32_t pp_set_geometry(pp_ctx *c, const pp_geometry *geo) {
   HIPCHK(hipMalloc(&c->d_iblank_ref, c->nref));
   HIPCHK(hipMemcpy(c->d_iblank_ref, c->h_iblank.data(), c->nref, hipMemcpyHostToDevice));
   c->have_geom = true;
+  auto t4 = now();
   rc = compute_null_vector(c);
   if (rc) { c->have_geom = false; return rc; }
   HIPCHK(hipDeviceSynchronize());
   c->have_geom = true;
+  auto t5 = now();
+  c->setup_ms[0] = ms(t0, t1);
+  c->setup_ms[1] = ms(t1, t2);
+  c->setup_ms[2] = ms(t2, t3);
+  c->setup_ms[3] = ms(t4, t5);
+  return PP_OK;
+}
+
+int32_t pp_setup_info(pp_ctx *c, double *phase_ms, int32_t *null_iterations, double *null_residual) {
+  if (!c) { set_error("pp_setup_info: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_setup_info: no geometry"); return PP_NO_GEOMETRY; }
+  if (phase_ms)
+    for (int q = 0; q < 4; q++) phase_ms[q] = c->setup_ms[q];
+  if (null_iterations) *null_iterations = c->null_iters;
+  if (null_residual) *null_residual = c->null_res;
   return PP_OK;
 }
 

@@ -334,19 +334,39 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
   int rc = 0;
   TriBodies tri;
   if (mode == PP_GEOM_ANALYTIC) {
-    parallel_for((long)G.nzc, [&](long k0, long k1) {
-      for (long k = k0 + 1; k <= k1; k++)
-        for (int j = 1; j <= G.nyc; j++)
-          for (int i = 1; i <= G.nxc; i++) {
-            double p[3] = {G.XC[0][i], G.XC[1][j], G.XC[2][k]};
-            for (int b = 0; b < nbody; b++)
-              if (inside_analytic(bs[b], p)) {
-                o->iblank[G.r(i, j, (int)k)] = 1;
-                bodynum[G.r(i, j, (int)k)] = b;
-                break;
+    // cell centres inside a body; bodies in order (the first one containing
+    // a centre owns it), each scanned over the cells of its bounding box
+    for (int b = 0; b < nbody; b++) {
+      const Body &B = bs[b];
+      double ca = std::cos(B.alpha_deg * M_PI / 180.0), sa = std::sin(B.alpha_deg * M_PI / 180.0);
+      double ext[3] = {std::sqrt(B.r[0] * B.r[0] * ca * ca + B.r[1] * B.r[1] * sa * sa),
+                       std::sqrt(B.r[0] * B.r[0] * sa * sa + B.r[1] * B.r[1] * ca * ca), B.r[2]};
+      int lo[3], hi[3];
+      int nn[3] = {G.nxc, G.nyc, G.nzc};
+      for (int d = 0; d < 3; d++) {
+        double a = B.c[d] - ext[d] * (1.0 + 1e-9) - 1e-12, z = B.c[d] + ext[d] * (1.0 + 1e-9) + 1e-12;
+        lo[d] = 1;
+        while (lo[d] <= nn[d] && G.XC[d][lo[d]] < a) lo[d]++;
+        hi[d] = nn[d];
+        while (hi[d] >= 1 && G.XC[d][hi[d]] > z) hi[d]--;
+      }
+      if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) continue;
+      parallel_for((long)(hi[2] - lo[2] + 1), [&](long k0, long k1) {
+        for (long kk = k0; kk < k1; kk++) {
+          int k = lo[2] + (int)kk;
+          for (int j = lo[1]; j <= hi[1]; j++)
+            for (int i = lo[0]; i <= hi[0]; i++) {
+              long r = G.r(i, j, k);
+              if (o->iblank[r]) continue;
+              double p[3] = {G.XC[0][i], G.XC[1][j], G.XC[2][k]};
+              if (inside_analytic(B, p)) {
+                o->iblank[r] = 1;
+                bodynum[r] = b;
               }
-          }
-    });
+            }
+        }
+      });
+    }
   } else {
     rc = tri_build(G, bs, tri);
     if (rc == 0) rc = tri_iblank(G, tri, o->iblank, bodynum);

@@ -11,6 +11,7 @@ void make_grid_dir(int n, const double *xn, std::vector<double> &x, std::vector<
 int build_levels(pp_ctx *c);
 void build_flags(pp_ctx *c);
 int upload_levels(pp_ctx *c);
+int upload_flags(pp_ctx *c);
 void free_levels(pp_ctx *c);
 void free_ghosts(pp_ctx *c);
 void derive_ghost_mark(pp_ctx *c);

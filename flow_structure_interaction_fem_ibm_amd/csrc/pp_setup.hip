@@ -254,6 +254,16 @@ void build_flags(pp_ctx *c) {
   }
 }
 
+// a new geometry on an existing hierarchy: only the flags change
+int upload_flags(pp_ctx *c) {
+  for (auto &L : c->lv)
+    if (hipMemcpy(L.flags, L.hflags.data(), L.hflags.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("flag upload failed");
+      return PP_HIP_ERROR;
+    }
+  return PP_OK;
+}
+
 int upload_levels(pp_ctx *c) {
   for (size_t l = 0; l < c->lv.size(); l++) {
     Level &L = c->lv[l];

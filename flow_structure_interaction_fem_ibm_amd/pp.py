@@ -133,6 +133,7 @@ def lib():
     L.pp_null_vector.argtypes = [P, P]
     L.pp_level_info.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.pp_time_kernel.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp]
+    L.pp_setup_info.argtypes = [P, dp, C.POINTER(C.c_int32), dp]
     L.pp_geom_build.argtypes = [C.POINTER(C.c_void_p), C.POINTER(GridDesc), C.c_int32,
                                 C.POINTER(BodyDesc), C.c_int32, C.c_double, P]
     L.pp_geom_counts.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -141,7 +142,7 @@ def lib():
     for name in ("pp_create", "pp_destroy", "pp_set_geometry", "pp_solve", "pp_solve_device",
                  "pp_adjoint", "pp_calc_residual", "pp_ghost_rows", "pp_mean_remove",
                  "pp_divergence", "pp_tdma_batched", "pp_null_vector", "pp_level_info", "pp_time_kernel",
-                 "pp_geom_build", "pp_geom_counts", "pp_geom_get", "pp_geom_destroy"):
+                 "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get", "pp_geom_destroy"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -150,7 +151,7 @@ def lib():
 EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_set_geometry",
             "pp_solve", "pp_solve_device", "pp_adjoint", "pp_calc_residual", "pp_ghost_rows",
             "pp_mean_remove", "pp_divergence", "pp_tdma_batched", "pp_null_vector",
-            "pp_level_info", "pp_time_kernel", "pp_geom_build", "pp_geom_counts", "pp_geom_get",
+            "pp_level_info", "pp_time_kernel", "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get",
             "pp_geom_destroy")
 
 
@@ -323,6 +324,15 @@ class PressureSolver:
         dims = (C.c_int32 * 48)()
         _check(lib().pp_level_info(self.h, C.byref(n), dims))
         return [(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]) for l in range(n.value)]
+
+    def setup_info(self):
+        """Phase times (ms) of the last set_geometry and the null vector's
+        iterations and relative residual."""
+        ms = (C.c_double * 4)()
+        it, res = C.c_int32(), C.c_double()
+        _check(lib().pp_setup_info(self.h, ms, C.byref(it), C.byref(res)))
+        return {"flags_ms": ms[0], "ghost_tables_ms": ms[1], "coarse_ms": ms[2], "null_vector_ms": ms[3],
+                "null_iterations": it.value, "null_residual": res.value}
 
     def time_kernel(self, kernel_class, level=0, precision=PP_PREC_MIXED, reps=50):
         """(avg_us, bytes_per_launch) of reps back-to-back launches of one

@@ -202,6 +202,12 @@ int32_t pp_tdma_batched(pp_ctx *ctx, const double *a, const double *b,
 int32_t pp_null_vector(pp_ctx *ctx, double *w);
 /* Level hierarchy: *n_levels and (cx,cy,cz) per level into dims[3*l]. */
 int32_t pp_level_info(pp_ctx *ctx, int32_t *n_levels, int32_t *dims);
+/* Diagnostics of the last pp_set_geometry (no counterpart in the reference):
+ * phase_ms[4] = host+device time of the flags/levels, ghost tables, coarsest
+ * factorisation and left-null-vector phases; the null vector's iterations
+ * and final relative residual.  Any pointer may be NULL. */
+int32_t pp_setup_info(pp_ctx *ctx, double *phase_ms, int32_t *null_iterations,
+                      double *null_residual);
 /* Kernel timing for roofline evidence (no counterpart in the reference):
  * reps back-to-back launches of one multigrid kernel class (PP_K_XLINE,
  * PP_K_YLINE, PP_K_ZLINE, PP_K_RESIDUAL) on one level in the given
