@@ -564,98 +564,38 @@ int reduce_sums_host(pp_ctx *c, int nq) {
 // updates the Givens QR of the Hessenberg matrix.  Stops at m steps, at the
 // iteration limit or when the least-squares residual meets the tolerance;
 // then p += Z y and the ghost rows are made exact.
-// Krylov vector storage: FP64 in the reference layout (FP64 V-cycles) or
-// FP32 in the level-0 layout (FP32 V-cycles).
-template <typename T>
-struct KryStore;
-template <>
-struct KryStore<double> {
-  using V = KryVecs;
-  using E = double;
-  static E *vec(pp_ctx *c, int q) { return c->d_kry[q]; }
-  static E *zvec(pp_ctx *c, int q) { return c->d_kry[c->kry_m + 1 + q]; }
-  // z_j from the V-cycle's correction (ghost rows solved in FP64), w = A z_j
-  static void apply(pp_ctx *c, Level &L0, CellLaunch cl, double cells, int j, E *z, E *w, const V &vv) {
-    LevelArgs<double> a0 = largs<double>(L0);
-    kbegin(c, PP_K_KRYLOV, 0, cells * (8.0 + 1.0 + 8.0));
-    k_kry_z<double><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, a0, lv_e<double>(L0), z);
-    kend(c);
-    ghost_exact_ref(c, z, false);
-    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1)));
-    k_kry_apply<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, a0, z, w, vv, c->d_part);
-    kend(c);
-  }
-  static void orth(pp_ctx *c, Level &L0, CellLaunch cl, double cells, int j, E *w, const V &vv) {
-    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1) + 16.0));
-    k_kry_orth<double, 256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(L0), w, vv,
-                                                                  lv_rhs<double>(L0), lv_e<double>(L0), c->d_part);
-    kend(c);
-  }
-  static void update(pp_ctx *c, Level &L0, CellLaunch cl, double cells, double *p, const V &vv) {
-    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * vv.n));
-    k_kry_update<<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(L0), p, vv);
-    kend(c);
-  }
-};
-template <>
-struct KryStore<float> {
-  using V = KryVecsF;
-  using E = float;
-  static E *vec(pp_ctx *c, int q) { return c->d_kryl[q]; }
-  static E *zvec(pp_ctx *c, int q) { return c->d_kryl[c->kryl_m + 1 + q]; }
-  static void apply(pp_ctx *c, Level &L0, CellLaunch cl, double cells, int j, E *z, E *w, const V &vv) {
-    ghost_sweep_level0<float>(c, lv_e<float>(L0));
-    kbegin(c, PP_K_KRYLOV, 0, cells * (4.0 + 1.0 + 4.0 + 4.0 + 4.0 * (j + 1)));
-    k_kryl_apply<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), lv_e<float>(L0), z, w, vv,
-                                                           c->d_part);
-    kend(c);
-  }
-  static void orth(pp_ctx *c, Level &L0, CellLaunch cl, double cells, int j, E *w, const V &vv) {
-    kbegin(c, PP_K_KRYLOV, 0, cells * (1.0 + 8.0 + 4.0 * (j + 1) + 8.0));
-    k_kryl_orth<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), w, vv, lv_rhs<float>(L0),
-                                                          lv_e<float>(L0), c->d_part);
-    kend(c);
-  }
-  static void update(pp_ctx *c, Level &L0, CellLaunch cl, double cells, double *p, const V &vv) {
-    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 4.0 * vv.n));
-    k_kryl_update<<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(L0), p, vv);
-    kend(c);
-  }
-};
-
-// One restart cycle.  On entry the first basis vector holds r = b - A p
-// (fluid cells) and the level-0 rhs holds r in T with e0 = 0; beta = |r|.
-// Each step applies the V-cycle(s) to the current basis vector (right
-// preconditioning), keeps the preconditioned vector z_j (flexible variant:
-// the V-cycle in FP32 is not exactly linear), orthogonalises A z_j by
-// classical Gram-Schmidt and updates the Givens QR of the Hessenberg
-// matrix.  Stops at m steps, at the iteration limit or when the
-// least-squares residual meets the tolerance; then p += Z y and the ghost
-// rows are made exact.
-template <typename T>
-int krylov_cycle(pp_ctx *c, const pp_solve_opts *o, double *p, bool have_bnd, double beta, double bden,
-                 int &outer, pp_solve_stats &S) {
-  using St = KryStore<T>;
-  using E = typename St::E;
-  Level &L0 = c->lv[0];
-  CellLaunch cl = ref_launch(c);
-  double cells = (double)c->nxc * c->nyc * c->nzc;
+// ---------------------------------------------------------------------
+// Flexible GMRES restart cycle, generic over the operator/storage policy
+// Ops (forward mixed, forward FP64, adjoint):
+//   Ops::V / Ops::E       vector-list type and element type of the vectors
+//   u(q), z(q)            basis vector q (m+1) and preconditioned vector q (m)
+//   precondition()        V-cycle(s) on the level-0 rhs (holds basis j)
+//   apply(j, V)           z_j from the V-cycle's correction, u_{j+1} = Op z_j,
+//                         partial dots <u_{j+1}, u_q> (q <= j)
+//   orth(j, V)            u_{j+1} -= sum c_q u_q, its norm, level-0 rhs = u_{j+1}
+//   update(V)             solution += sum c_q z_q (+ what follows an update)
+// On entry u(0) holds the residual r (fluid cells), the level-0 rhs holds r
+// and e0 = 0; beta = |r|.  The basis is kept unnormalised (v_q = sigma_q u_q)
+// and orthogonalised by classical Gram-Schmidt; the Hessenberg matrix is
+// reduced by Givens rotations on the host.  Stops at m steps, at the
+// iteration limit or when the least-squares residual meets the tolerance.
+// ---------------------------------------------------------------------
+template <typename Ops>
+int gmres_cycle(pp_ctx *c, const pp_solve_opts *o, Ops &ops, double beta, double bden, int &outer,
+                pp_solve_stats &S) {
   const int m = o->krylov;
-  E *U[KRY_MAX + 1], *Z[KRY_MAX];
-  for (int q = 0; q <= m; q++) U[q] = St::vec(c, q);
-  for (int q = 0; q < m; q++) Z[q] = St::zvec(c, q);
   std::vector<double> H((m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), gv(m + 1, 0.0), sig(m + 1, 0.0);
   auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
   sig[0] = 1.0 / beta;
   gv[0] = beta;
   int k = 0;
   for (int j = 0; j < m && outer < o->max_outer; j++) {
-    int rc = precondition<T>(c, o, S);
+    int rc = ops.precondition();
     if (rc) return rc;
-    typename St::V V{};
+    typename Ops::V V{};
     V.n = j + 1;
-    for (int q = 0; q <= j; q++) V.u[q] = U[q];
-    St::apply(c, L0, cl, cells, j, Z[j], U[j + 1], V);
+    for (int q = 0; q <= j; q++) V.u[q] = ops.u(q);
+    ops.apply(j, V);
     rc = reduce_sums_host(c, j + 1);
     if (rc) return rc;
     for (int q = 0; q <= j; q++) {
@@ -663,7 +603,7 @@ int krylov_cycle(pp_ctx *c, const pp_solve_opts *o, double *p, bool have_bnd, do
       h(q, j) = sig[j] * sig[q] * d;
       V.c[q] = sig[q] * sig[q] * d;
     }
-    St::orth(c, L0, cl, cells, j, U[j + 1], V);
+    ops.orth(j, V);
     rc = reduce_sums_host(c, 1);
     if (rc) return rc;
     double hn = std::sqrt(c->h_res[0]);
@@ -679,7 +619,7 @@ int krylov_cycle(pp_ctx *c, const pp_solve_opts *o, double *p, bool have_bnd, do
       set_error("non-finite Krylov coefficient after %d iterations", outer);
       return PP_NONFINITE;
     }
-    if (rho == 0.0) break;  // A z_j adds nothing: keep the first j columns
+    if (rho == 0.0) break;  // Op z_j adds nothing: keep the first j columns
     cs[j] = h(j, j) / rho;
     sn[j] = h(j + 1, j) / rho;
     h(j, j) = rho;
@@ -703,15 +643,108 @@ int krylov_cycle(pp_ctx *c, const pp_solve_opts *o, double *p, bool have_bnd, do
     for (int q = i + 1; q < k; q++) t -= h(i, q) * y[q];
     y[i] = t / h(i, i);
   }
-  typename St::V V{};
+  typename Ops::V V{};
   V.n = k;
   for (int q = 0; q < k; q++) {
-    V.u[q] = Z[q];
+    V.u[q] = ops.z(q);
     V.c[q] = y[q] * sig[q];
   }
-  St::update(c, L0, cl, cells, p, V);
-  ghost_exact_ref(c, p, have_bnd);
+  ops.update(V);
   return PP_OK;
+}
+
+// forward operator, FP64 vectors on the reference layout (FP64 V-cycles)
+struct FwdOps64 {
+  using V = KryVecs;
+  pp_ctx *c;
+  const pp_solve_opts *o;
+  pp_solve_stats *S;
+  double *p;
+  bool have_bnd;
+  double cells;
+  double *u(int q) { return c->d_kry[q]; }
+  double *z(int q) { return c->d_kry[c->kry_m + 1 + q]; }
+  int precondition() { return ::precondition<double>(c, o, *S); }
+  void apply(int j, const V &vv) {
+    Level &L0 = c->lv[0];
+    CellLaunch cl = ref_launch(c);
+    LevelArgs<double> a0 = largs<double>(L0);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (8.0 + 1.0 + 8.0));
+    k_kry_z<double><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, a0, lv_e<double>(L0), z(j));
+    kend(c);
+    ghost_exact_ref(c, z(j), false);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1)));
+    k_kry_apply<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, a0, z(j), u(j + 1), vv, c->d_part, nullptr);
+    kend(c);
+  }
+  void orth(int j, const V &vv) {
+    Level &L0 = c->lv[0];
+    CellLaunch cl = ref_launch(c);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1) + 16.0));
+    k_kry_orth<double, 256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(L0), u(j + 1), vv,
+                                                                  lv_rhs<double>(L0), lv_e<double>(L0), c->d_part);
+    kend(c);
+  }
+  void update(const V &vv) {
+    CellLaunch cl = ref_launch(c);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * vv.n));
+    k_kry_update<<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), p, vv);
+    kend(c);
+    ghost_exact_ref(c, p, have_bnd);
+  }
+};
+
+// forward operator, FP32 vectors on the level-0 layout (FP32 V-cycles):
+// products, dots and coefficients in FP64 on the stored values
+struct FwdOps32 {
+  using V = KryVecsF;
+  pp_ctx *c;
+  const pp_solve_opts *o;
+  pp_solve_stats *S;
+  double *p;
+  bool have_bnd;
+  double cells;
+  float *u(int q) { return c->d_kryl[q]; }
+  float *z(int q) { return c->d_kryl[c->kryl_m + 1 + q]; }
+  int precondition() { return ::precondition<float>(c, o, *S); }
+  void apply(int j, const V &vv) {
+    Level &L0 = c->lv[0];
+    CellLaunch cl = ref_launch(c);
+    ghost_sweep_level0<float>(c, lv_e<float>(L0));
+    kbegin(c, PP_K_KRYLOV, 0, cells * (4.0 + 1.0 + 4.0 + 4.0 + 4.0 * (j + 1)));
+    k_kryl_apply<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), lv_e<float>(L0), z(j), u(j + 1), vv,
+                                                           c->d_part);
+    kend(c);
+  }
+  void orth(int j, const V &vv) {
+    Level &L0 = c->lv[0];
+    CellLaunch cl = ref_launch(c);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (1.0 + 8.0 + 4.0 * (j + 1) + 8.0));
+    k_kryl_orth<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), u(j + 1), vv, lv_rhs<float>(L0),
+                                                          lv_e<float>(L0), c->d_part);
+    kend(c);
+  }
+  void update(const V &vv) {
+    CellLaunch cl = ref_launch(c);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 4.0 * vv.n));
+    k_kryl_update<<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), p, vv);
+    kend(c);
+    ghost_exact_ref(c, p, have_bnd);
+  }
+};
+
+template <typename T>
+struct FwdOpsFor;
+template <>
+struct FwdOpsFor<double> { using type = FwdOps64; };
+template <>
+struct FwdOpsFor<float> { using type = FwdOps32; };
+
+template <typename T>
+int krylov_cycle(pp_ctx *c, const pp_solve_opts *o, double *p, bool have_bnd, double beta, double bden,
+                 int &outer, pp_solve_stats &S) {
+  typename FwdOpsFor<T>::type ops{c, o, &S, p, have_bnd, (double)c->nxc * c->nyc * c->nzc};
+  return gmres_cycle(c, o, ops, beta, bden, outer, S);
 }
 
 // ---------------------------------------------------------------------
@@ -876,6 +909,85 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
 // null).  ref_norm < 0: converge relative to the first residual.
 // On return the exact lambda_g of the final mu is in gh.lam64[*lam_idx].
 // ---------------------------------------------------------------------
+// transposed ghost coupling of a correction z (fluid cells): lambda_g from
+// the fluid neighbours of each ghost (solved ghost-ghost rows), then
+// out = sum_g c lambda_g / V on the touched fluid cells, 0 elsewhere
+void adjoint_ghost_coupling(pp_ctx *c, const double *z, double *out) {
+  GhostTables &g = c->gh;
+  hipStream_t s = c->stream;
+  hipMemsetAsync(out, 0, sizeof(double) * c->nref, s);
+  if (g.n == 0) return;
+  dim3 gg((g.n + 255) / 256), gt((g.nt + 255) / 256), blk(256);
+  k_adj_ghost_rhs<double><<<gg, blk, 0, s>>>(g.n, g.nb_ref, g.nb_w64, nullptr, z, g.orhsg64);
+  k_fill<double><<<gg, blk, 0, s>>>(g.olam64[0], g.n, 0.0);
+  int cur = 0;
+  for (int q = 0; q < g.kexact; q++) {
+    k_adj_ghost_jacobi<double><<<gg, blk, 0, s>>>(g.n, g.g_tidx, g.t_ptr, g.t_ghost, g.t_coef64, g.orhsg64,
+                                                 g.olam64[cur], g.olam64[1 - cur]);
+    cur ^= 1;
+  }
+  if (g.nt > 0)
+    k_adj_touch<double><<<gt, blk, 0, s>>>(g.nt, g.t_ref, g.t_ptr, g.t_ghost, g.t_coef64, g.t_vinv64,
+                                           g.olam64[cur], nullptr, out);
+}
+
+// adjoint operator B mu = A mu - (transposed ghost coupling of mu) on the
+// fluid cells (mu = lambda/V), FP64 vectors on the reference layout,
+// preconditioned by the transposed V-cycle in precision T
+template <typename T>
+struct AdjOps {
+  using V = KryVecs;
+  pp_ctx *c;
+  const pp_solve_opts *o;
+  pp_solve_stats *S;
+  double cells;
+  double *u(int q) { return c->d_kry[q]; }
+  double *z(int q) { return c->d_kry[c->kry_m + 1 + q]; }
+  int precondition() {
+    GhostTables &g = c->gh;
+    Level &L0 = c->lv[0];
+    dim3 gg((g.n + 255) / 256), gt((g.nt + 255) / 256), blk(256);
+    if (g.nt > 0) k_gather_idx<T><<<gt, blk, 0, c->stream>>>(g.nt, g.t_al, lv_rhs<T>(L0), AdjPtrs<T>::tbase(g));
+    if (g.n > 0) {
+      k_fill<T><<<gg, blk, 0, c->stream>>>(AdjPtrs<T>::lam(g, 0), g.n, T(0));
+      g_adj_cur = 0;
+    }
+    c->adjoint_mode = true;
+    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
+      vcycle<T>(c, 0, o);
+      S->vcycles++;
+    }
+    c->adjoint_mode = false;
+    return PP_OK;
+  }
+  void apply(int j, const V &vv) {
+    Level &L0 = c->lv[0];
+    CellLaunch cl = ref_launch(c);
+    LevelArgs<double> a0 = largs<double>(L0);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (sizeof(T) + 1.0 + 8.0));
+    k_kry_z<T><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, a0, lv_e<T>(L0), z(j));
+    kend(c);
+    adjoint_ghost_coupling(c, z(j), c->d_tmp);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1)));
+    k_kry_apply<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, a0, z(j), u(j + 1), vv, c->d_part, c->d_tmp);
+    kend(c);
+  }
+  void orth(int j, const V &vv) {
+    Level &L0 = c->lv[0];
+    CellLaunch cl = ref_launch(c);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * (j + 1) + 2.0 * sizeof(T)));
+    k_kry_orth<T, 256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(L0), u(j + 1), vv,
+                                                             lv_rhs<T>(L0), lv_e<T>(L0), c->d_part);
+    kend(c);
+  }
+  void update(const V &vv) {
+    CellLaunch cl = ref_launch(c);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 8.0 * vv.n));
+    k_kry_update<<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), c->d_mu, vv);
+    kend(c);
+  }
+};
+
 template <typename T>
 int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref_norm, pp_solve_stats *st,
                  int *lam_idx) {
@@ -893,6 +1005,11 @@ int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref
   HIPCHK(hipEventCreate(&ev0));
   HIPCHK(hipEventCreate(&ev1));
   HIPCHK(hipEventRecord(ev0, s));
+  const int m = o->krylov;
+  if (m > 0) {
+    int rc = ensure_krylov(c, m);
+    if (rc) return rc;
+  }
   HIPCHK(hipMemcpyAsync(c->d_bwork, c->d_sv, sizeof(double) * c->nref, hipMemcpyDeviceToDevice, s));
   dim3 gg((g.n + 255) / 256), gt((g.nt + 255) / 256), blk(256);
   if (g.nt > 0) k_gather_idx<double><<<gt, blk, 0, s>>>(g.nt, g.t_ref, c->d_sv, g.otbase64);
@@ -915,7 +1032,8 @@ int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref
     }
     kbegin(c, PP_K_OUTER_RESIDUAL, 0, cells * (17.0 + 2.0 * sizeof(T)));
     k_outer_residual<T, 256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, c->d_bwork, lv_rhs<T>(L0),
-                                                          lv_e<T>(L0), c->d_part, nullptr, (T *)nullptr);
+                                                          lv_e<T>(L0), c->d_part, m > 0 ? c->d_kry[0] : nullptr,
+                                                          (T *)nullptr);
     kend(c);
     int rc = reduce_summax(c, (int)nblk);
     if (rc) return rc;
@@ -932,6 +1050,13 @@ int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref
     if (outer > 0 && outer <= 64) S.history[outer - 1] = rel;
     if ((o->rel_tol > 0.0 && rel <= o->rel_tol) || rl2 == 0.0) { status = PP_OK; break; }
     if (outer >= o->max_outer) break;
+    if (m > 0) {
+      AdjOps<T> ops{c, o, &S, cells};
+      int krc = gmres_cycle(c, o, ops, rl2, ref_norm > 0.0 ? ref_norm : 1.0, outer, S);
+      if (krc == PP_NOT_CONVERGED || krc == PP_NONFINITE) { status = krc; break; }
+      if (krc) return krc;
+      continue;
+    }
     if (g.nt > 0) k_gather_idx<T><<<gt, blk, 0, s>>>(g.nt, g.t_al, lv_rhs<T>(L0), AdjPtrs<T>::tbase(g));
     if (g.n > 0) {
       k_fill<T><<<gg, blk, 0, s>>>(AdjPtrs<T>::lam(g, 0), g.n, T(0));

@@ -502,7 +502,7 @@ __global__ void __launch_bounds__(256) k_adj_touch(int nt, const long *__restric
   if (vi == T(0)) return;  // ghost cell: handled by the Jacobi rows
   T s = T(0);
   for (int q = t_ptr[t]; q < t_ptr[t + 1]; q++) s += t_coef[q] * lam[t_ghost[q]];
-  target[t_idx[t]] = base[t] + vi * s;
+  target[t_idx[t]] = (base ? base[t] : T(0)) + vi * s;
 }
 
 template <typename T>
@@ -1175,11 +1175,14 @@ __global__ void __launch_bounds__(256) k_kry_z(RefArgs R, LevelArgs<double> L, c
   z[ridx(R, i, j, k)] = (L.fl[a] & FL_BLANK) ? 0.0 : (double)e0[a];
 }
 
-// w = A z on fluid cells (0 elsewhere) with z's ghost rows and outer layer
-// already exact; per-block partial dots <w, u_q>, q < V.n, at part[blk*V.n+q]
+// w = A z (- bsub) on fluid cells (0 elsewhere) with z's ghost rows and
+// outer layer already exact; per-block partial dots <w, u_q>, q < V.n, at
+// part[blk*V.n+q].  bsub (may be null): a field subtracted on fluid cells
+// (the transposed ghost coupling of the adjoint operator).
 template <int NT>
 __global__ void __launch_bounds__(NT) k_kry_apply(RefArgs R, LevelArgs<double> L, const double *__restrict__ z,
-                                                  double *__restrict__ w, KryVecs V, double *__restrict__ part) {
+                                                  double *__restrict__ w, KryVecs V, double *__restrict__ part,
+                                                  const double *__restrict__ bsub) {
   __shared__ double s[NT / 64];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
@@ -1194,6 +1197,7 @@ __global__ void __launch_bounds__(NT) k_kry_apply(RefArgs R, LevelArgs<double> L
       Row<double> rw = make_row(L, f, i, j, k);
       wv = z[r] * rw.diag + z[r - 1] * rw.amx + z[r + 1] * rw.apx + z[r - R.ex] * rw.amy + z[r + R.ex] * rw.apy +
            z[r - R.exy] * rw.amz + z[r + R.exy] * rw.apz;
+      if (bsub) wv -= bsub[r];
     }
     w[r] = wv;
   }

@@ -86,9 +86,8 @@ typedef struct pp_solve_opts {
   int32_t use_graph;         /* replay the outer iteration as a hipGraph (0/1) */
   int32_t profile;           /* time every kernel launch with HIP events (0/1) */
   int32_t krylov;            /* 0: FP64 iterative refinement around the V-cycles;
-                                m in 1..16: FP64 flexible GMRES(m) with the
-                                V-cycles as right preconditioner (pp_solve only;
-                                pp_adjoint always refines) */
+                                m in 1..16: flexible GMRES(m) with the V-cycles
+                                as right preconditioner (pp_solve, pp_adjoint) */
   int32_t nu_coarse;         /* smoothing steps before and after the coarse
                                 correction on levels below the finest
                                 (<= 0: nu_pre / nu_post everywhere) */
