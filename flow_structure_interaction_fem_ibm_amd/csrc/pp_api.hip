@@ -300,16 +300,65 @@ void adj_ghost_level0(pp_ctx *c, T *e, T *rhs) {
   kend(c);
 }
 
+// ghost rows of the level-0 correction before a smoothing step (forward:
+// the solved ghost rows; adjoint: the transposed ghost coupling into rhs)
 template <typename T>
-void smooth(pp_ctx *c, int l, T omega) {
+void ghost_step_level0(pp_ctx *c, T *e, T *rhs) {
+  if (c->adjoint_mode) adj_ghost_level0<T>(c, e, rhs);
+  else ghost_sweep_level0<T>(c, e);
+}
+
+// one zebra line pass set: x, y, z lines, two colours each
+template <typename T>
+void line_passes(pp_ctx *c, int l, T omega) {
   Level &L = c->lv[l];
-  if (l == 0) {
-    if (c->adjoint_mode) adj_ghost_level0<T>(c, lv_e<T>(L), lv_rhs<T>(L));
-    else ghost_sweep_level0<T>(c, lv_e<T>(L));
-  }
   for (int col = 0; col < 2; col++) launch_xline<T>(c, L, l, col, omega);
   for (int col = 0; col < 2; col++) launch_yzline<T, 1>(c, L, l, col, omega);
   for (int col = 0; col < 2; col++) launch_yzline<T, 2>(c, L, l, col, omega);
+}
+
+// one point-Jacobi sweep src -> dst on level l
+template <typename T>
+void jacobi_pass(pp_ctx *c, int l, const T *src, T *dst, T omega) {
+  Level &L = c->lv[l];
+  LevelArgs<T> a = largs<T>(L);
+  kbegin(c, PP_K_JACOBI, l, (double)L.cx * L.cy * L.cz * (3.0 * sizeof(T) + 1.0));
+  if (L.cx % 4 == 0) {
+    CellLaunch q = cell_launch(L.cx / 4, L.cy, L.cz);
+    k_jacobi<T, 4><<<q.grid, q.block, 0, c->stream>>>(a, src, lv_rhs<T>(L), dst, omega);
+  } else {
+    CellLaunch q = cell_launch(L.cx, L.cy, L.cz);
+    k_jacobi<T, 1><<<q.grid, q.block, 0, c->stream>>>(a, src, lv_rhs<T>(L), dst, omega);
+  }
+  kend(c);
+}
+
+// `steps` smoothing steps on level l (the ghost rows first on level 0 when
+// ghosts is set).  Point Jacobi alternates between e and the level's
+// residual buffer (free while smoothing) and copies back after an odd count.
+template <typename T>
+void smooth_steps(pp_ctx *c, int l, const pp_solve_opts *o, int steps, bool ghosts = true) {
+  Level &L = c->lv[l];
+  const T om = (T)o->omega;
+  if (o->smoother != PP_SMOOTH_POINT_JACOBI) {
+    for (int s = 0; s < steps; s++) {
+      if (l == 0 && ghosts) ghost_step_level0<T>(c, lv_e<T>(L), lv_rhs<T>(L));
+      line_passes<T>(c, l, om);
+    }
+    return;
+  }
+  T *buf[2] = {lv_e<T>(L), lv_res<T>(L)};
+  int cur = 0;
+  for (int s = 0; s < steps; s++) {
+    if (l == 0 && ghosts) ghost_step_level0<T>(c, buf[cur], lv_rhs<T>(L));
+    jacobi_pass<T>(c, l, buf[cur], buf[1 - cur], om);
+    cur ^= 1;
+  }
+  if (cur) {
+    kbegin(c, PP_K_JACOBI, l, 2.0 * sizeof(T) * L.total);
+    hipMemcpyAsync(buf[0], buf[1], sizeof(T) * L.total, hipMemcpyDeviceToDevice, c->stream);
+    kend(c);
+  }
 }
 
 // level residual res = rhs - L e (four cells per thread when rows allow)
@@ -339,10 +388,9 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
     kend(c);
     return;
   }
-  T om = (T)o->omega;
   const int npre = (l > 0 && o->nu_coarse > 0) ? o->nu_coarse : o->nu_pre;
   const int npost = (l > 0 && o->nu_coarse > 0) ? o->nu_coarse : o->nu_post;
-  for (int s = 0; s < npre; s++) smooth<T>(c, l, om);
+  smooth_steps<T>(c, l, o, npre);
   LevelArgs<T> a = largs<T>(L);
   CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
   kbegin(c, PP_K_RESIDUAL, l, cells * (3.0 * sizeof(T) + 1.0));
@@ -369,7 +417,7 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
     k_prolong<T><<<cl.grid, cl.block, 0, st>>>(a, ac, pdir<T>(L, 0), pdir<T>(L, 1), pdir<T>(L, 2), lv_e<T>(C),
                                                 lv_e<T>(L));
   kend(c);
-  for (int s = 0; s < npost; s++) smooth<T>(c, l, om);
+  smooth_steps<T>(c, l, o, npost);
 }
 
 // The preconditioner: cycles_per_outer V-cycles on the level-0 rhs (e0 = 0
@@ -385,7 +433,8 @@ int precondition(pp_ctx *c, const pp_solve_opts *o, pp_solve_stats &S) {
     return PP_OK;
   }
   long key = (long)sizeof(T) | ((long)o->nu_pre << 4) | ((long)o->nu_post << 12) | ((long)std::max(o->nu_coarse, 0) << 8) |
-             ((long)o->cycles_per_outer << 20) | ((long)std::lround(o->omega * 1e6) << 28);
+             ((long)o->cycles_per_outer << 20) | ((long)std::lround(o->omega * 1e6) << 28) |
+             ((long)(o->smoother & 3) << 56);
   if (!c->gexec || c->graph_key != key) {
     if (c->gexec) {
       HIPCHK(hipGraphExecDestroy(c->gexec));
@@ -510,7 +559,8 @@ int mean_remove_ref(pp_ctx *c, double *p, double *mean_out) {
 bool opts_ok(const pp_solve_opts *o) {
   return o && (o->precision == PP_PREC_FP64 || o->precision == PP_PREC_MIXED) && o->max_outer >= 0 &&
          o->max_outer <= 10000 && o->cycles_per_outer >= 1 && o->nu_pre >= 0 && o->nu_post >= 0 &&
-         o->omega > 0.0 && o->omega < 2.0 && o->krylov >= 0 && o->krylov <= KRY_MAX;
+         o->omega > 0.0 && o->omega < 2.0 && o->krylov >= 0 && o->krylov <= KRY_MAX &&
+         (o->smoother == PP_SMOOTH_LINE || o->smoother == PP_SMOOTH_POINT_JACOBI);
 }
 
 // ---------------------------------------------------------------------
@@ -1181,6 +1231,7 @@ void pp_default_opts(pp_solve_opts *o) {
   o->profile = 0;
   o->krylov = 8;
   o->nu_coarse = 1;
+  o->smoother = PP_SMOOTH_LINE;
 }
 
 const char *pp_last_error(void) { return pp::last_error(); }
@@ -1227,7 +1278,10 @@ int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
       }
     make_grid_dir(ns[d] + 1, xs[d], c->hx[d], c->rxc[d], c->rdx[d], c->rdxinv[d], c->rdxc[d], c->rdxcinv[d]);
   }
-  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  // a blocking stream: ordered with the legacy default stream, on which the
+  // synchronous hipMemcpy/hipMemset calls of the entry points and a caller's
+  // default-stream work (the producer of pp_solve_device's inputs) run
+  HIPCHK(hipStreamCreate(&c->stream));
   for (int d = 0; d < 3; d++) {
     size_t nd = sizeof(double) * c->rdx[d].size();
     HIPCHK(hipMalloc(&c->d_rdx[d], nd));
@@ -1540,6 +1594,39 @@ int32_t pp_null_vector(pp_ctx *c, double *w) {
   if (!c->have_geom) { set_error("pp_null_vector: no geometry"); return PP_NO_GEOMETRY; }
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMemcpy(w, c->d_w, sizeof(double) * c->nref, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+int32_t pp_smooth(pp_ctx *c, int32_t smoother, double omega, int32_t sweeps, double *var, const double *r) {
+  if (!c || !var || !r) { set_error("pp_smooth: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_smooth: no geometry"); return PP_NO_GEOMETRY; }
+  if ((smoother != PP_SMOOTH_LINE && smoother != PP_SMOOTH_POINT_JACOBI) || sweeps < 0 || sweeps > 10000 ||
+      !(omega > 0.0 && omega < 2.0)) {
+    set_error("pp_smooth: smoother must be PP_SMOOTH_LINE or PP_SMOOTH_POINT_JACOBI, 0 <= sweeps <= 10000, "
+              "0 < omega < 2");
+    return PP_BAD_ARG;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  size_t nb = sizeof(double) * c->nref;
+  HIPCHK(hipMemcpy(c->d_p, var, nb, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_bwork, r, nb, hipMemcpyHostToDevice));
+  Level &L = c->lv[0];
+  LevelArgs<double> a = largs<double>(L);
+  CellLaunch cl = ref_launch(c);
+  k_ref_to_level<double><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, a, c->d_p, L.e64);
+  k_ref_to_level<double><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, a, c->d_bwork, L.rhs64);
+  pp_solve_opts o;
+  pp_default_opts(&o);
+  o.smoother = smoother;
+  o.omega = omega;
+  const bool adj = c->adjoint_mode;
+  c->adjoint_mode = false;
+  smooth_steps<double>(c, 0, &o, sweeps, false);
+  c->adjoint_mode = adj;
+  k_level_to_ref<double><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, a, L.e64, c->d_p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(var, c->d_p, nb, hipMemcpyDeviceToHost));
   return PP_OK;
 }
 

@@ -632,6 +632,57 @@ __global__ void __launch_bounds__(256) k_residual4(LevelArgs<T> L, const T *__re
 }
 
 // ---------------------------------------------------------------------
+// weighted point Jacobi (MG_itsolv_Point_Jacobi, CORE_MG.F90:3716-3800),
+// out of place: out = e + omega (s - e) with s = (rhs - sum_nb a_nb e_nb) /
+// diag on fluid cells (the reference's operation order, contraction off)
+// and s = ghostMark * e on blank cells.  A fluid row whose six faces are
+// all closed keeps e (the diagonal guard of the line rows,
+// CORE_MG.F90:1977-1982; the reference's point sweep would divide by zero).
+// V consecutive cells per thread (V = 4 needs 16-byte aligned rows).
+// ---------------------------------------------------------------------
+template <typename T, int V>
+__global__ void __launch_bounds__(256) k_jacobi(LevelArgs<T> L, const T *__restrict__ e, const T *__restrict__ rhs,
+                                                T *__restrict__ out, T omega) {
+#pragma clang fp contract(off)
+  const int i0 = V * (blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i0 >= L.cx || j >= L.cy) return;
+  const int nv = min(V, L.cx - i0);
+  const long a = lidx(L.sx, L.sxy, i0, j, k);
+  T ec[V], ejm[V], ejp[V], ekm[V], ekp[V], rv[V], cmx[V], cpx[V], o[V];
+  uint8_t fl[V];
+  load_chunk<T, V>(e + a, nv, ec);
+  load_chunk<T, V>(e + a - L.sx, nv, ejm);
+  load_chunk<T, V>(e + a + L.sx, nv, ejp);
+  load_chunk<T, V>(e + a - L.sxy, nv, ekm);
+  load_chunk<T, V>(e + a + L.sxy, nv, ekp);
+  load_chunk<T, V>(rhs + a, nv, rv);
+  load_chunk<T, V>(L.cmx + i0, nv, cmx);
+  load_chunk<T, V>(L.cpx + i0, nv, cpx);
+  load_flags<V>(L.fl + a, nv, fl);
+  const T el = e[a - 1], er = e[a + nv];
+  const T my = L.cmy[j], py = L.cpy[j], mz = L.cmz[k], pz = L.cpz[k];
+#pragma unroll
+  for (int s = 0; s < V; s++) {
+    const uint8_t f = fl[s];
+    T sv;
+    if (f & FL_BLANK) {
+      sv = (f & FL_GHOST) ? ec[s] : T(0);
+    } else {
+      T amx = (f & FL_IM) ? T(0) : cmx[s], apx = (f & FL_IP) ? T(0) : cpx[s];
+      T amy = (f & FL_JM) ? T(0) : my, apy = (f & FL_JP) ? T(0) : py;
+      T amz = (f & FL_KM) ? T(0) : mz, apz = (f & FL_KP) ? T(0) : pz;
+      T diag = (-(amx + apx) + -(amy + apy)) + -(amz + apz);
+      T left = s == 0 ? el : ec[s - 1], right = s + 1 < nv ? ec[s + 1] : er;
+      T r = rv[s] - left * amx - right * apx - ejm[s] * amy - ejp[s] * apy - ekm[s] * amz - ekp[s] * apz;
+      sv = tabs(diag) < T(1.0e-12) ? ec[s] : r / diag;
+    }
+    o[s] = ec[s] + omega * (sv - ec[s]);
+  }
+  store_chunk<T, V>(out + a, nv, o);
+}
+
+// ---------------------------------------------------------------------
 // restriction: coarse rhs = fluid-volume-weighted mean of the children's
 // residuals (the reference's masked averaging, MG_Restrict_X/Y/Z
 // CORE_MG.F90:2511-2538, applied over all coarsened directions at once);

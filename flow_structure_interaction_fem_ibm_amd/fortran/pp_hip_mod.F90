@@ -17,7 +17,8 @@ module pp_hip_mod
        PP_BAD_GEOMETRY = 3, PP_HIP_ERROR = 4, PP_NONFINITE = 5, PP_NO_GEOMETRY = 6
   integer(c_int32_t), parameter, public :: PP_BC_NEUMANN = 2
   integer(c_int32_t), parameter, public :: PP_PREC_FP64 = 0, PP_PREC_MIXED = 1
-  integer, parameter, public :: PP_K_CLASSES = 12
+  integer(c_int32_t), parameter, public :: PP_SMOOTH_LINE = 0, PP_SMOOTH_POINT_JACOBI = 1
+  integer, parameter, public :: PP_K_CLASSES = 13
 
   type, bind(c), public :: pp_grid_desc
      integer(c_int32_t) :: nxc, nyc, nzc
@@ -44,7 +45,7 @@ module pp_hip_mod
      integer(c_int32_t) :: cycles_per_outer
      integer(c_int32_t) :: nu_pre, nu_post
      real(c_double) :: omega
-     integer(c_int32_t) :: mean_remove, project_rhs, use_graph, profile, krylov, nu_coarse
+     integer(c_int32_t) :: mean_remove, project_rhs, use_graph, profile, krylov, nu_coarse, smoother
   end type pp_solve_opts
 
   type, bind(c), public :: pp_solve_stats
@@ -61,7 +62,7 @@ module pp_hip_mod
 
   public :: pp_default_opts, pp_last_error, pp_create, pp_destroy, pp_set_geometry, pp_solve, &
        pp_solve_device, pp_adjoint, pp_calc_residual, pp_ghost_rows, pp_mean_remove, pp_divergence, &
-       pp_null_vector, pp_level_info, pp_error_text
+       pp_smooth, pp_null_vector, pp_level_info, pp_error_text
 
   interface
      subroutine pp_default_opts(o) bind(c, name="pp_default_opts")
@@ -146,6 +147,14 @@ module pp_hip_mod
        real(c_double), intent(out) :: sum
        integer(c_int32_t) :: rc
      end function pp_divergence
+
+     function pp_smooth(ctx, smoother, omega, sweeps, var, r) bind(c, name="pp_smooth") result(rc)
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, var, r
+       integer(c_int32_t), value :: smoother, sweeps
+       real(c_double), value :: omega
+       integer(c_int32_t) :: rc
+     end function pp_smooth
 
      function pp_null_vector(ctx, w) bind(c, name="pp_null_vector") result(rc)
        import :: c_ptr, c_int32_t

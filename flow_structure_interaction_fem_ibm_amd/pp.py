@@ -73,11 +73,15 @@ class SolveOpts(C.Structure):
                 ("nu_pre", C.c_int32), ("nu_post", C.c_int32), ("omega", C.c_double),
                 ("mean_remove", C.c_int32), ("project_rhs", C.c_int32), ("use_graph", C.c_int32),
                 ("profile", C.c_int32), ("krylov", C.c_int32),
-                ("nu_coarse", C.c_int32)]
+                ("nu_coarse", C.c_int32), ("smoother", C.c_int32)]
 
+
+PP_SMOOTH_LINE = 0
+PP_SMOOTH_POINT_JACOBI = 1
 
 KERNEL_CLASSES = ("outer_residual", "update", "xline", "yline", "zline", "residual", "restrict",
-                  "prolong", "ghost", "coarse", "other", "krylov")
+                  "prolong", "ghost", "coarse", "other", "krylov", "jacobi")
+NK = len(KERNEL_CLASSES)
 
 
 class SolveStats(C.Structure):
@@ -85,9 +89,9 @@ class SolveStats(C.Structure):
                 ("rel_l2", C.c_double), ("abs_linf", C.c_double), ("loc", C.c_int32 * 3),
                 ("b_l2", C.c_double), ("rhs_shift", C.c_double), ("mean_removed", C.c_double),
                 ("ms", C.c_double), ("bytes", C.c_double), ("history", C.c_double * 64),
-                ("kern_ms", C.c_double * 12), ("kern_bytes", C.c_double * 12),
-                ("kern_launches", C.c_int32 * 12), ("kern_ms_l0", C.c_double * 12),
-                ("kern_bytes_l0", C.c_double * 12), ("kern_launches_l0", C.c_int32 * 12)]
+                ("kern_ms", C.c_double * NK), ("kern_bytes", C.c_double * NK),
+                ("kern_launches", C.c_int32 * NK), ("kern_ms_l0", C.c_double * NK),
+                ("kern_bytes_l0", C.c_double * NK), ("kern_launches_l0", C.c_int32 * NK)]
 
     def as_dict(self):
         n = min(self.outer_iterations, 64)
@@ -130,6 +134,7 @@ def lib():
     L.pp_mean_remove.argtypes = [P, P, dp]
     L.pp_divergence.argtypes = [P, P, P, P, C.c_double, P, dp]
     L.pp_tdma_batched.argtypes = [P, P, P, P, P, P, C.c_int32, C.c_int32, C.c_int32]
+    L.pp_smooth.argtypes = [P, C.c_int32, C.c_double, C.c_int32, P, P]
     L.pp_null_vector.argtypes = [P, P]
     L.pp_level_info.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.pp_time_kernel.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp]
@@ -141,7 +146,7 @@ def lib():
     L.pp_geom_destroy.argtypes = [P]
     for name in ("pp_create", "pp_destroy", "pp_set_geometry", "pp_solve", "pp_solve_device",
                  "pp_adjoint", "pp_calc_residual", "pp_ghost_rows", "pp_mean_remove",
-                 "pp_divergence", "pp_tdma_batched", "pp_null_vector", "pp_level_info", "pp_time_kernel",
+                 "pp_divergence", "pp_smooth", "pp_tdma_batched", "pp_null_vector", "pp_level_info", "pp_time_kernel",
                  "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get", "pp_geom_destroy"):
         getattr(L, name).restype = C.c_int32
     _lib = L
@@ -150,7 +155,7 @@ def lib():
 
 EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_set_geometry",
             "pp_solve", "pp_solve_device", "pp_adjoint", "pp_calc_residual", "pp_ghost_rows",
-            "pp_mean_remove", "pp_divergence", "pp_tdma_batched", "pp_null_vector",
+            "pp_mean_remove", "pp_divergence", "pp_smooth", "pp_tdma_batched", "pp_null_vector",
             "pp_level_info", "pp_time_kernel", "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get",
             "pp_geom_destroy")
 
@@ -341,6 +346,15 @@ class PressureSolver:
         us, b = C.c_double(), C.c_double()
         _check(lib().pp_time_kernel(self.h, cls, level, precision, reps, C.byref(us), C.byref(b)))
         return us.value, b.value
+
+    def smooth(self, var, rhs, smoother=PP_SMOOTH_POINT_JACOBI, omega=1.0, sweeps=1):
+        """pp_smooth: sweeps of the finest-level smoother on a copy of var
+        (ghost rows not updated); returns the smoothed field."""
+        v = np.array(var, dtype=np.float64, order="C", copy=True)
+        r = np.ascontiguousarray(rhs, dtype=np.float64)
+        assert v.shape == self.shape and r.shape == self.shape
+        _check(lib().pp_smooth(self.h, int(smoother), float(omega), int(sweeps), v.ctypes.data, r.ctypes.data))
+        return v
 
     def tdma_batched(self, a, b, c, r, method):
         a, b, c, r = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, r))

@@ -91,7 +91,14 @@ typedef struct pp_solve_opts {
   int32_t nu_coarse;         /* smoothing steps before and after the coarse
                                 correction on levels below the finest
                                 (<= 0: nu_pre / nu_post everywhere) */
+  int32_t smoother;          /* PP_SMOOTH_LINE or PP_SMOOTH_POINT_JACOBI */
 } pp_solve_opts;
+
+/* smoothers (pp_solve_opts.smoother, pp_smooth) */
+#define PP_SMOOTH_LINE 0          /* zebra line relaxation x, y, z (MG_itsolv,
+                                     CORE_MG.F90:1883-2270, pp_solver_type 3) */
+#define PP_SMOOTH_POINT_JACOBI 1  /* weighted point Jacobi (MG_itsolv_Point_Jacobi,
+                                     CORE_MG.F90:3716-3800, pp_solver_type 8) */
 
 /* kernel classes of pp_solve_stats.kern_* (profile = 1) */
 #define PP_K_OUTER_RESIDUAL 0  /* FP64 b - A p on the reference layout */
@@ -106,7 +113,8 @@ typedef struct pp_solve_opts {
 #define PP_K_COARSE 9          /* coarsest-level dense solve */
 #define PP_K_OTHER 10
 #define PP_K_KRYLOV 11         /* flexible-GMRES apply / orthogonalise / update */
-#define PP_K_CLASSES 12
+#define PP_K_JACOBI 12         /* point-Jacobi sweep */
+#define PP_K_CLASSES 13
 
 typedef struct pp_solve_stats {
   int32_t status;
@@ -187,6 +195,16 @@ int32_t pp_mean_remove(pp_ctx *ctx, double *p, double *mean);
 int32_t pp_divergence(pp_ctx *ctx, const double *face_u, const double *face_v,
                       const double *face_w, double dtinv, double *nlu0,
                       double *sum);
+/* `sweeps` smoothing sweeps of the finest-level operator on var (inout)
+ * with right-hand side r, FP64, host arrays in the reference layout; the
+ * ghost rows are not updated (the caller's GCM step does that), as in the
+ * smoother routines themselves.  PP_SMOOTH_POINT_JACOBI is
+ * MG_itsolv_Point_Jacobi (CORE_MG.F90:3716-3800) called at level 1 of all
+ * three directions; PP_SMOOTH_LINE is one zebra x, y, z line pass set per
+ * sweep (the line rows of MG_itsolv, CORE_MG.F90:1945-2009, in zebra line
+ * order).  omega is the relaxation weight (MAIN.F90:478). */
+int32_t pp_smooth(pp_ctx *ctx, int32_t smoother, double omega, int32_t sweeps,
+                  double *var, const double *r);
 /* tdma (CORE_TDMA.F90:10-34) on a batch of independent systems, through
  * the device line solvers: n_lines systems of length n stored line after
  * line (a: sub-, b: main, c: super-diagonal, r: rhs, u: solution).

@@ -242,6 +242,40 @@ double or_calc_residual(const or_ctx *c, const double *var, const double *r,
   return resm;
 }
 
+/* MG_itsolv_Point_Jacobi (CORE_MG.F90:3716-3800) as MG_X/Y/Z call it on the
+ * finest level (nLevX = nLevY = nLevZ = 1): the level metrics are the grid's,
+ * ium_MG..kup_MG are the fine face flags and ghostcellMark_MG the ghost
+ * marks (MG_Prepare_BC at n = 1, CORE_MG.F90:1686-1700), nlw = 0, 3-D.
+ * Out of place over the interior cells: var1 = var + omega (rhss - var). */
+void or_point_jacobi(const or_ctx *c, double *var, const double *r, double omega) {
+  long n = or_field_size(c), sxy = c->ex * c->ey;
+  double *var1 = dalloc(n);
+  memcpy(var1, var, sizeof(double) * n);
+  for (int k = 1; k <= c->nzc; k++)
+    for (int j = 1; j <= c->nyc; j++)
+      for (int i = 1; i <= c->nxc; i++) {
+        long id = OR_IDX(c, i, j, k);
+        double bmx = c->dxcinv[i] * c->dxinv[i] * (1.0 - c->ium[id]);
+        double bpx = c->dxcinv[i + 1] * c->dxinv[i] * (1.0 - c->iup[id]);
+        double bcx = -(bmx + bpx);
+        double bmy = c->dycinv[j] * c->dyinv[j] * (1.0 - c->jum[id]);
+        double bpy = c->dycinv[j + 1] * c->dyinv[j] * (1.0 - c->jup[id]);
+        double bcy = -(bmy + bpy);
+        double bmz = c->dzcinv[k] * c->dzinv[k] * (1.0 - c->kum[id]) * 1.0;
+        double bpz = c->dzcinv[k + 1] * c->dzinv[k] * (1.0 - c->kup[id]) * 1.0;
+        double bcz = -(bmz + bpz);
+        double fl = 1.0 - (double)c->iblank[id];
+        bmx *= fl; bpx *= fl; bmy *= fl; bpy *= fl; bmz *= fl; bpz *= fl;
+        double bc = (bcx + bcy + bcz) * fl + (double)c->iblank[id];
+        double rhss = r[id] - var[id - 1] * bmx - var[id + 1] * bpx - var[id - c->ex] * bmy -
+                      var[id + c->ex] * bpy - var[id - sxy] * bmz - var[id + sxy] * bpz + 0.0;
+        rhss = (rhss / bc) * fl + (double)c->ghost[id] * var[id];
+        var1[id] = var[id] + omega * (rhss - var[id]);
+      }
+  memcpy(var, var1, sizeof(double) * n);
+  free(var1);
+}
+
 /* subtracting_average_pressure (CORE_POISSON.F90:419-490) */
 double or_mean_remove(const or_ctx *c, double *p) {
   double pt = 0.0, vt = 0.0;

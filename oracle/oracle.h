@@ -62,6 +62,9 @@ double or_calc_residual(const or_ctx *c, const double *var, const double *r,
                         double *res, int32_t *loc, double *l2);
 /* subtracting_average_pressure (CORE_POISSON.F90:419-490) */
 double or_mean_remove(const or_ctx *c, double *p);
+/* MG_itsolv_Point_Jacobi (CORE_MG.F90:3716-3800) on the finest level: one
+ * weighted point-Jacobi sweep of var (inout) against r, ghost rows held */
+void or_point_jacobi(const or_ctx *c, double *var, const double *r, double omega);
 /* set_outer_ghost_pres for all-Neumann outer faces
  * (CORE_SET_OUTER_GHOST.F90:66-213) */
 void or_outer_ghost(const or_ctx *c, double *p);

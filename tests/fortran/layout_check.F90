@@ -14,7 +14,7 @@ program layout_check
   call pp_default_opts(o)
   print '(A,I0,1X,ES12.5,1X,ES12.5,7(1X,I0),1X,F6.3,5(1X,I0))', 'opts ', o%precision, o%rel_tol, o%abs_linf_tol, &
        o%max_outer, o%cycles_per_outer, o%nu_pre, o%nu_post, o%mean_remove, o%project_rhs, o%use_graph, &
-       o%omega, o%profile, o%krylov, o%nu_coarse, 0, 0
+       o%omega, o%profile, o%krylov, o%nu_coarse, o%smoother, 0
   xs = [0.0d0, 0.5d0, 1.0d0]
   g%nxc = 2; g%nyc = 2; g%nzc = 2
   g%x = c_loc(xs); g%y = c_loc(xs); g%z = c_loc(xs)

@@ -46,6 +46,7 @@ def lib():
         L.or_set_fresh.argtypes = [P, P]
         L.or_calc_residual.restype = C.c_double
         L.or_calc_residual.argtypes = [P, P, P, P, P, P]
+        L.or_point_jacobi.argtypes = [P, P, P, C.c_double]
         L.or_mean_remove.restype = C.c_double
         L.or_mean_remove.argtypes = [P, P]
         L.or_outer_ghost.argtypes = [P, P]
@@ -126,6 +127,14 @@ class Oracle:
 
     def mean_remove(self, p):
         return lib().or_mean_remove(self.h, p.ctypes.data)
+
+    def point_jacobi(self, var, r, omega=1.0, sweeps=1):
+        """sweeps of MG_itsolv_Point_Jacobi on a copy of var"""
+        v = np.array(var, dtype=np.float64, order="C", copy=True)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        for _ in range(sweeps):
+            lib().or_point_jacobi(self.h, v.ctypes.data, r.ctypes.data, float(omega))
+        return v
 
     def outer_ghost(self, p):
         lib().or_outer_ghost(self.h, p.ctypes.data)

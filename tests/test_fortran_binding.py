@@ -34,7 +34,7 @@ def test_fortran_module_matches_the_c_abi(tmp_path):
     assert [int(v) for v in f[3:10]] == [o.max_outer, o.cycles_per_outer, o.nu_pre, o.nu_post, o.mean_remove,
                                          o.project_rhs, o.use_graph]
     assert float(f[10]) == pytest.approx(o.omega) and int(f[11]) == o.profile and int(f[12]) == o.krylov
-    assert int(f[13]) == o.nu_coarse
+    assert int(f[13]) == o.nu_coarse and int(f[14]) == o.smoother
     # a 2x2x2 grid is rejected before any device call, with the library's message
     assert int(out["create"][0]) == pp.PP_BAD_ARG
     assert "at least 4 cells" in " ".join(out["error"])

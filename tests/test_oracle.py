@@ -141,6 +141,36 @@ def test_ghost_rows_fixed_point_and_constants():
     assert np.abs(r[gflat]).max() < 1e-13
 
 
+@pytest.mark.parametrize("omega", [1.0, 0.7])
+def test_point_jacobi_matches_assembled_operator(omega):
+    """One MG_itsolv_Point_Jacobi sweep is var + omega D^-1 (b - A var) on
+    fluid rows (A assembled independently), holds ghost cells and scales the
+    other blank cells by 1 - omega."""
+    xn, yn, zn, ib, gs, _ = cases.sphere_case(14, r=0.22, stretch=1.5)
+    o = Oracle(xn, yn, zn)
+    o.set_iblank(ib)
+    o.set_stencils(gs)
+    A, unknown = cases.assemble(xn, yn, zn, ib, gs)
+    p = cases.random_field(ib.shape, 5)
+    b = cases.random_field(ib.shape, 6, ib)
+    out = o.point_jacobi(p, b, omega).ravel()
+    pf, bf = p.ravel(), b.ravel()
+    shp = ib.shape
+    inner = np.zeros(shp, bool)
+    inner[1:-2, 1:-2, 1:-2] = True
+    inner = inner.ravel()
+    fl = inner & (ib.ravel() == 0)
+    ghost = np.zeros(pf.size, bool)
+    ghost[(gs.ig + shp[2] * (gs.jg + shp[1] * gs.kg)).astype(np.int64)] = True
+    solid = inner & (ib.ravel() == 1) & ~ghost
+    d = A.diagonal()
+    ref = pf + omega * (bf - A @ pf) / np.where(fl, d, 1.0)
+    assert np.allclose(out[fl], ref[fl], rtol=1e-12, atol=1e-12)
+    assert np.array_equal(out[ghost], pf[ghost])
+    assert np.allclose(out[solid], (1 - omega) * pf[solid], rtol=1e-15, atol=0)
+    assert np.array_equal(out[~inner], pf[~inner])
+
+
 def test_apply_transpose_is_adjoint():
     xn, yn, zn, ib, gs, _ = cases.sphere_case(12, r=0.25, stretch=1.0)
     o = Oracle(xn, yn, zn)
