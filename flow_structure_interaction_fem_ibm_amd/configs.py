@@ -15,6 +15,7 @@ GPU with their throughput, timing and memory figures printed as JSON.
 
   python -m flow_structure_interaction_fem_ibm_amd.configs --config 3 [--steps 50]
   python -m flow_structure_interaction_fem_ibm_amd.configs --config 5
+  (--triangulated: the reference's triangulated-surface geometry)
 """
 from __future__ import annotations
 
@@ -47,7 +48,7 @@ def config3_body(step):
     return pp.Ellipsoid((1.5 + 0.01 * step, 2.0, 2.0), (0.3, 0.3, 0.3), 0.0, 48)
 
 
-def run_config3(steps=50, tol=1e-10):
+def run_config3(steps=50, tol=1e-10, mode=pp.PP_GEOM_ANALYTIC):
     import torch
     xn, yn, zn = config3_grid()
     dev = torch.device("cuda:0")
@@ -59,7 +60,7 @@ def run_config3(steps=50, tol=1e-10):
     t_all = time.perf_counter()
     for st_i in range(steps):
         t0 = time.perf_counter()
-        ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, [config3_body(st_i)], iblank_prev=ib_prev)
+        ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, [config3_body(st_i)], iblank_prev=ib_prev, mode=mode)
         t1 = time.perf_counter()
         s.set_geometry(ib, gs, fresh=fr if ib_prev is not None else None)
         t2 = time.perf_counter()
@@ -91,6 +92,7 @@ def run_config3(steps=50, tol=1e-10):
     gb = sum(r["GB"] for r in recs)
     return {
         "config": "3: 384x256x256 moving sphere, geometry rebuilt per step, mixed solve to 1e-10",
+        "geometry": "triangulated" if mode == pp.PP_GEOM_TRIANGULATED else "analytic",
         "steps": steps, "cells": 384 * 256 * 256,
         "solves_per_s": steps / solve_s, "device_ms_per_solve": dev_ms / steps,
         "steps_per_s_including_geometry": steps / wall,
@@ -159,14 +161,14 @@ def force_x_weights(yn, zn, ib):
     return w
 
 
-def run_config5():
+def run_config5(mode=pp.PP_GEOM_ANALYTIC):
     import torch
     xn, yn, zn = config5_grid()
     dev = torch.device("cuda:0")
     free0, total = torch.cuda.mem_get_info()
     t0 = time.perf_counter()
     bodies = config5_bodies(xn, yn, zn)
-    ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, bodies)
+    ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, bodies, mode=mode)
     t1 = time.perf_counter()
     s = pp.PressureSolver(xn, yn, zn)
     s.set_geometry(ib, gs)
@@ -189,6 +191,7 @@ def run_config5():
     fx = float((w * d_p.cpu().numpy()).sum())
     return {
         "config": "5: 512x512x384 (1.0e8 cells), 64 spheres, mixed solve to 1e-8 + one adjoint",
+        "geometry": "triangulated" if mode == pp.PP_GEOM_TRIANGULATED else "analytic",
         "cells": 512 * 512 * 384, "ghosts": gs.n, "blank": int(ib.sum()),
         "forward": {"status": st["status"], "iterations": st["outer_iterations"], "rel_l2": st["rel_l2"],
                     "wall_s": t4 - t3, "device_ms": st["ms"], "solves_per_s": 1.0 / (t4 - t3),
@@ -207,8 +210,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", type=int, choices=(3, 5), required=True)
     ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--triangulated", action="store_true",
+                    help="the reference's triangulated-surface geometry instead of the analytic ellipsoid")
     args = ap.parse_args()
-    out = run_config3(args.steps) if args.config == 3 else run_config5()
+    mode = pp.PP_GEOM_TRIANGULATED if args.triangulated else pp.PP_GEOM_ANALYTIC
+    out = run_config3(args.steps, mode=mode) if args.config == 3 else run_config5(mode)
     print(json.dumps(out))
 
 

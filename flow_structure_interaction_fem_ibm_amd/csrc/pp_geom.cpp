@@ -146,8 +146,8 @@ void closest_point_analytic(const Body &b, const double *p, double *bi, double *
 
 // find_isolatedIblank(1) (BOUNDARY_SET.F90:440-499): a blank cell with no
 // other blank cell in its clamped 3x3x3 neighbourhood is cleared (in place,
-// in loop order).
-void isolated_iblank(const Grid &G, std::vector<int8_t> &ib) {
+// in loop order) and loses its body number.
+void isolated_iblank(const Grid &G, std::vector<int8_t> &ib, std::vector<int> &bodynum) {
   for (int k = 1; k <= G.nzc; k++)
     for (int j = 1; j <= G.nyc; j++)
       for (int i = 1; i <= G.nxc; i++) {
@@ -156,7 +156,10 @@ void isolated_iblank(const Grid &G, std::vector<int8_t> &ib) {
         for (int kk = std::max(1, k - 1); kk <= std::min(G.nzc, k + 1); kk++)
           for (int jj = std::max(1, j - 1); jj <= std::min(G.nyc, j + 1); jj++)
             for (int ii = std::max(1, i - 1); ii <= std::min(G.nxc, i + 1); ii++) s += ib[G.r(ii, jj, kk)];
-        if (s <= 1) ib[G.r(i, j, k)] = 0;
+        if (s <= 1) {
+          ib[G.r(i, j, k)] = 0;
+          bodynum[G.r(i, j, k)] = -1;
+        }
       }
 }
 
@@ -384,7 +387,7 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
           long r = G.r(i, j, k);
           if (o->iblank[r] == 0 && iblank_prev[r] == 1) o->fresh[r] = 1;
         }
-  isolated_iblank(G, o->iblank);
+  isolated_iblank(G, o->iblank, bodynum);
   fill_cell(G, o->iblank, o->fresh);
   mark_ghosts(G, o->iblank, o->ghost, o->list);
   int ng = (int)o->list.size() / 3;
@@ -406,11 +409,16 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
           double d2 = (tb[0] - p[0]) * (tb[0] - p[0]) + (tb[1] - p[1]) * (tb[1] - p[1]) + (tb[2] - p[2]) * (tb[2] - p[2]);
           if (d2 < best || (bb == b0 && d2 == best)) {
             best = d2;
+            o->gout[q].body = bb;
             for (int d = 0; d < 3; d++) { bi[d] = tb[d]; nrm[d] = tn[d]; }
           }
         }
       } else {
-        tri_intercept(G, tri, bodynum[G.r(iG, jG, kG)], iG, jG, kG, bi, nrm, o->gout[q].elem);
+        const int b0 = bodynum[G.r(iG, jG, kG)];
+        if (b0 < 0) { err[q] = 7; continue; }
+        o->gout[q].body = b0;
+        err[q] = tri_intercept(G, tri, b0, iG, jG, kG, bi, nrm, o->gout[q].elem);
+        if (err[q]) continue;
       }
       err[q] = ghost_stencil(G, iG, jG, kG, bi, nrm, probe_norm, o->gout[q]);
     }
@@ -419,7 +427,11 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
     if (err[q]) {
       pp::set_error("pp_geom_build: ghost %d at (%d,%d,%d): %s", q, o->list[3 * q], o->list[3 * q + 1],
                     o->list[3 * q + 2],
-                    err[q] == 1 ? "image point cell not found" : err[q] == 2 ? "intercept too long" : "singular Vandermonde");
+                    err[q] == 1   ? "image point cell not found"
+                    : err[q] == 2 ? "intercept too long"
+                    : err[q] == 3 ? "singular Vandermonde"
+                    : err[q] == 5 ? "no element selected for the intercept"
+                                  : "ghost cell without a body");
       delete o;
       return PP_BAD_GEOMETRY;
     }
@@ -457,6 +469,15 @@ int32_t pp_geom_get(const pp_geom *o, int8_t *iblank, int8_t *fresh, int32_t *ig
       if (ip) ip[3 * q + d] = g.ip[d];
       if (nrm) nrm[3 * q + d] = g.nrm[d];
     }
+  }
+  return PP_OK;
+}
+
+int32_t pp_geom_ghost_elements(const pp_geom *o, int32_t *body, int32_t *elem) {
+  if (!o) { pp::set_error("pp_geom_ghost_elements: null handle"); return PP_BAD_ARG; }
+  for (size_t q = 0; q < o->gout.size(); q++) {
+    if (body) body[q] = o->gout[q].body + 1;
+    if (elem) elem[q] = o->gout[q].elem;
   }
   return PP_OK;
 }

@@ -33,30 +33,51 @@ struct Body {
 
 struct GhostOut {
   int ic = -1, jc = -1, kc = -1;
-  int elem = 0;
+  int elem = 0;   // closest surface element (1-based), triangulated mode
+  int body = -1;  // body of the ghost (0-based)
   double coeff[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double probe = 0, bi[3] = {0, 0, 0}, ip[3] = {0, 0, 0}, nrm[3] = {0, 0, 0};
 };
 
 // triangulated bodies (pp_geom_tri.cpp)
+// k markers closest to a point: a kd-tree over a body's markers whose
+// pruning bound never exceeds the computed squared distance of a marker in
+// the pruned box, so the result is exactly that of repeated MINLOC over
+// all markers (smallest squared distance first, lowest index on ties).
+struct MarkerTree {
+  struct Node {
+    double lo[3], hi[3];
+    int a, b, left, right;
+  };
+  std::vector<Node> nodes;
+  std::vector<int> perm;
+};
 struct TriBody {
   int nm = 0, ne = 0;
   std::vector<double> xm, ym, zm;            // markers (0-based)
   std::vector<int> tri;                      // 3 per element, 1-based marker ids
   std::vector<double> cx, cy, cz, nx, ny, nz, area;
-  std::vector<std::vector<int>> vert_elems;  // elements sharing each marker
+  std::vector<int> ve_ptr, ve_idx;           // elements at each marker, ascending (CSR)
+  MarkerTree tree;
 };
 struct TriBodies {
   std::vector<TriBody> b;
 };
+// 0 or an error code: 6 bad body or the orientation point (the origin)
+// inside a body's bounding box
 int tri_build(const Grid &G, const std::vector<Body> &bodies, TriBodies &T);
+// iblank of set_iblank_canonical_body_fast and bodynum (0-based body, -1
+// none); 0 or an error code: 4 no unblocked marker among the 7 closest,
+// 5 no element selected
 int tri_iblank(const Grid &G, const TriBodies &T, std::vector<int8_t> &iblank, std::vector<int> &bodynum);
-void tri_intercept(const Grid &G, const TriBodies &T, int body, int iG, int jG, int kG, double *bi, double *nrm,
-                   int &elem);
+// body intercept, inward element normal and element (1-based) of a ghost;
+// 0 or 5
+int tri_intercept(const Grid &G, const TriBodies &T, int body, int iG, int jG, int kG, double *bi, double *nrm,
+                  int &elem);
 
 bool inside_analytic(const Body &b, const double *p);
 void closest_point_analytic(const Body &b, const double *p, double *bi, double *nin);
-void isolated_iblank(const Grid &G, std::vector<int8_t> &ib);
+void isolated_iblank(const Grid &G, std::vector<int8_t> &ib, std::vector<int> &bodynum);
 void fill_cell(const Grid &G, std::vector<int8_t> &ib, std::vector<int8_t> &fresh);
 void mark_ghosts(const Grid &G, const std::vector<int8_t> &ib, std::vector<int8_t> &gm, std::vector<int> &list);
 int ghost_stencil(const Grid &G, int iG, int jG, int kG, const double *bi, const double *nrm, double probe_norm,

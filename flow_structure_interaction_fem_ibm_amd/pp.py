@@ -143,11 +143,13 @@ def lib():
                                 C.POINTER(BodyDesc), C.c_int32, C.c_double, P]
     L.pp_geom_counts.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.pp_geom_get.argtypes = [P] * 14
+    L.pp_geom_ghost_elements.argtypes = [P, P, P]
     L.pp_geom_destroy.argtypes = [P]
     for name in ("pp_create", "pp_destroy", "pp_set_geometry", "pp_solve", "pp_solve_device",
                  "pp_adjoint", "pp_calc_residual", "pp_ghost_rows", "pp_mean_remove",
                  "pp_divergence", "pp_smooth", "pp_tdma_batched", "pp_null_vector", "pp_level_info", "pp_time_kernel",
-                 "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get", "pp_geom_destroy"):
+                 "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get", "pp_geom_ghost_elements",
+                 "pp_geom_destroy"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -157,7 +159,7 @@ EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_s
             "pp_solve", "pp_solve_device", "pp_adjoint", "pp_calc_residual", "pp_ghost_rows",
             "pp_mean_remove", "pp_divergence", "pp_smooth", "pp_tdma_batched", "pp_null_vector",
             "pp_level_info", "pp_time_kernel", "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get",
-            "pp_geom_destroy")
+            "pp_geom_ghost_elements", "pp_geom_destroy")
 
 
 def _check(rc, allow=(PP_OK,)):
@@ -420,10 +422,12 @@ def build_geometry(xn, yn, zn, bodies, mode=PP_GEOM_ANALYTIC, probe_norm=2.0, ib
         _check(L.pp_geom_get(h, ib.ctypes.data, fr.ctypes.data, *[a.ctypes.data for a in ints],
                              coeff.ctypes.data, probe.ctypes.data, bi.ctypes.data, ip.ctypes.data,
                              nrm.ctypes.data))
+        body, elem = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+        _check(L.pp_geom_ghost_elements(h, body.ctypes.data, elem.ctypes.data))
     finally:
         L.pp_geom_destroy(h)
     gs = GhostSet(*[a[:n] for a in ints], coeff[:n], probe[:n])
-    return ib, fr, gs, {"bi": bi[:n], "ip": ip[:n], "nrm": nrm[:n]}
+    return ib, fr, gs, {"bi": bi[:n], "ip": ip[:n], "nrm": nrm[:n], "body": body[:n], "elem": elem[:n]}
 
 
 def tanh_nodes(n, length, beta):

@@ -268,6 +268,10 @@ int32_t pp_geom_get(const pp_geom *geo, int8_t *iblank, int8_t *fresh,
                     int32_t *ig, int32_t *jg, int32_t *kg, int32_t *ic,
                     int32_t *jc, int32_t *kc, double *coeff, double *probe,
                     double *bi, double *ip, double *nrm);
+/* Per ghost: its body (1-based, bodyNum) and, in PP_GEOM_TRIANGULATED
+ * mode, the surface element of its intercept (1-based, closestElementGC,
+ * GCM_SET_BOUNDARY.F90:136-137; 0 in analytic mode).  Either may be NULL. */
+int32_t pp_geom_ghost_elements(const pp_geom *geo, int32_t *body, int32_t *elem);
 int32_t pp_geom_destroy(pp_geom *geo);
 
 #ifdef __cplusplus

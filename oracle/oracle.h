@@ -121,24 +121,31 @@ void or_apply_AT(const or_ctx *c, const double *x, double *y);
  * owned by the oracle and read back with or_geom_get_surface. */
 int or_geom_ellipsoid(or_ctx *c, int n_theta, double xc0, double yc0,
                       double zc0, double ra, double rb, double rc,
-                      double alpha_deg);
+                      double alpha_deg);  /* markers, or -1 bad body, -2 body around the origin */
 void or_geom_clear_bodies(or_ctx *c);
 int or_geom_nbodies(const or_ctx *c);
 void or_geom_surface_sizes(const or_ctx *c, int body, int *nmarker,
                            int *nelem);
 void or_geom_get_surface(const or_ctx *c, int body, double *xyz, int32_t *tri,
                          double *cent, double *norm);
-/* set_internal_boundary for canonical solid bodies (BOUNDARY_SET.F90:267-410):
- * iblank by set_iblank_canonical_body_fast, find_isolatedIblank(1), fill_cell,
- * ghost marking, body intercepts, image points and Neumann coefficients.
- * iblank_prev may be NULL (first step: no fresh cells).  Returns 0 or an
- * error code (1: image point search failed, 2: intercept too long). */
+/* set_internal_boundary for canonical solid bodies (BOUNDARY_SET.F90:264-410):
+ * iblank by set_iblank_canonical_body_fast over all bodies, fresh cells
+ * against iblank_prev (may be NULL: none), find_isolatedIblank(1),
+ * fill_cell, ghost marking, body intercepts (GCM_calc_bodyIntercept_Unstruc),
+ * image points and Neumann coefficients.  Returns 0 or an error code:
+ * 1 image-point cell not found, 2 intercept too long, 3 singular
+ * Vandermonde matrix, 4 no unblocked marker among the 7 closest, 5 no
+ * element selected, 7 ghost cell without a body. */
 int or_geom_set_boundary(or_ctx *c, const int8_t *iblank_prev,
                          double probe_length_normalized);
 void or_geom_get_iblank(const or_ctx *c, int8_t *iblank, int8_t *fresh);
+/* per ghost: image-point cell, coeffGCMN(8,n), probeLength, body
+ * intercept (3), image point (3), closest element (1-based) and body
+ * (1-based); any pointer may be NULL */
 void or_geom_get_ghost_geometry(const or_ctx *c, int32_t *ic, int32_t *jc,
                                 int32_t *kc, double *coeff, double *probe,
-                                double *bi, double *ip, int32_t *elem);
+                                double *bi, double *ip, int32_t *elem,
+                                int32_t *body);
 
 #ifdef __cplusplus
 }

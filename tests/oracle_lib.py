@@ -58,6 +58,14 @@ def lib():
         L.or_mg_cycle.argtypes = [P, P, P, C.POINTER(MGOpts)]
         L.or_apply_A.argtypes = [P, P, P]
         L.or_apply_AT.argtypes = [P, P, P]
+        L.or_geom_ellipsoid.argtypes = [P, C.c_int] + [C.c_double] * 7
+        L.or_geom_clear_bodies.argtypes = [P]
+        L.or_geom_nbodies.argtypes = [P]
+        L.or_geom_surface_sizes.argtypes = [P, C.c_int, P, P]
+        L.or_geom_get_surface.argtypes = [P, C.c_int, P, P, P, P]
+        L.or_geom_set_boundary.argtypes = [P, P, C.c_double]
+        L.or_geom_get_iblank.argtypes = [P, P, P]
+        L.or_geom_get_ghost_geometry.argtypes = [P] * 10
         _L = L
     return _L
 
@@ -110,6 +118,50 @@ class Oracle:
         pr = np.ascontiguousarray(gs.probe, dtype=np.float64)
         lib().or_set_ghost_stencils(self.h, *[a.ctypes.data for a in arrs], co.ctypes.data, pr.ctypes.data)
         self._keep = (arrs, co, pr)
+
+    # ---- triangulated-body geometry (or_geom.c) ----
+    def add_ellipsoid(self, center, semi, alpha_deg=0.0, n_theta=24):
+        m = lib().or_geom_ellipsoid(self.h, int(n_theta), *[float(v) for v in center],
+                                    *[float(v) for v in semi], float(alpha_deg))
+        if m < 0:
+            raise ValueError(f"or_geom_ellipsoid rejected the body ({m})")
+        return m
+
+    def clear_bodies(self):
+        lib().or_geom_clear_bodies(self.h)
+
+    def surface(self, body):
+        nm, ne = C.c_int(), C.c_int()
+        lib().or_geom_surface_sizes(self.h, body, C.byref(nm), C.byref(ne))
+        xyz = np.zeros((nm.value, 3))
+        tri = np.zeros((ne.value, 3), np.int32)
+        cent = np.zeros((ne.value, 3))
+        nrm = np.zeros((ne.value, 3))
+        lib().or_geom_get_surface(self.h, body, xyz.ctypes.data, tri.ctypes.data, cent.ctypes.data,
+                                  nrm.ctypes.data)
+        return xyz, tri, cent, nrm
+
+    def set_boundary(self, iblank_prev=None, probe_norm=2.0):
+        """set_internal_boundary; returns (rc, iblank, fresh, ghosts dict)"""
+        prev = None if iblank_prev is None else np.ascontiguousarray(iblank_prev, dtype=np.int8)
+        rc = lib().or_geom_set_boundary(self.h, None if prev is None else prev.ctypes.data, float(probe_norm))
+        ib = np.zeros(self.shape, np.int8)
+        fr = np.zeros(self.shape, np.int8)
+        lib().or_geom_get_iblank(self.h, ib.ctypes.data, fr.ctypes.data)
+        if rc:
+            return rc, ib, fr, None
+        n = lib().or_nghost(self.h)
+        m = max(n, 1)
+        ig, jg, kg = self.ghost_list()
+        g = {"ig": ig, "jg": jg, "kg": kg, "ic": np.zeros(m, np.int32), "jc": np.zeros(m, np.int32),
+             "kc": np.zeros(m, np.int32), "coeff": np.zeros((m, 8)), "probe": np.zeros(m),
+             "bi": np.zeros((m, 3)), "ip": np.zeros((m, 3)), "elem": np.zeros(m, np.int32),
+             "body": np.zeros(m, np.int32)}
+        lib().or_geom_get_ghost_geometry(self.h, *[g[k].ctypes.data for k in
+                                                   ("ic", "jc", "kc", "coeff", "probe", "bi", "ip", "elem", "body")])
+        for k in ("ic", "jc", "kc", "coeff", "probe", "bi", "ip", "elem", "body"):
+            g[k] = g[k][:n]
+        return rc, ib, fr, g
 
     def set_fresh(self, fr):
         fr = np.ascontiguousarray(fr, dtype=np.int8)
