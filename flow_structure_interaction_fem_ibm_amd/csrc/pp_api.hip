@@ -602,8 +602,8 @@ int ensure_krylov_level(pp_ctx *c, int m) {
 }
 
 // sum of nq per-block partials -> c->h_res[0..nq)
-int reduce_sums_host(pp_ctx *c, int nq) {
-  launch_reduce_sums(c, c->nblk_red, nq);
+int reduce_sums_host(pp_ctx *c, int nq, int nblk = -1) {
+  launch_reduce_sums(c, nblk >= 0 ? nblk : c->nblk_red, nq);
   HIPCHK(hipMemcpyAsync(c->h_res, c->d_res, sizeof(double) * nq, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return PP_OK;
@@ -649,7 +649,7 @@ int gmres_cycle(pp_ctx *c, const pp_solve_opts *o, Ops &ops, double beta, double
     V.n = j + 1;
     for (int q = 0; q <= j; q++) V.u[q] = ops.u(q);
     ops.apply(j, V);
-    rc = reduce_sums_host(c, j + 1);
+    rc = reduce_sums_host(c, j + 1, ops.blocks());
     if (rc) return rc;
     for (int q = 0; q <= j; q++) {
       double d = c->h_res[q];
@@ -657,7 +657,7 @@ int gmres_cycle(pp_ctx *c, const pp_solve_opts *o, Ops &ops, double beta, double
       V.c[q] = sig[q] * sig[q] * d;
     }
     ops.orth(j, V);
-    rc = reduce_sums_host(c, 1);
+    rc = reduce_sums_host(c, 1, ops.blocks());
     if (rc) return rc;
     double hn = std::sqrt(c->h_res[0]);
     h(j + 1, j) = sig[j] * hn;
@@ -717,6 +717,7 @@ struct FwdOps64 {
   double cells;
   double *u(int q) { return c->d_kry[q]; }
   double *z(int q) { return c->d_kry[c->kry_m + 1 + q]; }
+  int blocks() const { return c->nblk_red; }
   int precondition() { return ::precondition<double>(c, o, *S); }
   void apply(int j, const V &vv) {
     Level &L0 = c->lv[0];
@@ -760,21 +761,39 @@ struct FwdOps32 {
   float *u(int q) { return c->d_kryl[q]; }
   float *z(int q) { return c->d_kryl[c->kryl_m + 1 + q]; }
   int precondition() { return ::precondition<float>(c, o, *S); }
+  // four cells per thread where the level-0 rows allow it
+  bool by4() const { return c->lv[0].cx % 4 == 0; }
+  CellLaunch launch() const {
+    const Level &L0 = c->lv[0];
+    return by4() ? cell_launch(L0.cx / 4, L0.cy, L0.cz) : ref_launch(c);
+  }
+  int blocks() const {
+    CellLaunch cl = launch();
+    return (int)((long)cl.grid.x * cl.grid.y * cl.grid.z);
+  }
   void apply(int j, const V &vv) {
     Level &L0 = c->lv[0];
-    CellLaunch cl = ref_launch(c);
+    CellLaunch cl = launch();
     ghost_sweep_level0<float>(c, lv_e<float>(L0));
     kbegin(c, PP_K_KRYLOV, 0, cells * (4.0 + 1.0 + 4.0 + 4.0 + 4.0 * (j + 1)));
-    k_kryl_apply<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), lv_e<float>(L0), z(j), u(j + 1), vv,
-                                                           c->d_part);
+    if (by4())
+      k_kryl_apply4<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), lv_e<float>(L0), z(j), u(j + 1),
+                                                              vv, c->d_part);
+    else
+      k_kryl_apply<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), lv_e<float>(L0), z(j), u(j + 1), vv,
+                                                             c->d_part);
     kend(c);
   }
   void orth(int j, const V &vv) {
     Level &L0 = c->lv[0];
-    CellLaunch cl = ref_launch(c);
+    CellLaunch cl = launch();
     kbegin(c, PP_K_KRYLOV, 0, cells * (1.0 + 8.0 + 4.0 * (j + 1) + 8.0));
-    k_kryl_orth<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), u(j + 1), vv, lv_rhs<float>(L0),
-                                                          lv_e<float>(L0), c->d_part);
+    if (by4())
+      k_kryl_orth4<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), u(j + 1), vv, lv_rhs<float>(L0),
+                                                             lv_e<float>(L0), c->d_part);
+    else
+      k_kryl_orth<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), u(j + 1), vv, lv_rhs<float>(L0),
+                                                            lv_e<float>(L0), c->d_part);
     kend(c);
   }
   void update(const V &vv) {
@@ -996,6 +1015,7 @@ struct AdjOps {
   double cells;
   double *u(int q) { return c->d_kry[q]; }
   double *z(int q) { return c->d_kry[c->kry_m + 1 + q]; }
+  int blocks() const { return c->nblk_red; }
   int precondition() {
     GhostTables &g = c->gh;
     Level &L0 = c->lv[0];

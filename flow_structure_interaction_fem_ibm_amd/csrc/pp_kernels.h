@@ -1386,6 +1386,130 @@ __global__ void __launch_bounds__(NT) k_kryl_orth(LevelArgs<double> L, float *__
   if (threadIdx.x == 0 && threadIdx.y == 0) part[blk] = d;
 }
 
+// Block sums of n <= KRY_MAX + 1 values per thread in one barrier: each
+// value is reduced across its wavefront by shuffles, the wavefront sums meet
+// in LDS and threads q < n add them (fixed order) into part[blk * n + q].
+template <int NT>
+__device__ inline void block_sums_to(double (&v)[KRY_MAX + 1], int n, double *s, double *part, long blk) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x, w = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int q = 0; q < KRY_MAX + 1; q++) {
+    if (q < n) {
+      double x = v[q];
+      for (int d = 32; d >= 1; d >>= 1) x += __shfl_down(x, d, 64);
+      if (lane == 0) s[w * (KRY_MAX + 1) + q] = x;
+    }
+  }
+  __syncthreads();
+  if (tid < n) {
+    double r = 0.0;
+    for (int ww = 0; ww < NT / 64; ww++) r += s[ww * (KRY_MAX + 1) + tid];
+    part[blk * n + tid] = r;
+  }
+}
+
+// k_kryl_apply for four consecutive cells per thread (rows whose interior
+// starts 16-byte aligned, cx % 4 == 0): vector loads of every field and all
+// dots reduced after one barrier.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_kryl_apply4(LevelArgs<double> L, const float *__restrict__ e0,
+                                                    float *__restrict__ z, float *__restrict__ w, KryVecsF V,
+                                                    double *__restrict__ part) {
+  __shared__ double s[(NT / 64) * (KRY_MAX + 1)];
+  const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  const bool in = i0 < L.cx && j < L.cy;
+  double acc[KRY_MAX + 1];
+#pragma unroll
+  for (int q = 0; q < KRY_MAX + 1; q++) acc[q] = 0.0;
+  if (in) {
+    const long a = lidx(L.sx, L.sxy, i0, j, k);
+    float zc[4], ejm[4], ejp[4], ekm[4], ekp[4], wf[4];
+    uint8_t fl[4];
+    double cmx[4], cpx[4], wd[4];
+    load_chunk<float, 4>(e0 + a, 4, zc);
+    load_chunk<float, 4>(e0 + a - L.sx, 4, ejm);
+    load_chunk<float, 4>(e0 + a + L.sx, 4, ejp);
+    load_chunk<float, 4>(e0 + a - L.sxy, 4, ekm);
+    load_chunk<float, 4>(e0 + a + L.sxy, 4, ekp);
+    load_flags<4>(L.fl + a, 4, fl);
+    load_chunk<double, 4>(L.cmx + i0, 4, cmx);
+    load_chunk<double, 4>(L.cpx + i0, 4, cpx);
+    const float el = e0[a - 1], er = e0[a + 4];
+    const double my = L.cmy[j], py = L.cpy[j], mz = L.cmz[k], pz = L.cpz[k];
+    store_chunk<float, 4>(z + a, 4, zc);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const uint8_t f = fl[t];
+      double wv = 0.0;
+      if (!(f & FL_BLANK)) {
+        double amx = (f & FL_IM) ? 0.0 : cmx[t], apx = (f & FL_IP) ? 0.0 : cpx[t];
+        double amy = (f & FL_JM) ? 0.0 : my, apy = (f & FL_JP) ? 0.0 : py;
+        double amz = (f & FL_KM) ? 0.0 : mz, apz = (f & FL_KP) ? 0.0 : pz;
+        double diag = (-(amx + apx) + -(amy + apy)) + -(amz + apz);
+        const float left = t == 0 ? el : zc[t - 1], right = t == 3 ? er : zc[t + 1];
+        wv = (double)zc[t] * diag + (double)left * amx + (double)right * apx + (double)ejm[t] * amy +
+             (double)ejp[t] * apy + (double)ekm[t] * amz + (double)ekp[t] * apz;
+      }
+      wf[t] = (float)wv;
+      wd[t] = (double)wf[t];
+    }
+    store_chunk<float, 4>(w + a, 4, wf);
+#pragma unroll
+    for (int q = 0; q < KRY_MAX + 1; q++) {
+      if (q < V.n) {
+        float uq[4];
+        load_chunk<float, 4>(V.u[q] + a, 4, uq);
+        acc[q] = wd[0] * (double)uq[0] + wd[1] * (double)uq[1] + wd[2] * (double)uq[2] + wd[3] * (double)uq[3];
+      }
+    }
+  }
+  const long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+  block_sums_to<NT>(acc, V.n, s, part, blk);
+}
+
+// k_kryl_orth for four consecutive cells per thread (cx % 4 == 0)
+template <int NT>
+__global__ void __launch_bounds__(NT) k_kryl_orth4(LevelArgs<double> L, float *__restrict__ w, KryVecsF V,
+                                                   float *__restrict__ rhs0, float *__restrict__ e0,
+                                                   double *__restrict__ part) {
+  __shared__ double s[NT / 64];
+  const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  double u2 = 0.0;
+  if (i0 < L.cx && j < L.cy) {
+    const long a = lidx(L.sx, L.sxy, i0, j, k);
+    float wv[4], uf[4];
+    uint8_t fl[4];
+    load_chunk<float, 4>(w + a, 4, wv);
+    load_flags<4>(L.fl + a, 4, fl);
+    double u[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) u[t] = (double)wv[t];
+#pragma unroll
+    for (int q = 0; q < KRY_MAX + 1; q++) {
+      if (q < V.n) {
+        float uq[4];
+        load_chunk<float, 4>(V.u[q] + a, 4, uq);
+#pragma unroll
+        for (int t = 0; t < 4; t++) u[t] -= V.c[q] * (double)uq[t];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      uf[t] = (fl[t] & FL_BLANK) ? 0.0f : (float)u[t];
+      u2 += (double)uf[t] * (double)uf[t];
+    }
+    store_chunk<float, 4>(w + a, 4, uf);
+    store_chunk<float, 4>(rhs0 + a, 4, uf);
+    const float zero[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    store_chunk<float, 4>(e0 + a, 4, zero);
+  }
+  const double d = block_sum<NT>(u2, s);
+  const long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[blk] = d;
+}
+
 // p (FP64, reference layout) += sum_q c_q z_q (FP32, level layout) on fluid cells
 __global__ void __launch_bounds__(256) k_kryl_update(RefArgs R, LevelArgs<double> L, double *__restrict__ p,
                                                      KryVecsF V) {
