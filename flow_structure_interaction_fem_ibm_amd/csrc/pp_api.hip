@@ -1,5 +1,6 @@
 // C-ABI entry points (include/pp_hip.h) and the multigrid driver.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -1174,19 +1175,26 @@ double fluid_volume(const pp_ctx *c, int i, int j, int k) {  // 0-based interior
 int compute_null_vector(pp_ctx *c) {
   std::vector<double> w(c->nref, 0.0), mu(c->nref, 0.0);
   const bool warm = (long)c->h_mu_prev.size() == c->nref;
-  double vsum = 0.0;
-  for (int k = 0; k < c->nzc; k++)
-    for (int j = 0; j < c->nyc; j++)
-      for (int i = 0; i < c->nxc; i++) {
-        long r = ridx(c->ref, i, j, k);
-        if (!c->h_iblank[r]) {
-          w[r] = fluid_volume(c, i, j, k);
-          // start from the previous geometry's null vector where it exists
-          // (moving bodies), from the fluid indicator elsewhere
-          mu[r] = (warm && c->h_mu_prev[r] != 0.0) ? c->h_mu_prev[r] : 1.0;
-          vsum += w[r];
+  std::vector<double> vpart(c->nzc, 0.0);
+  host_parallel(c->nzc, [&](long k0, long k1) {
+    for (int k = (int)k0; k < (int)k1; k++) {
+      double vs = 0.0;
+      for (int j = 0; j < c->nyc; j++)
+        for (int i = 0; i < c->nxc; i++) {
+          long r = ridx(c->ref, i, j, k);
+          if (!c->h_iblank[r]) {
+            w[r] = fluid_volume(c, i, j, k);
+            // start from the previous geometry's null vector where it exists
+            // (moving bodies), from the fluid indicator elsewhere
+            mu[r] = (warm && c->h_mu_prev[r] != 0.0) ? c->h_mu_prev[r] : 1.0;
+            vs += w[r];
+          }
         }
-      }
+      vpart[k] = vs;
+    }
+  });
+  double vsum = 0.0;
+  for (int k = 0; k < c->nzc; k++) vsum += vpart[k];
   if (c->gh.n > 0) {
     HIPCHK(hipMemcpy(c->d_mu, mu.data(), sizeof(double) * c->nref, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(c->d_sv, 0, sizeof(double) * c->nref));
@@ -1211,18 +1219,27 @@ int compute_null_vector(pp_ctx *c) {
     if (!warm) c->null_ref = st.b_l2;
     std::vector<double> lg(c->gh.n);
     HIPCHK(hipMemcpy(lg.data(), c->gh.olam64[lam], sizeof(double) * c->gh.n, hipMemcpyDeviceToHost));
-    double wsum = 0.0;
-    for (int k = 0; k < c->nzc; k++)
-      for (int j = 0; j < c->nyc; j++)
-        for (int i = 0; i < c->nxc; i++) {
-          long r = ridx(c->ref, i, j, k);
-          if (!c->h_iblank[r]) {
-            w[r] = fluid_volume(c, i, j, k) * mu[r];
-            wsum += w[r];
+    std::vector<double> wpart(c->nzc, 0.0);
+    host_parallel(c->nzc, [&](long k0, long k1) {
+      for (int k = (int)k0; k < (int)k1; k++) {
+        double ws = 0.0;
+        for (int j = 0; j < c->nyc; j++)
+          for (int i = 0; i < c->nxc; i++) {
+            long r = ridx(c->ref, i, j, k);
+            if (!c->h_iblank[r]) {
+              w[r] = fluid_volume(c, i, j, k) * mu[r];
+              ws += w[r];
+            }
           }
-        }
+        wpart[k] = ws;
+      }
+    });
+    double wsum = 0.0;
+    for (int k = 0; k < c->nzc; k++) wsum += wpart[k];
     double scale = vsum / wsum;
-    for (long r = 0; r < c->nref; r++) w[r] *= scale;
+    host_parallel(c->nref, [&](long a, long b) {
+      for (long r = a; r < b; r++) w[r] *= scale;
+    });
     for (int q = 0; q < c->gh.n; q++) w[c->gh.h_g_ref[q]] = lg[q] * scale;
     c->null_iters = st.outer_iterations;
     c->null_res = st.rel_l2;
@@ -1368,17 +1385,26 @@ int32_t pp_set_geometry(pp_ctx *c, const pp_geometry *geo) {
     c->graph_key = -1;
   }
   c->h_iblank.assign(geo->iblank, geo->iblank + c->nref);
-  for (long r = 0; r < c->nref; r++)
-    if (c->h_iblank[r] != 0 && c->h_iblank[r] != 1) {
+  {
+    std::atomic<long> bad(-1);
+    host_parallel(c->nref, [&](long a, long b) {
+      for (long r = a; r < b; r++)
+        if (c->h_iblank[r] != 0 && c->h_iblank[r] != 1) { bad.store(r); break; }
+    });
+    if (bad.load() >= 0) {
+      long r = bad.load();
       set_error("pp_set_geometry: iblank must be 0 or 1 (entry %ld = %d)", r, (int)c->h_iblank[r]);
       return PP_BAD_GEOMETRY;
     }
+  }
   // the outer layer carries no geometry
-  for (int k = 0; k < c->nzc + 3; k++)
-    for (int j = 0; j < c->nyc + 3; j++)
-      for (int i = 0; i < c->nxc + 3; i++)
-        if (i == 0 || j == 0 || k == 0 || i > c->nxc || j > c->nyc || k > c->nzc)
-          c->h_iblank[(long)i + c->ref.ex * j + c->ref.exy * k] = 0;
+  host_parallel(c->nzc + 3, [&](long k0, long k1) {
+    for (int k = (int)k0; k < (int)k1; k++)
+      for (int j = 0; j < c->nyc + 3; j++)
+        for (int i = 0; i < c->nxc + 3; i++)
+          if (i == 0 || j == 0 || k == 0 || i > c->nxc || j > c->nyc || k > c->nzc)
+            c->h_iblank[(long)i + c->ref.ex * j + c->ref.exy * k] = 0;
+  });
   derive_ghost_mark(c);
   long nmark = 0;
   for (long r = 0; r < c->nref; r++) nmark += c->h_ghost[r];

@@ -1215,6 +1215,28 @@ struct KryVecs {
   int n;
 };
 
+// Block sums of n <= KRY_MAX + 1 values per thread in one barrier: each
+// value is reduced across its wavefront by shuffles, the wavefront sums meet
+// in LDS and threads q < n add them (fixed order) into part[blk * n + q].
+template <int NT>
+__device__ inline void block_sums_to(double (&v)[KRY_MAX + 1], int n, double *s, double *part, long blk) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x, w = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int q = 0; q < KRY_MAX + 1; q++) {
+    if (q < n) {
+      double x = v[q];
+      for (int d = 32; d >= 1; d >>= 1) x += __shfl_down(x, d, 64);
+      if (lane == 0) s[w * (KRY_MAX + 1) + q] = x;
+    }
+  }
+  __syncthreads();
+  if (tid < n) {
+    double r = 0.0;
+    for (int ww = 0; ww < NT / 64; ww++) r += s[ww * (KRY_MAX + 1) + tid];
+    part[blk * n + tid] = r;
+  }
+}
+
 // z = e0 on fluid cells, 0 on blank cells (ghost rows are filled afterwards)
 template <typename TE>
 __global__ void __launch_bounds__(256) k_kry_z(RefArgs R, LevelArgs<double> L, const TE *__restrict__ e0,
@@ -1234,7 +1256,7 @@ template <int NT>
 __global__ void __launch_bounds__(NT) k_kry_apply(RefArgs R, LevelArgs<double> L, const double *__restrict__ z,
                                                   double *__restrict__ w, KryVecs V, double *__restrict__ part,
                                                   const double *__restrict__ bsub) {
-  __shared__ double s[NT / 64];
+  __shared__ double s[(NT / 64) * (KRY_MAX + 1)];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
   bool in = i < R.nxc && j < R.nyc;
@@ -1252,14 +1274,11 @@ __global__ void __launch_bounds__(NT) k_kry_apply(RefArgs R, LevelArgs<double> L
     }
     w[r] = wv;
   }
-  long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+  double acc[KRY_MAX + 1];
 #pragma unroll
-  for (int q = 0; q < KRY_MAX + 1; q++) {
-    if (q < V.n) {
-      double d = block_sum<NT>(in ? wv * V.u[q][r] : 0.0, s);
-      if (threadIdx.x == 0 && threadIdx.y == 0) part[blk * V.n + q] = d;
-    }
-  }
+  for (int q = 0; q < KRY_MAX + 1; q++) acc[q] = (in && q < V.n) ? wv * V.u[q][r] : 0.0;
+  long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+  block_sums_to<NT>(acc, V.n, s, part, blk);
 }
 
 // u = w - sum_q c_q u_q on fluid cells (in place in w), level-0 rhs = u,
@@ -1384,28 +1403,6 @@ __global__ void __launch_bounds__(NT) k_kryl_orth(LevelArgs<double> L, float *__
   double d = block_sum<NT>(u2, s);
   long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
   if (threadIdx.x == 0 && threadIdx.y == 0) part[blk] = d;
-}
-
-// Block sums of n <= KRY_MAX + 1 values per thread in one barrier: each
-// value is reduced across its wavefront by shuffles, the wavefront sums meet
-// in LDS and threads q < n add them (fixed order) into part[blk * n + q].
-template <int NT>
-__device__ inline void block_sums_to(double (&v)[KRY_MAX + 1], int n, double *s, double *part, long blk) {
-  const int tid = threadIdx.x + threadIdx.y * blockDim.x, w = tid >> 6, lane = tid & 63;
-#pragma unroll
-  for (int q = 0; q < KRY_MAX + 1; q++) {
-    if (q < n) {
-      double x = v[q];
-      for (int d = 32; d >= 1; d >>= 1) x += __shfl_down(x, d, 64);
-      if (lane == 0) s[w * (KRY_MAX + 1) + q] = x;
-    }
-  }
-  __syncthreads();
-  if (tid < n) {
-    double r = 0.0;
-    for (int ww = 0; ww < NT / 64; ww++) r += s[ww * (KRY_MAX + 1) + tid];
-    part[blk * n + tid] = r;
-  }
 }
 
 // k_kryl_apply for four consecutive cells per thread (rows whose interior

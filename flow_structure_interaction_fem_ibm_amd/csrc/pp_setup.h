@@ -1,10 +1,13 @@
 // Host-side set-up routines shared by the API translation unit.
 #pragma once
 
+#include <functional>
+
 #include "pp_internal.h"
 
 namespace pp {
 const char *last_error();
+void host_parallel(long n, const std::function<void(long, long)> &fn);
 void make_grid_dir(int n, const double *xn, std::vector<double> &x, std::vector<double> &xc,
                    std::vector<double> &dx, std::vector<double> &dxinv, std::vector<double> &dxc,
                    std::vector<double> &dxcinv);

@@ -6,6 +6,8 @@
 #include <cstdio>
 #include <cstring>
 #include <random>
+#include <thread>
+#include <vector>
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -27,6 +29,20 @@ void set_error(const char *fmt, ...) {
   g_err = buf;
 }
 const char *last_error() { return g_err.c_str(); }
+
+// fn(a, b) over [0, n) split across up to 16 host threads (contiguous ranges)
+void host_parallel(long n, const std::function<void(long, long)> &fn) {
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 64 || nt == 1) { fn(0, n); return; }
+  nt = (unsigned)std::min<long>(nt, n);
+  std::vector<std::thread> th;
+  const long chunk = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; t++) {
+    const long a = t * chunk, b = std::min(n, a + chunk);
+    if (a < b) th.emplace_back([&fn, a, b] { fn(a, b); });
+  }
+  for (auto &t : th) t.join();
+}
 
 // make_grid (CORE_GRID.F90:29-151) for one direction.  xn = x(1..n),
 // outputs indexed 0..n+1 like the reference arrays.
@@ -205,30 +221,35 @@ void build_flags(pp_ctx *c) {
     std::vector<uint8_t> blank((size_t)L.cx * L.cy * L.cz, 0), ghost((size_t)L.cx * L.cy * L.cz, 0);
     auto cidx = [&](int i, int j, int k) { return (size_t)i + (size_t)L.cx * (j + (size_t)L.cy * k); };
     if (l == 0) {
-      for (int k = 0; k < L.cz; k++)
-        for (int j = 0; j < L.cy; j++)
-          for (int i = 0; i < L.cx; i++) {
-            long r = ridx(c->ref, i, j, k);
-            blank[cidx(i, j, k)] = c->h_iblank[r] ? 1 : 0;
-            ghost[cidx(i, j, k)] = c->h_ghost[r] ? 1 : 0;
-          }
+      host_parallel(L.cz, [&](long k0, long k1) {
+        for (int k = (int)k0; k < (int)k1; k++)
+          for (int j = 0; j < L.cy; j++)
+            for (int i = 0; i < L.cx; i++) {
+              long r = ridx(c->ref, i, j, k);
+              blank[cidx(i, j, k)] = c->h_iblank[r] ? 1 : 0;
+              ghost[cidx(i, j, k)] = c->h_ghost[r] ? 1 : 0;
+            }
+      });
     } else {
       Level &F = c->lv[l - 1];
       const std::vector<uint8_t> &fh = F.hflags;
-      for (int k = 0; k < L.cz; k++)
-        for (int j = 0; j < L.cy; j++)
-          for (int i = 0; i < L.cx; i++) {
-            int all = 1;
-            for (int c3 = 0; c3 < L.cf[2]; c3++)
-              for (int b = 0; b < L.cf[1]; b++)
-                for (int a = 0; a < L.cf[0]; a++) {
-                  long q = lidx(F.sx, F.sxy, i * L.cf[0] + a, j * L.cf[1] + b, k * L.cf[2] + c3);
-                  all &= (fh[q] & FL_BLANK) ? 1 : 0;
-                }
-            blank[cidx(i, j, k)] = (uint8_t)all;
-          }
+      host_parallel(L.cz, [&](long k0, long k1) {
+        for (int k = (int)k0; k < (int)k1; k++)
+          for (int j = 0; j < L.cy; j++)
+            for (int i = 0; i < L.cx; i++) {
+              int all = 1;
+              for (int c3 = 0; c3 < L.cf[2]; c3++)
+                for (int b = 0; b < L.cf[1]; b++)
+                  for (int a = 0; a < L.cf[0]; a++) {
+                    long q = lidx(F.sx, F.sxy, i * L.cf[0] + a, j * L.cf[1] + b, k * L.cf[2] + c3);
+                    all &= (fh[q] & FL_BLANK) ? 1 : 0;
+                  }
+              blank[cidx(i, j, k)] = (uint8_t)all;
+            }
+      });
     }
-    for (int k = 0; k < L.cz; k++)
+    host_parallel(L.cz, [&](long k0, long k1) {
+    for (int k = (int)k0; k < (int)k1; k++)
       for (int j = 0; j < L.cy; j++)
         for (int i = 0; i < L.cx; i++) {
           uint8_t f = 0;
@@ -251,6 +272,7 @@ void build_flags(pp_ctx *c) {
           }
           L.hflags[lidx(L.sx, L.sxy, i, j, k)] = f;
         }
+    });
   }
 }
 
@@ -297,17 +319,19 @@ void derive_ghost_mark(pp_ctx *c) {
   c->h_ghost.assign(c->nref, 0);
   int nx = c->nxc + 1, ny = c->nyc + 1, nz = c->nzc + 1;
   auto R = [&](int i, int j, int k) { return (long)i + c->ref.ex * j + c->ref.exy * k; };
-  for (int k = 1; k <= nz - 1; k++)
-    for (int j = 1; j <= ny - 1; j++)
-      for (int i = 1; i <= nx - 1; i++) {
-        int iM = std::max(i - 1, 1), iP = std::min(i + 1, nx - 1);
-        int jM = std::max(j - 1, 1), jP = std::min(j + 1, ny - 1);
-        int kM = std::max(k - 1, 1), kP = std::min(k + 1, nz - 1);
-        int b = ib[R(i, j, k)];
-        if (b == 1 && (ib[R(iM, j, k)] == 0 || ib[R(iP, j, k)] == 0 || ib[R(i, jM, k)] == 0 ||
-                       ib[R(i, jP, k)] == 0 || ib[R(i, j, kM)] == 0 || ib[R(i, j, kP)] == 0))
-          c->h_ghost[R(i, j, k)] = 1;
-      }
+  host_parallel(nz - 1, [&](long k0, long k1) {
+    for (int k = (int)k0 + 1; k <= (int)k1; k++)
+      for (int j = 1; j <= ny - 1; j++)
+        for (int i = 1; i <= nx - 1; i++) {
+          int iM = std::max(i - 1, 1), iP = std::min(i + 1, nx - 1);
+          int jM = std::max(j - 1, 1), jP = std::min(j + 1, ny - 1);
+          int kM = std::max(k - 1, 1), kP = std::min(k + 1, nz - 1);
+          int b = ib[R(i, j, k)];
+          if (b == 1 && (ib[R(iM, j, k)] == 0 || ib[R(iP, j, k)] == 0 || ib[R(i, jM, k)] == 0 ||
+                         ib[R(i, jP, k)] == 0 || ib[R(i, j, kM)] == 0 || ib[R(i, j, kP)] == 0))
+            c->h_ghost[R(i, j, k)] = 1;
+        }
+  });
 }
 
 static const int incI[8] = {0, 0, 1, 1, 0, 0, 1, 1};
@@ -367,24 +391,46 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
   }
   // Jacobi sweeps that solve the ghost-ghost coupling to round-off
   // (the reference iterates Gauss-Seidel to restol_ad).
+  // ghost index of a reference offset (-1: not a ghost); the ghost list is
+  // in loop order (k, j, i), i.e. sorted by offset
+  std::vector<long> gsorted(G.h_g_ref.begin(), G.h_g_ref.end());
+  std::vector<int> gorder(n);
+  for (int q = 0; q < n; q++) gorder[q] = q;
+  if (!std::is_sorted(gsorted.begin(), gsorted.end())) {
+    std::sort(gorder.begin(), gorder.end(), [&](int a, int b) { return G.h_g_ref[a] < G.h_g_ref[b]; });
+    for (int q = 0; q < n; q++) gsorted[q] = G.h_g_ref[gorder[q]];
+  }
+  auto ghost_of = [&](long r) -> int {
+    auto it = std::lower_bound(gsorted.begin(), gsorted.end(), r);
+    return (it != gsorted.end() && *it == r) ? gorder[it - gsorted.begin()] : -1;
+  };
   {
-    std::vector<double> p(c->nref, 0.0), nw(n);
-    std::mt19937_64 rng(12345);
-    std::uniform_real_distribution<double> U(-1.0, 1.0);
-    for (long r = 0; r < c->nref; r++) p[r] = c->h_iblank[r] ? 0.0 : U(rng);
-    for (int q = 0; q < n; q++) p[G.h_g_ref[q]] = 0.0;
+    // stencil values: ghosts iterate, fluid cells hold fixed pseudo-random
+    // values (a hash of the offset), solid cells 0
+    std::vector<int> sg(8L * n);
+    std::vector<double> sv(8L * n), gv(n, 0.0), nw(n);
+    for (long e = 0; e < 8L * n; e++) {
+      long r = G.h_s_ref[e];
+      sg[e] = ghost_of(r);
+      uint64_t h = (uint64_t)r * 0x9E3779B97F4A7C15ull + 12345u;
+      h ^= h >> 31; h *= 0xBF58476D1CE4E5B9ull; h ^= h >> 27; h *= 0x94D049BB133111EBull; h ^= h >> 33;
+      sv[e] = c->h_iblank[r] ? 0.0 : (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
     int it = 0;
     double dmax = 1.0;
     while (it < 200 && dmax > 1e-15) {
       dmax = 0.0;
       for (int q = 0; q < n; q++) {
         double s = 0.0;
-        for (int r = 0; r < 8; r++) s += G.h_coef[8L * q + r] * p[G.h_s_ref[8L * q + r]];
+        for (int r = 0; r < 8; r++) {
+          long e = 8L * q + r;
+          s += G.h_coef[e] * (sg[e] >= 0 ? gv[sg[e]] : sv[e]);
+        }
         nw[q] = s;
       }
       for (int q = 0; q < n; q++) {
-        dmax = std::max(dmax, std::fabs(nw[q] - p[G.h_g_ref[q]]));
-        p[G.h_g_ref[q]] = nw[q];
+        dmax = std::max(dmax, std::fabs(nw[q] - gv[q]));
+        gv[q] = nw[q];
       }
       it++;
     }
@@ -400,8 +446,6 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
   // becomes one gather over fluid cells.  Stencil entries on solid
   // non-ghost cells read 0 (blank non-ghost cells hold 0 throughout).
   {
-    std::vector<int> gpos(c->nref, -1);
-    for (int q = 0; q < n; q++) gpos[G.h_g_ref[q]] = q;
     using Row = std::vector<std::pair<long, double>>;
     auto merge = [](Row &r) {
       std::sort(r.begin(), r.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
@@ -419,8 +463,9 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
         double cf = G.h_coef[8L * q + r];
         long rs = G.h_s_ref[8L * q + r];
         if (cf == 0.0 || rs == G.h_g_ref[q]) continue;
-        if (gpos[rs] >= 0) {
-          G.h_gg_idx.push_back(gpos[rs]);
+        const int gq = ghost_of(rs);
+        if (gq >= 0) {
+          G.h_gg_idx.push_back(gq);
           G.h_gg_w.push_back(cf);
         } else if (c->h_iblank[rs] == 0) {
           cgf[q].push_back({rs, cf});
@@ -498,8 +543,6 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     std::stable_sort(ent.begin(), ent.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
     G.h_t_ref.clear(); G.h_t_al.clear(); G.h_t_ptr.assign(1, 0); G.h_t_ghost.clear(); G.h_t_coef.clear();
     G.h_t_vinv.clear();
-    std::vector<long> gpos(c->nref, -1);
-    for (int q = 0; q < n; q++) gpos[G.h_g_ref[q]] = q;
     G.h_g_tidx.assign(n, -1);
     for (size_t e = 0; e < ent.size(); e++) {
       long rs = ent[e].first;
@@ -511,7 +554,8 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
         G.h_t_al.push_back(lidx(L0.sx, L0.sxy, (int)x - 1, (int)y - 1, (int)z - 1));
         bool fl = c->h_iblank[rs] == 0;
         G.h_t_vinv.push_back(fl ? 1.0 / V((int)x, (int)y, (int)z) : 0.0);
-        if (gpos[rs] >= 0) G.h_g_tidx[gpos[rs]] = t;
+        const int gq = ghost_of(rs);
+        if (gq >= 0) G.h_g_tidx[gq] = t;
       }
       G.h_t_ghost.push_back(ent[e].second.first);
       G.h_t_coef.push_back(ent[e].second.second);
