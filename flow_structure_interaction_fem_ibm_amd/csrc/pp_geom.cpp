@@ -145,70 +145,87 @@ void closest_point_analytic(const Body &b, const double *p, double *bi, double *
 // ---------------------------------------------------------------------
 
 // find_isolatedIblank(1) (BOUNDARY_SET.F90:440-499): a blank cell with no
-// other blank cell in its clamped 3x3x3 neighbourhood is cleared (in place,
-// in loop order) and loses its body number.
+// other blank cell in its clamped 3x3x3 neighbourhood is cleared and loses
+// its body number.  The reference clears in place in loop order; that equals
+// testing every cell against the original field: a neighbour visited earlier
+// saw this (still blank) cell and so was not cleared, and later neighbours
+// are unchanged at this cell's visit.  Hence a parallel two-pass form.
 void isolated_iblank(const Grid &G, std::vector<int8_t> &ib, std::vector<int> &bodynum) {
-  for (int k = 1; k <= G.nzc; k++)
-    for (int j = 1; j <= G.nyc; j++)
-      for (int i = 1; i <= G.nxc; i++) {
-        if (ib[G.r(i, j, k)] != 1) continue;
-        int s = 0;
-        for (int kk = std::max(1, k - 1); kk <= std::min(G.nzc, k + 1); kk++)
-          for (int jj = std::max(1, j - 1); jj <= std::min(G.nyc, j + 1); jj++)
-            for (int ii = std::max(1, i - 1); ii <= std::min(G.nxc, i + 1); ii++) s += ib[G.r(ii, jj, kk)];
-        if (s <= 1) {
-          ib[G.r(i, j, k)] = 0;
-          bodynum[G.r(i, j, k)] = -1;
+  std::vector<std::vector<long>> clear(G.nzc + 1);
+  parallel_for(G.nzc, [&](long k0, long k1) {
+    for (int k = (int)k0 + 1; k <= (int)k1; k++)
+      for (int j = 1; j <= G.nyc; j++)
+        for (int i = 1; i <= G.nxc; i++) {
+          if (ib[G.r(i, j, k)] != 1) continue;
+          int s = 0;
+          for (int kk = std::max(1, k - 1); kk <= std::min(G.nzc, k + 1); kk++)
+            for (int jj = std::max(1, j - 1); jj <= std::min(G.nyc, j + 1); jj++)
+              for (int ii = std::max(1, i - 1); ii <= std::min(G.nxc, i + 1); ii++) s += ib[G.r(ii, jj, kk)];
+          if (s <= 1) clear[k].push_back(G.r(i, j, k));
         }
-      }
+  });
+  for (auto &v : clear)
+    for (long r : v) {
+      ib[r] = 0;
+      bodynum[r] = -1;
+    }
 }
 
 // fill_cell (SSM_SET_BC.F90:1425-1511) with the outer-face flags of
 // set_outer_iup: a fluid cell on an outer face whose inward neighbour
-// along that axis is blank becomes blank (sequential sweeps in x, y, z).
+// along that axis is blank becomes blank (sweeps in x, then y, then z).
+// Only cells on the two faces normal to the sweep's axis can change, and
+// (at least 3 cells per direction) none of them is another's neighbour, so
+// each sweep visits just those faces.
 void fill_cell(const Grid &G, std::vector<int8_t> &ib, std::vector<int8_t> &fresh) {
-  auto sweep = [&](int axis) {
-    for (int k = 1; k <= G.nzc; k++)
-      for (int j = 1; j <= G.nyc; j++)
-        for (int i = 1; i <= G.nxc; i++) {
-          long r = G.r(i, j, k);
+  const int n[3] = {G.nxc, G.nyc, G.nzc};
+  const long st[3] = {1, G.ex, G.exy};
+  for (int axis = 0; axis < 3; axis++) {
+    const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+    for (int face = 0; face < 2; face++) {
+      const int c = face == 0 ? 1 : n[axis];
+      for (int v = 1; v <= n[a2]; v++)
+        for (int u = 1; u <= n[a1]; u++) {
+          int idx[3];
+          idx[axis] = c; idx[a1] = u; idx[a2] = v;
+          const long r = G.r(idx[0], idx[1], idx[2]);
           if (ib[r] != 0) continue;
-          long st = axis == 0 ? 1 : axis == 1 ? G.ex : G.exy;
-          int c = axis == 0 ? i : axis == 1 ? j : k;
-          int n = axis == 0 ? G.nxc : axis == 1 ? G.nyc : G.nzc;
-          bool up = (c == n), um = (c == 1);  // iup / ium of set_outer_iup
-          if ((ib[r - st] == 1 && up) || (ib[r + st] == 1 && um)) {
+          const bool up = (c == n[axis]), um = (c == 1);
+          if ((ib[r - st[axis]] == 1 && up) || (ib[r + st[axis]] == 1 && um)) {
             ib[r] = 1;
             if (fresh[r] == 1) fresh[r] = 0;
           }
         }
-  };
-  sweep(0);
-  sweep(1);
-  sweep(2);
+    }
+  }
 }
 
-// GCM ghost marking and list (GCM_SET_BOUNDARY.F90:49-118)
+// GCM ghost marking and list (GCM_SET_BOUNDARY.F90:49-118); per-plane lists
+// concatenated in plane order keep the reference's i-fastest list order
 void mark_ghosts(const Grid &G, const std::vector<int8_t> &ib, std::vector<int8_t> &gm, std::vector<int> &list) {
   gm.assign(G.size, 0);
   list.clear();
-  int nx = G.nxc + 1, ny = G.nyc + 1, nz = G.nzc + 1;
-  for (int k = 1; k <= nz - 1; k++)
-    for (int j = 1; j <= ny - 1; j++)
-      for (int i = 1; i <= nx - 1; i++) {
-        long r = G.r(i, j, k);
-        if (ib[r] != 1) continue;
-        int iM = std::max(i - 1, 1), iP = std::min(i + 1, nx - 1);
-        int jM = std::max(j - 1, 1), jP = std::min(j + 1, ny - 1);
-        int kM = std::max(k - 1, 1), kP = std::min(k + 1, nz - 1);
-        if (ib[G.r(iM, j, k)] == 0 || ib[G.r(iP, j, k)] == 0 || ib[G.r(i, jM, k)] == 0 ||
-            ib[G.r(i, jP, k)] == 0 || ib[G.r(i, j, kM)] == 0 || ib[G.r(i, j, kP)] == 0) {
-          gm[r] = 1;
-          list.push_back(i);
-          list.push_back(j);
-          list.push_back(k);
+  const int nx = G.nxc + 1, ny = G.nyc + 1, nz = G.nzc + 1;
+  std::vector<std::vector<int>> plane(nz);
+  parallel_for(nz - 1, [&](long k0, long k1) {
+    for (int k = (int)k0 + 1; k <= (int)k1; k++)
+      for (int j = 1; j <= ny - 1; j++)
+        for (int i = 1; i <= nx - 1; i++) {
+          long r = G.r(i, j, k);
+          if (ib[r] != 1) continue;
+          int iM = std::max(i - 1, 1), iP = std::min(i + 1, nx - 1);
+          int jM = std::max(j - 1, 1), jP = std::min(j + 1, ny - 1);
+          int kM = std::max(k - 1, 1), kP = std::min(k + 1, nz - 1);
+          if (ib[G.r(iM, j, k)] == 0 || ib[G.r(iP, j, k)] == 0 || ib[G.r(i, jM, k)] == 0 ||
+              ib[G.r(i, jP, k)] == 0 || ib[G.r(i, j, kM)] == 0 || ib[G.r(i, j, kP)] == 0) {
+            gm[r] = 1;
+            plane[k].push_back(i);
+            plane[k].push_back(j);
+            plane[k].push_back(k);
+          }
         }
-      }
+  });
+  for (auto &v : plane) list.insert(list.end(), v.begin(), v.end());
 }
 
 static const int incI[8] = {0, 0, 1, 1, 0, 0, 1, 1};
@@ -381,12 +398,14 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
   }
   // fresh cells: fluid now, blank before (BOUNDARY_IBLANK_FAST.F90:451-457)
   if (iblank_prev)
-    for (int k = 1; k <= G.nzc; k++)
-      for (int j = 1; j <= G.nyc; j++)
-        for (int i = 1; i <= G.nxc; i++) {
-          long r = G.r(i, j, k);
-          if (o->iblank[r] == 0 && iblank_prev[r] == 1) o->fresh[r] = 1;
-        }
+    parallel_for(G.nzc, [&](long k0, long k1) {
+      for (int k = (int)k0 + 1; k <= (int)k1; k++)
+        for (int j = 1; j <= G.nyc; j++)
+          for (int i = 1; i <= G.nxc; i++) {
+            long r = G.r(i, j, k);
+            if (o->iblank[r] == 0 && iblank_prev[r] == 1) o->fresh[r] = 1;
+          }
+    });
   isolated_iblank(G, o->iblank, bodynum);
   fill_cell(G, o->iblank, o->fresh);
   mark_ghosts(G, o->iblank, o->ghost, o->list);
@@ -435,7 +454,14 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
       delete o;
       return PP_BAD_GEOMETRY;
     }
-  for (long r = 0; r < G.size; r++) { o->nblank += o->iblank[r]; o->nfresh += o->fresh[r]; }
+  {
+    std::vector<long> nb(G.nzc + 3, 0), nf(G.nzc + 3, 0);
+    parallel_for(G.nzc + 3, [&](long k0, long k1) {
+      for (long k = k0; k < k1; k++)
+        for (long r = k * G.exy; r < (k + 1) * G.exy; r++) { nb[k] += o->iblank[r]; nf[k] += o->fresh[r]; }
+    });
+    for (int k = 0; k < G.nzc + 3; k++) { o->nblank += (int)nb[k]; o->nfresh += (int)nf[k]; }
+  }
   *out = o;
   return PP_OK;
 }
