@@ -1137,6 +1137,16 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
   const T *cpl = DIR == 1 ? Lv.cpy : Lv.cpz;
   long id0 = 0;
   if (ok) id0 = DIR == 1 ? lidx(Lv.sx, Lv.sxy, i, m0, plane) : lidx(Lv.sx, Lv.sxy, i, plane, m0);
+  // the line coefficients of this segment, loaded once ahead of the cells
+  // (wavefront-uniform scalar loads when the tile is 64 lines wide)
+  T cq_m[LS], cq_p[LS];
+  const int m0u = blockDim.x >= 64 ? __builtin_amdgcn_readfirstlane(m0) : m0;
+#pragma unroll
+  for (int m = 0; m < LS; m++) {
+    const int q = m0u + m;
+    cq_m[m] = q < n ? cml[q] : T(0);
+    cq_p[m] = q < n ? cpl[q] : T(0);
+  }
   T eo[LS];
   SegElim<T, LS> se;
 #pragma unroll
@@ -1154,7 +1164,7 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
       } else {
         T amx = (f & FL_IM) ? T(0) : mx, apx = (f & FL_IP) ? T(0) : px;
         T amo = (f & FO_M) ? T(0) : mo, apo = (f & FO_P) ? T(0) : po;
-        T am = (f & FL_M) ? T(0) : cml[q], ap = (f & FL_P) ? T(0) : cpl[q];
+        T am = (f & FL_M) ? T(0) : cq_m[m], ap = (f & FL_P) ? T(0) : cq_p[m];
         // (acx + acy + acz) in the reference's order
         T diag = DIR == 1 ? (-(amx + apx) + -(am + ap)) + -(amo + apo) : (-(amx + apx) + -(amo + apo)) + -(am + ap);
         T r = rhs[id] - e[id - 1] * amx - e[id + 1] * apx - e[id - ostride] * amo - e[id + ostride] * apo;
