@@ -234,11 +234,19 @@ void launch_yzline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
   const T *rhs = lv_rhs<T>(L);
   hipStream_t st = c->stream;
   kbegin(c, DIR == 1 ? PP_K_YLINE : PP_K_ZLINE, lvl, 0.5 * (double)L.cx * L.cy * L.cz * bytes_line(sizeof(T)));
+  // loads issued up front on the narrow-tile (latency-bound) FP32 levels
+  const bool upf = sizeof(T) == 4 && TX == 8;
   switch (LS) {
     case 1: k_yzline<T, 1, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
     case 2: k_yzline<T, 2, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
-    case 4: k_yzline<T, 4, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
-    default: k_yzline<T, 8, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega); break;
+    case 4:
+      if (upf) k_yzline<T, 4, DIR, true><<<grid, block, sh, st>>>(a, rhs, e, color, omega);
+      else k_yzline<T, 4, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega);
+      break;
+    default:
+      if (upf) k_yzline<T, 8, DIR, true><<<grid, block, sh, st>>>(a, rhs, e, color, omega);
+      else k_yzline<T, 8, DIR><<<grid, block, sh, st>>>(a, rhs, e, color, omega);
+      break;
   }
   kend(c);
 }

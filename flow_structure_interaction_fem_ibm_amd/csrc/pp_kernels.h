@@ -1109,7 +1109,7 @@ __device__ inline void seg_reduced_solve(T *sh, int P, int tx, int seg, SegElim<
 // and every thread back-substitutes.  The tile's loads cover 2*TX consecutive i, half of
 // them own cells and half the neighbour lines read by the stencil.
 // ---------------------------------------------------------------------
-template <typename T, int LS, int DIR>
+template <typename T, int LS, int DIR, bool UPF = false>
 __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__restrict__ rhs, T *__restrict__ e,
                                                  int color, T omega) {
   extern __shared__ unsigned char smem_raw[];
@@ -1147,6 +1147,29 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
     cq_m[m] = q < n ? cml[q] : T(0);
     cq_p[m] = q < n ? cpl[q] : T(0);
   }
+  // UPF: all loads of the segment are issued unconditionally (cells past the
+  // line or the level read a valid cell instead and are then ignored), so
+  // that they are all in flight before the first row is built (latency-bound
+  // small levels; on large levels the extra registers cost occupancy)
+  const long idsafe = lidx(Lv.sx, Lv.sxy, 0, 0, 0);
+  constexpr int NU = UPF ? LS : 1;
+  uint8_t fv[NU];
+  T evv[NU], elv[NU], erv[NU], ezm[NU], ezp[NU], rvv[NU];
+  auto load_cell = [&](int m, int slot) {
+    const bool live = ok && m0 + m < n;
+    const long id = live ? id0 + m * lstride : idsafe;
+    fv[slot] = Lv.fl[id];
+    evv[slot] = e[id];
+    elv[slot] = e[id - 1];
+    erv[slot] = e[id + 1];
+    ezm[slot] = e[id - ostride];
+    ezp[slot] = e[id + ostride];
+    rvv[slot] = rhs[id];
+  };
+  if constexpr (UPF) {
+#pragma unroll
+    for (int m = 0; m < LS; m++) load_cell(m, m);
+  }
   T eo[LS];
   SegElim<T, LS> se;
 #pragma unroll
@@ -1154,10 +1177,13 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
     T a = T(0), b = T(1), c = T(0), d = T(0);
     eo[m] = T(0);
     int q = m0 + m;
+    const int sl = UPF ? m : 0;
+    if constexpr (!UPF) {
+      if (ok && q < n) load_cell(m, 0);
+    }
     if (ok && q < n) {
-      long id = id0 + m * lstride;
-      uint8_t f = Lv.fl[id];
-      T ev = e[id];
+      uint8_t f = fv[sl];
+      T ev = evv[sl];
       eo[m] = ev;
       if (f & FL_BLANK) {
         d = (f & FL_GHOST) ? ev : T(0);
@@ -1167,7 +1193,7 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
         T am = (f & FL_M) ? T(0) : cq_m[m], ap = (f & FL_P) ? T(0) : cq_p[m];
         // (acx + acy + acz) in the reference's order
         T diag = DIR == 1 ? (-(amx + apx) + -(am + ap)) + -(amo + apo) : (-(amx + apx) + -(amo + apo)) + -(am + ap);
-        T r = rhs[id] - e[id - 1] * amx - e[id + 1] * apx - e[id - ostride] * amo - e[id + ostride] * apo;
+        T r = rvv[sl] - elv[sl] * amx - erv[sl] * apx - ezm[sl] * amo - ezp[sl] * apo;
         if (tabs(diag) < T(1.0e-12)) {
           d = ev;
         } else {
