@@ -395,8 +395,13 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
     LevelArgs<T> a = largs<T>(L);
     const CoarseSolve &cs = c->cs;
     int n = cs.n;
-    kbegin(c, PP_K_COARSE, l, 8.0 * (double)n * n + 2.0 * sizeof(T) * n);
-    k_coarse_mv<T><<<(n + 3) / 4, 256, sizeof(double) * n, st>>>(n, cs.cell, cs.G, lv_rhs<T>(L), lv_e<T>(L));
+    kbegin(c, PP_K_COARSE, l, (double)sizeof(T) * n * n + 2.0 * sizeof(T) * n);
+    if (sizeof(T) == 4)
+      k_coarse_mv<T, float><<<(n + 3) / 4, 256, sizeof(double) * n, st>>>(n, cs.cell, cs.G32, lv_rhs<T>(L),
+                                                                         lv_e<T>(L));
+    else
+      k_coarse_mv<T, double><<<(n + 3) / 4, 256, sizeof(double) * n, st>>>(n, cs.cell, cs.G, lv_rhs<T>(L),
+                                                                          lv_e<T>(L));
     kend(c);
     return;
   }

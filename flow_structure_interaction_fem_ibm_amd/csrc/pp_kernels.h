@@ -227,6 +227,26 @@ __device__ inline double block_sum(double v, double *s) {
   return r;
 }
 
+// Block sums of NV values per thread with one barrier: wavefront shuffles,
+// the wavefront sums meet in LDS (s: NT/64 * NV doubles) and threads
+// t < NV add them in a fixed order into part[blk * NV + t].
+template <int NT, int NV>
+__device__ inline void block_sums_n(double (&v)[NV], double *s, double *part, long blk) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x, w = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int t = 0; t < NV; t++) {
+    double x = v[t];
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_down(x, d, 64);
+    if (lane == 0) s[w * NV + t] = x;
+  }
+  __syncthreads();
+  if (tid < NV) {
+    double r = 0.0;
+    for (int ww = 0; ww < NT / 64; ww++) r += s[ww * NV + tid];
+    part[blk * NV + tid] = r;
+  }
+}
+
 // ---------------------------------------------------------------------
 // fills and copies
 // ---------------------------------------------------------------------
@@ -372,7 +392,7 @@ template <int NT>
 __global__ void __launch_bounds__(NT) k_fluid_sums(RefArgs R, LevelArgs<double> L, const double *__restrict__ w,
                                                    const double *__restrict__ b, const double *__restrict__ p,
                                                    double *__restrict__ part) {
-  __shared__ double s[NT / 64];
+  __shared__ double s[NT / 64 * 5];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
   double q[5] = {0, 0, 0, 0, 0};
@@ -391,10 +411,7 @@ __global__ void __launch_bounds__(NT) k_fluid_sums(RefArgs R, LevelArgs<double> 
     }
   }
   long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-  for (int t = 0; t < 5; t++) {
-    double v = block_sum<NT>(q[t], s);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part[blk * 5 + t] = v;
-  }
+  block_sums_n<NT, 5>(q, s, part, blk);
 }
 
 // x[fluid] += alpha (alpha read from device memory: scale * src[0] (+ scale2*src[1]))
@@ -1218,13 +1235,14 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
 // solution operator G of the coarsest level (bordered LU on the GPU at
 // set-up: the coarse equations with the fluid-volume mean constraint per
 // connected component), stored by output cell: GT[q*n + p] = response of
-// cell q to a unit rhs at cell p; cell[q] = offset of coarse cell q (i
-// fastest) in the level layout.  One wavefront per output cell streams
+// cell q to a unit rhs at cell p (FP32 copy for the FP32 V-cycles, FP64
+// accumulation either way); cell[q] = offset of coarse cell q (i fastest)
+// in the level layout.  One wavefront per output cell streams
 // its row of GT (coalesced) against the rhs staged in LDS, then reduces
 // across the wavefront in a fixed order.
 // ---------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(256) k_coarse_mv(int n, const int *__restrict__ cell, const double *__restrict__ GT,
+template <typename T, typename TG>
+__global__ void __launch_bounds__(256) k_coarse_mv(int n, const int *__restrict__ cell, const TG *__restrict__ GT,
                                                    const T *__restrict__ rhs, T *__restrict__ e) {
   extern __shared__ unsigned char smem_raw[];
   double *r = reinterpret_cast<double *>(smem_raw);
@@ -1232,9 +1250,9 @@ __global__ void __launch_bounds__(256) k_coarse_mv(int n, const int *__restrict_
   __syncthreads();
   const int q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (q >= n) return;  // uniform per wavefront
-  const double *row = GT + (long)q * n;
+  const TG *row = GT + (long)q * n;
   double s = 0.0;
-  for (int p = lane; p < n; p += 64) s += row[p] * r[p];
+  for (int p = lane; p < n; p += 64) s += (double)row[p] * r[p];
   for (int d = 32; d >= 1; d >>= 1) s += __shfl_down(s, d, 64);
   if (lane == 0) e[cell[q]] = (T)s;
 }
