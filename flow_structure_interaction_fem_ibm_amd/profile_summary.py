@@ -4,6 +4,8 @@ multigrid level).
 
   python -m flow_structure_interaction_fem_ibm_amd.profile_summary trace <kernel_trace.csv> [--last-solve]
   python -m flow_structure_interaction_fem_ibm_amd.profile_summary pmc <counter_collection.csv> [...more csv]
+  python -m flow_structure_interaction_fem_ibm_amd.profile_summary traffic <fetch.csv> <write.csv> \
+      --copy-mib 512 --out profiles/traffic.json
 """
 from __future__ import annotations
 
@@ -61,13 +63,63 @@ def pmc(paths):
     return "\n".join(out), res
 
 
+# finest-level (256x192x192) kernels of the FP32 V-cycle: class -> (name prefix, grid threads)
+FINEST = {"yline": ("pp::k_yzline<float, 8, 1", 589824), "zline": ("pp::k_yzline<float, 8, 2", 589824),
+          "xline": ("pp::k_xline<float, 32, 8>", 589824), "residual": ("pp::k_residual4<float>", 2359296)}
+ALGORITHMIC = {"yline": 6.5, "zline": 6.5, "xline": 6.5, "residual": 13.0}  # bytes per cell per launch
+
+
+def traffic(fetch_csv, write_csv, copy_mib, cells=256 * 192 * 192):
+    """HBM bytes per launch of the finest-level kernels from one FETCH_SIZE and
+    one WRITE_SIZE run, each scaled by what the counter reads for the
+    device-to-device copy of copy_mib MiB in the same run."""
+    def per_dispatch(path, counter):
+        out = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                out[(_short(r["Kernel_Name"]), int(r.get("Grid_Size", 0) or 0))].append(float(r["Counter_Value"]))
+        return out
+    fe, wr = per_dispatch(fetch_csv, "FETCH_SIZE"), per_dispatch(write_csv, "WRITE_SIZE")
+    copy_kib = copy_mib * 1024.0
+
+    def copy_value(d):
+        vals = [v for (k, g), vs in d.items() if "copyBuffer" in k for v in vs]
+        big = [v for v in vals if v > 0.25 * copy_kib]
+        return sum(big) / len(big)
+    ff = copy_kib / copy_value(fe)
+    wf = copy_kib / copy_value(wr)
+    res = {"workload": "256x192x192", "fetch_factor": ff, "write_factor": wf, "copy_mib": copy_mib, "kernels": {}}
+    for cls, (prefix, grid) in FINEST.items():
+        fk = [v for (k, g), vs in fe.items() if k.startswith(prefix) and g == grid for v in vs]
+        wk = [v for (k, g), vs in wr.items() if k.startswith(prefix) and g == grid for v in vs]
+        if not fk or not wk:
+            continue
+        fkib, wkib = sum(fk) / len(fk), sum(wk) / len(wk)
+        name = next(k for (k, g) in fe if k.startswith(prefix) and g == grid)
+        res["kernels"][cls] = {"kernel": name, "grid_threads": grid, "dispatches": len(fk),
+                               "fetch_kib": fkib, "write_kib": wkib,
+                               "hbm_bytes_per_launch": (fkib * ff + wkib * wf) * 1024.0,
+                               "hbm_bytes_unscaled": (fkib + wkib) * 1024.0,
+                               "algorithmic_bytes_per_launch": ALGORITHMIC[cls] * cells}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("kind", choices=("trace", "pmc"))
+    ap.add_argument("kind", choices=("trace", "pmc", "traffic"))
     ap.add_argument("paths", nargs="+")
     ap.add_argument("--last-solve", action="store_true")
     ap.add_argument("--json", default=None, help="pmc: also write the per-dispatch figures as JSON")
+    ap.add_argument("--copy-mib", type=int, default=512, help="traffic: size of the calibration copy")
+    ap.add_argument("--out", default=None, help="traffic: output JSON path")
     args = ap.parse_args()
+    if args.kind == "traffic":
+        res = traffic(args.paths[0], args.paths[1], args.copy_mib)
+        text = json.dumps(res, indent=1)
+        if args.out:
+            open(args.out, "w").write(text + "\n")
+        print(text)
+        return
     if args.kind == "trace":
         print(trace(args.paths[0], args.last_solve))
     else:
