@@ -410,21 +410,25 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   smooth_steps<T>(c, l, o, npre);
   LevelArgs<T> a = largs<T>(L);
   CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
-  kbegin(c, PP_K_RESIDUAL, l, cells * (3.0 * sizeof(T) + 1.0));
-  launch_residual<T>(c, L);
-  kend(c);
   Level &C = c->lv[l + 1];
   LevelArgs<T> ac = largs<T>(C);
   CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
-  kbegin(c, PP_K_RESTRICT, l, cells * (sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz);
   const bool by2 = C.cf[0] == 2 && C.cf[1] == 2 && C.cf[2] == 2 && L.cx == 2 * C.cx && L.cy == 2 * C.cy &&
                    L.cz == 2 * C.cz;
-  if (by2)
-    k_restrict2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, lv_res<T>(L), lv_rhs<T>(C), lv_e<T>(C));
-  else
+  if (by2) {
+    // residual formed inside the restriction
+    kbegin(c, PP_K_RESTRICT, l, cells * (2.0 * sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz);
+    k_resrestrict2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, lv_e<T>(L), lv_rhs<T>(L), lv_rhs<T>(C), lv_e<T>(C));
+    kend(c);
+  } else {
+    kbegin(c, PP_K_RESIDUAL, l, cells * (3.0 * sizeof(T) + 1.0));
+    launch_residual<T>(c, L);
+    kend(c);
+    kbegin(c, PP_K_RESTRICT, l, cells * (sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz);
     k_restrict<T><<<cc.grid, cc.block, 0, st>>>(a, ac, C.cf[0], C.cf[1], C.cf[2], lv_res<T>(L), lv_rhs<T>(C),
                                                  lv_e<T>(C));
-  kend(c);
+    kend(c);
+  }
   vcycle<T>(c, l + 1, o);
   kbegin(c, PP_K_PROLONG, l, cells * (2.0 * sizeof(T) + 1.0) + (sizeof(T) + 1.0) * C.cx * C.cy * C.cz);
   if (by2)

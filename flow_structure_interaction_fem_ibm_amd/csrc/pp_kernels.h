@@ -772,6 +772,62 @@ __global__ void __launch_bounds__(256) k_restrict2(LevelArgs<T> F, LevelArgs<T> 
   e_c[cid] = T(0);
 }
 
+// The residual and the by-2 restriction in one pass: each coarse cell forms
+// the residuals of its 8 children (the k_residual row, same operation
+// order) and their fluid-volume-weighted mean directly, so the fine residual
+// is never stored.  Also zeroes the coarse correction.
+template <typename T>
+__global__ void __launch_bounds__(256) k_resrestrict2(LevelArgs<T> F, LevelArgs<T> C, const T *__restrict__ e,
+                                                      const T *__restrict__ rhs, T *__restrict__ rhs_c,
+                                                      T *__restrict__ e_c) {
+  using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
+  if (I >= C.cx || J >= C.cy) return;
+  const int i0 = 2 * I;
+  const V2 wx = *reinterpret_cast<const V2 *>(F.wx + i0);
+  const V2 cmx = *reinterpret_cast<const V2 *>(F.cmx + i0);
+  const V2 cpx = *reinterpret_cast<const V2 *>(F.cpx + i0);
+  T num = T(0), den = T(0);
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const int k = 2 * K + c;
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int j = 2 * J + b;
+      const T wyz = F.wy[j] * F.wz[k];
+      const long id = lidx(F.sx, F.sxy, i0, j, k);
+      const V2 ev = *reinterpret_cast<const V2 *>(e + id);
+      const T el = e[id - 1], er = e[id + 2];
+      const V2 ejm = *reinterpret_cast<const V2 *>(e + id - F.sx);
+      const V2 ejp = *reinterpret_cast<const V2 *>(e + id + F.sx);
+      const V2 ekm = *reinterpret_cast<const V2 *>(e + id - F.sxy);
+      const V2 ekp = *reinterpret_cast<const V2 *>(e + id + F.sxy);
+      const V2 rv = *reinterpret_cast<const V2 *>(rhs + id);
+      const uint16_t f2 = *reinterpret_cast<const uint16_t *>(F.fl + id);
+      const T my = F.cmy[j], py = F.cpy[j], mz = F.cmz[k], pz = F.cpz[k];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const uint8_t f = (uint8_t)(f2 >> (8 * t));
+        if (f & FL_BLANK) continue;
+        const T amx = (f & FL_IM) ? T(0) : (t ? cmx.y : cmx.x), apx = (f & FL_IP) ? T(0) : (t ? cpx.y : cpx.x);
+        const T amy = (f & FL_JM) ? T(0) : my, apy = (f & FL_JP) ? T(0) : py;
+        const T amz = (f & FL_KM) ? T(0) : mz, apz = (f & FL_KP) ? T(0) : pz;
+        const T diag = (-(amx + apx) + -(amy + apy)) + -(amz + apz);
+        const T ec = t ? ev.y : ev.x, left = t ? ev.x : el, right = t ? er : ev.y;
+        const T r = (t ? rv.y : rv.x) - ec * diag - left * amx - right * apx - (t ? ejm.y : ejm.x) * amy -
+                    (t ? ejp.y : ejp.x) * apy - (t ? ekm.y : ekm.x) * amz - (t ? ekp.y : ekp.x) * apz;
+        const T v = (t ? wx.y : wx.x) * wyz;
+        num += v * r;
+        den += v;
+      }
+    }
+  }
+  const long cid = lidx(C.sx, C.sxy, I, J, K);
+  rhs_c[cid] = den > T(0) ? num / den : T(0);
+  e_c[cid] = T(0);
+}
+
 // ---------------------------------------------------------------------
 // prolongation + correction: e_f += trilinear interpolation of the coarse
 // correction over the fluid coarse cells (weights renormalised over fluid
