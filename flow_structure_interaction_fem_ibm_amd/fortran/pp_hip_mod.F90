@@ -19,6 +19,8 @@ module pp_hip_mod
   integer(c_int32_t), parameter, public :: PP_PREC_FP64 = 0, PP_PREC_MIXED = 1
   integer(c_int32_t), parameter, public :: PP_SMOOTH_LINE = 0, PP_SMOOTH_POINT_JACOBI = 1
   integer, parameter, public :: PP_K_CLASSES = 13
+  integer(c_int32_t), parameter, public :: PP_BODY_ELLIPSOID = 3
+  integer(c_int32_t), parameter, public :: PP_GEOM_ANALYTIC = 0, PP_GEOM_TRIANGULATED = 1
 
   type, bind(c), public :: pp_grid_desc
      integer(c_int32_t) :: nxc, nyc, nzc
@@ -48,6 +50,14 @@ module pp_hip_mod
      integer(c_int32_t) :: mean_remove, project_rhs, use_graph, profile, krylov, nu_coarse, smoother
   end type pp_solve_opts
 
+  ! one canonical body (canonical_body_type, xcent.., radiusx.., alpha, nPtsBodyMarker)
+  type, bind(c), public :: pp_body_desc
+     integer(c_int32_t) :: shape       ! PP_BODY_ELLIPSOID
+     real(c_double) :: center(3), semi(3)
+     real(c_double) :: alpha_deg
+     integer(c_int32_t) :: n_theta
+  end type pp_body_desc
+
   type, bind(c), public :: pp_solve_stats
      integer(c_int32_t) :: status, outer_iterations, vcycles
      real(c_double) :: rel_l2, abs_linf
@@ -62,7 +72,8 @@ module pp_hip_mod
 
   public :: pp_default_opts, pp_last_error, pp_create, pp_destroy, pp_set_geometry, pp_solve, &
        pp_solve_device, pp_adjoint, pp_calc_residual, pp_ghost_rows, pp_mean_remove, pp_divergence, &
-       pp_smooth, pp_null_vector, pp_level_info, pp_error_text
+       pp_smooth, pp_null_vector, pp_level_info, pp_error_text, pp_geom_build, pp_geom_counts, pp_geom_get, &
+       pp_geom_ghost_elements, pp_geom_destroy
 
   interface
      subroutine pp_default_opts(o) bind(c, name="pp_default_opts")
@@ -169,6 +180,46 @@ module pp_hip_mod
        integer(c_int32_t), intent(out) :: dims(*)
        integer(c_int32_t) :: rc
      end function pp_level_info
+
+     ! set_boundary replacement (host only): bodies -> iblank, fresh cells,
+     ! ghost lists and GCM coefficients, read back with pp_geom_get
+     function pp_geom_build(geo, g, n_body, bodies, mode, probe_norm, iblank_prev) &
+          bind(c, name="pp_geom_build") result(rc)
+       import :: c_ptr, c_int32_t, c_double, pp_grid_desc, pp_body_desc
+       type(c_ptr), intent(out) :: geo
+       type(pp_grid_desc), intent(in) :: g
+       integer(c_int32_t), value :: n_body, mode
+       type(pp_body_desc), intent(in) :: bodies(*)
+       real(c_double), value :: probe_norm
+       type(c_ptr), value :: iblank_prev
+       integer(c_int32_t) :: rc
+     end function pp_geom_build
+
+     function pp_geom_counts(geo, n_ghost, n_blank, n_fresh) bind(c, name="pp_geom_counts") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: geo
+       integer(c_int32_t), intent(out) :: n_ghost, n_blank, n_fresh
+       integer(c_int32_t) :: rc
+     end function pp_geom_counts
+
+     function pp_geom_get(geo, iblank, fresh, ig, jg, kg, ic, jc, kc, coeff, probe, bi, ip, nrm) &
+          bind(c, name="pp_geom_get") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: geo, iblank, fresh, ig, jg, kg, ic, jc, kc, coeff, probe, bi, ip, nrm
+       integer(c_int32_t) :: rc
+     end function pp_geom_get
+
+     function pp_geom_ghost_elements(geo, body, elem) bind(c, name="pp_geom_ghost_elements") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: geo, body, elem
+       integer(c_int32_t) :: rc
+     end function pp_geom_ghost_elements
+
+     function pp_geom_destroy(geo) bind(c, name="pp_geom_destroy") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: geo
+       integer(c_int32_t) :: rc
+     end function pp_geom_destroy
   end interface
 
 contains

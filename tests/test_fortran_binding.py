@@ -6,6 +6,7 @@ import os
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 from flow_structure_interaction_fem_ibm_amd import pp
@@ -40,3 +41,19 @@ def test_fortran_module_matches_the_c_abi(tmp_path):
     assert "at least 4 cells" in " ".join(out["error"])
     sizes = [int(v) for v in out["sizes"]]
     assert sizes == [C.sizeof(pp.SolveOpts), C.sizeof(pp.GridDesc), C.sizeof(pp.SolveStats), C.sizeof(pp.Geometry)]
+    assert int(out["bodysize"][0]) == C.sizeof(pp.BodyDesc)
+    # the same geometry through the ctypes binding
+    xn = 1.2 * np.arange(25) / 24.0
+    body = pp.Ellipsoid((0.61, 0.58, 0.6), (0.3, 0.2, 0.22), 20.0, 40)
+    ib, _, gs, ex = pp.build_geometry(xn, xn, xn, [body], mode=pp.PP_GEOM_TRIANGULATED, probe_norm=2.0)
+    n = len(gs.ig)
+    assert n > 0 and [int(v) for v in out["geom"]] == [pp.PP_OK]
+    assert [int(v) for v in out["counts"]] == [pp.PP_OK, n, int((ib[1:-1, 1:-1, 1:-1] == 1).sum()), 0]
+    g = out["get"]
+    assert int(g[0]) == pp.PP_OK and int(g[1]) == int(ib.astype(np.int64).sum())
+    # Fortran indices are the same 1-based cell indices the C side writes
+    assert [int(v) for v in g[2:8]] == [int(a.astype(np.int64).sum()) for a in (gs.ig, gs.jg, gs.kg, gs.ic, gs.jc, gs.kc)]
+    assert float(g[8]) == pytest.approx(float(gs.coeff.sum()), rel=1e-14, abs=1e-12)
+    assert float(g[9]) == pytest.approx(float(gs.probe.sum()), rel=1e-14)
+    assert [int(v) for v in out["elems"]] == [pp.PP_OK, int(ex["body"].sum()), int(ex["elem"].sum())]
+    assert int(out["destroy"][0]) == pp.PP_OK
