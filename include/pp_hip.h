@@ -80,7 +80,8 @@ typedef struct pp_solve_opts {
   int32_t max_outer;         /* iterMax_Poisson: outer iterations */
   int32_t cycles_per_outer;  /* V-cycles per outer iteration */
   int32_t nu_pre, nu_post;   /* smoothing steps before/after coarse correction */
-  double omega;              /* line-SOR weight */
+  double omega;              /* relaxation weight (MAIN.F90:478); default 1.3 for the
+                                zebra line smoother, use < 1 with point Jacobi */
   int32_t mean_remove;       /* subtract the fluid volume mean on return (0/1) */
   int32_t project_rhs;       /* compatibility projection of the RHS (0/1) */
   int32_t use_graph;         /* replay the outer iteration as a hipGraph (0/1) */
@@ -154,7 +155,10 @@ int32_t pp_destroy(pp_ctx *ctx);
  * BOUNDARY_MOVE.F90:210).  Derives ghostCellMark and the outer-face flags
  * (BOUNDARY_SET.F90:54-92, GCM_SET_BOUNDARY.F90:49-118) from iblank and
  * checks them against the ghost list; builds the level hierarchy, ghost
- * tables and the left null vector used by the compatibility projection. */
+ * tables and the left null vector used by the compatibility projection.
+ * Ghost stencil corners on solid non-ghost cells (value 0) leave the
+ * operator without an exact null vector; its near-null vector is used
+ * then (pp_setup_info reports the residual reached). */
 int32_t pp_set_geometry(pp_ctx *ctx, const pp_geometry *geo);
 
 /* solve_poisson (CORE_POISSON.F90:493-676): p_inout = pPrime (initial guess
