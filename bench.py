@@ -35,14 +35,42 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "MG Poisson solve/s @1e-8 resid (256x192x192): FP32+refine vs FP64, HBM GB/s"
 
 
-def workload(nx=256, ny=192, nz=192, seed=2):
+def workload(nx=256, ny=192, nz=192, seed=2, extras=False):
     xn = np.linspace(0.0, 4.0, nx + 1)
     yn = pp.tanh_nodes(ny, 3.0, 2.0)
     zn = pp.tanh_nodes(nz, 3.0, 2.0)
     body = pp.Ellipsoid((1.5, 1.5, 1.5), (0.5, 0.25, 0.25), 20.0, 96)
     ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, [body], mode=pp.PP_GEOM_ANALYTIC)
     rhs = divergence_rhs(xn, yn, zn, ib, seed)
+    if extras:
+        return xn, yn, zn, ib, gs, rhs, ex
     return xn, yn, zn, ib, gs, rhs
+
+
+def time_frac_step(solver, ex, dev, reps=10):
+    """correct_vel and update_pressure on device fields of the same grid
+    (the steps either side of the solve; not part of the headline value):
+    wall time per call around synchronised calls, and algorithmic GB/s
+    (correct_vel 129 B/cell: p, u, v, w, three faces, three bc arrays and
+    the flags read, u, v, w and the faces written; update_pressure 24 B/cell)."""
+    import torch
+    solver.set_vel_ghosts(ex["coeffD"], 2.0, 1.0)
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    f = [torch.randn(solver.shape, dtype=torch.float64, device=dev, generator=g) for _ in range(10)]
+    cells = float(np.prod([n - 3 for n in solver.shape]))
+    out = {}
+    for name, call, bpc in (("correct_vel", lambda: solver.correct_vel_device(0.01, *f), 129.0),
+                            ("update_pressure", lambda: solver.update_pressure_device(1, f[1], f[0]), 24.0)):
+        call()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            call()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) / reps * 1e3
+        out[name] = {"ms": ms, "GBs": bpc * cells / (ms * 1e-3) / 1e9, "bytes_per_cell": bpc}
+    return out
 
 
 def divergence_rhs(xn, yn, zn, ib, seed):
@@ -120,7 +148,7 @@ def main():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(2)
     t_setup = time.perf_counter()
-    xn, yn, zn, ib, gs, rhs = workload()
+    xn, yn, zn, ib, gs, rhs, gex = workload(extras=True)
     solver = pp.PressureSolver(xn, yn, zn)
     solver.set_geometry(ib, gs)
     t_setup = time.perf_counter() - t_setup
@@ -238,6 +266,7 @@ def main():
                           for k, v in km.items()},
         "kernels_fp64": {k: {"ms": v["ms"], "launches": v["launches"]} for k, v in prof_fp64[0]["kernels"].items()},
         "setup_s": t_setup,
+        "frac_step": time_frac_step(solver, gex, dev),
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <unordered_map>
 
 #include <rocblas/rocblas.h>
 
@@ -509,8 +510,10 @@ int reduce_summax(pp_ctx *c, int nblk) {
   return PP_OK;
 }
 
-// exact ghost rows + outer ghost layer on an FP64 reference-layout field
-void ghost_exact_ref(pp_ctx *c, double *p, bool with_bnd) {
+// exact ghost rows + outer ghost layer (unless outer is false) on an FP64
+// reference-layout field; the composite rows read a stencil corner in the
+// outer layer from the adjacent interior cell (its Neumann copy)
+void ghost_exact_ref(pp_ctx *c, double *p, bool with_bnd, bool outer = true) {
   GhostTables &g = c->gh;
   hipStream_t st = c->stream;
   if (g.n > 0) {
@@ -521,6 +524,7 @@ void ghost_exact_ref(pp_ctx *c, double *p, bool with_bnd) {
                                                              with_bnd ? g.bnd_eff : nullptr, p);
     kend(c);
   }
+  if (!outer) return;
   k_outer_ghost_x<<<dim3((c->nyc + 2 + 255) / 256, c->nzc + 2), 256, 0, st>>>(c->ref, p);
   k_outer_ghost_y<<<dim3((c->nxc + 2 + 255) / 256, c->nzc + 2), 256, 0, st>>>(c->ref, p);
   k_outer_ghost_z<<<dim3((c->nxc + 2 + 255) / 256, c->nyc + 2), 256, 0, st>>>(c->ref, p);
@@ -889,11 +893,9 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
     if (have_bnd) {
       // ghost-row part of w^T b
       std::vector<double> hw(g.n), hb(g.n);
-      for (int q = 0; q < g.n; q++) {
-        double wv;
-        HIPCHK(hipMemcpy(&wv, c->d_w + g.h_g_ref[q], sizeof(double), hipMemcpyDeviceToHost));
-        hw[q] = wv;
-      }
+      k_gather_idx<double><<<dim3((g.n + 255) / 256), 256, 0, s>>>(g.n, g.g_ref, c->d_w, c->d_tmp);
+      HIPCHK(hipStreamSynchronize(s));
+      HIPCHK(hipMemcpy(hw.data(), c->d_tmp, sizeof(double) * g.n, hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(hb.data(), g.bnd, sizeof(double) * g.n, hipMemcpyDeviceToHost));
       for (int q = 0; q < g.n; q++) num += hw[q] * hb[q];
     }
@@ -1358,6 +1360,14 @@ int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
     HIPCHK(hipMalloc(&c->d_rdxinv[d], nd));
     HIPCHK(hipMemcpy(c->d_rdx[d], c->rdx[d].data(), nd, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_rdxinv[d], c->rdxinv[d].data(), nd, hipMemcpyHostToDevice));
+    // fx(i) = (x(i) - xc(i-1)) / (xc(i) - xc(i-1)), i = 1..n (CORE_GRID.F90:172-182)
+    std::vector<double> fx(c->rdx[d].size(), 0.0);
+    const int nn = ns[d] + 1;
+    for (int i = 1; i <= nn; i++) fx[i] = (c->hx[d][i] - c->rxc[d][i - 1]) / (c->rxc[d][i] - c->rxc[d][i - 1]);
+    HIPCHK(hipMalloc(&c->d_rfx[d], nd));
+    HIPCHK(hipMalloc(&c->d_rdxcinv[d], nd));
+    HIPCHK(hipMemcpy(c->d_rfx[d], fx.data(), nd, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_rdxcinv[d], c->rdxcinv[d].data(), nd, hipMemcpyHostToDevice));
   }
   size_t nb = sizeof(double) * c->nref;
   HIPCHK(hipMalloc(&c->d_p, nb));
@@ -1391,7 +1401,9 @@ int32_t pp_destroy(pp_ctx *c) {
   hipFree(c->d_part); hipFree(c->d_part2); hipFree(c->d_res);
   for (double *q : c->d_kry) hipFree(q);
   for (float *q : c->d_kryl) hipFree(q);
-  for (int d = 0; d < 3; d++) { hipFree(c->d_rdx[d]); hipFree(c->d_rdxinv[d]); }
+  for (int d = 0; d < 3; d++) {
+    hipFree(c->d_rdx[d]); hipFree(c->d_rdxinv[d]); hipFree(c->d_rfx[d]); hipFree(c->d_rdxcinv[d]);
+  }
   if (c->h_res) hipHostFree(c->h_res);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   if (c->blas) rocblas_destroy_handle((rocblas_handle)c->blas);
@@ -1583,6 +1595,197 @@ int32_t pp_ghost_rows(pp_ctx *c, double *p, const double *bc) {
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(p, c->d_p, nb, hipMemcpyDeviceToHost));
   return PP_OK;
+}
+
+int32_t pp_set_vel_ghosts(pp_ctx *c, const double *coeffD, double bi_weight, double ip_weight) {
+  if (!c) { set_error("pp_set_vel_ghosts: null context"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_set_vel_ghosts: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  GhostTables &g = c->gh;
+  if (g.n > 0 && !coeffD) { set_error("pp_set_vel_ghosts: null coefficients"); return PP_BAD_ARG; }
+  if (!std::isfinite(bi_weight) || !std::isfinite(ip_weight)) {
+    set_error("pp_set_vel_ghosts: weights must be finite");
+    return PP_BAD_ARG;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  hipFree(g.v_cD); hipFree(g.v_buf[0]); hipFree(g.v_buf[1]); hipFree(g.v_kind); hipFree(g.v_off);
+  g.v_cD = g.v_buf[0] = g.v_buf[1] = nullptr; g.v_kind = nullptr; g.v_off = nullptr;
+  g.have_vel = false;
+  const int n = g.n;
+  std::vector<int> kind(8L * n);
+  std::vector<long> off(8L * n);
+  std::vector<double> cd(coeffD ? coeffD : nullptr, coeffD ? coeffD + 8L * n : nullptr);
+  std::unordered_map<long, int> gidx;
+  gidx.reserve(2L * n + 1);
+  for (int q = 0; q < n; q++) gidx[g.h_g_ref[q]] = q;
+  for (int q = 0; q < n; q++)
+    for (int r = 0; r < 8; r++) {
+      const long e = 8L * q + r;
+      if (!std::isfinite(cd[e])) { set_error("pp_set_vel_ghosts: ghost %d coefficient %d is not finite", q, r); return PP_BAD_ARG; }
+      const long o = g.h_s_raw[e];
+      off[e] = o;
+      if (o == g.h_g_ref[q]) kind[e] = -2;
+      else {
+        auto it = gidx.find(o);
+        kind[e] = it != gidx.end() ? it->second : -1;
+      }
+    }
+  // sweeps that bring the ghost-ghost coupling to round-off: the same
+  // Jacobi iteration on the host, field values and body-intercept values
+  // fixed pseudo-random numbers
+  int sweeps = 1;
+  if (n > 0) {
+    auto hv = [](long a) {
+      uint64_t h = (uint64_t)a * 0x9E3779B97F4A7C15ull + 777u;
+      h ^= h >> 31; h *= 0xBF58476D1CE4E5B9ull; h ^= h >> 29;
+      return (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    };
+    std::vector<double> x(n, 0.0), y(n);
+    double dmax = 1.0;
+    sweeps = 0;
+    while (sweeps < 1000 && dmax > 1e-15) {
+      dmax = 0.0;
+      for (int q = 0; q < n; q++) {
+        const double b = hv(-1 - (long)q);
+        double sum = 0.0;
+        for (int r = 0; r < 8; r++) {
+          const long e = 8L * q + r;
+          const double xv = kind[e] >= 0 ? x[kind[e]] : (kind[e] == -1 ? hv(off[e]) : b);
+          sum += cd[e] * xv;
+        }
+        y[q] = b * bi_weight - sum * ip_weight;
+      }
+      for (int q = 0; q < n; q++) {
+        dmax = std::max(dmax, std::fabs(y[q] - x[q]) / std::max(1.0, std::fabs(y[q])));
+        x[q] = y[q];
+      }
+      sweeps++;
+    }
+    if (dmax > 1e-15) {
+      set_error("pp_set_vel_ghosts: the velocity ghost rows do not converge (change %g after %d sweeps)", dmax, sweeps);
+      return PP_BAD_GEOMETRY;
+    }
+    HIPCHK(hipMalloc(&g.v_cD, sizeof(double) * 8 * n));
+    HIPCHK(hipMalloc(&g.v_kind, sizeof(int) * 8 * n));
+    HIPCHK(hipMalloc(&g.v_off, sizeof(long) * 8 * n));
+    HIPCHK(hipMalloc(&g.v_buf[0], sizeof(double) * 3 * n));
+    HIPCHK(hipMalloc(&g.v_buf[1], sizeof(double) * 3 * n));
+    HIPCHK(hipMemcpy(g.v_cD, cd.data(), sizeof(double) * 8 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g.v_kind, kind.data(), sizeof(int) * 8 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g.v_off, off.data(), sizeof(long) * 8 * n, hipMemcpyHostToDevice));
+  }
+  g.v_sweeps = sweeps;
+  g.v_bw = bi_weight;
+  g.v_iw = ip_weight;
+  g.have_vel = true;
+  return PP_OK;
+}
+
+int32_t pp_correct_vel_device(pp_ctx *c, double dt, const double *d_p, double *d_u, double *d_v, double *d_w,
+                              double *d_fu, double *d_fv, double *d_fw, const double *d_bcxu, const double *d_bcyv,
+                              const double *d_bczw, const double *d_vbi) {
+  if (!c || !d_p || !d_u || !d_v || !d_w || !d_fu || !d_fv || !d_fw || !d_bcxu || !d_bcyv || !d_bczw) {
+    set_error("pp_correct_vel: null argument");
+    return PP_BAD_ARG;
+  }
+  if (!c->have_geom) { set_error("pp_correct_vel: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  GhostTables &g = c->gh;
+  if (g.n > 0 && !g.have_vel) {
+    set_error("pp_correct_vel: no velocity ghost coefficients (call pp_set_vel_ghosts after pp_set_geometry)");
+    return PP_NO_GEOMETRY;
+  }
+  if (!std::isfinite(dt)) { set_error("pp_correct_vel: dt must be finite"); return PP_BAD_ARG; }
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  FracMetrics M;
+  for (int d = 0; d < 3; d++) { M.fx[d] = c->d_rfx[d]; M.dinv[d] = c->d_rdxinv[d]; M.dcinv[d] = c->d_rdxcinv[d]; }
+  CellLaunch cl = ref_launch(c);
+  k_correct_vel<256><<<cl.grid, cl.block, 0, s>>>(c->ref, largs<double>(c->lv[0]), M, dt, d_p, d_u, d_v, d_w, d_fu,
+                                                   d_fv, d_fw, d_bcxu, d_bcyv, d_bczw);
+  if (g.n > 0) {
+    dim3 gg((g.n + 255) / 256), blk(256);
+    k_vel_ghost_gather<double><<<gg, blk, 0, s>>>(g.n, g.g_ref, d_u, d_v, d_w, g.v_buf[0]);
+    int cur = 0;
+    for (int q = 0; q < g.v_sweeps; q++) {
+      k_vel_ghost_sweep<<<gg, blk, 0, s>>>(g.n, g.v_cD, g.v_kind, g.v_off, g.v_bw, g.v_iw, d_vbi, d_u, d_v, d_w,
+                                           g.v_buf[cur], g.v_buf[1 - cur]);
+      cur ^= 1;
+    }
+    k_vel_ghost_scatter<<<gg, blk, 0, s>>>(g.n, g.g_ref, g.v_buf[cur], d_u, d_v, d_w);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  return PP_OK;
+}
+
+int32_t pp_correct_vel(pp_ctx *c, double dt, const double *p, double *u, double *v, double *w, double *fu, double *fv,
+                       double *fw, const double *bcxu, const double *bcyv, const double *bczw, const double *vbi) {
+  if (!c || !p || !u || !v || !w || !fu || !fv || !fw || !bcxu || !bcyv || !bczw) {
+    set_error("pp_correct_vel: null argument");
+    return PP_BAD_ARG;
+  }
+  if (!c->have_geom) { set_error("pp_correct_vel: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nb = sizeof(double) * c->nref;
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, 10 * nb));
+  double *dv[10];
+  for (int q = 0; q < 10; q++) dv[q] = d + q * c->nref;
+  const double *hin[10] = {p, u, v, w, fu, fv, fw, bcxu, bcyv, bczw};
+  for (int q = 0; q < 10; q++) HIPCHK(hipMemcpy(dv[q], hin[q], nb, hipMemcpyHostToDevice));
+  double *dvbi = nullptr;
+  if (vbi && c->gh.n > 0) {
+    HIPCHK(hipMalloc(&dvbi, sizeof(double) * 3 * c->gh.n));
+    HIPCHK(hipMemcpy(dvbi, vbi, sizeof(double) * 3 * c->gh.n, hipMemcpyHostToDevice));
+  }
+  int rc = pp_correct_vel_device(c, dt, dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7], dv[8], dv[9], dvbi);
+  if (rc == PP_OK) {
+    double *hout[6] = {u, v, w, fu, fv, fw};
+    for (int q = 0; q < 6; q++) HIPCHK(hipMemcpy(hout[q], dv[1 + q], nb, hipMemcpyDeviceToHost));
+  }
+  hipFree(dvbi);
+  hipFree(d);
+  return rc;
+}
+
+int32_t pp_update_pressure_device(pp_ctx *c, int32_t van_kan, const double *d_pprime, double *d_p,
+                                  const double *d_bc) {
+  if (!c || !d_pprime || !d_p) { set_error("pp_update_pressure: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_update_pressure: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  GhostTables &g = c->gh;
+  const bool have = d_bc && g.n > 0;
+  if (have) {
+    std::vector<double> hs(g.n);
+    HIPCHK(hipMemcpy(hs.data(), d_bc, sizeof(double) * g.n, hipMemcpyDeviceToHost));
+    int rc = set_ghost_bnd(c, hs);
+    if (rc) return rc;
+  }
+  const long n = c->nref;
+  k_update_pressure<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(n, van_kan ? 1 : 0, d_pprime, d_p);
+  // GCM_p_set_bc_internal only: the outer layer keeps p + p' (both hold
+  // the Neumann copy after a solve, so their sum does too)
+  ghost_exact_ref(c, d_p, have, false);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PP_OK;
+}
+
+int32_t pp_update_pressure(pp_ctx *c, int32_t van_kan, const double *pprime, double *p, const double *bc) {
+  if (!c || !pprime || !p) { set_error("pp_update_pressure: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_update_pressure: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nb = sizeof(double) * c->nref;
+  HIPCHK(hipMemcpy(c->d_p, p, nb, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_bwork, pprime, nb, hipMemcpyHostToDevice));
+  double *dbc = nullptr;
+  if (bc && c->gh.n > 0) {
+    HIPCHK(hipMalloc(&dbc, sizeof(double) * c->gh.n));
+    HIPCHK(hipMemcpy(dbc, bc, sizeof(double) * c->gh.n, hipMemcpyHostToDevice));
+  }
+  int rc = pp_update_pressure_device(c, van_kan, c->d_bwork, c->d_p, dbc);
+  hipFree(dbc);
+  if (rc == PP_OK) HIPCHK(hipMemcpy(p, c->d_p, nb, hipMemcpyDeviceToHost));
+  return rc;
 }
 
 int32_t pp_mean_remove(pp_ctx *c, double *p, double *mean) {

@@ -283,12 +283,15 @@ int ghost_stencil(const Grid &G, int iG, int jG, int kG, const double *bi, const
       if (G.XC[d][i] <= ip[d] && G.XC[d][i + 1] > ip[d]) cell[d] = i;
   }
   if (cell[0] < 0 || cell[1] < 0 || cell[2] < 0) return 1;
-  double M[8][8];
+  double M[8][8], MD[8][8];
   for (int r = 0; r < 8; r++) {
     int i = cell[0] + incI[r], j = cell[1] + incJ[r], k = cell[2] + incK[r];
     double x1 = G.XC[0][i], x2 = G.XC[1][j], x3 = G.XC[2][k];
     if (i == iG && j == jG && k == kG) {
       x1 = bi[0]; x2 = bi[1]; x3 = bi[2];
+      // velocity (Dirichlet) row: the value at the body intercept
+      MD[r][0] = x1 * x2 * x3; MD[r][1] = x1 * x2; MD[r][2] = x1 * x3; MD[r][3] = x2 * x3;
+      MD[r][4] = x1; MD[r][5] = x2; MD[r][6] = x3; MD[r][7] = 1.0;
       double n1 = nrm[0], n2 = nrm[1], n3 = nrm[2];
       M[r][0] = n1 * x2 * x3 + n2 * x1 * x3 + n3 * x1 * x2;
       M[r][1] = n1 * x2 + n2 * x1;
@@ -298,10 +301,12 @@ int ghost_stencil(const Grid &G, int iG, int jG, int kG, const double *bi, const
     } else {
       M[r][0] = x1 * x2 * x3; M[r][1] = x1 * x2; M[r][2] = x1 * x3; M[r][3] = x2 * x3;
       M[r][4] = x1; M[r][5] = x2; M[r][6] = x3; M[r][7] = 1.0;
+      for (int q = 0; q < 8; q++) MD[r][q] = M[r][q];
     }
   }
   double beta[8] = {ip[0] * ip[1] * ip[2], ip[0] * ip[1], ip[0] * ip[2], ip[1] * ip[2], ip[0], ip[1], ip[2], 1.0};
   if (!solve8T(M, beta, o.coeff)) return 3;
+  if (!solve8T(MD, beta, o.coeffD)) return 3;
   o.ic = cell[0]; o.jc = cell[1]; o.kc = cell[2];
   o.probe = probe;
   for (int d = 0; d < 3; d++) { o.bi[d] = bi[d]; o.ip[d] = ip[d]; o.nrm[d] = nrm[d]; }
@@ -505,6 +510,12 @@ int32_t pp_geom_ghost_elements(const pp_geom *o, int32_t *body, int32_t *elem) {
     if (body) body[q] = o->gout[q].body + 1;
     if (elem) elem[q] = o->gout[q].elem;
   }
+  return PP_OK;
+}
+
+int32_t pp_geom_dirichlet(const pp_geom *o, double *coeffD) {
+  if (!o || !coeffD) { pp::set_error("pp_geom_dirichlet: null argument"); return PP_BAD_ARG; }
+  for (size_t q = 0; q < o->gout.size(); q++) memcpy(coeffD + 8 * q, o->gout[q].coeffD, 8 * sizeof(double));
   return PP_OK;
 }
 

@@ -35,7 +35,8 @@ struct GhostOut {
   int ic = -1, jc = -1, kc = -1;
   int elem = 0;   // closest surface element (1-based), triangulated mode
   int body = -1;  // body of the ghost (0-based)
-  double coeff[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double coeff[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // coeffGCMN (pressure, Neumann)
+  double coeffD[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // coeffGCMD (velocity, Dirichlet)
   double probe = 0, bi[3] = {0, 0, 0}, ip[3] = {0, 0, 0}, nrm[3] = {0, 0, 0};
 };
 

@@ -119,6 +119,16 @@ struct GhostTables {
   std::vector<int> h_t_ptr, h_t_ghost, h_g_tidx;
   std::vector<int> h_ig, h_jg, h_kg;            // 1-based reference indices
   int solid_in_stencil = 0;                     // stencil entries on solid non-ghost cells
+  // velocity ghost rows (GCM_vel_set_bc_internal), set by pp_set_vel_ghosts:
+  // Dirichlet coefficients, per stencil entry the ghost index it reads
+  // (>= 0), -1 for the field at v_off, -2 for the body-intercept value
+  std::vector<long> h_s_raw;                    // stencil offsets, outer layer not clamped
+  double *v_cD = nullptr, *v_buf[2] = {nullptr, nullptr};
+  int *v_kind = nullptr;
+  long *v_off = nullptr;
+  int v_sweeps = 0;                             // Jacobi sweeps that reach round-off
+  double v_bw = 0.0, v_iw = 0.0;
+  bool have_vel = false;
 };
 
 struct CoarseSolve {
@@ -145,6 +155,7 @@ struct pp_ctx {
   pp::CoarseSolve cs;
   void *blas = nullptr;                // rocblas_handle of the coarse factorisation
   double *d_rdx[3] = {}, *d_rdxinv[3] = {};  // make_grid dx, dxinv (0:n+1) per direction
+  double *d_rdxcinv[3] = {}, *d_rfx[3] = {};  // dxcinv and fx (0:n+1) per direction
   // FP64 work arrays in reference layout
   double *d_p = nullptr, *d_b = nullptr, *d_w = nullptr, *d_bwork = nullptr;
   double *d_tmp = nullptr;

@@ -362,6 +362,141 @@ __global__ void __launch_bounds__(NT) k_divergence(RefArgs R, const double *__re
   if (threadIdx.x == 0 && threadIdx.y == 0) part[blk] = t;
 }
 
+// ---------------------------------------------------------------------
+// Fractional step either side of the solve, FP64 reference layout.
+// Per-direction metrics of the reference grid (0:n+1): fx (face
+// interpolation weight, CORE_GRID.F90:172-182), dxinv, dxcinv.
+// ---------------------------------------------------------------------
+struct FracMetrics {
+  const double *fx[3], *dinv[3], *dcinv[3];
+};
+
+// correct_vel (CORE_FRAC_STEP_ROUTINES.F90:270-444) and
+// GCM_set_face_vel_body (GCM_GHOSTCELL_VEL_UPDATE_UNSTRUC.F90:387-423) for
+// one interior cell: its three velocities, and its west/south/back faces
+// (the last cell of a row also its east/north/front face), each face with
+// the reference's overwrite and mask order:
+//   f = f_old - dt ((p_c - p_w) dcinv)(1 - m_c);
+//   f = f (1 - p_w) + bc_w p_w   (not for the first face of a row);
+//   f = f (1 - m_c) + bc_c m_c;  0 if either cell is blank and not a ghost
+// with m = ium/jum/kum and p = iup/jup/kup (the level-0 outer-face flags).
+// One thread owns every face it writes, so the pass is race free.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_correct_vel(RefArgs R, LevelArgs<double> L0, FracMetrics M, double dt,
+                                                    const double *__restrict__ p, double *__restrict__ u,
+                                                    double *__restrict__ v, double *__restrict__ w,
+                                                    double *__restrict__ fu, double *__restrict__ fv,
+                                                    double *__restrict__ fw, const double *__restrict__ bcu,
+                                                    const double *__restrict__ bcv, const double *__restrict__ bcw) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= R.nxc || j >= R.nyc) return;
+  const long id = ridx(R, i, j, k);
+  const long st[3] = {1, R.ex, R.exy};
+  const int I[3] = {i + 1, j + 1, k + 1};
+  const int nc[3] = {R.nxc, R.nyc, R.nzc};
+  const long lst[3] = {1, (long)L0.sx, L0.sxy};
+  const long la = lidx(L0.sx, L0.sxy, i, j, k);
+  const uint8_t f = L0.fl[la];
+  const uint8_t FM[3] = {FL_IM, FL_JM, FL_KM}, FP[3] = {FL_IP, FL_JP, FL_KP};
+  const bool solid = (f & FL_BLANK) && !(f & FL_GHOST);
+  double *vel[3] = {u, v, w}, *face[3] = {fu, fv, fw};
+  const double *bc[3] = {bcu, bcv, bcw};
+  const double pc = p[id];
+  const double fl = (double)(1 - ((f & FL_BLANK) ? 1 : 0));
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int mflag = (f & FM[d]) ? 1 : 0, pflag = (f & FP[d]) ? 1 : 0;
+    // cell velocity (:276-367)
+    const double fe = M.fx[d][I[d] + 1], fw_ = M.fx[d][I[d]];
+    const double pe = (fe * p[id + st[d]] + (1.0 - fe) * pc) * (double)(1 - pflag) + pc * (double)pflag;
+    const double pw = (fw_ * pc + (1.0 - fw_) * p[id - st[d]]) * (double)(1 - mflag) + pc * (double)mflag;
+    const double pg = (pe - pw) * M.dinv[d][I[d]];
+    vel[d][id] = vel[d][id] - dt * pg * fl;
+    // west face of the cell
+    const double gw = (pc - p[id - st[d]]) * M.dcinv[d][I[d]];
+    double fwv = face[d][id] - dt * gw * (double)(1 - mflag);
+    bool zero = solid;
+    if (I[d] > 1) {
+      const uint8_t fo = L0.fl[la - lst[d]];
+      const int po = (fo & FP[d]) ? 1 : 0;
+      fwv = fwv * (double)(1 - po) + bc[d][id - st[d]] * (double)po;
+      zero = zero || ((fo & FL_BLANK) && !(fo & FL_GHOST));
+    }
+    fwv = fwv * (double)(1 - mflag) + bc[d][id] * (double)mflag;
+    face[d][id] = zero ? 0.0 : fwv;
+    // east face of the last cell
+    if (I[d] == nc[d]) {
+      const double ge = (p[id + st[d]] - pc) * M.dcinv[d][I[d] + 1];
+      double fev = face[d][id + st[d]] - dt * ge * (double)(1 - pflag);
+      fev = fev * (double)(1 - pflag) + bc[d][id] * (double)pflag;
+      face[d][id + st[d]] = solid ? 0.0 : fev;
+    }
+  }
+}
+
+// velocity ghost rows (GCM_vel_set_bc_internal): gather the ghost cells'
+// current (u, v, w), Jacobi sweeps of
+//   u_g = u_b bw - (sum_r cD_r x_r) iw
+// (x_r: another ghost's iterate when kind >= 0, the field at off when
+// kind == -1, the body-intercept value u_b when kind == -2), then scatter
+template <typename T>
+__global__ void k_vel_ghost_gather(int n, const long *__restrict__ gref, const T *__restrict__ u,
+                                   const T *__restrict__ v, const T *__restrict__ w, T *__restrict__ g) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const long r = gref[q];
+  g[3 * q] = u[r]; g[3 * q + 1] = v[r]; g[3 * q + 2] = w[r];
+}
+
+__global__ void k_vel_ghost_sweep(int n, const double *__restrict__ cD, const int *__restrict__ kind,
+                                  const long *__restrict__ off, double bw, double iw, const double *__restrict__ vbi,
+                                  const double *__restrict__ u, const double *__restrict__ v,
+                                  const double *__restrict__ w, const double *__restrict__ gin,
+                                  double *__restrict__ gout) {
+#pragma clang fp contract(off)
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const double ub = vbi ? vbi[3 * q] : 0.0, vb = vbi ? vbi[3 * q + 1] : 0.0, wb = vbi ? vbi[3 * q + 2] : 0.0;
+  double su = 0.0, sv = 0.0, sw = 0.0;
+  for (int r = 0; r < 8; r++) {
+    const long e = 8L * q + r;
+    const double cf = cD[e];
+    const int kd = kind[e];
+    double xu, xv, xw;
+    if (kd >= 0) {
+      xu = gin[3L * kd]; xv = gin[3L * kd + 1]; xw = gin[3L * kd + 2];
+    } else if (kd == -1) {
+      const long o = off[e];
+      xu = u[o]; xv = v[o]; xw = w[o];
+    } else {
+      xu = ub; xv = vb; xw = wb;
+    }
+    su += cf * xu; sv += cf * xv; sw += cf * xw;
+  }
+  gout[3L * q] = ub * bw - su * iw;
+  gout[3L * q + 1] = vb * bw - sv * iw;
+  gout[3L * q + 2] = wb * bw - sw * iw;
+}
+
+__global__ void k_vel_ghost_scatter(int n, const long *__restrict__ gref, const double *__restrict__ g,
+                                    double *__restrict__ u, double *__restrict__ v, double *__restrict__ w) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const long r = gref[q];
+  u[r] = g[3 * q]; v[r] = g[3 * q + 1]; w[r] = g[3 * q + 2];
+}
+
+// update_pressure (CORE_FRAC_STEP_ROUTINES.F90:958-962) over the whole
+// field: p = p + p' (van Kan) or p = p'
+__global__ void k_update_pressure(long n, int van_kan, const double *__restrict__ pp, double *__restrict__ p) {
+#pragma clang fp contract(off)
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  p[q] = van_kan ? p[q] + pp[q] : pp[q];
+}
+
 // p += e on fluid cells (refinement update, FP64 reference layout)
 template <typename TE>
 __global__ void __launch_bounds__(256) k_update(RefArgs R, LevelArgs<double> L, double *__restrict__ p,

@@ -131,7 +131,7 @@ void free_ghosts(pp_ctx *c) {
                   g.nb_ref, g.nb_al, g.nb_w64, g.nb_w32, g.t_ref, g.t_al, g.t_ptr, g.t_ghost, g.t_coef64,
                   g.t_coef32, g.t_vinv64, g.t_vinv32, g.g_tidx, g.lam64[0], g.lam64[1], g.rhsg64,
                   g.tbase64, g.lam32[0], g.lam32[1], g.rhsg32, g.tbase32, g.olam64[0], g.olam64[1],
-                  g.orhsg64, g.otbase64};
+                  g.orhsg64, g.otbase64, g.v_cD, g.v_buf[0], g.v_buf[1], g.v_kind, g.v_off};
   for (void *p : ptrs) hipFree(p);
   g = GhostTables();
   hipFree(c->cs.G); hipFree(c->cs.G32); hipFree(c->cs.cell);
@@ -345,7 +345,7 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
   G.n = n;
   G.h_ig.resize(n); G.h_jg.resize(n); G.h_kg.resize(n);
   G.h_g_ref.resize(n); G.h_g_al.resize(n);
-  G.h_s_ref.resize(8L * n); G.h_s_al.resize(8L * n); G.h_coef.resize(8L * n);
+  G.h_s_ref.resize(8L * n); G.h_s_al.resize(8L * n); G.h_coef.resize(8L * n); G.h_s_raw.resize(8L * n);
   G.h_bself.resize(n);
   G.solid_in_stencil = 0;
   auto inb = [&](int i, int j, int k) {
@@ -371,6 +371,7 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
         set_error("ghost %d stencil cell (%d,%d,%d) outside the grid", q, ii, jj, kk);
         return PP_BAD_GEOMETRY;
       }
+      G.h_s_raw[8L * q + r] = ridx(c->ref, ii - 1, jj - 1, kk - 1);
       if (ii == iG && jj == jG && kk == kG) {
         cself += cf;
         G.h_coef[8L * q + r] = 0.0;

@@ -73,7 +73,8 @@ module pp_hip_mod
   public :: pp_default_opts, pp_last_error, pp_create, pp_destroy, pp_set_geometry, pp_solve, &
        pp_solve_device, pp_adjoint, pp_calc_residual, pp_ghost_rows, pp_mean_remove, pp_divergence, &
        pp_smooth, pp_null_vector, pp_level_info, pp_error_text, pp_geom_build, pp_geom_counts, pp_geom_get, &
-       pp_geom_ghost_elements, pp_geom_destroy
+       pp_geom_ghost_elements, pp_geom_destroy, pp_geom_dirichlet, pp_set_vel_ghosts, pp_correct_vel, &
+       pp_correct_vel_device, pp_update_pressure, pp_update_pressure_device
 
   interface
      subroutine pp_default_opts(o) bind(c, name="pp_default_opts")
@@ -214,6 +215,51 @@ module pp_hip_mod
        type(c_ptr), value :: geo, body, elem
        integer(c_int32_t) :: rc
      end function pp_geom_ghost_elements
+
+     function pp_geom_dirichlet(geo, coeffD) bind(c, name="pp_geom_dirichlet") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: geo, coeffD
+       integer(c_int32_t) :: rc
+     end function pp_geom_dirichlet
+
+     ! fractional step either side of the solve: correct_vel and update_pressure
+     function pp_set_vel_ghosts(ctx, coeffD, bi_weight, ip_weight) bind(c, name="pp_set_vel_ghosts") result(rc)
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, coeffD
+       real(c_double), value :: bi_weight, ip_weight
+       integer(c_int32_t) :: rc
+     end function pp_set_vel_ghosts
+
+     function pp_correct_vel(ctx, dt, p, u, v, w, face_u, face_v, face_w, bcxu, bcyv, bczw, vel_bi) &
+          bind(c, name="pp_correct_vel") result(rc)
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, p, u, v, w, face_u, face_v, face_w, bcxu, bcyv, bczw, vel_bi
+       real(c_double), value :: dt
+       integer(c_int32_t) :: rc
+     end function pp_correct_vel
+
+     function pp_correct_vel_device(ctx, dt, p, u, v, w, face_u, face_v, face_w, bcxu, bcyv, bczw, vel_bi) &
+          bind(c, name="pp_correct_vel_device") result(rc)
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, p, u, v, w, face_u, face_v, face_w, bcxu, bcyv, bczw, vel_bi
+       real(c_double), value :: dt
+       integer(c_int32_t) :: rc
+     end function pp_correct_vel_device
+
+     function pp_update_pressure(ctx, van_kan, pprime, p, bc_src) bind(c, name="pp_update_pressure") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx, pprime, p, bc_src
+       integer(c_int32_t), value :: van_kan
+       integer(c_int32_t) :: rc
+     end function pp_update_pressure
+
+     function pp_update_pressure_device(ctx, van_kan, pprime, p, bc_src) &
+          bind(c, name="pp_update_pressure_device") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx, pprime, p, bc_src
+       integer(c_int32_t), value :: van_kan
+       integer(c_int32_t) :: rc
+     end function pp_update_pressure_device
 
      function pp_geom_destroy(geo) bind(c, name="pp_geom_destroy") result(rc)
        import :: c_ptr, c_int32_t

@@ -145,11 +145,18 @@ def lib():
     L.pp_geom_get.argtypes = [P] * 14
     L.pp_geom_ghost_elements.argtypes = [P, P, P]
     L.pp_geom_destroy.argtypes = [P]
+    L.pp_geom_dirichlet.argtypes = [P, P]
+    L.pp_set_vel_ghosts.argtypes = [P, P, C.c_double, C.c_double]
+    L.pp_correct_vel.argtypes = [P, C.c_double] + [P] * 11
+    L.pp_correct_vel_device.argtypes = [P, C.c_double] + [P] * 11
+    L.pp_update_pressure.argtypes = [P, C.c_int32, P, P, P]
+    L.pp_update_pressure_device.argtypes = [P, C.c_int32, P, P, P]
     for name in ("pp_create", "pp_destroy", "pp_set_geometry", "pp_solve", "pp_solve_device",
                  "pp_adjoint", "pp_calc_residual", "pp_ghost_rows", "pp_mean_remove",
                  "pp_divergence", "pp_smooth", "pp_tdma_batched", "pp_null_vector", "pp_level_info", "pp_time_kernel",
                  "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get", "pp_geom_ghost_elements",
-                 "pp_geom_destroy"):
+                 "pp_geom_destroy", "pp_geom_dirichlet", "pp_set_vel_ghosts", "pp_correct_vel",
+                 "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -159,7 +166,8 @@ EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_s
             "pp_solve", "pp_solve_device", "pp_adjoint", "pp_calc_residual", "pp_ghost_rows",
             "pp_mean_remove", "pp_divergence", "pp_smooth", "pp_tdma_batched", "pp_null_vector",
             "pp_level_info", "pp_time_kernel", "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get",
-            "pp_geom_ghost_elements", "pp_geom_destroy")
+            "pp_geom_ghost_elements", "pp_geom_destroy", "pp_geom_dirichlet", "pp_set_vel_ghosts",
+            "pp_correct_vel", "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device")
 
 
 def _check(rc, allow=(PP_OK,)):
@@ -305,6 +313,42 @@ class PressureSolver:
         _check(lib().pp_ghost_rows(self.h, p.ctypes.data, None if bc is None else bc.ctypes.data))
         return p
 
+    def set_vel_ghosts(self, coeffD, bi_weight=2.0, ip_weight=1.0):
+        """pp_set_vel_ghosts: coeffGCMD (n_ghost x 8) and the GCM weights."""
+        cd = np.ascontiguousarray(coeffD, dtype=np.float64)
+        _check(lib().pp_set_vel_ghosts(self.h, cd.ctypes.data if cd.size else None, float(bi_weight),
+                                       float(ip_weight)))
+
+    def correct_vel(self, dt, p, u, v, w, face_u, face_v, face_w, bcxu, bcyv, bczw, vel_bi=None):
+        """pp_correct_vel on copies of the host fields; returns (u, v, w, face_u, face_v, face_w)."""
+        out = [np.array(a, dtype=np.float64, order="C", copy=True) for a in (u, v, w, face_u, face_v, face_w)]
+        ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (p, bcxu, bcyv, bczw)]
+        for a in out + ins:
+            assert a.shape == self.shape
+        vb = None if vel_bi is None else np.ascontiguousarray(vel_bi, dtype=np.float64)
+        _check(lib().pp_correct_vel(self.h, float(dt), ins[0].ctypes.data, *[a.ctypes.data for a in out],
+                                    *[a.ctypes.data for a in ins[1:]], None if vb is None else vb.ctypes.data))
+        return out
+
+    def correct_vel_device(self, dt, p, u, v, w, face_u, face_v, face_w, bcxu, bcyv, bczw, vel_bi=None):
+        """pp_correct_vel_device on torch float64 tensors (updated in place)."""
+        t = (p, u, v, w, face_u, face_v, face_w, bcxu, bcyv, bczw)
+        _check(lib().pp_correct_vel_device(self.h, float(dt), *[a.data_ptr() for a in t],
+                                           None if vel_bi is None else vel_bi.data_ptr()))
+
+    def update_pressure(self, van_kan, pprime, p, bc_src=None):
+        """pp_update_pressure on a copy of p; returns the new p."""
+        out = np.array(p, dtype=np.float64, order="C", copy=True)
+        pr = np.ascontiguousarray(pprime, dtype=np.float64)
+        bc = None if bc_src is None else np.ascontiguousarray(bc_src, dtype=np.float64)
+        _check(lib().pp_update_pressure(self.h, int(van_kan), pr.ctypes.data, out.ctypes.data,
+                                        None if bc is None else bc.ctypes.data))
+        return out
+
+    def update_pressure_device(self, van_kan, pprime, p, bc_src=None):
+        _check(lib().pp_update_pressure_device(self.h, int(van_kan), pprime.data_ptr(), p.data_ptr(),
+                                               None if bc_src is None else bc_src.data_ptr()))
+
     def divergence(self, face_u, face_v, face_w, dtinv=1.0):
         """rhs_poisson (non-cut-cell): returns (nlu0, sum(nlu0*dV))."""
         f = [np.ascontiguousarray(a, dtype=np.float64) for a in (face_u, face_v, face_w)]
@@ -424,10 +468,13 @@ def build_geometry(xn, yn, zn, bodies, mode=PP_GEOM_ANALYTIC, probe_norm=2.0, ib
                              nrm.ctypes.data))
         body, elem = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
         _check(L.pp_geom_ghost_elements(h, body.ctypes.data, elem.ctypes.data))
+        coeffD = np.zeros((max(n, 1), 8))
+        _check(L.pp_geom_dirichlet(h, coeffD.ctypes.data))
     finally:
         L.pp_geom_destroy(h)
     gs = GhostSet(*[a[:n] for a in ints], coeff[:n], probe[:n])
-    return ib, fr, gs, {"bi": bi[:n], "ip": ip[:n], "nrm": nrm[:n], "body": body[:n], "elem": elem[:n]}
+    return ib, fr, gs, {"bi": bi[:n], "ip": ip[:n], "nrm": nrm[:n], "body": body[:n], "elem": elem[:n],
+                        "coeffD": coeffD[:n]}
 
 
 def tanh_nodes(n, length, beta):

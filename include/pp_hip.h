@@ -209,6 +209,50 @@ int32_t pp_divergence(pp_ctx *ctx, const double *face_u, const double *face_v,
  * order).  omega is the relaxation weight (MAIN.F90:478). */
 int32_t pp_smooth(pp_ctx *ctx, int32_t smoother, double omega, int32_t sweeps,
                   double *var, const double *r);
+/* ---- fractional step either side of the solve ----------------------
+ * Velocity ghost rows for correct_vel: coeffGCMD(8, n_ghost) column-major
+ * (GCM_Calc_VanMatrixDN, GCM_SET_BOUNDARY.F90:307-494; pp_geom_dirichlet
+ * gives them for the built-in geometry) for the ghost list and stencil
+ * cells of the last pp_set_geometry, and the GCM weights
+ * bodyInterceptWeight, imagePointWeight (MAIN.F90:913-914: p/(p-1) and
+ * 1/(p-1) for probeLengthNormalized p).  Call after every pp_set_geometry.
+ * The number of Jacobi sweeps that solves the ghost-ghost coupling to
+ * round-off is fixed here (PP_BAD_GEOMETRY if it does not converge). */
+int32_t pp_set_vel_ghosts(pp_ctx *ctx, const double *coeffD, double bi_weight,
+                          double ip_weight);
+/* correct_vel (CORE_FRAC_STEP_ROUTINES.F90:244-452) for non-periodic outer
+ * boundaries without cut cells: u,v,w -= dt grad p (face values by the fx
+ * interpolation, outer faces by the cell value) on cells with iblank = 0;
+ * the ghost velocities by GCM_vel_set_bc_internal
+ * (GCM_GHOSTCELL_VEL_UPDATE_UNSTRUC.F90:126-247) solved to round-off (the
+ * reference iterates Gauss-Seidel to restol_ad); face_u,v,w -= dt times
+ * the face pressure gradient, outer faces set from bcxu/bcyv/bczw; faces
+ * of blank non-ghost cells zeroed (GCM_set_face_vel_body, :387-423).
+ * Every field is a host array in the reference layout (0:nx+1,0:ny+1,0:nz+1);
+ * vel_bi = (u,v,w) at each body intercept (3*n_ghost, uBodyIntercept..;
+ * NULL: bodies at rest). */
+int32_t pp_correct_vel(pp_ctx *ctx, double dt, const double *p, double *u,
+                       double *v, double *w, double *face_u, double *face_v,
+                       double *face_w, const double *bcxu, const double *bcyv,
+                       const double *bczw, const double *vel_bi);
+/* pp_correct_vel on device arrays (no host copies). */
+int32_t pp_correct_vel_device(pp_ctx *ctx, double dt, const double *d_p,
+                              double *d_u, double *d_v, double *d_w,
+                              double *d_face_u, double *d_face_v,
+                              double *d_face_w, const double *d_bcxu,
+                              const double *d_bcyv, const double *d_bczw,
+                              const double *d_vel_bi);
+/* update_pressure (CORE_FRAC_STEP_ROUTINES.F90:949-967): p = p + pPrime
+ * (van_kan != 0, frac_step_type == VAN_KAN) or p = pPrime on the whole
+ * field, then the pressure ghost rows (GCM_p_set_bc_internal) on p with
+ * bc_src as in pp_solve (NULL: homogeneous).  The outer layer is left as
+ * the sum (after a solve both hold the Neumann copy). */
+int32_t pp_update_pressure(pp_ctx *ctx, int32_t van_kan, const double *pprime,
+                           double *p, const double *bc_src);
+int32_t pp_update_pressure_device(pp_ctx *ctx, int32_t van_kan,
+                                  const double *d_pprime, double *d_p,
+                                  const double *d_bc_src);
+
 /* tdma (CORE_TDMA.F90:10-34) on a batch of independent systems, through
  * the device line solvers: n_lines systems of length n stored line after
  * line (a: sub-, b: main, c: super-diagonal, r: rhs, u: solution).
@@ -276,6 +320,9 @@ int32_t pp_geom_get(const pp_geom *geo, int8_t *iblank, int8_t *fresh,
  * mode, the surface element of its intercept (1-based, closestElementGC,
  * GCM_SET_BOUNDARY.F90:136-137; 0 in analytic mode).  Either may be NULL. */
 int32_t pp_geom_ghost_elements(const pp_geom *geo, int32_t *body, int32_t *elem);
+/* coeffGCMD(8,n) column-major: the velocity (Dirichlet) coefficients of
+ * GCM_Calc_VanMatrixDN for the same ghosts, for pp_set_vel_ghosts. */
+int32_t pp_geom_dirichlet(const pp_geom *geo, double *coeffD);
 int32_t pp_geom_destroy(pp_geom *geo);
 
 #ifdef __cplusplus

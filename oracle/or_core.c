@@ -73,9 +73,9 @@ or_ctx *or_create(int nxc, int nyc, int nzc, const double *xn,
 static void free_ghosts(or_ctx *c) {
   free(c->ig); free(c->jg); free(c->kg);
   free(c->ic); free(c->jc); free(c->kc); free(c->elem);
-  free(c->coeff); free(c->probe); free(c->bi); free(c->ipt);
+  free(c->coeff); free(c->probe); free(c->bi); free(c->ipt); free(c->coeffD);
   c->ig = c->jg = c->kg = c->ic = c->jc = c->kc = c->elem = NULL;
-  c->coeff = c->probe = c->bi = c->ipt = NULL;
+  c->coeff = c->probe = c->bi = c->ipt = c->coeffD = NULL;
 }
 
 void or_destroy(or_ctx *c) {
@@ -155,6 +155,7 @@ void or_mark_ghosts(or_ctx *c) {
   c->elem = (int32_t *)calloc(ng, 4);
   c->coeff = dalloc(8 * ng); c->probe = dalloc(ng);
   c->bi = dalloc(3 * ng); c->ipt = dalloc(3 * ng);
+  c->coeffD = dalloc(8 * ng);
   nb = 0;
   for (int k = 1; k <= c->nz - 1; k++)
     for (int j = 1; j <= c->ny - 1; j++)

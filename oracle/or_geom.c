@@ -693,6 +693,9 @@ int or_geom_set_boundary(or_ctx *c, const int8_t *iblank_prev, double probe_norm
   isolated_iblank(c, c->iblank, c->bodynum);
   fill_cell(c, c->iblank, c->fresh);
   or_mark_ghosts(c);
+  /* MAIN.F90:913-914 */
+  c->bi_weight = probe_norm / (probe_norm - 1.0);
+  c->ip_weight = 1.0 / (probe_norm - 1.0);
   for (int n = 0; n < c->nghost; n++) {
     int iG = c->ig[n], jG = c->jg[n], kG = c->kg[n];
     long id = OR_IDX(c, iG, jG, kG);
@@ -720,13 +723,16 @@ int or_geom_set_boundary(or_ctx *c, const int8_t *iblank_prev, double probe_norm
         if (cen[d][i] <= ip[d] && cen[d][i + 1] > ip[d]) cell[d] = i;
       if (cell[d] < 0) return 1;
     }
-    double V[8][8], Vi[8][8];
+    double V[8][8], Vi[8][8], VD[8][8], VDi[8][8];
     double nrm[3] = {B->nxv[elem], B->nyv[elem], B->nzv[elem]};
     for (int r = 0; r < 8; r++) {
       int i = cell[0] + incI[r], j = cell[1] + incJ[r], k = cell[2] + incK[r];
       double x1 = c->xc[i], x2 = c->yc[j], x3 = c->zc[k];
       if (i == iG && j == jG && k == kG) {
         x1 = bi[0]; x2 = bi[1]; x3 = bi[2];
+        /* Dirichlet row: the value at the body intercept */
+        VD[r][0] = x1 * x2 * x3; VD[r][1] = x1 * x2; VD[r][2] = x1 * x3; VD[r][3] = x2 * x3;
+        VD[r][4] = x1; VD[r][5] = x2; VD[r][6] = x3; VD[r][7] = 1.0;
         V[r][0] = nrm[0] * x2 * x3 + nrm[1] * x1 * x3 + nrm[2] * x1 * x2;
         V[r][1] = nrm[0] * x2 + nrm[1] * x1;
         V[r][2] = nrm[0] * x3 + nrm[2] * x1;
@@ -735,9 +741,15 @@ int or_geom_set_boundary(or_ctx *c, const int8_t *iblank_prev, double probe_norm
       } else {
         V[r][0] = x1 * x2 * x3; V[r][1] = x1 * x2; V[r][2] = x1 * x3; V[r][3] = x2 * x3;
         V[r][4] = x1; V[r][5] = x2; V[r][6] = x3; V[r][7] = 1.0;
+        for (int q = 0; q < 8; q++) VD[r][q] = V[r][q];
       }
     }
     if (!invert8(V, Vi)) return 3;
+    if (!invert8(VD, VDi)) return 3;
+    for (int r = 0; r < 8; r++)
+      c->coeffD[8 * n + r] = VDi[0][r] * ip[0] * ip[1] * ip[2] + VDi[1][r] * ip[0] * ip[1] +
+                             VDi[2][r] * ip[0] * ip[2] + VDi[3][r] * ip[1] * ip[2] + VDi[4][r] * ip[0] +
+                             VDi[5][r] * ip[1] + VDi[6][r] * ip[2] + VDi[7][r];
     for (int r = 0; r < 8; r++)
       c->coeff[8 * n + r] = Vi[0][r] * ip[0] * ip[1] * ip[2] + Vi[1][r] * ip[0] * ip[1] + Vi[2][r] * ip[0] * ip[2] +
                             Vi[3][r] * ip[1] * ip[2] + Vi[4][r] * ip[0] + Vi[5][r] * ip[1] + Vi[6][r] * ip[2] +
@@ -748,6 +760,10 @@ int or_geom_set_boundary(or_ctx *c, const int8_t *iblank_prev, double probe_norm
     for (int d = 0; d < 3; d++) { c->bi[3 * n + d] = bi[d]; c->ipt[3 * n + d] = ip[d]; }
   }
   return 0;
+}
+
+void or_geom_get_dirichlet(const or_ctx *c, double *coeffD) {
+  memcpy(coeffD, c->coeffD, sizeof(double) * 8 * (size_t)c->nghost);
 }
 
 void or_geom_get_iblank(const or_ctx *c, int8_t *iblank, int8_t *fresh) {

@@ -30,6 +30,8 @@ struct or_ctx {
   int nghost;
   int32_t *ig, *jg, *kg, *ic, *jc, *kc, *elem;
   double *coeff, *probe, *bi, *ipt;
+  double *coeffD;                /* coeffGCMD(8,n), velocity (Dirichlet) set */
+  double bi_weight, ip_weight;   /* bodyInterceptWeight, imagePointWeight */
   /* bodies */
   int nbody;
   or_body body[64];

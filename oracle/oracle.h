@@ -147,6 +147,28 @@ void or_geom_get_ghost_geometry(const or_ctx *c, int32_t *ic, int32_t *jc,
                                 double *bi, double *ip, int32_t *elem,
                                 int32_t *body);
 
+/* coeffGCMD(8,n), column-major, of the last or_geom_set_boundary (the
+ * velocity set of GCM_Calc_VanMatrixDN, GCM_SET_BOUNDARY.F90:307-494) */
+void or_geom_get_dirichlet(const or_ctx *c, double *coeffD);
+
+/* ---- fractional step either side of the solve (oracle/or_frac.c) ----- */
+/* coeffGCMD(8,n) for the current ghost list and the GCM weights
+ * bodyInterceptWeight, imagePointWeight (MAIN.F90:913-914) */
+void or_set_vel_ghosts(or_ctx *c, const double *coeffD, double bi_weight, double ip_weight);
+/* GCM_vel_set_bc_internal (GCM_GHOSTCELL_VEL_UPDATE_UNSTRUC.F90:126-247);
+ * vbi: (u,v,w) at each body intercept, NULL = 0.  Returns sweeps. */
+int or_vel_ghosts(const or_ctx *c, double *u, double *v, double *w, const double *vbi, int itermax,
+                  double restol);
+/* correct_vel (CORE_FRAC_STEP_ROUTINES.F90:244-452), non-periodic, no cut
+ * cells: cell velocities, ghost velocities, face velocities, solid faces */
+void or_correct_vel(const or_ctx *c, double dt, const double *p, double *u, double *v, double *w, double *fu,
+                    double *fv, double *fw, const double *bcxu, const double *bcyv, const double *bczw,
+                    const double *vbi, int itermax_ad, double restol_ad);
+/* update_pressure (CORE_FRAC_STEP_ROUTINES.F90:949-967): p += p' (van_kan)
+ * or p = p', then GCM_p_set_bc_internal on p */
+void or_update_pressure(const or_ctx *c, int van_kan, const double *pprime, double *p, const double *g, int itermax,
+                        double restol);
+
 #ifdef __cplusplus
 }
 #endif

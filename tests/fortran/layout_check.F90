@@ -19,7 +19,7 @@ program layout_check
   integer(c_int32_t) :: ng, nb, nf, i
   integer(1), allocatable, target :: ib(:,:,:)
   integer(c_int32_t), allocatable, target :: gi(:,:), gb(:), ge(:)
-  real(c_double), allocatable, target :: coeff(:,:), probe(:)
+  real(c_double), allocatable, target :: coeff(:,:), probe(:), coeffD(:,:)
   call pp_default_opts(o)
   print '(A,I0,1X,ES12.5,1X,ES12.5,7(1X,I0),1X,F6.3,5(1X,I0))', 'opts ', o%precision, o%rel_tol, o%abs_linf_tol, &
        o%max_outer, o%cycles_per_outer, o%nu_pre, o%nu_post, o%mean_remove, o%project_rhs, o%use_graph, &
@@ -56,6 +56,9 @@ program layout_check
        sum(gi(:, 3)), sum(gi(:, 4)), sum(gi(:, 5)), sum(gi(:, 6)), sum(coeff), sum(probe)
   rc = pp_geom_ghost_elements(geo, c_loc(gb), c_loc(ge))
   print '(A,I0,1X,I0,1X,I0)', 'elems ', rc, sum(gb), sum(ge)
+  allocate(coeffD(8, ng))
+  rc = pp_geom_dirichlet(geo, c_loc(coeffD))
+  print '(A,I0,1X,ES24.16)', 'dirichlet ', rc, sum(coeffD)
   rc = pp_geom_destroy(geo)
   print '(A,I0)', 'destroy ', rc
 end program layout_check

@@ -66,6 +66,11 @@ def lib():
         L.or_geom_set_boundary.argtypes = [P, P, C.c_double]
         L.or_geom_get_iblank.argtypes = [P, P, P]
         L.or_geom_get_ghost_geometry.argtypes = [P] * 10
+        L.or_geom_get_dirichlet.argtypes = [P, P]
+        L.or_set_vel_ghosts.argtypes = [P, P, C.c_double, C.c_double]
+        L.or_vel_ghosts.argtypes = [P] * 5 + [C.c_int, C.c_double]
+        L.or_correct_vel.argtypes = [P, C.c_double] + [P] * 11 + [C.c_int, C.c_double]
+        L.or_update_pressure.argtypes = [P, C.c_int, P, P, P, C.c_int, C.c_double]
         _L = L
     return _L
 
@@ -159,9 +164,38 @@ class Oracle:
              "body": np.zeros(m, np.int32)}
         lib().or_geom_get_ghost_geometry(self.h, *[g[k].ctypes.data for k in
                                                    ("ic", "jc", "kc", "coeff", "probe", "bi", "ip", "elem", "body")])
-        for k in ("ic", "jc", "kc", "coeff", "probe", "bi", "ip", "elem", "body"):
+        g["coeffD"] = np.zeros((m, 8))
+        lib().or_geom_get_dirichlet(self.h, g["coeffD"].ctypes.data)
+        for k in ("ic", "jc", "kc", "coeff", "probe", "bi", "ip", "elem", "body", "coeffD"):
             g[k] = g[k][:n]
         return rc, ib, fr, g
+
+    def set_vel_ghosts(self, coeffD, bi_weight=2.0, ip_weight=1.0):
+        cd = np.ascontiguousarray(coeffD, dtype=np.float64)
+        lib().or_set_vel_ghosts(self.h, cd.ctypes.data, float(bi_weight), float(ip_weight))
+
+    def vel_ghosts(self, u, v, w, vbi=None, itermax=1000, restol=0.0):
+        vb = None if vbi is None else np.ascontiguousarray(vbi, dtype=np.float64)
+        return lib().or_vel_ghosts(self.h, u.ctypes.data, v.ctypes.data, w.ctypes.data,
+                                   None if vb is None else vb.ctypes.data, itermax, restol)
+
+    def correct_vel(self, dt, p, u, v, w, fu, fv, fw, bcxu, bcyv, bczw, vbi=None, itermax_ad=1000, restol_ad=0.0):
+        """correct_vel on copies; returns (u, v, w, fu, fv, fw)"""
+        out = [np.array(a, dtype=np.float64, order="C", copy=True) for a in (u, v, w, fu, fv, fw)]
+        ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (p, bcxu, bcyv, bczw)]
+        vb = None if vbi is None else np.ascontiguousarray(vbi, dtype=np.float64)
+        lib().or_correct_vel(self.h, float(dt), ins[0].ctypes.data, *[a.ctypes.data for a in out],
+                             *[a.ctypes.data for a in ins[1:]], None if vb is None else vb.ctypes.data,
+                             itermax_ad, restol_ad)
+        return out
+
+    def update_pressure(self, van_kan, pprime, p, g=None, itermax=1000, restol=0.0):
+        out = np.array(p, dtype=np.float64, order="C", copy=True)
+        pp_ = np.ascontiguousarray(pprime, dtype=np.float64)
+        gg = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
+        lib().or_update_pressure(self.h, int(van_kan), pp_.ctypes.data, out.ctypes.data,
+                                 None if gg is None else gg.ctypes.data, itermax, restol)
+        return out
 
     def set_fresh(self, fr):
         fr = np.ascontiguousarray(fr, dtype=np.int8)

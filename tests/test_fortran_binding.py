@@ -56,4 +56,6 @@ def test_fortran_module_matches_the_c_abi(tmp_path):
     assert float(g[8]) == pytest.approx(float(gs.coeff.sum()), rel=1e-14, abs=1e-12)
     assert float(g[9]) == pytest.approx(float(gs.probe.sum()), rel=1e-14)
     assert [int(v) for v in out["elems"]] == [pp.PP_OK, int(ex["body"].sum()), int(ex["elem"].sum())]
+    d = out["dirichlet"]
+    assert int(d[0]) == pp.PP_OK and float(d[1]) == pytest.approx(float(ex["coeffD"].sum()), rel=1e-14)
     assert int(out["destroy"][0]) == pp.PP_OK
