@@ -554,9 +554,10 @@ int set_ghost_bnd(pp_ctx *c, const std::vector<double> &bc) {
 
 // fluid sums: out[0..4] = {sum w*b, sum w, sum b*b, sum V*p, sum V}
 int fluid_sums(pp_ctx *c, const double *w, const double *b, const double *p) {
-  CellLaunch cl = ref_launch(c);
+  constexpr int CPT = 4;
+  CellLaunch cl = cell_launch((c->nxc + CPT - 1) / CPT, c->nyc, c->nzc);
   long nblk = (long)cl.grid.x * cl.grid.y * cl.grid.z;
-  k_fluid_sums<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), w, b, p, c->d_part);
+  k_fluid_sums<256, CPT><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), w, b, p, c->d_part);
   launch_reduce_sums(c, (int)nblk, 5);
   return PP_OK;
 }

@@ -522,27 +522,41 @@ __global__ void __launch_bounds__(256) k_mask_rhs(RefArgs R, LevelArgs<double> L
 }
 
 // per-block partial sums over fluid cells: q0 = sum(w*b), q1 = sum(w),
-// q2 = sum(b*b), q3 = sum(V*p), q4 = sum(V)
-template <int NT>
+// q2 = sum(b*b), q3 = sum(V*p), q4 = sum(V).  CPT consecutive cells of a
+// row per thread (the grid's x extent covers nxc / CPT), all loads issued
+// before the flags are tested, so that a wavefront has CPT cells of every
+// array in flight instead of one flag round trip ahead of one cell.
+template <int NT, int CPT>
 __global__ void __launch_bounds__(NT) k_fluid_sums(RefArgs R, LevelArgs<double> L, const double *__restrict__ w,
                                                    const double *__restrict__ b, const double *__restrict__ p,
                                                    double *__restrict__ part) {
   __shared__ double s[NT / 64 * 5];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * CPT;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
   double q[5] = {0, 0, 0, 0, 0};
-  if (i < R.nxc && j < R.nyc) {
-    long a = lidx(L.sx, L.sxy, i, j, k);
-    if (!(L.fl[a] & FL_BLANK)) {
-      long r = ridx(R, i, j, k);
-      double vol = L.wx[i] * L.wy[j] * L.wz[k];
-      double wv = w ? w[r] : vol;
-      double bv = b ? b[r] : 0.0;
-      q[0] = wv * bv;
-      q[1] = wv;
-      q[2] = bv * bv;
-      q[3] = p ? p[r] * vol : 0.0;
-      q[4] = vol;
+  if (i0 < R.nxc && j < R.nyc) {
+    const long a0 = lidx(L.sx, L.sxy, i0, j, k);
+    const long r0 = ridx(R, i0, j, k);
+    uint8_t f[CPT];
+    double wv[CPT], bv[CPT], pv[CPT], vol[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+      const bool in = i0 + c < R.nxc;
+      f[c] = in ? L.fl[a0 + c] : (uint8_t)FL_BLANK;
+      vol[c] = in ? L.wx[i0 + c] * L.wy[j] * L.wz[k] : 0.0;
+      wv[c] = (in && w) ? w[r0 + c] : 0.0;
+      bv[c] = (in && b) ? b[r0 + c] : 0.0;
+      pv[c] = (in && p) ? p[r0 + c] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+      if (f[c] & FL_BLANK) continue;
+      const double wc = w ? wv[c] : vol[c];
+      q[0] += wc * bv[c];
+      q[1] += wc;
+      q[2] += bv[c] * bv[c];
+      q[3] += p ? pv[c] * vol[c] : 0.0;
+      q[4] += vol[c];
     }
   }
   long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
