@@ -28,6 +28,17 @@ namespace {
 // ---------------------------------------------------------------------
 // launch geometry
 // ---------------------------------------------------------------------
+// temporary device allocation of an entry point, freed on every return path
+struct DevTemp {
+  void *p = nullptr;
+  DevTemp() = default;
+  DevTemp(const DevTemp &) = delete;
+  DevTemp &operator=(const DevTemp &) = delete;
+  ~DevTemp() { if (p) hipFree(p); }
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes); }
+  double *d() const { return static_cast<double *>(p); }
+};
+
 struct CellLaunch {
   dim3 grid, block;
 };
@@ -1537,13 +1548,12 @@ int32_t pp_solve(pp_ctx *c, double *p, const double *rhs, const double *bc, cons
   size_t nb = sizeof(double) * c->nref;
   HIPCHK(hipMemcpy(c->d_p, p, nb, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_bwork, rhs, nb, hipMemcpyHostToDevice));
-  double *dbc = nullptr;
+  DevTemp dbc;
   if (bc && c->gh.n > 0) {
-    HIPCHK(hipMalloc(&dbc, sizeof(double) * c->gh.n));
-    HIPCHK(hipMemcpy(dbc, bc, sizeof(double) * c->gh.n, hipMemcpyHostToDevice));
+    HIPCHK(dbc.alloc(sizeof(double) * c->gh.n));
+    HIPCHK(hipMemcpy(dbc.d(), bc, sizeof(double) * c->gh.n, hipMemcpyHostToDevice));
   }
-  int rc = pp_solve_device(c, c->d_p, c->d_bwork, dbc, o, st);
-  if (dbc) hipFree(dbc);
+  int rc = pp_solve_device(c, c->d_p, c->d_bwork, dbc.d(), o, st);
   if (rc == PP_OK || rc == PP_NOT_CONVERGED) HIPCHK(hipMemcpy(p, c->d_p, nb, hipMemcpyDeviceToHost));
   return rc;
 }
@@ -1727,24 +1737,22 @@ int32_t pp_correct_vel(pp_ctx *c, double dt, const double *p, double *u, double 
   if (!c->have_geom) { set_error("pp_correct_vel: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
   HIPCHK(hipSetDevice(c->device));
   const size_t nb = sizeof(double) * c->nref;
-  double *d = nullptr;
-  HIPCHK(hipMalloc(&d, 10 * nb));
+  DevTemp d, dvbi;
+  HIPCHK(d.alloc(10 * nb));
   double *dv[10];
-  for (int q = 0; q < 10; q++) dv[q] = d + q * c->nref;
+  for (int q = 0; q < 10; q++) dv[q] = d.d() + q * c->nref;
   const double *hin[10] = {p, u, v, w, fu, fv, fw, bcxu, bcyv, bczw};
   for (int q = 0; q < 10; q++) HIPCHK(hipMemcpy(dv[q], hin[q], nb, hipMemcpyHostToDevice));
-  double *dvbi = nullptr;
   if (vbi && c->gh.n > 0) {
-    HIPCHK(hipMalloc(&dvbi, sizeof(double) * 3 * c->gh.n));
-    HIPCHK(hipMemcpy(dvbi, vbi, sizeof(double) * 3 * c->gh.n, hipMemcpyHostToDevice));
+    HIPCHK(dvbi.alloc(sizeof(double) * 3 * c->gh.n));
+    HIPCHK(hipMemcpy(dvbi.d(), vbi, sizeof(double) * 3 * c->gh.n, hipMemcpyHostToDevice));
   }
-  int rc = pp_correct_vel_device(c, dt, dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7], dv[8], dv[9], dvbi);
+  int rc = pp_correct_vel_device(c, dt, dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7], dv[8], dv[9],
+                                 dvbi.d());
   if (rc == PP_OK) {
     double *hout[6] = {u, v, w, fu, fv, fw};
     for (int q = 0; q < 6; q++) HIPCHK(hipMemcpy(hout[q], dv[1 + q], nb, hipMemcpyDeviceToHost));
   }
-  hipFree(dvbi);
-  hipFree(d);
   return rc;
 }
 
@@ -1778,13 +1786,12 @@ int32_t pp_update_pressure(pp_ctx *c, int32_t van_kan, const double *pprime, dou
   const size_t nb = sizeof(double) * c->nref;
   HIPCHK(hipMemcpy(c->d_p, p, nb, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_bwork, pprime, nb, hipMemcpyHostToDevice));
-  double *dbc = nullptr;
+  DevTemp dbc;
   if (bc && c->gh.n > 0) {
-    HIPCHK(hipMalloc(&dbc, sizeof(double) * c->gh.n));
-    HIPCHK(hipMemcpy(dbc, bc, sizeof(double) * c->gh.n, hipMemcpyHostToDevice));
+    HIPCHK(dbc.alloc(sizeof(double) * c->gh.n));
+    HIPCHK(hipMemcpy(dbc.d(), bc, sizeof(double) * c->gh.n, hipMemcpyHostToDevice));
   }
-  int rc = pp_update_pressure_device(c, van_kan, c->d_bwork, c->d_p, dbc);
-  hipFree(dbc);
+  int rc = pp_update_pressure_device(c, van_kan, c->d_bwork, c->d_p, dbc.d());
   if (rc == PP_OK) HIPCHK(hipMemcpy(p, c->d_p, nb, hipMemcpyDeviceToHost));
   return rc;
 }
@@ -1921,8 +1928,12 @@ int32_t pp_tdma_batched(pp_ctx *c, const double *a, const double *b, const doubl
   if (method != 0 && method != 1) { set_error("pp_tdma_batched: method must be 0 or 1"); return PP_BAD_ARG; }
   HIPCHK(hipSetDevice(c->device));
   size_t nb = sizeof(double) * (size_t)n * nl;
+  DevTemp dt[5];
   double *d[5];
-  for (int q = 0; q < 5; q++) HIPCHK(hipMalloc(&d[q], nb));
+  for (int q = 0; q < 5; q++) {
+    HIPCHK(dt[q].alloc(nb));
+    d[q] = dt[q].d();
+  }
   std::vector<double> tmp;
   const double *src[4] = {a, b, cc, r};
   for (int q = 0; q < 4; q++) {
@@ -1966,7 +1977,6 @@ int32_t pp_tdma_batched(pp_ctx *c, const double *a, const double *b, const doubl
   } else {
     HIPCHK(hipMemcpy(u, d[4], nb, hipMemcpyDeviceToHost));
   }
-  for (int q = 0; q < 5; q++) hipFree(d[q]);
   return PP_OK;
 }
 
