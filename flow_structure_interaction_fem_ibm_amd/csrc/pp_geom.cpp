@@ -19,9 +19,9 @@
 
 namespace ppg {
 
-void parallel_for(long n, const std::function<void(long, long)> &fn) {
+void parallel_for(long n, const std::function<void(long, long)> &fn, long grain) {
   unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  if (n < 4096 || nt == 1) { fn(0, n); return; }
+  if (n < grain || n < 2 || nt == 1) { fn(0, n); return; }
   std::vector<std::thread> th;
   long chunk = (n + nt - 1) / nt;
   for (unsigned t = 0; t < nt; t++) {
@@ -150,7 +150,7 @@ void closest_point_analytic(const Body &b, const double *p, double *bi, double *
 // testing every cell against the original field: a neighbour visited earlier
 // saw this (still blank) cell and so was not cleared, and later neighbours
 // are unchanged at this cell's visit.  Hence a parallel two-pass form.
-void isolated_iblank(const Grid &G, std::vector<int8_t> &ib, std::vector<int> &bodynum) {
+void isolated_iblank(const Grid &G, ByteField &ib, IntField &bodynum) {
   std::vector<std::vector<long>> clear(G.nzc + 1);
   parallel_for(G.nzc, [&](long k0, long k1) {
     for (int k = (int)k0 + 1; k <= (int)k1; k++)
@@ -163,7 +163,7 @@ void isolated_iblank(const Grid &G, std::vector<int8_t> &ib, std::vector<int> &b
               for (int ii = std::max(1, i - 1); ii <= std::min(G.nxc, i + 1); ii++) s += ib[G.r(ii, jj, kk)];
           if (s <= 1) clear[k].push_back(G.r(i, j, k));
         }
-  });
+  }, kPlaneGrain);
   for (auto &v : clear)
     for (long r : v) {
       ib[r] = 0;
@@ -177,7 +177,7 @@ void isolated_iblank(const Grid &G, std::vector<int8_t> &ib, std::vector<int> &b
 // Only cells on the two faces normal to the sweep's axis can change, and
 // (at least 3 cells per direction) none of them is another's neighbour, so
 // each sweep visits just those faces.
-void fill_cell(const Grid &G, std::vector<int8_t> &ib, std::vector<int8_t> &fresh) {
+void fill_cell(const Grid &G, ByteField &ib, ByteField &fresh) {
   const int n[3] = {G.nxc, G.nyc, G.nzc};
   const long st[3] = {1, G.ex, G.exy};
   for (int axis = 0; axis < 3; axis++) {
@@ -202,8 +202,8 @@ void fill_cell(const Grid &G, std::vector<int8_t> &ib, std::vector<int8_t> &fres
 
 // GCM ghost marking and list (GCM_SET_BOUNDARY.F90:49-118); per-plane lists
 // concatenated in plane order keep the reference's i-fastest list order
-void mark_ghosts(const Grid &G, const std::vector<int8_t> &ib, std::vector<int8_t> &gm, std::vector<int> &list) {
-  gm.assign(G.size, 0);
+void mark_ghosts(const Grid &G, const ByteField &ib, ByteField &gm, std::vector<int> &list) {
+  par_fill(gm, G.size, (int8_t)0);
   list.clear();
   const int nx = G.nxc + 1, ny = G.nyc + 1, nz = G.nzc + 1;
   std::vector<std::vector<int>> plane(nz);
@@ -224,7 +224,7 @@ void mark_ghosts(const Grid &G, const std::vector<int8_t> &ib, std::vector<int8_
             plane[k].push_back(k);
           }
         }
-  });
+  }, kPlaneGrain);
   for (auto &v : plane) list.insert(list.end(), v.begin(), v.end());
 }
 
@@ -321,7 +321,7 @@ int ghost_stencil(const Grid &G, int iG, int jG, int kG, const double *bi, const
 using namespace ppg;
 
 struct pp_geom {
-  std::vector<int8_t> iblank, fresh, ghost;
+  ByteField iblank, fresh, ghost;
   std::vector<int> list;       // ig, jg, kg triples (1-based)
   std::vector<GhostOut> gout;
   int nblank = 0, nfresh = 0;
@@ -353,9 +353,10 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
     bs[b].n_theta = d.n_theta;
   }
   pp_geom *o = new pp_geom();
-  o->iblank.assign(G.size, 0);
-  o->fresh.assign(G.size, 0);
-  std::vector<int> bodynum(G.size, -1);
+  par_fill(o->iblank, G.size, (int8_t)0);
+  par_fill(o->fresh, G.size, (int8_t)0);
+  IntField bodynum;
+  par_fill(bodynum, G.size, -1);
   int rc = 0;
   TriBodies tri;
   if (mode == PP_GEOM_ANALYTIC) {
@@ -390,7 +391,7 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
               }
             }
         }
-      });
+      }, kPlaneGrain);
     }
   } else {
     rc = tri_build(G, bs, tri);
@@ -410,7 +411,7 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
             long r = G.r(i, j, k);
             if (o->iblank[r] == 0 && iblank_prev[r] == 1) o->fresh[r] = 1;
           }
-    });
+    }, kPlaneGrain);
   isolated_iblank(G, o->iblank, bodynum);
   fill_cell(G, o->iblank, o->fresh);
   mark_ghosts(G, o->iblank, o->ghost, o->list);
@@ -464,7 +465,7 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
     parallel_for(G.nzc + 3, [&](long k0, long k1) {
       for (long k = k0; k < k1; k++)
         for (long r = k * G.exy; r < (k + 1) * G.exy; r++) { nb[k] += o->iblank[r]; nf[k] += o->fresh[r]; }
-    });
+    }, kPlaneGrain);
     for (int k = 0; k < G.nzc + 3; k++) { o->nblank += (int)nb[k]; o->nfresh += (int)nf[k]; }
   }
   *out = o;

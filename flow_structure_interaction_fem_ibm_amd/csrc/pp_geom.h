@@ -1,7 +1,9 @@
 // Internal declarations of the host geometry module.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
+#include <memory>
 #include <functional>
 #include <vector>
 
@@ -13,7 +15,37 @@ void set_error(const char *fmt, ...);
 
 namespace ppg {
 
-void parallel_for(long n, const std::function<void(long, long)> &fn);
+// std::vector whose resize leaves the elements uninitialised: the field
+// arrays (tens of millions of cells) are filled in parallel right after
+// (par_fill) instead of value-initialised on one thread
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind { using other = NoInitAlloc<U>; };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+  template <typename U>
+  void construct(U *p) noexcept { ::new ((void *)p) U; }
+  template <typename U, typename... A>
+  void construct(U *p, A &&...a) { ::new ((void *)p) U(std::forward<A>(a)...); }
+};
+using ByteField = std::vector<int8_t, NoInitAlloc<int8_t>>;
+using IntField = std::vector<int, NoInitAlloc<int>>;
+
+// fn over [0, n) split into contiguous ranges on up to 16 threads; serial
+// below `grain` items (per-cell and per-ghost loops use the default, loops
+// over planes of the grid pass a small grain)
+constexpr long kPlaneGrain = 8;
+void parallel_for(long n, const std::function<void(long, long)> &fn, long grain = 4096);
+// v = n copies of x, written by all threads
+template <typename F, typename T>
+void par_fill(F &v, long n, T x) {
+  v.resize(n);
+  parallel_for((n + 65535) / 65536, [&](long a, long b) {
+    std::fill(v.data() + a * 65536, v.data() + std::min(n, b * 65536), x);
+  }, 2);
+}
 
 // Reference grid arrays x(0:n+1), xc(0:n+1), dx(0:n+1) per direction
 // (make_grid, CORE_GRID.F90:29-151) and the reference field indexing.
@@ -70,7 +102,8 @@ int tri_build(const Grid &G, const std::vector<Body> &bodies, TriBodies &T);
 // iblank of set_iblank_canonical_body_fast and bodynum (0-based body, -1
 // none); 0 or an error code: 4 no unblocked marker among the 7 closest,
 // 5 no element selected
-int tri_iblank(const Grid &G, const TriBodies &T, std::vector<int8_t> &iblank, std::vector<int> &bodynum);
+// bodynum must hold -1 everywhere on entry
+int tri_iblank(const Grid &G, const TriBodies &T, ByteField &iblank, IntField &bodynum);
 // body intercept, inward element normal and element (1-based) of a ghost;
 // 0 or 5
 int tri_intercept(const Grid &G, const TriBodies &T, int body, int iG, int jG, int kG, double *bi, double *nrm,
@@ -78,9 +111,9 @@ int tri_intercept(const Grid &G, const TriBodies &T, int body, int iG, int jG, i
 
 bool inside_analytic(const Body &b, const double *p);
 void closest_point_analytic(const Body &b, const double *p, double *bi, double *nin);
-void isolated_iblank(const Grid &G, std::vector<int8_t> &ib, std::vector<int> &bodynum);
-void fill_cell(const Grid &G, std::vector<int8_t> &ib, std::vector<int8_t> &fresh);
-void mark_ghosts(const Grid &G, const std::vector<int8_t> &ib, std::vector<int8_t> &gm, std::vector<int> &list);
+void isolated_iblank(const Grid &G, ByteField &ib, IntField &bodynum);
+void fill_cell(const Grid &G, ByteField &ib, ByteField &fresh);
+void mark_ghosts(const Grid &G, const ByteField &ib, ByteField &gm, std::vector<int> &list);
 int ghost_stencil(const Grid &G, int iG, int jG, int kG, const double *bi, const double *nrm, double probe_norm,
                   GhostOut &o);
 

@@ -481,19 +481,20 @@ int tri_build(const Grid &G, const std::vector<Body> &bodies, TriBodies &T) {
 }
 
 // ---- iblank ---------------------------------------------------------------
-int tri_iblank(const Grid &G, const TriBodies &T, std::vector<int8_t> &ib, std::vector<int> &bodynum) {
+int tri_iblank(const Grid &G, const TriBodies &T, ByteField &ib, IntField &bodynum) {
   const int nxc = G.nxc, nyc = G.nyc, nzc = G.nzc, nx = nxc + 1, ny = nyc + 1, nz = nzc + 1;
   const long ex = G.ex, exy = G.exy;
   for (int d = 0; d < 3; d++)
     for (size_t i = 1; i < G.XC[d].size(); i++)
       if (!(G.XC[d][i] > G.XC[d][i - 1])) return 6;
-  std::vector<int8_t> und(G.size, 0), temp(G.size, 0);
+  ByteField und, temp;
+  par_fill(temp, G.size, (int8_t)0);
+  par_fill(und, G.size, (int8_t)0);
   parallel_for(nz + 1, [&](long k0, long k1) {
     for (long k = k0; k < k1; k++)
       for (int j = 0; j <= ny; j++)
         for (int i = 0; i <= nx; i++) und[G.r(i, j, (int)k)] = kUndecided;
-  });
-  std::fill(bodynum.begin(), bodynum.end(), -1);
+  }, kPlaneGrain);
   // the band: cells in the bounding box of any element
   const int nn[3] = {nx, ny, nz}, ncc[3] = {nxc, nyc, nzc};
   for (const TriBody &B : T.b)
@@ -524,12 +525,17 @@ int tri_iblank(const Grid &G, const TriBodies &T, std::vector<int8_t> &ib, std::
         for (int j = lo[1]; j <= hi[1]; j++)
           for (int i = lo[0]; i <= hi[0]; i++) und[G.r(i, j, k)] = -kUndecided;
     }
-  // dot-normal classification of the band cells
+  // dot-normal classification of the band cells (listed in index order,
+  // plane by plane in parallel)
+  std::vector<std::vector<long>> bandk(nzc + 1);
+  parallel_for(nzc, [&](long k0, long k1) {
+    for (int k = (int)k0 + 1; k <= (int)k1; k++)
+      for (int j = 1; j <= nyc; j++)
+        for (int i = 1; i <= nxc; i++)
+          if (und[G.r(i, j, k)] == -kUndecided) bandk[k].push_back(G.r(i, j, k));
+  }, kPlaneGrain);
   std::vector<long> band;
-  for (int k = 1; k <= nzc; k++)
-    for (int j = 1; j <= nyc; j++)
-      for (int i = 1; i <= nxc; i++)
-        if (und[G.r(i, j, k)] == -kUndecided) band.push_back(G.r(i, j, k));
+  for (int k = 1; k <= nzc; k++) band.insert(band.end(), bandk[k].begin(), bandk[k].end());
   std::atomic<int> err(0);
   parallel_for((long)band.size(), [&](long a, long b) {
     std::vector<int> scratch;
@@ -564,7 +570,7 @@ int tri_iblank(const Grid &G, const TriBodies &T, std::vector<int8_t> &ib, std::
           if (temp[id] == 1) bodynum[id] = bodynum[id - 1];
         }
       }
-  });
+  }, kPlaneGrain);
   parallel_for(nzc, [&](long k0, long k1) {
     for (int k = (int)k0 + 1; k <= (int)k1; k++)
       for (int i = 1; i <= nxc; i++) {
@@ -577,7 +583,7 @@ int tri_iblank(const Grid &G, const TriBodies &T, std::vector<int8_t> &ib, std::
           if (temp[id] == 1) bodynum[id] = bodynum[id - ex];
         }
       }
-  });
+  }, kPlaneGrain);
   parallel_for(nyc, [&](long j0, long j1) {
     for (int j = (int)j0 + 1; j <= (int)j1; j++)
       for (int i = 1; i <= nxc; i++) {
@@ -590,12 +596,12 @@ int tri_iblank(const Grid &G, const TriBodies &T, std::vector<int8_t> &ib, std::
           if (temp[id] == 1) bodynum[id] = bodynum[id - exy];
         }
       }
-  });
+  }, kPlaneGrain);
   parallel_for(nzc, [&](long k0, long k1) {
     for (int k = (int)k0 + 1; k <= (int)k1; k++)
       for (int j = 1; j <= nyc; j++)
         for (int i = 1; i <= nxc; i++) ib[G.r(i, j, k)] = temp[G.r(i, j, k)];
-  });
+  }, kPlaneGrain);
   return 0;
 }
 
