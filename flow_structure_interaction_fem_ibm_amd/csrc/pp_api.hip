@@ -1196,39 +1196,26 @@ int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref
   return status;
 }
 
-double fluid_volume(const pp_ctx *c, int i, int j, int k) {  // 0-based interior cell
-  return c->rdx[0][i + 1] * c->rdx[1][j + 1] * c->rdx[2][k + 1];
-}
-
 // Left null vector of the ghost-cell operator: the transposed iteration
 // with s = 0 started from mu = 1 (the fluid volumes, exact without ghost
 // rows) converges to w; normalised so sum_f w_f = sum_f V_f.
 int compute_null_vector(pp_ctx *c) {
-  std::vector<double> w(c->nref, 0.0), mu(c->nref, 0.0);
-  const bool warm = (long)c->h_mu_prev.size() == c->nref;
-  std::vector<double> vpart(c->nzc, 0.0);
-  host_parallel(c->nzc, [&](long k0, long k1) {
-    for (int k = (int)k0; k < (int)k1; k++) {
-      double vs = 0.0;
-      for (int j = 0; j < c->nyc; j++)
-        for (int i = 0; i < c->nxc; i++) {
-          long r = ridx(c->ref, i, j, k);
-          if (!c->h_iblank[r]) {
-            w[r] = fluid_volume(c, i, j, k);
-            // start from the previous geometry's null vector where it exists
-            // (moving bodies), from the fluid indicator elsewhere
-            mu[r] = (warm && c->h_mu_prev[r] != 0.0) ? c->h_mu_prev[r] : 1.0;
-            vs += w[r];
-          }
-        }
-      vpart[k] = vs;
-    }
-  });
-  double vsum = 0.0;
-  for (int k = 0; k < c->nzc; k++) vsum += vpart[k];
+  hipStream_t s = c->stream;
+  LevelArgs<double> a0 = largs<double>(c->lv[0]);
+  CellLaunch cl = ref_launch(c);
+  const bool warm = c->d_mu_prev != nullptr;
+  // vsum = sum of fluid volumes
+  k_fluid_sum_vol<256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, nullptr, c->d_part);
+  int rc = reduce_sums_host(c, 2);
+  if (rc) return rc;
+  const double vsum = c->h_res[1];
+  HIPCHK(hipMemsetAsync(c->d_w, 0, sizeof(double) * c->nref, s));
   if (c->gh.n > 0) {
-    HIPCHK(hipMemcpy(c->d_mu, mu.data(), sizeof(double) * c->nref, hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(c->d_sv, 0, sizeof(double) * c->nref));
+    // mu: the previous geometry's null vector where it exists (moving
+    // bodies), the fluid indicator elsewhere; s = 0
+    HIPCHK(hipMemsetAsync(c->d_mu, 0, sizeof(double) * c->nref, s));
+    HIPCHK(hipMemsetAsync(c->d_sv, 0, sizeof(double) * c->nref, s));
+    k_null_init<<<cl.grid, cl.block, 0, s>>>(c->ref, a0, warm ? c->d_mu_prev : nullptr, c->d_mu);
     pp_solve_opts o;
     pp_default_opts(&o);
     o.precision = PP_PREC_MIXED;
@@ -1249,44 +1236,34 @@ int compute_null_vector(pp_ctx *c) {
     int lam = 0;
     // a warm start converges relative to the cold start's initial residual
     // (of the previous geometry), not to its own much smaller one
-    int rc = adjoint_impl<float>(c, nullptr, &o, warm ? c->null_ref : -1.0, &st, &lam);
+    rc = adjoint_impl<float>(c, nullptr, &o, warm ? c->null_ref : -1.0, &st, &lam);
     if (rc != PP_OK && rc != PP_NOT_CONVERGED) return rc;
     if (!(st.rel_l2 <= accept)) {
       set_error("left null vector did not converge (relative residual %g after %d iterations, %d ghost "
                 "stencil entries on solid cells)", st.rel_l2, st.outer_iterations, c->gh.solid_in_stencil);
       return PP_BAD_GEOMETRY;
     }
-    HIPCHK(hipMemcpy(mu.data(), c->d_mu, sizeof(double) * c->nref, hipMemcpyDeviceToHost));
-    c->h_mu_prev = mu;
+    if (!c->d_mu_prev) HIPCHK(hipMalloc(&c->d_mu_prev, sizeof(double) * c->nref));
+    HIPCHK(hipMemcpyAsync(c->d_mu_prev, c->d_mu, sizeof(double) * c->nref, hipMemcpyDeviceToDevice, s));
     if (!warm) c->null_ref = st.b_l2;
-    std::vector<double> lg(c->gh.n);
-    HIPCHK(hipMemcpy(lg.data(), c->gh.olam64[lam], sizeof(double) * c->gh.n, hipMemcpyDeviceToHost));
-    std::vector<double> wpart(c->nzc, 0.0);
-    host_parallel(c->nzc, [&](long k0, long k1) {
-      for (int k = (int)k0; k < (int)k1; k++) {
-        double ws = 0.0;
-        for (int j = 0; j < c->nyc; j++)
-          for (int i = 0; i < c->nxc; i++) {
-            long r = ridx(c->ref, i, j, k);
-            if (!c->h_iblank[r]) {
-              w[r] = fluid_volume(c, i, j, k) * mu[r];
-              ws += w[r];
-            }
-          }
-        wpart[k] = ws;
-      }
-    });
-    double wsum = 0.0;
-    for (int k = 0; k < c->nzc; k++) wsum += wpart[k];
-    double scale = vsum / wsum;
-    host_parallel(c->nref, [&](long a, long b) {
-      for (long r = a; r < b; r++) w[r] *= scale;
-    });
-    for (int q = 0; q < c->gh.n; q++) w[c->gh.h_g_ref[q]] = lg[q] * scale;
+    // w = V mu normalised to sum_f w = sum_f V; ghost entries lambda_g likewise
+    k_mu_vol_sums<256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, nullptr, c->d_part);
+    rc = reduce_sums_host(c, 2);
+    if (rc) return rc;
+    const double scale = vsum / c->h_res[0];
+    k_null_scale<<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, scale, c->d_w);
+    k_ghost_scatter_scaled<<<(c->gh.n + 255) / 256, 256, 0, s>>>(c->gh.n, c->gh.g_ref, c->gh.olam64[lam], scale,
+                                                                 c->d_w);
     c->null_iters = st.outer_iterations;
     c->null_res = st.rel_l2;
+  } else {
+    // no ghost rows: the operator is volume-symmetric, w = V
+    HIPCHK(hipMemsetAsync(c->d_mu, 0, sizeof(double) * c->nref, s));
+    k_null_init<<<cl.grid, cl.block, 0, s>>>(c->ref, a0, nullptr, c->d_mu);
+    k_null_scale<<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, 1.0, c->d_w);
   }
-  HIPCHK(hipMemcpy(c->d_w, w.data(), sizeof(double) * c->nref, hipMemcpyHostToDevice));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
   return PP_OK;
 }
 
@@ -1409,7 +1386,7 @@ int32_t pp_destroy(pp_ctx *c) {
   free_levels(c);
   free_ghosts(c);
   hipFree(c->d_p); hipFree(c->d_b); hipFree(c->d_w); hipFree(c->d_bwork); hipFree(c->d_tmp);
-  hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_mu); hipFree(c->d_sv);
+  hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_mu); hipFree(c->d_sv); hipFree(c->d_mu_prev);
   hipFree(c->d_part); hipFree(c->d_part2); hipFree(c->d_res);
   for (double *q : c->d_kry) hipFree(q);
   for (float *q : c->d_kryl) hipFree(q);
@@ -2011,74 +1988,76 @@ int32_t pp_divergence(pp_ctx *c, const double *fu, const double *fv, const doubl
   return rc;
 }
 
-int32_t pp_adjoint(pp_ctx *c, const double *wobj, double *lambda, double *dJ_dg, const pp_solve_opts *o,
-                   pp_solve_stats *st) {
-  if (!c || !wobj || !lambda) { set_error("pp_adjoint: null argument"); return PP_BAD_ARG; }
+int32_t pp_adjoint_device(pp_ctx *c, const double *d_wobj, double *d_lambda, double *d_dJ_dg,
+                          const pp_solve_opts *o, pp_solve_stats *st) {
+  if (!c || !d_wobj || !d_lambda) { set_error("pp_adjoint: null argument"); return PP_BAD_ARG; }
   if (!c->have_geom) { set_error("pp_adjoint: no geometry"); return PP_NO_GEOMETRY; }
   pp_solve_opts def;
   if (!o) { pp_default_opts(&def); o = &def; }
   if (!opts_ok(o)) { set_error("pp_adjoint: invalid options"); return PP_BAD_ARG; }
   HIPCHK(hipSetDevice(c->device));
   GhostTables &g = c->gh;
-  // J = wobj . M p with M the fluid-volume mean removal: s = M^T wobj
-  double S = 0.0, vsum = 0.0;
-  for (int k = 0; k < c->nzc; k++)
-    for (int j = 0; j < c->nyc; j++)
-      for (int i = 0; i < c->nxc; i++) {
-        long r = ridx(c->ref, i, j, k);
-        if (!c->h_iblank[r]) { S += wobj[r]; vsum += fluid_volume(c, i, j, k); }
-      }
-  std::vector<double> sg(g.n > 0 ? g.n : 1, 0.0);
-  for (int q = 0; q < g.n; q++) { sg[q] = wobj[g.h_g_ref[q]]; S += sg[q]; }
-  std::vector<double> sv(c->nref, 0.0), mu(c->nref, 0.0);
-  double ref = 0.0;
-  for (int k = 0; k < c->nzc; k++)
-    for (int j = 0; j < c->nyc; j++)
-      for (int i = 0; i < c->nxc; i++) {
-        long r = ridx(c->ref, i, j, k);
-        if (c->h_iblank[r]) continue;
-        double V = fluid_volume(c, i, j, k);
-        sv[r] = (wobj[r] - V * S / vsum) / V;
-        ref += sv[r] * sv[r];
-      }
-  ref = std::sqrt(ref);
-  HIPCHK(hipMemcpy(c->d_sv, sv.data(), sizeof(double) * c->nref, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_mu, mu.data(), sizeof(double) * c->nref, hipMemcpyHostToDevice));
-  double *dsg = nullptr;
+  hipStream_t s = c->stream;
+  LevelArgs<double> a0 = largs<double>(c->lv[0]);
+  CellLaunch cl = ref_launch(c);
+  // J = wobj . M p with M the fluid-volume mean removal: s = M^T wobj,
+  // S = sum of wobj over fluid and ghost cells, vsum = fluid volume
+  k_fluid_sum_vol<256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, d_wobj, c->d_part);
+  int rc = reduce_sums_host(c, 2);
+  if (rc) return rc;
+  double S = c->h_res[0];
+  const double vsum = c->h_res[1];
+  DevTemp dsg;
   if (g.n > 0) {
-    HIPCHK(hipMalloc(&dsg, sizeof(double) * g.n));
-    HIPCHK(hipMemcpy(dsg, sg.data(), sizeof(double) * g.n, hipMemcpyHostToDevice));
+    HIPCHK(dsg.alloc(sizeof(double) * g.n));
+    k_gather_idx<double><<<(g.n + 255) / 256, 256, 0, s>>>(g.n, g.g_ref, d_wobj, dsg.d());
+    std::vector<double> hs(g.n);
+    HIPCHK(hipMemcpyAsync(hs.data(), dsg.d(), sizeof(double) * g.n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int q = 0; q < g.n; q++) S += hs[q];
   }
+  HIPCHK(hipMemsetAsync(c->d_sv, 0, sizeof(double) * c->nref, s));
+  HIPCHK(hipMemsetAsync(c->d_mu, 0, sizeof(double) * c->nref, s));
+  k_adj_source<256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, d_wobj, S, vsum, c->d_sv, c->d_part);
+  rc = reduce_sums_host(c, 1);
+  if (rc) return rc;
+  const double ref = std::sqrt(c->h_res[0]);
   int lam = 0;
-  int rc = o->precision == PP_PREC_MIXED ? adjoint_impl<float>(c, dsg, o, ref > 0 ? ref : 1.0, st, &lam)
-                                         : adjoint_impl<double>(c, dsg, o, ref > 0 ? ref : 1.0, st, &lam);
-  if (dsg) hipFree(dsg);
+  rc = o->precision == PP_PREC_MIXED ? adjoint_impl<float>(c, dsg.d(), o, ref > 0 ? ref : 1.0, st, &lam)
+                                     : adjoint_impl<double>(c, dsg.d(), o, ref > 0 ? ref : 1.0, st, &lam);
   if (rc != PP_OK && rc != PP_NOT_CONVERGED) return rc;
-  HIPCHK(hipMemcpy(mu.data(), c->d_mu, sizeof(double) * c->nref, hipMemcpyDeviceToHost));
-  std::vector<double> lg(g.n > 0 ? g.n : 1, 0.0), w(c->nref);
-  if (g.n > 0) HIPCHK(hipMemcpy(lg.data(), g.olam64[lam], sizeof(double) * g.n, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(w.data(), c->d_w, sizeof(double) * c->nref, hipMemcpyDeviceToHost));
-  // dJ/db = Pi^T lambda with Pi the compatibility projection
-  double lsum = 0.0, wsum = 0.0;
-  for (int k = 0; k < c->nzc; k++)
-    for (int j = 0; j < c->nyc; j++)
-      for (int i = 0; i < c->nxc; i++) {
-        long r = ridx(c->ref, i, j, k);
-        if (c->h_iblank[r]) continue;
-        mu[r] *= fluid_volume(c, i, j, k);   // lambda_f
-        lsum += mu[r];
-        wsum += w[r];
-      }
-  double zeta = lsum / wsum;
-  for (long r = 0; r < c->nref; r++) lambda[r] = 0.0;
-  for (int k = 0; k < c->nzc; k++)
-    for (int j = 0; j < c->nyc; j++)
-      for (int i = 0; i < c->nxc; i++) {
-        long r = ridx(c->ref, i, j, k);
-        if (!c->h_iblank[r]) lambda[r] = mu[r] - zeta * w[r];
-      }
-  if (dJ_dg)
-    for (int q = 0; q < g.n; q++) dJ_dg[q] = g.h_bself[q] * (lg[q] - zeta * w[g.h_g_ref[q]]);
+  // dJ/db = Pi^T lambda with Pi the compatibility projection:
+  // lambda_f = mu V - zeta w, zeta = sum_f mu V / sum_f w
+  k_mu_vol_sums<256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, c->d_w, c->d_part);
+  int rc2 = reduce_sums_host(c, 2);
+  if (rc2) return rc2;
+  const double zeta = c->h_res[0] / c->h_res[1];
+  HIPCHK(hipMemsetAsync(d_lambda, 0, sizeof(double) * c->nref, s));
+  k_adj_lambda<<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, c->d_w, zeta, d_lambda);
+  if (d_dJ_dg && g.n > 0)
+    k_ghost_combine<<<(g.n + 255) / 256, 256, 0, s>>>(g.n, g.g_ref, c->d_w, -zeta, g.olam64[lam], 1.0, g.bself,
+                                                      d_dJ_dg);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  return rc;
+}
+
+int32_t pp_adjoint(pp_ctx *c, const double *wobj, double *lambda, double *dJ_dg, const pp_solve_opts *o,
+                   pp_solve_stats *st) {
+  if (!c || !wobj || !lambda) { set_error("pp_adjoint: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_adjoint: no geometry"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nb = sizeof(double) * c->nref;
+  DevTemp dw, dl, dg;
+  HIPCHK(dw.alloc(nb));
+  HIPCHK(dl.alloc(nb));
+  HIPCHK(hipMemcpy(dw.d(), wobj, nb, hipMemcpyHostToDevice));
+  const int ng = c->gh.n;
+  if (dJ_dg && ng > 0) HIPCHK(dg.alloc(sizeof(double) * ng));
+  int rc = pp_adjoint_device(c, dw.d(), dl.d(), dg.d(), o, st);
+  if (rc != PP_OK && rc != PP_NOT_CONVERGED) return rc;
+  HIPCHK(hipMemcpy(lambda, dl.d(), nb, hipMemcpyDeviceToHost));
+  if (dJ_dg && ng > 0) HIPCHK(hipMemcpy(dJ_dg, dg.d(), sizeof(double) * ng, hipMemcpyDeviceToHost));
   return rc;
 }
 

@@ -169,7 +169,7 @@ struct pp_ctx {
   int nblk_red = 0;
   bool adjoint_mode = false;
   double setup_ms[4] = {};             // last pp_set_geometry: flags, ghost tables, coarse operator, null vector
-  std::vector<double> h_mu_prev;       // last null vector in mu = w/V form (warm start of the next one)
+  double *d_mu_prev = nullptr;         // last null vector in mu = w/V form (warm start of the next one)
   double null_ref = 1.0;               // initial residual norm of the cold-started null vector
   int null_iters = 0;
   double null_res = 0.0;           // level-0 smoother uses the transposed ghost rows

@@ -563,6 +563,119 @@ __global__ void __launch_bounds__(NT) k_fluid_sums(RefArgs R, LevelArgs<double> 
   block_sums_n<NT, 5>(q, s, part, blk);
 }
 
+// ---------------------------------------------------------------------
+// Adjoint and left-null-vector set-up on the device (reference layout,
+// fluid cells only; V = cell volume, mu = lambda / V):
+//   k_fluid_sum_vol   partials {sum a, sum V} (a may be null)
+//   k_adj_source      sv = (wobj - (V S) / vsum) / V, partial sum sv^2
+//   k_mu_vol_sums     partials {sum mu V, sum w}
+//   k_adj_lambda      lambda = mu V - zeta w
+//   k_null_init       mu = mu_prev where nonzero (warm start), else 1
+//   k_null_scale      w = (V mu) scale
+// Blank cells are left untouched (the callers zero those fields first).
+// ---------------------------------------------------------------------
+template <int NT>
+__global__ void __launch_bounds__(NT) k_fluid_sum_vol(RefArgs R, LevelArgs<double> L, const double *__restrict__ a,
+                                                      double *__restrict__ part) {
+  __shared__ double s[NT / 64 * 2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  double q[2] = {0.0, 0.0};
+  if (i < R.nxc && j < R.nyc && !(L.fl[lidx(L.sx, L.sxy, i, j, k)] & FL_BLANK)) {
+    q[0] = a ? a[ridx(R, i, j, k)] : 0.0;
+    q[1] = L.wx[i] * L.wy[j] * L.wz[k];
+  }
+  const long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+  block_sums_n<NT, 2>(q, s, part, blk);
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_adj_source(RefArgs R, LevelArgs<double> L, const double *__restrict__ wobj,
+                                                   double S, double vsum, double *__restrict__ sv,
+                                                   double *__restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ double s[NT / 64];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  double q[1] = {0.0};
+  if (i < R.nxc && j < R.nyc && !(L.fl[lidx(L.sx, L.sxy, i, j, k)] & FL_BLANK)) {
+    const long r = ridx(R, i, j, k);
+    const double V = L.wx[i] * L.wy[j] * L.wz[k];
+    const double v = (wobj[r] - V * S / vsum) / V;
+    sv[r] = v;
+    q[0] = v * v;
+  }
+  const long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+  block_sums_n<NT, 1>(q, s, part, blk);
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_mu_vol_sums(RefArgs R, LevelArgs<double> L, const double *__restrict__ mu,
+                                                    const double *__restrict__ w, double *__restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ double s[NT / 64 * 2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  double q[2] = {0.0, 0.0};
+  if (i < R.nxc && j < R.nyc && !(L.fl[lidx(L.sx, L.sxy, i, j, k)] & FL_BLANK)) {
+    const long r = ridx(R, i, j, k);
+    q[0] = mu[r] * (L.wx[i] * L.wy[j] * L.wz[k]);
+    q[1] = w ? w[r] : 0.0;
+  }
+  const long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+  block_sums_n<NT, 2>(q, s, part, blk);
+}
+
+__global__ void __launch_bounds__(256) k_adj_lambda(RefArgs R, LevelArgs<double> L, const double *__restrict__ mu,
+                                                    const double *__restrict__ w, double zeta,
+                                                    double *__restrict__ lambda) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= R.nxc || j >= R.nyc || (L.fl[lidx(L.sx, L.sxy, i, j, k)] & FL_BLANK)) return;
+  const long r = ridx(R, i, j, k);
+  lambda[r] = mu[r] * (L.wx[i] * L.wy[j] * L.wz[k]) - zeta * w[r];
+}
+
+__global__ void __launch_bounds__(256) k_null_init(RefArgs R, LevelArgs<double> L, const double *__restrict__ prev,
+                                                   double *__restrict__ mu) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= R.nxc || j >= R.nyc || (L.fl[lidx(L.sx, L.sxy, i, j, k)] & FL_BLANK)) return;
+  const long r = ridx(R, i, j, k);
+  mu[r] = (prev && prev[r] != 0.0) ? prev[r] : 1.0;
+}
+
+__global__ void __launch_bounds__(256) k_null_scale(RefArgs R, LevelArgs<double> L, const double *__restrict__ mu,
+                                                    double scale, double *__restrict__ w) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= R.nxc || j >= R.nyc || (L.fl[lidx(L.sx, L.sxy, i, j, k)] & FL_BLANK)) return;
+  const long r = ridx(R, i, j, k);
+  w[r] = (L.wx[i] * L.wy[j] * L.wz[k] * mu[r]) * scale;
+}
+
+// dst[ref[q]] = vals[q] * scale
+__global__ void __launch_bounds__(256) k_ghost_scatter_scaled(int n, const long *__restrict__ ref,
+                                                              const double *__restrict__ vals, double scale,
+                                                              double *__restrict__ dst) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q < n) dst[ref[q]] = vals[q] * scale;
+}
+
+// per ghost: out[q] = a * src[ref[q]] (+ b * add[q] when add is given)
+__global__ void __launch_bounds__(256) k_ghost_combine(int n, const long *__restrict__ ref, const double *__restrict__ src,
+                                                       double a, const double *__restrict__ add, double b,
+                                                       const double *__restrict__ mul, double *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= n) return;
+  double v = a * src[ref[q]];
+  if (add) v = b * add[q] + v;
+  out[q] = mul ? mul[q] * v : v;
+}
+
 // x[fluid] += alpha (alpha read from device memory: scale * src[0] (+ scale2*src[1]))
 __global__ void __launch_bounds__(256) k_fluid_shift(RefArgs R, LevelArgs<double> L, double *__restrict__ x,
                                                      const double *__restrict__ num, const double *__restrict__ den,

@@ -71,7 +71,7 @@ module pp_hip_mod
   end type pp_solve_stats
 
   public :: pp_default_opts, pp_last_error, pp_create, pp_destroy, pp_set_geometry, pp_solve, &
-       pp_solve_device, pp_adjoint, pp_calc_residual, pp_ghost_rows, pp_mean_remove, pp_divergence, &
+       pp_solve_device, pp_adjoint, pp_adjoint_device, pp_calc_residual, pp_ghost_rows, pp_mean_remove, pp_divergence, &
        pp_smooth, pp_null_vector, pp_level_info, pp_error_text, pp_geom_build, pp_geom_counts, pp_geom_get, &
        pp_geom_ghost_elements, pp_geom_destroy, pp_geom_dirichlet, pp_set_vel_ghosts, pp_correct_vel, &
        pp_correct_vel_device, pp_update_pressure, pp_update_pressure_device
@@ -130,6 +130,14 @@ module pp_hip_mod
        type(pp_solve_stats), intent(out) :: st
        integer(c_int32_t) :: rc
      end function pp_adjoint
+
+     function pp_adjoint_device(ctx, w, lambda, dj_dg, o, st) bind(c, name="pp_adjoint_device") result(rc)
+       import :: c_ptr, c_int32_t, pp_solve_opts, pp_solve_stats
+       type(c_ptr), value :: ctx, w, lambda, dj_dg
+       type(pp_solve_opts), intent(in) :: o
+       type(pp_solve_stats), intent(out) :: st
+       integer(c_int32_t) :: rc
+     end function pp_adjoint_device
 
      function pp_calc_residual(ctx, var, r, res, resm, loc, l2) bind(c, name="pp_calc_residual") result(rc)
        import :: c_ptr, c_int32_t, c_double

@@ -129,6 +129,7 @@ def lib():
     L.pp_solve.argtypes = [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     L.pp_solve_device.argtypes = [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     L.pp_adjoint.argtypes = [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
+    L.pp_adjoint_device.argtypes = [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     L.pp_calc_residual.argtypes = [P, P, P, P, dp, C.POINTER(C.c_int32), dp]
     L.pp_ghost_rows.argtypes = [P, P, P]
     L.pp_mean_remove.argtypes = [P, P, dp]
@@ -156,7 +157,7 @@ def lib():
                  "pp_divergence", "pp_smooth", "pp_tdma_batched", "pp_null_vector", "pp_level_info", "pp_time_kernel",
                  "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get", "pp_geom_ghost_elements",
                  "pp_geom_destroy", "pp_geom_dirichlet", "pp_set_vel_ghosts", "pp_correct_vel",
-                 "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device"):
+                 "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device", "pp_adjoint_device"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -167,7 +168,8 @@ EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_s
             "pp_mean_remove", "pp_divergence", "pp_smooth", "pp_tdma_batched", "pp_null_vector",
             "pp_level_info", "pp_time_kernel", "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get",
             "pp_geom_ghost_elements", "pp_geom_destroy", "pp_geom_dirichlet", "pp_set_vel_ghosts",
-            "pp_correct_vel", "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device")
+            "pp_correct_vel", "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device",
+            "pp_adjoint_device")
 
 
 def _check(rc, allow=(PP_OK,)):
@@ -295,6 +297,16 @@ class PressureSolver:
         if bc_grad:
             return lam, dg[: self.ghosts.n], st.as_dict()
         return lam, st.as_dict()
+
+    def adjoint_device(self, w, lam, dj_dg=None, **kw):
+        """pp_adjoint_device on torch float64 tensors: lam (reference layout) and
+        dj_dg (n_ghost, optional) are written in place; returns the stats."""
+        o = default_opts(**kw)
+        st = SolveStats()
+        rc = lib().pp_adjoint_device(self.h, w.data_ptr(), lam.data_ptr(),
+                                     None if dj_dg is None else dj_dg.data_ptr(), C.byref(o), C.byref(st))
+        _check(rc, (PP_OK, PP_NOT_CONVERGED))
+        return st.as_dict()
 
     def calc_residual(self, var, r):
         var = np.ascontiguousarray(var, dtype=np.float64)

@@ -180,6 +180,11 @@ int32_t pp_solve_device(pp_ctx *ctx, double *d_p, const double *d_rhs,
  * No counterpart in the reference (it has no adjoint). */
 int32_t pp_adjoint(pp_ctx *ctx, const double *w, double *lambda,
                    double *dJ_dg, const pp_solve_opts *o, pp_solve_stats *st);
+/* pp_adjoint on device arrays (wobj, lambda in the reference layout; dJ_dg
+ * n_ghost values or NULL); nothing is copied to or from the host. */
+int32_t pp_adjoint_device(pp_ctx *ctx, const double *d_wobj, double *d_lambda,
+                          double *d_dJ_dg, const pp_solve_opts *opts,
+                          pp_solve_stats *stats);
 
 /* ---- single routines of the reference path (parity checks, callers that
  *      keep their own loop) ------------------------------------------- */

@@ -220,3 +220,22 @@ def test_adjoint_matches_sparse_lu_gradient(prec):
     sf = np.array([sens[pos[c]] for c in fl_flat])
     grad[fl_flat] = sf - wf * (sf.sum() / wf.sum())
     assert np.linalg.norm(lam.ravel() - grad) / np.linalg.norm(grad) < 1e-9
+
+
+def test_adjoint_device_entry_matches_host_entry():
+    """pp_adjoint_device on device tensors gives the host entry's lambda and
+    dJ/dg bit for bit (the host entry copies around the same device path)."""
+    import torch
+    announce("adjoint device entry 24x20x22 stretched sphere")
+    s, xn, yn, zn, ib, gs, A, unknown, V = setup(24, ny=20, nz=22)
+    w = cases.random_field(ib.shape, 71, ib)
+    lam, dg, st = s.adjoint(w, bc_grad=True, rel_tol=1e-10, max_outer=100)
+    dev = torch.device("cuda:0")
+    tw = torch.from_numpy(w.copy()).to(dev)
+    tl = torch.zeros(ib.shape, dtype=torch.float64, device=dev)
+    tg = torch.zeros(len(gs.ig), dtype=torch.float64, device=dev)
+    st2 = s.adjoint_device(tw, tl, tg, rel_tol=1e-10, max_outer=100)
+    torch.cuda.synchronize()
+    assert st2["outer_iterations"] == st["outer_iterations"]
+    assert np.array_equal(tl.cpu().numpy(), lam)
+    assert np.array_equal(tg.cpu().numpy(), dg)
