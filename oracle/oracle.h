@@ -169,6 +169,27 @@ void or_correct_vel(const or_ctx *c, double dt, const double *p, double *u, doub
 void or_update_pressure(const or_ctx *c, int van_kan, const double *pprime, double *p, const double *g, int itermax,
                         double restol);
 
+/* ---- implicit momentum solve (oracle/or_ad.c) ------------------------ */
+/* set_solve_ad (CORE_SET_SOLVERS.F90:20-223) without fresh cells: coef =
+ * amx_ad, apx_ad, amy_ad, apy_ad, amz_ad, apz_ad (6 fields).  visc is
+ * viscTot; bxv, byv, bzv the boundary-face viscosities bcx/y/zvisc. */
+void or_set_solve_ad(const or_ctx *c, double dt, const double *visc, const double *bxv, const double *byv,
+                     const double *bzv, double *coef);
+/* calc_residual_ad (CORE_SOLVE_AD.F90:766-907): returns max |res| over
+ * the cells, loc = its 1-based (i,j,k), res (may be NULL) the field.
+ * half_dt = dt/2 for CRANK_NICOLSON1, 0 for ADAMS_BASHFORTH2. */
+double or_calc_residual_ad(const or_ctx *c, double half_dt, const double *coef, const double *fu, const double *fv,
+                           const double *fw, const double *var, const double *r, double *res, int32_t *loc);
+/* itsolv_ad (CORE_SOLVE_AD.F90:272-762): x, y and z line sweeps in the
+ * reference's loop order, Thomas algorithm per line, in place */
+void or_itsolv_ad(const or_ctx *c, double half_dt, const double *coef, const double *fu, const double *fv,
+                  const double *fw, double *var, const double *r);
+/* solve_ad, GCM branch (CORE_SOLVE_AD.F90:200-267): returns iterations,
+ * *maxres the last maximum residual of (u, v, w) */
+int or_solve_ad(const or_ctx *c, double half_dt, const double *coef, const double *fu, const double *fv,
+                const double *fw, double *u, double *v, double *w, const double *nlu, const double *nlv,
+                const double *nlw, const double *vbi, int itermax, double restol, double *maxres);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,12 @@ def lib():
         L.or_vel_ghosts.argtypes = [P] * 5 + [C.c_int, C.c_double]
         L.or_correct_vel.argtypes = [P, C.c_double] + [P] * 11 + [C.c_int, C.c_double]
         L.or_update_pressure.argtypes = [P, C.c_int, P, P, P, C.c_int, C.c_double]
+        L.or_set_solve_ad.argtypes = [P, C.c_double] + [P] * 5
+        L.or_calc_residual_ad.restype = C.c_double
+        L.or_calc_residual_ad.argtypes = [P, C.c_double] + [P] * 8
+        L.or_itsolv_ad.argtypes = [P, C.c_double] + [P] * 6
+        L.or_solve_ad.restype = C.c_int
+        L.or_solve_ad.argtypes = [P, C.c_double] + [P] * 11 + [C.c_int, C.c_double, P]
         _L = L
     return _L
 
@@ -196,6 +202,53 @@ class Oracle:
         lib().or_update_pressure(self.h, int(van_kan), pp_.ctypes.data, out.ctypes.data,
                                  None if gg is None else gg.ctypes.data, itermax, restol)
         return out
+
+    # ---- implicit momentum solve (or_ad.c) ----
+    def set_solve_ad(self, dt, visc, bxv, byv, bzv):
+        """amx_ad, apx_ad, amy_ad, apy_ad, amz_ad, apz_ad as a (6, ...) array"""
+        ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (visc, bxv, byv, bzv)]
+        coef = np.zeros((6,) + self.shape)
+        lib().or_set_solve_ad(self.h, float(dt), *[a.ctypes.data for a in ins], coef.ctypes.data)
+        return coef
+
+    @staticmethod
+    def _faces(faces):
+        return [np.ascontiguousarray(a, dtype=np.float64) for a in faces]
+
+    def calc_residual_ad(self, half_dt, coef, faces, var, r):
+        """(max |res|, 1-based loc, res field)"""
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        f = self._faces(faces)
+        var = np.ascontiguousarray(var, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        res = np.zeros(self.shape)
+        loc = np.zeros(3, np.int32)
+        m = lib().or_calc_residual_ad(self.h, float(half_dt), coef.ctypes.data, *[a.ctypes.data for a in f],
+                                      var.ctypes.data, r.ctypes.data, res.ctypes.data, loc.ctypes.data)
+        return m, tuple(int(v) for v in loc), res
+
+    def itsolv_ad(self, half_dt, coef, faces, var, r):
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        f = self._faces(faces)
+        out = np.array(var, dtype=np.float64, order="C", copy=True)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        lib().or_itsolv_ad(self.h, float(half_dt), coef.ctypes.data, *[a.ctypes.data for a in f], out.ctypes.data,
+                           r.ctypes.data)
+        return out
+
+    def solve_ad(self, half_dt, coef, faces, u, v, w, nlu, nlv, nlw, vbi=None, itermax=1000, restol=1e-12):
+        """GCM loop of solve_ad on copies; returns (u, v, w, iterations, maxres)"""
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        f = self._faces(faces)
+        out = [np.array(a, dtype=np.float64, order="C", copy=True) for a in (u, v, w)]
+        rs = [np.ascontiguousarray(a, dtype=np.float64) for a in (nlu, nlv, nlw)]
+        vb = None if vbi is None else np.ascontiguousarray(vbi, dtype=np.float64)
+        mr = C.c_double()
+        it = lib().or_solve_ad(self.h, float(half_dt), coef.ctypes.data, *[a.ctypes.data for a in f],
+                               *[a.ctypes.data for a in out], *[a.ctypes.data for a in rs],
+                               None if vb is None else vb.ctypes.data, int(itermax), float(restol),
+                               C.addressof(mr))
+        return out[0], out[1], out[2], it, mr.value
 
     def set_fresh(self, fr):
         fr = np.ascontiguousarray(fr, dtype=np.int8)
