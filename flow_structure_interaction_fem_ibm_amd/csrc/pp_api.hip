@@ -1387,6 +1387,7 @@ int32_t pp_destroy(pp_ctx *c) {
   free_ghosts(c);
   hipFree(c->d_p); hipFree(c->d_b); hipFree(c->d_w); hipFree(c->d_bwork); hipFree(c->d_tmp);
   hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_mu); hipFree(c->d_sv); hipFree(c->d_mu_prev);
+  hipFree(c->d_ad_coef);
   hipFree(c->d_part); hipFree(c->d_part2); hipFree(c->d_res);
   for (double *q : c->d_kry) hipFree(q);
   for (float *q : c->d_kryl) hipFree(q);
@@ -1410,6 +1411,7 @@ int32_t pp_set_geometry(pp_ctx *c, const pp_geometry *geo) {
   }
   HIPCHK(hipSetDevice(c->device));
   c->have_geom = false;
+  c->have_ad = false;  // set_solve_ad follows every set_boundary
   if (c->gexec) {  // captured launches point at the old hierarchy
     HIPCHK(hipGraphExecDestroy(c->gexec));
     c->gexec = nullptr;
@@ -1668,6 +1670,29 @@ int32_t pp_set_vel_ghosts(pp_ctx *c, const double *coeffD, double bi_weight, dou
   return PP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// GCM_vel_set_bc_internal on device fields: gather the ghost cells'
+// (u, v, w), the Jacobi sweeps fixed by pp_set_vel_ghosts, scatter
+void vel_ghosts(pp_ctx *c, double *d_u, double *d_v, double *d_w, const double *d_vbi) {
+  GhostTables &g = c->gh;
+  if (g.n == 0) return;
+  hipStream_t s = c->stream;
+  dim3 gg((g.n + 255) / 256), blk(256);
+  k_vel_ghost_gather<double><<<gg, blk, 0, s>>>(g.n, g.g_ref, d_u, d_v, d_w, g.v_buf[0]);
+  int cur = 0;
+  for (int q = 0; q < g.v_sweeps; q++) {
+    k_vel_ghost_sweep<<<gg, blk, 0, s>>>(g.n, g.v_cD, g.v_kind, g.v_off, g.v_bw, g.v_iw, d_vbi, d_u, d_v, d_w,
+                                         g.v_buf[cur], g.v_buf[1 - cur]);
+    cur ^= 1;
+  }
+  k_vel_ghost_scatter<<<gg, blk, 0, s>>>(g.n, g.g_ref, g.v_buf[cur], d_u, d_v, d_w);
+}
+}  // namespace
+
+extern "C" {
+
 int32_t pp_correct_vel_device(pp_ctx *c, double dt, const double *d_p, double *d_u, double *d_v, double *d_w,
                               double *d_fu, double *d_fv, double *d_fw, const double *d_bcxu, const double *d_bcyv,
                               const double *d_bczw, const double *d_vbi) {
@@ -1689,17 +1714,7 @@ int32_t pp_correct_vel_device(pp_ctx *c, double dt, const double *d_p, double *d
   CellLaunch cl = ref_launch(c);
   k_correct_vel<256><<<cl.grid, cl.block, 0, s>>>(c->ref, largs<double>(c->lv[0]), M, dt, d_p, d_u, d_v, d_w, d_fu,
                                                    d_fv, d_fw, d_bcxu, d_bcyv, d_bczw);
-  if (g.n > 0) {
-    dim3 gg((g.n + 255) / 256), blk(256);
-    k_vel_ghost_gather<double><<<gg, blk, 0, s>>>(g.n, g.g_ref, d_u, d_v, d_w, g.v_buf[0]);
-    int cur = 0;
-    for (int q = 0; q < g.v_sweeps; q++) {
-      k_vel_ghost_sweep<<<gg, blk, 0, s>>>(g.n, g.v_cD, g.v_kind, g.v_off, g.v_bw, g.v_iw, d_vbi, d_u, d_v, d_w,
-                                           g.v_buf[cur], g.v_buf[1 - cur]);
-      cur ^= 1;
-    }
-    k_vel_ghost_scatter<<<gg, blk, 0, s>>>(g.n, g.g_ref, g.v_buf[cur], d_u, d_v, d_w);
-  }
+  vel_ghosts(c, d_u, d_v, d_w, d_vbi);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
   return PP_OK;
@@ -2059,6 +2074,281 @@ int32_t pp_adjoint(pp_ctx *c, const double *wobj, double *lambda, double *dJ_dg,
   HIPCHK(hipMemcpy(lambda, dl.d(), nb, hipMemcpyDeviceToHost));
   if (dJ_dg && ng > 0) HIPCHK(hipMemcpy(dJ_dg, dg.d(), sizeof(double) * ng, hipMemcpyDeviceToHost));
   return rc;
+}
+
+}  // extern "C"
+
+// =====================================================================
+// implicit momentum solve (set_solve_ad, solve_ad, calc_residual_ad)
+// =====================================================================
+namespace {
+constexpr int kAdLS = 4;  // cells per thread of a line segment
+
+FracMetrics ref_metrics(const pp_ctx *c) {
+  FracMetrics M;
+  for (int d = 0; d < 3; d++) { M.fx[d] = c->d_rfx[d]; M.dinv[d] = c->d_rdxinv[d]; M.dcinv[d] = c->d_rdxcinv[d]; }
+  return M;
+}
+
+struct AdLineLaunch {
+  dim3 grid, block;
+  size_t lds;
+  int nlines;
+};
+// TX lines x P segments per block, TX a power of two up to 32 with at most
+// 512 threads (one segment per thread: P = ceil(n / kAdLS) <= 128)
+AdLineLaunch ad_line_launch(const pp_ctx *c, int dir) {
+  const int n = dir == 0 ? c->nxc : (dir == 1 ? c->nyc : c->nzc);
+  const int P = (n + kAdLS - 1) / kAdLS;
+  int tx = 1;
+  while (tx < 32 && 2 * tx * P <= 512) tx *= 2;
+  AdLineLaunch L;
+  L.block = dim3(tx, P);
+  L.lds = sizeof(double) * 8 * (size_t)P * tx;
+  if (dir == 0) {
+    L.nlines = ((c->nyc + 1) / 2) * c->nzc;
+    L.grid = dim3((L.nlines + tx - 1) / tx);
+  } else {
+    L.nlines = 0;
+    L.grid = dim3((c->nxc + 2 * tx - 1) / (2 * tx), dir == 1 ? c->nzc : c->nyc);
+  }
+  return L;
+}
+
+int ad_half_dt(int32_t scheme, double dt, double *half_dt, const char *who) {
+  if (scheme == PP_ADAMS_BASHFORTH2) { *half_dt = 0.0; return PP_OK; }
+  if (scheme == PP_CRANK_NICOLSON1) { *half_dt = 0.5 * dt; return PP_OK; }
+  if (scheme == PP_CRANK_NICOLSON2)
+    set_error("%s: CRANK_NICOLSON2 recomputes the face velocities inside the loop (face_vel); not supported", who);
+  else
+    set_error("%s: unknown advec_scheme %d", who, (int)scheme);
+  return PP_BAD_ARG;
+}
+
+// the rows of k_ad_rows (7 fields) for the current coefficients
+void ad_build_rows(pp_ctx *c, double half_dt, const double *fu, const double *fv, const double *fw, double *row) {
+  CellLaunch cl = ref_launch(c);
+  k_ad_rows<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), ref_metrics(c), half_dt,
+                                                      c->d_ad_coef, fu, fv, fw, row, c->nref);
+}
+
+// itsolv_ad: x, y and z line sweeps, two zebra passes each
+void ad_sweeps(pp_ctx *c, const double *row, const double *r, double *var) {
+  LevelArgs<double> L0 = largs<double>(c->lv[0]);
+  for (int dir = 0; dir < 3; dir++) {
+    AdLineLaunch L = ad_line_launch(c, dir);
+    for (int color = 0; color < 2; color++) {
+      if (dir == 0)
+        k_ad_line<0, kAdLS><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, r, var, color, L.nlines);
+      else if (dir == 1)
+        k_ad_line<1, kAdLS><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, r, var, color, L.nlines);
+      else
+        k_ad_line<2, kAdLS><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, r, var, color, L.nlines);
+    }
+  }
+}
+
+// calc_residual_ad of nv fields: h_res[3q + 1] = max |res|, h_res[3q + 2]
+// = its reference-layout offset (first in k, j, i order), rout the field
+// of the first when not NULL
+int ad_residual(pp_ctx *c, const double *row, int nv, AdFields F, double *rout) {
+  CellLaunch cl = ref_launch(c);
+  const long nblk = c->nblk_red;
+  LevelArgs<double> L0 = largs<double>(c->lv[0]);
+  hipStream_t s = c->stream;
+  if (nv == 3)
+    k_ad_residual<256, 3><<<cl.grid, cl.block, 0, s>>>(c->ref, L0, row, c->nref, F, c->d_part, nblk, rout);
+  else
+    k_ad_residual<256, 1><<<cl.grid, cl.block, 0, s>>>(c->ref, L0, row, c->nref, F, c->d_part, nblk, rout);
+  const int chunk = (int)((nblk + kRedStage - 1) / kRedStage);
+  for (int q = 0; q < nv; q++) {
+    k_reduce_summax_stage<<<kRedStage, 256, 0, s>>>(c->d_part + 3 * nblk * q, (int)nblk, chunk,
+                                                   c->d_part2 + 3 * kRedStage * q);
+    k_reduce_summax<<<1, 256, 0, s>>>(c->d_part2 + 3 * kRedStage * q, kRedStage, c->d_res + 3 * q);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->h_res, c->d_res, 3 * nv * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return PP_OK;
+}
+
+void ad_loc(const pp_ctx *c, double mx, double arg, int32_t loc[3]) {
+  if (mx > 0.0 && arg >= 0.0) {
+    const long q = (long)arg;
+    loc[0] = (int32_t)(q % c->ref.ex);
+    loc[1] = (int32_t)((q / c->ref.ex) % (c->nyc + 3));
+    loc[2] = (int32_t)(q / c->ref.exy);
+  } else {
+    loc[0] = loc[1] = loc[2] = 0;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int32_t pp_set_solve_ad_device(pp_ctx *c, double dt, const double *d_visc, const double *d_bxv, const double *d_byv,
+                               const double *d_bzv) {
+  if (!c || !d_visc || !d_bxv || !d_byv || !d_bzv) { set_error("pp_set_solve_ad: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_set_solve_ad: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  if (!std::isfinite(dt)) { set_error("pp_set_solve_ad: dt must be finite"); return PP_BAD_ARG; }
+  HIPCHK(hipSetDevice(c->device));
+  c->have_ad = false;
+  const size_t nb = sizeof(double) * 6 * (size_t)c->nref;
+  if (!c->d_ad_coef) HIPCHK(hipMalloc(&c->d_ad_coef, nb));
+  HIPCHK(hipMemsetAsync(c->d_ad_coef, 0, nb, c->stream));
+  CellLaunch cl = ref_launch(c);
+  k_set_solve_ad<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), ref_metrics(c), dt,
+                                                           d_visc, d_bxv, d_byv, d_bzv, c->d_ad_coef, c->nref);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->ad_dt = dt;
+  c->have_ad = true;
+  return PP_OK;
+}
+
+int32_t pp_set_solve_ad(pp_ctx *c, double dt, const double *visc, const double *bxv, const double *byv,
+                        const double *bzv) {
+  if (!c || !visc || !bxv || !byv || !bzv) { set_error("pp_set_solve_ad: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_set_solve_ad: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nb = sizeof(double) * c->nref;
+  DevTemp d;
+  HIPCHK(d.alloc(4 * nb));
+  const double *hin[4] = {visc, bxv, byv, bzv};
+  for (int q = 0; q < 4; q++) HIPCHK(hipMemcpy(d.d() + q * c->nref, hin[q], nb, hipMemcpyHostToDevice));
+  return pp_set_solve_ad_device(c, dt, d.d(), d.d() + c->nref, d.d() + 2 * c->nref, d.d() + 3 * c->nref);
+}
+
+int32_t pp_solve_ad_device(pp_ctx *c, int32_t advec_scheme, double *d_u, double *d_v, double *d_w,
+                           const double *d_nlu, const double *d_nlv, const double *d_nlw, const double *d_fu,
+                           const double *d_fv, const double *d_fw, const double *d_vbi, int32_t itermax,
+                           double restol, pp_ad_stats *st) {
+  if (!c || !d_u || !d_v || !d_w || !d_nlu || !d_nlv || !d_nlw || !d_fu || !d_fv || !d_fw) {
+    set_error("pp_solve_ad: null argument");
+    return PP_BAD_ARG;
+  }
+  if (!c->have_geom) { set_error("pp_solve_ad: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  if (!c->have_ad) {
+    set_error("pp_solve_ad: no momentum coefficients (call pp_set_solve_ad after pp_set_geometry)");
+    return PP_NO_GEOMETRY;
+  }
+  if (c->gh.n > 0 && !c->gh.have_vel) {
+    set_error("pp_solve_ad: no velocity ghost coefficients (call pp_set_vel_ghosts after pp_set_geometry)");
+    return PP_NO_GEOMETRY;
+  }
+  if (itermax < 0 || !std::isfinite(restol)) { set_error("pp_solve_ad: bad itermax_ad or restol_ad"); return PP_BAD_ARG; }
+  double half_dt = 0.0;
+  int rc = ad_half_dt(advec_scheme, c->ad_dt, &half_dt, "pp_solve_ad");
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  DevTemp rows;
+  HIPCHK(rows.alloc(sizeof(double) * 7 * (size_t)c->nref));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s));
+  ad_build_rows(c, half_dt, d_fu, d_fv, d_fw, rows.d());
+  double *vars[3] = {d_u, d_v, d_w};
+  const double *rhs[3] = {d_nlu, d_nlv, d_nlw};
+  AdFields F;
+  for (int q = 0; q < 3; q++) { F.var[q] = vars[q]; F.r[q] = rhs[q]; }
+  int it = 0;
+  double maxres = 1.0e10, ruvw[3] = {0.0, 0.0, 0.0};
+  int32_t loc[3] = {0, 0, 0};
+  while (it < itermax && maxres > restol) {
+    for (int q = 0; q < 3; q++) ad_sweeps(c, rows.d(), rhs[q], vars[q]);
+    vel_ghosts(c, d_u, d_v, d_w, d_vbi);
+    rc = ad_residual(c, rows.d(), 3, F, nullptr);
+    if (rc) break;
+    int qm = 0;
+    for (int q = 0; q < 3; q++) {
+      ruvw[q] = c->h_res[3 * q + 1];
+      if (ruvw[q] > ruvw[qm]) qm = q;
+    }
+    maxres = ruvw[qm];
+    ad_loc(c, c->h_res[3 * qm + 1], c->h_res[3 * qm + 2], loc);
+    it++;
+  }
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  const int32_t status = maxres > restol ? PP_NOT_CONVERGED : PP_OK;
+  if (st) {
+    st->status = status;
+    st->iterations = it;
+    st->max_residual = maxres;
+    for (int q = 0; q < 3; q++) { st->residual_uvw[q] = ruvw[q]; st->loc[q] = loc[q]; }
+    st->ms = ms;
+  }
+  if (status != PP_OK) set_error("pp_solve_ad: residual %g after %d iterations", maxres, it);
+  return status;
+}
+
+int32_t pp_solve_ad(pp_ctx *c, int32_t advec_scheme, double *u, double *v, double *w, const double *nlu,
+                    const double *nlv, const double *nlw, const double *fu, const double *fv, const double *fw,
+                    const double *vbi, int32_t itermax, double restol, pp_ad_stats *st) {
+  if (!c || !u || !v || !w || !nlu || !nlv || !nlw || !fu || !fv || !fw) {
+    set_error("pp_solve_ad: null argument");
+    return PP_BAD_ARG;
+  }
+  if (!c->have_geom) { set_error("pp_solve_ad: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nb = sizeof(double) * c->nref;
+  DevTemp d, dvbi;
+  HIPCHK(d.alloc(9 * nb));
+  double *dv[9];
+  for (int q = 0; q < 9; q++) dv[q] = d.d() + q * c->nref;
+  const double *hin[9] = {u, v, w, nlu, nlv, nlw, fu, fv, fw};
+  for (int q = 0; q < 9; q++) HIPCHK(hipMemcpy(dv[q], hin[q], nb, hipMemcpyHostToDevice));
+  if (vbi && c->gh.n > 0) {
+    HIPCHK(dvbi.alloc(sizeof(double) * 3 * c->gh.n));
+    HIPCHK(hipMemcpy(dvbi.d(), vbi, sizeof(double) * 3 * c->gh.n, hipMemcpyHostToDevice));
+  }
+  int rc = pp_solve_ad_device(c, advec_scheme, dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7], dv[8],
+                              dvbi.d(), itermax, restol, st);
+  if (rc == PP_OK || rc == PP_NOT_CONVERGED) {
+    double *hout[3] = {u, v, w};
+    for (int q = 0; q < 3; q++) HIPCHK(hipMemcpy(hout[q], dv[q], nb, hipMemcpyDeviceToHost));
+  }
+  return rc;
+}
+
+int32_t pp_calc_residual_ad(pp_ctx *c, int32_t advec_scheme, const double *var, const double *r, const double *fu,
+                            const double *fv, const double *fw, double *res, double *resm, int32_t loc[3]) {
+  if (!c || !var || !r || !fu || !fv || !fw || !resm || !loc) {
+    set_error("pp_calc_residual_ad: null argument");
+    return PP_BAD_ARG;
+  }
+  if (!c->have_geom) { set_error("pp_calc_residual_ad: no geometry"); return PP_NO_GEOMETRY; }
+  if (!c->have_ad) { set_error("pp_calc_residual_ad: no momentum coefficients (call pp_set_solve_ad)"); return PP_NO_GEOMETRY; }
+  double half_dt = 0.0;
+  int rc = ad_half_dt(advec_scheme, c->ad_dt, &half_dt, "pp_calc_residual_ad");
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nb = sizeof(double) * c->nref;
+  DevTemp d, rows;
+  HIPCHK(d.alloc(6 * nb));
+  HIPCHK(rows.alloc(7 * nb));
+  double *dv[6];
+  for (int q = 0; q < 6; q++) dv[q] = d.d() + q * c->nref;
+  const double *hin[5] = {var, r, fu, fv, fw};
+  for (int q = 0; q < 5; q++) HIPCHK(hipMemcpy(dv[q], hin[q], nb, hipMemcpyHostToDevice));
+  HIPCHK(hipMemsetAsync(dv[5], 0, nb, c->stream));
+  ad_build_rows(c, half_dt, dv[2], dv[3], dv[4], rows.d());
+  AdFields F{};
+  F.var[0] = dv[0];
+  F.r[0] = dv[1];
+  rc = ad_residual(c, rows.d(), 1, F, dv[5]);
+  if (rc) return rc;
+  *resm = c->h_res[1];
+  ad_loc(c, c->h_res[1], c->h_res[2], loc);
+  if (res) HIPCHK(hipMemcpy(res, dv[5], nb, hipMemcpyDeviceToHost));
+  return PP_OK;
 }
 
 }  // extern "C"

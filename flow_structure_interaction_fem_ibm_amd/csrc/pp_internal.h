@@ -170,6 +170,11 @@ struct pp_ctx {
   bool adjoint_mode = false;
   double setup_ms[4] = {};             // last pp_set_geometry: flags, ghost tables, coarse operator, null vector
   double *d_mu_prev = nullptr;         // last null vector in mu = w/V form (warm start of the next one)
+  // implicit momentum solve: amx_ad..apz_ad (6 fields, reference layout)
+  // of the last pp_set_solve_ad, and its dt
+  double *d_ad_coef = nullptr;
+  double ad_dt = 0.0;
+  bool have_ad = false;
   double null_ref = 1.0;               // initial residual norm of the cold-started null vector
   int null_iters = 0;
   double null_res = 0.0;           // level-0 smoother uses the transposed ghost rows

@@ -1988,4 +1988,238 @@ __global__ void __launch_bounds__(1024) k_tdma_seg(int n, int nlines, const doub
   }
 }
 
+
+// =====================================================================
+// Implicit momentum (advection-diffusion) solve of the fractional step,
+// FP64 on the reference layout: set_solve_ad, the line sweeps
+// itsolv_ad_x/y/z and calc_residual_ad (CORE_SET_SOLVERS.F90:20-223,
+// CORE_SOLVE_AD.F90:314-907), without fresh cells or periodic faces.
+// The level-0 flags give iblank (FL_BLANK), ghostCellMark (FL_GHOST) and
+// ium..kup (FL_IM..FL_KP).
+// =====================================================================
+
+// set_solve_ad: coef = amx_ad, apx_ad, amy_ad, apy_ad, amz_ad, apz_ad
+// (nref apart) on the interior cells; the caller clears the outer layer.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_set_solve_ad(RefArgs R, LevelArgs<double> L0, FracMetrics M, double dt,
+                                                     const double *__restrict__ visc, const double *__restrict__ bxv,
+                                                     const double *__restrict__ byv, const double *__restrict__ bzv,
+                                                     double *__restrict__ coef, long nref) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= R.nxc || j >= R.nyc) return;
+  const long id = ridx(R, i, j, k);
+  const uint8_t f = L0.fl[lidx(L0.sx, L0.sxy, i, j, k)];
+  const long st[3] = {1, R.ex, R.exy};
+  const int I[3] = {i + 1, j + 1, k + 1};
+  const uint8_t FM[3] = {FL_IM, FL_JM, FL_KM}, FP[3] = {FL_IP, FL_JP, FL_KP};
+  const double *bv[3] = {bxv, byv, bzv};
+  const double hdt = 0.5 * dt, vc = visc[id];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int um = (f & FM[d]) ? 1 : 0, up = (f & FP[d]) ? 1 : 0;
+    const int q = I[d];
+    const double fp = M.fx[d][q + 1], fm = M.fx[d][q];
+    const double nup = (fp * visc[id + st[d]] + (1.0 - fp) * vc) * (double)(1 - up) + bv[d][id] * (double)up;
+    const double num = (fm * vc + (1.0 - fm) * visc[id - st[d]]) * (double)(1 - um) + bv[d][id] * (double)um;
+    const double di = M.dinv[d][q];
+    const double am = (M.dcinv[d][q] * (double)(1 - um) + di * (double)um * 2.0) * di;
+    const double ap = (M.dcinv[d][q + 1] * (double)(1 - up) + di * (double)up * 2.0) * di;
+    // amz_ad, apz_ad carry the factor ndim - 2 = 1 of a 3-D grid
+    coef[(2 * d) * nref + id] = d == 2 ? -(hdt * num) * am * 1.0 : -(hdt * num) * am;
+    coef[(2 * d + 1) * nref + id] = d == 2 ? -(hdt * nup) * ap * 1.0 : -(hdt * nup) * ap;
+  }
+}
+
+// The operator of one cell as itsolv_ad_* and calc_residual_ad build it
+// before masking (CORE_SOLVE_AD.F90:391-421): the viscous coefficients
+// plus the Crank-Nicolson advection terms (half_dt = 0 for
+// ADAMS_BASHFORTH2), and the diagonal sum acx + acy + acz.  Face
+// velocities are fixed during one solve_ad call, so the rows are built
+// once per call: row = amx, apx, amy, apy, amz, apz, sum (nref apart).
+template <int NT>
+__global__ void __launch_bounds__(NT) k_ad_rows(RefArgs R, LevelArgs<double> L0, FracMetrics M, double half_dt,
+                                                const double *__restrict__ coef, const double *__restrict__ fu,
+                                                const double *__restrict__ fv, const double *__restrict__ fw,
+                                                double *__restrict__ row, long nref) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= R.nxc || j >= R.nyc) return;
+  const long id = ridx(R, i, j, k);
+  const uint8_t f = L0.fl[lidx(L0.sx, L0.sxy, i, j, k)];
+  const long st[3] = {1, R.ex, R.exy};
+  const int I[3] = {i + 1, j + 1, k + 1};
+  const uint8_t FM[3] = {FL_IM, FL_JM, FL_KM}, FP[3] = {FL_IP, FL_JP, FL_KP};
+  const double *face[3] = {fu, fv, fw};
+  double ac[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int um = (f & FM[d]) ? 1 : 0, up = (f & FP[d]) ? 1 : 0;
+    const int q = I[d];
+    double am = coef[(2 * d) * nref + id], ap = coef[(2 * d + 1) * nref + id];
+    double a = -(am + ap);
+    const double fm = M.fx[d][q], fp = M.fx[d][q + 1], di = M.dinv[d][q];
+    const double vm = face[d][id], vp = face[d][id + st[d]];
+    double t1 = 1.0 - fm, t2 = fp;
+    am = am - half_dt * vm * t1 * di;
+    ap = ap + half_dt * vp * t2 * di;
+    t1 = fm * (double)(1 - um);
+    t2 = (1.0 - fp) * (double)(1 - up);
+    a = a + half_dt * (vp * t2 - vm * t1) * di;
+    row[(2 * d) * nref + id] = am;
+    row[(2 * d + 1) * nref + id] = ap;
+    ac[d] = a;
+  }
+  row[6 * nref + id] = ac[0] + ac[1] + ac[2];
+}
+
+// One zebra pass of the momentum line sweeps along DIR (0 x, 1 y, 2 z):
+// lines of colour `color` (the parity of the sum of their two transverse
+// indices), so every off-line neighbour has the other colour and the pass
+// is deterministic.  A block of TX lines x P segments of LS cells; each
+// thread eliminates its segment in registers and the reduced system of a
+// line is solved by cyclic reduction in LDS (seg_reduced_solve).  For
+// x-lines the segments of one line are adjacent lanes (coalesced along
+// the row); for y- and z-lines adjacent lanes are every other i.
+// Row of a cell (itsolv_ad_x/y/z): off-line terms subtracted from r in the
+// reference's grouping, masked line coefficients, diagonal
+// 1 + sum * (1 - iblank), right-hand side r' (1 - iblank) + var * ghost.
+template <int DIR, int LS>
+__global__ void __launch_bounds__(1024) k_ad_line(RefArgs R, LevelArgs<double> L0, const double *__restrict__ row,
+                                                  long nref, const double *__restrict__ r, double *__restrict__ var,
+                                                  int color, int nlines) {
+#pragma clang fp contract(off)
+  extern __shared__ unsigned char smem_raw[];
+  double *sh = reinterpret_cast<double *>(smem_raw);
+  const int TX = blockDim.x, P = blockDim.y;
+  int tx, seg, i = 0, j = 0, k = 0;
+  bool ok;
+  if (DIR == 0) {
+    const int flat = threadIdx.x + threadIdx.y * TX;
+    seg = flat % P;
+    tx = flat / P;
+    const int line = blockIdx.x * TX + tx;
+    const int nyh = (R.nyc + 1) >> 1;
+    k = line / nyh;
+    j = 2 * (line - k * nyh) + ((k + color) & 1);
+    ok = line < nlines && k < R.nzc && j < R.nyc;
+  } else {
+    tx = threadIdx.x;
+    seg = threadIdx.y;
+    const int plane = blockIdx.y;
+    i = 2 * (blockIdx.x * TX + tx) + ((plane + color) & 1);
+    if (DIR == 1) k = plane; else j = plane;
+    ok = i < R.nxc && plane < (DIR == 1 ? R.nzc : R.nyc);
+  }
+  const int n = DIR == 0 ? R.nxc : (DIR == 1 ? R.nyc : R.nzc);
+  const long rst = DIR == 0 ? 1 : (DIR == 1 ? R.ex : R.exy);
+  const long lst = DIR == 0 ? 1 : (DIR == 1 ? (long)L0.sx : L0.sxy);
+  const int m0 = seg * LS;
+  long id0 = 0, la0 = 0;
+  if (ok) {
+    const int i0 = DIR == 0 ? m0 : i, j0 = DIR == 1 ? m0 : j, k0 = DIR == 2 ? m0 : k;
+    id0 = ridx(R, i0, j0, k0);
+    la0 = lidx(L0.sx, L0.sxy, i0, j0, k0);
+  }
+  SegElim<double, LS> se;
+#pragma unroll
+  for (int m = 0; m < LS; m++) {
+    double a = 0.0, b = 1.0, c = 0.0, d = 0.0;
+    if (ok && m0 + m < n) {
+      const long id = id0 + m * rst;
+      const uint8_t f = L0.fl[la0 + m * lst];
+      const double rib = (f & FL_BLANK) ? 0.0 : 1.0;
+      const double gh = (f & FL_GHOST) ? 1.0 : 0.0;
+      const double amx = row[id], apx = row[nref + id], amy = row[2 * nref + id], apy = row[3 * nref + id];
+      const double amz = row[4 * nref + id], apz = row[5 * nref + id], sum = row[6 * nref + id];
+      const double oim = (f & FL_IM) ? 0.0 : 1.0, oip = (f & FL_IP) ? 0.0 : 1.0;
+      const double ojm = (f & FL_JM) ? 0.0 : 1.0, ojp = (f & FL_JP) ? 0.0 : 1.0;
+      const double okm = (f & FL_KM) ? 0.0 : 1.0, okp = (f & FL_KP) ? 0.0 : 1.0;
+      double rhs, am, ap;
+      if (DIR == 0) {
+        rhs = r[id] - (var[id - R.ex] * amy * ojm + var[id + R.ex] * apy * ojp + var[id - R.exy] * amz * okm +
+                       var[id + R.exy] * apz * okp);
+        am = amx * rib * oim;
+        ap = apx * rib * oip;
+      } else if (DIR == 1) {
+        rhs = r[id] - (var[id - 1] * amx * oim + var[id + 1] * apx * oip + var[id - R.exy] * amz * okm +
+                       var[id + R.exy] * apz * okp);
+        am = amy * rib * ojm;
+        ap = apy * rib * ojp;
+      } else {
+        rhs = r[id] - var[id - R.ex] * amy * ojm - var[id + R.ex] * apy * ojp - var[id - 1] * amx * oim -
+              var[id + 1] * apx * oip;
+        am = amz * rib * okm;
+        ap = apz * rib * okp;
+      }
+      a = am;
+      c = ap;
+      b = 1.0 + sum * rib;
+      d = rhs * rib + var[id] * gh;
+    }
+    se.push(m, a, b, c, d);
+  }
+  double XL, XS;
+  seg_reduced_solve<double, LS>(sh, P, tx, seg, se, XL, XS);
+  if (!ok) return;
+  se.combine(XL, XS);
+#pragma unroll
+  for (int m = 0; m < LS; m++)
+    if (m0 + m < n) var[id0 + m * rst] = se.dp[m];
+}
+
+// calc_residual_ad for NV fields (u, v, w share the rows): per-block
+// partials (0, max |res|, argmax) of each field at part + 3*nblk*q, the
+// argmax the smallest reference-layout offset of the block's maximum.
+struct AdFields {
+  const double *var[3], *r[3];
+};
+template <int NT, int NV>
+__global__ void __launch_bounds__(NT) k_ad_residual(RefArgs R, LevelArgs<double> L0, const double *__restrict__ row,
+                                                    long nref, AdFields F, double *__restrict__ part, long nblk,
+                                                    double *__restrict__ rout) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  const bool in = i < R.nxc && j < R.nyc;
+  long id = 0;
+  uint8_t f = FL_BLANK;
+  double amx = 0, apx = 0, amy = 0, apy = 0, amz = 0, apz = 0, sum = 0;
+  if (in) {
+    id = ridx(R, i, j, k);
+    f = L0.fl[lidx(L0.sx, L0.sxy, i, j, k)];
+    amx = row[id]; apx = row[nref + id]; amy = row[2 * nref + id]; apy = row[3 * nref + id];
+    amz = row[4 * nref + id]; apz = row[5 * nref + id]; sum = row[6 * nref + id];
+  }
+  const double oim = (f & FL_IM) ? 0.0 : 1.0, oip = (f & FL_IP) ? 0.0 : 1.0;
+  const double ojm = (f & FL_JM) ? 0.0 : 1.0, ojp = (f & FL_JP) ? 0.0 : 1.0;
+  const double okm = (f & FL_KM) ? 0.0 : 1.0, okp = (f & FL_KP) ? 0.0 : 1.0;
+  const double fl = (f & FL_BLANK) ? 0.0 : 1.0;
+  const long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    SumMax v{0.0, 0.0, -1};
+    if (in) {
+      const double *x = F.var[q];
+      double s = F.r[q][id] - x[id] * (1.0 + sum * 1.0) - x[id - 1] * amx * oim - x[id + 1] * apx * oip -
+                 x[id - R.ex] * amy * ojm - x[id + R.ex] * apy * ojp - x[id - R.exy] * amz * okm -
+                 x[id + R.exy] * apz * okp;
+      s = s * fl;
+      if (rout) rout[id] = s;
+      v.mx = fabs(s);
+      v.arg = id;
+    }
+    v = block_summax<NT>(v);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      double *pq = part + 3 * nblk * q;
+      pq[3 * blk] = 0.0;
+      pq[3 * blk + 1] = v.mx;
+      pq[3 * blk + 2] = (double)v.arg;
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace pp

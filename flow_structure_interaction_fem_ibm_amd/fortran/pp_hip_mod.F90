@@ -21,6 +21,8 @@ module pp_hip_mod
   integer, parameter, public :: PP_K_CLASSES = 13
   integer(c_int32_t), parameter, public :: PP_BODY_ELLIPSOID = 3
   integer(c_int32_t), parameter, public :: PP_GEOM_ANALYTIC = 0, PP_GEOM_TRIANGULATED = 1
+  ! advec_scheme (AMODULES.F90:60-62)
+  integer(c_int32_t), parameter, public :: PP_ADAMS_BASHFORTH2 = 1, PP_CRANK_NICOLSON1 = 2, PP_CRANK_NICOLSON2 = 3
 
   type, bind(c), public :: pp_grid_desc
      integer(c_int32_t) :: nxc, nyc, nzc
@@ -70,11 +72,21 @@ module pp_hip_mod
      integer(c_int32_t) :: kern_launches_l0(PP_K_CLASSES)
   end type pp_solve_stats
 
+  ! solve_ad: iterations, max(maxresu, maxresv, maxresw), the three, loc
+  type, bind(c), public :: pp_ad_stats
+     integer(c_int32_t) :: status, iterations
+     real(c_double) :: max_residual
+     real(c_double) :: residual_uvw(3)
+     integer(c_int32_t) :: loc(3)
+     real(c_double) :: ms
+  end type pp_ad_stats
+
   public :: pp_default_opts, pp_last_error, pp_create, pp_destroy, pp_set_geometry, pp_solve, &
        pp_solve_device, pp_adjoint, pp_adjoint_device, pp_calc_residual, pp_ghost_rows, pp_mean_remove, pp_divergence, &
        pp_smooth, pp_null_vector, pp_level_info, pp_error_text, pp_geom_build, pp_geom_counts, pp_geom_get, &
        pp_geom_ghost_elements, pp_geom_destroy, pp_geom_dirichlet, pp_set_vel_ghosts, pp_correct_vel, &
-       pp_correct_vel_device, pp_update_pressure, pp_update_pressure_device
+       pp_correct_vel_device, pp_update_pressure, pp_update_pressure_device, pp_set_solve_ad, &
+       pp_set_solve_ad_device, pp_solve_ad, pp_solve_ad_device, pp_calc_residual_ad
 
   interface
      subroutine pp_default_opts(o) bind(c, name="pp_default_opts")
@@ -268,6 +280,55 @@ module pp_hip_mod
        integer(c_int32_t), value :: van_kan
        integer(c_int32_t) :: rc
      end function pp_update_pressure_device
+
+     ! set_solve_ad: viscTot and bcx/y/zvisc, reference layout
+     function pp_set_solve_ad(ctx, dt, visc_tot, bcxvisc, bcyvisc, bczvisc) &
+          bind(c, name="pp_set_solve_ad") result(rc)
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, visc_tot, bcxvisc, bcyvisc, bczvisc
+       real(c_double), value :: dt
+       integer(c_int32_t) :: rc
+     end function pp_set_solve_ad
+
+     function pp_set_solve_ad_device(ctx, dt, visc_tot, bcxvisc, bcyvisc, bczvisc) &
+          bind(c, name="pp_set_solve_ad_device") result(rc)
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, visc_tot, bcxvisc, bcyvisc, bczvisc
+       real(c_double), value :: dt
+       integer(c_int32_t) :: rc
+     end function pp_set_solve_ad_device
+
+     ! solve_ad, GCM branch: u, v, w in/out; nlu, nlv, nlw; face_u/v/w; vel_bi (3, nGhost) or C_NULL_PTR
+     function pp_solve_ad(ctx, advec_scheme, u, v, w, nlu, nlv, nlw, face_u, face_v, face_w, vel_bi, &
+          itermax_ad, restol_ad, st) bind(c, name="pp_solve_ad") result(rc)
+       import :: c_ptr, c_int32_t, c_double, pp_ad_stats
+       type(c_ptr), value :: ctx, u, v, w, nlu, nlv, nlw, face_u, face_v, face_w, vel_bi
+       integer(c_int32_t), value :: advec_scheme, itermax_ad
+       real(c_double), value :: restol_ad
+       type(pp_ad_stats), intent(out) :: st
+       integer(c_int32_t) :: rc
+     end function pp_solve_ad
+
+     function pp_solve_ad_device(ctx, advec_scheme, u, v, w, nlu, nlv, nlw, face_u, face_v, face_w, vel_bi, &
+          itermax_ad, restol_ad, st) bind(c, name="pp_solve_ad_device") result(rc)
+       import :: c_ptr, c_int32_t, c_double, pp_ad_stats
+       type(c_ptr), value :: ctx, u, v, w, nlu, nlv, nlw, face_u, face_v, face_w, vel_bi
+       integer(c_int32_t), value :: advec_scheme, itermax_ad
+       real(c_double), value :: restol_ad
+       type(pp_ad_stats), intent(out) :: st
+       integer(c_int32_t) :: rc
+     end function pp_solve_ad_device
+
+     ! calc_residual_ad of one field: resm, loc (1-based), res (or C_NULL_PTR)
+     function pp_calc_residual_ad(ctx, advec_scheme, var, r, face_u, face_v, face_w, res, resm, loc) &
+          bind(c, name="pp_calc_residual_ad") result(rc)
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, var, r, face_u, face_v, face_w, res
+       integer(c_int32_t), value :: advec_scheme
+       real(c_double), intent(out) :: resm
+       integer(c_int32_t), intent(out) :: loc(3)
+       integer(c_int32_t) :: rc
+     end function pp_calc_residual_ad
 
      function pp_geom_destroy(geo) bind(c, name="pp_geom_destroy") result(rc)
        import :: c_ptr, c_int32_t

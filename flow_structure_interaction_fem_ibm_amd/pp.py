@@ -84,6 +84,19 @@ KERNEL_CLASSES = ("outer_residual", "update", "xline", "yline", "zline", "residu
 NK = len(KERNEL_CLASSES)
 
 
+PP_ADAMS_BASHFORTH2, PP_CRANK_NICOLSON1, PP_CRANK_NICOLSON2 = 1, 2, 3
+
+
+class ADStats(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("max_residual", C.c_double),
+                ("residual_uvw", C.c_double * 3), ("loc", C.c_int32 * 3), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {"status": _STATUS.get(self.status, self.status), "iterations": self.iterations,
+                "max_residual": self.max_residual, "residual_uvw": list(self.residual_uvw),
+                "loc": list(self.loc), "ms": self.ms}
+
+
 class SolveStats(C.Structure):
     _fields_ = [("status", C.c_int32), ("outer_iterations", C.c_int32), ("vcycles", C.c_int32),
                 ("rel_l2", C.c_double), ("abs_linf", C.c_double), ("loc", C.c_int32 * 3),
@@ -152,12 +165,19 @@ def lib():
     L.pp_correct_vel_device.argtypes = [P, C.c_double] + [P] * 11
     L.pp_update_pressure.argtypes = [P, C.c_int32, P, P, P]
     L.pp_update_pressure_device.argtypes = [P, C.c_int32, P, P, P]
+    L.pp_set_solve_ad.argtypes = [P, C.c_double] + [P] * 4
+    L.pp_set_solve_ad_device.argtypes = [P, C.c_double] + [P] * 4
+    L.pp_solve_ad.argtypes = [P, C.c_int32] + [P] * 10 + [C.c_int32, C.c_double, C.POINTER(ADStats)]
+    L.pp_solve_ad_device.argtypes = [P, C.c_int32] + [P] * 10 + [C.c_int32, C.c_double, C.POINTER(ADStats)]
+    L.pp_calc_residual_ad.argtypes = [P, C.c_int32] + [P] * 6 + [dp, C.POINTER(C.c_int32)]
     for name in ("pp_create", "pp_destroy", "pp_set_geometry", "pp_solve", "pp_solve_device",
                  "pp_adjoint", "pp_calc_residual", "pp_ghost_rows", "pp_mean_remove",
                  "pp_divergence", "pp_smooth", "pp_tdma_batched", "pp_null_vector", "pp_level_info", "pp_time_kernel",
                  "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get", "pp_geom_ghost_elements",
                  "pp_geom_destroy", "pp_geom_dirichlet", "pp_set_vel_ghosts", "pp_correct_vel",
-                 "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device", "pp_adjoint_device"):
+                 "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device", "pp_adjoint_device",
+                 "pp_set_solve_ad", "pp_set_solve_ad_device", "pp_solve_ad", "pp_solve_ad_device",
+                 "pp_calc_residual_ad"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -169,7 +189,8 @@ EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_s
             "pp_level_info", "pp_time_kernel", "pp_setup_info", "pp_geom_build", "pp_geom_counts", "pp_geom_get",
             "pp_geom_ghost_elements", "pp_geom_destroy", "pp_geom_dirichlet", "pp_set_vel_ghosts",
             "pp_correct_vel", "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device",
-            "pp_adjoint_device")
+            "pp_adjoint_device", "pp_set_solve_ad", "pp_set_solve_ad_device", "pp_solve_ad", "pp_solve_ad_device",
+            "pp_calc_residual_ad")
 
 
 def _check(rc, allow=(PP_OK,)):
@@ -360,6 +381,54 @@ class PressureSolver:
     def update_pressure_device(self, van_kan, pprime, p, bc_src=None):
         _check(lib().pp_update_pressure_device(self.h, int(van_kan), pprime.data_ptr(), p.data_ptr(),
                                                None if bc_src is None else bc_src.data_ptr()))
+
+    # ---- implicit momentum solve ----
+    def set_solve_ad(self, dt, visc_tot, bcxvisc, bcyvisc, bczvisc):
+        """pp_set_solve_ad: the implicit viscous coefficients (set_solve_ad)."""
+        f = [np.ascontiguousarray(a, dtype=np.float64) for a in (visc_tot, bcxvisc, bcyvisc, bczvisc)]
+        for a in f:
+            assert a.shape == self.shape
+        _check(lib().pp_set_solve_ad(self.h, float(dt), *[a.ctypes.data for a in f]))
+
+    def set_solve_ad_device(self, dt, visc_tot, bcxvisc, bcyvisc, bczvisc):
+        _check(lib().pp_set_solve_ad_device(self.h, float(dt), *[a.data_ptr() for a in
+                                                                 (visc_tot, bcxvisc, bcyvisc, bczvisc)]))
+
+    def solve_ad(self, advec_scheme, u, v, w, nlu, nlv, nlw, face_u, face_v, face_w, vel_bi=None,
+                 itermax_ad=1000, restol_ad=1e-10):
+        """pp_solve_ad on copies of the host fields; returns (u, v, w, stats)."""
+        out = [np.array(a, dtype=np.float64, order="C", copy=True) for a in (u, v, w)]
+        ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (nlu, nlv, nlw, face_u, face_v, face_w)]
+        for a in out + ins:
+            assert a.shape == self.shape
+        vb = None if vel_bi is None else np.ascontiguousarray(vel_bi, dtype=np.float64)
+        st = ADStats()
+        _check(lib().pp_solve_ad(self.h, int(advec_scheme), *[a.ctypes.data for a in out],
+                                 *[a.ctypes.data for a in ins], None if vb is None else vb.ctypes.data,
+                                 int(itermax_ad), float(restol_ad), C.byref(st)), allow=(PP_OK, PP_NOT_CONVERGED))
+        return out[0], out[1], out[2], st.as_dict()
+
+    def solve_ad_device(self, advec_scheme, u, v, w, nlu, nlv, nlw, face_u, face_v, face_w, vel_bi=None,
+                        itermax_ad=1000, restol_ad=1e-10):
+        """pp_solve_ad_device on torch float64 tensors (u, v, w updated in place); returns the stats."""
+        st = ADStats()
+        t = (u, v, w, nlu, nlv, nlw, face_u, face_v, face_w)
+        _check(lib().pp_solve_ad_device(self.h, int(advec_scheme), *[a.data_ptr() for a in t],
+                                        None if vel_bi is None else vel_bi.data_ptr(), int(itermax_ad),
+                                        float(restol_ad), C.byref(st)), allow=(PP_OK, PP_NOT_CONVERGED))
+        return st.as_dict()
+
+    def calc_residual_ad(self, advec_scheme, var, r, face_u, face_v, face_w):
+        """pp_calc_residual_ad: returns (max |res|, 1-based loc, res field)."""
+        f = [np.ascontiguousarray(a, dtype=np.float64) for a in (var, r, face_u, face_v, face_w)]
+        for a in f:
+            assert a.shape == self.shape
+        res = np.zeros(self.shape)
+        m = C.c_double()
+        loc = (C.c_int32 * 3)()
+        _check(lib().pp_calc_residual_ad(self.h, int(advec_scheme), *[a.ctypes.data for a in f], res.ctypes.data,
+                                         C.byref(m), loc))
+        return m.value, tuple(loc), res
 
     def divergence(self, face_u, face_v, face_w, dtinv=1.0):
         """rhs_poisson (non-cut-cell): returns (nlu0, sum(nlu0*dV))."""

@@ -258,6 +258,76 @@ int32_t pp_update_pressure_device(pp_ctx *ctx, int32_t van_kan,
                                   const double *d_pprime, double *d_p,
                                   const double *d_bc_src);
 
+/* ---- implicit momentum solve of the fractional step ------------------
+ * advec_scheme as the reference's (AMODULES.F90:60-62): ADAMS_BASHFORTH2
+ * keeps only the viscous terms implicit (the itsolv_ad_*_AB routines),
+ * CRANK_NICOLSON1 adds the face-velocity advection terms with half_dt =
+ * dt/2.  CRANK_NICOLSON2 recomputes the face velocities inside the loop
+ * (face_vel) and is rejected with PP_BAD_ARG. */
+#define PP_ADAMS_BASHFORTH2 1
+#define PP_CRANK_NICOLSON1 2
+#define PP_CRANK_NICOLSON2 3
+
+typedef struct pp_ad_stats {
+  int32_t status;
+  int32_t iterations;        /* outer iterations of the GCM loop */
+  double max_residual;       /* max over u, v, w of calc_residual_ad */
+  double residual_uvw[3];    /* last calc_residual_ad of u, v, w */
+  int32_t loc[3];            /* argmax (1-based i, j, k) of the largest */
+  double ms;                 /* device time of the call (HIP events) */
+} pp_ad_stats;
+
+/* set_solve_ad (CORE_SET_SOLVERS.F90:20-223; called by the reference after
+ * every set_boundary, MAIN.F90 time loop): the implicit viscous
+ * coefficients amx_ad..apz_ad of the momentum equations, kept on the
+ * device until the next call or pp_set_geometry.  visc_tot = viscTot
+ * (reference layout, outer layer included), bcxvisc..bczvisc the
+ * boundary-face viscosities.  Fresh cells (GCM_set_freshcell branch,
+ * :190-221) are not handled: call it on a geometry without fresh cells.
+ * Requires pp_set_geometry. */
+int32_t pp_set_solve_ad(pp_ctx *ctx, double dt, const double *visc_tot,
+                        const double *bcxvisc, const double *bcyvisc,
+                        const double *bczvisc);
+int32_t pp_set_solve_ad_device(pp_ctx *ctx, double dt, const double *d_visc_tot,
+                               const double *d_bcxvisc, const double *d_bcyvisc,
+                               const double *d_bczvisc);
+
+/* solve_ad, GCM branch (CORE_SOLVE_AD.F90:200-267): until max(calc_residual_ad
+ * of u, v, w) <= restol_ad or itermax_ad iterations, line sweeps in x, y
+ * and z of u, v and w (itsolv_ad, :272-762; here zebra-ordered line passes
+ * instead of the reference's lexicographic order, same fixed point) and
+ * the velocity ghost update GCM_GhostCell_Vel
+ * (GCM_GHOSTCELL_VEL_UPDATE_UNSTRUC.F90:2-124, Dirichlet outer faces: its
+ * mass-flux correction does not act).  u, v, w: initial guess in, solution
+ * out; nlu, nlv, nlw: right-hand sides; face_u/v/w: face velocities of the
+ * advection terms (read for CRANK_NICOLSON1); vel_bi: (u, v, w) per ghost
+ * at the body intercept or NULL.  Requires pp_set_solve_ad and, with
+ * ghost cells, pp_set_vel_ghosts.  Returns PP_OK or PP_NOT_CONVERGED (the
+ * fields are written either way, as the reference continues). */
+int32_t pp_solve_ad(pp_ctx *ctx, int32_t advec_scheme, double *u, double *v,
+                    double *w, const double *nlu, const double *nlv,
+                    const double *nlw, const double *face_u,
+                    const double *face_v, const double *face_w,
+                    const double *vel_bi, int32_t itermax_ad, double restol_ad,
+                    pp_ad_stats *st);
+int32_t pp_solve_ad_device(pp_ctx *ctx, int32_t advec_scheme, double *d_u,
+                           double *d_v, double *d_w, const double *d_nlu,
+                           const double *d_nlv, const double *d_nlw,
+                           const double *d_face_u, const double *d_face_v,
+                           const double *d_face_w, const double *d_vel_bi,
+                           int32_t itermax_ad, double restol_ad,
+                           pp_ad_stats *st);
+
+/* calc_residual_ad (CORE_SOLVE_AD.F90:766-907) of one field: *resm = max
+ * |res| over the cells, loc its 1-based (i, j, k) (first in the
+ * reference's k, j, i loop order), res (may be NULL) the residual field.
+ * Host pointers.  Requires pp_set_solve_ad. */
+int32_t pp_calc_residual_ad(pp_ctx *ctx, int32_t advec_scheme,
+                            const double *var, const double *r,
+                            const double *face_u, const double *face_v,
+                            const double *face_w, double *res, double *resm,
+                            int32_t loc[3]);
+
 /* tdma (CORE_TDMA.F90:10-34) on a batch of independent systems, through
  * the device line solvers: n_lines systems of length n stored line after
  * line (a: sub-, b: main, c: super-diagonal, r: rhs, u: solution).

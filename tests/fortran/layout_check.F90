@@ -11,6 +11,7 @@ program layout_check
   type(pp_solve_opts) :: o
   type(pp_grid_desc) :: g
   type(pp_solve_stats) :: st
+  type(pp_ad_stats) :: ast
   type(c_ptr) :: ctx
   integer(c_int32_t) :: rc
   real(c_double), target :: xs(3), xn(25)
@@ -36,6 +37,7 @@ program layout_check
        storage_size(pp_geometry(c_null_ptr, c_null_ptr, 0, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
        c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr)) / 8
   print '(A,I0)', 'bodysize ', storage_size(body(1)) / 8
+  print '(A,I0)', 'adstatsize ', storage_size(ast) / 8
   xn = [(1.2d0 * i / 24.0d0, i = 0, 24)]
   g%nxc = 24; g%nyc = 24; g%nzc = 24
   g%x = c_loc(xn); g%y = c_loc(xn); g%z = c_loc(xn)
