@@ -1314,6 +1314,16 @@ int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
       set_error("pp_create: face %d has pressure BC %d; only PP_BC_NEUMANN (2) is implemented", f, g->bc[f]);
       return PP_BAD_ARG;
     }
+  {
+    const long nc = coarsest_cells(g->nxc, g->nyc, g->nzc);
+    if (nc > kCoarseDenseMax) {
+      set_error("pp_create: %dx%dx%d cells coarsen to a %ld-cell coarsest level (a direction is halved while its "
+                "cell count is even and at least 4); at most %ld cells are solved directly: give the grid even "
+                "cell counts with more factors of 2",
+                g->nxc, g->nyc, g->nzc, nc, kCoarseDenseMax);
+      return PP_BAD_ARG;
+    }
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     set_error("pp_create: no HIP device available");

@@ -144,6 +144,22 @@ void free_ghosts(pp_ctx *c) {
 static constexpr long kCoarseMax = 4096;
 static constexpr int kMaxLevels = 12;
 
+long coarsest_cells(int nxc, int nyc, int nzc) {
+  int n[3] = {nxc, nyc, nzc};
+  for (int l = 1; l < kMaxLevels; l++) {
+    if ((long)n[0] * n[1] * n[2] <= kCoarseMax) break;
+    int cf[3];
+    bool any = false;
+    for (int d = 0; d < 3; d++) {
+      cf[d] = (n[d] % 2 == 0 && n[d] >= 4) ? 2 : 1;
+      any |= cf[d] == 2;
+    }
+    if (!any) break;
+    for (int d = 0; d < 3; d++) n[d] /= cf[d];
+  }
+  return (long)n[0] * n[1] * n[2];
+}
+
 int build_levels(pp_ctx *c) {
   free_levels(c);
   std::vector<Level> lv;
@@ -605,6 +621,11 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
 int build_coarse(pp_ctx *c) {
   Level &L = c->lv.back();
   int n = L.cx * L.cy * L.cz;
+  if (n > kCoarseDenseMax) {
+    set_error("coarsest level of %d cells (%dx%dx%d): at most %ld are solved directly", n, L.cx, L.cy, L.cz,
+              kCoarseDenseMax);
+    return PP_BAD_ARG;
+  }
   auto cid = [&](int i, int j, int k) { return i + L.cx * (j + L.cy * k); };
   std::vector<int> comp(n, -1);
   std::vector<char> fluid(n, 0);

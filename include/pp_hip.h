@@ -147,7 +147,10 @@ const char *pp_last_error(void);
 
 /* Create a context on one GPU for a grid (replaces MG_initial,
  * CORE_MG.F90:31-278, and the grid half of init_simulation).  4 to 512
- * cells per direction (PP_BAD_ARG otherwise). */
+ * cells per direction, and a grid whose levels (a direction is halved
+ * while its cell count is even and at least 4, down to at most 4096 cells)
+ * end at a coarsest level of at most 16384 cells, which is solved directly
+ * (PP_BAD_ARG otherwise). */
 int32_t pp_create(pp_ctx **ctx, const pp_grid_desc *g);
 int32_t pp_destroy(pp_ctx *ctx);
 

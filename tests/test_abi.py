@@ -46,3 +46,24 @@ def test_create_rejects_bad_grids_without_touching_a_device():
     g.bc[0] = 1                        # Dirichlet: not implemented
     assert lib.pp_create(ctypes.byref(h), ctypes.byref(g)) == pp.PP_BAD_ARG
     assert b"PP_BC_NEUMANN" in lib.pp_last_error()
+
+
+def test_create_rejects_grids_without_a_small_coarsest_level():
+    """A direction is halved only while its cell count is even; a grid that
+    cannot be coarsened to a small level (65 x 49 x 41 cells: 130585) would
+    need a dense coarsest operator of n^2 entries.  pp_create refuses it
+    before touching a device; 25^3 (15625 cells, no coarsening) is within
+    the direct-solve bound and passes this check (it then fails here only
+    for want of a device)."""
+    import numpy as np
+    lib = pp.lib()
+    h = ctypes.c_void_p()
+    g = pp._grid_desc(np.linspace(0, 1, 66), np.linspace(0, 1, 50), np.linspace(0, 1, 42))
+    assert lib.pp_create(ctypes.byref(h), ctypes.byref(g)) == pp.PP_BAD_ARG
+    msg = lib.pp_last_error()
+    assert b"130585-cell coarsest level" in msg, msg
+    g = pp._grid_desc(np.linspace(0, 1, 26), np.linspace(0, 1, 26), np.linspace(0, 1, 26))
+    rc = lib.pp_create(ctypes.byref(h), ctypes.byref(g))
+    assert rc != pp.PP_BAD_ARG or b"coarsest" not in lib.pp_last_error()
+    if rc == pp.PP_OK:
+        lib.pp_destroy(h)
