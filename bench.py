@@ -73,6 +73,45 @@ def time_frac_step(solver, ex, dev, reps=10):
     return out
 
 
+def time_solve_ad(solver, dev, reps=3):
+    """The implicit momentum solve (pp_set_solve_ad_device +
+    pp_solve_ad_device, Crank-Nicolson, GCM ghost velocities) on device
+    fields of the same grid and geometry (not part of the headline value):
+    nu = 1e-3, dt = 1e-3, face velocities and right-hand sides N(0,1), from
+    u = v = w = 0 to max residual 1e-10.  Wall time per call around
+    synchronised calls, iterations, and algorithmic GB/s at 564 B/cell per
+    iteration: 6 line passes (x, y, z, two colours) of u, v and w together,
+    76.5 B/cell each (half of the 7 FP64 row coefficients 28 and of the
+    flags 0.5; per field half of r, var read and var written 12, and the
+    other colour's var 4), and the three residuals, 105 B/cell."""
+    import torch
+    shape = solver.shape
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    dt = 1e-3
+    visc = torch.full(shape, 1e-3, dtype=torch.float64, device=dev)
+    solver.set_solve_ad_device(dt, visc, visc, visc, visc)
+    faces = [torch.randn(shape, dtype=torch.float64, device=dev, generator=g) for _ in range(3)]
+    nl = [torch.randn(shape, dtype=torch.float64, device=dev, generator=g) for _ in range(3)]
+    u = [torch.zeros(shape, dtype=torch.float64, device=dev) for _ in range(3)]
+    st = solver.solve_ad_device(pp.PP_CRANK_NICOLSON1, *u, *nl, *faces, itermax_ad=100, restol_ad=1e-10)
+    ms = []
+    for _ in range(reps):
+        for t in u:
+            t.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solver.solve_ad_device(pp.PP_CRANK_NICOLSON1, *u, *nl, *faces, itermax_ad=100, restol_ad=1e-10)
+        torch.cuda.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    cells = float(np.prod([n - 3 for n in shape]))
+    m = min(ms)
+    its = max(st["iterations"], 1)
+    return {"ms": m, "device_ms": st["ms"], "iterations": st["iterations"], "max_residual": st["max_residual"],
+            "status": st["status"], "ms_per_iteration": m / its, "bytes_per_cell_per_iteration": 564.0,
+            "GBs": 564.0 * cells * its / (m * 1e-3) / 1e9}
+
+
 def divergence_rhs(xn, yn, zn, ib, seed):
     """rhs_poisson (CORE_POISSON.F90:281-288) with dt = 1 of face velocities
     N(0,1) * sin(pi x/4) sin(pi y/3) sin(pi z/3) sampled at face centres."""
@@ -267,6 +306,7 @@ def main():
         "kernels_fp64": {k: {"ms": v["ms"], "launches": v["launches"]} for k, v in prof_fp64[0]["kernels"].items()},
         "setup_s": t_setup,
         "frac_step": time_frac_step(solver, gex, dev),
+        "momentum": time_solve_ad(solver, dev),
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -2142,18 +2142,19 @@ void ad_build_rows(pp_ctx *c, double half_dt, const double *fu, const double *fv
                                                       c->d_ad_coef, fu, fv, fw, row, c->nref);
 }
 
-// itsolv_ad: x, y and z line sweeps, two zebra passes each
-void ad_sweeps(pp_ctx *c, const double *row, const double *r, double *var) {
+// itsolv_ad of u, v and w: x, y and z line sweeps, two zebra passes each
+// (the three fields in each pass)
+void ad_sweeps(pp_ctx *c, const double *row, AdLineFields F) {
   LevelArgs<double> L0 = largs<double>(c->lv[0]);
   for (int dir = 0; dir < 3; dir++) {
     AdLineLaunch L = ad_line_launch(c, dir);
     for (int color = 0; color < 2; color++) {
       if (dir == 0)
-        k_ad_line<0, kAdLS><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, r, var, color, L.nlines);
+        k_ad_line<0, kAdLS><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, F, color, L.nlines);
       else if (dir == 1)
-        k_ad_line<1, kAdLS><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, r, var, color, L.nlines);
+        k_ad_line<1, kAdLS><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, F, color, L.nlines);
       else
-        k_ad_line<2, kAdLS><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, r, var, color, L.nlines);
+        k_ad_line<2, kAdLS><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, F, color, L.nlines);
     }
   }
 }
@@ -2267,7 +2268,9 @@ int32_t pp_solve_ad_device(pp_ctx *c, int32_t advec_scheme, double *d_u, double 
   double maxres = 1.0e10, ruvw[3] = {0.0, 0.0, 0.0};
   int32_t loc[3] = {0, 0, 0};
   while (it < itermax && maxres > restol) {
-    for (int q = 0; q < 3; q++) ad_sweeps(c, rows.d(), rhs[q], vars[q]);
+    AdLineFields LF;
+    for (int q = 0; q < 3; q++) { LF.var[q] = vars[q]; LF.r[q] = rhs[q]; }
+    ad_sweeps(c, rows.d(), LF);
     vel_ghosts(c, d_u, d_v, d_w, d_vbi);
     rc = ad_residual(c, rows.d(), 3, F, nullptr);
     if (rc) break;

@@ -2075,21 +2075,27 @@ __global__ void __launch_bounds__(NT) k_ad_rows(RefArgs R, LevelArgs<double> L0,
   row[6 * nref + id] = ac[0] + ac[1] + ac[2];
 }
 
-// One zebra pass of the momentum line sweeps along DIR (0 x, 1 y, 2 z):
-// lines of colour `color` (the parity of the sum of their two transverse
-// indices), so every off-line neighbour has the other colour and the pass
-// is deterministic.  A block of TX lines x P segments of LS cells; each
-// thread eliminates its segment in registers and the reduced system of a
-// line is solved by cyclic reduction in LDS (seg_reduced_solve).  For
-// x-lines the segments of one line are adjacent lanes (coalesced along
-// the row); for y- and z-lines adjacent lanes are every other i.
+// One zebra pass of the momentum line sweeps along DIR (0 x, 1 y, 2 z)
+// for u, v and w together: the three share the row coefficients, which are
+// read once, and are independent until the ghost update, so solving them
+// in one pass gives what three passes would.  Lines of colour `color` (the
+// parity of the sum of their two transverse indices), so every off-line
+// neighbour has the other colour and the pass is deterministic.  A block
+// of TX lines x P segments of LS cells; each thread eliminates its segment
+// of each field in registers and the reduced system of a line is solved by
+// cyclic reduction in LDS (seg_reduced_solve, once per field).  For
+// x-lines the segments of one line are adjacent lanes (coalesced along the
+// row); for y- and z-lines adjacent lanes are every other i.
 // Row of a cell (itsolv_ad_x/y/z): off-line terms subtracted from r in the
 // reference's grouping, masked line coefficients, diagonal
 // 1 + sum * (1 - iblank), right-hand side r' (1 - iblank) + var * ghost.
+struct AdLineFields {
+  double *var[3];
+  const double *r[3];
+};
 template <int DIR, int LS>
-__global__ void __launch_bounds__(1024) k_ad_line(RefArgs R, LevelArgs<double> L0, const double *__restrict__ row,
-                                                  long nref, const double *__restrict__ r, double *__restrict__ var,
-                                                  int color, int nlines) {
+__global__ void __launch_bounds__(512) k_ad_line(RefArgs R, LevelArgs<double> L0, const double *__restrict__ row,
+                                                 long nref, AdLineFields F, int color, int nlines) {
 #pragma clang fp contract(off)
   extern __shared__ unsigned char smem_raw[];
   double *sh = reinterpret_cast<double *>(smem_raw);
@@ -2123,10 +2129,10 @@ __global__ void __launch_bounds__(1024) k_ad_line(RefArgs R, LevelArgs<double> L
     id0 = ridx(R, i0, j0, k0);
     la0 = lidx(L0.sx, L0.sxy, i0, j0, k0);
   }
-  SegElim<double, LS> se;
+  SegElim<double, LS> se[3];
 #pragma unroll
   for (int m = 0; m < LS; m++) {
-    double a = 0.0, b = 1.0, c = 0.0, d = 0.0;
+    double a = 0.0, b = 1.0, c = 0.0, d[3] = {0.0, 0.0, 0.0};
     if (ok && m0 + m < n) {
       const long id = id0 + m * rst;
       const uint8_t f = L0.fl[la0 + m * lst];
@@ -2137,37 +2143,50 @@ __global__ void __launch_bounds__(1024) k_ad_line(RefArgs R, LevelArgs<double> L
       const double oim = (f & FL_IM) ? 0.0 : 1.0, oip = (f & FL_IP) ? 0.0 : 1.0;
       const double ojm = (f & FL_JM) ? 0.0 : 1.0, ojp = (f & FL_JP) ? 0.0 : 1.0;
       const double okm = (f & FL_KM) ? 0.0 : 1.0, okp = (f & FL_KP) ? 0.0 : 1.0;
-      double rhs, am, ap;
       if (DIR == 0) {
-        rhs = r[id] - (var[id - R.ex] * amy * ojm + var[id + R.ex] * apy * ojp + var[id - R.exy] * amz * okm +
-                       var[id + R.exy] * apz * okp);
-        am = amx * rib * oim;
-        ap = apx * rib * oip;
+        a = amx * rib * oim;
+        c = apx * rib * oip;
       } else if (DIR == 1) {
-        rhs = r[id] - (var[id - 1] * amx * oim + var[id + 1] * apx * oip + var[id - R.exy] * amz * okm +
-                       var[id + R.exy] * apz * okp);
-        am = amy * rib * ojm;
-        ap = apy * rib * ojp;
+        a = amy * rib * ojm;
+        c = apy * rib * ojp;
       } else {
-        rhs = r[id] - var[id - R.ex] * amy * ojm - var[id + R.ex] * apy * ojp - var[id - 1] * amx * oim -
-              var[id + 1] * apx * oip;
-        am = amz * rib * okm;
-        ap = apz * rib * okp;
+        a = amz * rib * okm;
+        c = apz * rib * okp;
       }
-      a = am;
-      c = ap;
       b = 1.0 + sum * rib;
-      d = rhs * rib + var[id] * gh;
-    }
-    se.push(m, a, b, c, d);
-  }
-  double XL, XS;
-  seg_reduced_solve<double, LS>(sh, P, tx, seg, se, XL, XS);
-  if (!ok) return;
-  se.combine(XL, XS);
 #pragma unroll
-  for (int m = 0; m < LS; m++)
-    if (m0 + m < n) var[id0 + m * rst] = se.dp[m];
+      for (int q = 0; q < 3; q++) {
+        const double *var = F.var[q];
+        double rhs;
+        if (DIR == 0)
+          rhs = F.r[q][id] - (var[id - R.ex] * amy * ojm + var[id + R.ex] * apy * ojp +
+                              var[id - R.exy] * amz * okm + var[id + R.exy] * apz * okp);
+        else if (DIR == 1)
+          rhs = F.r[q][id] - (var[id - 1] * amx * oim + var[id + 1] * apx * oip + var[id - R.exy] * amz * okm +
+                              var[id + R.exy] * apz * okp);
+        else
+          rhs = F.r[q][id] - var[id - R.ex] * amy * ojm - var[id + R.ex] * apy * ojp - var[id - 1] * amx * oim -
+                var[id + 1] * apx * oip;
+        d[q] = rhs * rib + var[id] * gh;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) se[q].push(m, a, b, c, d[q]);
+  }
+  double XL[3], XS[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    if (q > 0) __syncthreads();  // the previous solve's last LDS reads
+    seg_reduced_solve<double, LS>(sh, P, tx, seg, se[q], XL[q], XS[q]);
+  }
+  if (!ok) return;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    se[q].combine(XL[q], XS[q]);
+#pragma unroll
+    for (int m = 0; m < LS; m++)
+      if (m0 + m < n) F.var[q][id0 + m * rst] = se[q].dp[m];
+  }
 }
 
 // calc_residual_ad for NV fields (u, v, w share the rows): per-block
