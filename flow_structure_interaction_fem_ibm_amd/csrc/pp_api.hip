@@ -2092,7 +2092,9 @@ int32_t pp_adjoint(pp_ctx *c, const double *wobj, double *lambda, double *dJ_dg,
 // implicit momentum solve (set_solve_ad, solve_ad, calc_residual_ad)
 // =====================================================================
 namespace {
-constexpr int kAdLS = 4;  // cells per thread of a line segment
+// cells per thread of a line segment (x-lines: 2 and 8 measured slower,
+// 31 and 39 ms against 27.5 ms for the 256x192x192 momentum solve)
+constexpr int kAdLS = 4, kAdLSx = 4;
 
 FracMetrics ref_metrics(const pp_ctx *c) {
   FracMetrics M;
@@ -2106,10 +2108,11 @@ struct AdLineLaunch {
   int nlines;
 };
 // TX lines x P segments per block, TX a power of two up to 32 with at most
-// 512 threads (one segment per thread: P = ceil(n / kAdLS) <= 128)
+// 512 threads (one segment per thread: P = ceil(n / LS) <= 256)
 AdLineLaunch ad_line_launch(const pp_ctx *c, int dir) {
   const int n = dir == 0 ? c->nxc : (dir == 1 ? c->nyc : c->nzc);
-  const int P = (n + kAdLS - 1) / kAdLS;
+  const int ls = dir == 0 ? kAdLSx : kAdLS;
+  const int P = (n + ls - 1) / ls;
   int tx = 1;
   while (tx < 32 && 2 * tx * P <= 512) tx *= 2;
   AdLineLaunch L;
@@ -2150,7 +2153,7 @@ void ad_sweeps(pp_ctx *c, const double *row, AdLineFields F) {
     AdLineLaunch L = ad_line_launch(c, dir);
     for (int color = 0; color < 2; color++) {
       if (dir == 0)
-        k_ad_line<0, kAdLS><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, F, color, L.nlines);
+        k_ad_line<0, kAdLSx><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, F, color, L.nlines);
       else if (dir == 1)
         k_ad_line<1, kAdLS><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, F, color, L.nlines);
       else
