@@ -2117,7 +2117,7 @@ AdLineLaunch ad_line_launch(const pp_ctx *c, int dir) {
   while (tx < 32 && 2 * tx * P <= 512) tx *= 2;
   AdLineLaunch L;
   L.block = dim3(tx, P);
-  L.lds = sizeof(double) * 8 * (size_t)P * tx;
+  L.lds = sizeof(double) * 12 * (size_t)P * tx;  // seg_reduced_solve_n with 3 right-hand sides
   if (dir == 0) {
     L.nlines = ((c->nyc + 1) / 2) * c->nzc;
     L.grid = dim3((L.nlines + tx - 1) / tx);

@@ -2075,6 +2075,122 @@ __global__ void __launch_bounds__(NT) k_ad_rows(RefArgs R, LevelArgs<double> L0,
   row[6 * nref + id] = ac[0] + ac[1] + ac[2];
 }
 
+// SegElim for NR right-hand sides of one tridiagonal matrix: the
+// elimination of the matrix (cp, fp) is done once, each right-hand side
+// follows it with exactly the operations SegElim applies to its own.
+template <typename T, int S, int NR>
+struct SegElimN {
+  T cp[S], fp[S], dp[NR][S];
+  __device__ inline void push(int m, T a, T b, T c, const T (&d)[NR]) {
+    if (m == 0) {
+      T inv = fast_rcp(b);
+      cp[0] = c * inv;
+#pragma unroll
+      for (int q = 0; q < NR; q++) dp[q][0] = d[q] * inv;
+      fp[0] = -a * inv;
+    } else {
+      T inv = fast_rcp(b - a * cp[m - 1]);
+      cp[m] = c * inv;
+#pragma unroll
+      for (int q = 0; q < NR; q++) dp[q][m] = (d[q] - a * dp[q][m - 1]) * inv;
+      fp[m] = -a * fp[m - 1] * inv;
+    }
+  }
+  __device__ inline void back(T &cl, T (&dl)[NR], T &fl) {
+    cl = cp[S - 1]; fl = fp[S - 1];
+    T Dn[NR], Fn = T(0), Gn = T(1);
+#pragma unroll
+    for (int q = 0; q < NR; q++) { dl[q] = dp[q][S - 1]; Dn[q] = T(0); }
+#pragma unroll
+    for (int m = S - 2; m >= 0; m--) {
+      T c = cp[m];
+#pragma unroll
+      for (int q = 0; q < NR; q++) { Dn[q] = dp[q][m] - c * Dn[q]; dp[q][m] = Dn[q]; }
+      Fn = fp[m] - c * Fn;
+      Gn = -c * Gn;
+      fp[m] = Fn; cp[m] = Gn;
+    }
+#pragma unroll
+    for (int q = 0; q < NR; q++) dp[q][S - 1] = T(0);
+    fp[S - 1] = T(0); cp[S - 1] = T(1);
+  }
+  __device__ inline void combine(int q, T XL, T XS) {
+#pragma unroll
+    for (int m = 0; m < S - 1; m++) dp[q][m] = dp[q][m] + fp[m] * XL + cp[m] * XS;
+    dp[q][S - 1] = XS;
+  }
+};
+
+// seg_reduced_solve for SegElimN: the reduced matrix (A, B, C) once, NR
+// right-hand sides R; LDS [buf][seg][field][tx] with 3 + NR fields
+// (2 * (3 + NR) * P * blockDim.x elements).  Every thread of the block must
+// call it.
+template <typename T, int LS, int NR>
+__device__ inline void seg_reduced_solve_n(T *sh, int P, int tx, int seg, SegElimN<T, LS, NR> &se, T (&XL)[NR],
+                                           T (&XS)[NR]) {
+  constexpr int NF = 3 + NR;
+  T cl, dl[NR], fl;
+  se.back(cl, dl, fl);
+  const int TXb = blockDim.x;
+  T *bufA = sh, *bufB = sh + NF * TXb * P;
+  auto E_ = [&](T *buf, int s, int fld) -> T & { return buf[((long)s * NF + fld) * TXb + tx]; };
+#pragma unroll
+  for (int q = 0; q < NR; q++) E_(bufB, seg, q) = se.dp[q][0];
+  E_(bufB, seg, NR) = se.fp[0]; E_(bufB, seg, NR + 1) = se.cp[0];
+  __syncthreads();
+  T D0n[NR], F0n = T(0), G0n = T(0);
+#pragma unroll
+  for (int q = 0; q < NR; q++) D0n[q] = T(0);
+  if (seg + 1 < P) {
+#pragma unroll
+    for (int q = 0; q < NR; q++) D0n[q] = E_(bufB, seg + 1, q);
+    F0n = E_(bufB, seg + 1, NR); G0n = E_(bufB, seg + 1, NR + 1);
+  }
+  T A = -fl, B = T(1) + cl * F0n, C = cl * G0n, R[NR];
+#pragma unroll
+  for (int q = 0; q < NR; q++) R[q] = dl[q] - cl * D0n[q];
+  T *cur = bufA, *nxt = bufB;
+  E_(cur, seg, 0) = A; E_(cur, seg, 1) = B; E_(cur, seg, 2) = C;
+#pragma unroll
+  for (int q = 0; q < NR; q++) E_(cur, seg, 3 + q) = R[q];
+  __syncthreads();
+  for (int d = 1; d < P; d <<= 1) {
+    T Am = T(0), Bm = T(1), Cm = T(0), Ap = T(0), Bp = T(1), Cp = T(0), Rm[NR], Rp[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) { Rm[q] = T(0); Rp[q] = T(0); }
+    if (seg - d >= 0) {
+      Am = E_(cur, seg - d, 0); Bm = E_(cur, seg - d, 1); Cm = E_(cur, seg - d, 2);
+#pragma unroll
+      for (int q = 0; q < NR; q++) Rm[q] = E_(cur, seg - d, 3 + q);
+    }
+    if (seg + d < P) {
+      Ap = E_(cur, seg + d, 0); Bp = E_(cur, seg + d, 1); Cp = E_(cur, seg + d, 2);
+#pragma unroll
+      for (int q = 0; q < NR; q++) Rp[q] = E_(cur, seg + d, 3 + q);
+    }
+    T k1 = A * fast_rcp(Bm), k2 = C * fast_rcp(Bp);
+    T nA = -Am * k1, nC = -Cp * k2;
+    T nB = B - Cm * k1 - Ap * k2;
+#pragma unroll
+    for (int q = 0; q < NR; q++) R[q] = R[q] - Rm[q] * k1 - Rp[q] * k2;
+    A = nA; B = nB; C = nC;
+    E_(nxt, seg, 0) = A; E_(nxt, seg, 1) = B; E_(nxt, seg, 2) = C;
+#pragma unroll
+    for (int q = 0; q < NR; q++) E_(nxt, seg, 3 + q) = R[q];
+    __syncthreads();
+    T *t = cur; cur = nxt; nxt = t;
+  }
+  const T rB = fast_rcp(B);
+#pragma unroll
+  for (int q = 0; q < NR; q++) {
+    XS[q] = R[q] * rB;
+    E_(nxt, seg, q) = XS[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NR; q++) XL[q] = seg > 0 ? E_(nxt, seg - 1, q) : T(0);
+}
+
 // One zebra pass of the momentum line sweeps along DIR (0 x, 1 y, 2 z)
 // for u, v and w together: the three share the row coefficients, which are
 // read once, and are independent until the ghost update, so solving them
@@ -2082,8 +2198,9 @@ __global__ void __launch_bounds__(NT) k_ad_rows(RefArgs R, LevelArgs<double> L0,
 // parity of the sum of their two transverse indices), so every off-line
 // neighbour has the other colour and the pass is deterministic.  A block
 // of TX lines x P segments of LS cells; each thread eliminates its segment
-// of each field in registers and the reduced system of a line is solved by
-// cyclic reduction in LDS (seg_reduced_solve, once per field).  For
+// of the three fields in registers (one elimination of the shared matrix)
+// and the reduced system of a line is solved by cyclic reduction in LDS
+// with three right-hand sides (seg_reduced_solve_n).  For
 // x-lines the segments of one line are adjacent lanes (coalesced along the
 // row); for y- and z-lines adjacent lanes are every other i.
 // Row of a cell (itsolv_ad_x/y/z): off-line terms subtracted from r in the
@@ -2129,7 +2246,7 @@ __global__ void __launch_bounds__(512) k_ad_line(RefArgs R, LevelArgs<double> L0
     id0 = ridx(R, i0, j0, k0);
     la0 = lidx(L0.sx, L0.sxy, i0, j0, k0);
   }
-  SegElim<double, LS> se[3];
+  SegElimN<double, LS, 3> se;
 #pragma unroll
   for (int m = 0; m < LS; m++) {
     double a = 0.0, b = 1.0, c = 0.0, d[3] = {0.0, 0.0, 0.0};
@@ -2170,22 +2287,17 @@ __global__ void __launch_bounds__(512) k_ad_line(RefArgs R, LevelArgs<double> L0
         d[q] = rhs * rib + var[id] * gh;
       }
     }
-#pragma unroll
-    for (int q = 0; q < 3; q++) se[q].push(m, a, b, c, d[q]);
+    se.push(m, a, b, c, d);
   }
   double XL[3], XS[3];
-#pragma unroll
-  for (int q = 0; q < 3; q++) {
-    if (q > 0) __syncthreads();  // the previous solve's last LDS reads
-    seg_reduced_solve<double, LS>(sh, P, tx, seg, se[q], XL[q], XS[q]);
-  }
+  seg_reduced_solve_n<double, LS, 3>(sh, P, tx, seg, se, XL, XS);
   if (!ok) return;
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    se[q].combine(XL[q], XS[q]);
+    se.combine(q, XL[q], XS[q]);
 #pragma unroll
     for (int m = 0; m < LS; m++)
-      if (m0 + m < n) F.var[q][id0 + m * rst] = se[q].dp[m];
+      if (m0 + m < n) F.var[q][id0 + m * rst] = se.dp[q][m];
   }
 }
 
