@@ -3,6 +3,7 @@ short markdown, grouped by kernel and grid size (one grid size per
 multigrid level).
 
   python -m flow_structure_interaction_fem_ibm_amd.profile_summary trace <kernel_trace.csv> [--last-solve]
+      [--match SUBSTRING ...]
   python -m flow_structure_interaction_fem_ibm_amd.profile_summary pmc <counter_collection.csv> [...more csv]
   python -m flow_structure_interaction_fem_ibm_amd.profile_summary traffic <fetch.csv> <write.csv> \
       --copy-mib 512 --out profiles/traffic.json
@@ -20,13 +21,18 @@ def _short(name):
     return n
 
 
-def trace(path, last_solve=False, top=30):
+def trace(path, last_solve=False, top=30, match=None):
+    """Per-kernel table of a kernel trace; last_solve keeps the kernels from
+    the last solve's first kernel on, match (substrings) keeps the kernels
+    whose name holds one of them."""
     rows = list(csv.DictReader(open(path)))
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     if last_solve:
         starts = [i for i, r in enumerate(rows) if "k_mask_rhs" in r["Kernel_Name"]]
         if starts:
             rows = rows[starts[-1]:]
+    if match:
+        rows = [r for r in rows if any(m in r["Kernel_Name"] for m in match)]
     agg = collections.defaultdict(lambda: [0, 0.0])
     for r in rows:
         key = (_short(r["Kernel_Name"]), f'{r["Grid_Size_X"]}x{r["Grid_Size_Y"]}x{r["Grid_Size_Z"]}',
@@ -109,6 +115,7 @@ def main():
     ap.add_argument("kind", choices=("trace", "pmc", "traffic"))
     ap.add_argument("paths", nargs="+")
     ap.add_argument("--last-solve", action="store_true")
+    ap.add_argument("--match", nargs="*", default=None, help="trace: keep kernels whose name holds one of these")
     ap.add_argument("--json", default=None, help="pmc: also write the per-dispatch figures as JSON")
     ap.add_argument("--copy-mib", type=int, default=512, help="traffic: size of the calibration copy")
     ap.add_argument("--out", default=None, help="traffic: output JSON path")
@@ -121,7 +128,7 @@ def main():
         print(text)
         return
     if args.kind == "trace":
-        print(trace(args.paths[0], args.last_solve))
+        print(trace(args.paths[0], args.last_solve, match=args.match))
     else:
         text, res = pmc(args.paths)
         print(text)
