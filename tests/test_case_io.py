@@ -1,0 +1,126 @@
+"""The case-file readers and writers (case_io.py): input.dat in the record
+order of read_inputs (MAIN.F90:398-517) with the list-directed forms the
+reference's decks use, the grid files and the uniform-grid rule of make_grid
+(CORE_GRID.F90:36-80), and the ELLIPSOID bodies of canonical_body_in.dat
+(MAIN.F90:915-1166).  The reference ships no case files, so the files here
+are written record by record from those READ statements; a case read back
+builds the same geometry as the bodies given directly.  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+from flow_structure_interaction_fem_ibm_amd import case_io, pp
+
+
+def default_values(**kw):
+    v = {n: {"i": 1, "r": 0.5, "l": False}[k] for n, k in case_io.INPUT_KINDS.items()}
+    v.update(kw)
+    return v
+
+
+def test_input_dat_round_trip_and_record_count():
+    # 120 READs, 49 of them with items, 133 items in all
+    assert sum(1 for r in case_io.INPUT_DAT if r) == 49
+    assert len(case_io.INPUT_KINDS) == 133
+    v = default_values(nx=65, ny=49, nz=41, re=1234.5, dt=2.5e-4, advec_scheme=2, Full_Coarsening=True,
+                       restol_Poisson=1e-9, pp_solver_type=3)
+    p = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"input_{os.getpid()}.dat")
+    case_io.write_input_dat(p, v)
+    try:
+        assert case_io.read_input_dat(p) == v
+    finally:
+        os.remove(p)
+
+
+def test_input_dat_list_directed_forms(tmp_path):
+    v = default_values(nx=64, ny=48, nz=40, re=100.0, dt=0.01, Full_Coarsening=False,
+                       cure_pressure_oscillations=False, pbcx1=2, pppx1=7.0)
+    p = tmp_path / "input.dat"
+    case_io.write_input_dat(str(p), v)
+    lines = p.read_text().splitlines()
+
+    def at(name):
+        return next(q for q, l in enumerate(lines) if not l.startswith("#") and
+                    l == " ".join(case_io._fmt(v[n], k) for n, k in
+                                  next(r for r in case_io.INPUT_DAT if r and r[0][0] == name)))
+    # a READ continues on the next record until its items are filled
+    q = at("nx")
+    lines[q:q + 1] = ["64,", "  48 , 40"]
+    # repeat counts, D exponents, a comma after blanks, extra values dropped
+    lines[at("uinit")] = "3*0.0, 1.5"
+    lines[at("re")] = "1.0D2 1.0d-2 99 junk"
+    # logicals in both forms; a trailing comment after the values
+    lines[at("Full_Coarsening")] = ".TRUE., f   ! full coarsening on"
+    # a slash ends the record: pppx1 is not read
+    lines[at("pbcx1")] = "2 /"
+    p.write_text("\n".join(lines) + "\n")
+    r = case_io.read_input_dat(str(p))
+    assert (r["nx"], r["ny"], r["nz"]) == (64, 48, 40)
+    assert (r["uinit"], r["vinit"], r["winit"], r["vper"]) == (0.0, 0.0, 0.0, 1.5)
+    assert (r["re"], r["dt"]) == (100.0, 0.01)
+    assert r["Full_Coarsening"] is True and r["cure_pressure_oscillations"] is False
+    assert r["pbcx1"] == 2 and "pppx1" not in r
+    # everything after the changed records is still read in order
+    assert r["evaluationInterval"] == 1 and r["errorPermitted"] == 0.5
+    # a deck that ends early says where
+    p.write_text("\n".join(lines[:40]) + "\n")
+    with pytest.raises(ValueError, match="record"):
+        case_io.read_input_dat(str(p))
+
+
+def test_grid_files_and_uniform_rule(tmp_path):
+    xs = pp.tanh_nodes(64, 4.0, 2.0)
+    case_io.write_grid_dat(str(tmp_path / "ygrid.dat"), xs)
+    assert np.array_equal(case_io.read_grid_dat(str(tmp_path / "ygrid.dat"), len(xs)), xs)
+    v = default_values(nx=33, ny=len(xs), nz=17, xgrid_unif=case_io.UNIFORM_GRID, xout=4.0,
+                       ygrid_unif=case_io.NONUNIFORM_GRID, yout=4.0, zgrid_unif=case_io.UNIFORM_GRID, zout=1.6)
+    x, y, z = case_io.case_nodes(str(tmp_path), v)
+    assert np.array_equal(y, xs)
+    # make_grid: x(i) = (i-1) * (xout/(nx-1)), evaluated in that order
+    assert x[0] == 0.0 and x[-1] == 4.0 and np.array_equal(np.diff(x), np.full(32, 0.125))
+    assert np.allclose(z, np.linspace(0, 1.6, 17), rtol=0, atol=1e-15) and z[1] == 1.6 / 16
+
+
+def test_ellipsoid_n_phi_closed_forms():
+    # even n_theta: IDNINT(n_theta/2 * Aratio); odd: IDNINT((n_theta-1)/2 * Aratio) + 1
+    assert case_io.ellipsoid_n_phi(96, 0.5, 0.25) == 24
+    assert case_io.ellipsoid_n_phi(25, 0.3, 0.3) == 13
+    assert case_io.ellipsoid_n_phi(10, 0.4, 0.5) == 6       # 5 * 1.25 = 6.25
+    assert case_io.ellipsoid_n_phi(10, 1.0, 0.5) == 3       # 2.5 rounds away from zero
+    assert case_io.ellipsoid_n_phi(10, 1.0, 0.5, boundary_formulation=1) == 5
+
+
+def test_case_directory_builds_the_same_geometry(tmp_path):
+    nodes = (pp.tanh_nodes(40, 1.0, 1.2), pp.tanh_nodes(32, 1.0, 0.9), pp.tanh_nodes(36, 1.0, 1.5))
+    bodies = [pp.Ellipsoid((0.5, 0.47, 0.53), (0.23, 0.2, 0.25), 30.0, 24),
+              pp.Ellipsoid((0.2, 0.2, 0.25), (0.08, 0.08, 0.08), 0.0, 16)]
+    v = default_values(nx=len(nodes[0]), ny=len(nodes[1]), nz=len(nodes[2]), xgrid_unif=2, ygrid_unif=2,
+                       zgrid_unif=2, xout=1.0, yout=1.0, zout=1.0, internal_boundary_present=1, body_type=2,
+                       boundary_formulation=2, probeLengthNormalized=2.0)
+    case_io.write_input_dat(str(tmp_path / "input.dat"), v)
+    for name, x in zip(("xgrid.dat", "ygrid.dat", "zgrid.dat"), nodes):
+        case_io.write_grid_dat(str(tmp_path / name), x)
+    case_io.write_canonical_bodies(str(tmp_path / "canonical_body_in.dat"), [
+        {"n_theta": b.n_theta, "radiusx": b.semi[0], "radiusy": b.semi[1], "radiusz": b.semi[2],
+         "xcent": b.center[0], "ycent": b.center[1], "zcent": b.center[2], "alpha": b.alpha_deg} for b in bodies])
+    head, read = case_io.read_canonical_bodies(str(tmp_path / "canonical_body_in.dat"))
+    assert head["nbody"] == 2 and head["zoneSeparate"] is False
+    assert [b["n_phi"] for b in read] == [case_io.ellipsoid_n_phi(24, 0.23, 0.25),
+                                          case_io.ellipsoid_n_phi(16, 0.08, 0.08)]
+    xn, yn, zn = case_io.case_nodes(str(tmp_path))
+    got = case_io.case_bodies(str(tmp_path))
+    for a, b in zip((xn, yn, zn), nodes):
+        assert np.array_equal(a, b)
+    for mode in (pp.PP_GEOM_ANALYTIC, pp.PP_GEOM_TRIANGULATED):
+        ib1, fr1, gs1, ex1 = pp.build_geometry(xn, yn, zn, got, mode=mode)
+        ib2, fr2, gs2, ex2 = pp.build_geometry(*nodes, bodies, mode=mode)
+        assert np.array_equal(ib1, ib2) and gs1.n == gs2.n
+        assert np.array_equal(gs1.coeff, gs2.coeff) and np.array_equal(ex1["coeffD"], ex2["coeffD"])
+
+
+def test_other_body_types_are_named(tmp_path):
+    p = tmp_path / "canonical_body_in.dat"
+    p.write_text("1 1 0 0 F F\n1 2 0\n0\n")
+    with pytest.raises(NotImplementedError, match="canonical_body_type 1"):
+        case_io.read_canonical_bodies(str(p))
