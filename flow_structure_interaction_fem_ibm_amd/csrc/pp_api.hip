@@ -2258,9 +2258,14 @@ int32_t pp_solve_ad_device(pp_ctx *c, int32_t advec_scheme, double *d_u, double 
   hipStream_t s = c->stream;
   DevTemp rows;
   HIPCHK(rows.alloc(sizeof(double) * 7 * (size_t)c->nref));
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  // the call's timing events, destroyed on every return path
+  struct Events {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events() { if (a) hipEventDestroy(a); if (b) hipEventDestroy(b); }
+  } ev;
+  HIPCHK(hipEventCreate(&ev.a));
+  HIPCHK(hipEventCreate(&ev.b));
+  hipEvent_t e0 = ev.a, e1 = ev.b;
   HIPCHK(hipEventRecord(e0, s));
   ad_build_rows(c, half_dt, d_fu, d_fv, d_fw, rows.d());
   double *vars[3] = {d_u, d_v, d_w};
@@ -2290,8 +2295,6 @@ int32_t pp_solve_ad_device(pp_ctx *c, int32_t advec_scheme, double *d_u, double 
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
   if (rc) return rc;
   const int32_t status = maxres > restol ? PP_NOT_CONVERGED : PP_OK;
   if (st) {
