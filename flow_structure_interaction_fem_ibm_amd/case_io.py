@@ -11,10 +11,12 @@ handed back to it):
   n_phi from n_theta and the aspect ratio.
 
 Fortran list-directed input is restated for what these files use: values
-separated by blanks and/or one comma, r*value repeat counts, a slash ending
-the record (later items keep their previous value), D exponents, logical
-T/F/.TRUE./.FALSE.; a READ takes further records until its items are filled
-and drops the rest of its last record.
+separated by blanks and/or one comma, null values (two commas with only
+blanks between, a comma opening a record, or r* with no value: the item
+keeps its value), r*value repeat counts, a slash ending the record (later
+items keep their value), D exponents, logical T/F/.TRUE./.FALSE.; a READ
+takes further records until its items are filled and drops the rest of its
+last record.
 """
 from __future__ import annotations
 
@@ -27,7 +29,38 @@ import numpy as np
 # ---------------------------------------------------------------------------
 # list-directed input
 # ---------------------------------------------------------------------------
-_TOKEN = re.compile(r"\s*(?:,\s*)?([^\s,]+)")
+def _fields(line):
+    """The values of one record in order: strings, None for a null value,
+    "/" for a slash (the end of the READ); a "!" starts a comment."""
+    out, i, n = [], 0, len(line)
+    after_value = False   # a value was read since the last comma
+    while i < n:
+        ch = line[i]
+        if ch in " \t":
+            i += 1
+        elif ch == ",":
+            if not after_value:
+                out.append(None)
+            after_value = False
+            i += 1
+        elif ch == "/":
+            out.append("/")
+            break
+        elif ch == "!":
+            break
+        else:
+            j = i
+            while j < n and line[j] not in " \t,/":
+                j += 1
+            tok = line[i:j]
+            m = re.fullmatch(r"(\d+)\*(.*)", tok)
+            if m:
+                out.extend([m.group(2) or None] * int(m.group(1)))
+            else:
+                out.append(tok)
+            after_value = True
+            i = j
+    return out
 
 
 def _convert(tok, kind, name):
@@ -57,23 +90,13 @@ def read_record(lines, items, values=None):
         return out
     pos = 0
     while pos < len(want):
-        line = next(lines)
-        for m in _TOKEN.finditer(line):
-            tok = m.group(1)
-            if tok.startswith("/"):
+        for tok in _fields(next(lines)):
+            if tok == "/":
                 return out
-            if tok.startswith("!"):
-                break
-            rep = 1
-            if re.fullmatch(r"\d+\*.+", tok):
-                r_, tok = tok.split("*", 1)
-                rep = int(r_)
-            for _ in range(rep):
-                if pos >= len(want):
-                    break
-                name, kind = want[pos]
+            name, kind = want[pos]
+            if tok is not None:
                 out[name] = _convert(tok, kind, name)
-                pos += 1
+            pos += 1
             if pos >= len(want):
                 break
     return out

@@ -69,6 +69,23 @@ def test_input_dat_list_directed_forms(tmp_path):
         case_io.read_input_dat(str(p))
 
 
+def test_null_values_keep_the_items():
+    items = [("a", "i"), ("b", "r"), ("c", "l"), ("d", "i")]
+    prev = {"a": 7, "b": 2.5, "c": True, "d": 9}
+    # two commas: b is null; c, d read
+    v = case_io.read_record(iter(["1,, F 4"]), items, dict(prev))
+    assert v == {"a": 1, "b": 2.5, "c": False, "d": 4}
+    # a record opening with a comma: a is null; r* with no value: two nulls
+    v = case_io.read_record(iter([", 3.0 2*"]), items, dict(prev))
+    assert v == {"a": 7, "b": 3.0, "c": True, "d": 9}
+    # blanks around the comma are one separator, not a null
+    v = case_io.read_record(iter(["1 , 2.0 ,T,5"]), items, dict(prev))
+    assert v == {"a": 1, "b": 2.0, "c": True, "d": 5}
+    # the items left after a slash keep their values
+    v = case_io.read_record(iter(["0 /", "99"]), items, dict(prev))
+    assert v == {"a": 0, "b": 2.5, "c": True, "d": 9}
+
+
 def test_grid_files_and_uniform_rule(tmp_path):
     xs = pp.tanh_nodes(64, 4.0, 2.0)
     case_io.write_grid_dat(str(tmp_path / "ygrid.dat"), xs)
