@@ -8,7 +8,9 @@ handed back to it):
   (CORE_GRID.F90:46-80), and the uniform-grid rule x(i) = (i-1) xout/(nx-1);
 * canonical_body_in.dat: the header and the ELLIPSOID bodies of
   read_inputs_bound (MAIN.F90:915-1166), with the reference's marker count
-  n_phi from n_theta and the aspect ratio.
+  n_phi from n_theta and the aspect ratio;
+* unstruc_surface_in.dat: one triangulated surface per UNSTRUCTURED_SURFACE
+  body (BOUNDARY_MARKER_INIT.F90:209-248), for pp.build_geometry_surfaces.
 
 Fortran list-directed input is restated for what these files use: values
 separated by blanks and/or one comma, null values (two commas with only
@@ -310,3 +312,44 @@ def case_bodies(case_dir, vals=None):
                                       vals.get("boundary_formulation", GCM_METHOD))
     return [pp.Ellipsoid((b["xcent"], b["ycent"], b["zcent"]), (b["radiusx"], b["radiusy"], b["radiusz"]),
                          b["alpha"], b["n_theta"]) for b in bodies]
+
+
+# ---------------------------------------------------------------------------
+# unstruc_surface_in.dat (BOUNDARY_MARKER_INIT.F90:209-248), one body
+# ---------------------------------------------------------------------------
+def read_unstruc_surface(path_or_lines, zone_separate=False, channel_flow=False):
+    """(xyz (n_markers, 3), tri (n_elems, 3) 1-based, outside point): a
+    comment record, "nPtsBodyMarkerIn totNumTriElemIn", a comment record,
+    "i x y z" per marker (plus zoneMarker or gateLabel when zoneSeparate or
+    channel_flow), a comment record, "i n1 n2 n3" per element, a comment
+    record, pointOutsideBody.  A line iterator continues with the next body."""
+    lines = iter(open(path_or_lines).read().splitlines()) if isinstance(path_or_lines, str) else path_or_lines
+    read_record(lines, [])
+    h = read_record(lines, _spec("n:i ne:i"))
+    read_record(lines, [])
+    extra = " label:i" if (zone_separate or channel_flow) else ""
+    xyz = np.empty((h["n"], 3))
+    for m in range(h["n"]):
+        r = read_record(lines, _spec("i:i x:r y:r z:r" + extra))
+        xyz[m] = (r["x"], r["y"], r["z"])
+    read_record(lines, [])
+    tri = np.empty((h["ne"], 3), np.int32)
+    for e in range(h["ne"]):
+        r = read_record(lines, _spec("i:i a:i b:i c:i"))
+        tri[e] = (r["a"], r["b"], r["c"])
+    read_record(lines, [])
+    o = read_record(lines, _spec("x:r y:r z:r"))
+    return xyz, tri, (o["x"], o["y"], o["z"])
+
+
+def write_unstruc_surface(path, xyz, tri, outside):
+    xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
+    tri = np.asarray(tri).reshape(-1, 3)
+    out = ["# nPtsBodyMarker totNumTriElem", f"{len(xyz)} {len(tri)}", "# markers"]
+    out += [f"{m + 1} {x!r} {y!r} {z!r}" for m, (x, y, z) in enumerate(xyz.tolist())]
+    out.append("# elements")
+    out += [f"{e + 1} {a} {b} {c}" for e, (a, b, c) in enumerate(tri.tolist())]
+    out.append("# pointOutsideBody")
+    out.append(" ".join(repr(float(v)) for v in outside))
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")

@@ -327,31 +327,12 @@ struct pp_geom {
   int nblank = 0, nfresh = 0;
 };
 
-extern "C" {
-
-int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const pp_body_desc *bodies, int32_t mode,
-                      double probe_norm, const int8_t *iblank_prev) {
-  if (!out || !g || nbody < 0 || nbody > 256 || (nbody > 0 && !bodies) || probe_norm <= 0.0) {
-    pp::set_error("pp_geom_build: bad argument");
-    return PP_BAD_ARG;
-  }
-  *out = nullptr;
-  if (mode != PP_GEOM_ANALYTIC && mode != PP_GEOM_TRIANGULATED) {
-    pp::set_error("pp_geom_build: unknown mode %d", mode);
-    return PP_BAD_ARG;
-  }
-  Grid G(g->nxc, g->nyc, g->nzc, g->x, g->y, g->z);
-  std::vector<Body> bs(nbody);
-  for (int b = 0; b < nbody; b++) {
-    const pp_body_desc &d = bodies[b];
-    if (d.shape != PP_BODY_ELLIPSOID || d.semi[0] <= 0 || d.semi[1] <= 0 || d.semi[2] <= 0) {
-      pp::set_error("pp_geom_build: body %d is not a valid ellipsoid", b);
-      return PP_BAD_ARG;
-    }
-    for (int q = 0; q < 3; q++) { bs[b].c[q] = d.center[q]; bs[b].r[q] = d.semi[q]; }
-    bs[b].alpha_deg = d.alpha_deg;
-    bs[b].n_theta = d.n_theta;
-  }
+namespace {
+// the part of pp_geom_build after its bodies are known: canonical bodies
+// (bs) or, in the triangulated mode, caller-given surfaces (surf)
+int32_t build_geometry(pp_geom **out, const Grid &G, int32_t mode, const std::vector<Body> &bs,
+                       const std::vector<Surface> *surf, double probe_norm, const int8_t *iblank_prev) {
+  const int nbody = surf ? (int)surf->size() : (int)bs.size();
   pp_geom *o = new pp_geom();
   par_fill(o->iblank, G.size, (int8_t)0);
   par_fill(o->fresh, G.size, (int8_t)0);
@@ -394,7 +375,7 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
       }, kPlaneGrain);
     }
   } else {
-    rc = tri_build(G, bs, tri);
+    rc = surf ? tri_build_surfaces(*surf, tri) : tri_build(G, bs, tri);
     if (rc == 0) rc = tri_iblank(G, tri, o->iblank, bodynum);
     if (rc) {
       delete o;
@@ -469,6 +450,91 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
     for (int k = 0; k < G.nzc + 3; k++) { o->nblank += (int)nb[k]; o->nfresh += (int)nf[k]; }
   }
   *out = o;
+  return PP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const pp_body_desc *bodies, int32_t mode,
+                      double probe_norm, const int8_t *iblank_prev) {
+  if (!out || !g || nbody < 0 || nbody > 256 || (nbody > 0 && !bodies) || probe_norm <= 0.0) {
+    pp::set_error("pp_geom_build: bad argument");
+    return PP_BAD_ARG;
+  }
+  *out = nullptr;
+  if (mode != PP_GEOM_ANALYTIC && mode != PP_GEOM_TRIANGULATED) {
+    pp::set_error("pp_geom_build: unknown mode %d", mode);
+    return PP_BAD_ARG;
+  }
+  Grid G(g->nxc, g->nyc, g->nzc, g->x, g->y, g->z);
+  std::vector<Body> bs(nbody);
+  for (int b = 0; b < nbody; b++) {
+    const pp_body_desc &d = bodies[b];
+    if (d.shape != PP_BODY_ELLIPSOID || d.semi[0] <= 0 || d.semi[1] <= 0 || d.semi[2] <= 0) {
+      pp::set_error("pp_geom_build: body %d is not a valid ellipsoid", b);
+      return PP_BAD_ARG;
+    }
+    for (int q = 0; q < 3; q++) { bs[b].c[q] = d.center[q]; bs[b].r[q] = d.semi[q]; }
+    bs[b].alpha_deg = d.alpha_deg;
+    bs[b].n_theta = d.n_theta;
+  }
+  return build_geometry(out, G, mode, bs, nullptr, probe_norm, iblank_prev);
+}
+
+int32_t pp_geom_build_surfaces(pp_geom **out, const pp_grid_desc *g, int32_t n_surface, const pp_surface_desc *s,
+                               double probe_norm, const int8_t *iblank_prev) {
+  if (!out || !g || n_surface < 0 || n_surface > 256 || (n_surface > 0 && !s) || probe_norm <= 0.0) {
+    pp::set_error("pp_geom_build_surfaces: bad argument");
+    return PP_BAD_ARG;
+  }
+  *out = nullptr;
+  std::vector<Surface> sv(n_surface);
+  for (int b = 0; b < n_surface; b++) {
+    if (s[b].n_markers < 3 || s[b].n_elems < 1 || !s[b].xyz || !s[b].tri) {
+      pp::set_error("pp_geom_build_surfaces: surface %d has no markers or elements", b);
+      return PP_BAD_ARG;
+    }
+    sv[b].nm = s[b].n_markers;
+    sv[b].ne = s[b].n_elems;
+    sv[b].xyz = s[b].xyz;
+    sv[b].tri = s[b].tri;
+    for (int d = 0; d < 3; d++) sv[b].out[d] = s[b].outside[d];
+  }
+  Grid G(g->nxc, g->nyc, g->nzc, g->x, g->y, g->z);
+  return build_geometry(out, G, PP_GEOM_TRIANGULATED, std::vector<Body>(), &sv, probe_norm, iblank_prev);
+}
+
+int32_t pp_geom_canonical_surface(const pp_body_desc *d, int32_t *n_markers, int32_t *n_elems, double *xyz,
+                                  int32_t *tri) {
+  if (!d || !n_markers || !n_elems || d->shape != PP_BODY_ELLIPSOID || d->semi[0] <= 0 || d->semi[1] <= 0 ||
+      d->semi[2] <= 0) {
+    pp::set_error("pp_geom_canonical_surface: bad argument");
+    return PP_BAD_ARG;
+  }
+  std::vector<Body> bs(1);
+  for (int q = 0; q < 3; q++) { bs[0].c[q] = d->center[q]; bs[0].r[q] = d->semi[q]; }
+  bs[0].alpha_deg = d->alpha_deg;
+  bs[0].n_theta = d->n_theta;
+  const double x1[2] = {0.0, 1.0};
+  Grid G(1, 1, 1, x1, x1, x1);
+  TriBodies T;
+  if (tri_build(G, bs, T)) {
+    pp::set_error("pp_geom_canonical_surface: the body's surface cannot be built");
+    return PP_BAD_GEOMETRY;
+  }
+  const TriBody &B = T.b[0];
+  if (xyz && tri) {
+    if (*n_markers < B.nm || *n_elems < B.ne) {
+      pp::set_error("pp_geom_canonical_surface: arrays hold %d markers and %d elements, need %d and %d",
+                    *n_markers, *n_elems, B.nm, B.ne);
+      return PP_BAD_ARG;
+    }
+    for (int q = 0; q < B.nm; q++) { xyz[3 * q] = B.xm[q]; xyz[3 * q + 1] = B.ym[q]; xyz[3 * q + 2] = B.zm[q]; }
+    for (int q = 0; q < 3 * B.ne; q++) tri[q] = B.tri[q];
+  }
+  *n_markers = B.nm;
+  *n_elems = B.ne;
   return PP_OK;
 }
 

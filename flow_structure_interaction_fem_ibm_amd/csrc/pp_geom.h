@@ -96,9 +96,24 @@ struct TriBody {
 struct TriBodies {
   std::vector<TriBody> b;
 };
+// A triangulated surface given by the caller (unstruc_surface_in.dat):
+// nm markers (x, y, z each), ne elements (three 1-based marker ids each),
+// and the point outside the body that orients the normals.
+struct Surface {
+  int nm = 0, ne = 0;
+  const double *xyz = nullptr;
+  const int32_t *tri = nullptr;
+  double out[3] = {0.0, 0.0, 0.0};
+};
 // 0 or an error code: 6 bad body or the orientation point (the origin)
 // inside a body's bounding box
 int tri_build(const Grid &G, const std::vector<Body> &bodies, TriBodies &T);
+// caller-given surfaces instead of canonical bodies; 0, 6 (P inside a
+// body's bounding box, too few markers) or 8 (an element names a marker
+// out of range)
+int tri_build_surfaces(const std::vector<Surface> &surf, TriBodies &T);
+// orientation, normals, element lists and search tree of one surface
+int tri_finish(TriBody &B, const double *P);
 // iblank of set_iblank_canonical_body_fast and bodynum (0-based body, -1
 // none); 0 or an error code: 4 no unblocked marker among the 7 closest,
 // 5 no element selected

@@ -414,18 +414,34 @@ int tri_build(const Grid &G, const std::vector<Body> &bodies, TriBodies &T) {
     }
     for (int i = m - nt; i <= m - 1; i++) add(i, m, (i + 1 == m) ? m - nt : i + 1);
     B.ne = (int)B.tri.size() / 3;
-    // orientation: the element at the marker closest to the origin must have
-    // its normal pointing away from the origin
+    const double origin[3] = {0.0, 0.0, 0.0};
+    const int rc = tri_finish(B, origin);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// The rest of a body's surface set-up once its markers and elements are
+// known (the orientation step of determine_norm_dir_unstruc with the point
+// outside the body P): the element at the marker closest to P gets its
+// normal pointing away from P (every element flipped otherwise); element
+// normals, areas and centroids; the elements at each marker; the marker
+// search tree.  0, or 6 when P lies inside the markers' bounding box or the
+// surface is too small.
+int tri_finish(TriBody &B, const double *P) {
+  {
+    const int m = B.nm;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int q = 0; q < m; q++) {
       const double v[3] = {B.xm[q], B.ym[q], B.zm[q]};
       for (int d = 0; d < 3; d++) { lo[d] = std::min(lo[d], v[d]); hi[d] = std::max(hi[d], v[d]); }
     }
-    if (lo[0] <= 0.0 && hi[0] >= 0.0 && lo[1] <= 0.0 && hi[1] >= 0.0 && lo[2] <= 0.0 && hi[2] >= 0.0) return 6;
+    if (lo[0] <= P[0] && hi[0] >= P[0] && lo[1] <= P[1] && hi[1] >= P[1] && lo[2] <= P[2] && hi[2] >= P[2]) return 6;
     double dmin = 1.0e8;
     int cnode = 0;
     for (int q = 1; q <= m; q++) {
-      const double d = std::sqrt(B.xm[q - 1] * B.xm[q - 1] + B.ym[q - 1] * B.ym[q - 1] + B.zm[q - 1] * B.zm[q - 1]);
+      const double dx = B.xm[q - 1] - P[0], dy = B.ym[q - 1] - P[1], dz = B.zm[q - 1] - P[2];
+      const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
       if (d <= dmin) { dmin = d; cnode = q; }
     }
     auto raw_normal = [&](int e, double *n) {
@@ -445,7 +461,7 @@ int tri_build(const Grid &G, const std::vector<Body> &bodies, TriBodies &T) {
       const int a = B.tri[3 * celem] - 1, b = B.tri[3 * celem + 1] - 1, c = B.tri[3 * celem + 2] - 1;
       const double gx = (B.xm[a] + B.xm[b] + B.xm[c]) / 3.0, gy = (B.ym[a] + B.ym[b] + B.ym[c]) / 3.0,
                    gz = (B.zm[a] + B.zm[b] + B.zm[c]) / 3.0;
-      const double vp = (gx - 0.0) * n0[0] + (gy - 0.0) * n0[1] + (gz - 0.0) * n0[2];
+      const double vp = (gx - P[0]) * n0[0] + (gy - P[1]) * n0[1] + (gz - P[2]) * n0[2];
       if (vp < 0.0)
         for (int e = 0; e < B.ne; e++) std::swap(B.tri[3 * e + 1], B.tri[3 * e + 2]);
     }
@@ -476,6 +492,25 @@ int tri_build(const Grid &G, const std::vector<Body> &bodies, TriBodies &T) {
       }
     if (m < kSave) return 6;
     tree_build(B);
+  }
+  return 0;
+}
+
+int tri_build_surfaces(const std::vector<Surface> &surf, TriBodies &T) {
+  T.b.assign(surf.size(), TriBody());
+  for (size_t bi = 0; bi < surf.size(); bi++) {
+    const Surface &S = surf[bi];
+    TriBody &B = T.b[bi];
+    if (S.nm < 3 || S.ne < 1) return 6;
+    B.nm = S.nm;
+    B.ne = S.ne;
+    B.xm.resize(S.nm); B.ym.resize(S.nm); B.zm.resize(S.nm);
+    for (int q = 0; q < S.nm; q++) { B.xm[q] = S.xyz[3 * q]; B.ym[q] = S.xyz[3 * q + 1]; B.zm[q] = S.xyz[3 * q + 2]; }
+    B.tri.assign(S.tri, S.tri + 3L * S.ne);
+    for (int v : B.tri)
+      if (v < 1 || v > S.nm) return 8;
+    const int rc = tri_finish(B, S.out);
+    if (rc) return rc;
   }
   return 0;
 }

@@ -72,6 +72,14 @@ module pp_hip_mod
      integer(c_int32_t) :: kern_launches_l0(PP_K_CLASSES)
   end type pp_solve_stats
 
+  ! one UNSTRUCTURED_SURFACE body of unstruc_surface_in.dat: xBodyMarker.. as
+  ! (3, nPtsBodyMarker), triElemNeig as (3, totNumTriElem), pointOutsideBody
+  type, bind(c), public :: pp_surface_desc
+     integer(c_int32_t) :: n_markers, n_elems
+     type(c_ptr) :: xyz, tri
+     real(c_double) :: outside(3)
+  end type pp_surface_desc
+
   ! solve_ad: iterations, max(maxresu, maxresv, maxresw), the three, loc
   type, bind(c), public :: pp_ad_stats
      integer(c_int32_t) :: status, iterations
@@ -86,7 +94,8 @@ module pp_hip_mod
        pp_smooth, pp_null_vector, pp_level_info, pp_error_text, pp_geom_build, pp_geom_counts, pp_geom_get, &
        pp_geom_ghost_elements, pp_geom_destroy, pp_geom_dirichlet, pp_set_vel_ghosts, pp_correct_vel, &
        pp_correct_vel_device, pp_update_pressure, pp_update_pressure_device, pp_set_solve_ad, &
-       pp_set_solve_ad_device, pp_solve_ad, pp_solve_ad_device, pp_calc_residual_ad
+       pp_set_solve_ad_device, pp_solve_ad, pp_solve_ad_device, pp_calc_residual_ad, pp_geom_build_surfaces, &
+       pp_geom_canonical_surface
 
   interface
      subroutine pp_default_opts(o) bind(c, name="pp_default_opts")
@@ -329,6 +338,28 @@ module pp_hip_mod
        integer(c_int32_t), intent(out) :: loc(3)
        integer(c_int32_t) :: rc
      end function pp_calc_residual_ad
+
+     function pp_geom_build_surfaces(geo, g, n_surface, surfaces, probe_length_normalized, iblank_prev) &
+          bind(c, name="pp_geom_build_surfaces") result(rc)
+       import :: c_ptr, c_int32_t, c_double, pp_grid_desc, pp_surface_desc
+       type(c_ptr), intent(out) :: geo
+       type(pp_grid_desc), intent(in) :: g
+       integer(c_int32_t), value :: n_surface
+       type(pp_surface_desc), intent(in) :: surfaces(*)
+       real(c_double), value :: probe_length_normalized
+       type(c_ptr), value :: iblank_prev
+       integer(c_int32_t) :: rc
+     end function pp_geom_build_surfaces
+
+     ! counts with xyz = tri = C_NULL_PTR, then the arrays
+     function pp_geom_canonical_surface(body, n_markers, n_elems, xyz, tri) &
+          bind(c, name="pp_geom_canonical_surface") result(rc)
+       import :: c_ptr, c_int32_t, pp_body_desc
+       type(pp_body_desc), intent(in) :: body
+       integer(c_int32_t), intent(inout) :: n_markers, n_elems
+       type(c_ptr), value :: xyz, tri
+       integer(c_int32_t) :: rc
+     end function pp_geom_canonical_surface
 
      function pp_geom_destroy(geo) bind(c, name="pp_geom_destroy") result(rc)
        import :: c_ptr, c_int32_t

@@ -62,6 +62,11 @@ class BodyDesc(C.Structure):
                 ("alpha_deg", C.c_double), ("n_theta", C.c_int32)]
 
 
+class SurfaceDesc(C.Structure):
+    _fields_ = [("n_markers", C.c_int32), ("n_elems", C.c_int32), ("xyz", C.c_void_p), ("tri", C.c_void_p),
+                ("outside", C.c_double * 3)]
+
+
 PP_BODY_ELLIPSOID = 3
 PP_GEOM_ANALYTIC = 0
 PP_GEOM_TRIANGULATED = 1
@@ -160,6 +165,9 @@ def lib():
     L.pp_geom_ghost_elements.argtypes = [P, P, P]
     L.pp_geom_destroy.argtypes = [P]
     L.pp_geom_dirichlet.argtypes = [P, P]
+    L.pp_geom_build_surfaces.argtypes = [C.POINTER(C.c_void_p), C.POINTER(GridDesc), C.c_int32,
+                                         C.POINTER(SurfaceDesc), C.c_double, P]
+    L.pp_geom_canonical_surface.argtypes = [C.POINTER(BodyDesc), C.POINTER(C.c_int32), C.POINTER(C.c_int32), P, P]
     L.pp_set_vel_ghosts.argtypes = [P, P, C.c_double, C.c_double]
     L.pp_correct_vel.argtypes = [P, C.c_double] + [P] * 11
     L.pp_correct_vel_device.argtypes = [P, C.c_double] + [P] * 11
@@ -177,7 +185,7 @@ def lib():
                  "pp_geom_destroy", "pp_geom_dirichlet", "pp_set_vel_ghosts", "pp_correct_vel",
                  "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device", "pp_adjoint_device",
                  "pp_set_solve_ad", "pp_set_solve_ad_device", "pp_solve_ad", "pp_solve_ad_device",
-                 "pp_calc_residual_ad"):
+                 "pp_calc_residual_ad", "pp_geom_build_surfaces", "pp_geom_canonical_surface"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -190,7 +198,7 @@ EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_s
             "pp_geom_ghost_elements", "pp_geom_destroy", "pp_geom_dirichlet", "pp_set_vel_ghosts",
             "pp_correct_vel", "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device",
             "pp_adjoint_device", "pp_set_solve_ad", "pp_set_solve_ad_device", "pp_solve_ad", "pp_solve_ad_device",
-            "pp_calc_residual_ad")
+            "pp_calc_residual_ad", "pp_geom_build_surfaces", "pp_geom_canonical_surface")
 
 
 def _check(rc, allow=(PP_OK,)):
@@ -533,6 +541,51 @@ def build_geometry(xn, yn, zn, bodies, mode=PP_GEOM_ANALYTIC, probe_norm=2.0, ib
     h = C.c_void_p()
     _check(L.pp_geom_build(C.byref(h), C.byref(g), len(bodies), arr, mode, probe_norm,
                            None if prev is None else prev.ctypes.data))
+    return _geometry_result(h, xn, yn, zn)
+
+
+def build_geometry_surfaces(xn, yn, zn, surfaces, probe_norm=2.0, iblank_prev=None):
+    """set_boundary for UNSTRUCTURED_SURFACE bodies (pp_geom_build_surfaces):
+    surfaces = [(xyz (n_markers, 3), tri (n_elems, 3) 1-based, outside point)].
+    Returns what build_geometry returns."""
+    L = lib()
+    xn, yn, zn = (np.ascontiguousarray(a, dtype=np.float64) for a in (xn, yn, zn))
+    g = _grid_desc(xn, yn, zn)
+    keep = []
+    arr = (SurfaceDesc * max(len(surfaces), 1))()
+    for q, (xyz, tri, out) in enumerate(surfaces):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1, 3)
+        keep += [xyz, tri]
+        arr[q].n_markers, arr[q].n_elems = len(xyz), len(tri)
+        arr[q].xyz, arr[q].tri = xyz.ctypes.data, tri.ctypes.data
+        for d in range(3):
+            arr[q].outside[d] = out[d]
+    prev = None if iblank_prev is None else np.ascontiguousarray(iblank_prev, dtype=np.int8)
+    h = C.c_void_p()
+    _check(L.pp_geom_build_surfaces(C.byref(h), C.byref(g), len(surfaces), arr, probe_norm,
+                                    None if prev is None else prev.ctypes.data))
+    return _geometry_result(h, xn, yn, zn)
+
+
+def canonical_surface(body):
+    """The marker surface PP_GEOM_TRIANGULATED builds for one Ellipsoid:
+    (xyz (n_markers, 3), tri (n_elems, 3), 1-based marker ids)."""
+    L = lib()
+    d = BodyDesc()
+    d.shape = PP_BODY_ELLIPSOID
+    for q in range(3):
+        d.center[q], d.semi[q] = body.center[q], body.semi[q]
+    d.alpha_deg, d.n_theta = body.alpha_deg, body.n_theta
+    nm, ne = C.c_int32(0), C.c_int32(0)
+    _check(L.pp_geom_canonical_surface(C.byref(d), C.byref(nm), C.byref(ne), None, None))
+    xyz, tri = np.zeros((nm.value, 3)), np.zeros((ne.value, 3), np.int32)
+    _check(L.pp_geom_canonical_surface(C.byref(d), C.byref(nm), C.byref(ne), xyz.ctypes.data, tri.ctypes.data))
+    return xyz, tri
+
+
+def _geometry_result(h, xn, yn, zn):
+    L = lib()
     try:
         ng, nb, nf = C.c_int32(), C.c_int32(), C.c_int32()
         _check(L.pp_geom_counts(h, C.byref(ng), C.byref(nb), C.byref(nf)))

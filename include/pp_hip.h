@@ -386,6 +386,32 @@ typedef struct pp_geom pp_geom;
 int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t n_body,
                       const pp_body_desc *bodies, int32_t mode,
                       double probe_length_normalized, const int8_t *iblank_prev);
+
+/* One caller-given triangulated surface, as unstruc_surface_in.dat holds it
+ * (BOUNDARY_MARKER_INIT.F90:209-248): n_markers markers (x, y, z each),
+ * n_elems elements (triElemNeig: three 1-based marker ids each) and
+ * pointOutsideBody, the point that orients the element normals. */
+typedef struct pp_surface_desc {
+  int32_t n_markers;
+  int32_t n_elems;
+  const double *xyz;
+  const int32_t *tri;
+  double outside[3];
+} pp_surface_desc;
+
+/* pp_geom_build in PP_GEOM_TRIANGULATED mode for UNSTRUCTURED_SURFACE bodies
+ * (canonical_body_type 4, BOUNDARY_MARKER_INIT.F90:209-248 feeding the same
+ * set_boundary pipeline) instead of canonical ellipsoids. */
+int32_t pp_geom_build_surfaces(pp_geom **out, const pp_grid_desc *g,
+                               int32_t n_surface, const pp_surface_desc *s,
+                               double probe_norm, const int8_t *iblank_prev);
+
+/* The marker surface PP_GEOM_TRIANGULATED builds for one canonical body
+ * (BOUNDARY_MARKER_INIT.F90 ELLIPSOID branch, oriented against the origin),
+ * e.g. to write it as unstruc_surface_in.dat.  Call with xyz = tri = NULL
+ * for the counts, then with arrays of 3*n_markers and 3*n_elems. */
+int32_t pp_geom_canonical_surface(const pp_body_desc *body, int32_t *n_markers,
+                                  int32_t *n_elems, double *xyz, int32_t *tri);
 int32_t pp_geom_counts(const pp_geom *geo, int32_t *n_ghost, int32_t *n_blank,
                        int32_t *n_fresh);
 /* Any output pointer may be NULL.  coeff is coeffGCMN(8,n) column-major;

@@ -12,6 +12,7 @@ program layout_check
   type(pp_grid_desc) :: g
   type(pp_solve_stats) :: st
   type(pp_ad_stats) :: ast
+  type(pp_surface_desc) :: srf
   type(c_ptr) :: ctx
   integer(c_int32_t) :: rc
   real(c_double), target :: xs(3), xn(25)
@@ -38,6 +39,7 @@ program layout_check
        c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr)) / 8
   print '(A,I0)', 'bodysize ', storage_size(body(1)) / 8
   print '(A,I0)', 'adstatsize ', storage_size(ast) / 8
+  print '(A,I0)', 'surfsize ', storage_size(srf) / 8
   xn = [(1.2d0 * i / 24.0d0, i = 0, 24)]
   g%nxc = 24; g%nyc = 24; g%nzc = 24
   g%x = c_loc(xn); g%y = c_loc(xn); g%z = c_loc(xn)

@@ -141,3 +141,36 @@ def test_other_body_types_are_named(tmp_path):
     p.write_text("1 1 0 0 F F\n1 2 0\n0\n")
     with pytest.raises(NotImplementedError, match="canonical_body_type 1"):
         case_io.read_canonical_bodies(str(p))
+
+
+def test_unstructured_surface_builds_the_canonical_geometry(tmp_path):
+    """An ellipsoid's own marker surface written as unstruc_surface_in.dat
+    (pointOutsideBody at the origin, the point the canonical bodies are
+    oriented against) and read back gives the geometry of the canonical
+    body bit for bit; the surface and element counts are the reference's
+    (n_theta (n_phi - 2) + 2 markers, 2 n_theta (n_phi - 2) elements)."""
+    nodes = (pp.tanh_nodes(40, 1.0, 1.2), pp.tanh_nodes(32, 1.0, 0.9), pp.tanh_nodes(36, 1.0, 1.5))
+    body = pp.Ellipsoid((0.5, 0.47, 0.53), (0.23, 0.2, 0.25), 30.0, 24)
+    xyz, tri = pp.canonical_surface(body)
+    nphi = case_io.ellipsoid_n_phi(24, 0.23, 0.25)
+    assert len(xyz) == 24 * (nphi - 2) + 2 and len(tri) == 2 * 24 * (nphi - 2)
+    p = str(tmp_path / "unstruc_surface_in.dat")
+    case_io.write_unstruc_surface(p, xyz, tri, (0.0, 0.0, 0.0))
+    xyz2, tri2, out = case_io.read_unstruc_surface(p)
+    assert np.array_equal(xyz2, xyz) and np.array_equal(tri2, tri) and out == (0.0, 0.0, 0.0)
+    ib1, fr1, gs1, ex1 = pp.build_geometry_surfaces(*nodes, [(xyz2, tri2, out)])
+    ib2, fr2, gs2, ex2 = pp.build_geometry(*nodes, [body], mode=pp.PP_GEOM_TRIANGULATED)
+    assert np.array_equal(ib1, ib2) and gs1.n == gs2.n
+    for a, b in ((gs1.ig, gs2.ig), (gs1.ic, gs2.ic), (gs1.coeff, gs2.coeff), (ex1["coeffD"], ex2["coeffD"]),
+                 (ex1["elem"], ex2["elem"])):
+        assert np.array_equal(a, b)
+    # the elements listed in the opposite orientation are flipped back (the
+    # same cells; the normals' vertex order changes their last bits)
+    ib3, _, gs3, _ = pp.build_geometry_surfaces(*nodes, [(xyz2, tri2[:, ::-1], out)])
+    assert np.array_equal(ib3, ib2) and np.array_equal(gs3.ig, gs2.ig) and np.array_equal(gs3.ic, gs2.ic)
+    assert np.abs(gs3.coeff - gs2.coeff).max() < 1e-10  # O(1) weights; intercepts agree to 1e-15
+    # an element naming a marker that does not exist is refused
+    bad = tri2.copy()
+    bad[0, 0] = len(xyz2) + 1
+    with pytest.raises(pp.PPError):
+        pp.build_geometry_surfaces(*nodes, [(xyz2, bad, out)])

@@ -43,6 +43,7 @@ def test_fortran_module_matches_the_c_abi(tmp_path):
     assert sizes == [C.sizeof(pp.SolveOpts), C.sizeof(pp.GridDesc), C.sizeof(pp.SolveStats), C.sizeof(pp.Geometry)]
     assert int(out["bodysize"][0]) == C.sizeof(pp.BodyDesc)
     assert int(out["adstatsize"][0]) == C.sizeof(pp.ADStats)
+    assert int(out["surfsize"][0]) == C.sizeof(pp.SurfaceDesc)
     # the same geometry through the ctypes binding
     xn = 1.2 * np.arange(25) / 24.0
     body = pp.Ellipsoid((0.61, 0.58, 0.6), (0.3, 0.2, 0.22), 20.0, 40)
