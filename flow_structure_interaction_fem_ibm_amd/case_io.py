@@ -31,11 +31,15 @@ import numpy as np
 # ---------------------------------------------------------------------------
 # list-directed input
 # ---------------------------------------------------------------------------
-def _fields(line):
+def _fields(line, after_value=False):
     """The values of one record in order: strings, None for a null value,
-    "/" for a slash (the end of the READ); a "!" starts a comment."""
+    "/" for a slash (the end of the READ); a "!" starts a comment.
+
+    after_value carries the separator state of the READ across its records:
+    the end of a record acts as a blank, so a record that opens with a comma
+    after a record that ended in a value has one separator, not a null.
+    Returns (values, after_value at the end of the record)."""
     out, i, n = [], 0, len(line)
-    after_value = False   # a value was read since the last comma
     while i < n:
         ch = line[i]
         if ch in " \t":
@@ -62,7 +66,7 @@ def _fields(line):
                 out.append(tok)
             after_value = True
             i = j
-    return out
+    return out, after_value
 
 
 def _convert(tok, kind, name):
@@ -91,8 +95,10 @@ def read_record(lines, items, values=None):
         next(lines)  # READ(u,*) with no list skips one record
         return out
     pos = 0
+    after_value = False  # no value yet: a leading comma is a null
     while pos < len(want):
-        for tok in _fields(next(lines)):
+        toks, after_value = _fields(next(lines), after_value)
+        for tok in toks:
             if tok == "/":
                 return out
             name, kind = want[pos]

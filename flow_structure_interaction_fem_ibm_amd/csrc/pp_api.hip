@@ -407,13 +407,11 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
     LevelArgs<T> a = largs<T>(L);
     const CoarseSolve &cs = c->cs;
     int n = cs.n;
-    kbegin(c, PP_K_COARSE, l, (double)sizeof(T) * n * n + 2.0 * sizeof(T) * n);
-    if (sizeof(T) == 4)
-      k_coarse_mv<T, float><<<(n + 3) / 4, 256, sizeof(double) * n, st>>>(n, cs.cell, cs.G32, lv_rhs<T>(L),
-                                                                         lv_e<T>(L));
-    else
-      k_coarse_mv<T, double><<<(n + 3) / 4, 256, sizeof(double) * n, st>>>(n, cs.cell, cs.G, lv_rhs<T>(L),
-                                                                          lv_e<T>(L));
+    // the dense FP64 solution operator in both precisions (the coarsest
+    // solve is one of the FP64 parts of the mixed scheme)
+    kbegin(c, PP_K_COARSE, l, 8.0 * n * n + 2.0 * sizeof(T) * n);
+    k_coarse_mv<T, double><<<(n + 3) / 4, 256, sizeof(double) * n, st>>>(n, cs.cell, cs.G, lv_rhs<T>(L),
+                                                                        lv_e<T>(L));
     kend(c);
     return;
   }
@@ -2128,6 +2126,20 @@ AdLineLaunch ad_line_launch(const pp_ctx *c, int dir) {
   return L;
 }
 
+// Fresh cells (fluid now, blank at the previous geometry) take the
+// GCM_set_freshcell rows (coeffGCMFreshD, CORE_SET_SOLVERS.F90:175-200) and
+// the off-stencil corrections GCM_correct_rhs_ad / GCM_correct_res_ad
+// (GCM_FRESHCELL_AD.F90:2-137) in the reference; those rows are not built
+// here, so a geometry with fresh cells is refused rather than solved with the
+// ordinary rows.
+int ad_no_fresh(const pp_ctx *c, const char *who) {
+  if (c->h_fresh.empty()) return PP_OK;
+  set_error("%s: the geometry has fresh cells (pp_geometry.fresh_cell); their momentum rows "
+            "(GCM_set_freshcell, GCM_correct_rhs_ad/res_ad) are not supported: pass fresh_cell = NULL or a "
+            "geometry without fresh cells", who);
+  return PP_BAD_ARG;
+}
+
 int ad_half_dt(int32_t scheme, double dt, double *half_dt, const char *who) {
   if (scheme == PP_ADAMS_BASHFORTH2) { *half_dt = 0.0; return PP_OK; }
   if (scheme == PP_CRANK_NICOLSON1) { *half_dt = 0.5 * dt; return PP_OK; }
@@ -2205,6 +2217,7 @@ int32_t pp_set_solve_ad_device(pp_ctx *c, double dt, const double *d_visc, const
   if (!c || !d_visc || !d_bxv || !d_byv || !d_bzv) { set_error("pp_set_solve_ad: null argument"); return PP_BAD_ARG; }
   if (!c->have_geom) { set_error("pp_set_solve_ad: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
   if (!std::isfinite(dt)) { set_error("pp_set_solve_ad: dt must be finite"); return PP_BAD_ARG; }
+  if (int rf = ad_no_fresh(c, "pp_set_solve_ad")) return rf;
   HIPCHK(hipSetDevice(c->device));
   c->have_ad = false;
   const size_t nb = sizeof(double) * 6 * (size_t)c->nref;
@@ -2224,6 +2237,7 @@ int32_t pp_set_solve_ad(pp_ctx *c, double dt, const double *visc, const double *
                         const double *bzv) {
   if (!c || !visc || !bxv || !byv || !bzv) { set_error("pp_set_solve_ad: null argument"); return PP_BAD_ARG; }
   if (!c->have_geom) { set_error("pp_set_solve_ad: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  if (int rf = ad_no_fresh(c, "pp_set_solve_ad")) return rf;
   HIPCHK(hipSetDevice(c->device));
   const size_t nb = sizeof(double) * c->nref;
   DevTemp d;
@@ -2251,6 +2265,7 @@ int32_t pp_solve_ad_device(pp_ctx *c, int32_t advec_scheme, double *d_u, double 
     return PP_NO_GEOMETRY;
   }
   if (itermax < 0 || !std::isfinite(restol)) { set_error("pp_solve_ad: bad itermax_ad or restol_ad"); return PP_BAD_ARG; }
+  if (int rf = ad_no_fresh(c, "pp_solve_ad")) return rf;
   double half_dt = 0.0;
   int rc = ad_half_dt(advec_scheme, c->ad_dt, &half_dt, "pp_solve_ad");
   if (rc) return rc;
@@ -2316,6 +2331,7 @@ int32_t pp_solve_ad(pp_ctx *c, int32_t advec_scheme, double *u, double *v, doubl
     return PP_BAD_ARG;
   }
   if (!c->have_geom) { set_error("pp_solve_ad: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  if (int rf = ad_no_fresh(c, "pp_solve_ad")) return rf;
   HIPCHK(hipSetDevice(c->device));
   const size_t nb = sizeof(double) * c->nref;
   DevTemp d, dvbi;
@@ -2345,6 +2361,7 @@ int32_t pp_calc_residual_ad(pp_ctx *c, int32_t advec_scheme, const double *var, 
   }
   if (!c->have_geom) { set_error("pp_calc_residual_ad: no geometry"); return PP_NO_GEOMETRY; }
   if (!c->have_ad) { set_error("pp_calc_residual_ad: no momentum coefficients (call pp_set_solve_ad)"); return PP_NO_GEOMETRY; }
+  if (int rf = ad_no_fresh(c, "pp_calc_residual_ad")) return rf;
   double half_dt = 0.0;
   int rc = ad_half_dt(advec_scheme, c->ad_dt, &half_dt, "pp_calc_residual_ad");
   if (rc) return rc;

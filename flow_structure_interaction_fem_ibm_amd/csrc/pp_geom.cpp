@@ -379,7 +379,7 @@ int32_t build_geometry(pp_geom **out, const Grid &G, int32_t mode, const std::ve
     if (rc == 0) rc = tri_iblank(G, tri, o->iblank, bodynum);
     if (rc) {
       delete o;
-      pp::set_error("pp_geom_build: triangulated iblank failed (%d)", rc);
+      pp::set_error("pp_geom_build: triangulated set_boundary failed (%d: %s)", rc, tri_error_text(rc));
       return PP_BAD_GEOMETRY;
     }
   }

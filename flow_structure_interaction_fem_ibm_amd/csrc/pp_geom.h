@@ -105,15 +105,22 @@ struct Surface {
   const int32_t *tri = nullptr;
   double out[3] = {0.0, 0.0, 0.0};
 };
-// 0 or an error code: 6 bad body or the orientation point (the origin)
-// inside a body's bounding box
+// error codes of the triangulated pipeline (0 = success)
+enum {
+  TRI_ERR_NO_MARKER = 4,   // no unblocked marker among the 7 closest
+  TRI_ERR_NO_ELEMENT = 5,  // no element selected for a cell or ghost
+  TRI_ERR_SMALL = 6,       // too few markers/elements or rings, or grid centres not increasing
+  TRI_ERR_MARKER_ID = 8,   // an element names a marker out of range
+  TRI_ERR_ORIENT = 9,      // the origin (canonical bodies' orientation point) inside a body's bounding box
+};
+const char *tri_error_text(int rc);
+// 0 or an error code: TRI_ERR_SMALL or TRI_ERR_ORIENT
 int tri_build(const Grid &G, const std::vector<Body> &bodies, TriBodies &T);
-// caller-given surfaces instead of canonical bodies; 0, 6 (P inside a
-// body's bounding box, too few markers) or 8 (an element names a marker
-// out of range)
+// caller-given surfaces instead of canonical bodies; 0, TRI_ERR_SMALL or
+// TRI_ERR_MARKER_ID (pointOutsideBody may lie anywhere)
 int tri_build_surfaces(const std::vector<Surface> &surf, TriBodies &T);
 // orientation, normals, element lists and search tree of one surface
-int tri_finish(TriBody &B, const double *P);
+int tri_finish(TriBody &B, const double *P, bool check_box);
 // iblank of set_iblank_canonical_body_fast and bodynum (0-based body, -1
 // none); 0 or an error code: 4 no unblocked marker among the 7 closest,
 // 5 no element selected

@@ -335,7 +335,7 @@ int dot_norm(const TriBody &B, const double *P, double &dotnorm, std::vector<int
     }
     if (!change) choice = s;
   }
-  if (choice < 0) return 4;
+  if (choice < 0) return TRI_ERR_NO_MARKER;
   const int cm = cv[choice];
   if (edge_marker(S, cm)) return 0;
   double dmin = 1.0e16, dist_in = 1.0e16;
@@ -353,7 +353,7 @@ int dot_norm(const TriBody &B, const double *P, double &dotnorm, std::vector<int
       if (dbe <= dmin) { dmin = dbe; closest = e; }
     }
   }
-  if (closest < 0) return 5;
+  if (closest < 0) return TRI_ERR_NO_ELEMENT;
   dotnorm = (P[0] - B.cx[closest]) * B.nx[closest] + (P[1] - B.cy[closest]) * B.ny[closest] +
             (P[2] - B.cz[closest]) * B.nz[closest];
   return 0;
@@ -379,11 +379,11 @@ int tri_build(const Grid &G, const std::vector<Body> &bodies, TriBodies &T) {
     const Body &Bd = bodies[bi];
     TriBody &B = T.b[bi];
     const int nt = Bd.n_theta;
-    if (nt < 3) return 6;
+    if (nt < 3) return TRI_ERR_SMALL;
     const double aratio = Bd.r[2] / Bd.r[0];
     const int nphi = (nt % 2 == 0) ? (int)std::lround(0.5 * (double)nt * aratio)
                                    : (int)std::lround(0.5 * (double)(nt - 1) * aratio) + 1;
-    if (nphi < 3) return 6;
+    if (nphi < 3) return TRI_ERR_SMALL;
     const double ca = std::cos(Bd.alpha_deg * PI / 180.0), sa = std::sin(Bd.alpha_deg * PI / 180.0);
     for (int i = 1; i <= nphi; i++) {
       const double phi = ((double)i - 1.0) * PI / ((double)nphi - 1.0);
@@ -415,7 +415,7 @@ int tri_build(const Grid &G, const std::vector<Body> &bodies, TriBodies &T) {
     for (int i = m - nt; i <= m - 1; i++) add(i, m, (i + 1 == m) ? m - nt : i + 1);
     B.ne = (int)B.tri.size() / 3;
     const double origin[3] = {0.0, 0.0, 0.0};
-    const int rc = tri_finish(B, origin);
+    const int rc = tri_finish(B, origin, true);
     if (rc) return rc;
   }
   return 0;
@@ -426,9 +426,13 @@ int tri_build(const Grid &G, const std::vector<Body> &bodies, TriBodies &T) {
 // outside the body P): the element at the marker closest to P gets its
 // normal pointing away from P (every element flipped otherwise); element
 // normals, areas and centroids; the elements at each marker; the marker
-// search tree.  0, or 6 when P lies inside the markers' bounding box or the
-// surface is too small.
-int tri_finish(TriBody &B, const double *P) {
+// search tree.  0, or TRI_ERR_ORIENT when P lies inside the markers'
+// bounding box and check_box is set (the canonical bodies, whose orientation
+// point is the origin: one inside the box cannot be relied on to see the
+// outside of the surface).  A caller-given pointOutsideBody is used wherever
+// it lies, as determine_norm_dir_unstruc does
+// (BOUNDARY_SET_ARCLENGTH_NORM.F90:229-317).
+int tri_finish(TriBody &B, const double *P, bool check_box) {
   {
     const int m = B.nm;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -436,7 +440,9 @@ int tri_finish(TriBody &B, const double *P) {
       const double v[3] = {B.xm[q], B.ym[q], B.zm[q]};
       for (int d = 0; d < 3; d++) { lo[d] = std::min(lo[d], v[d]); hi[d] = std::max(hi[d], v[d]); }
     }
-    if (lo[0] <= P[0] && hi[0] >= P[0] && lo[1] <= P[1] && hi[1] >= P[1] && lo[2] <= P[2] && hi[2] >= P[2]) return 6;
+    if (check_box && lo[0] <= P[0] && hi[0] >= P[0] && lo[1] <= P[1] && hi[1] >= P[1] && lo[2] <= P[2] &&
+        hi[2] >= P[2])
+      return TRI_ERR_ORIENT;
     double dmin = 1.0e8;
     int cnode = 0;
     for (int q = 1; q <= m; q++) {
@@ -490,10 +496,22 @@ int tri_finish(TriBody &B, const double *P) {
         const int q = B.tri[3 * e + v] - 1;
         B.ve_idx[fill[q]++] = e;
       }
-    if (m < kSave) return 6;
+    if (m < kSave) return TRI_ERR_SMALL;
     tree_build(B);
   }
   return 0;
+}
+
+const char *tri_error_text(int rc) {
+  switch (rc) {
+    case TRI_ERR_NO_MARKER: return "no unblocked marker among the 7 closest to a band cell";
+    case TRI_ERR_NO_ELEMENT: return "no surface element selected for a cell or ghost";
+    case TRI_ERR_SMALL: return "too few markers, elements or rings, or grid centres not increasing";
+    case TRI_ERR_MARKER_ID: return "an element names a marker out of range";
+    case TRI_ERR_ORIENT:
+      return "the origin, which orients a canonical body's normals, lies inside the body's bounding box";
+    default: return "unknown error";
+  }
 }
 
 int tri_build_surfaces(const std::vector<Surface> &surf, TriBodies &T) {
@@ -501,15 +519,15 @@ int tri_build_surfaces(const std::vector<Surface> &surf, TriBodies &T) {
   for (size_t bi = 0; bi < surf.size(); bi++) {
     const Surface &S = surf[bi];
     TriBody &B = T.b[bi];
-    if (S.nm < 3 || S.ne < 1) return 6;
+    if (S.nm < 3 || S.ne < 1) return TRI_ERR_SMALL;
     B.nm = S.nm;
     B.ne = S.ne;
     B.xm.resize(S.nm); B.ym.resize(S.nm); B.zm.resize(S.nm);
     for (int q = 0; q < S.nm; q++) { B.xm[q] = S.xyz[3 * q]; B.ym[q] = S.xyz[3 * q + 1]; B.zm[q] = S.xyz[3 * q + 2]; }
     B.tri.assign(S.tri, S.tri + 3L * S.ne);
     for (int v : B.tri)
-      if (v < 1 || v > S.nm) return 8;
-    const int rc = tri_finish(B, S.out);
+      if (v < 1 || v > S.nm) return TRI_ERR_MARKER_ID;
+    const int rc = tri_finish(B, S.out, false);
     if (rc) return rc;
   }
   return 0;
@@ -521,7 +539,7 @@ int tri_iblank(const Grid &G, const TriBodies &T, ByteField &ib, IntField &bodyn
   const long ex = G.ex, exy = G.exy;
   for (int d = 0; d < 3; d++)
     for (size_t i = 1; i < G.XC[d].size(); i++)
-      if (!(G.XC[d][i] > G.XC[d][i - 1])) return 6;
+      if (!(G.XC[d][i] > G.XC[d][i - 1])) return TRI_ERR_SMALL;
   ByteField und, temp;
   par_fill(temp, G.size, (int8_t)0);
   par_fill(und, G.size, (int8_t)0);
@@ -671,7 +689,7 @@ int tri_intercept(const Grid &G, const TriBodies &T, int body, int iG, int jG, i
         for (int d = 0; d < 3; d++) xb[d] = t[d];
       }
     }
-    if (closest < 0) return 5;
+    if (closest < 0) return TRI_ERR_NO_ELEMENT;
     if (!done) {
       // the closest point on the element's two edges through the marker
       double cvp[3];

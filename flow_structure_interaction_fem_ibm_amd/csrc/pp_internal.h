@@ -133,8 +133,7 @@ struct GhostTables {
 
 struct CoarseSolve {
   int n = 0;                      // cells of the coarsest level (at most 4096)
-  double *G = nullptr;            // n*n by output cell: G[q*n+p] = d e_q / d r_p
-  float *G32 = nullptr;           // the same in FP32 (FP32 V-cycles)
+  double *G = nullptr;            // n*n by output cell: G[q*n+p] = d e_q / d r_p (FP64 in both precisions)
   int *cell = nullptr;            // level-layout offset of coarse cell q (i fastest)
 };
 

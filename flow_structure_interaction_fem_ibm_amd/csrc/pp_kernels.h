@@ -1553,8 +1553,8 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
 // solution operator G of the coarsest level (bordered LU on the GPU at
 // set-up: the coarse equations with the fluid-volume mean constraint per
 // connected component), stored by output cell: GT[q*n + p] = response of
-// cell q to a unit rhs at cell p (FP32 copy for the FP32 V-cycles, FP64
-// accumulation either way); cell[q] = offset of coarse cell q (i fastest)
+// cell q to a unit rhs at cell p (FP64 in both precisions, FP64
+// accumulation); cell[q] = offset of coarse cell q (i fastest)
 // in the level layout.  One wavefront per output cell streams
 // its row of GT (coalesced) against the rhs staged in LDS, then reduces
 // across the wavefront in a fixed order.

@@ -134,7 +134,7 @@ void free_ghosts(pp_ctx *c) {
                   g.orhsg64, g.otbase64, g.v_cD, g.v_buf[0], g.v_buf[1], g.v_kind, g.v_off};
   for (void *p : ptrs) hipFree(p);
   g = GhostTables();
-  hipFree(c->cs.G); hipFree(c->cs.G32); hipFree(c->cs.cell);
+  hipFree(c->cs.G); hipFree(c->cs.cell);
   c->cs = CoarseSolve();
 }
 
@@ -726,15 +726,14 @@ int build_coarse(pp_ctx *c) {
     } else {
       for (int p = 0; p < n; p++)
         for (int q = 0; q < n; q++) Gt[(long)q * n + p] = X[(long)p * m + q];
-      hipFree(c->cs.G); hipFree(c->cs.G32); hipFree(c->cs.cell);
+      hipFree(c->cs.G); hipFree(c->cs.cell);
       c->cs = CoarseSolve();
       c->cs.n = n;
       c->cs.G = dev_upload(Gt);
-      c->cs.G32 = dev_upload(to_f32(Gt));
       std::vector<int> off(n);
       for (int q = 0; q < n; q++) off[q] = (int)lidx(L.sx, L.sxy, q % L.cx, (q / L.cx) % L.cy, q / (L.cx * L.cy));
       c->cs.cell = dev_upload(off);
-      if (!c->cs.G || !c->cs.G32 || !c->cs.cell) {
+      if (!c->cs.G || !c->cs.cell) {
         set_error("coarse solve: device allocation failed");
         rc = PP_HIP_ERROR;
       }

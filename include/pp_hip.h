@@ -286,8 +286,9 @@ typedef struct pp_ad_stats {
  * device until the next call or pp_set_geometry.  visc_tot = viscTot
  * (reference layout, outer layer included), bcxvisc..bczvisc the
  * boundary-face viscosities.  Fresh cells (GCM_set_freshcell branch,
- * :190-221) are not handled: call it on a geometry without fresh cells.
- * Requires pp_set_geometry. */
+ * :175-200, and GCM_correct_rhs_ad/res_ad, GCM_FRESHCELL_AD.F90:2-137) are
+ * not built: with a geometry that has fresh cells this call, pp_solve_ad and
+ * pp_calc_residual_ad return PP_BAD_ARG.  Requires pp_set_geometry. */
 int32_t pp_set_solve_ad(pp_ctx *ctx, double dt, const double *visc_tot,
                         const double *bcxvisc, const double *bcyvisc,
                         const double *bczvisc);

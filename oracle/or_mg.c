@@ -26,8 +26,12 @@
 #include "or_internal.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* threads the zebra variant runs on (OpenMP's default team size) */
+int or_threads(void) { return omp_get_max_threads(); }
 
 #define MAXLEV 12
 
@@ -116,12 +120,13 @@ typedef struct {
 
 /* MG_Prepare (CORE_MG.F90:1716-1879): stair-step Neumann faces from the
  * level iblank, then outer faces Neumann (non-periodic, PBC_NEUMANN). */
-static void mg_prepare(level *L) {
+static void mg_prepare(level *L, int par) {
   box *b = &L->b;
   int mx = b->cx, my = b->cy, mz = b->cz;
   long n = bsize(b);
   for (int a = 0; a < 6; a++) memset(L->f[a], 0, n);
   int8_t *ib = L->ib;
+#pragma omp parallel for if (par) schedule(static)
   for (int k = 1; k <= mz; k++) {
     for (int j = 1; j <= my; j++) {
       for (int i = 1; i <= mx - 1; i++) {
@@ -144,6 +149,7 @@ static void mg_prepare(level *L) {
         L->f[2][bidx(b, i, j, k)] = (int8_t)(int)(t + 0.5);
       }
   }
+#pragma omp parallel for if (par) schedule(static)
   for (int j = 1; j <= my; j++)
     for (int i = 1; i <= mx; i++) {
       for (int k = 1; k <= mz - 1; k++) {
@@ -191,80 +197,84 @@ static coef7 mg_coef(const mgctx *M, const level *L, int i, int j, int k, long i
   return q;
 }
 
-/* MG_itsolv (CORE_MG.F90:1883-2270) with iRedBlack = 0 */
+/* One line of MG_itsolv (CORE_MG.F90:1945-2009): rows of the cells along
+ * direction dir through (t1, t2) (the two transverse indices in the order
+ * x-lines (j,k), y-lines (i,k), z-lines (i,j)), tdma, SOR blend.  buf holds
+ * 5 * (nmax + 1) doubles. */
+static void mg_line(const mgctx *M, const level *L, double *var, const double *R, int dir, int t1, int t2,
+                    double *buf, int nmax) {
+  const box *b = &L->b;
+  double *a = buf, *bb = buf + (nmax + 1), *cc = buf + 2 * (nmax + 1);
+  double *rhs = buf + 3 * (nmax + 1), *sol = buf + 4 * (nmax + 1);
+  const double om = M->o->omega;
+  const long sy = b->ex, sz = b->ex * b->ey;
+  const int n = dir == 0 ? b->cx : dir == 1 ? b->cy : b->cz;
+  for (int m = 1; m <= n; m++) {
+    int i = dir == 0 ? m : t1, j = dir == 1 ? m : (dir == 0 ? t1 : t2), k = dir == 2 ? m : t2;
+    long id = bidx(b, i, j, k);
+    coef7 q = mg_coef(M, L, i, j, k, id);
+    double r, am, ap;
+    if (dir == 0) {
+      r = R[id] - var[id - sy] * q.amy - var[id + sy] * q.apy - var[id - sz] * q.amz - var[id + sz] * q.apz;
+      am = q.amx; ap = q.apx;
+    } else if (dir == 1) {
+      r = R[id] - var[id - 1] * q.amx - var[id + 1] * q.apx - var[id - sz] * q.amz - var[id + sz] * q.apz;
+      am = q.amy; ap = q.apy;
+    } else {
+      r = R[id] - var[id - sy] * q.amy - var[id + sy] * q.apy - var[id - 1] * q.amx - var[id + 1] * q.apx;
+      am = q.amz; ap = q.apz;
+    }
+    double fl = 1.0 - (double)L->ib[id];
+    a[m] = am * fl;
+    cc[m] = ap * fl;
+    bb[m] = (q.acx + q.acy + q.acz) * fl + (double)L->ib[id];
+    rhs[m] = r * fl + (double)(L->gm ? L->gm[id] : 0) * var[id];
+    if (fabs(bb[m]) < 1.0e-12) { a[m] = 0; cc[m] = 0; bb[m] = 1.0; rhs[m] = var[id]; }
+  }
+  or_tdma(a, bb, cc, rhs, sol, 1, n);
+  for (int m = 1; m <= n; m++) {
+    int i = dir == 0 ? m : t1, j = dir == 1 ? m : (dir == 0 ? t1 : t2), k = dir == 2 ? m : t2;
+    long id = bidx(b, i, j, k);
+    var[id] = var[id] * (1.0 - om) + om * sol[m];
+  }
+}
+
+/* MG_itsolv (CORE_MG.F90:1883-2270).  zebra = 0: iRedBlack = 0, the lines
+ * of each direction in the reference's loop order (Gauss-Seidel across
+ * lines).  zebra = 1: the lines of each direction in two colours by the
+ * parity of their transverse indices, every line of a colour solved in
+ * parallel (OpenMP): the same line rows, another order, the same fixed
+ * point (the all-core baseline variant). */
 static void mg_itsolv(const mgctx *M, const level *L, double *var, const double *R) {
   const box *b = &L->b;
-  int mx = b->cx + 1, my = b->cy + 1, mz = b->cz + 1;
-  int nmax = mx > my ? mx : my;
-  if (mz > nmax) nmax = mz;
-  double *a = calloc(nmax + 1, 8), *bb = calloc(nmax + 1, 8), *cc = calloc(nmax + 1, 8);
-  double *rhs = calloc(nmax + 1, 8), *sol = calloc(nmax + 1, 8);
-  double om = M->o->omega;
-  long sy = b->ex, sz = b->ex * b->ey;
-  /* x-lines */
-  for (int k = 1; k <= mz - 1; k++)
-    for (int j = 1; j <= my - 1; j++) {
-      for (int i = 1; i <= mx - 1; i++) {
-        long id = bidx(b, i, j, k);
-        coef7 q = mg_coef(M, L, i, j, k, id);
-        double r = R[id] - var[id - sy] * q.amy - var[id + sy] * q.apy -
-                   var[id - sz] * q.amz - var[id + sz] * q.apz;
-        double fl = 1.0 - (double)L->ib[id];
-        a[i] = q.amx * fl;
-        cc[i] = q.apx * fl;
-        bb[i] = (q.acx + q.acy + q.acz) * fl + (double)L->ib[id];
-        rhs[i] = r * fl + (double)(L->gm ? L->gm[id] : 0) * var[id];
-        if (fabs(bb[i]) < 1.0e-12) { a[i] = 0; cc[i] = 0; bb[i] = 1.0; rhs[i] = var[id]; }
-      }
-      or_tdma(a, bb, cc, rhs, sol, 1, mx - 1);
-      for (int i = 1; i <= mx - 1; i++) {
-        long id = bidx(b, i, j, k);
-        var[id] = var[id] * (1.0 - om) + om * sol[i];
-      }
-    }
-  /* y-lines */
-  for (int k = 1; k <= mz - 1; k++)
-    for (int i = 1; i <= mx - 1; i++) {
-      for (int j = 1; j <= my - 1; j++) {
-        long id = bidx(b, i, j, k);
-        coef7 q = mg_coef(M, L, i, j, k, id);
-        double r = R[id] - var[id - 1] * q.amx - var[id + 1] * q.apx -
-                   var[id - sz] * q.amz - var[id + sz] * q.apz;
-        double fl = 1.0 - (double)L->ib[id];
-        a[j] = q.amy * fl;
-        cc[j] = q.apy * fl;
-        bb[j] = (q.acx + q.acy + q.acz) * fl + (double)L->ib[id];
-        rhs[j] = r * fl + (double)(L->gm ? L->gm[id] : 0) * var[id];
-        if (fabs(bb[j]) < 1.0e-12) { a[j] = 0; cc[j] = 0; bb[j] = 1.0; rhs[j] = var[id]; }
-      }
-      or_tdma(a, bb, cc, rhs, sol, 1, my - 1);
-      for (int j = 1; j <= my - 1; j++) {
-        long id = bidx(b, i, j, k);
-        var[id] = var[id] * (1.0 - om) + om * sol[j];
+  int nmax = b->cx > b->cy ? b->cx : b->cy;
+  if (b->cz > nmax) nmax = b->cz;
+  const int T1[3] = {b->cy, b->cx, b->cx}, T2[3] = {b->cz, b->cz, b->cy};
+  if (!M->o->zebra) {
+    double *buf = calloc(5 * (size_t)(nmax + 1), 8);
+    for (int t2 = 1; t2 <= b->cz; t2++)       /* x-lines: k outer, j inner */
+      for (int t1 = 1; t1 <= b->cy; t1++) mg_line(M, L, var, R, 0, t1, t2, buf, nmax);
+    for (int t2 = 1; t2 <= b->cz; t2++)       /* y-lines: k outer, i inner */
+      for (int t1 = 1; t1 <= b->cx; t1++) mg_line(M, L, var, R, 1, t1, t2, buf, nmax);
+    for (int t2 = 1; t2 <= b->cy; t2++)       /* z-lines: j outer, i inner */
+      for (int t1 = 1; t1 <= b->cx; t1++) mg_line(M, L, var, R, 2, t1, t2, buf, nmax);
+    free(buf);
+    return;
+  }
+  for (int dir = 0; dir < 3; dir++)
+    for (int col = 0; col < 2; col++) {
+      const int n1 = T1[dir], n2 = T2[dir];
+#pragma omp parallel
+      {
+        double *buf = calloc(5 * (size_t)(nmax + 1), 8);
+#pragma omp for schedule(static)
+        for (long q = 0; q < (long)n1 * n2; q++) {
+          int t1 = 1 + (int)(q % n1), t2 = 1 + (int)(q / n1);
+          if (((t1 + t2) & 1) == col) mg_line(M, L, var, R, dir, t1, t2, buf, nmax);
+        }
+        free(buf);
       }
     }
-  /* z-lines */
-  for (int j = 1; j <= my - 1; j++)
-    for (int i = 1; i <= mx - 1; i++) {
-      for (int k = 1; k <= mz - 1; k++) {
-        long id = bidx(b, i, j, k);
-        coef7 q = mg_coef(M, L, i, j, k, id);
-        double r = R[id] - var[id - sy] * q.amy - var[id + sy] * q.apy -
-                   var[id - 1] * q.amx - var[id + 1] * q.apx;
-        double fl = 1.0 - (double)L->ib[id];
-        a[k] = q.amz * fl;
-        cc[k] = q.apz * fl;
-        bb[k] = (q.acx + q.acy + q.acz) * fl + (double)L->ib[id];
-        rhs[k] = r * fl + (double)(L->gm ? L->gm[id] : 0) * var[id];
-        if (fabs(bb[k]) < 1.0e-12) { a[k] = 0; cc[k] = 0; bb[k] = 1.0; rhs[k] = var[id]; }
-      }
-      or_tdma(a, bb, cc, rhs, sol, 1, mz - 1);
-      for (int k = 1; k <= mz - 1; k++) {
-        long id = bidx(b, i, j, k);
-        var[id] = var[id] * (1.0 - om) + om * sol[k];
-      }
-    }
-  free(a); free(bb); free(cc); free(rhs); free(sol);
 }
 
 /* MG_residual (CORE_MG.F90:2274-2387) */
@@ -272,6 +282,7 @@ static void mg_residual(const mgctx *M, const level *L, const double *var,
                         const double *R, double *res) {
   const box *b = &L->b;
   long sy = b->ex, sz = b->ex * b->ey;
+#pragma omp parallel for if (M->o->zebra) schedule(static)
   for (int k = 1; k <= b->cz; k++)
     for (int j = 1; j <= b->cy; j++)
       for (int i = 1; i <= b->cx; i++) {
@@ -342,6 +353,7 @@ static void mg_restrict(const mgctx *M, int dir, int n, const level *Lf,
                         const level *Lc, const double *var, double *r1) {
   const box *bf = &Lf->b, *bc = &Lc->b;
   const double *X = M->d[dir].x[n];
+#pragma omp parallel for if (M->o->zebra) schedule(static)
   for (int k = 1; k <= bc->cz; k++)
     for (int j = 1; j <= bc->cy; j++)
       for (int i = 1; i <= bc->cx; i++) {
@@ -384,6 +396,7 @@ static void mg_prolong(const mgctx *M, int dir, int n, const level *Lf,
   /* loop over the two transverse directions of the fine/coarse box */
   int t1n = dir == 0 ? bf->cy : bf->cx;
   int t2n = dir == 2 ? bf->cy : bf->cz;
+#pragma omp parallel for if (M->o->zebra) schedule(static)
   for (int t2 = 1; t2 <= t2n; t2++)
     for (int t1 = 1; t1 <= t1n; t1++) {
 #define FI(a) (dir == 0 ? bidx(bf, (a), t1, t2) : dir == 1 ? bidx(bf, t1, (a), t2) : bidx(bf, t1, t2, (a)))
@@ -427,7 +440,7 @@ static void cycle_level(mgctx *M, cycle *C, int n, double *var, const double *rh
   level *L = &C->L[n];
   int dir = C->dir;
   int totIter = n == 1 ? M->o->iter_finest : (n == C->nlev ? M->o->iter_coarsest : 1);
-  if (n != 1) mg_prepare(L);
+  if (n != 1) mg_prepare(L, M->o->zebra);
   if (n == 1) gcm_pressure(M, &L->b, var);
   for (int it = 0; it < totIter; it++) mg_itsolv(M, L, var, rhs);
   if (n == C->nlev) return;
@@ -441,13 +454,14 @@ static void cycle_level(mgctx *M, cycle *C, int n, double *var, const double *rh
   cycle_level(M, C, n + 1, phi, r1);
   mg_prolong(M, dir, n, L, Lc, r, phi);
   const box *b = &L->b;
+#pragma omp parallel for if (M->o->zebra) schedule(static)
   for (int k = 1; k <= b->cz; k++)
     for (int j = 1; j <= b->cy; j++)
       for (int i = 1; i <= b->cx; i++) {
         long id = bidx(b, i, j, k);
         var[id] = var[id] + r[id] * (1.0 - (double)L->ib[id]);
       }
-  if (n != 1) mg_prepare(L);
+  if (n != 1) mg_prepare(L, M->o->zebra);
   int post;
   if (dir == 2) {
     post = n == 1 ? M->o->iter_finest : 1;
@@ -483,6 +497,7 @@ static void cycle_build(mgctx *M, cycle *C, int dir) {
     if (n == 1) {
       L->gm = calloc(sz, 1);
       const int8_t *src[6] = {c->ium, c->iup, c->jum, c->jup, c->kum, c->kup};
+#pragma omp parallel for if (M->o->zebra) schedule(static)
       for (int k = 0; k <= c->nzc + 1; k++)
         for (int j = 0; j <= c->nyc + 1; j++)
           for (int i = 0; i <= c->nxc + 1; i++) {
@@ -493,6 +508,7 @@ static void cycle_build(mgctx *M, cycle *C, int dir) {
           }
     } else {
       level *P = &C->L[n - 1];
+#pragma omp parallel for if (M->o->zebra) schedule(static)
       for (int k = 1; k <= L->b.cz; k++)
         for (int j = 1; j <= L->b.cy; j++)
           for (int i = 1; i <= L->b.cx; i++) {
@@ -532,6 +548,7 @@ static void mg_solver(mgctx *M, double *var, const double *rr) {
   box b1 = {c->nxc, c->nyc, c->nzc, c->nxc + 2, c->nyc + 2};
   long sz = bsize(&b1);
   double *v1 = calloc(sz, 8), *r1 = calloc(sz, 8);
+#pragma omp parallel for if (M->o->zebra) schedule(static)
   for (int k = 0; k <= c->nz; k++)
     for (int j = 0; j <= c->ny; j++)
       for (int i = 0; i <= c->nx; i++) {
@@ -544,6 +561,7 @@ static void mg_solver(mgctx *M, double *var, const double *rr) {
     cycle_build(M, &C, dir);
     cycle_level(M, &C, 1, v1, r1);
     cycle_free(&C);
+#pragma omp parallel for if (M->o->zebra) schedule(static)
     for (int k = 0; k <= c->nz; k++)
       for (int j = 0; j <= c->ny; j++)
         for (int i = 0; i <= c->nx; i++) var[OR_IDX(c, i, j, k)] = v1[bidx(&b1, i, j, k)];

@@ -90,7 +90,13 @@ typedef struct or_mg_opts {
   int itermax_ad;      /* iterMax_ad for ghost rows */
   double restol_ad;    /* restol_ad for ghost rows */
   int mean_remove;     /* call subtracting_average_pressure at the start */
+  int zebra;           /* 0: the reference's line order (iRedBlack = 0, one
+                          thread); 1: two-colour line order, lines and cell
+                          loops on all OpenMP threads (same fixed point) */
 } or_mg_opts;
+
+/* OpenMP threads of the zebra variant */
+int or_threads(void);
 
 typedef struct or_mg_stats {
   int iterations;

@@ -14,7 +14,8 @@ ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 class MGOpts(C.Structure):
     _fields_ = [("iter_max", C.c_int), ("restol", C.c_double), ("rel_tol", C.c_double),
                 ("iter_finest", C.c_int), ("iter_coarsest", C.c_int), ("omega", C.c_double),
-                ("itermax_ad", C.c_int), ("restol_ad", C.c_double), ("mean_remove", C.c_int)]
+                ("itermax_ad", C.c_int), ("restol_ad", C.c_double), ("mean_remove", C.c_int),
+                ("zebra", C.c_int)]
 
 
 class MGStats(C.Structure):
@@ -56,6 +57,7 @@ def lib():
         L.or_tdma.argtypes = [P] * 5 + [C.c_int, C.c_int]
         L.or_mg_solve.argtypes = [P, P, P, C.POINTER(MGOpts), C.POINTER(MGStats)]
         L.or_mg_cycle.argtypes = [P, P, P, C.POINTER(MGOpts)]
+        L.or_threads.restype = C.c_int
         L.or_apply_A.argtypes = [P, P, P]
         L.or_apply_AT.argtypes = [P, P, P]
         L.or_geom_ellipsoid.argtypes = [P, C.c_int] + [C.c_double] * 7
@@ -82,9 +84,9 @@ def lib():
 
 
 def mg_opts(iter_max=200, restol=0.0, rel_tol=1e-12, iter_finest=2, iter_coarsest=4, omega=1.0,
-            itermax_ad=500, restol_ad=1e-15, mean_remove=1):
+            itermax_ad=500, restol_ad=1e-15, mean_remove=1, zebra=0):
     return MGOpts(iter_max, restol, rel_tol, iter_finest, iter_coarsest, omega, itermax_ad,
-                  restol_ad, mean_remove)
+                  restol_ad, mean_remove, zebra)
 
 
 class Oracle:
@@ -320,3 +322,16 @@ def tdma(a, b, c, r):
     A[1:] = a; B[1:] = b; Cc[1:] = c; R[1:] = r
     lib().or_tdma(A.ctypes.data, B.ctypes.data, Cc.ctypes.data, R.ctypes.data, U.ctypes.data, 1, n)
     return U[1:]
+
+
+def threads():
+    """OpenMP threads the oracle's zebra variant runs on."""
+    return int(lib().or_threads())
+
+
+def build_flags():
+    """The C compiler flags the oracle is built with (oracle/Makefile)."""
+    import re
+    txt = open(os.path.join(ROOT, "oracle", "Makefile")).read()
+    m = re.search(r"^CFLAGS\s*\?=\s*(.*)$", txt, re.M)
+    return m.group(1).strip() if m else ""

@@ -84,6 +84,12 @@ def test_null_values_keep_the_items():
     # the items left after a slash keep their values
     v = case_io.read_record(iter(["0 /", "99"]), items, dict(prev))
     assert v == {"a": 0, "b": 2.5, "c": True, "d": 9}
+    # the end of a record is a blank: a continuation record opening with a
+    # comma after a value has one separator (c = F), after a comma a null (b)
+    v = case_io.read_record(iter(["1 2", ",F 6"]), items, dict(prev))
+    assert v == {"a": 1, "b": 2.0, "c": False, "d": 6}
+    v = case_io.read_record(iter(["1,", ",F 6"]), items, dict(prev))
+    assert v == {"a": 1, "b": 2.5, "c": False, "d": 6}
 
 
 def test_grid_files_and_uniform_rule(tmp_path):
@@ -174,3 +180,29 @@ def test_unstructured_surface_builds_the_canonical_geometry(tmp_path):
     bad[0, 0] = len(xyz2) + 1
     with pytest.raises(pp.PPError):
         pp.build_geometry_surfaces(*nodes, [(xyz2, bad, out)])
+
+
+def test_unstructured_surface_outside_point_inside_the_bounding_box():
+    """pointOutsideBody may lie anywhere outside the body, also inside its
+    markers' bounding box (determine_norm_dir_unstruc has no such
+    restriction): the surface is oriented as with the origin and builds the
+    same geometry; the canonical bodies, oriented against the origin, still
+    refuse an origin inside their bounding box with a specific message."""
+    nodes = (pp.tanh_nodes(40, 1.0, 1.2), pp.tanh_nodes(32, 1.0, 0.9), pp.tanh_nodes(36, 1.0, 1.5))
+    c, r = (0.5, 0.47, 0.53), 0.22
+    body = pp.Ellipsoid(c, (r, r, r), 0.0, 24)
+    xyz, tri = pp.canonical_surface(body)
+    inside_box = tuple(v - 0.16 for v in c)      # 0.277 from the centre: outside the sphere
+    assert np.linalg.norm(np.subtract(inside_box, c)) > r
+    assert all(xyz[:, d].min() < inside_box[d] < xyz[:, d].max() for d in range(3))
+    ib_ref, _, gs_ref, ex_ref = pp.build_geometry(*nodes, [body], mode=pp.PP_GEOM_TRIANGULATED)
+    for out in (inside_box, (0.5, 0.47, 0.99), (-3.0, 2.0, 0.1)):
+        ib, _, gs, ex = pp.build_geometry_surfaces(*nodes, [(xyz, tri, out)])
+        assert np.array_equal(ib, ib_ref) and np.array_equal(gs.ig, gs_ref.ig) and np.array_equal(gs.ic, gs_ref.ic)
+        assert np.array_equal(ex["elem"], ex_ref["elem"])
+        assert np.abs(gs.coeff - gs_ref.coeff).max() < 1e-12
+    # a canonical body around the origin cannot be oriented against it
+    with pytest.raises(pp.PPError, match="bounding box"):
+        pp.build_geometry(np.linspace(-1, 1, 17), np.linspace(-1, 1, 17), np.linspace(-1, 1, 17),
+                          [pp.Ellipsoid((0.05, 0.0, 0.0), (0.3, 0.3, 0.3), 0.0, 24)],
+                          mode=pp.PP_GEOM_TRIANGULATED)

@@ -135,6 +135,39 @@ def test_solve_ad_iteration_limit_and_errors():
     assert e.value.code == pp.PP_NO_GEOMETRY
 
 
+def test_solve_ad_refuses_fresh_cells():
+    """A geometry built with iblank_prev has fresh cells; their momentum rows
+    (GCM_set_freshcell, GCM_correct_rhs_ad/res_ad) are not built, so every
+    momentum entry point refuses it instead of solving with the ordinary
+    rows; the same geometry without the fresh mask is accepted."""
+    announce("solve_ad fresh-cell refusal 24^3 moving sphere")
+    xn, yn, zn = (cases.uniform(24),) * 3
+    ib0, _, _, _ = pp.build_geometry(xn, yn, zn, [pp.Ellipsoid((0.45, 0.5, 0.5), (0.2, 0.2, 0.2), 0.0, 48)])
+    ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, [pp.Ellipsoid((0.55, 0.5, 0.5), (0.2, 0.2, 0.2), 0.0, 48)],
+                                       iblank_prev=ib0)
+    assert fr.sum() > 0
+    s = pp.PressureSolver(xn, yn, zn)
+    s.set_geometry(ib, gs, fresh=fr)
+    s.set_vel_ghosts(ex["coeffD"])
+    z = np.zeros(s.shape)
+    one = np.full(s.shape, 0.01)
+    with pytest.raises(pp.PPError) as e:
+        s.set_solve_ad(0.1, one, one, one, one)
+    assert e.value.code == pp.PP_BAD_ARG and "fresh cells" in str(e.value)
+    s.set_geometry(ib, gs)
+    s.set_vel_ghosts(ex["coeffD"])
+    s.set_solve_ad(0.1, one, one, one, one)
+    u, v, w, st = s.solve_ad(AB2, z, z, z, one, one, one, z, z, z, itermax_ad=50, restol_ad=1e-10)
+    assert st["status"] == "PP_OK"
+    # coefficients set, then a geometry with fresh cells: solve and residual refuse
+    s.set_geometry(ib, gs, fresh=fr)
+    for call in (lambda: s.solve_ad(AB2, z, z, z, one, one, one, z, z, z),
+                 lambda: s.calc_residual_ad(AB2, z, z, z, z, z)):
+        with pytest.raises(pp.PPError) as e:
+            call()
+        assert e.value.code in (pp.PP_BAD_ARG, pp.PP_NO_GEOMETRY)
+
+
 def test_solve_ad_converges_on_a_stretched_64x48x40_grid():
     """A larger case with the analytic geometry: the oracle's residual of
     the device solution (computed independently on the CPU) meets the
