@@ -133,6 +133,11 @@ def divergence_rhs(xn, yn, zn, ib, seed):
     return rhs * (1 - ib)
 
 
+def progress(msg):
+    """A progress line on stderr (the run's phases; the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -163,25 +168,30 @@ def cpu_baseline(xn, yn, zn, ib, gs, rhs_projected, reps=3, ref_tol=1e-13):
     o.set_stencils(gs)
     threads = oracle_lib.threads()
     p = np.zeros(ib.shape)
+    t0 = time.perf_counter()
     o.mg_cycle(p, rhs_projected, zebra=1)           # warm-up (page faults, thread pool)
+    progress(f"oracle warm-up iteration on {threads} threads: {time.perf_counter() - t0:.1f} s")
     times, stats = [], None
-    for _ in range(reps):
+    for r in range(reps):
         p = np.zeros(ib.shape)
         t0 = time.perf_counter()
         rc, stats = o.mg_solve(p, rhs_projected, rel_tol=1e-8, iter_max=200, zebra=1)
         times.append(time.perf_counter() - t0)
+        progress(f"oracle solve {r + 1}/{reps} to 1e-8: {stats['iterations']} iterations, {times[-1]:.1f} s")
         if rc != 0:
             raise RuntimeError(f"oracle MG did not reach 1e-8: {stats}")
     it_1e8 = stats["iterations"]
     t0 = time.perf_counter()
     rc, st_ref = o.mg_solve(p, rhs_projected, rel_tol=ref_tol, iter_max=200, zebra=1, mean_remove=0)
     t_ref = time.perf_counter() - t0
+    progress(f"oracle continued to {ref_tol:g}: {st_ref['iterations']} more iterations, {t_ref:.1f} s")
     o.mean_remove(p)
     # one outer iteration in the reference's line order on one core
     q = np.zeros(ib.shape)
     t0 = time.perf_counter()
     o.mg_cycle(q, rhs_projected, zebra=0)
     t_1core = time.perf_counter() - t0
+    progress(f"one reference-order iteration on one core: {t_1core:.1f} s")
     best = min(times)
     out = {"value": 1.0 / best, "unit": "solves/s", "cores": threads, "kind": "port",
            "sample": (f"full solves from p = 0 to relative residual 1e-8 ({it_1e8} outer iterations of the "
@@ -243,6 +253,7 @@ def main():
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, stats
 
+    progress(f"set-up {t_setup:.1f} s; timing {args.steps} mixed solves")
     ms_mixed, st_mixed = run(pp.PP_PREC_MIXED, args.steps, args.warmup)
     p_mixed = d_p.cpu().numpy()
     ms_fp64, st_fp64 = run(pp.PP_PREC_FP64, max(2, args.steps // 2), 1)
@@ -270,6 +281,7 @@ def main():
     d_p.zero_()
     st_mt = solver.solve_device(d_p, d_rhs, precision=pp.PP_PREC_MIXED, **tight)
     p_mt = d_p.cpu().numpy()
+    progress(f"mixed {1e3 * ms_mixed:.2f} ms, FP64 {1e3 * ms_fp64:.2f} ms per solve; kernel timing, steps either side")
     km = prof_mixed[0]["kernels"]
     # dominant kernel: the kernel class with the largest device time on the
     # finest level (one kernel instantiation per class there); per-launch
@@ -346,6 +358,7 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu:
+        progress("CPU baseline (oracle)")
         # the same projected RHS the GPU solved
         w = solver.null_vector()
         bp = rhs.copy()

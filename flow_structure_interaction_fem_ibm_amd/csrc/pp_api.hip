@@ -158,11 +158,11 @@ double bytes_line(int s) { return 3.0 * s + 1.0; }
 // ---------------------------------------------------------------------
 // line passes
 // ---------------------------------------------------------------------
-// lane-group width G and rows per lane S for a line of n <= 512 cells
-// (FP64 holds at most 8 rows per lane in registers)
+// lane-group width G and rows per lane S for a line of n <= 1024 cells
+// (FP64 holds 8 rows per lane in registers up to 512 cells)
 void line_shape(int n, int &G, int &S, int elem) {
   if (elem == 8) G = n <= 256 ? 32 : 64;
-  else G = n <= 128 ? 16 : 32;  // FP32: 16-lane groups measured faster up to 128 cells
+  else G = n <= 128 ? 16 : (n <= 512 ? 32 : 64);  // FP32: 16-lane groups measured faster up to 128 cells
   int need = (n + G - 1) / G;
   static const int opts[] = {1, 2, 3, 4, 6, 8, 12, 16};
   S = 16;
@@ -225,7 +225,8 @@ void yz_shape(int n, int &LS, int &P, int &TX, int elem = 4) {
     if ((n + o - 1) / o <= 16) { LS = o; break; }
   }
   P = (n + LS - 1) / LS;
-  TX = P <= 16 ? 64 : (P <= 32 ? 32 : 16);
+  // TX x P <= 1024 threads: lines of up to 1024 cells
+  TX = P <= 16 ? 64 : (P <= 32 ? 32 : (P <= 64 ? 16 : 8));
   (void)elem;
 }
 
@@ -285,6 +286,9 @@ struct AdjPtrs<float> {
   static const float *nbw(GhostTables &g) { return g.nb_w32; }
   static const float *tcoef(GhostTables &g) { return g.t_coef32; }
   static const float *tvinv(GhostTables &g) { return g.t_vinv32; }
+  static const float *ctw(GhostTables &g) { return g.ct_w32; }
+  static const float *ctvinv(GhostTables &g) { return g.ct_vinv32; }
+  static float *ctbase(GhostTables &g) { return g.ctbase32; }
 };
 template <>
 struct AdjPtrs<double> {
@@ -294,34 +298,41 @@ struct AdjPtrs<double> {
   static const double *nbw(GhostTables &g) { return g.nb_w64; }
   static const double *tcoef(GhostTables &g) { return g.t_coef64; }
   static const double *tvinv(GhostTables &g) { return g.t_vinv64; }
+  static const double *ctw(GhostTables &g) { return g.ct_w64; }
+  static const double *ctvinv(GhostTables &g) { return g.ct_vinv64; }
+  static double *ctbase(GhostTables &g) { return g.ctbase64; }
 };
 
 // Transposed ghost rows on the level-0 correction mu (homogeneous: the
-// ghost sources belong to the outer residual): lambda_g from the fluid
-// neighbours of each ghost, a few Jacobi sweeps of the transposed
-// ghost-ghost coupling (warm-started across smoothing steps), then the
-// touched fluid cells' rhs = base + (sum c lambda)/V.
-int g_adj_cur = 0;
+// ghost sources belong to the outer residual): the ghost sources from the
+// fluid neighbours of each ghost, then the fluid cells of the composite rows
+// get rhs = base + (M^T sources)/V: the ghost-ghost coupling solved exactly,
+// the transpose of the forward smoother's composite ghost rows.
 template <typename T>
 void adj_ghost_level0(pp_ctx *c, T *e, T *rhs) {
   GhostTables &g = c->gh;
   if (g.n == 0) return;
   hipStream_t st = c->stream;
   dim3 grid((g.n + 255) / 256), block(256);
-  kbegin(c, PP_K_GHOST, 0, g.n * 6.0 * (8 + 2 * sizeof(T)) + (double)g.nt * (24 + 2 * sizeof(T)));
+  const double nnz = (double)g.h_ct_ghost.size();
+  kbegin(c, PP_K_GHOST, 0, g.n * 6.0 * (8 + sizeof(T)) + g.n * sizeof(T) +
+                              (double)g.nct * (8 + 8 + 2 * sizeof(T)) + nnz * (4 + 2 * sizeof(T)));
   k_adj_ghost_rhs<T><<<grid, block, 0, st>>>(g.n, g.nb_al, AdjPtrs<T>::nbw(g), nullptr, e, AdjPtrs<T>::rhsg(g));
-  int sweeps = std::min(g.kexact, 2);
-  for (int q = 0; q < sweeps; q++) {
-    k_adj_ghost_jacobi<T><<<grid, block, 0, st>>>(g.n, g.g_tidx, g.t_ptr, g.t_ghost, AdjPtrs<T>::tcoef(g),
-                                                  AdjPtrs<T>::rhsg(g), AdjPtrs<T>::lam(g, g_adj_cur),
-                                                  AdjPtrs<T>::lam(g, 1 - g_adj_cur));
-    g_adj_cur ^= 1;
-  }
-  if (g.nt > 0)
-    k_adj_touch<T><<<dim3((g.nt + 255) / 256), block, 0, st>>>(g.nt, g.t_al, g.t_ptr, g.t_ghost, AdjPtrs<T>::tcoef(g),
-                                                               AdjPtrs<T>::tvinv(g), AdjPtrs<T>::lam(g, g_adj_cur),
-                                                               AdjPtrs<T>::tbase(g), rhs);
+  if (g.nct > 0)
+    k_adj_composite<T><<<dim3((g.nct + 255) / 256), block, 0, st>>>(g.nct, g.ct_al, g.ct_ptr, g.ct_ghost,
+                                                                    AdjPtrs<T>::ctw(g), AdjPtrs<T>::ctvinv(g),
+                                                                    AdjPtrs<T>::rhsg(g), AdjPtrs<T>::ctbase(g), rhs);
   kend(c);
+}
+
+// the rhs of the composite rows' fluid cells before a transposed V-cycle
+// (the base the transposed ghost coupling adds to)
+template <typename T>
+void adj_ghost_base(pp_ctx *c) {
+  GhostTables &g = c->gh;
+  if (g.nct > 0)
+    k_gather_idx<T><<<dim3((g.nct + 255) / 256), 256, 0, c->stream>>>(g.nct, g.ct_al, lv_rhs<T>(c->lv[0]),
+                                                                     AdjPtrs<T>::ctbase(g));
 }
 
 // ghost rows of the level-0 correction before a smoothing step (forward:
@@ -398,13 +409,62 @@ void launch_residual(pp_ctx *c, Level &L) {
   }
 }
 
+// the level halves in all three directions (the fused by-2 kernels apply)
+bool halves_by2(const Level &L, const Level &C) {
+  return C.cf[0] == 2 && C.cf[1] == 2 && C.cf[2] == 2 && L.cx == 2 * C.cx && L.cy == 2 * C.cy && L.cz == 2 * C.cz;
+}
+
+// residual of level l restricted onto level l + 1 (its rhs; its correction
+// zeroed): one fused pass where the level halves in all three directions,
+// else the level residual and the general restriction
+template <typename T>
+void restrict_step(pp_ctx *c, int l) {
+  Level &L = c->lv[l], &C = c->lv[l + 1];
+  hipStream_t st = c->stream;
+  const double cells = (double)L.cx * L.cy * L.cz;
+  LevelArgs<T> a = largs<T>(L), ac = largs<T>(C);
+  CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
+  if (halves_by2(L, C)) {
+    kbegin(c, PP_K_RESTRICT, l, cells * (2.0 * sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz);
+    k_resrestrict2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, lv_e<T>(L), lv_rhs<T>(L), lv_rhs<T>(C), lv_e<T>(C));
+    kend(c);
+  } else {
+    kbegin(c, PP_K_RESIDUAL, l, cells * (3.0 * sizeof(T) + 1.0));
+    launch_residual<T>(c, L);
+    kend(c);
+    kbegin(c, PP_K_RESTRICT, l, cells * (sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz);
+    k_restrict<T><<<cc.grid, cc.block, 0, st>>>(a, ac, C.cf[0], C.cf[1], C.cf[2], lv_res<T>(L), lv_rhs<T>(C),
+                                                 lv_e<T>(C));
+    kend(c);
+  }
+}
+
+// the correction of level l + 1 prolongated onto level l and added to its
+// correction on fluid cells
+template <typename T>
+void prolong_step(pp_ctx *c, int l) {
+  Level &L = c->lv[l], &C = c->lv[l + 1];
+  hipStream_t st = c->stream;
+  const double cells = (double)L.cx * L.cy * L.cz;
+  LevelArgs<T> a = largs<T>(L), ac = largs<T>(C);
+  kbegin(c, PP_K_PROLONG, l, cells * (2.0 * sizeof(T) + 1.0) + (sizeof(T) + 1.0) * C.cx * C.cy * C.cz);
+  if (halves_by2(L, C)) {
+    CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
+    k_prolong2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, pdir<T>(L, 0).wpar, pdir<T>(L, 1).wpar, pdir<T>(L, 2).wpar,
+                                                 lv_e<T>(C), lv_e<T>(L));
+  } else {
+    CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
+    k_prolong<T><<<cl.grid, cl.block, 0, st>>>(a, ac, pdir<T>(L, 0), pdir<T>(L, 1), pdir<T>(L, 2), lv_e<T>(C),
+                                                lv_e<T>(L));
+  }
+  kend(c);
+}
+
 template <typename T>
 void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   Level &L = c->lv[l];
   hipStream_t st = c->stream;
-  double cells = (double)L.cx * L.cy * L.cz;
   if (l == (int)c->lv.size() - 1) {
-    LevelArgs<T> a = largs<T>(L);
     const CoarseSolve &cs = c->cs;
     int n = cs.n;
     // the dense FP64 solution operator in both precisions (the coarsest
@@ -418,36 +478,9 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   const int npre = (l > 0 && o->nu_coarse > 0) ? o->nu_coarse : o->nu_pre;
   const int npost = (l > 0 && o->nu_coarse > 0) ? o->nu_coarse : o->nu_post;
   smooth_steps<T>(c, l, o, npre);
-  LevelArgs<T> a = largs<T>(L);
-  CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
-  Level &C = c->lv[l + 1];
-  LevelArgs<T> ac = largs<T>(C);
-  CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
-  const bool by2 = C.cf[0] == 2 && C.cf[1] == 2 && C.cf[2] == 2 && L.cx == 2 * C.cx && L.cy == 2 * C.cy &&
-                   L.cz == 2 * C.cz;
-  if (by2) {
-    // residual formed inside the restriction
-    kbegin(c, PP_K_RESTRICT, l, cells * (2.0 * sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz);
-    k_resrestrict2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, lv_e<T>(L), lv_rhs<T>(L), lv_rhs<T>(C), lv_e<T>(C));
-    kend(c);
-  } else {
-    kbegin(c, PP_K_RESIDUAL, l, cells * (3.0 * sizeof(T) + 1.0));
-    launch_residual<T>(c, L);
-    kend(c);
-    kbegin(c, PP_K_RESTRICT, l, cells * (sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz);
-    k_restrict<T><<<cc.grid, cc.block, 0, st>>>(a, ac, C.cf[0], C.cf[1], C.cf[2], lv_res<T>(L), lv_rhs<T>(C),
-                                                 lv_e<T>(C));
-    kend(c);
-  }
+  restrict_step<T>(c, l);
   vcycle<T>(c, l + 1, o);
-  kbegin(c, PP_K_PROLONG, l, cells * (2.0 * sizeof(T) + 1.0) + (sizeof(T) + 1.0) * C.cx * C.cy * C.cz);
-  if (by2)
-    k_prolong2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, pdir<T>(L, 0).wpar, pdir<T>(L, 1).wpar, pdir<T>(L, 2).wpar,
-                                                 lv_e<T>(C), lv_e<T>(L));
-  else
-    k_prolong<T><<<cl.grid, cl.block, 0, st>>>(a, ac, pdir<T>(L, 0), pdir<T>(L, 1), pdir<T>(L, 2), lv_e<T>(C),
-                                                lv_e<T>(L));
-  kend(c);
+  prolong_step<T>(c, l);
   smooth_steps<T>(c, l, o, npost);
 }
 
@@ -571,14 +604,14 @@ int fluid_sums(pp_ctx *c, const double *w, const double *b, const double *p) {
   return PP_OK;
 }
 
-int mean_remove_ref(pp_ctx *c, double *p, double *mean_out) {
+// subtracting_average_pressure (CORE_POISSON.F90:419-490); with fresh =
+// false the mean subtraction alone (the fresh-cell fill belongs to the
+// initial guess only: the reference calls it once, before iterating, :535)
+int mean_remove_ref(pp_ctx *c, double *p, double *mean_out, bool fresh = true) {
   CellLaunch cl = ref_launch(c);
   fluid_sums(c, nullptr, nullptr, p);
-  if (!c->h_fresh.empty()) {
-    HIPCHK(hipMemcpyAsync(c->d_tmp, p, sizeof(double) * c->nref, hipMemcpyDeviceToDevice, c->stream));
-  }
   k_mean_sub<<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), p, c->d_res);
-  if (!c->h_fresh.empty()) {
+  if (fresh && !c->h_fresh.empty()) {
     // fresh cells read the mean-removed snapshot
     HIPCHK(hipMemcpyAsync(c->d_tmp, p, sizeof(double) * c->nref, hipMemcpyDeviceToDevice, c->stream));
     k_fresh_fill<<<cl.grid, cl.block, 0, c->stream>>>(c->ref, c->d_iblank_ref, c->d_fresh, c->d_tmp, p);
@@ -982,8 +1015,10 @@ int solve_impl(pp_ctx *c, double *p, const double *rhs, const double *bc_src, co
     outer++;
   }
   if (o->mean_remove) {
+    // zero fluid-volume mean of the solution; fresh cells keep their solved
+    // values (the reference fills them on the initial guess only)
     double mean = 0.0;
-    mean_remove_ref(c, p, &mean);
+    mean_remove_ref(c, p, &mean, false);
     S.mean_removed = mean;
     ghost_exact_ref(c, p, have_bnd);
   }
@@ -1019,18 +1054,12 @@ void adjoint_ghost_coupling(pp_ctx *c, const double *z, double *out) {
   hipStream_t s = c->stream;
   hipMemsetAsync(out, 0, sizeof(double) * c->nref, s);
   if (g.n == 0) return;
-  dim3 gg((g.n + 255) / 256), gt((g.nt + 255) / 256), blk(256);
+  dim3 gg((g.n + 255) / 256), blk(256);
   k_adj_ghost_rhs<double><<<gg, blk, 0, s>>>(g.n, g.nb_ref, g.nb_w64, nullptr, z, g.orhsg64);
-  k_fill<double><<<gg, blk, 0, s>>>(g.olam64[0], g.n, 0.0);
-  int cur = 0;
-  for (int q = 0; q < g.kexact; q++) {
-    k_adj_ghost_jacobi<double><<<gg, blk, 0, s>>>(g.n, g.g_tidx, g.t_ptr, g.t_ghost, g.t_coef64, g.orhsg64,
-                                                 g.olam64[cur], g.olam64[1 - cur]);
-    cur ^= 1;
-  }
-  if (g.nt > 0)
-    k_adj_touch<double><<<gt, blk, 0, s>>>(g.nt, g.t_ref, g.t_ptr, g.t_ghost, g.t_coef64, g.t_vinv64,
-                                           g.olam64[cur], nullptr, out);
+  if (g.nct > 0)
+    k_adj_composite<double><<<dim3((g.nct + 255) / 256), blk, 0, s>>>(g.nct, g.ct_ref, g.ct_ptr, g.ct_ghost,
+                                                                      g.ct_w64, g.ct_vinv64, g.orhsg64, nullptr,
+                                                                      out);
 }
 
 // adjoint operator B mu = A mu - (transposed ghost coupling of mu) on the
@@ -1047,14 +1076,7 @@ struct AdjOps {
   double *z(int q) { return c->d_kry[c->kry_m + 1 + q]; }
   int blocks() const { return c->nblk_red; }
   int precondition() {
-    GhostTables &g = c->gh;
-    Level &L0 = c->lv[0];
-    dim3 gg((g.n + 255) / 256), gt((g.nt + 255) / 256), blk(256);
-    if (g.nt > 0) k_gather_idx<T><<<gt, blk, 0, c->stream>>>(g.nt, g.t_al, lv_rhs<T>(L0), AdjPtrs<T>::tbase(g));
-    if (g.n > 0) {
-      k_fill<T><<<gg, blk, 0, c->stream>>>(AdjPtrs<T>::lam(g, 0), g.n, T(0));
-      g_adj_cur = 0;
-    }
+    adj_ghost_base<T>(c);
     c->adjoint_mode = true;
     for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
       vcycle<T>(c, 0, o);
@@ -1160,11 +1182,7 @@ int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref
       if (krc) return krc;
       continue;
     }
-    if (g.nt > 0) k_gather_idx<T><<<gt, blk, 0, s>>>(g.nt, g.t_al, lv_rhs<T>(L0), AdjPtrs<T>::tbase(g));
-    if (g.n > 0) {
-      k_fill<T><<<gg, blk, 0, s>>>(AdjPtrs<T>::lam(g, 0), g.n, T(0));
-      g_adj_cur = 0;
-    }
+    adj_ghost_base<T>(c);
     c->adjoint_mode = true;
     for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
       vcycle<T>(c, 0, o);
@@ -1302,9 +1320,9 @@ int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
     set_error("pp_create: grid must have at least 4 cells per direction and node arrays");
     return PP_BAD_ARG;
   }
-  if (g->nxc > 512 || g->nyc > 512 || g->nzc > 512) {
-    set_error("pp_create: %dx%dx%d cells; the line solvers hold at most 512 cells per line", g->nxc, g->nyc,
-              g->nzc);
+  if (g->nxc > kMaxLine || g->nyc > kMaxLine || g->nzc > kMaxLine) {
+    set_error("pp_create: %dx%dx%d cells; the line solvers hold at most %d cells per line", g->nxc, g->nyc,
+              g->nzc, kMaxLine);
     return PP_BAD_ARG;
   }
   for (int f = 0; f < 6; f++)
@@ -1313,12 +1331,12 @@ int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
       return PP_BAD_ARG;
     }
   {
+    // every grid coarsens to at most 4096 cells; the check guards the
+    // dense coarsest-level operator (n^2 entries) against a future rule
     const long nc = coarsest_cells(g->nxc, g->nyc, g->nzc);
     if (nc > kCoarseDenseMax) {
-      set_error("pp_create: %dx%dx%d cells coarsen to a %ld-cell coarsest level (a direction is halved while its "
-                "cell count is even and at least 4); at most %ld cells are solved directly: give the grid even "
-                "cell counts with more factors of 2",
-                g->nxc, g->nyc, g->nzc, nc, kCoarseDenseMax);
+      set_error("pp_create: %dx%dx%d cells coarsen to a %ld-cell coarsest level; at most %ld cells are solved "
+                "directly", g->nxc, g->nyc, g->nzc, nc, kCoarseDenseMax);
       return PP_BAD_ARG;
     }
   }
@@ -1328,9 +1346,9 @@ int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
     return PP_HIP_ERROR;
   }
   if (g->device < 0 || g->device >= ndev) { set_error("pp_create: bad device %d", g->device); return PP_BAD_ARG; }
+  HIPCHK(hipSetDevice(g->device));
   pp_ctx *c = new pp_ctx();
   c->device = g->device;
-  HIPCHK(hipSetDevice(c->device));
   c->nxc = g->nxc; c->nyc = g->nyc; c->nzc = g->nzc;
   c->ref.nxc = g->nxc; c->ref.nyc = g->nyc; c->ref.nzc = g->nzc;
   c->ref.ex = g->nxc + 3;
@@ -1869,6 +1887,113 @@ int32_t pp_time_kernel(pp_ctx *c, int32_t cls, int32_t level, int32_t precision,
                                     : time_kernel<double>(c, cls, level, reps, avg_us, bytes);
 }
 
+int32_t pp_level_data(pp_ctx *c, int32_t level, uint8_t *flags, double *w, double *xc, double *cm, double *cp) {
+  if (!c) { set_error("pp_level_data: null context"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_level_data: no geometry"); return PP_NO_GEOMETRY; }
+  if (level < 0 || level >= (int)c->lv.size()) { set_error("pp_level_data: bad level %d", level); return PP_BAD_ARG; }
+  const Level &L = c->lv[level];
+  if (flags)
+    for (int k = 0; k < L.cz; k++)
+      for (int j = 0; j < L.cy; j++)
+        for (int i = 0; i < L.cx; i++)
+          flags[i + (long)L.cx * (j + (long)L.cy * k)] = L.hflags[lidx(L.sx, L.sxy, i, j, k)];
+  double *outs[4] = {w, xc, cm, cp};
+  const std::vector<double> *src[4] = {L.w, L.xc, L.cm, L.cp};
+  for (int q = 0; q < 4; q++) {
+    if (!outs[q]) continue;
+    long off = 0;
+    for (int d = 0; d < 3; d++) {
+      std::copy(src[q][d].begin(), src[q][d].end(), outs[q] + off);
+      off += (long)src[q][d].size();
+    }
+  }
+  return PP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// dense interior (i fastest) <-> level layout, host side, converted to T
+template <typename T>
+int level_upload(pp_ctx *c, const Level &L, const double *src, T *dst) {
+  std::vector<T> h((size_t)L.total, T(0));
+  for (int k = 0; k < L.cz; k++)
+    for (int j = 0; j < L.cy; j++)
+      for (int i = 0; i < L.cx; i++)
+        h[lidx(L.sx, L.sxy, i, j, k)] = (T)src[i + (long)L.cx * (j + (long)L.cy * k)];
+  HIPCHK(hipMemcpy(dst, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return PP_OK;
+}
+template <typename T>
+int level_download(pp_ctx *c, const Level &L, const T *src, double *dst) {
+  std::vector<T> h((size_t)L.total);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(h.data(), src, sizeof(T) * h.size(), hipMemcpyDeviceToHost));
+  for (int k = 0; k < L.cz; k++)
+    for (int j = 0; j < L.cy; j++)
+      for (int i = 0; i < L.cx; i++) dst[i + (long)L.cx * (j + (long)L.cy * k)] = (double)h[lidx(L.sx, L.sxy, i, j, k)];
+  return PP_OK;
+}
+
+template <typename T>
+int level_apply(pp_ctx *c, int op, int l, double omega, const double *a, const double *b, double *out) {
+  Level &L = c->lv[l];
+  kreset(c, false);
+  int rc = PP_OK;
+  switch (op) {
+    case PP_LOP_RESIDUAL:
+      if ((rc = level_upload<T>(c, L, a, lv_e<T>(L))) || (rc = level_upload<T>(c, L, b, lv_rhs<T>(L)))) return rc;
+      launch_residual<T>(c, L);
+      return level_download<T>(c, L, lv_res<T>(L), out);
+    case PP_LOP_RESTRICT: {
+      Level &C = c->lv[l + 1];
+      if ((rc = level_upload<T>(c, L, a, lv_res<T>(L)))) return rc;
+      CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
+      k_restrict<T><<<cc.grid, cc.block, 0, c->stream>>>(largs<T>(L), largs<T>(C), C.cf[0], C.cf[1], C.cf[2],
+                                                          lv_res<T>(L), lv_rhs<T>(C), lv_e<T>(C));
+      return level_download<T>(c, C, lv_rhs<T>(C), out);
+    }
+    case PP_LOP_RESRESTRICT:
+      if ((rc = level_upload<T>(c, L, a, lv_e<T>(L))) || (rc = level_upload<T>(c, L, b, lv_rhs<T>(L)))) return rc;
+      restrict_step<T>(c, l);
+      return level_download<T>(c, c->lv[l + 1], lv_rhs<T>(c->lv[l + 1]), out);
+    case PP_LOP_PROLONG:
+      if ((rc = level_upload<T>(c, c->lv[l + 1], a, lv_e<T>(c->lv[l + 1]))) ||
+          (rc = level_upload<T>(c, L, b, lv_e<T>(L))))
+        return rc;
+      prolong_step<T>(c, l);
+      return level_download<T>(c, L, lv_e<T>(L), out);
+    default:  // PP_LOP_LINE_PASS
+      if ((rc = level_upload<T>(c, L, a, lv_e<T>(L))) || (rc = level_upload<T>(c, L, b, lv_rhs<T>(L)))) return rc;
+      line_passes<T>(c, l, (T)omega);
+      return level_download<T>(c, L, lv_e<T>(L), out);
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int32_t pp_level_apply(pp_ctx *c, int32_t op, int32_t level, int32_t precision, double omega, const double *a,
+                       const double *b, double *out) {
+  if (!c || !a || !out) { set_error("pp_level_apply: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_level_apply: no geometry"); return PP_NO_GEOMETRY; }
+  const int nl = (int)c->lv.size();
+  const bool two_levels = op == PP_LOP_RESTRICT || op == PP_LOP_RESRESTRICT || op == PP_LOP_PROLONG;
+  if (op < PP_LOP_RESIDUAL || op > PP_LOP_LINE_PASS || level < 0 || level >= nl - (two_levels ? 1 : 0) ||
+      (precision != PP_PREC_FP64 && precision != PP_PREC_MIXED) || (op != PP_LOP_RESTRICT && !b) ||
+      (op == PP_LOP_LINE_PASS && !(omega > 0.0 && omega < 2.0))) {
+    set_error("pp_level_apply: bad op %d, level %d (of %d), precision %d, omega %g or missing b", op, level, nl,
+              precision, omega);
+    return PP_BAD_ARG;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  int rc = precision == PP_PREC_MIXED ? level_apply<float>(c, op, level, omega, a, b, out)
+                                      : level_apply<double>(c, op, level, omega, a, b, out);
+  if (rc) return rc;
+  HIPCHK(hipGetLastError());
+  return PP_OK;
+}
+
 int32_t pp_level_info(pp_ctx *c, int32_t *n_levels, int32_t *dims) {
   if (!c || !n_levels) { set_error("pp_level_info: null argument"); return PP_BAD_ARG; }
   *n_levels = (int32_t)c->lv.size();
@@ -1923,8 +2048,7 @@ int32_t pp_smooth(pp_ctx *c, int32_t smoother, double omega, int32_t sweeps, dou
 int32_t pp_tdma_batched(pp_ctx *c, const double *a, const double *b, const double *cc, const double *r, double *u,
                         int32_t n, int32_t nl, int32_t method) {
   if (!c || !a || !b || !cc || !r || !u || n < 1 || nl < 1) { set_error("pp_tdma_batched: bad argument"); return PP_BAD_ARG; }
-  if (method == 0 && n > 512) { set_error("pp_tdma_batched: wavefront PCR holds at most 512 rows"); return PP_BAD_ARG; }
-  if (method == 1 && n > 512) { set_error("pp_tdma_batched: partitioned Thomas holds at most 512 rows"); return PP_BAD_ARG; }
+  if (n > kMaxLine) { set_error("pp_tdma_batched: the line solvers hold at most %d rows", kMaxLine); return PP_BAD_ARG; }
   if (method != 0 && method != 1) { set_error("pp_tdma_batched: method must be 0 or 1"); return PP_BAD_ARG; }
   HIPCHK(hipSetDevice(c->device));
   size_t nb = sizeof(double) * (size_t)n * nl;

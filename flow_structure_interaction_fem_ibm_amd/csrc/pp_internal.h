@@ -108,6 +108,17 @@ struct GhostTables {
   double *t_coef64 = nullptr, *t_vinv64 = nullptr;
   float *t_coef32 = nullptr, *t_vinv32 = nullptr;  // vinv = 1/V (fluid), 0 (ghost)
   int *g_tidx = nullptr;                        // touched index of a ghost cell or -1
+  // transpose of the composite rows, CSR by fluid cell: the solved
+  // transposed ghost coupling of the fluid rows in one gather,
+  // out_f = base_f + vinv_f sum_g M[g,f] r_g with r_g the ghost sources
+  int nct = 0;
+  long *ct_ref = nullptr, *ct_al = nullptr;
+  int *ct_ptr = nullptr, *ct_ghost = nullptr;
+  double *ct_w64 = nullptr, *ct_vinv64 = nullptr, *ctbase64 = nullptr;
+  float *ct_w32 = nullptr, *ct_vinv32 = nullptr, *ctbase32 = nullptr;
+  std::vector<long> h_ct_ref, h_ct_al;
+  std::vector<int> h_ct_ptr, h_ct_ghost;
+  std::vector<double> h_ct_w, h_ct_vinv;
   // adjoint work buffers (inner V-cycle, per precision) and the FP64
   // outer loop's own copies
   double *lam64[2] = {nullptr, nullptr}, *rhsg64 = nullptr, *tbase64 = nullptr;

@@ -784,6 +784,23 @@ __global__ void __launch_bounds__(256) k_adj_touch(int nt, const long *__restric
   target[t_idx[t]] = (base ? base[t] : T(0)) + vi * s;
 }
 
+// the solved transposed ghost coupling in one gather (transpose of the
+// composite rows M, CSR by fluid cell):
+//   target[ct_idx[t]] = base[t] + vinv[t] * sum_q M[g_q, f_t] rhsg[g_q]
+// (base may be null: 0).  Fixed summation order: deterministic.
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_composite(int nct, const long *__restrict__ ct_idx,
+                                                       const int *__restrict__ ct_ptr,
+                                                       const int *__restrict__ ct_ghost, const T *__restrict__ ct_w,
+                                                       const T *__restrict__ vinv, const T *__restrict__ rhsg,
+                                                       const T *__restrict__ base, T *__restrict__ target) {
+  int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nct) return;
+  T s = T(0);
+  for (int q = ct_ptr[t]; q < ct_ptr[t + 1]; q++) s += ct_w[q] * rhsg[ct_ghost[q]];
+  target[ct_idx[t]] = (base ? base[t] : T(0)) + vinv[t] * s;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_gather_idx(int n, const long *__restrict__ idx, const T *__restrict__ src,
                                                     T *__restrict__ dst) {
@@ -965,7 +982,10 @@ __global__ void __launch_bounds__(256) k_jacobi(LevelArgs<T> L, const T *__restr
 // restriction: coarse rhs = fluid-volume-weighted mean of the children's
 // residuals (the reference's masked averaging, MG_Restrict_X/Y/Z
 // CORE_MG.F90:2511-2538, applied over all coarsened directions at once);
-// 0 where no child is fluid.  Also zeroes the coarse correction.
+// 0 where no child is fluid.  Also zeroes the coarse correction.  The
+// children of coarse cell I along a direction coarsened by f are f*I ..
+// f*I+f-1, and the last coarse cell also takes the cell an odd fine count
+// leaves over.
 // ---------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_restrict(LevelArgs<T> F, LevelArgs<T> C, int fx, int fy, int fz,
@@ -974,14 +994,14 @@ __global__ void __launch_bounds__(256) k_restrict(LevelArgs<T> F, LevelArgs<T> C
   int I = blockIdx.x * blockDim.x + threadIdx.x;
   int J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
   if (I >= C.cx || J >= C.cy) return;
+  const int i0 = I * fx, i1 = I == C.cx - 1 ? F.cx : i0 + fx;
+  const int j0 = J * fy, j1 = J == C.cy - 1 ? F.cy : j0 + fy;
+  const int k0 = K * fz, k1 = K == C.cz - 1 ? F.cz : k0 + fz;
   T num = T(0), den = T(0);
-  for (int c = 0; c < fz; c++) {
-    int k = K * fz + c;
-    for (int b = 0; b < fy; b++) {
-      int j = J * fy + b;
+  for (int k = k0; k < k1; k++) {
+    for (int j = j0; j < j1; j++) {
       T wyz = F.wy[j] * F.wz[k];
-      for (int a = 0; a < fx; a++) {
-        int i = I * fx + a;
+      for (int i = i0; i < i1; i++) {
         long id = lidx(F.sx, F.sxy, i, j, k);
         if (!(F.fl[id] & FL_BLANK)) {
           T v = F.wx[i] * wyz;

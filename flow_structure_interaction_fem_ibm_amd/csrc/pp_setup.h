@@ -13,10 +13,12 @@ void make_grid_dir(int n, const double *xn, std::vector<double> &x, std::vector<
                    std::vector<double> &dxcinv);
 int build_levels(pp_ctx *c);
 // Cells of the coarsest level build_levels reaches for a grid, and the most
-// cells the coarsest level may have: its dense solution operator (FP64 and
-// FP32 copies, n^2 entries each) and the bordered LU behind it grow as n^2.
+// cells the coarsest level may have: its dense FP64 solution operator
+// (n^2 entries) and the bordered LU behind it grow as n^2.
 long coarsest_cells(int nxc, int nyc, int nzc);
 constexpr long kCoarseDenseMax = 16384;
+// longest line the x-line (wavefront) and y/z-line (partitioned) solvers hold
+constexpr int kMaxLine = 1024;
 void build_flags(pp_ctx *c);
 int upload_levels(pp_ctx *c);
 int upload_flags(pp_ctx *c);

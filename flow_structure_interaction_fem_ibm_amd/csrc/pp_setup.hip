@@ -131,7 +131,9 @@ void free_ghosts(pp_ctx *c) {
                   g.nb_ref, g.nb_al, g.nb_w64, g.nb_w32, g.t_ref, g.t_al, g.t_ptr, g.t_ghost, g.t_coef64,
                   g.t_coef32, g.t_vinv64, g.t_vinv32, g.g_tidx, g.lam64[0], g.lam64[1], g.rhsg64,
                   g.tbase64, g.lam32[0], g.lam32[1], g.rhsg32, g.tbase32, g.olam64[0], g.olam64[1],
-                  g.orhsg64, g.otbase64, g.v_cD, g.v_buf[0], g.v_buf[1], g.v_kind, g.v_off};
+                  g.orhsg64, g.otbase64, g.v_cD, g.v_buf[0], g.v_buf[1], g.v_kind, g.v_off,
+                  g.ct_ref, g.ct_al, g.ct_ptr, g.ct_ghost, g.ct_w64, g.ct_vinv64, g.ctbase64,
+                  g.ct_w32, g.ct_vinv32, g.ctbase32};
   for (void *p : ptrs) hipFree(p);
   g = GhostTables();
   hipFree(c->cs.G); hipFree(c->cs.cell);
@@ -139,10 +141,17 @@ void free_ghosts(pp_ctx *c) {
 }
 
 // Level hierarchy: level 0 is the grid; a direction is halved while its
-// cell count is even and at least 4; coarsening stops when the level has at
-// most kCoarseMax cells or no direction can be halved.
+// cell count is at least 4 (n -> n/2 coarse cells); coarsening stops when
+// the level has at most kCoarseMax cells or no direction can be halved.
+// An odd count keeps every other fine node and the last one, so the last
+// coarse cell agglomerates three fine cells and the coarse level spans the
+// whole domain (the reference's MG_initial halves odd counts by dropping the
+// last cell instead, CORE_MG.F90:98-126; the coarse levels only precondition,
+// so the converged solution is the same).
 static constexpr long kCoarseMax = 4096;
 static constexpr int kMaxLevels = 12;
+
+static int coarsen_factor(int n) { return n >= 4 ? 2 : 1; }
 
 long coarsest_cells(int nxc, int nyc, int nzc) {
   int n[3] = {nxc, nyc, nzc};
@@ -151,7 +160,7 @@ long coarsest_cells(int nxc, int nyc, int nzc) {
     int cf[3];
     bool any = false;
     for (int d = 0; d < 3; d++) {
-      cf[d] = (n[d] % 2 == 0 && n[d] >= 4) ? 2 : 1;
+      cf[d] = coarsen_factor(n[d]);
       any |= cf[d] == 2;
     }
     if (!any) break;
@@ -178,7 +187,7 @@ int build_levels(pp_ctx *c) {
     int cf[3];
     bool any = false;
     for (int d = 0; d < 3; d++) {
-      cf[d] = (ncell[d] % 2 == 0 && ncell[d] >= 4) ? 2 : 1;
+      cf[d] = coarsen_factor(ncell[d]);
       any |= cf[d] == 2;
     }
     if (!any) break;
@@ -189,9 +198,12 @@ int build_levels(pp_ctx *c) {
       if (cf[d] == 1) {
         C.node[d] = F.node[d];
       } else {
+        // every other fine node; the last coarse node is the last fine node
+        // (for an odd count the last coarse cell spans three fine cells)
         int nc = ncell[d] / 2;
         C.node[d].resize(nc + 1);
-        for (int q = 0; q <= nc; q++) C.node[d][q] = F.node[d][2 * q];
+        for (int q = 0; q < nc; q++) C.node[d][q] = F.node[d][2 * q];
+        C.node[d][nc] = F.node[d][ncell[d]];
       }
     }
     lv.push_back(C);
@@ -211,10 +223,13 @@ int build_levels(pp_ctx *c) {
       F.par[d].assign(nf, 0); F.oth[d].assign(nf, 0); F.wpar[d].assign(nf, 1.0);
       for (int i = 0; i < nf; i++) {
         if (C.cf[d] == 1) { F.par[d][i] = i; F.oth[d][i] = i; F.wpar[d][i] = 1.0; continue; }
-        int P = i / 2;
-        int O = (i % 2 == 0) ? P - 1 : P + 1;
+        // parent, and the coarse neighbour on the child's side of the
+        // parent's centre (none for a child centred on its parent)
+        const int P = std::min(i / 2, nc - 1);
+        const double xf = F.xc[d][i], xp = C.xc[d][P];
+        const int O = xf < xp ? P - 1 : (xf > xp ? P + 1 : P);
         F.par[d][i] = P;
-        if (O < 0 || O >= nc) { F.oth[d][i] = P; F.wpar[d][i] = 1.0; continue; }
+        if (O < 0 || O >= nc || O == P) { F.oth[d][i] = P; F.wpar[d][i] = 1.0; continue; }
         F.oth[d][i] = O;
         F.wpar[d][i] = (F.xc[d][i] - C.xc[d][O]) / (C.xc[d][P] - C.xc[d][O]);
       }
@@ -253,13 +268,15 @@ void build_flags(pp_ctx *c) {
         for (int k = (int)k0; k < (int)k1; k++)
           for (int j = 0; j < L.cy; j++)
             for (int i = 0; i < L.cx; i++) {
+              // children: cf per direction, the last coarse cell of a
+              // direction also takes the fine cell an odd count leaves over
+              const int i0 = i * L.cf[0], i1 = i == L.cx - 1 ? F.cx : i0 + L.cf[0];
+              const int j0 = j * L.cf[1], j1 = j == L.cy - 1 ? F.cy : j0 + L.cf[1];
+              const int k0c = k * L.cf[2], k1c = k == L.cz - 1 ? F.cz : k0c + L.cf[2];
               int all = 1;
-              for (int c3 = 0; c3 < L.cf[2]; c3++)
-                for (int b = 0; b < L.cf[1]; b++)
-                  for (int a = 0; a < L.cf[0]; a++) {
-                    long q = lidx(F.sx, F.sxy, i * L.cf[0] + a, j * L.cf[1] + b, k * L.cf[2] + c3);
-                    all &= (fh[q] & FL_BLANK) ? 1 : 0;
-                  }
+              for (int kk = k0c; kk < k1c; kk++)
+                for (int jj = j0; jj < j1; jj++)
+                  for (int ii = i0; ii < i1; ii++) all &= (fh[lidx(F.sx, F.sxy, ii, jj, kk)] & FL_BLANK) ? 1 : 0;
               blank[cidx(i, j, k)] = (uint8_t)all;
             }
       });
@@ -579,6 +596,27 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     }
     if (!ent.empty()) G.h_t_ptr.push_back((int)G.h_t_ghost.size());
     G.nt = (int)G.h_t_ref.size();
+    // transpose of the composite rows by fluid cell
+    std::vector<std::pair<long, std::pair<int, double>>> ct;
+    for (int q = 0; q < n; q++)
+      for (int e = G.h_c_ptr[q]; e < G.h_c_ptr[q + 1]; e++) ct.push_back({G.h_c_ref[e], {q, G.h_c_w[e]}});
+    std::stable_sort(ct.begin(), ct.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    G.h_ct_ref.clear(); G.h_ct_al.clear(); G.h_ct_ptr.assign(1, 0); G.h_ct_ghost.clear(); G.h_ct_w.clear();
+    G.h_ct_vinv.clear();
+    for (size_t e = 0; e < ct.size(); e++) {
+      const long rs = ct[e].first;
+      if (e == 0 || rs != ct[e - 1].first) {
+        if (e > 0) G.h_ct_ptr.push_back((int)G.h_ct_ghost.size());
+        long z = rs / c->ref.exy, rem = rs % c->ref.exy, y = rem / c->ref.ex, x = rem % c->ref.ex;
+        G.h_ct_ref.push_back(rs);
+        G.h_ct_al.push_back(lidx(L0.sx, L0.sxy, (int)x - 1, (int)y - 1, (int)z - 1));
+        G.h_ct_vinv.push_back(1.0 / V((int)x, (int)y, (int)z));
+      }
+      G.h_ct_ghost.push_back(ct[e].second.first);
+      G.h_ct_w.push_back(ct[e].second.second);
+    }
+    if (!ct.empty()) G.h_ct_ptr.push_back((int)G.h_ct_ghost.size());
+    G.nct = (int)G.h_ct_ref.size();
   }
   G.g_ref = dev_upload(G.h_g_ref); G.g_al = dev_upload(G.h_g_al);
   G.c_ptr = dev_upload(G.h_c_ptr); G.c_ref = dev_upload(G.h_c_ref); G.c_al = dev_upload(G.h_c_al);
@@ -589,6 +627,13 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
   G.t_ghost = dev_upload(G.h_t_ghost); G.t_coef64 = dev_upload(G.h_t_coef);
   G.t_coef32 = dev_upload(to_f32(G.h_t_coef)); G.t_vinv64 = dev_upload(G.h_t_vinv);
   G.t_vinv32 = dev_upload(to_f32(G.h_t_vinv)); G.g_tidx = dev_upload(G.h_g_tidx);
+  G.ct_ref = dev_upload(G.h_ct_ref); G.ct_al = dev_upload(G.h_ct_al); G.ct_ptr = dev_upload(G.h_ct_ptr);
+  G.ct_ghost = dev_upload(G.h_ct_ghost); G.ct_w64 = dev_upload(G.h_ct_w); G.ct_w32 = dev_upload(to_f32(G.h_ct_w));
+  G.ct_vinv64 = dev_upload(G.h_ct_vinv); G.ct_vinv32 = dev_upload(to_f32(G.h_ct_vinv));
+  if (G.nct > 0) {
+    hipMalloc(&G.ctbase64, sizeof(double) * G.nct);
+    hipMalloc(&G.ctbase32, sizeof(float) * G.nct);
+  }
   if (n > 0) {
     for (int b = 0; b < 2; b++) {
       hipMalloc(&G.lam64[b], sizeof(double) * n);

@@ -83,6 +83,7 @@ class SolveOpts(C.Structure):
 
 PP_SMOOTH_LINE = 0
 PP_SMOOTH_POINT_JACOBI = 1
+PP_LOP_RESIDUAL, PP_LOP_RESTRICT, PP_LOP_RESRESTRICT, PP_LOP_PROLONG, PP_LOP_LINE_PASS = range(5)
 
 KERNEL_CLASSES = ("outer_residual", "update", "xline", "yline", "zline", "residual", "restrict",
                   "prolong", "ghost", "coarse", "other", "krylov", "jacobi")
@@ -158,6 +159,8 @@ def lib():
     L.pp_level_info.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.pp_time_kernel.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp]
     L.pp_setup_info.argtypes = [P, dp, C.POINTER(C.c_int32), dp]
+    L.pp_level_apply.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_double, P, P, P]
+    L.pp_level_data.argtypes = [P, C.c_int32, P, P, P, P, P]
     L.pp_geom_build.argtypes = [C.POINTER(C.c_void_p), C.POINTER(GridDesc), C.c_int32,
                                 C.POINTER(BodyDesc), C.c_int32, C.c_double, P]
     L.pp_geom_counts.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -185,7 +188,8 @@ def lib():
                  "pp_geom_destroy", "pp_geom_dirichlet", "pp_set_vel_ghosts", "pp_correct_vel",
                  "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device", "pp_adjoint_device",
                  "pp_set_solve_ad", "pp_set_solve_ad_device", "pp_solve_ad", "pp_solve_ad_device",
-                 "pp_calc_residual_ad", "pp_geom_build_surfaces", "pp_geom_canonical_surface"):
+                 "pp_calc_residual_ad", "pp_geom_build_surfaces", "pp_geom_canonical_surface",
+                 "pp_level_apply", "pp_level_data"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -198,7 +202,8 @@ EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_s
             "pp_geom_ghost_elements", "pp_geom_destroy", "pp_geom_dirichlet", "pp_set_vel_ghosts",
             "pp_correct_vel", "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device",
             "pp_adjoint_device", "pp_set_solve_ad", "pp_set_solve_ad_device", "pp_solve_ad", "pp_solve_ad_device",
-            "pp_calc_residual_ad", "pp_geom_build_surfaces", "pp_geom_canonical_surface")
+            "pp_calc_residual_ad", "pp_geom_build_surfaces", "pp_geom_canonical_surface", "pp_level_apply",
+            "pp_level_data")
 
 
 def _check(rc, allow=(PP_OK,)):
@@ -481,6 +486,26 @@ class PressureSolver:
         us, b = C.c_double(), C.c_double()
         _check(lib().pp_time_kernel(self.h, cls, level, precision, reps, C.byref(us), C.byref(b)))
         return us.value, b.value
+
+    def level_data(self, level):
+        """pp_level_data: (flags (cz, cy, cx) uint8, {w, xc, cm, cp: [x, y, z] arrays})."""
+        cx, cy, cz = self.level_info()[level]
+        fl = np.zeros((cz, cy, cx), np.uint8)
+        arrs = {k: np.zeros(cx + cy + cz) for k in ("w", "xc", "cm", "cp")}
+        _check(lib().pp_level_data(self.h, level, fl.ctypes.data, *[arrs[k].ctypes.data for k in ("w", "xc", "cm", "cp")]))
+        return fl, {k: [v[:cx], v[cx:cx + cy], v[cx + cy:]] for k, v in arrs.items()}
+
+    def level_apply(self, op, level, a, b=None, precision=PP_PREC_FP64, omega=1.0):
+        """pp_level_apply on dense level arrays (cz, cy, cx); returns out."""
+        dims = self.level_info()
+        lo = level + 1 if op in (PP_LOP_RESTRICT, PP_LOP_RESRESTRICT) else level
+        cx, cy, cz = dims[lo]
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        out = np.zeros((cz, cy, cx))
+        _check(lib().pp_level_apply(self.h, int(op), int(level), int(precision), float(omega), a.ctypes.data,
+                                    None if bb is None else bb.ctypes.data, out.ctypes.data))
+        return out
 
     def smooth(self, var, rhs, smoother=PP_SMOOTH_POINT_JACOBI, omega=1.0, sweeps=1):
         """pp_smooth: sweeps of the finest-level smoother on a copy of var

@@ -146,11 +146,11 @@ void pp_default_opts(pp_solve_opts *o);
 const char *pp_last_error(void);
 
 /* Create a context on one GPU for a grid (replaces MG_initial,
- * CORE_MG.F90:31-278, and the grid half of init_simulation).  4 to 512
- * cells per direction, and a grid whose levels (a direction is halved
- * while its cell count is even and at least 4, down to at most 4096 cells)
- * end at a coarsest level of at most 16384 cells, which is solved directly
- * (PP_BAD_ARG otherwise). */
+ * CORE_MG.F90:31-278, and the grid half of init_simulation).  4 to 1024
+ * cells per direction (PP_BAD_ARG otherwise).  Levels: a direction is
+ * halved while its cell count is at least 4 (an odd count's last coarse cell
+ * spans three fine cells), down to at most 4096 cells, which are solved
+ * directly. */
 int32_t pp_create(pp_ctx **ctx, const pp_grid_desc *g);
 int32_t pp_destroy(pp_ctx *ctx);
 
@@ -333,13 +333,42 @@ int32_t pp_calc_residual_ad(pp_ctx *ctx, int32_t advec_scheme,
                             int32_t loc[3]);
 
 /* tdma (CORE_TDMA.F90:10-34) on a batch of independent systems, through
- * the device line solvers: n_lines systems of length n stored line after
+ * the device line solvers (n <= 1024): n_lines systems of length n stored line after
  * line (a: sub-, b: main, c: super-diagonal, r: rhs, u: solution).
  * method 0 = wavefront PCR (lines along the contiguous index),
  * method 1 = partitioned Thomas (lines across it). FP64. */
 int32_t pp_tdma_batched(pp_ctx *ctx, const double *a, const double *b,
                         const double *c, const double *r, double *u,
                         int32_t n, int32_t n_lines, int32_t method);
+/* ---- single multigrid operators of one level (test and diagnostic access;
+ *      each is one step of the V-cycle: no counterpart in the reference as a
+ *      separate call) ----------------------------------------------------
+ * Host arrays hold the cells of a level densely, i fastest: cx*cy*cz values
+ * (pp_level_info dims).  precision: PP_PREC_FP64 or PP_PREC_MIXED (FP32, the
+ * mixed solve's V-cycles).  The level's work arrays are overwritten (later
+ * solves start from fresh ones). */
+#define PP_LOP_RESIDUAL 0     /* out(l) = b(l) - A_l a(l) on fluid cells, 0 on blank
+                                 cells (MG_residual, CORE_MG.F90:2274-2387) */
+#define PP_LOP_RESTRICT 1     /* out(l+1) = fluid-volume-weighted mean of a(l) over the
+                                 children (MG_Restrict_X/Y/Z, CORE_MG.F90:2464-2723) */
+#define PP_LOP_RESRESTRICT 2  /* out(l+1) = the restriction of b(l) - A_l a(l), as the
+                                 V-cycle forms it (fused where level l halves in x, y, z) */
+#define PP_LOP_PROLONG 3      /* out(l) = b(l) + prolongation of a(l+1) on fluid cells
+                                 (MG_Prolong_X/Y/Z + correction, CORE_MG.F90:2885-3523,
+                                 1354-1355) */
+#define PP_LOP_LINE_PASS 4    /* out(l) = a(l) after one zebra x, y, z line pass set with
+                                 right-hand side b(l) and weight omega (MG_itsolv,
+                                 CORE_MG.F90:1883-2270) */
+int32_t pp_level_apply(pp_ctx *ctx, int32_t op, int32_t level, int32_t precision,
+                       double omega, const double *a, const double *b, double *out);
+/* Level l's flags (cx*cy*cz bytes, i fastest: bit0 blank, bit1 ghost, bits
+ * 2-7 face closed on the i-,i+,j-,j+,k-,k+ side; MG_Prepare_Iblank,
+ * CORE_MG.F90:699-822, and MG_Prepare, :1716-1879) and per direction the
+ * cell widths, centres and face coefficients dxcinv(face)*dxinv(cell)
+ * (MG_initial, :186-205), each as cx + cy + cz values (x, then y, then z).
+ * Any pointer may be NULL. */
+int32_t pp_level_data(pp_ctx *ctx, int32_t level, uint8_t *flags, double *w,
+                      double *xc, double *cm, double *cp);
 /* Left null vector of the ghost-cell operator (A^T w = 0), normalised so
  * that sum over fluid cells of w equals the fluid volume; reference layout,
  * fluid and ghost entries.  No counterpart in the reference. */

@@ -48,22 +48,21 @@ def test_create_rejects_bad_grids_without_touching_a_device():
     assert b"PP_BC_NEUMANN" in lib.pp_last_error()
 
 
-def test_create_rejects_grids_without_a_small_coarsest_level():
-    """A direction is halved only while its cell count is even; a grid that
-    cannot be coarsened to a small level (65 x 49 x 41 cells: 130585) would
-    need a dense coarsest operator of n^2 entries.  pp_create refuses it
-    before touching a device; 25^3 (15625 cells, no coarsening) is within
-    the direct-solve bound and passes this check (it then fails here only
-    for want of a device)."""
+def test_create_accepts_the_grids_the_reference_coarsens():
+    """Odd cell counts coarsen too (the last coarse cell takes three fine
+    cells), so grids the reference's MG_initial halves by dropping a cell
+    (CORE_MG.F90:98-126) pass pp_create's checks: 150^3, 130x98x82,
+    600x64x64, 65x49x41, 1024 cells along a line.  Here they fail only for
+    want of a device.  A line of 1025 cells is refused."""
     import numpy as np
     lib = pp.lib()
     h = ctypes.c_void_p()
-    g = pp._grid_desc(np.linspace(0, 1, 66), np.linspace(0, 1, 50), np.linspace(0, 1, 42))
+    for dims in ((150, 150, 150), (130, 98, 82), (600, 64, 64), (65, 49, 41), (1024, 8, 6)):
+        g = pp._grid_desc(*(np.linspace(0, 1, n + 1) for n in dims))
+        rc = lib.pp_create(ctypes.byref(h), ctypes.byref(g))
+        assert rc != pp.PP_BAD_ARG, (dims, lib.pp_last_error())
+        if rc == pp.PP_OK:
+            lib.pp_destroy(h)
+    g = pp._grid_desc(np.linspace(0, 1, 1026), np.linspace(0, 1, 9), np.linspace(0, 1, 9))
     assert lib.pp_create(ctypes.byref(h), ctypes.byref(g)) == pp.PP_BAD_ARG
-    msg = lib.pp_last_error()
-    assert b"130585-cell coarsest level" in msg, msg
-    g = pp._grid_desc(np.linspace(0, 1, 26), np.linspace(0, 1, 26), np.linspace(0, 1, 26))
-    rc = lib.pp_create(ctypes.byref(h), ctypes.byref(g))
-    assert rc != pp.PP_BAD_ARG or b"coarsest" not in lib.pp_last_error()
-    if rc == pp.PP_OK:
-        lib.pp_destroy(h)
+    assert b"1024 cells per line" in lib.pp_last_error()

@@ -68,10 +68,13 @@ def test_solve_with_body_matches_sparse_direct(prec):
     assert st["rhs_shift"] == pytest.approx((wd[fl] * b[fl]).sum() / wd[fl].sum(), rel=1e-8, abs=1e-14)
 
 
-def test_config1_geometry_solve_matches_oracle_mg():
-    """Config 1 geometry (64^3, sphere r=0.15): the mixed-precision solve and
-    the oracle's reference multigrid on the same projected RHS agree."""
-    announce("config-1 64^3 sphere: GPU mixed vs oracle MG")
+@pytest.mark.parametrize("prec", [pp.PP_PREC_FP64, pp.PP_PREC_MIXED])
+def test_config1_geometry_solve_matches_oracle_mg(prec):
+    """Config 1 (64^3 uniform, sphere r=0.15, U[-1,1] RHS): the plain FP64
+    multigrid (FP64 V-cycles, nu = 2+2, iterative refinement without Krylov)
+    and the mixed-precision solve agree with the oracle's reference
+    multigrid on the same projected RHS."""
+    announce(f"config-1 64^3 sphere: GPU prec={prec} vs oracle MG")
     n = 64
     xn = cases.uniform(n)
     ib, fr, gs, ex = pp.build_geometry(xn, xn, xn, [pp.Ellipsoid((0.5, 0.5, 0.5), (0.15, 0.15, 0.15))])
@@ -90,7 +93,8 @@ def test_config1_geometry_solve_matches_oracle_mg():
     assert np.abs(atw).max() < 1e-8 * np.abs(w).max() * n * n * 6
     bp, fl = projected_rhs(b, w, ib)
     p = np.zeros(ib.shape)
-    st = s.solve(p, b, rel_tol=1e-11, max_outer=80)
+    kw = dict(krylov=0, nu_pre=2, nu_post=2, nu_coarse=2, omega=1.0) if prec == pp.PP_PREC_FP64 else {}
+    st = s.solve(p, b, rel_tol=1e-11, max_outer=200, precision=prec, **kw)
     assert st["status"] == "PP_OK", st
     q = np.zeros(ib.shape)
     rc, ost = o.mg_solve(q, bp, rel_tol=1e-12, iter_max=400)
