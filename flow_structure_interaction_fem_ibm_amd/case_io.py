@@ -10,7 +10,11 @@ handed back to it):
   read_inputs_bound (MAIN.F90:915-1166), with the reference's marker count
   n_phi from n_theta and the aspect ratio;
 * unstruc_surface_in.dat: one triangulated surface per UNSTRUCTURED_SURFACE
-  body (BOUNDARY_MARKER_INIT.F90:209-248), for pp.build_geometry_surfaces.
+  body (BOUNDARY_MARKER_INIT.F90:209-248), for pp.build_geometry_surfaces;
+* the unformatted restart pair restart_flow_out{1,2}.dat and
+  restart_body_out{1,2}.dat (write_restart, MAIN.F90:2035-2191; read back by
+  read_restart_flow / read_restart_body, :1665-2031): the flow fields and the
+  body state, iblank and fresh cells, record by record.
 
 Fortran list-directed input is restated for what these files use: values
 separated by blanks and/or one comma, null values (two commas with only
@@ -25,6 +29,7 @@ from __future__ import annotations
 import math
 import os
 import re
+import struct
 
 import numpy as np
 
@@ -241,16 +246,36 @@ def case_nodes(case_dir, vals=None):
 
 
 # ---------------------------------------------------------------------------
-# canonical_body_in.dat (MAIN.F90:915-1166), ELLIPSOID bodies
+# canonical_body_in.dat (read_inputs_bound, MAIN.F90:915-1300)
 # ---------------------------------------------------------------------------
 ELLIPSOID = 3
+UNSTRUCTURED_SURFACE = 4
 GCM_METHOD = 2
-_ELLIPSOID_RECORDS = [
-    "n_theta:i dummy:i", "radiusx:r radiusy:r radiusz:r", "xcent:r ycent:r zcent:r", "alpha:r",
+_MOTION_RECORDS = [
     "vxcentTrans:r vycentTrans:r vzcentTrans:r", "ampx:r ampy:r ampz:r", "freqx:r freqy:r freqz:r",
     "angvx:r angvy:r angvz:r", "phase:r", "ampangx:r ampangy:r ampangz:r", "freqangx:r freqangy:r freqangz:r",
     "density_fluid:r density_solid:r", "xcentConstr:r ycentConstr:r zcentConstr:r",
 ]
+# per canonical_body_type, the records after wall_type (MAIN.F90:1118-1212)
+_BODY_RECORDS = {
+    1: ["nPtsBodyMarker:i dummy:i", "radiusx:r radiusy:r", "xcent:r ycent:r", "alpha:r"] + _MOTION_RECORDS,
+    2: ["nPtsBodyMarker:i dummy:i", "radiusx:r radiusy:r", "xcent:r ycent:r", "alpha:r"] + _MOTION_RECORDS,
+    ELLIPSOID: ["n_theta:i dummy:i", "radiusx:r radiusy:r radiusz:r", "xcent:r ycent:r zcent:r", "alpha:r"]
+               + _MOTION_RECORDS,
+    4: ["dof1:i dof2:i dof3:i dof4:i dof5:i dof6:i", "nPtsBodyMarker:i totNumTriElem:i",
+        "rigidRef1:i rigidRef2:i rigidRef3:i", "dummy1:r dummy2:r dummy3:r", "xcent:r ycent:r zcent:r",
+        "quat0:r quat1:r quat2:r quat3:r"] + _MOTION_RECORDS
+       + ["hingemarker:i", "thickoverlength:r", "Limiter_angle_amp:r", "A_x:r A_theta:r",
+          "FreqN_torsion:r Freq_prsb:r"],
+}
+_ELLIPSOID_RECORDS = _BODY_RECORDS[ELLIPSOID]
+
+
+def _body_records(b):
+    recs = list(_BODY_RECORDS[b["canonical_body_type"]])
+    if b["canonical_body_type"] == 4 and b.get("body_dim", 3) > 2:
+        recs.append("DepthOverLength:r")
+    return recs
 
 
 def ellipsoid_n_phi(n_theta, radiusx, radiusz, boundary_formulation=GCM_METHOD):
@@ -267,8 +292,10 @@ def ellipsoid_n_phi(n_theta, radiusx, radiusz, boundary_formulation=GCM_METHOD):
 
 
 def read_canonical_bodies(path, boundary_formulation=GCM_METHOD):
-    """(header, bodies): the header record and one dict per body; bodies
-    other than ELLIPSOID raise NotImplementedError."""
+    """(header, bodies): the header record, one dict per body (ELLIPTIC_CYLINDER,
+    GENERAL_CYLINDER, ELLIPSOID or UNSTRUCTURED_SURFACE records) and, in the
+    header, the channel-flow gates (a list of dicts) and Prsb_MomentRef that
+    follow the bodies."""
     with open(path) as f:
         lines = iter(f.read().splitlines())
     head = read_record(lines, _spec("nbody:i nbody_solid:i nbody_membrane:i nSection:i channel_flow:l "
@@ -280,44 +307,109 @@ def read_canonical_bodies(path, boundary_formulation=GCM_METHOD):
             spec += " zoneMax:i"
         b = read_record(lines, _spec(spec))
         read_record(lines, _spec("wall_type:i"), b)
-        if b["canonical_body_type"] != ELLIPSOID:
-            raise NotImplementedError(f"body {n + 1}: canonical_body_type {b['canonical_body_type']} "
-                                      f"(only ELLIPSOID = {ELLIPSOID} is read)")
-        for rec in _ELLIPSOID_RECORDS:
+        if b["canonical_body_type"] not in _BODY_RECORDS:
+            raise ValueError(f"body {n + 1}: unknown canonical_body_type {b['canonical_body_type']}")
+        for rec in _body_records(b):
             read_record(lines, _spec(rec), b)
-        b["n_phi"] = ellipsoid_n_phi(b["n_theta"], b["radiusx"], b["radiusz"], boundary_formulation)
+        if b["canonical_body_type"] == ELLIPSOID:
+            b["n_phi"] = ellipsoid_n_phi(b["n_theta"], b["radiusx"], b["radiusz"], boundary_formulation)
+            b["nPtsBodyMarker"] = b["n_theta"] * b["n_phi"]
+        if b["canonical_body_type"] == UNSTRUCTURED_SURFACE:
+            b["alpha"] = 0.0
         bodies.append(b)
+    gates = []
+    if head["channel_flow"]:
+        ng = read_record(lines, _spec("nGate:i"))["nGate"]
+        for _ in range(ng):
+            gates.append(read_record(lines, _spec("bcTypeGate:i bcGateU:r bcGateV:r bcGateW:r freqGate:r")))
+    head["gates"] = gates
+    try:
+        head.update(read_record(lines, _spec("Prsb_MomentRef:l")))
+    except StopIteration:
+        head["Prsb_MomentRef"] = False
     return head, bodies
 
 
-def write_canonical_bodies(path, bodies, channel_flow=False, n_section=0):
-    """canonical_body_in.dat for resting solid ELLIPSOID bodies (dicts with
-    n_theta, radiusx/y/z, xcent/ycent/zcent, alpha; the motion, density and
-    constraint records are written as zeros)."""
+def write_canonical_bodies(path, bodies, channel_flow=False, n_section=0, gates=(), prsb_moment_ref=False):
+    """canonical_body_in.dat for solid bodies: dicts with canonical_body_type
+    (default ELLIPSOID) and the records of its type (missing values are
+    written as zeros; an ELLIPSOID needs n_theta, radiusx/y/z,
+    xcent/ycent/zcent, alpha; an UNSTRUCTURED_SURFACE nPtsBodyMarker and
+    totNumTriElem matching its unstruc_surface_in.dat block)."""
     out = [f"{len(bodies)} {len(bodies)} 0 {n_section} {'T' if channel_flow else 'F'} F"]
     for b in bodies:
-        out.append(f"{ELLIPSOID} 3 0")
-        out.append("0")
-        vals = dict.fromkeys(n for rec in _ELLIPSOID_RECORDS for n, _ in _spec(rec))
+        b = dict(b)
+        b.setdefault("canonical_body_type", ELLIPSOID)
+        b.setdefault("body_dim", 3)
+        out.append(f"{b['canonical_body_type']} {b['body_dim']} {b.get('boundary_motion_type', 0)}")
+        out.append(str(b.get("wall_type", 0)))
+        recs = _body_records(b)
+        vals = dict.fromkeys(n for rec in recs for n, _ in _spec(rec))
         vals.update({k: 0 for k in vals})
         vals.update(b)
-        for rec in _ELLIPSOID_RECORDS:
+        for rec in recs:
             out.append(" ".join(_fmt(vals[n], k) for n, k in _spec(rec)))
+    if channel_flow:
+        out.append(str(len(gates)))
+        for g in gates:
+            out.append(" ".join(_fmt(g[n], k) for n, k in
+                                _spec("bcTypeGate:i bcGateU:r bcGateV:r bcGateW:r freqGate:r")))
+    out.append("T" if prsb_moment_ref else "F")
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
 
 
 def case_bodies(case_dir, vals=None):
     """The case's ellipsoids as pp.Ellipsoid (centre, semi-axes, alpha in
-    degrees, n_theta)."""
+    degrees, n_theta); a case with other body types raises ValueError (use
+    case_geometry)."""
     from . import pp
     vals = vals or read_input_dat(os.path.join(case_dir, "input.dat"))
     if vals.get("internal_boundary_present", 1) == 0:
         return []
     _, bodies = read_canonical_bodies(os.path.join(case_dir, "canonical_body_in.dat"),
                                       vals.get("boundary_formulation", GCM_METHOD))
+    if any(b["canonical_body_type"] != ELLIPSOID for b in bodies):
+        raise ValueError("the case has bodies other than ELLIPSOID: use case_geometry")
     return [pp.Ellipsoid((b["xcent"], b["ycent"], b["zcent"]), (b["radiusx"], b["radiusy"], b["radiusz"]),
                          b["alpha"], b["n_theta"]) for b in bodies]
+
+
+def case_geometry(case_dir, vals=None, nodes=None, iblank_prev=None, probe_norm=None):
+    """set_boundary of a case whose bodies are ELLIPSOID or UNSTRUCTURED_SURFACE
+    (in any mix): the ellipsoids' marker surfaces (as the reference builds
+    them, oriented against the origin) and the surfaces of
+    unstruc_surface_in.dat (one block per UNSTRUCTURED_SURFACE body, in body
+    order) through the triangulated pipeline (pp.build_geometry_surfaces).
+    Returns what pp.build_geometry returns.  ELLIPTIC_CYLINDER and
+    GENERAL_CYLINDER bodies (2-D) raise NotImplementedError."""
+    from . import pp
+    vals = vals or read_input_dat(os.path.join(case_dir, "input.dat"))
+    nodes = nodes or case_nodes(case_dir, vals)
+    head, bodies = read_canonical_bodies(os.path.join(case_dir, "canonical_body_in.dat"),
+                                         vals.get("boundary_formulation", GCM_METHOD))
+    lines = None
+    surfaces = []
+    for n, b in enumerate(bodies):
+        t = b["canonical_body_type"]
+        if t == ELLIPSOID:
+            xyz, tri = pp.canonical_surface(pp.Ellipsoid((b["xcent"], b["ycent"], b["zcent"]),
+                                                         (b["radiusx"], b["radiusy"], b["radiusz"]),
+                                                         b["alpha"], b["n_theta"]))
+            surfaces.append((xyz, tri, (0.0, 0.0, 0.0)))
+        elif t == UNSTRUCTURED_SURFACE:
+            if lines is None:
+                with open(os.path.join(case_dir, "unstruc_surface_in.dat")) as f:
+                    lines = iter(f.read().splitlines())
+            xyz, tri, out = read_unstruc_surface(lines, head["zoneSeparate"], head["channel_flow"])
+            if len(xyz) != b["nPtsBodyMarker"] or len(tri) != b["totNumTriElem"]:
+                raise ValueError(f"body {n + 1}: canonical_body_in.dat and unstruc_surface_in.dat disagree "
+                                 f"({b['nPtsBodyMarker']}/{b['totNumTriElem']} against {len(xyz)}/{len(tri)})")
+            surfaces.append((xyz, tri, out))
+        else:
+            raise NotImplementedError(f"body {n + 1}: canonical_body_type {t} (2-D cylinder) is not built")
+    probe = vals.get("probeLengthNormalized", 2.0) if probe_norm is None else probe_norm
+    return pp.build_geometry_surfaces(*nodes, surfaces, probe_norm=probe, iblank_prev=iblank_prev)
 
 
 # ---------------------------------------------------------------------------
@@ -359,3 +451,252 @@ def write_unstruc_surface(path, xyz, tri, outside):
     out.append(" ".join(repr(float(v)) for v in outside))
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
+
+
+# ---------------------------------------------------------------------------
+# unformatted sequential restart files (write_restart, MAIN.F90:2035-2191)
+# ---------------------------------------------------------------------------
+# A record is written as a 4-byte little-endian length, the data and the
+# length again.  A record longer than SUBRECORD_MAX bytes is split into
+# subrecords; the leading marker of a subrecord that another one follows and
+# the trailing marker of a subrecord that continues another one are negated
+# (the layout gfortran documents for its unformatted sequential files and
+# writes compatibly with ifort).  Files of the reference's builds with other
+# record-marker options (8-byte markers, big-endian) are not read.
+SUBRECORD_MAX = 2 ** 31 - 9
+
+# the flow record, in write order: 3-D fields (0:nx+1,0:ny+1,0:nz+1) and the
+# 2-D outer pressure gradients pgradx1/2 (0:ny+1,0:nz+1), pgrady1/2
+# (0:nx+1,0:nz+1), pgradz1/2 (0:nx+1,0:ny+1); REAL(8) throughout
+FLOW_FIELDS = ["u", "v", "w", "face_u", "face_v", "face_w", "p", "nluold", "nlvold", "nlwold",
+               "bcxu", "bcxv", "bcxw", "bcyu", "bcyv", "bcyw", "bczu", "bczv", "bczw",
+               "pgradx1", "pgradx2", "pgrady1", "pgrady2", "pgradz1", "pgradz2",
+               "viscTot", "bcxVisc", "bcyVisc", "bczVisc"]
+ELLIPTIC_CYLINDER, GENERAL_CYLINDER = 1, 2
+
+
+def flow_field_shape(name, nx, ny, nz):
+    """numpy shape of a flow-record field for node counts nx, ny, nz."""
+    if name.startswith("pgradx"):
+        return (nz + 2, ny + 2)
+    if name.startswith("pgrady"):
+        return (nz + 2, nx + 2)
+    if name.startswith("pgradz"):
+        return (ny + 2, nx + 2)
+    return (nz + 2, ny + 2, nx + 2)
+
+
+def write_unformatted_record(f, *items):
+    """One unformatted sequential record holding the items' bytes in order
+    (numpy arrays in Fortran order: pass C-ordered arrays whose last axis is
+    the Fortran first index)."""
+    data = b"".join(np.ascontiguousarray(a).tobytes() for a in items)
+    n = len(data)
+    if n <= SUBRECORD_MAX:
+        f.write(struct.pack("<i", n) + data + struct.pack("<i", n))
+        return
+    pos = 0
+    while pos < n:
+        m = min(SUBRECORD_MAX, n - pos)
+        lead = -m if pos + m < n else m
+        trail = -m if pos > 0 else m
+        f.write(struct.pack("<i", lead) + data[pos:pos + m] + struct.pack("<i", trail))
+        pos += m
+
+
+def read_unformatted_record(f):
+    """The data bytes of the next record (subrecords joined); EOFError at the
+    end of the file."""
+    parts = []
+    while True:
+        head = f.read(4)
+        if len(head) < 4:
+            raise EOFError("no record left")
+        lead = struct.unpack("<i", head)[0]
+        m = abs(lead)
+        data = f.read(m)
+        tail = f.read(4)
+        if len(data) != m or len(tail) != 4 or abs(struct.unpack("<i", tail)[0]) != m:
+            raise ValueError("malformed unformatted record")
+        parts.append(data)
+        if lead >= 0:
+            return b"".join(parts)
+
+
+class _Rec:
+    """Sequential reader of the items of one record."""
+
+    def __init__(self, data):
+        self.data, self.pos = data, 0
+
+    def take(self, dtype, count=1, shape=None):
+        dt = np.dtype(dtype)
+        n = dt.itemsize * count
+        if self.pos + n > len(self.data):
+            raise ValueError("record shorter than its items")
+        a = np.frombuffer(self.data, dtype=dt, count=count, offset=self.pos).copy()
+        self.pos += n
+        return a.reshape(shape) if shape is not None else a
+
+    def done(self):
+        if self.pos != len(self.data):
+            raise ValueError(f"record has {len(self.data) - self.pos} bytes beyond its items")
+
+
+def _rec(f):
+    return _Rec(read_unformatted_record(f))
+
+
+def write_restart_flow(path, nx, ny, nz, ntime, time, dt, re_, fields):
+    """restart_flow_out*.dat: nx, ny, nz (node counts, INTEGER); ntime
+    (INTEGER), time, dt, re (REAL(8)); the flow record (FLOW_FIELDS; a field
+    left out of `fields` is written as zeros)."""
+    arrays = []
+    for name in FLOW_FIELDS:
+        shp = flow_field_shape(name, nx, ny, nz)
+        a = fields.get(name)
+        a = np.zeros(shp) if a is None else np.asarray(a, dtype="<f8")
+        if a.shape != shp:
+            raise ValueError(f"{name}: shape {a.shape}, expected {shp}")
+        arrays.append(a)
+    with open(path, "wb") as f:
+        write_unformatted_record(f, np.array([nx, ny, nz], "<i4"))
+        write_unformatted_record(f, np.array([ntime], "<i4"), np.array([time, dt, re_], "<f8"))
+        write_unformatted_record(f, *arrays)
+
+
+def read_restart_flow(path):
+    """read_restart_flow (MAIN.F90:1665-1754): a dict with nx, ny, nz, ntime,
+    time, dt, re and `fields` (name -> array, FLOW_FIELDS order)."""
+    with open(path, "rb") as f:
+        r = _rec(f)
+        nx, ny, nz = (int(v) for v in r.take("<i4", 3))
+        r.done()
+        r = _rec(f)
+        ntime = int(r.take("<i4")[0])
+        time_, dt, re_ = (float(v) for v in r.take("<f8", 3))
+        r.done()
+        r = _rec(f)
+        fields = {}
+        for name in FLOW_FIELDS:
+            shp = flow_field_shape(name, nx, ny, nz)
+            fields[name] = r.take("<f8", int(np.prod(shp)), shp)
+        r.done()
+    return {"nx": nx, "ny": ny, "nz": nz, "ntime": ntime, "time": time_, "dt": dt, "re": re_, "fields": fields}
+
+
+# per body, in write order: (key, numpy dtype, count); the marker and
+# element records follow
+_BODY_HEAD = [
+    [("npts_orig", "<i4", 1), ("npts", "<i4", 1)],
+    [("canonical_body_type", "<i4", 1), ("body_dim", "<i4", 1), ("boundary_motion_type", "<i4", 1)],
+    [("wall_type", "<i4", 1)],
+    [("radius", "<f8", 3)], [("cent", "<f8", 3)], [("alpha", "<f8", 1)], [("vcent_trans", "<f8", 3)],
+    [("amp", "<f8", 3)], [("freq", "<f8", 3)], [("angv", "<f8", 3)], [("phase", "<f8", 1)],
+    [("ampang", "<f8", 3)], [("freqang", "<f8", 3)],
+]
+
+
+def _has_elements(t):
+    return t in (ELLIPTIC_CYLINDER, GENERAL_CYLINDER, UNSTRUCTURED_SURFACE)
+
+
+def write_restart_body(path, ntime, time, dt, re_, bodies, iblank, fresh, body_num=None, n_fresh=None):
+    """restart_body_out*.dat: ntime, time, dt, re; nBody; per body the head
+    records (_BODY_HEAD), the markers x(1:n), y(1:n), z(1:n), for cylinder and
+    UNSTRUCTURED_SURFACE bodies the element count, triElemNeig(1,:),
+    triElemNeig(2,:), triElemNeig(3,:) (1-based marker ids) and normDirFlag,
+    and the marker velocities; then iblank and fresh_cell (INTEGER(1),
+    reference layout), bodyNum (INTEGER, GCM only: pass body_num) and nFresh.
+    A body is a dict with the _BODY_HEAD keys, "markers" (n, 3),
+    "marker_vel" (n, 3) and, when it has elements, "tri" (ne, 3) and
+    "norm_dir_flag"."""
+    with open(path, "wb") as f:
+        write_unformatted_record(f, np.array([ntime], "<i4"), np.array([time, dt, re_], "<f8"))
+        write_unformatted_record(f, np.array([len(bodies)], "<i4"))
+        for b in bodies:
+            for rec in _BODY_HEAD:
+                items = []
+                for key, dt_, cnt in rec:
+                    v = np.atleast_1d(np.asarray(b[key], dtype=dt_))
+                    if v.size != cnt:
+                        raise ValueError(f"body {key}: {v.size} values, expected {cnt}")
+                    items.append(v)
+                write_unformatted_record(f, *items)
+            m = np.asarray(b["markers"], dtype="<f8").reshape(-1, 3)
+            if len(m) != int(b["npts"]):
+                raise ValueError("markers do not match npts")
+            write_unformatted_record(f, m[:, 0], m[:, 1], m[:, 2])
+            if _has_elements(int(b["canonical_body_type"])):
+                t = np.asarray(b["tri"], dtype="<i4").reshape(-1, 3)
+                write_unformatted_record(f, np.array([len(t)], "<i4"))
+                write_unformatted_record(f, t[:, 0], t[:, 1], t[:, 2])
+                write_unformatted_record(f, np.array([b["norm_dir_flag"]], "<f8"))
+            mv = np.asarray(b["marker_vel"], dtype="<f8").reshape(-1, 3)
+            write_unformatted_record(f, mv[:, 0], mv[:, 1], mv[:, 2])
+        ib = np.asarray(iblank, dtype=np.int8)
+        fr = np.asarray(fresh, dtype=np.int8)
+        if ib.shape != fr.shape:
+            raise ValueError("iblank and fresh_cell shapes differ")
+        write_unformatted_record(f, ib, fr)
+        if body_num is not None:
+            write_unformatted_record(f, np.asarray(body_num, dtype="<i4"))
+        nf = int((fr == 1).sum()) if n_fresh is None else int(n_fresh)
+        write_unformatted_record(f, np.array([nf], "<i4"))
+
+
+def read_restart_body(path, nx, ny, nz, gcm=True):
+    """read_restart_body (MAIN.F90:1758-2031) for node counts nx, ny, nz: a
+    dict with ntime, time, dt, re, bodies (as write_restart_body takes them),
+    iblank, fresh, body_num (gcm only) and n_fresh."""
+    shp = (nz + 2, ny + 2, nx + 2)
+    n3 = int(np.prod(shp))
+    with open(path, "rb") as f:
+        r = _rec(f)
+        ntime = int(r.take("<i4")[0])
+        time_, dt, re_ = (float(v) for v in r.take("<f8", 3))
+        r.done()
+        r = _rec(f)
+        nbody = int(r.take("<i4")[0])
+        r.done()
+        bodies = []
+        for _ in range(nbody):
+            b = {}
+            for rec in _BODY_HEAD:
+                r = _rec(f)
+                for key, dt_, cnt in rec:
+                    v = r.take(dt_, cnt)
+                    b[key] = v if cnt > 1 else v[0].item()
+                r.done()
+            n = int(b["npts"])
+            r = _rec(f)
+            b["markers"] = r.take("<f8", 3 * n, (3, n)).T.copy()
+            r.done()
+            if _has_elements(int(b["canonical_body_type"])):
+                r = _rec(f)
+                ne = int(r.take("<i4")[0])
+                r.done()
+                r = _rec(f)
+                b["tri"] = r.take("<i4", 3 * ne, (3, ne)).T.copy()
+                r.done()
+                r = _rec(f)
+                b["norm_dir_flag"] = float(r.take("<f8")[0])
+                r.done()
+            r = _rec(f)
+            b["marker_vel"] = r.take("<f8", 3 * n, (3, n)).T.copy()
+            r.done()
+            bodies.append(b)
+        r = _rec(f)
+        ib = r.take("i1", n3, shp)
+        fr = r.take("i1", n3, shp)
+        r.done()
+        body_num = None
+        if gcm:
+            r = _rec(f)
+            body_num = r.take("<i4", n3, shp)
+            r.done()
+        r = _rec(f)
+        nfr = int(r.take("<i4")[0])
+        r.done()
+    return {"ntime": ntime, "time": time_, "dt": dt, "re": re_, "bodies": bodies, "iblank": ib, "fresh": fr,
+            "body_num": body_num, "n_fresh": nfr}

@@ -1,10 +1,13 @@
 """The case-file readers and writers (case_io.py): input.dat in the record
 order of read_inputs (MAIN.F90:398-517) with the list-directed forms the
 reference's decks use, the grid files and the uniform-grid rule of make_grid
-(CORE_GRID.F90:36-80), and the ELLIPSOID bodies of canonical_body_in.dat
-(MAIN.F90:915-1166).  The reference ships no case files, so the files here
-are written record by record from those READ statements; a case read back
-builds the same geometry as the bodies given directly.  CPU only."""
+(CORE_GRID.F90:36-80), every body type of canonical_body_in.dat
+(MAIN.F90:915-1300), unstruc_surface_in.dat, and the unformatted restart
+pair of write_restart (MAIN.F90:2035-2191).  The reference ships no case or
+restart files, so the files here are written record by record from those
+READ and WRITE statements (the restart records are also decoded byte by
+byte); a case read back builds the same geometry as the bodies given
+directly.  CPU only."""
 import os
 
 import numpy as np
@@ -142,11 +145,61 @@ def test_case_directory_builds_the_same_geometry(tmp_path):
         assert np.array_equal(gs1.coeff, gs2.coeff) and np.array_equal(ex1["coeffD"], ex2["coeffD"])
 
 
-def test_other_body_types_are_named(tmp_path):
+def test_every_body_type_is_read(tmp_path):
+    """The records of each canonical_body_type (MAIN.F90:1118-1212) and the
+    trailing gate and Prsb_MomentRef records are read in order; a file the
+    writer made reads back to the same values."""
     p = tmp_path / "canonical_body_in.dat"
-    p.write_text("1 1 0 0 F F\n1 2 0\n0\n")
+    p.write_text("\n".join([
+        "3 3 0 1 T F",
+        "1 2 0", "0", "96 0", "0.3 0.2", "1.5 1.4", "12.5", "0 0 0", "0 0 0", "0 0 0", "0 0 0", "0", "0 0 0",
+        "0 0 0", "1.0 2.0", "0 0 0",
+        "3 3 1", "1", "24 0", "0.2 0.1 0.3", "0.5 0.5 0.5", "0", "0.01 0 0", "0 0 0", "0 0 0", "0 0 0", "0",
+        "0 0 0", "0 0 0", "1.0 1.0", "0 0 0",
+        "4 3 0", "0", "1 0 0 0 0 1", "100 196", "1 2 3", "0 0 0", "0.5 0.5 0.5", "30 0 0 1", "0 0 0", "0 0 0",
+        "0 0 0", "0 0 0", "0", "0 0 0", "0 0 0", "1.0 3.0", "0 0 0", "7", "0.05", "20.0", "0.1 0.2", "3.0 4.0",
+        "0.75",
+        "2", "1 1.0 0.0 0.0 0.5", "2 0.0 0.0 0.0 0.0",
+        "T"]) + "\n")
+    head, bodies = case_io.read_canonical_bodies(str(p))
+    assert [b["canonical_body_type"] for b in bodies] == [1, 3, 4]
+    assert bodies[0]["nPtsBodyMarker"] == 96 and bodies[0]["alpha"] == 12.5 and bodies[0]["density_solid"] == 2.0
+    assert bodies[1]["n_theta"] == 24 and bodies[1]["vxcentTrans"] == 0.01
+    assert bodies[2]["nPtsBodyMarker"] == 100 and bodies[2]["totNumTriElem"] == 196
+    assert bodies[2]["quat0"] == 30.0 and bodies[2]["hingemarker"] == 7 and bodies[2]["DepthOverLength"] == 0.75
+    assert len(head["gates"]) == 2 and head["gates"][0]["freqGate"] == 0.5 and head["Prsb_MomentRef"] is True
+    q = tmp_path / "written.dat"
+    case_io.write_canonical_bodies(str(q), bodies, channel_flow=True, n_section=1, gates=head["gates"],
+                                   prsb_moment_ref=True)
+    head2, bodies2 = case_io.read_canonical_bodies(str(q))
+    assert bodies2 == bodies and head2["gates"] == head["gates"]
+
+
+def test_case_geometry_mixes_ellipsoids_and_surfaces(tmp_path):
+    """A case with an ELLIPSOID and an UNSTRUCTURED_SURFACE body (the second
+    ellipsoid's own marker surface in unstruc_surface_in.dat) builds the
+    triangulated geometry of the two ellipsoids bit for bit; a 2-D cylinder
+    body is named."""
+    nodes = (pp.tanh_nodes(40, 1.0, 1.1), pp.tanh_nodes(32, 1.0, 0.9), pp.tanh_nodes(36, 1.0, 1.2))
+    e1 = pp.Ellipsoid((0.3, 0.5, 0.5), (0.12, 0.1, 0.15), 15.0, 24)
+    e2 = pp.Ellipsoid((0.7, 0.45, 0.55), (0.14, 0.12, 0.1), 0.0, 24)
+    xyz, tri = pp.canonical_surface(e2)
+    case_io.write_unstruc_surface(str(tmp_path / "unstruc_surface_in.dat"), xyz, tri, (0.0, 0.0, 0.0))
+    bodies = [dict(n_theta=24, radiusx=0.12, radiusy=0.1, radiusz=0.15, xcent=0.3, ycent=0.5, zcent=0.5, alpha=15.0),
+              dict(canonical_body_type=case_io.UNSTRUCTURED_SURFACE, nPtsBodyMarker=len(xyz),
+                   totNumTriElem=len(tri), xcent=0.7, ycent=0.45, zcent=0.55)]
+    case_io.write_canonical_bodies(str(tmp_path / "canonical_body_in.dat"), bodies)
+    ib, fr, gs, ex = case_io.case_geometry(str(tmp_path), vals={"boundary_formulation": 2,
+                                                                "probeLengthNormalized": 2.0}, nodes=nodes)
+    ib2, fr2, gs2, ex2 = pp.build_geometry(*nodes, [e1, e2], mode=pp.PP_GEOM_TRIANGULATED)
+    assert np.array_equal(ib, ib2) and np.array_equal(gs.ig, gs2.ig) and np.array_equal(gs.ic, gs2.ic)
+    assert np.array_equal(gs.coeff, gs2.coeff) and np.array_equal(ex["elem"], ex2["elem"])
+    assert np.array_equal(ex["body"], ex2["body"])
+    case_io.write_canonical_bodies(str(tmp_path / "canonical_body_in.dat"),
+                                   [dict(canonical_body_type=1, nPtsBodyMarker=64, radiusx=0.1, radiusy=0.1,
+                                         xcent=0.5, ycent=0.5, body_dim=2)])
     with pytest.raises(NotImplementedError, match="canonical_body_type 1"):
-        case_io.read_canonical_bodies(str(p))
+        case_io.case_geometry(str(tmp_path), vals={"boundary_formulation": 2}, nodes=nodes)
 
 
 def test_unstructured_surface_builds_the_canonical_geometry(tmp_path):
@@ -206,3 +259,109 @@ def test_unstructured_surface_outside_point_inside_the_bounding_box():
         pp.build_geometry(np.linspace(-1, 1, 17), np.linspace(-1, 1, 17), np.linspace(-1, 1, 17),
                           [pp.Ellipsoid((0.05, 0.0, 0.0), (0.3, 0.3, 0.3), 0.0, 24)],
                           mode=pp.PP_GEOM_TRIANGULATED)
+
+
+# ---------------------------------------------------------------------------
+# unformatted restart files (write_restart, MAIN.F90:2035-2191)
+# ---------------------------------------------------------------------------
+def _markers(raw):
+    """(leading, trailing, payload length) of every subrecord, decoded
+    independently of case_io."""
+    import struct
+    out, pos = [], 0
+    while pos < len(raw):
+        lead = struct.unpack_from("<i", raw, pos)[0]
+        m = abs(lead)
+        trail = struct.unpack_from("<i", raw, pos + 4 + m)[0]
+        out.append((lead, trail, m))
+        pos += 8 + m
+    assert pos == len(raw)
+    return out
+
+
+def _flow_fields(nx, ny, nz, seed):
+    rng = np.random.default_rng(seed)
+    return {k: rng.standard_normal(case_io.flow_field_shape(k, nx, ny, nz)) for k in case_io.FLOW_FIELDS}
+
+
+def test_restart_flow_round_trip_and_record_layout(tmp_path):
+    nx, ny, nz = 9, 7, 5              # node counts: 8 x 6 x 4 cells
+    f = _flow_fields(nx, ny, nz, 1)
+    p = str(tmp_path / "restart_flow_out1.dat")
+    case_io.write_restart_flow(p, nx, ny, nz, 1234, 5.25, 1e-3, 150.0, f)
+    r = case_io.read_restart_flow(p)
+    assert (r["nx"], r["ny"], r["nz"], r["ntime"], r["time"], r["dt"], r["re"]) == (9, 7, 5, 1234, 5.25, 1e-3, 150.0)
+    for k in case_io.FLOW_FIELDS:
+        assert np.array_equal(r["fields"][k], f[k]), k
+    raw = open(p, "rb").read()
+    recs = _markers(raw)
+    n3 = (nx + 2) * (ny + 2) * (nz + 2)
+    n2 = 2 * ((ny + 2) * (nz + 2) + (nx + 2) * (nz + 2) + (nx + 2) * (ny + 2))
+    assert [m for _, _, m in recs] == [12, 28, 8 * (23 * n3 + n2)]
+    assert all(a == b > 0 for a, b, _ in recs)
+    # the first field starts right after the third record's marker, i fastest
+    off = 4 + 12 + 4 + 4 + 28 + 4 + 4
+    assert np.frombuffer(raw, "<f8", count=3, offset=off).tolist() == f["u"][0, 0, :3].tolist()
+
+
+def test_restart_long_records_split_into_subrecords(tmp_path, monkeypatch):
+    monkeypatch.setattr(case_io, "SUBRECORD_MAX", 1000)
+    nx, ny, nz = 6, 5, 4
+    f = _flow_fields(nx, ny, nz, 2)
+    p = str(tmp_path / "restart_flow_out2.dat")
+    case_io.write_restart_flow(p, nx, ny, nz, 7, 0.5, 0.01, 100.0, f)
+    recs = _markers(open(p, "rb").read())
+    big = recs[2:]
+    assert len(big) > 2 and all(m <= 1000 for _, _, m in big)
+    # leading marker negative while another subrecord follows, trailing
+    # marker negative when the subrecord continues a previous one
+    assert all(lead < 0 for lead, _, _ in big[:-1]) and big[-1][0] > 0
+    assert big[0][1] > 0 and all(tr < 0 for _, tr, _ in big[1:])
+    r = case_io.read_restart_flow(p)
+    for k in case_io.FLOW_FIELDS:
+        assert np.array_equal(r["fields"][k], f[k])
+
+
+def test_restart_body_round_trip_with_a_moving_geometry(tmp_path):
+    nodes = (pp.tanh_nodes(20, 1.0, 1.1), pp.tanh_nodes(16, 1.0, 0.9), pp.tanh_nodes(18, 1.0, 1.2))
+    b0 = pp.Ellipsoid((0.45, 0.5, 0.5), (0.22, 0.2, 0.2), 10.0, 24)
+    b1 = pp.Ellipsoid((0.55, 0.5, 0.5), (0.22, 0.2, 0.2), 10.0, 24)
+    ib0, _, _, _ = pp.build_geometry(*nodes, [b0], mode=pp.PP_GEOM_TRIANGULATED)
+    ib, fr, gs, ex = pp.build_geometry(*nodes, [b1], mode=pp.PP_GEOM_TRIANGULATED, iblank_prev=ib0)
+    assert fr.sum() > 0
+    xyz, tri = pp.canonical_surface(b1)
+    rng = np.random.default_rng(3)
+    common = dict(body_dim=3, boundary_motion_type=2, wall_type=0, radius=b1.semi, cent=b1.center, alpha=10.0,
+                  vcent_trans=(0.01, 0.0, 0.0), amp=(0, 0, 0), freq=(0, 0, 0), angv=(0, 0, 0), phase=0.0,
+                  ampang=(0, 0, 0), freqang=(0, 0, 0))
+    bodies = [dict(common, npts_orig=len(xyz), npts=len(xyz), canonical_body_type=3, markers=xyz,
+                   marker_vel=rng.standard_normal(xyz.shape)),
+              dict(common, npts_orig=len(xyz), npts=len(xyz), canonical_body_type=case_io.UNSTRUCTURED_SURFACE,
+                   markers=xyz + 1.0, tri=tri, norm_dir_flag=-1.0, marker_vel=rng.standard_normal(xyz.shape))]
+    body_num = (ib * 1).astype(np.int32)
+    p = str(tmp_path / "restart_body_out1.dat")
+    case_io.write_restart_body(p, 50, 0.5, 0.01, 100.0, bodies, ib, fr, body_num=body_num)
+    nx, ny, nz = (len(a) for a in nodes)      # node counts
+    r = case_io.read_restart_body(p, nx, ny, nz, gcm=True)
+    assert (r["ntime"], r["time"], r["dt"], r["re"]) == (50, 0.5, 0.01, 100.0)
+    assert np.array_equal(r["iblank"], ib) and np.array_equal(r["fresh"], fr) and np.array_equal(r["body_num"], body_num)
+    assert r["n_fresh"] == int(fr.sum())
+    for got, want in zip(r["bodies"], bodies):
+        assert np.array_equal(got["markers"], want["markers"]) and np.array_equal(got["marker_vel"], want["marker_vel"])
+        assert np.allclose(got["radius"], want["radius"]) and got["canonical_body_type"] == want["canonical_body_type"]
+    assert "tri" not in r["bodies"][0] and np.array_equal(r["bodies"][1]["tri"], tri)
+    assert r["bodies"][1]["norm_dir_flag"] == -1.0
+    # the geometry read back drives the same ghost marking
+    from oracle_lib import Oracle
+    o = Oracle(*nodes)
+    assert o.set_iblank(r["iblank"]) == gs.n
+
+
+def test_restart_truncated_file_is_refused(tmp_path):
+    nx, ny, nz = 5, 5, 5
+    p = str(tmp_path / "restart_flow_out1.dat")
+    case_io.write_restart_flow(p, nx, ny, nz, 1, 0.0, 0.1, 1.0, {})
+    raw = open(p, "rb").read()
+    open(p, "wb").write(raw[:-10])
+    with pytest.raises((ValueError, EOFError)):
+        case_io.read_restart_flow(p)
