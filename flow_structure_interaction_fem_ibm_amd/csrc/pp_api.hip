@@ -1283,6 +1283,11 @@ int compute_null_vector(pp_ctx *c) {
   return PP_OK;
 }
 
+void free_fresh_rows(pp_ctx *c) {
+  hipFree(c->fr.ids); hipFree(c->fr.face); hipFree(c->fr.coff); hipFree(c->fr.cc);
+  c->fr = FreshRows{};
+}
+
 }  // namespace
 
 // ======================================================================
@@ -1414,6 +1419,7 @@ int32_t pp_destroy(pp_ctx *c) {
   hipFree(c->d_p); hipFree(c->d_b); hipFree(c->d_w); hipFree(c->d_bwork); hipFree(c->d_tmp);
   hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_mu); hipFree(c->d_sv); hipFree(c->d_mu_prev);
   hipFree(c->d_ad_coef);
+  free_fresh_rows(c);
   hipFree(c->d_part); hipFree(c->d_part2); hipFree(c->d_res);
   for (double *q : c->d_kry) hipFree(q);
   for (float *q : c->d_kryl) hipFree(q);
@@ -1438,6 +1444,7 @@ int32_t pp_set_geometry(pp_ctx *c, const pp_geometry *geo) {
   HIPCHK(hipSetDevice(c->device));
   c->have_geom = false;
   c->have_ad = false;  // set_solve_ad follows every set_boundary
+  free_fresh_rows(c);
   if (c->gexec) {  // captured launches point at the old hierarchy
     HIPCHK(hipGraphExecDestroy(c->gexec));
     c->gexec = nullptr;
@@ -2251,26 +2258,31 @@ AdLineLaunch ad_line_launch(const pp_ctx *c, int dir) {
 }
 
 // Fresh cells (fluid now, blank at the previous geometry) take the
-// GCM_set_freshcell rows (coeffGCMFreshD, CORE_SET_SOLVERS.F90:175-200) and
-// the off-stencil corrections GCM_correct_rhs_ad / GCM_correct_res_ad
-// (GCM_FRESHCELL_AD.F90:2-137) in the reference; those rows are not built
-// here, so a geometry with fresh cells is refused rather than solved with the
-// ordinary rows.
-int ad_no_fresh(const pp_ctx *c, const char *who) {
-  if (c->h_fresh.empty()) return PP_OK;
-  set_error("%s: the geometry has fresh cells (pp_geometry.fresh_cell); their momentum rows "
-            "(GCM_set_freshcell, GCM_correct_rhs_ad/res_ad) are not supported: pass fresh_cell = NULL or a "
-            "geometry without fresh cells", who);
-  return PP_BAD_ARG;
+// GCM_set_freshcell rows (CORE_SET_SOLVERS.F90:175-218): a geometry with a
+// fresh_cell mask needs the tables of pp_set_fresh_cells first.
+int ad_fresh_ready(const pp_ctx *c, const char *who) {
+  if (c->h_fresh.empty() || c->fr.n > 0) return PP_OK;
+  set_error("%s: the geometry has fresh cells (pp_geometry.fresh_cell): call pp_set_fresh_cells with their "
+            "tables (pp_geom_fresh) after pp_set_geometry", who);
+  return PP_NO_GEOMETRY;
+}
+
+FreshRes fresh_res(const pp_ctx *c) {
+  FreshRes r{};
+  if (c->fr.n > 0) {
+    r.mask = c->d_fresh;
+    r.ids = c->fr.ids;
+    r.coff = c->fr.coff;
+    r.cc = c->fr.cc;
+    r.n = c->fr.n;
+  }
+  return r;
 }
 
 int ad_half_dt(int32_t scheme, double dt, double *half_dt, const char *who) {
   if (scheme == PP_ADAMS_BASHFORTH2) { *half_dt = 0.0; return PP_OK; }
-  if (scheme == PP_CRANK_NICOLSON1) { *half_dt = 0.5 * dt; return PP_OK; }
-  if (scheme == PP_CRANK_NICOLSON2)
-    set_error("%s: CRANK_NICOLSON2 recomputes the face velocities inside the loop (face_vel); not supported", who);
-  else
-    set_error("%s: unknown advec_scheme %d", who, (int)scheme);
+  if (scheme == PP_CRANK_NICOLSON1 || scheme == PP_CRANK_NICOLSON2) { *half_dt = 0.5 * dt; return PP_OK; }
+  set_error("%s: unknown advec_scheme %d", who, (int)scheme);
   return PP_BAD_ARG;
 }
 
@@ -2278,16 +2290,21 @@ int ad_half_dt(int32_t scheme, double dt, double *half_dt, const char *who) {
 void ad_build_rows(pp_ctx *c, double half_dt, const double *fu, const double *fv, const double *fw, double *row) {
   CellLaunch cl = ref_launch(c);
   k_ad_rows<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), ref_metrics(c), half_dt,
-                                                      c->d_ad_coef, fu, fv, fw, row, c->nref);
+                                                      c->d_ad_coef, fu, fv, fw, c->fr.n > 0 ? c->d_fresh : nullptr,
+                                                      row, c->nref);
 }
 
 // itsolv_ad of u, v and w: x, y and z line sweeps, two zebra passes each
-// (the three fields in each pass)
-void ad_sweeps(pp_ctx *c, const double *row, AdLineFields F) {
+// (the three fields in each pass); with fresh cells F.r are the call's
+// copies of the right-hand sides that k_fresh_rhs refreshes before each pass
+void ad_sweeps(pp_ctx *c, const double *row, AdLineFields F, const FreshCorr *fc) {
   LevelArgs<double> L0 = largs<double>(c->lv[0]);
   for (int dir = 0; dir < 3; dir++) {
     AdLineLaunch L = ad_line_launch(c, dir);
     for (int color = 0; color < 2; color++) {
+      if (fc)
+        k_fresh_rhs<<<(3 * c->fr.n + 255) / 256, 256, 0, c->stream>>>(c->fr.n, 3, c->fr.ids, c->fr.coff, c->fr.cc,
+                                                                      *fc);
       if (dir == 0)
         k_ad_line<0, kAdLSx><<<L.grid, L.block, L.lds, c->stream>>>(c->ref, L0, row, c->nref, F, color, L.nlines);
       else if (dir == 1)
@@ -2306,10 +2323,11 @@ int ad_residual(pp_ctx *c, const double *row, int nv, AdFields F, double *rout) 
   const long nblk = c->nblk_red;
   LevelArgs<double> L0 = largs<double>(c->lv[0]);
   hipStream_t s = c->stream;
+  const FreshRes FR = fresh_res(c);
   if (nv == 3)
-    k_ad_residual<256, 3><<<cl.grid, cl.block, 0, s>>>(c->ref, L0, row, c->nref, F, c->d_part, nblk, rout);
+    k_ad_residual<256, 3><<<cl.grid, cl.block, 0, s>>>(c->ref, L0, row, c->nref, F, FR, c->d_part, nblk, rout);
   else
-    k_ad_residual<256, 1><<<cl.grid, cl.block, 0, s>>>(c->ref, L0, row, c->nref, F, c->d_part, nblk, rout);
+    k_ad_residual<256, 1><<<cl.grid, cl.block, 0, s>>>(c->ref, L0, row, c->nref, F, FR, c->d_part, nblk, rout);
   const int chunk = (int)((nblk + kRedStage - 1) / kRedStage);
   for (int q = 0; q < nv; q++) {
     k_reduce_summax_stage<<<kRedStage, 256, 0, s>>>(c->d_part + 3 * nblk * q, (int)nblk, chunk,
@@ -2336,12 +2354,97 @@ void ad_loc(const pp_ctx *c, double mx, double arg, int32_t loc[3]) {
 
 extern "C" {
 
+int32_t pp_set_fresh_cells(pp_ctx *c, int32_t n, const int32_t *ifr, const int32_t *jfr, const int32_t *kfr,
+                           const int32_t *ic, const int32_t *jc, const int32_t *kc, const double *coeffD) {
+  if (!c || n < 0 || (n > 0 && (!ifr || !jfr || !kfr || !ic || !jc || !kc || !coeffD))) {
+    set_error("pp_set_fresh_cells: null argument");
+    return PP_BAD_ARG;
+  }
+  if (!c->have_geom) { set_error("pp_set_fresh_cells: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  c->have_ad = false;  // the coefficients take the new rows in pp_set_solve_ad
+  free_fresh_rows(c);
+  long nmask = 0;
+  for (long r = 0; r < (long)c->h_fresh.size(); r++) nmask += c->h_fresh[r] == 1;
+  if (nmask != n) {
+    set_error("pp_set_fresh_cells: the geometry has %ld fresh cells, %d given", nmask, n);
+    return PP_BAD_ARG;
+  }
+  if (n == 0) return PP_OK;
+  // stencil order of coeffGCMFreshD (incI, incJ, incK of the ghost stencils)
+  static const int inc[3][8] = {{0, 0, 1, 1, 0, 0, 1, 1}, {0, 1, 1, 0, 0, 1, 1, 0}, {0, 0, 0, 0, 1, 1, 1, 1}};
+  const int nc[3] = {c->nxc, c->nyc, c->nzc};
+  std::vector<int> order(n);
+  std::vector<long> off(n);
+  for (int q = 0; q < n; q++) {
+    const int cell[3] = {ifr[q], jfr[q], kfr[q]}, lo[3] = {ic[q], jc[q], kc[q]};
+    for (int d = 0; d < 3; d++)
+      if (cell[d] < 1 || cell[d] > nc[d] || lo[d] < cell[d] - 1 || lo[d] > cell[d] || lo[d] < 1) {
+        set_error("pp_set_fresh_cells: fresh cell %d (%d,%d,%d) with stencil corner (%d,%d,%d): the corner must "
+                  "be the cell or the one below it", q, cell[0], cell[1], cell[2], lo[0], lo[1],
+                  lo[2]);
+        return PP_BAD_ARG;
+      }
+    off[q] = (long)cell[0] + c->ref.ex * cell[1] + c->ref.exy * cell[2];
+    if (c->h_fresh[off[q]] != 1) {
+      set_error("pp_set_fresh_cells: cell (%d,%d,%d) is not a fresh cell of the geometry", cell[0], cell[1], cell[2]);
+      return PP_BAD_ARG;
+    }
+    for (int m = 0; m < 8; m++)
+      if (!std::isfinite(coeffD[8 * q + m])) {
+        set_error("pp_set_fresh_cells: coefficient %d of fresh cell %d is not finite", m, q);
+        return PP_BAD_ARG;
+      }
+    order[q] = q;
+  }
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return off[a] < off[b]; });
+  for (int q = 1; q < n; q++)
+    if (off[order[q]] == off[order[q - 1]]) {
+      set_error("pp_set_fresh_cells: cell (%d,%d,%d) given twice", ifr[order[q]], jfr[order[q]], kfr[order[q]]);
+      return PP_BAD_ARG;
+    }
+  std::vector<long> h_ids(n), h_coff(4 * (size_t)n);
+  std::vector<double> h_face(6 * (size_t)n, 0.0), h_cc(4 * (size_t)n);
+  for (int t = 0; t < n; t++) {
+    const int q = order[t];
+    const int cell[3] = {ifr[q], jfr[q], kfr[q]}, lo[3] = {ic[q], jc[q], kc[q]};
+    h_ids[t] = off[q];
+    int ncorner = 0;
+    for (int m = 0; m < 8; m++) {
+      int node[3], ndiff = 0, dd = -1;
+      for (int d = 0; d < 3; d++) {
+        node[d] = lo[d] + inc[d][m];
+        if (node[d] != cell[d]) { ndiff++; dd = d; }
+      }
+      const double cD = coeffD[8 * q + m];
+      if (ndiff == 1)  // face neighbour: amx_ad/apx_ad/... = -coeffGCMFreshD
+        h_face[6 * t + 2 * dd + (node[dd] > cell[dd] ? 1 : 0)] = -cD;
+      else if (ndiff >= 2) {
+        h_coff[4 * t + ncorner] = (long)node[0] + c->ref.ex * node[1] + c->ref.exy * node[2];
+        h_cc[4 * t + ncorner] = cD;
+        ncorner++;
+      }
+    }
+  }
+  FreshRows &F = c->fr;
+  HIPCHK(hipMalloc(&F.ids, sizeof(long) * n));
+  HIPCHK(hipMalloc(&F.face, sizeof(double) * 6 * n));
+  HIPCHK(hipMalloc(&F.coff, sizeof(long) * 4 * n));
+  HIPCHK(hipMalloc(&F.cc, sizeof(double) * 4 * n));
+  HIPCHK(hipMemcpy(F.ids, h_ids.data(), sizeof(long) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(F.face, h_face.data(), sizeof(double) * 6 * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(F.coff, h_coff.data(), sizeof(long) * 4 * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(F.cc, h_cc.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+  F.n = n;
+  return PP_OK;
+}
+
 int32_t pp_set_solve_ad_device(pp_ctx *c, double dt, const double *d_visc, const double *d_bxv, const double *d_byv,
                                const double *d_bzv) {
   if (!c || !d_visc || !d_bxv || !d_byv || !d_bzv) { set_error("pp_set_solve_ad: null argument"); return PP_BAD_ARG; }
   if (!c->have_geom) { set_error("pp_set_solve_ad: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
   if (!std::isfinite(dt)) { set_error("pp_set_solve_ad: dt must be finite"); return PP_BAD_ARG; }
-  if (int rf = ad_no_fresh(c, "pp_set_solve_ad")) return rf;
+  if (int rf = ad_fresh_ready(c, "pp_set_solve_ad")) return rf;
   HIPCHK(hipSetDevice(c->device));
   c->have_ad = false;
   const size_t nb = sizeof(double) * 6 * (size_t)c->nref;
@@ -2350,6 +2453,9 @@ int32_t pp_set_solve_ad_device(pp_ctx *c, double dt, const double *d_visc, const
   CellLaunch cl = ref_launch(c);
   k_set_solve_ad<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), ref_metrics(c), dt,
                                                            d_visc, d_bxv, d_byv, d_bzv, c->d_ad_coef, c->nref);
+  if (c->fr.n > 0)
+    k_fresh_ad_coef<<<(c->fr.n + 255) / 256, 256, 0, c->stream>>>(c->fr.n, c->fr.ids, c->fr.face, c->d_ad_coef,
+                                                                 c->nref);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   c->ad_dt = dt;
@@ -2361,7 +2467,7 @@ int32_t pp_set_solve_ad(pp_ctx *c, double dt, const double *visc, const double *
                         const double *bzv) {
   if (!c || !visc || !bxv || !byv || !bzv) { set_error("pp_set_solve_ad: null argument"); return PP_BAD_ARG; }
   if (!c->have_geom) { set_error("pp_set_solve_ad: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
-  if (int rf = ad_no_fresh(c, "pp_set_solve_ad")) return rf;
+  if (int rf = ad_fresh_ready(c, "pp_set_solve_ad")) return rf;
   HIPCHK(hipSetDevice(c->device));
   const size_t nb = sizeof(double) * c->nref;
   DevTemp d;
@@ -2371,10 +2477,31 @@ int32_t pp_set_solve_ad(pp_ctx *c, double dt, const double *visc, const double *
   return pp_set_solve_ad_device(c, dt, d.d(), d.d() + c->nref, d.d() + 2 * c->nref, d.d() + 3 * c->nref);
 }
 
-int32_t pp_solve_ad_device(pp_ctx *c, int32_t advec_scheme, double *d_u, double *d_v, double *d_w,
-                           const double *d_nlu, const double *d_nlv, const double *d_nlw, const double *d_fu,
-                           const double *d_fv, const double *d_fw, const double *d_vbi, int32_t itermax,
-                           double restol, pp_ad_stats *st) {
+}  // extern "C"
+
+namespace {
+// the arguments of face_vel for a CRANK_NICOLSON2 solve (device pointers)
+struct FaceVelArgs {
+  int van_kan;
+  double dt;
+  const double *p, *bcxu, *bcyv, *bczw;
+};
+
+void face_vel_launch(pp_ctx *c, const FaceVelArgs &a, const double *u, const double *v, const double *w, double *fu,
+                     double *fv, double *fw) {
+  FracMetrics M;
+  for (int d = 0; d < 3; d++) { M.fx[d] = c->d_rfx[d]; M.dinv[d] = c->d_rdxinv[d]; M.dcinv[d] = c->d_rdxcinv[d]; }
+  CellLaunch cl = ref_launch(c);
+  k_face_vel<256><<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), M, a.van_kan, a.dt, a.p, u,
+                                                       v, w, a.bcxu, a.bcyv, a.bczw, fu, fv, fw);
+}
+
+// The GCM loop of solve_ad (CORE_SOLVE_AD.F90:218-247) on device fields;
+// with cn2 the face velocities are recomputed by face_vel after each ghost
+// update (CRANK_NICOLSON2) and written to d_fu, d_fv, d_fw.
+int32_t solve_ad_run(pp_ctx *c, int32_t advec_scheme, double *d_u, double *d_v, double *d_w, const double *d_nlu,
+                     const double *d_nlv, const double *d_nlw, double *d_fu, double *d_fv, double *d_fw,
+                     const FaceVelArgs *cn2, const double *d_vbi, int32_t itermax, double restol, pp_ad_stats *st) {
   if (!c || !d_u || !d_v || !d_w || !d_nlu || !d_nlv || !d_nlw || !d_fu || !d_fv || !d_fw) {
     set_error("pp_solve_ad: null argument");
     return PP_BAD_ARG;
@@ -2389,7 +2516,12 @@ int32_t pp_solve_ad_device(pp_ctx *c, int32_t advec_scheme, double *d_u, double 
     return PP_NO_GEOMETRY;
   }
   if (itermax < 0 || !std::isfinite(restol)) { set_error("pp_solve_ad: bad itermax_ad or restol_ad"); return PP_BAD_ARG; }
-  if (int rf = ad_no_fresh(c, "pp_solve_ad")) return rf;
+  if (int rf = ad_fresh_ready(c, "pp_solve_ad")) return rf;
+  if (!cn2 && advec_scheme == PP_CRANK_NICOLSON2) {
+    set_error("pp_solve_ad: CRANK_NICOLSON2 recomputes the face velocities after every ghost update (face_vel): "
+              "call pp_solve_ad_cn2 with the pressure and the outer-face velocities");
+    return PP_BAD_ARG;
+  }
   double half_dt = 0.0;
   int rc = ad_half_dt(advec_scheme, c->ad_dt, &half_dt, "pp_solve_ad");
   if (rc) return rc;
@@ -2411,14 +2543,33 @@ int32_t pp_solve_ad_device(pp_ctx *c, int32_t advec_scheme, double *d_u, double 
   const double *rhs[3] = {d_nlu, d_nlv, d_nlw};
   AdFields F;
   for (int q = 0; q < 3; q++) { F.var[q] = vars[q]; F.r[q] = rhs[q]; }
+  // fresh cells: the line passes read copies of nlu, nlv, nlw whose fresh
+  // entries carry the off-stencil terms (k_fresh_rhs)
+  DevTemp rw;
+  FreshCorr fc{};
+  const bool fresh = c->fr.n > 0;
+  if (fresh) {
+    const size_t nb = sizeof(double) * (size_t)c->nref;
+    HIPCHK(rw.alloc(3 * nb));
+    for (int q = 0; q < 3; q++) {
+      fc.var[q] = vars[q];
+      fc.r[q] = rhs[q];
+      fc.rw[q] = rw.d() + q * c->nref;
+      HIPCHK(hipMemcpyAsync(fc.rw[q], rhs[q], nb, hipMemcpyDeviceToDevice, s));
+    }
+  }
   int it = 0;
   double maxres = 1.0e10, ruvw[3] = {0.0, 0.0, 0.0};
   int32_t loc[3] = {0, 0, 0};
   while (it < itermax && maxres > restol) {
     AdLineFields LF;
-    for (int q = 0; q < 3; q++) { LF.var[q] = vars[q]; LF.r[q] = rhs[q]; }
-    ad_sweeps(c, rows.d(), LF);
+    for (int q = 0; q < 3; q++) { LF.var[q] = vars[q]; LF.r[q] = fresh ? fc.rw[q] : rhs[q]; }
+    ad_sweeps(c, rows.d(), LF, fresh ? &fc : nullptr);
     vel_ghosts(c, d_u, d_v, d_w, d_vbi);
+    if (cn2) {  // face_vel, then the rows of the new face velocities
+      face_vel_launch(c, *cn2, d_u, d_v, d_w, d_fu, d_fv, d_fw);
+      ad_build_rows(c, half_dt, d_fu, d_fv, d_fw, rows.d());
+    }
     rc = ad_residual(c, rows.d(), 3, F, nullptr);
     if (rc) break;
     int qm = 0;
@@ -2447,6 +2598,105 @@ int32_t pp_solve_ad_device(pp_ctx *c, int32_t advec_scheme, double *d_u, double 
   return status;
 }
 
+}  // namespace
+
+extern "C" {
+
+int32_t pp_solve_ad_device(pp_ctx *c, int32_t advec_scheme, double *d_u, double *d_v, double *d_w,
+                           const double *d_nlu, const double *d_nlv, const double *d_nlw, const double *d_fu,
+                           const double *d_fv, const double *d_fw, const double *d_vbi, int32_t itermax,
+                           double restol, pp_ad_stats *st) {
+  // the face velocities are read only (no CRANK_NICOLSON2 here)
+  return solve_ad_run(c, advec_scheme, d_u, d_v, d_w, d_nlu, d_nlv, d_nlw, const_cast<double *>(d_fu),
+                      const_cast<double *>(d_fv), const_cast<double *>(d_fw), nullptr, d_vbi, itermax, restol, st);
+}
+
+int32_t pp_face_vel_device(pp_ctx *c, int32_t van_kan, double dt, const double *d_p, const double *d_u,
+                           const double *d_v, const double *d_w, const double *d_bcxu, const double *d_bcyv,
+                           const double *d_bczw, double *d_fu, double *d_fv, double *d_fw) {
+  if (!c || (van_kan && !d_p) || !d_u || !d_v || !d_w || !d_bcxu || !d_bcyv || !d_bczw || !d_fu || !d_fv || !d_fw) {
+    set_error("pp_face_vel: null argument");
+    return PP_BAD_ARG;
+  }
+  if (!c->have_geom) { set_error("pp_face_vel: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  if (!std::isfinite(dt)) { set_error("pp_face_vel: dt must be finite"); return PP_BAD_ARG; }
+  HIPCHK(hipSetDevice(c->device));
+  FaceVelArgs a{van_kan ? 1 : 0, dt, d_p ? d_p : d_u, d_bcxu, d_bcyv, d_bczw};
+  face_vel_launch(c, a, d_u, d_v, d_w, d_fu, d_fv, d_fw);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PP_OK;
+}
+
+int32_t pp_face_vel(pp_ctx *c, int32_t van_kan, double dt, const double *p, const double *u, const double *v,
+                    const double *w, const double *bcxu, const double *bcyv, const double *bczw, double *fu,
+                    double *fv, double *fw) {
+  if (!c || (van_kan && !p) || !u || !v || !w || !bcxu || !bcyv || !bczw || !fu || !fv || !fw) {
+    set_error("pp_face_vel: null argument");
+    return PP_BAD_ARG;
+  }
+  if (!c->have_geom) { set_error("pp_face_vel: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nb = sizeof(double) * c->nref;
+  DevTemp d;
+  HIPCHK(d.alloc(10 * nb));
+  double *dv[10];
+  for (int q = 0; q < 10; q++) dv[q] = d.d() + q * c->nref;
+  const double *hin[10] = {p ? p : u, u, v, w, bcxu, bcyv, bczw, fu, fv, fw};
+  for (int q = 0; q < 10; q++) HIPCHK(hipMemcpy(dv[q], hin[q], nb, hipMemcpyHostToDevice));
+  int rc = pp_face_vel_device(c, van_kan, dt, dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7], dv[8], dv[9]);
+  if (rc == PP_OK) {
+    double *hout[3] = {fu, fv, fw};
+    for (int q = 0; q < 3; q++) HIPCHK(hipMemcpy(hout[q], dv[7 + q], nb, hipMemcpyDeviceToHost));
+  }
+  return rc;
+}
+
+int32_t pp_solve_ad_cn2_device(pp_ctx *c, double *d_u, double *d_v, double *d_w, const double *d_nlu,
+                               const double *d_nlv, const double *d_nlw, double *d_fu, double *d_fv, double *d_fw,
+                               int32_t van_kan, const double *d_p, const double *d_bcxu, const double *d_bcyv,
+                               const double *d_bczw, const double *d_vbi, int32_t itermax, double restol,
+                               pp_ad_stats *st) {
+  if (!c || (van_kan && !d_p) || !d_bcxu || !d_bcyv || !d_bczw) {
+    set_error("pp_solve_ad_cn2: null argument");
+    return PP_BAD_ARG;
+  }
+  FaceVelArgs a{van_kan ? 1 : 0, c->ad_dt, d_p ? d_p : d_bcxu, d_bcxu, d_bcyv, d_bczw};
+  return solve_ad_run(c, PP_CRANK_NICOLSON2, d_u, d_v, d_w, d_nlu, d_nlv, d_nlw, d_fu, d_fv, d_fw, &a, d_vbi,
+                      itermax, restol, st);
+}
+
+int32_t pp_solve_ad_cn2(pp_ctx *c, double *u, double *v, double *w, const double *nlu, const double *nlv,
+                        const double *nlw, double *fu, double *fv, double *fw, int32_t van_kan, const double *p,
+                        const double *bcxu, const double *bcyv, const double *bczw, const double *vbi,
+                        int32_t itermax, double restol, pp_ad_stats *st) {
+  if (!c || !u || !v || !w || !nlu || !nlv || !nlw || !fu || !fv || !fw || (van_kan && !p) || !bcxu || !bcyv ||
+      !bczw) {
+    set_error("pp_solve_ad_cn2: null argument");
+    return PP_BAD_ARG;
+  }
+  if (!c->have_geom) { set_error("pp_solve_ad: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nb = sizeof(double) * c->nref;
+  DevTemp d, dvbi;
+  HIPCHK(d.alloc(13 * nb));
+  double *dv[13];
+  for (int q = 0; q < 13; q++) dv[q] = d.d() + q * c->nref;
+  const double *hin[13] = {u, v, w, nlu, nlv, nlw, fu, fv, fw, p ? p : u, bcxu, bcyv, bczw};
+  for (int q = 0; q < 13; q++) HIPCHK(hipMemcpy(dv[q], hin[q], nb, hipMemcpyHostToDevice));
+  if (vbi && c->gh.n > 0) {
+    HIPCHK(dvbi.alloc(sizeof(double) * 3 * c->gh.n));
+    HIPCHK(hipMemcpy(dvbi.d(), vbi, sizeof(double) * 3 * c->gh.n, hipMemcpyHostToDevice));
+  }
+  int rc = pp_solve_ad_cn2_device(c, dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7], dv[8], van_kan, dv[9],
+                                  dv[10], dv[11], dv[12], dvbi.d(), itermax, restol, st);
+  if (rc == PP_OK || rc == PP_NOT_CONVERGED) {
+    double *hout[6] = {u, v, w, fu, fv, fw};
+    for (int q = 0; q < 6; q++) HIPCHK(hipMemcpy(hout[q], dv[q < 3 ? q : q + 3], nb, hipMemcpyDeviceToHost));
+  }
+  return rc;
+}
+
 int32_t pp_solve_ad(pp_ctx *c, int32_t advec_scheme, double *u, double *v, double *w, const double *nlu,
                     const double *nlv, const double *nlw, const double *fu, const double *fv, const double *fw,
                     const double *vbi, int32_t itermax, double restol, pp_ad_stats *st) {
@@ -2455,7 +2705,7 @@ int32_t pp_solve_ad(pp_ctx *c, int32_t advec_scheme, double *u, double *v, doubl
     return PP_BAD_ARG;
   }
   if (!c->have_geom) { set_error("pp_solve_ad: no geometry (call pp_set_geometry first)"); return PP_NO_GEOMETRY; }
-  if (int rf = ad_no_fresh(c, "pp_solve_ad")) return rf;
+  if (int rf = ad_fresh_ready(c, "pp_solve_ad")) return rf;
   HIPCHK(hipSetDevice(c->device));
   const size_t nb = sizeof(double) * c->nref;
   DevTemp d, dvbi;
@@ -2485,7 +2735,7 @@ int32_t pp_calc_residual_ad(pp_ctx *c, int32_t advec_scheme, const double *var, 
   }
   if (!c->have_geom) { set_error("pp_calc_residual_ad: no geometry"); return PP_NO_GEOMETRY; }
   if (!c->have_ad) { set_error("pp_calc_residual_ad: no momentum coefficients (call pp_set_solve_ad)"); return PP_NO_GEOMETRY; }
-  if (int rf = ad_no_fresh(c, "pp_calc_residual_ad")) return rf;
+  if (int rf = ad_fresh_ready(c, "pp_calc_residual_ad")) return rf;
   double half_dt = 0.0;
   int rc = ad_half_dt(advec_scheme, c->ad_dt, &half_dt, "pp_calc_residual_ad");
   if (rc) return rc;

@@ -313,6 +313,45 @@ int ghost_stencil(const Grid &G, int iG, int jG, int kG, const double *bi, const
   return 0;
 }
 
+// GCM_set_freshcell for one cell o.i, o.j, o.k whose body intercept is bi
+// (GCM_SET_FRESHCELL.F90:101-262): the image point, the stencil search over
+// the cell and its two neighbours in each direction (clamped to 1..n-1 of
+// the cell centres) with the reference's test that the stencil corner is the
+// cell or the one below it, and the Dirichlet Vandermonde weights at the
+// cell centre.  Returns 0, 3, FRESH_ERR_STENCIL or FRESH_ERR_AT_BI.
+int fresh_stencil(const Grid &G, const double *bi, double probe_norm, FreshOut &o) {
+  const int idx[3] = {o.i, o.j, o.k}, nn[3] = {G.nxc, G.nyc, G.nzc};
+  const double fc[3] = {G.XC[0][o.i], G.XC[1][o.j], G.XC[2][o.k]};
+  const double ds = std::sqrt((fc[1] - bi[1]) * (fc[1] - bi[1]) + (fc[0] - bi[0]) * (fc[0] - bi[0]) +
+                              (fc[2] - bi[2]) * (fc[2] - bi[2]));
+  if (!(ds > 0.0)) return FRESH_ERR_AT_BI;
+  const double dsinv = 1.0 / ds, probe = ds * probe_norm;
+  int cell[3];
+  for (int d = 0; d < 3; d++) {
+    const double slope = (fc[d] - bi[d]) * dsinv;
+    const double fip = bi[d] + slope * probe;
+    const int lo = std::max(idx[d] - 1, 1), hi = std::min(idx[d] + 1, nn[d]);
+    cell[d] = -1;
+    for (int i = lo; i <= hi; i++)
+      if (G.XC[d][i] <= fip && G.XC[d][i + 1] > fip) cell[d] = i;
+    if (cell[d] < idx[d] - 1 || cell[d] > idx[d]) return FRESH_ERR_STENCIL;
+  }
+  double MD[8][8];
+  for (int r = 0; r < 8; r++) {
+    const int i = cell[0] + incI[r], j = cell[1] + incJ[r], k = cell[2] + incK[r];
+    double x1 = G.XC[0][i], x2 = G.XC[1][j], x3 = G.XC[2][k];
+    if (i == o.i && j == o.j && k == o.k) { x1 = bi[0]; x2 = bi[1]; x3 = bi[2]; }
+    MD[r][0] = x1 * x2 * x3; MD[r][1] = x1 * x2; MD[r][2] = x1 * x3; MD[r][3] = x2 * x3;
+    MD[r][4] = x1; MD[r][5] = x2; MD[r][6] = x3; MD[r][7] = 1.0;
+  }
+  const double beta[8] = {fc[0] * fc[1] * fc[2], fc[0] * fc[1], fc[0] * fc[2], fc[1] * fc[2],
+                          fc[0], fc[1], fc[2], 1.0};
+  if (!solve8T(MD, beta, o.coeffD)) return 3;
+  o.ic = cell[0]; o.jc = cell[1]; o.kc = cell[2];
+  for (int d = 0; d < 3; d++) o.bi[d] = bi[d];
+  return 0;
+}
+
 }  // namespace ppg
 
 // ======================================================================
@@ -325,6 +364,9 @@ struct pp_geom {
   std::vector<int> list;       // ig, jg, kg triples (1-based)
   std::vector<GhostOut> gout;
   int nblank = 0, nfresh = 0;
+  // fresh cells in k, j, i order and the first failure of their stencils
+  std::vector<FreshOut> fout;
+  int fresh_err = 0, fresh_err_cell[3] = {0, 0, 0};
 };
 
 namespace {
@@ -448,6 +490,61 @@ int32_t build_geometry(pp_geom **out, const Grid &G, int32_t mode, const std::ve
         for (long r = k * G.exy; r < (k + 1) * G.exy; r++) { nb[k] += o->iblank[r]; nf[k] += o->fresh[r]; }
     }, kPlaneGrain);
     for (int k = 0; k < G.nzc + 3; k++) { o->nblank += (int)nb[k]; o->nfresh += (int)nf[k]; }
+  }
+  // fresh-cell interpolation (GCM_set_freshcell).  The reference takes the
+  // body intercept on bodyNum of the cell, the body that blanked it at the
+  // previous geometry; that is not known here, so the body is the nearest
+  // one (closest surface point, analytic mode; closest marker, triangulated
+  // mode; the lower index on ties), which is the same body whenever the one
+  // that left the cell is still the nearest.
+  if (o->nfresh > 0) {
+    for (int k = 1; k <= G.nzc; k++)
+      for (int j = 1; j <= G.nyc; j++)
+        for (int i = 1; i <= G.nxc; i++)
+          if (o->fresh[G.r(i, j, k)] == 1) {
+            FreshOut f;
+            f.i = i; f.j = j; f.k = k;
+            o->fout.push_back(f);
+          }
+    const int nfr = (int)o->fout.size();
+    std::vector<int> ferr(nfr, 0);
+    parallel_for(nfr, [&](long a, long b) {
+      for (long q = a; q < b; q++) {
+        FreshOut &f = o->fout[q];
+        const double p[3] = {G.XC[0][f.i], G.XC[1][f.j], G.XC[2][f.k]};
+        double bi[3] = {0.0, 0.0, 0.0}, nrm[3];
+        double best = 1e300;
+        if (mode == PP_GEOM_ANALYTIC) {
+          for (int bb = 0; bb < nbody; bb++) {
+            double tb[3], tn[3];
+            closest_point_analytic(bs[bb], p, tb, tn);
+            const double d2 = (tb[0] - p[0]) * (tb[0] - p[0]) + (tb[1] - p[1]) * (tb[1] - p[1]) +
+                              (tb[2] - p[2]) * (tb[2] - p[2]);
+            if (d2 < best) {
+              best = d2;
+              f.body = bb;
+              for (int d = 0; d < 3; d++) bi[d] = tb[d];
+            }
+          }
+        } else {
+          for (int bb = 0; bb < nbody; bb++) {
+            const double d2 = tri_nearest_marker_d2(tri, bb, p);
+            if (d2 < best) { best = d2; f.body = bb; }
+          }
+          if (f.body >= 0) ferr[q] = tri_intercept(G, tri, f.body, f.i, f.j, f.k, bi, nrm, f.elem);
+        }
+        if (f.body < 0) ferr[q] = 7;
+        if (!ferr[q]) ferr[q] = fresh_stencil(G, bi, probe_norm, f);
+      }
+    });
+    for (int q = 0; q < nfr; q++)
+      if (ferr[q]) {
+        o->fresh_err = ferr[q];
+        o->fresh_err_cell[0] = o->fout[q].i;
+        o->fresh_err_cell[1] = o->fout[q].j;
+        o->fresh_err_cell[2] = o->fout[q].k;
+        break;
+      }
   }
   *out = o;
   return PP_OK;
@@ -583,6 +680,36 @@ int32_t pp_geom_ghost_elements(const pp_geom *o, int32_t *body, int32_t *elem) {
 int32_t pp_geom_dirichlet(const pp_geom *o, double *coeffD) {
   if (!o || !coeffD) { pp::set_error("pp_geom_dirichlet: null argument"); return PP_BAD_ARG; }
   for (size_t q = 0; q < o->gout.size(); q++) memcpy(coeffD + 8 * q, o->gout[q].coeffD, 8 * sizeof(double));
+  return PP_OK;
+}
+
+int32_t pp_geom_fresh(const pp_geom *o, int32_t *ifr, int32_t *jfr, int32_t *kfr, int32_t *ic, int32_t *jc,
+                      int32_t *kc, double *coeffD, double *bi, int32_t *body, int32_t *elem) {
+  if (!o) { pp::set_error("pp_geom_fresh: null handle"); return PP_BAD_ARG; }
+  if (o->fresh_err) {
+    const int e = o->fresh_err;
+    pp::set_error("pp_geom_fresh: fresh cell (%d,%d,%d): %s", o->fresh_err_cell[0], o->fresh_err_cell[1],
+                  o->fresh_err_cell[2],
+                  e == FRESH_ERR_STENCIL ? "image point for fresh cell is not inside the 2x2x2 stencil"
+                  : e == FRESH_ERR_AT_BI ? "body intercept at the cell centre"
+                  : e == 3               ? "singular Vandermonde"
+                  : e == 7               ? "no body"
+                                         : tri_error_text(e));
+    return PP_BAD_GEOMETRY;
+  }
+  for (size_t q = 0; q < o->fout.size(); q++) {
+    const FreshOut &f = o->fout[q];
+    if (ifr) ifr[q] = f.i;
+    if (jfr) jfr[q] = f.j;
+    if (kfr) kfr[q] = f.k;
+    if (ic) ic[q] = f.ic;
+    if (jc) jc[q] = f.jc;
+    if (kc) kc[q] = f.kc;
+    if (coeffD) memcpy(coeffD + 8 * q, f.coeffD, 8 * sizeof(double));
+    if (bi) memcpy(bi + 3 * q, f.bi, 3 * sizeof(double));
+    if (body) body[q] = f.body + 1;
+    if (elem) elem[q] = f.elem;
+  }
   return PP_OK;
 }
 

@@ -130,6 +130,8 @@ int tri_iblank(const Grid &G, const TriBodies &T, ByteField &iblank, IntField &b
 // 0 or 5
 int tri_intercept(const Grid &G, const TriBodies &T, int body, int iG, int jG, int kG, double *bi, double *nrm,
                   int &elem);
+// squared distance from P to the body's closest marker
+double tri_nearest_marker_d2(const TriBodies &T, int body, const double *P);
 
 bool inside_analytic(const Body &b, const double *p);
 void closest_point_analytic(const Body &b, const double *p, double *bi, double *nin);
@@ -138,5 +140,24 @@ void fill_cell(const Grid &G, ByteField &ib, ByteField &fresh);
 void mark_ghosts(const Grid &G, const ByteField &ib, ByteField &gm, std::vector<int> &list);
 int ghost_stencil(const Grid &G, int iG, int jG, int kG, const double *bi, const double *nrm, double probe_norm,
                   GhostOut &o);
+
+// One fresh cell's velocity interpolation (GCM_set_freshcell,
+// GCM_SET_FRESHCELL.F90:86-262): the 2x2x2 stencil below the image point
+// FIP = BI + (FC - BI) probeLengthNormalized and coeffGCMFreshD, the weights
+// that give the value at the cell centre FC from the stencil values with the
+// cell's own node replaced by the body intercept BI.
+struct FreshOut {
+  int i = 0, j = 0, k = 0;         // the fresh cell (1-based)
+  int ic = -1, jc = -1, kc = -1;   // iFreshCellIndex..: the stencil's lower corner
+  int body = -1, elem = 0;         // body (0-based) and surface element (1-based, triangulated mode)
+  double bi[3] = {0, 0, 0};
+  double coeffD[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+// fresh-cell error codes of fresh_stencil (0 = success, 3 singular Vandermonde)
+enum {
+  FRESH_ERR_STENCIL = 10,   // the image point's cell is not the fresh cell or the one below it
+  FRESH_ERR_AT_BI = 11,     // the body intercept coincides with the cell centre
+};
+int fresh_stencil(const Grid &G, const double *bi, double probe_norm, FreshOut &o);
 
 }  // namespace ppg

@@ -659,6 +659,12 @@ int tri_iblank(const Grid &G, const TriBodies &T, ByteField &ib, IntField &bodyn
 }
 
 // ---- body intercepts ----------------------------------------------------------
+double tri_nearest_marker_d2(const TriBodies &T, int body, const double *P) {
+  int m;
+  closest_markers(T.b[body], P, 1, &m);
+  return marker_d2(T.b[body], m - 1, P);
+}
+
 int tri_intercept(const Grid &G, const TriBodies &T, int body, int iG, int jG, int kG, double *bi, double *nrm,
                   int &elem) {
   const TriBody &B = T.b[body];

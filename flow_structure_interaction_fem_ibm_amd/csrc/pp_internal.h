@@ -142,6 +142,15 @@ struct GhostTables {
   bool have_vel = false;
 };
 
+// fresh cells of the momentum solve (pp_set_fresh_cells)
+struct FreshRows {
+  int n = 0;
+  long *ids = nullptr;      // reference offsets, ascending
+  double *face = nullptr;   // 6 per cell: the fresh row's amx_ad..apz_ad
+  long *coff = nullptr;     // 4 per cell: offsets of the off-stencil nodes, stencil order
+  double *cc = nullptr;     // 4 per cell: their coeffGCMFreshD
+};
+
 struct CoarseSolve {
   int n = 0;                      // cells of the coarsest level (at most 4096)
   double *G = nullptr;            // n*n by output cell: G[q*n+p] = d e_q / d r_p (FP64 in both precisions)
@@ -185,6 +194,7 @@ struct pp_ctx {
   double *d_ad_coef = nullptr;
   double ad_dt = 0.0;
   bool have_ad = false;
+  pp::FreshRows fr;                    // fresh-cell rows of the current geometry
   double null_ref = 1.0;               // initial residual norm of the cold-started null vector
   int null_iters = 0;
   double null_res = 0.0;           // level-0 smoother uses the transposed ghost rows

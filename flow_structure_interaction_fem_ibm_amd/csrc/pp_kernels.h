@@ -436,6 +436,110 @@ __global__ void __launch_bounds__(NT) k_correct_vel(RefArgs R, LevelArgs<double>
   }
 }
 
+// face_vel (CORE_FRAC_STEP_ROUTINES.F90:6-240) for one interior cell: its
+// west/south/back faces (the last cell of a row also its east/north/front
+// face), each the composition of the reference's sequential writes to it:
+//   uT = u + dt grad_d p (1 - iblank) for VAN_KAN (:36-70), else u;
+//   f  = (fx uT_c + (1 - fx) uT_w)(1 - m_c) + bc_c m_c          (:80-107)
+//   VAN_KAN: f -= dt ((p_c - p_w) dcinv)(1 - m_c), then the outer-face
+//   masks f = f (1 - p_w) + bc_w p_w; f = f (1 - m_c) + bc_c m_c (:111-175)
+//   the same two masks again (:179-197)
+//   VAN_KAN: f = (1 - g_w) f + g_w I; f = (1 - g_c) f + g_c I with I the
+//   interpolation of u itself (ghost cells, :201-234)
+//   0 if either cell is blank and not a ghost (:238)
+// with m = ium/jum/kum and p = iup/jup/kup (the level-0 outer-face flags) and
+// the west cell's terms only where it is an interior cell.  The cell
+// gradient is the cell's own from fx-interpolated face pressures (the cell
+// value on an outer face).  One thread owns every face it writes.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_face_vel(RefArgs R, LevelArgs<double> L0, FracMetrics M, int van_kan,
+                                                 double dt, const double *__restrict__ p,
+                                                 const double *__restrict__ u, const double *__restrict__ v,
+                                                 const double *__restrict__ w, const double *__restrict__ bcu,
+                                                 const double *__restrict__ bcv, const double *__restrict__ bcw,
+                                                 double *__restrict__ fu, double *__restrict__ fv,
+                                                 double *__restrict__ fw) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= R.nxc || j >= R.nyc) return;
+  const long id = ridx(R, i, j, k);
+  const long st[3] = {1, R.ex, R.exy};
+  const int I[3] = {i + 1, j + 1, k + 1};
+  const int nc[3] = {R.nxc, R.nyc, R.nzc};
+  const long lst[3] = {1, (long)L0.sx, L0.sxy};
+  const long la = lidx(L0.sx, L0.sxy, i, j, k);
+  const uint8_t f = L0.fl[la];
+  const uint8_t FM[3] = {FL_IM, FL_JM, FL_KM}, FP[3] = {FL_IP, FL_JP, FL_KP};
+  const double *vel[3] = {u, v, w}, *bc[3] = {bcu, bcv, bcw};
+  double *face[3] = {fu, fv, fw};
+  // uTilde of the d-component at interior cell q (flags fq, index c along d)
+  auto utilde = [&](int d, long q, uint8_t fq, int c) -> double {
+    if (!van_kan) return vel[d][q];
+    const int mq = (fq & FM[d]) ? 1 : 0, pq = (fq & FP[d]) ? 1 : 0;
+    const double fe = M.fx[d][c + 1], fwt = M.fx[d][c], pc = p[q];
+    const double pe = (fe * p[q + st[d]] + (1.0 - fe) * pc) * (double)(1 - pq) + pc * (double)pq;
+    const double pw = (fwt * pc + (1.0 - fwt) * p[q - st[d]]) * (double)(1 - mq) + pc * (double)mq;
+    const double pg = (pe - pw) * M.dinv[d][c];
+    return vel[d][q] + dt * pg * (double)(1 - ((fq & FL_BLANK) ? 1 : 0));
+  };
+  const bool solid = (f & FL_BLANK) && !(f & FL_GHOST);
+  const double gc = (f & FL_GHOST) ? 1.0 : 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int c = I[d];
+    const int mc = (f & FM[d]) ? 1 : 0, pc = (f & FP[d]) ? 1 : 0;
+    const double *U = vel[d];
+    const double uTc = utilde(d, id, f, c);
+    // west face
+    {
+      const bool has_w = c > 1;
+      uint8_t fo = 0;
+      if (has_w) fo = L0.fl[la - lst[d]];
+      const double uTw = has_w ? utilde(d, id - st[d], fo, c - 1) : U[id - st[d]];
+      const int pw = (fo & FP[d]) ? 1 : 0;
+      const double fxc = M.fx[d][c];
+      double x = (fxc * uTc + (1.0 - fxc) * uTw) * (double)(1 - mc) + bc[d][id] * (double)mc;
+      auto masks = [&](double y) {
+        if (has_w) y = y * (double)(1 - pw) + bc[d][id - st[d]] * (double)pw;
+        return y * (double)(1 - mc) + bc[d][id] * (double)mc;
+      };
+      if (van_kan) {
+        const double gw = (p[id] - p[id - st[d]]) * M.dcinv[d][c];
+        x = masks(x - dt * gw * (double)(1 - mc));
+      }
+      x = masks(x);
+      bool zero = solid;
+      if (van_kan) {
+        const double ip = fxc * U[id] + (1.0 - fxc) * U[id - st[d]];
+        if (has_w) {
+          const double gwf = (fo & FL_GHOST) ? 1.0 : 0.0;
+          x = (1.0 - gwf) * x + gwf * ip;
+        }
+        x = (1.0 - gc) * x + gc * ip;
+      }
+      if (has_w) zero = zero || ((fo & FL_BLANK) && !(fo & FL_GHOST));
+      face[d][id] = zero ? 0.0 : x;
+    }
+    // east face of the last cell of a row
+    if (c == nc[d]) {
+      const double fxe = M.fx[d][c + 1];
+      double x = (fxe * U[id + st[d]] + (1.0 - fxe) * uTc) * (double)(1 - pc) + bc[d][id] * (double)pc;
+      if (van_kan) {
+        const double ge = (p[id + st[d]] - p[id]) * M.dcinv[d][c + 1];
+        x = x - dt * ge * (double)(1 - pc);
+        x = x * (double)(1 - pc) + bc[d][id] * (double)pc;
+      }
+      x = x * (double)(1 - pc) + bc[d][id] * (double)pc;
+      if (van_kan) {
+        const double ip = fxe * U[id + st[d]] + (1.0 - fxe) * U[id];
+        x = (1.0 - gc) * x + gc * ip;
+      }
+      face[d][id + st[d]] = solid ? 0.0 : x;
+    }
+  }
+}
+
 // velocity ghost rows (GCM_vel_set_bc_internal): gather the ghost cells'
 // current (u, v, w), Jacobi sweeps of
 //   u_g = u_b bw - (sum_r cD_r x_r) iw
@@ -724,7 +828,9 @@ __global__ void __launch_bounds__(256) k_fresh_fill(RefArgs R, const int8_t *__r
         long q = r + di + dj * R.ex + dk * R.exy;
         if (iblank_ref[q] != 1 && fresh[q] != 1) { n++; s += src[q]; }
       }
-  p[r] = s / (double)n;
+  // no fluid non-fresh neighbour (a body that moved more than a cell): the
+  // reference divides by zero; the cell keeps its value
+  if (n > 0) p[r] = s / (double)n;
 }
 
 // ---------------------------------------------------------------------
@@ -2013,7 +2119,8 @@ __global__ void __launch_bounds__(1024) k_tdma_seg(int n, int nlines, const doub
 // Implicit momentum (advection-diffusion) solve of the fractional step,
 // FP64 on the reference layout: set_solve_ad, the line sweeps
 // itsolv_ad_x/y/z and calc_residual_ad (CORE_SET_SOLVERS.F90:20-223,
-// CORE_SOLVE_AD.F90:314-907), without fresh cells or periodic faces.
+// CORE_SOLVE_AD.F90:314-907) with the GCM fresh-cell rows, without periodic
+// faces.
 // The level-0 flags give iblank (FL_BLANK), ghostCellMark (FL_GHOST) and
 // ium..kup (FL_IM..FL_KP).
 // =====================================================================
@@ -2062,7 +2169,8 @@ template <int NT>
 __global__ void __launch_bounds__(NT) k_ad_rows(RefArgs R, LevelArgs<double> L0, FracMetrics M, double half_dt,
                                                 const double *__restrict__ coef, const double *__restrict__ fu,
                                                 const double *__restrict__ fv, const double *__restrict__ fw,
-                                                double *__restrict__ row, long nref) {
+                                                const int8_t *__restrict__ fresh, double *__restrict__ row,
+                                                long nref) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
@@ -2092,7 +2200,58 @@ __global__ void __launch_bounds__(NT) k_ad_rows(RefArgs R, LevelArgs<double> L0,
     row[(2 * d + 1) * nref + id] = ap;
     ac[d] = a;
   }
-  row[6 * nref + id] = ac[0] + ac[1] + ac[2];
+  // killForGCMFresh: a fresh cell's diagonal is 1 (CORE_SOLVE_AD.F90:425-427, 866)
+  const double kill = fresh ? 1.0 - 1.0 * (double)fresh[id] : 1.0;
+  row[6 * nref + id] = (ac[0] + ac[1] + ac[2]) * kill;
+}
+
+// Fresh-cell rows of set_solve_ad's GCM branch (CORE_SET_SOLVERS.F90:
+// 175-218): amx_ad..apz_ad of each fresh cell replaced by -coeffGCMFreshD of
+// its face neighbour in the stencil, 0 where the stencil has none (face: 6
+// per cell, prepared with the tables).
+__global__ void k_fresh_ad_coef(int n, const long *__restrict__ ids, const double *__restrict__ face,
+                                double *__restrict__ coef, long nref) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const long id = ids[q];
+#pragma unroll
+  for (int d = 0; d < 6; d++) coef[d * nref + id] = face[6 * q + d];
+}
+
+// The off-stencil terms of GCM_correct_rhs_ad (GCM_FRESHCELL_AD.F90:2-70)
+// for a zebra line pass: rw = r + sum_m cc_m var(coff_m) at each fresh cell
+// (the four stencil nodes that differ from the cell in two or more indices,
+// in stencil order), from the fields as the pass starts, so every line of
+// the pass reads the same values.  rw is the call's copy of r, equal to r
+// off the fresh cells.  One thread per cell and field.
+struct FreshCorr {
+  const double *var[3];
+  const double *r[3];
+  double *rw[3];
+};
+__global__ void k_fresh_rhs(int n, int nv, const long *__restrict__ ids, const long *__restrict__ coff,
+                            const double *__restrict__ cc, FreshCorr F) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * nv) return;
+  const int q = t % n, f = t / n;
+  const double *var = F.var[f];
+  double s = 0.0;
+#pragma unroll
+  for (int m = 0; m < 4; m++) s = s + cc[4 * q + m] * var[coff[4 * q + m]];
+  F.rw[f][ids[q]] = F.r[f][ids[q]] + s;
+}
+
+// index of reference offset id in the ascending fresh list, -1 if absent
+__device__ inline int fresh_find(const long *__restrict__ ids, int n, long id) {
+  int lo = 0, hi = n - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const long v = ids[mid];
+    if (v == id) return mid;
+    if (v < id) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
 }
 
 // SegElim for NR right-hand sides of one tridiagonal matrix: the
@@ -2327,10 +2486,18 @@ __global__ void __launch_bounds__(512) k_ad_line(RefArgs R, LevelArgs<double> L0
 struct AdFields {
   const double *var[3], *r[3];
 };
+// fresh cells of the residual: mask (reference layout, NULL = none), the
+// ascending offsets and the four off-stencil nodes and weights of each
+struct FreshRes {
+  const int8_t *mask;
+  const long *ids, *coff;
+  const double *cc;
+  int n;
+};
 template <int NT, int NV>
 __global__ void __launch_bounds__(NT) k_ad_residual(RefArgs R, LevelArgs<double> L0, const double *__restrict__ row,
-                                                    long nref, AdFields F, double *__restrict__ part, long nblk,
-                                                    double *__restrict__ rout) {
+                                                    long nref, AdFields F, FreshRes FR, double *__restrict__ part,
+                                                    long nblk, double *__restrict__ rout) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
@@ -2348,6 +2515,9 @@ __global__ void __launch_bounds__(NT) k_ad_residual(RefArgs R, LevelArgs<double>
   const double ojm = (f & FL_JM) ? 0.0 : 1.0, ojp = (f & FL_JP) ? 0.0 : 1.0;
   const double okm = (f & FL_KM) ? 0.0 : 1.0, okp = (f & FL_KP) ? 0.0 : 1.0;
   const double fl = (f & FL_BLANK) ? 0.0 : 1.0;
+  // GCM_correct_res_ad (GCM_FRESHCELL_AD.F90:73-137): the off-stencil terms
+  // of a fresh cell, added one by one in stencil order before the iblank mask
+  const int fq = (in && FR.mask && FR.mask[id]) ? fresh_find(FR.ids, FR.n, id) : -1;
   const long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
 #pragma unroll
   for (int q = 0; q < NV; q++) {
@@ -2357,6 +2527,8 @@ __global__ void __launch_bounds__(NT) k_ad_residual(RefArgs R, LevelArgs<double>
       double s = F.r[q][id] - x[id] * (1.0 + sum * 1.0) - x[id - 1] * amx * oim - x[id + 1] * apx * oip -
                  x[id - R.ex] * amy * ojm - x[id + R.ex] * apy * ojp - x[id - R.exy] * amz * okm -
                  x[id + R.exy] * apz * okp;
+      if (fq >= 0)
+        for (int m = 0; m < 4; m++) s = s + FR.cc[4 * fq + m] * x[FR.coff[4 * fq + m]];
       s = s * fl;
       if (rout) rout[id] = s;
       v.mx = fabs(s);

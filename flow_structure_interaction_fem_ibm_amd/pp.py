@@ -168,6 +168,8 @@ def lib():
     L.pp_geom_ghost_elements.argtypes = [P, P, P]
     L.pp_geom_destroy.argtypes = [P]
     L.pp_geom_dirichlet.argtypes = [P, P]
+    L.pp_geom_fresh.argtypes = [P] * 11
+    L.pp_set_fresh_cells.argtypes = [P, C.c_int32] + [P] * 7
     L.pp_geom_build_surfaces.argtypes = [C.POINTER(C.c_void_p), C.POINTER(GridDesc), C.c_int32,
                                          C.POINTER(SurfaceDesc), C.c_double, P]
     L.pp_geom_canonical_surface.argtypes = [C.POINTER(BodyDesc), C.POINTER(C.c_int32), C.POINTER(C.c_int32), P, P]
@@ -181,6 +183,10 @@ def lib():
     L.pp_solve_ad.argtypes = [P, C.c_int32] + [P] * 10 + [C.c_int32, C.c_double, C.POINTER(ADStats)]
     L.pp_solve_ad_device.argtypes = [P, C.c_int32] + [P] * 10 + [C.c_int32, C.c_double, C.POINTER(ADStats)]
     L.pp_calc_residual_ad.argtypes = [P, C.c_int32] + [P] * 6 + [dp, C.POINTER(C.c_int32)]
+    L.pp_face_vel.argtypes = [P, C.c_int32, C.c_double] + [P] * 10
+    L.pp_face_vel_device.argtypes = [P, C.c_int32, C.c_double] + [P] * 10
+    L.pp_solve_ad_cn2.argtypes = [P] + [P] * 9 + [C.c_int32] + [P] * 5 + [C.c_int32, C.c_double, C.POINTER(ADStats)]
+    L.pp_solve_ad_cn2_device.argtypes = L.pp_solve_ad_cn2.argtypes
     for name in ("pp_create", "pp_destroy", "pp_set_geometry", "pp_solve", "pp_solve_device",
                  "pp_adjoint", "pp_calc_residual", "pp_ghost_rows", "pp_mean_remove",
                  "pp_divergence", "pp_smooth", "pp_tdma_batched", "pp_null_vector", "pp_level_info", "pp_time_kernel",
@@ -189,7 +195,8 @@ def lib():
                  "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device", "pp_adjoint_device",
                  "pp_set_solve_ad", "pp_set_solve_ad_device", "pp_solve_ad", "pp_solve_ad_device",
                  "pp_calc_residual_ad", "pp_geom_build_surfaces", "pp_geom_canonical_surface",
-                 "pp_level_apply", "pp_level_data"):
+                 "pp_level_apply", "pp_level_data", "pp_geom_fresh", "pp_set_fresh_cells",
+                 "pp_face_vel", "pp_face_vel_device", "pp_solve_ad_cn2", "pp_solve_ad_cn2_device"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -203,7 +210,8 @@ EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_s
             "pp_correct_vel", "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device",
             "pp_adjoint_device", "pp_set_solve_ad", "pp_set_solve_ad_device", "pp_solve_ad", "pp_solve_ad_device",
             "pp_calc_residual_ad", "pp_geom_build_surfaces", "pp_geom_canonical_surface", "pp_level_apply",
-            "pp_level_data")
+            "pp_level_data", "pp_geom_fresh", "pp_set_fresh_cells", "pp_face_vel",
+            "pp_face_vel_device", "pp_solve_ad_cn2", "pp_solve_ad_cn2_device")
 
 
 def _check(rc, allow=(PP_OK,)):
@@ -396,6 +404,15 @@ class PressureSolver:
                                                None if bc_src is None else bc_src.data_ptr()))
 
     # ---- implicit momentum solve ----
+    def set_fresh_cells(self, tables):
+        """pp_set_fresh_cells from the tables build_geometry returns as
+        extras["fresh"] (keys i, j, k, ic, jc, kc, coeffD)."""
+        a = [np.ascontiguousarray(tables[k], dtype=np.int32) for k in ("i", "j", "k", "ic", "jc", "kc")]
+        cd = np.ascontiguousarray(tables["coeffD"], dtype=np.float64).reshape(-1, 8)
+        n = len(a[0])
+        assert all(len(x) == n for x in a) and len(cd) == n
+        _check(lib().pp_set_fresh_cells(self.h, n, *[x.ctypes.data for x in a], cd.ctypes.data))
+
     def set_solve_ad(self, dt, visc_tot, bcxvisc, bcyvisc, bczvisc):
         """pp_set_solve_ad: the implicit viscous coefficients (set_solve_ad)."""
         f = [np.ascontiguousarray(a, dtype=np.float64) for a in (visc_tot, bcxvisc, bcyvisc, bczvisc)]
@@ -430,6 +447,35 @@ class PressureSolver:
                                         None if vel_bi is None else vel_bi.data_ptr(), int(itermax_ad),
                                         float(restol_ad), C.byref(st)), allow=(PP_OK, PP_NOT_CONVERGED))
         return st.as_dict()
+
+    def solve_ad_cn2(self, u, v, w, nlu, nlv, nlw, face_u, face_v, face_w, van_kan, p, bcxu, bcyv, bczw,
+                     vel_bi=None, itermax_ad=1000, restol_ad=1e-10):
+        """pp_solve_ad_cn2 on copies; returns (u, v, w, (face_u, face_v, face_w), stats)."""
+        out = [np.array(a, dtype=np.float64, order="C", copy=True) for a in (u, v, w, face_u, face_v, face_w)]
+        ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (nlu, nlv, nlw)]
+        fv_in = [None if p is None else np.ascontiguousarray(p, dtype=np.float64)] + \
+                [np.ascontiguousarray(a, dtype=np.float64) for a in (bcxu, bcyv, bczw)]
+        for a in out + ins + [x for x in fv_in if x is not None]:
+            assert a.shape == self.shape
+        vb = None if vel_bi is None else np.ascontiguousarray(vel_bi, dtype=np.float64)
+        st = ADStats()
+        _check(lib().pp_solve_ad_cn2(self.h, *[a.ctypes.data for a in out[:3]], *[a.ctypes.data for a in ins],
+                                     *[a.ctypes.data for a in out[3:]], int(van_kan),
+                                     *[None if a is None else a.ctypes.data for a in fv_in],
+                                     None if vb is None else vb.ctypes.data, int(itermax_ad), float(restol_ad),
+                                     C.byref(st)), allow=(PP_OK, PP_NOT_CONVERGED))
+        return out[0], out[1], out[2], tuple(out[3:]), st.as_dict()
+
+    def face_vel(self, van_kan, dt, p, u, v, w, bcxu, bcyv, bczw, faces=None):
+        """pp_face_vel: returns (face_u, face_v, face_w), written over copies of
+        `faces` (zeros when None)."""
+        ins = [None if p is None else np.ascontiguousarray(p, dtype=np.float64)] + \
+              [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, bcxu, bcyv, bczw)]
+        out = [np.zeros(self.shape) if faces is None else np.array(f, dtype=np.float64, order="C", copy=True)
+               for f in (faces or (None, None, None))]
+        _check(lib().pp_face_vel(self.h, int(van_kan), float(dt), *[None if a is None else a.ctypes.data for a in ins],
+                                 *[a.ctypes.data for a in out]))
+        return out
 
     def calc_residual_ad(self, advec_scheme, var, r, face_u, face_v, face_w):
         """pp_calc_residual_ad: returns (max |res|, 1-based loc, res field)."""
@@ -629,11 +675,28 @@ def _geometry_result(h, xn, yn, zn):
         _check(L.pp_geom_ghost_elements(h, body.ctypes.data, elem.ctypes.data))
         coeffD = np.zeros((max(n, 1), 8))
         _check(L.pp_geom_dirichlet(h, coeffD.ctypes.data))
+        fresh, fresh_error = _fresh_tables(h, nf.value)
     finally:
         L.pp_geom_destroy(h)
     gs = GhostSet(*[a[:n] for a in ints], coeff[:n], probe[:n])
     return ib, fr, gs, {"bi": bi[:n], "ip": ip[:n], "nrm": nrm[:n], "body": body[:n], "elem": elem[:n],
-                        "coeffD": coeffD[:n]}
+                        "coeffD": coeffD[:n], "fresh": fresh, "fresh_error": fresh_error}
+
+
+FRESH_KEYS = ("i", "j", "k", "ic", "jc", "kc", "coeffD", "bi", "body", "elem")
+
+
+def _fresh_tables(h, n):
+    """pp_geom_fresh: (tables, None), or (None, message) when a fresh cell's
+    stencil fails (the pressure path does not need the tables)."""
+    m = max(n, 1)
+    t = {k: np.zeros(m, np.int32) for k in ("i", "j", "k", "ic", "jc", "kc", "body", "elem")}
+    t["coeffD"] = np.zeros((m, 8))
+    t["bi"] = np.zeros((m, 3))
+    rc = lib().pp_geom_fresh(h, *[t[k].ctypes.data for k in FRESH_KEYS])
+    if rc != PP_OK:
+        return None, lib().pp_last_error().decode()
+    return {k: v[:n] for k, v in t.items()}, None
 
 
 def tanh_nodes(n, length, beta):

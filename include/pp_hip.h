@@ -285,16 +285,31 @@ typedef struct pp_ad_stats {
  * coefficients amx_ad..apz_ad of the momentum equations, kept on the
  * device until the next call or pp_set_geometry.  visc_tot = viscTot
  * (reference layout, outer layer included), bcxvisc..bczvisc the
- * boundary-face viscosities.  Fresh cells (GCM_set_freshcell branch,
- * :175-200, and GCM_correct_rhs_ad/res_ad, GCM_FRESHCELL_AD.F90:2-137) are
- * not built: with a geometry that has fresh cells this call, pp_solve_ad and
- * pp_calc_residual_ad return PP_BAD_ARG.  Requires pp_set_geometry. */
+ * boundary-face viscosities.  With fresh cells in the geometry (its
+ * fresh_cell mask) the fresh rows of the GCM branch (:175-218) are set from
+ * the tables of pp_set_fresh_cells, which must be called first; pp_solve_ad
+ * and pp_calc_residual_ad then add the off-stencil terms of
+ * GCM_correct_rhs_ad / GCM_correct_res_ad (GCM_FRESHCELL_AD.F90:2-137).
+ * Requires pp_set_geometry. */
 int32_t pp_set_solve_ad(pp_ctx *ctx, double dt, const double *visc_tot,
                         const double *bcxvisc, const double *bcyvisc,
                         const double *bczvisc);
 int32_t pp_set_solve_ad_device(pp_ctx *ctx, double dt, const double *d_visc_tot,
                                const double *d_bcxvisc, const double *d_bcyvisc,
                                const double *d_bczvisc);
+/* The fresh cells of the last pp_set_geometry for the momentum solve
+ * (GCM_set_freshcell's tables, as pp_geom_fresh gives them): n_fresh cells
+ * (1-based i, j, k), exactly those of the geometry's fresh_cell mask in any
+ * order, each with the lower corner of its 2x2x2 stencil (the cell or the
+ * one below it in each direction) and coeffGCMFreshD(8, n_fresh)
+ * column-major.  The fresh rows are u_f - sum_(stencil nodes but the cell)
+ * c u_node = nlu_f with nlu_f = c_self u_BI formed by the caller
+ * (rhs_adjust_bc, CORE_RHS_AD.F90:796-815).  pp_set_geometry clears the
+ * tables; call pp_set_solve_ad after this. */
+int32_t pp_set_fresh_cells(pp_ctx *ctx, int32_t n_fresh, const int32_t *i_fresh,
+                           const int32_t *j_fresh, const int32_t *k_fresh,
+                           const int32_t *i_cell, const int32_t *j_cell,
+                           const int32_t *k_cell, const double *coeffD);
 
 /* solve_ad, GCM branch (CORE_SOLVE_AD.F90:200-267): until max(calc_residual_ad
  * of u, v, w) <= restol_ad or itermax_ad iterations, line sweeps in x, y
@@ -304,7 +319,8 @@ int32_t pp_set_solve_ad_device(pp_ctx *ctx, double dt, const double *d_visc_tot,
  * (GCM_GHOSTCELL_VEL_UPDATE_UNSTRUC.F90:2-124, Dirichlet outer faces: its
  * mass-flux correction does not act).  u, v, w: initial guess in, solution
  * out; nlu, nlv, nlw: right-hand sides; face_u/v/w: face velocities of the
- * advection terms (read for CRANK_NICOLSON1); vel_bi: (u, v, w) per ghost
+ * advection terms (read for CRANK_NICOLSON1; CRANK_NICOLSON2 goes through
+ * pp_solve_ad_cn2); vel_bi: (u, v, w) per ghost
  * at the body intercept or NULL.  Requires pp_set_solve_ad and, with
  * ghost cells, pp_set_vel_ghosts.  Returns PP_OK or PP_NOT_CONVERGED (the
  * fields are written either way, as the reference continues). */
@@ -321,6 +337,47 @@ int32_t pp_solve_ad_device(pp_ctx *ctx, int32_t advec_scheme, double *d_u,
                            const double *d_face_w, const double *d_vel_bi,
                            int32_t itermax_ad, double restol_ad,
                            pp_ad_stats *st);
+
+/* solve_ad with advec_scheme CRANK_NICOLSON2 (CORE_SOLVE_AD.F90:218-247):
+ * the loop of pp_solve_ad with face_vel after every ghost update, so the
+ * residual and the next sweeps use the face velocities of the new fields.
+ * face_u/v/w: the face velocities of the first sweeps in, those of the
+ * returned fields out; van_kan, p, bcxu..bczw as for pp_face_vel (dt is
+ * pp_set_solve_ad's).  pp_solve_ad refuses CRANK_NICOLSON2. */
+int32_t pp_solve_ad_cn2(pp_ctx *ctx, double *u, double *v, double *w,
+                        const double *nlu, const double *nlv, const double *nlw,
+                        double *face_u, double *face_v, double *face_w,
+                        int32_t van_kan, const double *p, const double *bcxu,
+                        const double *bcyv, const double *bczw,
+                        const double *vel_bi, int32_t itermax_ad,
+                        double restol_ad, pp_ad_stats *st);
+int32_t pp_solve_ad_cn2_device(pp_ctx *ctx, double *d_u, double *d_v,
+                               double *d_w, const double *d_nlu,
+                               const double *d_nlv, const double *d_nlw,
+                               double *d_face_u, double *d_face_v,
+                               double *d_face_w, int32_t van_kan,
+                               const double *d_p, const double *d_bcxu,
+                               const double *d_bcyv, const double *d_bczw,
+                               const double *d_vel_bi, int32_t itermax_ad,
+                               double restol_ad, pp_ad_stats *st);
+
+/* face_vel (CORE_FRAC_STEP_ROUTINES.F90:6-240), GCM, non-periodic outer
+ * boundaries: face velocities from the cell velocities by the fx
+ * interpolation of uTilde = u + dt grad p (1 - iblank) (van_kan = 1,
+ * frac_step_type VAN_KAN) or of u, the face pressure correction for
+ * VAN_KAN, outer faces from bcxu/bcyv/bczw, the faces of ghost cells from
+ * the interpolation of u itself (VAN_KAN), the faces of blank non-ghost
+ * cells zero.  p may be NULL when van_kan = 0.  Reference layout. */
+int32_t pp_face_vel(pp_ctx *ctx, int32_t van_kan, double dt, const double *p,
+                    const double *u, const double *v, const double *w,
+                    const double *bcxu, const double *bcyv, const double *bczw,
+                    double *face_u, double *face_v, double *face_w);
+int32_t pp_face_vel_device(pp_ctx *ctx, int32_t van_kan, double dt,
+                           const double *d_p, const double *d_u,
+                           const double *d_v, const double *d_w,
+                           const double *d_bcxu, const double *d_bcyv,
+                           const double *d_bczw, double *d_face_u,
+                           double *d_face_v, double *d_face_w);
 
 /* calc_residual_ad (CORE_SOLVE_AD.F90:766-907) of one field: *resm = max
  * |res| over the cells, loc its 1-based (i, j, k) (first in the
@@ -457,6 +514,21 @@ int32_t pp_geom_ghost_elements(const pp_geom *geo, int32_t *body, int32_t *elem)
 /* coeffGCMD(8,n) column-major: the velocity (Dirichlet) coefficients of
  * GCM_Calc_VanMatrixDN for the same ghosts, for pp_set_vel_ghosts. */
 int32_t pp_geom_dirichlet(const pp_geom *geo, double *coeffD);
+/* Fresh cells of a geometry built with iblank_prev (GCM_set_freshcell,
+ * GCM_SET_FRESHCELL.F90:1-262), n_fresh of pp_geom_counts in the reference's
+ * order (i fastest, then j, then k): the cell (1-based, iFresh..), the
+ * lower corner of its 2x2x2 velocity stencil (iFreshCellIndex..),
+ * coeffGCMFreshD(8, n) column-major in the stencil order of the ghost
+ * coefficients, the body intercept (3 per cell, xBodyInterceptFresh..), the
+ * body (1-based) and, in PP_GEOM_TRIANGULATED mode, the surface element
+ * (1-based, closestElementFresh; 0 in analytic mode).  The intercept is taken
+ * on the nearest body (the reference uses the body that blanked the cell at
+ * the previous geometry).  Any output may be NULL.  PP_BAD_GEOMETRY when a
+ * fresh cell's image point leaves the stencil (where the reference stops). */
+int32_t pp_geom_fresh(const pp_geom *geo, int32_t *i_fresh, int32_t *j_fresh,
+                      int32_t *k_fresh, int32_t *i_cell, int32_t *j_cell,
+                      int32_t *k_cell, double *coeffD, double *bi,
+                      int32_t *body, int32_t *elem);
 int32_t pp_geom_destroy(pp_geom *geo);
 
 #ifdef __cplusplus

@@ -1,16 +1,18 @@
 /*
  * or_ad.c -- the implicit momentum (advection-diffusion) solve of the
- * fractional step: set_solve_ad (CORE_SET_SOLVERS.F90:20-223, the branch
- * without fresh cells), the line sweeps itsolv_ad_x/y/z
+ * fractional step: set_solve_ad (CORE_SET_SOLVERS.F90:20-223, with the
+ * GCM fresh-cell rows of :175-218), the line sweeps itsolv_ad_x/y/z
  * (CORE_SOLVE_AD.F90:314-762; with half_dt = 0 they are the
  * itsolv_ad_*_AB routines of the same file), calc_residual_ad (:766-907)
  * and the GCM loop of solve_ad (:200-267) with the velocity ghost update
  * GCM_GhostCell_Vel (GCM_GHOSTCELL_VEL_UPDATE_UNSTRUC.F90:2-124; its
  * GCM_enforce_global_mass_consv acts only on ZERO_GRADIENT outer faces and
  * is a no-op for Dirichlet ones).  Test infrastructure only (see
- * oracle.h).  Restated for the configurations of the path: no periodic
- * outer boundaries, no fresh cells (the GCM_correct_rhs_ad /
- * GCM_correct_res_ad corrections are not restated), advec_scheme
+ * oracle.h).  Fresh cells (fresh mask with the tables of
+ * or_geom_set_freshcell or or_set_fresh_tables) take the diagonal 1
+ * (killForGCMFresh) and the off-stencil terms of GCM_correct_rhs_ad /
+ * GCM_correct_res_ad (GCM_FRESHCELL_AD.F90:2-137).  Restated for the
+ * configurations of the path: no periodic outer boundaries, advec_scheme
  * ADAMS_BASHFORTH2 (half_dt = 0) or CRANK_NICOLSON1 (half_dt = dt/2).
  *
  * coef holds amx_ad, apx_ad, amy_ad, apy_ad, amz_ad, apz_ad, one field
@@ -31,12 +33,31 @@
  *                    equals a dense solve (numpy) of the system assembled
  *                    column by column from calc_residual_ad and the
  *                    ghost rows u_g = bw u_BI - iw sum_r cD_r u_r.
+ * Fresh-cell pins (tests/test_oracle_fresh_ad.py):
+ *   set_freshcell    the weights reproduce every trilinear function at the
+ *                    cell centre from its stencil values with the cell's own
+ *                    node taken at BI; BI of a sphere is the radial
+ *                    projection of the centre within the marker spacing;
+ *                    the stencil corner follows a brute-force search.
+ *   fresh residual   ADAMS_BASHFORTH2: the residual at a fresh cell is
+ *                    r - (u_f - sum over the other seven stencil nodes of
+ *                    cD u), the full interpolation row; CRANK_NICOLSON1 adds
+ *                    exactly the advection terms an ordinary cell's row
+ *                    adds (the same difference with the cell's value 0).
+ *   fresh solve_ad   the converged solution equals the dense solve with the
+ *                    fresh rows u_f - sum cD u = r_f, and satisfies the
+ *                    interpolation at each fresh cell.
  */
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "or_internal.h"
+
+/* stencil order of the GCM interpolation (incI, incJ, incK) */
+static const int or_incI[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+static const int or_incJ[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+static const int or_incK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 
 /* fx(i) = (x(i) - xc(i-1)) / (xc(i) - xc(i-1)) (CORE_GRID.F90:172-174) */
 static double fint(const double *x, const double *xc, int i) { return (x[i] - xc[i - 1]) / (xc[i] - xc[i - 1]); }
@@ -78,6 +99,43 @@ void or_set_solve_ad(const or_ctx *c, double dt, const double *visc, const doubl
         a = (c->dzcinv[k + 1] * (1 - kup) + c->dzinv[k] * kup * 2.0) * c->dzinv[k];
         apz[id] = -(hdt * nuF) * a * rnDim;
       }
+  /* GCM branch (:175-218): the rows of the fresh cells, zero but for the
+   * face neighbours in the stencil, which take -coeffGCMFreshD */
+  for (int n = 0; n < c->nfr; n++) {
+    const int iF = c->fr_i[n], jF = c->fr_j[n], kF = c->fr_k[n];
+    const long id = OR_IDX(c, iF, jF, kF);
+    amx[id] = apx[id] = amy[id] = apy[id] = amz[id] = apz[id] = 0.0;
+    for (int r = 0; r < 8; r++) {
+      const int i = c->fr_ic[n] + or_incI[r], j = c->fr_jc[n] + or_incJ[r], k = c->fr_kc[n] + or_incK[r];
+      const double cD = c->fr_cD[8 * n + r];
+      if (i == iF - 1 && j == jF && k == kF) amx[id] = -cD;
+      if (i == iF + 1 && j == jF && k == kF) apx[id] = -cD;
+      if (i == iF && j == jF - 1 && k == kF) amy[id] = -cD;
+      if (i == iF && j == jF + 1 && k == kF) apy[id] = -cD;
+      if (i == iF && j == jF && k == kF - 1) amz[id] = -cD;
+      if (i == iF && j == jF && k == kF + 1) apz[id] = -cD;
+    }
+  }
+}
+
+/* the fresh-table entry of cell (i, j, k), -1 if none */
+static int fresh_entry(const or_ctx *c, int i, int j, int k) {
+  for (int n = 0; n < c->nfr; n++)
+    if (c->fr_i[n] == i && c->fr_j[n] == j && c->fr_k[n] == k) return n;
+  return -1;
+}
+
+/* GCM_correct_rhs_ad / GCM_correct_res_ad (GCM_FRESHCELL_AD.F90:2-137):
+ * x + coeffGCMFreshD * var at each stencil node outside the 7-point
+ * stencil (two indices off by one), added one by one in stencil order */
+static double fresh_correct(const or_ctx *c, int n, const double *var, double x) {
+  const int iF = c->fr_i[n], jF = c->fr_j[n], kF = c->fr_k[n];
+  for (int r = 0; r < 8; r++) {
+    const int i = c->fr_ic[n] + or_incI[r], j = c->fr_jc[n] + or_incJ[r], k = c->fr_kc[n] + or_incK[r];
+    const int di = i == iF - 1 || i == iF + 1, dj = j == jF - 1 || j == jF + 1, dk = k == kF - 1 || k == kF + 1;
+    if ((di && dj) || (di && dk) || (dj && dk)) x = x + c->fr_cD[8 * n + r] * var[OR_IDX(c, i, j, k)];
+  }
+  return x;
 }
 
 /* The per-cell operator of itsolv_ad_* and calc_residual_ad before any
@@ -139,6 +197,10 @@ double or_calc_residual_ad(const or_ctx *c, double half_dt, const double *coef, 
                    var[id + 1] * a.apx * (1 - c->iup[id]) - var[id - ex] * a.amy * (1 - c->jum[id]) -
                    var[id + ex] * a.apy * (1 - c->jup[id]) - var[id - exy] * a.amz * (1 - c->kum[id]) -
                    var[id + exy] * a.apz * (1 - c->kup[id]);
+        if (c->fresh[id] == 1) {
+          const int n = fresh_entry(c, i, j, k);
+          if (n >= 0) s = fresh_correct(c, n, var, s);
+        }
         s = s * (double)(1 - c->iblank[id]);
         if (res) res[id] = s;
         if (fabs(s) > resm) {
@@ -184,6 +246,10 @@ static void ad_line_row(const or_ctx *c, int dir, double half_dt, const double *
   L->cc[m] = ap;
   L->b[m] = 1.0 + a.sum * riblank * kill;
   L->r[m] = rhs * riblank + var[id] * (double)c->ghost[id];
+  if (c->fresh[id] == 1) {
+    const int n = fresh_entry(c, i, j, k);
+    if (n >= 0) L->r[m] = fresh_correct(c, n, var, L->r[m]);
+  }
 }
 
 void or_itsolv_ad(const or_ctx *c, double half_dt, const double *coef, const double *fu, const double *fv,
@@ -226,6 +292,35 @@ int or_solve_ad(const or_ctx *c, double half_dt, const double *coef, const doubl
     or_itsolv_ad(c, half_dt, coef, fu, fv, fw, v, nlv);
     or_itsolv_ad(c, half_dt, coef, fu, fv, fw, w, nlw);
     or_vel_ghosts(c, u, v, w, vbi, itermax, restol);
+    const double ru = or_calc_residual_ad(c, half_dt, coef, fu, fv, fw, u, nlu, NULL, NULL);
+    const double rv = or_calc_residual_ad(c, half_dt, coef, fu, fv, fw, v, nlv, NULL, NULL);
+    const double rw = or_calc_residual_ad(c, half_dt, coef, fu, fv, fw, w, nlw, NULL, NULL);
+    maxres = ru;
+    if (fabs(rv) > fabs(maxres)) maxres = rv;
+    if (fabs(rw) > fabs(maxres)) maxres = rw;
+    iter++;
+  }
+  if (maxres_out) *maxres_out = maxres;
+  return iter;
+}
+
+/* solve_ad with advec_scheme CRANK_NICOLSON2 (CORE_SOLVE_AD.F90:218-247):
+ * the GCM loop of or_solve_ad with face_vel after each ghost update, so the
+ * residual and the next sweeps use the face velocities of the new fields
+ * (fu, fv, fw in/out). */
+int or_solve_ad_cn2(const or_ctx *c, double dt, const double *coef, double *fu, double *fv, double *fw, double *u,
+                    double *v, double *w, const double *nlu, const double *nlv, const double *nlw, const double *vbi,
+                    int van_kan, const double *p, const double *bcxu, const double *bcyv, const double *bczw,
+                    int itermax, double restol, double *maxres_out) {
+  const double half_dt = 0.5 * dt;
+  int iter = 0;
+  double maxres = 1.0e10;
+  while (iter < itermax && maxres > restol) {
+    or_itsolv_ad(c, half_dt, coef, fu, fv, fw, u, nlu);
+    or_itsolv_ad(c, half_dt, coef, fu, fv, fw, v, nlv);
+    or_itsolv_ad(c, half_dt, coef, fu, fv, fw, w, nlw);
+    or_vel_ghosts(c, u, v, w, vbi, itermax, restol);
+    or_face_vel(c, van_kan, dt, p, u, v, w, bcxu, bcyv, bczw, fu, fv, fw);
     const double ru = or_calc_residual_ad(c, half_dt, coef, fu, fv, fw, u, nlu, NULL, NULL);
     const double rv = or_calc_residual_ad(c, half_dt, coef, fu, fv, fw, v, nlv, NULL, NULL);
     const double rw = or_calc_residual_ad(c, half_dt, coef, fu, fv, fw, w, nlw, NULL, NULL);

@@ -78,6 +78,36 @@ static void free_ghosts(or_ctx *c) {
   c->coeff = c->probe = c->bi = c->ipt = c->coeffD = NULL;
 }
 
+void or_free_fresh(or_ctx *c) {
+  free(c->fr_i); free(c->fr_j); free(c->fr_k); free(c->fr_ic); free(c->fr_jc); free(c->fr_kc);
+  free(c->fr_body); free(c->fr_elem); free(c->fr_cD); free(c->fr_bi);
+  c->fr_i = c->fr_j = c->fr_k = c->fr_ic = c->fr_jc = c->fr_kc = c->fr_body = c->fr_elem = NULL;
+  c->fr_cD = c->fr_bi = NULL;
+  c->nfr = 0;
+}
+
+void or_alloc_fresh(or_ctx *c, int n) {
+  or_free_fresh(c);
+  c->nfr = n;
+  size_t m = n > 0 ? (size_t)n : 1;
+  c->fr_i = calloc(m, 4); c->fr_j = calloc(m, 4); c->fr_k = calloc(m, 4);
+  c->fr_ic = calloc(m, 4); c->fr_jc = calloc(m, 4); c->fr_kc = calloc(m, 4);
+  c->fr_body = calloc(m, 4); c->fr_elem = calloc(m, 4);
+  c->fr_cD = calloc(8 * m, sizeof(double)); c->fr_bi = calloc(3 * m, sizeof(double));
+}
+
+void or_set_fresh_tables(or_ctx *c, int n, const int32_t *ifr, const int32_t *jfr, const int32_t *kfr,
+                         const int32_t *ic, const int32_t *jc, const int32_t *kc, const double *coeffD) {
+  or_alloc_fresh(c, n);
+  for (int q = 0; q < n; q++) {
+    c->fr_i[q] = ifr[q]; c->fr_j[q] = jfr[q]; c->fr_k[q] = kfr[q];
+    c->fr_ic[q] = ic[q]; c->fr_jc[q] = jc[q]; c->fr_kc[q] = kc[q];
+    memcpy(c->fr_cD + 8 * q, coeffD + 8 * q, 8 * sizeof(double));
+  }
+}
+
+int or_nfresh_tables(const or_ctx *c) { return c->nfr; }
+
 void or_destroy(or_ctx *c) {
   if (!c) return;
   double *arr[] = {c->x, c->xc, c->dx, c->dxinv, c->dxc, c->dxcinv,
@@ -88,6 +118,7 @@ void or_destroy(or_ctx *c) {
   free(c->ium); free(c->iup); free(c->jum); free(c->jup);
   free(c->kum); free(c->kup); free(c->bodynum);
   free_ghosts(c);
+  or_free_fresh(c);
   or_geom_clear_bodies(c);
   free(c);
 }
@@ -203,6 +234,7 @@ void or_set_ghost_stencils(or_ctx *c, const int32_t *ic, const int32_t *jc,
 
 void or_set_fresh(or_ctx *c, const int8_t *fresh) {
   memcpy(c->fresh, fresh, or_field_size(c));
+  or_free_fresh(c);
 }
 
 /* calc_residual (CORE_POISSON.F90:1111-1222), non-cut-cell branch, 3-D,
@@ -308,7 +340,10 @@ double or_mean_remove(const or_ctx *c, double *p) {
               long q = OR_IDX(c, ii, jj, kk);
               if (c->iblank[q] != 1 && c->fresh[q] != 1) { i1++; p1 += p[q]; }
             }
-        p[OR_IDX(c, i, j, k)] = p1 / (double)i1;
+        /* the reference divides by i1 = 0 for a fresh cell with no fluid,
+         * non-fresh neighbour (a body that moved more than a cell); such a
+         * cell keeps its value here */
+        if (i1 > 0) p[OR_IDX(c, i, j, k)] = p1 / (double)i1;
       }
   return pav;
 }

@@ -679,6 +679,7 @@ static int invert8(double A[8][8], double Ainv[8][8]) {
 int or_geom_set_boundary(or_ctx *c, const int8_t *iblank_prev, double probe_norm) {
   long nf = or_field_size(c);
   int8_t *temp = calloc(nf, 1);
+  or_free_fresh(c);
   int rc = set_iblank_fast(c, temp, c->bodynum);
   if (rc) { free(temp); return rc; }
   memset(c->fresh, 0, nf);
@@ -760,6 +761,100 @@ int or_geom_set_boundary(or_ctx *c, const int8_t *iblank_prev, double probe_norm
     for (int d = 0; d < 3; d++) { c->bi[3 * n + d] = bi[d]; c->ipt[3 * n + d] = ip[d]; }
   }
   return 0;
+}
+
+/* GCM_set_freshcell (GCM_SET_FRESHCELL.F90:1-262) for the fresh cells of
+ * the last or_geom_set_boundary: the list in i-fastest order (:56-72), the
+ * body intercept of each cell centre (GCM_calc_bodyIntercept_Unstruc on
+ * the body whose closest marker is nearest, the lower index on ties; the
+ * reference reads bodyNum, which for a fresh cell still holds the body that
+ * blanked it), the image point FIP = BI + (FC - BI)/|FC - BI| * |FC - BI| *
+ * probeLengthNormalized (:101-121), the stencil search over the cell and
+ * its two neighbours clamped to 1..n-1 (:123-145) with the reference's stop
+ * when the corner is neither the cell nor the one below it (:149-166), and
+ * coeffGCMFreshD from the inverse of the Dirichlet Vandermonde matrix whose
+ * row at the cell itself is taken at BI, evaluated at the cell centre
+ * (:196-260).  Returns 0, or 10 (image point outside the stencil), 11 (BI
+ * at the cell centre), 3 (singular) or an intercept error (4, 5). */
+int or_geom_set_freshcell(or_ctx *c, double probe_norm) {
+  int n = 0;
+  for (int k = 1; k <= c->nz - 1; k++)
+    for (int j = 1; j <= c->ny - 1; j++)
+      for (int i = 1; i <= c->nx - 1; i++) n += c->fresh[OR_IDX(c, i, j, k)] == 1;
+  or_alloc_fresh(c, n);
+  n = 0;
+  for (int k = 1; k <= c->nz - 1; k++)
+    for (int j = 1; j <= c->ny - 1; j++)
+      for (int i = 1; i <= c->nx - 1; i++)
+        if (c->fresh[OR_IDX(c, i, j, k)] == 1) { c->fr_i[n] = i; c->fr_j[n] = j; c->fr_k[n] = k; n++; }
+  for (int q = 0; q < n; q++) {
+    const int iF = c->fr_i[q], jF = c->fr_j[q], kF = c->fr_k[q];
+    const double FC[3] = {c->xc[iF], c->yc[jF], c->zc[kF]};
+    /* nearest body by its closest marker */
+    int bsel = -1;
+    double dbest = 0.0;
+    for (int b = 0; b < c->nbody; b++) {
+      const or_body *B = &c->body[b];
+      for (int m = 0; m < B->nmarker; m++) {
+        const double dx = B->xm[m] - FC[0], dy = B->ym[m] - FC[1], dz = B->zm[m] - FC[2];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (bsel < 0 || d2 < dbest) { dbest = d2; bsel = b; }
+      }
+    }
+    if (bsel < 0) return 7;
+    double bi[3];
+    int elem;
+    int rc = body_intercept(&c->body[bsel], FC, bi, &elem);
+    if (rc) return rc;
+    c->fr_body[q] = bsel + 1;
+    c->fr_elem[q] = elem + 1;
+    for (int d = 0; d < 3; d++) c->fr_bi[3 * q + d] = bi[d];
+    const double ds = sqrt((FC[1] - bi[1]) * (FC[1] - bi[1]) + (FC[0] - bi[0]) * (FC[0] - bi[0]) +
+                           (FC[2] - bi[2]) * (FC[2] - bi[2]));
+    if (!(ds > 0.0)) return 11;
+    const double dsinv = 1.0 / ds, plen = ds * probe_norm;
+    const double *cen[3] = {c->xc, c->yc, c->zc};
+    const int F3[3] = {iF, jF, kF}, nmax[3] = {c->nx - 1, c->ny - 1, c->nz - 1};
+    int cell[3];
+    for (int d = 0; d < 3; d++) {
+      const double fip = bi[d] + (FC[d] - bi[d]) * dsinv * plen;
+      const int lo = F3[d] - 1 > 1 ? F3[d] - 1 : 1, hi = F3[d] + 1 < nmax[d] ? F3[d] + 1 : nmax[d];
+      cell[d] = -1;
+      for (int i = lo; i <= hi; i++)
+        if (cen[d][i] <= fip && cen[d][i + 1] > fip) cell[d] = i;
+      if (cell[d] - F3[d] > 0 || cell[d] - F3[d] < -1) return 10;
+    }
+    double VD[8][8], VDi[8][8];
+    for (int r = 0; r < 8; r++) {
+      const int i = cell[0] + incI[r], j = cell[1] + incJ[r], k = cell[2] + incK[r];
+      double x1 = c->xc[i], x2 = c->yc[j], x3 = c->zc[k];
+      if (i == iF && j == jF && k == kF) { x1 = bi[0]; x2 = bi[1]; x3 = bi[2]; }
+      VD[r][0] = x1 * x2 * x3; VD[r][1] = x1 * x2; VD[r][2] = x1 * x3; VD[r][3] = x2 * x3;
+      VD[r][4] = x1; VD[r][5] = x2; VD[r][6] = x3; VD[r][7] = 1.0;
+    }
+    if (!invert8(VD, VDi)) return 3;
+    const double x = FC[0], y = FC[1], z = FC[2];
+    for (int r = 0; r < 8; r++)
+      c->fr_cD[8 * q + r] = VDi[0][r] * x * y * z + VDi[1][r] * x * y + VDi[2][r] * x * z + VDi[3][r] * y * z +
+                            VDi[4][r] * x + VDi[5][r] * y + VDi[6][r] * z + VDi[7][r];
+    c->fr_ic[q] = cell[0]; c->fr_jc[q] = cell[1]; c->fr_kc[q] = cell[2];
+  }
+  return 0;
+}
+
+void or_geom_get_fresh(const or_ctx *c, int32_t *ifr, int32_t *jfr, int32_t *kfr, int32_t *ic, int32_t *jc,
+                       int32_t *kc, double *coeffD, double *bi, int32_t *body, int32_t *elem) {
+  size_t n = (size_t)c->nfr;
+  if (ifr) memcpy(ifr, c->fr_i, 4 * n);
+  if (jfr) memcpy(jfr, c->fr_j, 4 * n);
+  if (kfr) memcpy(kfr, c->fr_k, 4 * n);
+  if (ic) memcpy(ic, c->fr_ic, 4 * n);
+  if (jc) memcpy(jc, c->fr_jc, 4 * n);
+  if (kc) memcpy(kc, c->fr_kc, 4 * n);
+  if (coeffD) memcpy(coeffD, c->fr_cD, 64 * n);
+  if (bi) memcpy(bi, c->fr_bi, 24 * n);
+  if (body) memcpy(body, c->fr_body, 4 * n);
+  if (elem) memcpy(elem, c->fr_elem, 4 * n);
 }
 
 void or_geom_get_dirichlet(const or_ctx *c, double *coeffD) {

@@ -32,6 +32,11 @@ struct or_ctx {
   double *coeff, *probe, *bi, *ipt;
   double *coeffD;                /* coeffGCMD(8,n), velocity (Dirichlet) set */
   double bi_weight, ip_weight;   /* bodyInterceptWeight, imagePointWeight */
+  /* fresh cells (GCM_set_freshcell, AMODULES.F90:560-570): iFresh..,
+   * iFreshCellIndex.., coeffGCMFreshD(8,n), body intercept, body, element */
+  int nfr;
+  int32_t *fr_i, *fr_j, *fr_k, *fr_ic, *fr_jc, *fr_kc, *fr_body, *fr_elem;
+  double *fr_cD, *fr_bi;
   /* bodies */
   int nbody;
   or_body body[64];
@@ -42,5 +47,7 @@ struct or_ctx {
   ((long)(i) + (c)->ex * ((long)(j) + (c)->ey * (long)(k)))
 
 void or_mark_ghosts(or_ctx *c);
+void or_free_fresh(or_ctx *c);
+void or_alloc_fresh(or_ctx *c, int n);
 
 #endif

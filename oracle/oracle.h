@@ -153,6 +153,25 @@ void or_geom_get_ghost_geometry(const or_ctx *c, int32_t *ic, int32_t *jc,
                                 double *bi, double *ip, int32_t *elem,
                                 int32_t *body);
 
+/* GCM_set_freshcell (GCM_SET_FRESHCELL.F90:1-262) for the fresh cells of
+ * the last or_geom_set_boundary (or or_set_fresh): the i-fastest list,
+ * image-point stencils and coeffGCMFreshD; returns 0 or 10 (image point
+ * outside the 2x2x2 stencil), 11 (intercept at the cell centre), 3
+ * (singular), 4/5 (intercept), 7 (no body).  or_geom_get_fresh reads the
+ * tables (n = or_nfresh_tables; any pointer may be NULL; body and element
+ * 1-based).  or_set_fresh_tables installs caller tables instead (BI, body
+ * and element left 0).  or_set_fresh and or_geom_set_boundary clear them. */
+int or_geom_set_freshcell(or_ctx *c, double probe_length_normalized);
+int or_nfresh_tables(const or_ctx *c);
+void or_geom_get_fresh(const or_ctx *c, int32_t *ifr, int32_t *jfr,
+                       int32_t *kfr, int32_t *ic, int32_t *jc, int32_t *kc,
+                       double *coeffD, double *bi, int32_t *body,
+                       int32_t *elem);
+void or_set_fresh_tables(or_ctx *c, int n, const int32_t *ifr,
+                         const int32_t *jfr, const int32_t *kfr,
+                         const int32_t *ic, const int32_t *jc,
+                         const int32_t *kc, const double *coeffD);
+
 /* coeffGCMD(8,n), column-major, of the last or_geom_set_boundary (the
  * velocity set of GCM_Calc_VanMatrixDN, GCM_SET_BOUNDARY.F90:307-494) */
 void or_geom_get_dirichlet(const or_ctx *c, double *coeffD);
@@ -172,11 +191,19 @@ void or_correct_vel(const or_ctx *c, double dt, const double *p, double *u, doub
                     const double *vbi, int itermax_ad, double restol_ad);
 /* update_pressure (CORE_FRAC_STEP_ROUTINES.F90:949-967): p += p' (van_kan)
  * or p = p', then GCM_p_set_bc_internal on p */
+/* face_vel (CORE_FRAC_STEP_ROUTINES.F90:6-240), GCM, non-periodic: face
+ * velocities from u, v, w (uTilde = u + dt grad p for VAN_KAN), outer faces
+ * from bcxu/bcyv/bczw, VAN_KAN face pressure correction and ghost-cell
+ * faces from u itself, solid faces zero; fu, fv, fw are written. */
+void or_face_vel(const or_ctx *c, int van_kan, double dt, const double *p, const double *u, const double *v,
+                 const double *w, const double *bcxu, const double *bcyv, const double *bczw, double *fu, double *fv,
+                 double *fw);
 void or_update_pressure(const or_ctx *c, int van_kan, const double *pprime, double *p, const double *g, int itermax,
                         double restol);
 
 /* ---- implicit momentum solve (oracle/or_ad.c) ------------------------ */
-/* set_solve_ad (CORE_SET_SOLVERS.F90:20-223) without fresh cells: coef =
+/* set_solve_ad (CORE_SET_SOLVERS.F90:20-223), fresh rows from the fresh
+ * tables (none without them): coef =
  * amx_ad, apx_ad, amy_ad, apy_ad, amz_ad, apz_ad (6 fields).  visc is
  * viscTot; bxv, byv, bzv the boundary-face viscosities bcx/y/zvisc. */
 void or_set_solve_ad(const or_ctx *c, double dt, const double *visc, const double *bxv, const double *byv,
@@ -195,6 +222,12 @@ void or_itsolv_ad(const or_ctx *c, double half_dt, const double *coef, const dou
 int or_solve_ad(const or_ctx *c, double half_dt, const double *coef, const double *fu, const double *fv,
                 const double *fw, double *u, double *v, double *w, const double *nlu, const double *nlv,
                 const double *nlw, const double *vbi, int itermax, double restol, double *maxres);
+/* solve_ad with CRANK_NICOLSON2 (CORE_SOLVE_AD.F90:218-247): face_vel after
+ * every ghost update (fu, fv, fw in/out); half_dt = dt/2. */
+int or_solve_ad_cn2(const or_ctx *c, double dt, const double *coef, double *fu, double *fv, double *fw, double *u,
+                    double *v, double *w, const double *nlu, const double *nlv, const double *nlw, const double *vbi,
+                    int van_kan, const double *p, const double *bcxu, const double *bcyv, const double *bczw,
+                    int itermax, double restol, double *maxres_out);
 
 #ifdef __cplusplus
 }

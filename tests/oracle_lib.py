@@ -73,6 +73,15 @@ def lib():
         L.or_vel_ghosts.argtypes = [P] * 5 + [C.c_int, C.c_double]
         L.or_correct_vel.argtypes = [P, C.c_double] + [P] * 11 + [C.c_int, C.c_double]
         L.or_update_pressure.argtypes = [P, C.c_int, P, P, P, C.c_int, C.c_double]
+        L.or_face_vel.argtypes = [P, C.c_int, C.c_double] + [P] * 10
+        L.or_solve_ad_cn2.restype = C.c_int
+        L.or_solve_ad_cn2.argtypes = [P, C.c_double] + [P] * 11 + [C.c_int] + [P] * 4 + [C.c_int, C.c_double, P]
+        L.or_geom_set_freshcell.restype = C.c_int
+        L.or_geom_set_freshcell.argtypes = [P, C.c_double]
+        L.or_nfresh_tables.restype = C.c_int
+        L.or_nfresh_tables.argtypes = [P]
+        L.or_geom_get_fresh.argtypes = [P] * 11
+        L.or_set_fresh_tables.argtypes = [P, C.c_int] + [P] * 7
         L.or_set_solve_ad.argtypes = [P, C.c_double] + [P] * 5
         L.or_calc_residual_ad.restype = C.c_double
         L.or_calc_residual_ad.argtypes = [P, C.c_double] + [P] * 8
@@ -205,6 +214,15 @@ class Oracle:
                                  None if gg is None else gg.ctypes.data, itermax, restol)
         return out
 
+    def face_vel(self, van_kan, dt, p, u, v, w, bcxu, bcyv, bczw, faces=None):
+        """face_vel; returns (face_u, face_v, face_w), written over copies of
+        `faces` (zeros when None)"""
+        ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (p, u, v, w, bcxu, bcyv, bczw)]
+        out = [np.zeros(self.shape) if faces is None else np.array(f, dtype=np.float64, order="C", copy=True)
+               for f in (faces or (None, None, None))]
+        lib().or_face_vel(self.h, int(van_kan), float(dt), *[a.ctypes.data for a in ins], *[a.ctypes.data for a in out])
+        return out
+
     # ---- implicit momentum solve (or_ad.c) ----
     def set_solve_ad(self, dt, visc, bxv, byv, bzv):
         """amx_ad, apx_ad, amy_ad, apy_ad, amz_ad, apz_ad as a (6, ...) array"""
@@ -252,9 +270,47 @@ class Oracle:
                                C.addressof(mr))
         return out[0], out[1], out[2], it, mr.value
 
+    def solve_ad_cn2(self, dt, coef, faces, u, v, w, nlu, nlv, nlw, van_kan, p, bcxu, bcyv, bczw, vbi=None,
+                     itermax=1000, restol=1e-12):
+        """solve_ad with CRANK_NICOLSON2 on copies; returns (u, v, w, faces, iterations, maxres)"""
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        f = [np.array(a, dtype=np.float64, order="C", copy=True) for a in faces]
+        out = [np.array(a, dtype=np.float64, order="C", copy=True) for a in (u, v, w)]
+        rs = [np.ascontiguousarray(a, dtype=np.float64) for a in (nlu, nlv, nlw)]
+        ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (p, bcxu, bcyv, bczw)]
+        vb = None if vbi is None else np.ascontiguousarray(vbi, dtype=np.float64)
+        mr = C.c_double()
+        it = lib().or_solve_ad_cn2(self.h, float(dt), coef.ctypes.data, *[a.ctypes.data for a in f],
+                                   *[a.ctypes.data for a in out], *[a.ctypes.data for a in rs],
+                                   None if vb is None else vb.ctypes.data, int(van_kan),
+                                   *[a.ctypes.data for a in ins], int(itermax), float(restol), C.addressof(mr))
+        return out[0], out[1], out[2], f, it, mr.value
+
     def set_fresh(self, fr):
         fr = np.ascontiguousarray(fr, dtype=np.int8)
         lib().or_set_fresh(self.h, fr.ctypes.data)
+
+    FRESH_KEYS = ("i", "j", "k", "ic", "jc", "kc", "coeffD", "bi", "body", "elem")
+
+    def set_freshcell(self, probe_norm=2.0):
+        """GCM_set_freshcell for the current fresh mask; returns (rc, tables)"""
+        rc = lib().or_geom_set_freshcell(self.h, float(probe_norm))
+        return rc, (self.fresh_tables() if rc == 0 else None)
+
+    def fresh_tables(self):
+        n = lib().or_nfresh_tables(self.h)
+        m = max(n, 1)
+        t = {k: np.zeros(m, np.int32) for k in ("i", "j", "k", "ic", "jc", "kc", "body", "elem")}
+        t["coeffD"] = np.zeros((m, 8))
+        t["bi"] = np.zeros((m, 3))
+        lib().or_geom_get_fresh(self.h, *[t[k].ctypes.data for k in self.FRESH_KEYS])
+        return {k: v[:n] for k, v in t.items()}
+
+    def set_fresh_tables(self, t):
+        a = {k: np.ascontiguousarray(t[k], dtype=np.int32) for k in ("i", "j", "k", "ic", "jc", "kc")}
+        cd = np.ascontiguousarray(t["coeffD"], dtype=np.float64)
+        lib().or_set_fresh_tables(self.h, len(a["i"]), *[a[k].ctypes.data for k in ("i", "j", "k", "ic", "jc", "kc")],
+                                  cd.ctypes.data)
 
     def calc_residual(self, var, r):
         var = np.ascontiguousarray(var, dtype=np.float64)

@@ -135,17 +135,18 @@ def test_solve_ad_iteration_limit_and_errors():
     assert e.value.code == pp.PP_NO_GEOMETRY
 
 
-def test_solve_ad_refuses_fresh_cells():
-    """A geometry built with iblank_prev has fresh cells; their momentum rows
-    (GCM_set_freshcell, GCM_correct_rhs_ad/res_ad) are not built, so every
-    momentum entry point refuses it instead of solving with the ordinary
-    rows; the same geometry without the fresh mask is accepted."""
-    announce("solve_ad fresh-cell refusal 24^3 moving sphere")
+def test_fresh_cells_need_their_tables():
+    """A geometry built with iblank_prev has fresh cells: the momentum entry
+    points need GCM_set_freshcell's tables (pp_set_fresh_cells) before
+    pp_set_solve_ad, refuse tables that do not match the fresh mask, and a
+    new geometry drops both."""
+    announce("solve_ad fresh-cell tables required 24^3 moving sphere")
     xn, yn, zn = (cases.uniform(24),) * 3
     ib0, _, _, _ = pp.build_geometry(xn, yn, zn, [pp.Ellipsoid((0.45, 0.5, 0.5), (0.2, 0.2, 0.2), 0.0, 48)])
     ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, [pp.Ellipsoid((0.55, 0.5, 0.5), (0.2, 0.2, 0.2), 0.0, 48)],
                                        iblank_prev=ib0)
-    assert fr.sum() > 0
+    assert fr.sum() > 0 and ex["fresh"] is not None
+    t = ex["fresh"]
     s = pp.PressureSolver(xn, yn, zn)
     s.set_geometry(ib, gs, fresh=fr)
     s.set_vel_ghosts(ex["coeffD"])
@@ -153,19 +154,33 @@ def test_solve_ad_refuses_fresh_cells():
     one = np.full(s.shape, 0.01)
     with pytest.raises(pp.PPError) as e:
         s.set_solve_ad(0.1, one, one, one, one)
-    assert e.value.code == pp.PP_BAD_ARG and "fresh cells" in str(e.value)
-    s.set_geometry(ib, gs)
-    s.set_vel_ghosts(ex["coeffD"])
+    assert e.value.code == pp.PP_NO_GEOMETRY and "pp_set_fresh_cells" in str(e.value)
+    # tables that do not match the mask
+    short = {k: v[1:] for k, v in t.items()}
+    with pytest.raises(pp.PPError) as e:
+        s.set_fresh_cells(short)
+    assert e.value.code == pp.PP_BAD_ARG
+    bad = {k: v.copy() for k, v in t.items()}
+    bad["ic"][0] = bad["i"][0] + 1          # the corner must be the cell or the one below
+    with pytest.raises(pp.PPError) as e:
+        s.set_fresh_cells(bad)
+    assert e.value.code == pp.PP_BAD_ARG and "corner" in str(e.value)
+    s.set_fresh_cells(t)
     s.set_solve_ad(0.1, one, one, one, one)
     u, v, w, st = s.solve_ad(AB2, z, z, z, one, one, one, z, z, z, itermax_ad=50, restol_ad=1e-10)
     assert st["status"] == "PP_OK"
-    # coefficients set, then a geometry with fresh cells: solve and residual refuse
+    # tables set again: the coefficients must be set again
+    s.set_fresh_cells(t)
+    with pytest.raises(pp.PPError) as e:
+        s.solve_ad(AB2, z, z, z, one, one, one, z, z, z)
+    assert e.value.code == pp.PP_NO_GEOMETRY
+    # a new geometry with fresh cells drops the tables and the coefficients
     s.set_geometry(ib, gs, fresh=fr)
     for call in (lambda: s.solve_ad(AB2, z, z, z, one, one, one, z, z, z),
                  lambda: s.calc_residual_ad(AB2, z, z, z, z, z)):
         with pytest.raises(pp.PPError) as e:
             call()
-        assert e.value.code in (pp.PP_BAD_ARG, pp.PP_NO_GEOMETRY)
+        assert e.value.code == pp.PP_NO_GEOMETRY
 
 
 def test_solve_ad_converges_on_a_stretched_64x48x40_grid():

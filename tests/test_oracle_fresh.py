@@ -106,3 +106,27 @@ def test_fresh_fill_brute_force(seed):
     # every other cell holds the snapshot
     other = fr != 1
     assert np.array_equal(q[other], snap[other])
+
+
+def test_fresh_cell_without_fluid_neighbours_keeps_its_value():
+    """A fresh cell whose 26 neighbours are all blank or fresh (a body that
+    moved more than a cell in one step) has nothing to average; the
+    reference divides by zero there, the oracle and the library keep the
+    mean-removed value instead."""
+    n = 10
+    xn = np.linspace(0.0, 1.0, n + 1)
+    o = Oracle(xn, xn, xn)
+    ib = np.zeros(o.shape, np.int8)
+    fr = np.zeros(o.shape, np.int8)
+    fr[4:7, 4:7, 4:7] = 1                  # a 3x3x3 block of fresh cells
+    o.set_iblank(ib)
+    o.set_fresh(fr)
+    rng = np.random.default_rng(6)
+    p = rng.standard_normal(o.shape)
+    q = p.copy()
+    m = o.mean_remove(q)
+    assert np.isfinite(q).all()
+    assert q[5, 5, 5] == pytest.approx(p[5, 5, 5] - m, rel=1e-13, abs=1e-14)   # the enclosed cell
+    # a cell on the block's face still averages its fluid neighbours
+    vals = [p[k, j, i] - m for k in (3, 4, 5) for j in (3, 4, 5) for i in (3, 4, 5) if fr[k, j, i] != 1]
+    assert q[4, 4, 4] == pytest.approx(sum(vals) / len(vals), rel=1e-12)
