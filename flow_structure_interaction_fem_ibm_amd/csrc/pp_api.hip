@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <unordered_map>
 
 #include <rocblas/rocblas.h>
@@ -22,6 +23,42 @@ using namespace pp;
       return PP_HIP_ERROR;                                                        \
     }                                                                             \
   } while (0)
+
+// ---------------------------------------------------------------------
+// Scratch memory of the allocation-free entry point (pp_divergence_noalloc):
+// one array per GPU for the whole process, holding the per-workgroup
+// partial sums of the reduction and its two later stages.  Host code never
+// uses the array's name as an address: pp_create looks up its device
+// address once per device (hipGetSymbolAddress with the device current) and
+// the entry point passes that address to the kernels.
+// ---------------------------------------------------------------------
+constexpr long kNoAllocPartials = 1L << 19;   // workgroups of the first stage
+constexpr long kNoAllocDoubles = kNoAllocPartials + 256 + 64;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device (once per device)
+int scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device %d: at most %d devices", dev, kMaxDevices);
+    return PP_BAD_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return PP_OK;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+    return PP_HIP_ERROR;
+  }
+  g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  return PP_OK;
+}
+}  // namespace
 
 namespace {
 
@@ -1352,6 +1389,10 @@ int32_t pp_create(pp_ctx **out, const pp_grid_desc *g) {
   }
   if (g->device < 0 || g->device >= ndev) { set_error("pp_create: bad device %d", g->device); return PP_BAD_ARG; }
   HIPCHK(hipSetDevice(g->device));
+  {
+    int rs = scratch_setup(g->device);
+    if (rs) return rs;
+  }
   pp_ctx *c = new pp_ctx();
   c->device = g->device;
   c->nxc = g->nxc; c->nyc = g->nyc; c->nzc = g->nzc;
@@ -2107,6 +2148,39 @@ int32_t pp_tdma_batched(pp_ctx *c, const double *a, const double *b, const doubl
       for (int m = 0; m < n; m++) u[(size_t)l * n + m] = tmp[(size_t)m * nl + l];
   } else {
     HIPCHK(hipMemcpy(u, d[4], nb, hipMemcpyDeviceToHost));
+  }
+  return PP_OK;
+}
+
+int32_t pp_divergence_noalloc(pp_ctx *c, const double *d_fu, const double *d_fv, const double *d_fw, double dtinv,
+                              double *d_nlu0, double *sum) {
+  if (!c || !d_fu || !d_fv || !d_fw || !d_nlu0) { set_error("pp_divergence_noalloc: null argument"); return PP_BAD_ARG; }
+  if (!c->have_geom) { set_error("pp_divergence_noalloc: no geometry (iblank)"); return PP_NO_GEOMETRY; }
+  if (c->nblk_red > kNoAllocPartials) {
+    set_error("pp_divergence_noalloc: %dx%dx%d cells need %d partial sums; the scratch array holds %ld",
+              c->nxc, c->nyc, c->nzc, c->nblk_red, kNoAllocPartials);
+    return PP_BAD_ARG;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  double *ptr = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+  if (!ptr) { set_error("pp_divergence_noalloc: no scratch address for device %d", c->device); return PP_HIP_ERROR; }
+  double *part = ptr, *part2 = ptr + kNoAllocPartials, *res = ptr + kNoAllocPartials + 256;
+  hipStream_t s = c->stream;
+  // the output's outer layer and blank cells are 0, as pp_divergence leaves
+  // them (a fill kernel, not a memset, so the call can be captured)
+  k_fill<double><<<(unsigned)((c->nref + 255) / 256), 256, 0, s>>>(d_nlu0, c->nref, 0.0);
+  CellLaunch cl = ref_launch(c);
+  k_divergence<256><<<cl.grid, cl.block, 0, s>>>(c->ref, c->d_rdx[0], c->d_rdx[1], c->d_rdx[2], c->d_rdxinv[0],
+                                                 c->d_rdxinv[1], c->d_rdxinv[2], c->d_iblank_ref, d_fu, d_fv, d_fw,
+                                                 dtinv, d_nlu0, part);
+  const int nblk = c->nblk_red, chunk = (nblk + kRedStage - 1) / kRedStage;
+  k_reduce_sums_stage<<<kRedStage, 256, 0, s>>>(part, nblk, 1, chunk, part2);
+  k_reduce_sums<<<1, 256, 0, s>>>(part2, kRedStage, 1, res);
+  HIPCHK(hipGetLastError());
+  if (sum) {
+    HIPCHK(hipMemcpyAsync(c->h_res, res, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *sum = c->h_res[0];
   }
   return PP_OK;
 }

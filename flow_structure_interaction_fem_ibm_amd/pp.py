@@ -160,6 +160,7 @@ def lib():
     L.pp_time_kernel.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp]
     L.pp_setup_info.argtypes = [P, dp, C.POINTER(C.c_int32), dp]
     L.pp_level_apply.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_double, P, P, P]
+    L.pp_divergence_noalloc.argtypes = [P, P, P, P, C.c_double, P, dp]
     L.pp_level_data.argtypes = [P, C.c_int32, P, P, P, P, P]
     L.pp_geom_build.argtypes = [C.POINTER(C.c_void_p), C.POINTER(GridDesc), C.c_int32,
                                 C.POINTER(BodyDesc), C.c_int32, C.c_double, P]
@@ -195,7 +196,7 @@ def lib():
                  "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device", "pp_adjoint_device",
                  "pp_set_solve_ad", "pp_set_solve_ad_device", "pp_solve_ad", "pp_solve_ad_device",
                  "pp_calc_residual_ad", "pp_geom_build_surfaces", "pp_geom_canonical_surface",
-                 "pp_level_apply", "pp_level_data", "pp_geom_fresh", "pp_set_fresh_cells",
+                 "pp_level_apply", "pp_level_data", "pp_divergence_noalloc", "pp_geom_fresh", "pp_set_fresh_cells",
                  "pp_face_vel", "pp_face_vel_device", "pp_solve_ad_cn2", "pp_solve_ad_cn2_device"):
         getattr(L, name).restype = C.c_int32
     _lib = L
@@ -210,7 +211,7 @@ EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_s
             "pp_correct_vel", "pp_correct_vel_device", "pp_update_pressure", "pp_update_pressure_device",
             "pp_adjoint_device", "pp_set_solve_ad", "pp_set_solve_ad_device", "pp_solve_ad", "pp_solve_ad_device",
             "pp_calc_residual_ad", "pp_geom_build_surfaces", "pp_geom_canonical_surface", "pp_level_apply",
-            "pp_level_data", "pp_geom_fresh", "pp_set_fresh_cells", "pp_face_vel",
+            "pp_level_data", "pp_divergence_noalloc", "pp_geom_fresh", "pp_set_fresh_cells", "pp_face_vel",
             "pp_face_vel_device", "pp_solve_ad_cn2", "pp_solve_ad_cn2_device")
 
 
@@ -499,6 +500,15 @@ class PressureSolver:
         _check(lib().pp_divergence(self.h, f[0].ctypes.data, f[1].ctypes.data, f[2].ctypes.data, float(dtinv),
                                    out.ctypes.data, C.byref(s)))
         return out, s.value
+
+    def divergence_noalloc(self, face_u, face_v, face_w, nlu0, dtinv=1.0, want_sum=True):
+        """pp_divergence_noalloc on torch float64 device tensors (nlu0 written
+        in place); returns sum(nlu0*dV), or None with want_sum=False (the call
+        then returns without synchronising)."""
+        s = C.c_double()
+        _check(lib().pp_divergence_noalloc(self.h, face_u.data_ptr(), face_v.data_ptr(), face_w.data_ptr(),
+                                           float(dtinv), nlu0.data_ptr(), C.byref(s) if want_sum else None))
+        return s.value if want_sum else None
 
     def mean_remove(self, p):
         m = C.c_double()

@@ -207,6 +207,20 @@ int32_t pp_mean_remove(pp_ctx *ctx, double *p, double *mean);
 int32_t pp_divergence(pp_ctx *ctx, const double *face_u, const double *face_v,
                       const double *face_w, double dtinv, double *nlu0,
                       double *sum);
+/* pp_divergence on device arrays without allocating: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight time-step loops, harnesses that forbid hipMalloc).
+ * face_u/v/w and nlu0 are device arrays in the reference layout (nlu0's
+ * outer layer and blank cells are set to 0).  The reduction's partial sums
+ * live in one static device array per GPU (about 4 MB, held until the
+ * process ends): grids whose cell launch needs more than 524288 workgroups
+ * (beyond about 512x1024x1024 cells) return PP_BAD_ARG.  Calls that overlap
+ * on one GPU (any contexts) race on that array.  sum may be NULL: then the
+ * call only enqueues work on the context's stream and returns without
+ * synchronising; otherwise it waits and writes sum(nlu0*dV). */
+int32_t pp_divergence_noalloc(pp_ctx *ctx, const double *d_face_u,
+                              const double *d_face_v, const double *d_face_w,
+                              double dtinv, double *d_nlu0, double *sum);
 /* `sweeps` smoothing sweeps of the finest-level operator on var (inout)
  * with right-hand side r, FP64, host arrays in the reference layout; the
  * ghost rows are not updated (the caller's GCM step does that), as in the
