@@ -328,6 +328,56 @@ module pp_hip_mod
        integer(c_int32_t) :: rc
      end function pp_solve_ad_device
 
+     ! solve_ad with CRANK_NICOLSON2: face_vel after every ghost update; face_u/v/w in/out;
+     ! van_kan = 1 for frac_step_type VAN_KAN (p is then read), bcxu/bcyv/bczw the outer faces
+     function pp_solve_ad_cn2(ctx, u, v, w, nlu, nlv, nlw, face_u, face_v, face_w, van_kan, p, bcxu, bcyv, &
+          bczw, vel_bi, itermax_ad, restol_ad, st) bind(c, name="pp_solve_ad_cn2") result(rc)
+       import :: c_ptr, c_int32_t, c_double, pp_ad_stats
+       type(c_ptr), value :: ctx, u, v, w, nlu, nlv, nlw, face_u, face_v, face_w, p, bcxu, bcyv, bczw, vel_bi
+       integer(c_int32_t), value :: van_kan, itermax_ad
+       real(c_double), value :: restol_ad
+       type(pp_ad_stats), intent(out) :: st
+       integer(c_int32_t) :: rc
+     end function pp_solve_ad_cn2
+
+     function pp_solve_ad_cn2_device(ctx, u, v, w, nlu, nlv, nlw, face_u, face_v, face_w, van_kan, p, bcxu, &
+          bcyv, bczw, vel_bi, itermax_ad, restol_ad, st) bind(c, name="pp_solve_ad_cn2_device") result(rc)
+       import :: c_ptr, c_int32_t, c_double, pp_ad_stats
+       type(c_ptr), value :: ctx, u, v, w, nlu, nlv, nlw, face_u, face_v, face_w, p, bcxu, bcyv, bczw, vel_bi
+       integer(c_int32_t), value :: van_kan, itermax_ad
+       real(c_double), value :: restol_ad
+       type(pp_ad_stats), intent(out) :: st
+       integer(c_int32_t) :: rc
+     end function pp_solve_ad_cn2_device
+
+     ! face_vel: face_u/v/w from u, v, w (uTilde with p for VAN_KAN), reference layout
+     function pp_face_vel(ctx, van_kan, dt, p, u, v, w, bcxu, bcyv, bczw, face_u, face_v, face_w) &
+          bind(c, name="pp_face_vel") result(rc)
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, p, u, v, w, bcxu, bcyv, bczw, face_u, face_v, face_w
+       integer(c_int32_t), value :: van_kan
+       real(c_double), value :: dt
+       integer(c_int32_t) :: rc
+     end function pp_face_vel
+
+     function pp_face_vel_device(ctx, van_kan, dt, p, u, v, w, bcxu, bcyv, bczw, face_u, face_v, face_w) &
+          bind(c, name="pp_face_vel_device") result(rc)
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, p, u, v, w, bcxu, bcyv, bczw, face_u, face_v, face_w
+       integer(c_int32_t), value :: van_kan
+       real(c_double), value :: dt
+       integer(c_int32_t) :: rc
+     end function pp_face_vel_device
+
+     ! fresh cells of the momentum solve (GCM_set_freshcell's tables, as pp_geom_fresh gives them)
+     function pp_set_fresh_cells(ctx, n_fresh, i_fresh, j_fresh, k_fresh, i_cell, j_cell, k_cell, coeffD) &
+          bind(c, name="pp_set_fresh_cells") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx, i_fresh, j_fresh, k_fresh, i_cell, j_cell, k_cell, coeffD
+       integer(c_int32_t), value :: n_fresh
+       integer(c_int32_t) :: rc
+     end function pp_set_fresh_cells
+
      ! calc_residual_ad of one field: resm, loc (1-based), res (or C_NULL_PTR)
      function pp_calc_residual_ad(ctx, advec_scheme, var, r, face_u, face_v, face_w, res, resm, loc) &
           bind(c, name="pp_calc_residual_ad") result(rc)
@@ -360,6 +410,15 @@ module pp_hip_mod
        type(c_ptr), value :: xyz, tri
        integer(c_int32_t) :: rc
      end function pp_geom_canonical_surface
+
+     ! GCM_set_freshcell for the fresh cells of a geometry built with iblank_prev (n_fresh of
+     ! pp_geom_counts): cells, stencil corners, coeffGCMFreshD(8, n), intercepts (3, n), body, element
+     function pp_geom_fresh(geo, i_fresh, j_fresh, k_fresh, i_cell, j_cell, k_cell, coeffD, bi, body, elem) &
+          bind(c, name="pp_geom_fresh") result(rc)
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: geo, i_fresh, j_fresh, k_fresh, i_cell, j_cell, k_cell, coeffD, bi, body, elem
+       integer(c_int32_t) :: rc
+     end function pp_geom_fresh
 
      function pp_geom_destroy(geo) bind(c, name="pp_geom_destroy") result(rc)
        import :: c_ptr, c_int32_t

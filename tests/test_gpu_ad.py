@@ -143,10 +143,16 @@ def test_fresh_cells_need_their_tables():
     announce("solve_ad fresh-cell tables required 24^3 moving sphere")
     xn, yn, zn = (cases.uniform(24),) * 3
     ib0, _, _, _ = pp.build_geometry(xn, yn, zn, [pp.Ellipsoid((0.45, 0.5, 0.5), (0.2, 0.2, 0.2), 0.0, 48)])
-    ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, [pp.Ellipsoid((0.55, 0.5, 0.5), (0.2, 0.2, 0.2), 0.0, 48)],
+    # half a cell (a time step's move): a move of several cells leaves fresh
+    # cells whose image point is outside their stencil (pp_geom_fresh refuses
+    # those, where the reference stops)
+    ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, [pp.Ellipsoid((0.47, 0.5, 0.5), (0.2, 0.2, 0.2), 0.0, 48)],
                                        iblank_prev=ib0)
-    assert fr.sum() > 0 and ex["fresh"] is not None
+    assert fr.sum() > 0 and ex["fresh"] is not None, ex["fresh_error"]
     t = ex["fresh"]
+    _, _, _, ex_far = pp.build_geometry(xn, yn, zn, [pp.Ellipsoid((0.55, 0.5, 0.5), (0.2, 0.2, 0.2), 0.0, 48)],
+                                        iblank_prev=ib0)
+    assert ex_far["fresh"] is None and "stencil" in ex_far["fresh_error"]
     s = pp.PressureSolver(xn, yn, zn)
     s.set_geometry(ib, gs, fresh=fr)
     s.set_vel_ghosts(ex["coeffD"])
