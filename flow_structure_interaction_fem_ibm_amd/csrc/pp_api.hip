@@ -476,6 +476,13 @@ void restrict_step(pp_ctx *c, int l) {
   }
 }
 
+template <typename T>
+const T *nshape(pp_ctx *c);
+template <>
+const float *nshape<float>(pp_ctx *c) { return c->d_nshape32; }
+template <>
+const double *nshape<double>(pp_ctx *c) { return c->d_nshape64; }
+
 // the correction of level l + 1 prolongated onto level l and added to its
 // correction on fluid cells
 template <typename T>
@@ -484,15 +491,18 @@ void prolong_step(pp_ctx *c, int l) {
   hipStream_t st = c->stream;
   const double cells = (double)L.cx * L.cy * L.cz;
   LevelArgs<T> a = largs<T>(L), ac = largs<T>(C);
+  // adjoint: the level-0 correction takes the shape of the null space
+  // (w/V), which the coarse levels, whose null space is the constants, miss
+  const T *scale = (c->adjoint_mode && l == 0 && c->have_nshape) ? nshape<T>(c) : nullptr;
   kbegin(c, PP_K_PROLONG, l, cells * (2.0 * sizeof(T) + 1.0) + (sizeof(T) + 1.0) * C.cx * C.cy * C.cz);
   if (halves_by2(L, C)) {
     CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
     k_prolong2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, pdir<T>(L, 0).wpar, pdir<T>(L, 1).wpar, pdir<T>(L, 2).wpar,
-                                                 lv_e<T>(C), lv_e<T>(L));
+                                                 lv_e<T>(C), lv_e<T>(L), scale);
   } else {
     CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
     k_prolong<T><<<cl.grid, cl.block, 0, st>>>(a, ac, pdir<T>(L, 0), pdir<T>(L, 1), pdir<T>(L, 2), lv_e<T>(C),
-                                                lv_e<T>(L));
+                                                lv_e<T>(L), scale);
   }
   kend(c);
 }
@@ -1284,6 +1294,9 @@ int compute_null_vector(pp_ctx *c) {
     // number of cycles and a loose floor.
     const bool exact = c->gh.solid_in_stencil == 0;
     o.max_outer = exact ? 200 : 40;
+    // a warm start also takes the previous null vector's shape for the
+    // transposed cycles' coarse correction
+    c->have_nshape = warm && c->d_nshape32 != nullptr;
     const double accept = exact ? 1e-10 : 1e-4;
     pp_solve_stats st;
     int lam = 0;
@@ -1309,6 +1322,17 @@ int compute_null_vector(pp_ctx *c) {
                                                                  c->d_w);
     c->null_iters = st.outer_iterations;
     c->null_res = st.rel_l2;
+    // the null space shape w/V for the adjoint V-cycle's coarse correction
+    Level &L0 = c->lv[0];
+    if (!c->d_nshape32) {
+      HIPCHK(hipMalloc(&c->d_nshape32, sizeof(float) * L0.total));
+      HIPCHK(hipMalloc(&c->d_nshape64, sizeof(double) * L0.total));
+      HIPCHK(hipMemsetAsync(c->d_nshape32, 0, sizeof(float) * L0.total, s));
+      HIPCHK(hipMemsetAsync(c->d_nshape64, 0, sizeof(double) * L0.total, s));
+    }
+    k_null_shape<float><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_w, c->d_nshape32);
+    k_null_shape<double><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_w, c->d_nshape64);
+    c->have_nshape = true;
   } else {
     // no ghost rows: the operator is volume-symmetric, w = V
     HIPCHK(hipMemsetAsync(c->d_mu, 0, sizeof(double) * c->nref, s));
@@ -1460,6 +1484,7 @@ int32_t pp_destroy(pp_ctx *c) {
   hipFree(c->d_p); hipFree(c->d_b); hipFree(c->d_w); hipFree(c->d_bwork); hipFree(c->d_tmp);
   hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_mu); hipFree(c->d_sv); hipFree(c->d_mu_prev);
   hipFree(c->d_ad_coef);
+  hipFree(c->d_nshape32); hipFree(c->d_nshape64);
   free_fresh_rows(c);
   hipFree(c->d_part); hipFree(c->d_part2); hipFree(c->d_res);
   for (double *q : c->d_kry) hipFree(q);
@@ -1485,6 +1510,7 @@ int32_t pp_set_geometry(pp_ctx *c, const pp_geometry *geo) {
   HIPCHK(hipSetDevice(c->device));
   c->have_geom = false;
   c->have_ad = false;  // set_solve_ad follows every set_boundary
+  c->have_nshape = false;
   free_fresh_rows(c);
   if (c->gexec) {  // captured launches point at the old hierarchy
     HIPCHK(hipGraphExecDestroy(c->gexec));

@@ -189,6 +189,11 @@ struct pp_ctx {
   bool adjoint_mode = false;
   double setup_ms[4] = {};             // last pp_set_geometry: flags, ghost tables, coarse operator, null vector
   double *d_mu_prev = nullptr;         // last null vector in mu = w/V form (warm start of the next one)
+  // w/V of the null vector in the level-0 layout (FP32, FP64): the adjoint
+  // V-cycle scales its level-0 coarse-grid correction by it
+  float *d_nshape32 = nullptr;
+  double *d_nshape64 = nullptr;
+  bool have_nshape = false;
   // implicit momentum solve: amx_ad..apz_ad (6 fields, reference layout)
   // of the last pp_set_solve_ad, and its dt
   double *d_ad_coef = nullptr;

@@ -750,6 +750,19 @@ __global__ void __launch_bounds__(256) k_null_init(RefArgs R, LevelArgs<double> 
   mu[r] = (prev && prev[r] != 0.0) ? prev[r] : 1.0;
 }
 
+// shape of the adjoint's null space on level 0: s = w / V on fluid cells (w
+// the left null vector of the operator, V the cell volume; s = 1 on the
+// open domain far from bodies), 1 on blank cells, in the level layout
+template <typename T>
+__global__ void __launch_bounds__(256) k_null_shape(RefArgs R, LevelArgs<double> L, const double *__restrict__ w,
+                                                    T *__restrict__ s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= R.nxc || j >= R.nyc) return;
+  const long a = lidx(L.sx, L.sxy, i, j, k);
+  s[a] = (L.fl[a] & FL_BLANK) ? T(1) : (T)(w[ridx(R, i, j, k)] / (L.wx[i] * L.wy[j] * L.wz[k]));
+}
+
 __global__ void __launch_bounds__(256) k_null_scale(RefArgs R, LevelArgs<double> L, const double *__restrict__ mu,
                                                     double scale, double *__restrict__ w) {
 #pragma clang fp contract(off)
@@ -1224,7 +1237,8 @@ __global__ void __launch_bounds__(256) k_resrestrict2(LevelArgs<T> F, LevelArgs<
 // ---------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_prolong(LevelArgs<T> F, LevelArgs<T> C, ProlongDir<T> px, ProlongDir<T> py,
-                                                 ProlongDir<T> pz, const T *__restrict__ e_c, T *__restrict__ e_f) {
+                                                 ProlongDir<T> pz, const T *__restrict__ e_c, T *__restrict__ e_f,
+                                                 const T *__restrict__ scale) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
   if (i >= F.cx || j >= F.cy) return;
@@ -1250,18 +1264,23 @@ __global__ void __launch_bounds__(256) k_prolong(LevelArgs<T> F, LevelArgs<T> C,
         num += w * e_c[q];
         den += w;
       }
-  if (den > T(0)) e_f[id] = e_f[id] + num / den;
+  if (den > T(0)) {
+    const T corr = num / den;
+    e_f[id] = e_f[id] + (scale ? scale[id] * corr : corr);
+  }
 }
 
 // Prolongation + correction when every direction is coarsened by 2: one
 // thread per coarse cell (I,J,K) updates its 2x2x2 children, reading the
 // 3x3x3 coarse neighbourhood once (the children's interpolation partners
 // are I-1 / I+1 etc.; a partner outside the level carries weight 0).
-// Same weights, masks and renormalisation as k_prolong.
+// Same weights, masks and renormalisation as k_prolong.  scale (fine level
+// layout, may be NULL) multiplies each cell's correction.
 template <typename T>
 __global__ void __launch_bounds__(256) k_prolong2(LevelArgs<T> F, LevelArgs<T> C, const T *__restrict__ wx,
                                                   const T *__restrict__ wy, const T *__restrict__ wz,
-                                                  const T *__restrict__ e_c, T *__restrict__ e_f) {
+                                                  const T *__restrict__ e_c, T *__restrict__ e_f,
+                                                  const T *__restrict__ scale) {
   using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
@@ -1309,7 +1328,10 @@ __global__ void __launch_bounds__(256) k_prolong2(LevelArgs<T> F, LevelArgs<T> C
               num += w * ev[ck[z]][bj[y]][ai[x]];
               den += w;
             }
-        if (den > T(0)) out[ic] = out[ic] + num / den;
+        if (den > T(0)) {
+          const T corr = num / den;
+          out[ic] = out[ic] + (scale ? scale[id + ic] * corr : corr);
+        }
       }
       ef.x = out[0];
       ef.y = out[1];
