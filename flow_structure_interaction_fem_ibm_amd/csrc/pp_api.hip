@@ -1262,6 +1262,38 @@ int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref
 // Left null vector of the ghost-cell operator: the transposed iteration
 // with s = 0 started from mu = 1 (the fluid volumes, exact without ghost
 // rows) converges to w; normalised so sum_f w_f = sum_f V_f.
+// the level-0 shape arrays w/V from c->d_w
+int null_shape_arrays(pp_ctx *c) {
+  hipStream_t s = c->stream;
+  Level &L0 = c->lv[0];
+  CellLaunch cl = ref_launch(c);
+  LevelArgs<double> a0 = largs<double>(L0);
+  if (!c->d_nshape32) {
+    HIPCHK(hipMalloc(&c->d_nshape32, sizeof(float) * L0.total));
+    HIPCHK(hipMalloc(&c->d_nshape64, sizeof(double) * L0.total));
+    HIPCHK(hipMemsetAsync(c->d_nshape32, 0, sizeof(float) * L0.total, s));
+    HIPCHK(hipMemsetAsync(c->d_nshape64, 0, sizeof(double) * L0.total, s));
+  }
+  k_null_shape<float><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_w, c->d_nshape32);
+  k_null_shape<double><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_w, c->d_nshape64);
+  HIPCHK(hipGetLastError());
+  return PP_OK;
+}
+
+// the shape arrays of the current null-vector iterate mu (w = V mu scaled
+// to sum_f w = vsum, written to c->d_w)
+int null_shape_from_mu(pp_ctx *c, double vsum) {
+  hipStream_t s = c->stream;
+  LevelArgs<double> a0 = largs<double>(c->lv[0]);
+  CellLaunch cl = ref_launch(c);
+  k_mu_vol_sums<256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, nullptr, c->d_part);
+  int rc = reduce_sums_host(c, 2);
+  if (rc) return rc;
+  if (!(c->h_res[0] > 0.0)) return PP_OK;   // no usable iterate: keep the unit shape
+  k_null_scale<<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, vsum / c->h_res[0], c->d_w);
+  return null_shape_arrays(c);
+}
+
 int compute_null_vector(pp_ctx *c) {
   hipStream_t s = c->stream;
   LevelArgs<double> a0 = largs<double>(c->lv[0]);
@@ -1283,7 +1315,7 @@ int compute_null_vector(pp_ctx *c) {
     pp_default_opts(&o);
     o.precision = PP_PREC_MIXED;
     o.rel_tol = 1e-11;
-    o.omega = 1.0;   // the transposed cycles converge faster without over-relaxation
+
     // When every ghost stencil reads fluid or ghost cells, the composite
     // operator maps constants to zero and A^T has an exact null vector,
     // solved to 1e-11.  A stencil corner on a solid non-ghost cell (which
@@ -1294,16 +1326,32 @@ int compute_null_vector(pp_ctx *c) {
     // number of cycles and a loose floor.
     const bool exact = c->gh.solid_in_stencil == 0;
     o.max_outer = exact ? 200 : 40;
-    // a warm start also takes the previous null vector's shape for the
-    // transposed cycles' coarse correction
-    c->have_nshape = warm && c->d_nshape32 != nullptr;
     const double accept = exact ? 1e-10 : 1e-4;
     pp_solve_stats st;
     int lam = 0;
+    // The transposed cycles' coarse correction takes the null vector's
+    // shape: a warm start uses the previous geometry's; a cold start first
+    // runs a few cycles without it and takes the shape of that iterate.
+    double ref = warm ? c->null_ref : -1.0;
+    int stage1 = 0;
+    c->have_nshape = warm && c->d_nshape32 != nullptr;
+    if (!c->have_nshape) {
+      pp_solve_opts o1 = o;
+      o1.max_outer = 3;
+      rc = adjoint_impl<float>(c, nullptr, &o1, -1.0, &st, &lam);
+      if (rc != PP_OK && rc != PP_NOT_CONVERGED) return rc;
+      ref = st.b_l2;
+      stage1 = st.outer_iterations;
+      rc = null_shape_from_mu(c, vsum);
+      if (rc) return rc;
+      c->have_nshape = c->d_nshape32 != nullptr;
+    }
     // a warm start converges relative to the cold start's initial residual
     // (of the previous geometry), not to its own much smaller one
-    rc = adjoint_impl<float>(c, nullptr, &o, warm ? c->null_ref : -1.0, &st, &lam);
+    rc = adjoint_impl<float>(c, nullptr, &o, ref, &st, &lam);
     if (rc != PP_OK && rc != PP_NOT_CONVERGED) return rc;
+    st.outer_iterations += stage1;
+    if (!warm) st.b_l2 = ref;
     if (!(st.rel_l2 <= accept)) {
       set_error("left null vector did not converge (relative residual %g after %d iterations, %d ghost "
                 "stencil entries on solid cells)", st.rel_l2, st.outer_iterations, c->gh.solid_in_stencil);
@@ -1323,15 +1371,8 @@ int compute_null_vector(pp_ctx *c) {
     c->null_iters = st.outer_iterations;
     c->null_res = st.rel_l2;
     // the null space shape w/V for the adjoint V-cycle's coarse correction
-    Level &L0 = c->lv[0];
-    if (!c->d_nshape32) {
-      HIPCHK(hipMalloc(&c->d_nshape32, sizeof(float) * L0.total));
-      HIPCHK(hipMalloc(&c->d_nshape64, sizeof(double) * L0.total));
-      HIPCHK(hipMemsetAsync(c->d_nshape32, 0, sizeof(float) * L0.total, s));
-      HIPCHK(hipMemsetAsync(c->d_nshape64, 0, sizeof(double) * L0.total, s));
-    }
-    k_null_shape<float><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_w, c->d_nshape32);
-    k_null_shape<double><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_w, c->d_nshape64);
+    rc = null_shape_arrays(c);
+    if (rc) return rc;
     c->have_nshape = true;
   } else {
     // no ghost rows: the operator is volume-symmetric, w = V
