@@ -1345,6 +1345,7 @@ int compute_null_vector(pp_ctx *c) {
       rc = null_shape_from_mu(c, vsum);
       if (rc) return rc;
       c->have_nshape = c->d_nshape32 != nullptr;
+      o.max_outer -= stage1;  // the bound counts both stages
     }
     // a warm start converges relative to the cold start's initial residual
     // (of the previous geometry), not to its own much smaller one

@@ -95,7 +95,8 @@ module pp_hip_mod
        pp_geom_ghost_elements, pp_geom_destroy, pp_geom_dirichlet, pp_set_vel_ghosts, pp_correct_vel, &
        pp_correct_vel_device, pp_update_pressure, pp_update_pressure_device, pp_set_solve_ad, &
        pp_set_solve_ad_device, pp_solve_ad, pp_solve_ad_device, pp_calc_residual_ad, pp_geom_build_surfaces, &
-       pp_geom_canonical_surface
+       pp_geom_canonical_surface, pp_geom_fresh, pp_set_fresh_cells, pp_face_vel, pp_face_vel_device, &
+       pp_solve_ad_cn2, pp_solve_ad_cn2_device
 
   interface
      subroutine pp_default_opts(o) bind(c, name="pp_default_opts")
