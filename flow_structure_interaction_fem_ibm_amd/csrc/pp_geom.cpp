@@ -228,128 +228,15 @@ void mark_ghosts(const Grid &G, const ByteField &ib, ByteField &gm, std::vector<
   for (auto &v : plane) list.insert(list.end(), v.begin(), v.end());
 }
 
-static const int incI[8] = {0, 0, 1, 1, 0, 0, 1, 1};
-static const int incJ[8] = {0, 1, 1, 0, 0, 1, 1, 0};
-static const int incK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-
-// Solve M^T c = beta for the 8x8 Neumann Vandermonde matrix M (partial
-// pivoting); c are the coefficients of GCM_Calc_VanMatrixDN, i.e. row beta of
-// the inverse applied to the basis at the image point.
-static bool solve8T(const double M[8][8], const double beta[8], double c[8]) {
-  double A[8][9];
-  for (int i = 0; i < 8; i++) {
-    for (int j = 0; j < 8; j++) A[i][j] = M[j][i];
-    A[i][8] = beta[i];
-  }
-  for (int k = 0; k < 8; k++) {
-    int p = k;
-    for (int i = k + 1; i < 8; i++)
-      if (std::fabs(A[i][k]) > std::fabs(A[p][k])) p = i;
-    if (A[p][k] == 0.0) return false;
-    if (p != k)
-      for (int j = 0; j < 9; j++) std::swap(A[k][j], A[p][j]);
-    for (int i = k + 1; i < 8; i++) {
-      double f = A[i][k] / A[k][k];
-      for (int j = k; j < 9; j++) A[i][j] -= f * A[k][j];
-    }
-  }
-  for (int i = 7; i >= 0; i--) {
-    double s = A[i][8];
-    for (int j = i + 1; j < 8; j++) s -= A[i][j] * c[j];
-    c[i] = s / A[i][i];
-  }
-  return true;
-}
-
-// Image point, its interpolation cell and the Neumann coefficients of one
-// ghost (GCM_SET_BOUNDARY.F90:141-257, 399-491).  Returns 0, 1 (image point
-// cell not found) or 2 (intercept longer than twice the cell diagonal).
+// Image point, interpolation cell and coefficients of one ghost, and one
+// fresh cell's interpolation: the shared routines of pp_geom_core.h.
 int ghost_stencil(const Grid &G, int iG, int jG, int kG, const double *bi, const double *nrm, double probe_norm,
                   GhostOut &o) {
-  double xg = G.XC[0][iG], yg = G.XC[1][jG], zg = G.XC[2][kG];
-  double ds = std::sqrt((xg - bi[0]) * (xg - bi[0]) + (yg - bi[1]) * (yg - bi[1]) + (zg - bi[2]) * (zg - bi[2]));
-  double diag = std::sqrt(G.dxc(0, iG) * G.dxc(0, iG) + G.dxc(1, jG) * G.dxc(1, jG) + G.dxc(2, kG) * G.dxc(2, kG));
-  if (ds > diag && ds / diag > 2.0) return 2;
-  double sx = (bi[0] - xg) / ds, sy = (bi[1] - yg) / ds, sz = (bi[2] - zg) / ds;
-  double probe = ds * probe_norm;
-  double ip[3] = {xg + sx * probe, yg + sy * probe, zg + sz * probe};
-  int idx[3] = {iG, jG, kG};
-  int cell[3] = {-1, -1, -1};
-  int nn[3] = {G.nxc + 1, G.nyc + 1, G.nzc + 1};
-  for (int d = 0; d < 3; d++) {
-    int range = (int)std::lround(probe / G.DX[d][idx[d]]) + 1;
-    int lo = std::max(idx[d] - range, 0), hi = std::min(idx[d] + range, nn[d]);
-    for (int i = lo; i <= hi; i++)
-      if (G.XC[d][i] <= ip[d] && G.XC[d][i + 1] > ip[d]) cell[d] = i;
-  }
-  if (cell[0] < 0 || cell[1] < 0 || cell[2] < 0) return 1;
-  double M[8][8], MD[8][8];
-  for (int r = 0; r < 8; r++) {
-    int i = cell[0] + incI[r], j = cell[1] + incJ[r], k = cell[2] + incK[r];
-    double x1 = G.XC[0][i], x2 = G.XC[1][j], x3 = G.XC[2][k];
-    if (i == iG && j == jG && k == kG) {
-      x1 = bi[0]; x2 = bi[1]; x3 = bi[2];
-      // velocity (Dirichlet) row: the value at the body intercept
-      MD[r][0] = x1 * x2 * x3; MD[r][1] = x1 * x2; MD[r][2] = x1 * x3; MD[r][3] = x2 * x3;
-      MD[r][4] = x1; MD[r][5] = x2; MD[r][6] = x3; MD[r][7] = 1.0;
-      double n1 = nrm[0], n2 = nrm[1], n3 = nrm[2];
-      M[r][0] = n1 * x2 * x3 + n2 * x1 * x3 + n3 * x1 * x2;
-      M[r][1] = n1 * x2 + n2 * x1;
-      M[r][2] = n1 * x3 + n3 * x1;
-      M[r][3] = n2 * x3 + n3 * x2;
-      M[r][4] = n1; M[r][5] = n2; M[r][6] = n3; M[r][7] = 0.0;
-    } else {
-      M[r][0] = x1 * x2 * x3; M[r][1] = x1 * x2; M[r][2] = x1 * x3; M[r][3] = x2 * x3;
-      M[r][4] = x1; M[r][5] = x2; M[r][6] = x3; M[r][7] = 1.0;
-      for (int q = 0; q < 8; q++) MD[r][q] = M[r][q];
-    }
-  }
-  double beta[8] = {ip[0] * ip[1] * ip[2], ip[0] * ip[1], ip[0] * ip[2], ip[1] * ip[2], ip[0], ip[1], ip[2], 1.0};
-  if (!solve8T(M, beta, o.coeff)) return 3;
-  if (!solve8T(MD, beta, o.coeffD)) return 3;
-  o.ic = cell[0]; o.jc = cell[1]; o.kc = cell[2];
-  o.probe = probe;
-  for (int d = 0; d < 3; d++) { o.bi[d] = bi[d]; o.ip[d] = ip[d]; o.nrm[d] = nrm[d]; }
-  return 0;
+  return core::ghost_stencil(G.view(), iG, jG, kG, bi, nrm, probe_norm, o);
 }
 
-// GCM_set_freshcell for one cell o.i, o.j, o.k whose body intercept is bi
-// (GCM_SET_FRESHCELL.F90:101-262): the image point, the stencil search over
-// the cell and its two neighbours in each direction (clamped to 1..n-1 of
-// the cell centres) with the reference's test that the stencil corner is the
-// cell or the one below it, and the Dirichlet Vandermonde weights at the
-// cell centre.  Returns 0, 3, FRESH_ERR_STENCIL or FRESH_ERR_AT_BI.
 int fresh_stencil(const Grid &G, const double *bi, double probe_norm, FreshOut &o) {
-  const int idx[3] = {o.i, o.j, o.k}, nn[3] = {G.nxc, G.nyc, G.nzc};
-  const double fc[3] = {G.XC[0][o.i], G.XC[1][o.j], G.XC[2][o.k]};
-  const double ds = std::sqrt((fc[1] - bi[1]) * (fc[1] - bi[1]) + (fc[0] - bi[0]) * (fc[0] - bi[0]) +
-                              (fc[2] - bi[2]) * (fc[2] - bi[2]));
-  if (!(ds > 0.0)) return FRESH_ERR_AT_BI;
-  const double dsinv = 1.0 / ds, probe = ds * probe_norm;
-  int cell[3];
-  for (int d = 0; d < 3; d++) {
-    const double slope = (fc[d] - bi[d]) * dsinv;
-    const double fip = bi[d] + slope * probe;
-    const int lo = std::max(idx[d] - 1, 1), hi = std::min(idx[d] + 1, nn[d]);
-    cell[d] = -1;
-    for (int i = lo; i <= hi; i++)
-      if (G.XC[d][i] <= fip && G.XC[d][i + 1] > fip) cell[d] = i;
-    if (cell[d] < idx[d] - 1 || cell[d] > idx[d]) return FRESH_ERR_STENCIL;
-  }
-  double MD[8][8];
-  for (int r = 0; r < 8; r++) {
-    const int i = cell[0] + incI[r], j = cell[1] + incJ[r], k = cell[2] + incK[r];
-    double x1 = G.XC[0][i], x2 = G.XC[1][j], x3 = G.XC[2][k];
-    if (i == o.i && j == o.j && k == o.k) { x1 = bi[0]; x2 = bi[1]; x3 = bi[2]; }
-    MD[r][0] = x1 * x2 * x3; MD[r][1] = x1 * x2; MD[r][2] = x1 * x3; MD[r][3] = x2 * x3;
-    MD[r][4] = x1; MD[r][5] = x2; MD[r][6] = x3; MD[r][7] = 1.0;
-  }
-  const double beta[8] = {fc[0] * fc[1] * fc[2], fc[0] * fc[1], fc[0] * fc[2], fc[1] * fc[2],
-                          fc[0], fc[1], fc[2], 1.0};
-  if (!solve8T(MD, beta, o.coeffD)) return 3;
-  o.ic = cell[0]; o.jc = cell[1]; o.kc = cell[2];
-  for (int d = 0; d < 3; d++) o.bi[d] = bi[d];
-  return 0;
+  return core::fresh_stencil(G.view(), bi, probe_norm, o);
 }
 
 }  // namespace ppg

@@ -7,6 +7,7 @@
 #include <functional>
 #include <vector>
 
+#include "pp_geom_core.h"
 #include "pp_hip.h"
 
 namespace pp {
@@ -56,20 +57,20 @@ struct Grid {
   Grid(int nxc, int nyc, int nzc, const double *xn, const double *yn, const double *zn);
   long r(int i, int j, int k) const { return (long)i + ex * j + exy * k; }
   double dxc(int d, int i) const { return XC[d][i] - XC[d][i - 1]; }
+  GridV view() const {
+    GridV v;
+    v.nxc = nxc; v.nyc = nyc; v.nzc = nzc;
+    for (int d = 0; d < 3; d++) { v.X[d] = X[d].data(); v.XC[d] = XC[d].data(); v.DX[d] = DX[d].data(); }
+    v.ex = ex; v.exy = exy;
+    return v;
+  }
 };
+// pi = 4 atan(1), pi/10, cos(pi/10), pi/2 on the host's math library
+GeomConsts geom_consts();
 
 struct Body {
   double c[3], r[3], alpha_deg;
   int n_theta;
-};
-
-struct GhostOut {
-  int ic = -1, jc = -1, kc = -1;
-  int elem = 0;   // closest surface element (1-based), triangulated mode
-  int body = -1;  // body of the ghost (0-based)
-  double coeff[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // coeffGCMN (pressure, Neumann)
-  double coeffD[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // coeffGCMD (velocity, Dirichlet)
-  double probe = 0, bi[3] = {0, 0, 0}, ip[3] = {0, 0, 0}, nrm[3] = {0, 0, 0};
 };
 
 // triangulated bodies (pp_geom_tri.cpp)
@@ -78,10 +79,7 @@ struct GhostOut {
 // the pruned box, so the result is exactly that of repeated MINLOC over
 // all markers (smallest squared distance first, lowest index on ties).
 struct MarkerTree {
-  struct Node {
-    double lo[3], hi[3];
-    int a, b, left, right;
-  };
+  using Node = TreeNode;
   std::vector<Node> nodes;
   std::vector<int> perm;
 };
@@ -92,6 +90,17 @@ struct TriBody {
   std::vector<double> cx, cy, cz, nx, ny, nz, area;
   std::vector<int> ve_ptr, ve_idx;           // elements at each marker, ascending (CSR)
   MarkerTree tree;
+  TriV view() const {
+    TriV v;
+    v.nm = nm; v.ne = ne;
+    v.xm = xm.data(); v.ym = ym.data(); v.zm = zm.data();
+    v.tri = tri.data();
+    v.cx = cx.data(); v.cy = cy.data(); v.cz = cz.data();
+    v.nx = nx.data(); v.ny = ny.data(); v.nz = nz.data();
+    v.ve_ptr = ve_ptr.data(); v.ve_idx = ve_idx.data();
+    v.nodes = tree.nodes.data(); v.perm = tree.perm.data();
+    return v;
+  }
 };
 struct TriBodies {
   std::vector<TriBody> b;
@@ -104,14 +113,6 @@ struct Surface {
   const double *xyz = nullptr;
   const int32_t *tri = nullptr;
   double out[3] = {0.0, 0.0, 0.0};
-};
-// error codes of the triangulated pipeline (0 = success)
-enum {
-  TRI_ERR_NO_MARKER = 4,   // no unblocked marker among the 7 closest
-  TRI_ERR_NO_ELEMENT = 5,  // no element selected for a cell or ghost
-  TRI_ERR_SMALL = 6,       // too few markers/elements or rings, or grid centres not increasing
-  TRI_ERR_MARKER_ID = 8,   // an element names a marker out of range
-  TRI_ERR_ORIENT = 9,      // the origin (canonical bodies' orientation point) inside a body's bounding box
 };
 const char *tri_error_text(int rc);
 // 0 or an error code: TRI_ERR_SMALL or TRI_ERR_ORIENT
@@ -141,23 +142,6 @@ void mark_ghosts(const Grid &G, const ByteField &ib, ByteField &gm, std::vector<
 int ghost_stencil(const Grid &G, int iG, int jG, int kG, const double *bi, const double *nrm, double probe_norm,
                   GhostOut &o);
 
-// One fresh cell's velocity interpolation (GCM_set_freshcell,
-// GCM_SET_FRESHCELL.F90:86-262): the 2x2x2 stencil below the image point
-// FIP = BI + (FC - BI) probeLengthNormalized and coeffGCMFreshD, the weights
-// that give the value at the cell centre FC from the stencil values with the
-// cell's own node replaced by the body intercept BI.
-struct FreshOut {
-  int i = 0, j = 0, k = 0;         // the fresh cell (1-based)
-  int ic = -1, jc = -1, kc = -1;   // iFreshCellIndex..: the stencil's lower corner
-  int body = -1, elem = 0;         // body (0-based) and surface element (1-based, triangulated mode)
-  double bi[3] = {0, 0, 0};
-  double coeffD[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-};
-// fresh-cell error codes of fresh_stencil (0 = success, 3 singular Vandermonde)
-enum {
-  FRESH_ERR_STENCIL = 10,   // the image point's cell is not the fresh cell or the one below it
-  FRESH_ERR_AT_BI = 11,     // the body intercept coincides with the cell centre
-};
 int fresh_stencil(const Grid &G, const double *bi, double probe_norm, FreshOut &o);
 
 }  // namespace ppg

@@ -3,7 +3,7 @@ short markdown, grouped by kernel and grid size (one grid size per
 multigrid level).
 
   python -m flow_structure_interaction_fem_ibm_amd.profile_summary trace <kernel_trace.csv> [--last-solve]
-      [--match SUBSTRING ...]
+      [--match SUBSTRING ...] [--by-level]
   python -m flow_structure_interaction_fem_ibm_amd.profile_summary pmc <counter_collection.csv> [...more csv]
   python -m flow_structure_interaction_fem_ibm_amd.profile_summary traffic <fetch.csv> <write.csv> \
       --copy-mib 512 --out profiles/traffic.json
@@ -45,6 +45,47 @@ def trace(path, last_solve=False, top=30, match=None):
            "", "| kernel | grid (threads) | block | calls | total us | avg us |", "|---|---|---|---|---|---|"]
     for (k, g, b), (n, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
         out.append(f"| `{k}` | {g} | {b} | {n} | {t:.1f} | {t / n:.2f} |")
+    return "\n".join(out)
+
+
+_DOWN = ("k_resrestrict2", "k_restrict<")   # a transfer to the next coarser level
+_UP = ("k_prolong2", "k_prolong<")           # a transfer to the next finer level
+
+
+def by_level(path, last_solve=True):
+    """Device time per multigrid level of the last solve, from the launch
+    order: a restriction ends the work of its fine level, a prolongation
+    starts the work of its fine level again; the transfer kernels are counted
+    on their fine level, the dense coarsest solve on the coarsest, and the
+    outer-iteration kernels (Krylov, outer residual, ghost rows, reductions)
+    on level 0."""
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    if last_solve:
+        starts = [i for i, r in enumerate(rows) if "k_mask_rhs" in r["Kernel_Name"]]
+        if starts:
+            rows = rows[starts[-1]:]
+    depth = 0
+    lev = collections.defaultdict(lambda: [0, 0.0])
+    for r in rows:
+        n = _short(r["Kernel_Name"])
+        t = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+        if any(n.startswith("pp::" + k) or n.startswith(k) for k in _UP):
+            depth -= 1
+            key = depth
+        elif any(n.startswith("pp::" + k) or n.startswith(k) for k in _DOWN):
+            key = depth
+            depth += 1
+        else:
+            key = depth
+        lev[key][0] += 1
+        lev[key][1] += t
+    total = sum(v[1] for v in lev.values())
+    out = ["| level | launches | total us | share |", "|---|---|---|---|"]
+    for k in sorted(lev):
+        out.append(f"| {k} | {lev[k][0]} | {lev[k][1]:.1f} | {lev[k][1] / total:.1%} |")
+    below = sum(v[1] for k, v in lev.items() if k > 0)
+    out.append(f"\nlevels below the finest: {below / 1e3:.3f} ms of {total / 1e3:.3f} ms summed kernel time")
     return "\n".join(out)
 
 
@@ -116,6 +157,7 @@ def main():
     ap.add_argument("paths", nargs="+")
     ap.add_argument("--last-solve", action="store_true")
     ap.add_argument("--match", nargs="*", default=None, help="trace: keep kernels whose name holds one of these")
+    ap.add_argument("--by-level", action="store_true", help="trace: also the device time per multigrid level")
     ap.add_argument("--json", default=None, help="pmc: also write the per-dispatch figures as JSON")
     ap.add_argument("--copy-mib", type=int, default=512, help="traffic: size of the calibration copy")
     ap.add_argument("--out", default=None, help="traffic: output JSON path")
@@ -129,6 +171,9 @@ def main():
         return
     if args.kind == "trace":
         print(trace(args.paths[0], args.last_solve, match=args.match))
+        if args.by_level:
+            print()
+            print(by_level(args.paths[0], args.last_solve))
     else:
         text, res = pmc(args.paths)
         print(text)
