@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["csrc/pp_api.hip", "csrc/pp_setup.hip", "csrc/pp_geom.cpp", "csrc/pp_geom_tri.cpp"]
+HIP_SOURCES = ["csrc/pp_api.hip", "csrc/pp_setup.hip", "csrc/pp_geom.cpp", "csrc/pp_geom_tri.cpp",
+               "csrc/pp_geom_dev.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are disabled for all device code

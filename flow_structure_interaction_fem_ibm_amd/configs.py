@@ -15,7 +15,8 @@ GPU with their throughput, timing and memory figures printed as JSON.
 
   python -m flow_structure_interaction_fem_ibm_amd.configs --config 3 [--steps 50]
   python -m flow_structure_interaction_fem_ibm_amd.configs --config 5
-  (--triangulated: the reference's triangulated-surface geometry)
+  (--triangulated: the reference's triangulated-surface geometry;
+   --device-geometry: the same, built on the GPU)
 """
 from __future__ import annotations
 
@@ -48,7 +49,7 @@ def config3_body(step):
     return pp.Ellipsoid((1.5 + 0.01 * step, 2.0, 2.0), (0.3, 0.3, 0.3), 0.0, 48)
 
 
-def run_config3(steps=50, tol=1e-10, mode=pp.PP_GEOM_ANALYTIC):
+def run_config3(steps=50, tol=1e-10, mode=pp.PP_GEOM_ANALYTIC, device_geometry=False):
     import torch
     xn, yn, zn = config3_grid()
     dev = torch.device("cuda:0")
@@ -60,7 +61,8 @@ def run_config3(steps=50, tol=1e-10, mode=pp.PP_GEOM_ANALYTIC):
     t_all = time.perf_counter()
     for st_i in range(steps):
         t0 = time.perf_counter()
-        ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, [config3_body(st_i)], iblank_prev=ib_prev, mode=mode)
+        ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, [config3_body(st_i)], iblank_prev=ib_prev, mode=mode,
+                                           device=0 if device_geometry else None)
         t1 = time.perf_counter()
         s.set_geometry(ib, gs, fresh=fr if ib_prev is not None else None)
         t2 = time.perf_counter()
@@ -94,7 +96,8 @@ def run_config3(steps=50, tol=1e-10, mode=pp.PP_GEOM_ANALYTIC):
     gb = sum(r["GB"] for r in recs)
     return {
         "config": "3: 384x256x256 moving sphere, geometry rebuilt per step, mixed solve to 1e-10",
-        "geometry": "triangulated" if mode == pp.PP_GEOM_TRIANGULATED else "analytic",
+        "geometry": ("triangulated" if mode == pp.PP_GEOM_TRIANGULATED else "analytic") +
+                    (", built on the GPU" if device_geometry else ", built on the host"),
         "steps": steps, "cells": 384 * 256 * 256,
         "solves_per_s": steps / solve_s, "device_ms_per_solve": dev_ms / steps,
         "steps_per_s_including_geometry": steps / wall,
@@ -163,14 +166,14 @@ def force_x_weights(yn, zn, ib):
     return w
 
 
-def run_config5(mode=pp.PP_GEOM_ANALYTIC):
+def run_config5(mode=pp.PP_GEOM_ANALYTIC, device_geometry=False):
     import torch
     xn, yn, zn = config5_grid()
     dev = torch.device("cuda:0")
     free0, total = torch.cuda.mem_get_info()
     t0 = time.perf_counter()
     bodies = config5_bodies(xn, yn, zn)
-    ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, bodies, mode=mode)
+    ib, fr, gs, ex = pp.build_geometry(xn, yn, zn, bodies, mode=mode, device=0 if device_geometry else None)
     t1 = time.perf_counter()
     s = pp.PressureSolver(xn, yn, zn)
     s.set_geometry(ib, gs)
@@ -193,7 +196,8 @@ def run_config5(mode=pp.PP_GEOM_ANALYTIC):
     fx = float((w * d_p.cpu().numpy()).sum())
     return {
         "config": "5: 512x512x384 (1.0e8 cells), 64 spheres, mixed solve to 1e-8 + one adjoint",
-        "geometry": "triangulated" if mode == pp.PP_GEOM_TRIANGULATED else "analytic",
+        "geometry": ("triangulated" if mode == pp.PP_GEOM_TRIANGULATED else "analytic") +
+                    (", built on the GPU" if device_geometry else ", built on the host"),
         "cells": 512 * 512 * 384, "ghosts": gs.n, "blank": int(ib.sum()),
         "forward": {"status": st["status"], "iterations": st["outer_iterations"], "rel_l2": st["rel_l2"],
                     "wall_s": t4 - t3, "device_ms": st["ms"], "solves_per_s": 1.0 / (t4 - t3),
@@ -214,9 +218,14 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--triangulated", action="store_true",
                     help="the reference's triangulated-surface geometry instead of the analytic ellipsoid")
+    ap.add_argument("--device-geometry", action="store_true",
+                    help="build the triangulated geometry on the GPU (pp_geom_build_device)")
     args = ap.parse_args()
-    mode = pp.PP_GEOM_TRIANGULATED if args.triangulated else pp.PP_GEOM_ANALYTIC
-    out = run_config3(args.steps, mode=mode) if args.config == 3 else run_config5(mode)
+    mode = pp.PP_GEOM_TRIANGULATED if args.triangulated or args.device_geometry else pp.PP_GEOM_ANALYTIC
+    if args.config == 3:
+        out = run_config3(args.steps, mode=mode, device_geometry=args.device_geometry)
+    else:
+        out = run_config5(mode, device_geometry=args.device_geometry)
     print(json.dumps(out))
 
 

@@ -246,15 +246,25 @@ int fresh_stencil(const Grid &G, const double *bi, double probe_norm, FreshOut &
 // ======================================================================
 using namespace ppg;
 
-struct pp_geom {
-  ByteField iblank, fresh, ghost;
-  std::vector<int> list;       // ig, jg, kg triples (1-based)
-  std::vector<GhostOut> gout;
-  int nblank = 0, nfresh = 0;
-  // fresh cells in k, j, i order and the first failure of their stencils
-  std::vector<FreshOut> fout;
-  int fresh_err = 0, fresh_err_cell[3] = {0, 0, 0};
-};
+namespace ppg {
+const char *ghost_error_text(int e) {
+  return e == GHOST_ERR_IP_CELL    ? "image point cell not found"
+         : e == GHOST_ERR_LONG     ? "intercept too long"
+         : e == GHOST_ERR_SINGULAR ? "singular Vandermonde"
+         : e == TRI_ERR_NO_ELEMENT ? "no element selected for the intercept"
+                                   : "ghost cell without a body";
+}
+
+void count_cells(const Grid &G, pp_geom *o) {
+  std::vector<long> nb(G.nzc + 3, 0), nf(G.nzc + 3, 0);
+  parallel_for(G.nzc + 3, [&](long k0, long k1) {
+    for (long k = k0; k < k1; k++)
+      for (long r = k * G.exy; r < (k + 1) * G.exy; r++) { nb[k] += o->iblank[r]; nf[k] += o->fresh[r]; }
+  }, kPlaneGrain);
+  o->nblank = o->nfresh = 0;
+  for (int k = 0; k < G.nzc + 3; k++) { o->nblank += (int)nb[k]; o->nfresh += (int)nf[k]; }
+}
+}  // namespace ppg
 
 namespace {
 // the part of pp_geom_build after its bodies are known: canonical bodies
@@ -361,23 +371,11 @@ int32_t build_geometry(pp_geom **out, const Grid &G, int32_t mode, const std::ve
   for (int q = 0; q < ng; q++)
     if (err[q]) {
       pp::set_error("pp_geom_build: ghost %d at (%d,%d,%d): %s", q, o->list[3 * q], o->list[3 * q + 1],
-                    o->list[3 * q + 2],
-                    err[q] == 1   ? "image point cell not found"
-                    : err[q] == 2 ? "intercept too long"
-                    : err[q] == 3 ? "singular Vandermonde"
-                    : err[q] == 5 ? "no element selected for the intercept"
-                                  : "ghost cell without a body");
+                    o->list[3 * q + 2], ghost_error_text(err[q]));
       delete o;
       return PP_BAD_GEOMETRY;
     }
-  {
-    std::vector<long> nb(G.nzc + 3, 0), nf(G.nzc + 3, 0);
-    parallel_for(G.nzc + 3, [&](long k0, long k1) {
-      for (long k = k0; k < k1; k++)
-        for (long r = k * G.exy; r < (k + 1) * G.exy; r++) { nb[k] += o->iblank[r]; nf[k] += o->fresh[r]; }
-    }, kPlaneGrain);
-    for (int k = 0; k < G.nzc + 3; k++) { o->nblank += (int)nb[k]; o->nfresh += (int)nf[k]; }
-  }
+  count_cells(G, o);
   // fresh-cell interpolation (GCM_set_freshcell).  The reference takes the
   // body intercept on bodyNum of the cell, the body that blanked it at the
   // previous geometry; that is not known here, so the body is the nearest
@@ -464,6 +462,72 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
     bs[b].n_theta = d.n_theta;
   }
   return build_geometry(out, G, mode, bs, nullptr, probe_norm, iblank_prev);
+}
+
+// the triangulated bodies of pp_geom_build's arguments, or a status
+static int32_t tri_bodies_of(const char *who, const Grid &G, int32_t nbody, const pp_body_desc *bodies,
+                             TriBodies &tri) {
+  std::vector<Body> bs(nbody);
+  for (int b = 0; b < nbody; b++) {
+    const pp_body_desc &d = bodies[b];
+    if (d.shape != PP_BODY_ELLIPSOID || d.semi[0] <= 0 || d.semi[1] <= 0 || d.semi[2] <= 0) {
+      pp::set_error("%s: body %d is not a valid ellipsoid", who, b);
+      return PP_BAD_ARG;
+    }
+    for (int q = 0; q < 3; q++) { bs[b].c[q] = d.center[q]; bs[b].r[q] = d.semi[q]; }
+    bs[b].alpha_deg = d.alpha_deg;
+    bs[b].n_theta = d.n_theta;
+  }
+  const int rc = tri_build(G, bs, tri);
+  if (rc) {
+    pp::set_error("%s: triangulated set_boundary failed (%d: %s)", who, rc, tri_error_text(rc));
+    return PP_BAD_GEOMETRY;
+  }
+  return PP_OK;
+}
+
+int32_t pp_geom_build_device(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const pp_body_desc *bodies,
+                             double probe_norm, const int8_t *iblank_prev) {
+  if (!out || !g || nbody < 0 || nbody > 256 || (nbody > 0 && !bodies) || probe_norm <= 0.0) {
+    pp::set_error("pp_geom_build_device: bad argument");
+    return PP_BAD_ARG;
+  }
+  *out = nullptr;
+  Grid G(g->nxc, g->nyc, g->nzc, g->x, g->y, g->z);
+  TriBodies tri;
+  const int32_t rc = tri_bodies_of("pp_geom_build_device", G, nbody, bodies, tri);
+  if (rc) return rc;
+  return build_geometry_device(out, G, tri, probe_norm, iblank_prev, g->device);
+}
+
+int32_t pp_geom_build_surfaces_device(pp_geom **out, const pp_grid_desc *g, int32_t n_surface,
+                                      const pp_surface_desc *s, double probe_norm, const int8_t *iblank_prev) {
+  if (!out || !g || n_surface < 0 || n_surface > 256 || (n_surface > 0 && !s) || probe_norm <= 0.0) {
+    pp::set_error("pp_geom_build_surfaces_device: bad argument");
+    return PP_BAD_ARG;
+  }
+  *out = nullptr;
+  std::vector<Surface> sv(n_surface);
+  for (int b = 0; b < n_surface; b++) {
+    if (s[b].n_markers < 3 || s[b].n_elems < 1 || !s[b].xyz || !s[b].tri) {
+      pp::set_error("pp_geom_build_surfaces_device: surface %d has no markers or elements", b);
+      return PP_BAD_ARG;
+    }
+    sv[b].nm = s[b].n_markers;
+    sv[b].ne = s[b].n_elems;
+    sv[b].xyz = s[b].xyz;
+    sv[b].tri = s[b].tri;
+    for (int d = 0; d < 3; d++) sv[b].out[d] = s[b].outside[d];
+  }
+  Grid G(g->nxc, g->nyc, g->nzc, g->x, g->y, g->z);
+  TriBodies tri;
+  const int rc = tri_build_surfaces(sv, tri);
+  if (rc) {
+    pp::set_error("pp_geom_build_surfaces_device: triangulated set_boundary failed (%d: %s)", rc,
+                  tri_error_text(rc));
+    return PP_BAD_GEOMETRY;
+  }
+  return build_geometry_device(out, G, tri, probe_norm, iblank_prev, g->device);
 }
 
 int32_t pp_geom_build_surfaces(pp_geom **out, const pp_grid_desc *g, int32_t n_surface, const pp_surface_desc *s,

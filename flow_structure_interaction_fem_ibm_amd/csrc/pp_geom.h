@@ -143,5 +143,26 @@ int ghost_stencil(const Grid &G, int iG, int jG, int kG, const double *bi, const
                   GhostOut &o);
 
 int fresh_stencil(const Grid &G, const double *bi, double probe_norm, FreshOut &o);
+const char *ghost_error_text(int e);
 
+}  // namespace ppg
+
+// the built geometry behind the C-ABI handle
+struct pp_geom {
+  ppg::ByteField iblank, fresh, ghost;
+  std::vector<int> list;       // ig, jg, kg triples (1-based)
+  std::vector<ppg::GhostOut> gout;
+  int nblank = 0, nfresh = 0;
+  // fresh cells in k, j, i order and the first failure of their stencils
+  std::vector<ppg::FreshOut> fout;
+  int fresh_err = 0, fresh_err_cell[3] = {0, 0, 0};
+};
+
+namespace ppg {
+// nblank and nfresh of o from its iblank and fresh fields
+void count_cells(const Grid &G, pp_geom *o);
+// the triangulated pipeline of pp_geom_build on device dev: bodies built,
+// the rest on the GPU; PP_OK or a status with the error text set
+int32_t build_geometry_device(pp_geom **out, const Grid &G, const TriBodies &tri, double probe_norm,
+                              const int8_t *iblank_prev, int dev);
 }  // namespace ppg

@@ -96,7 +96,7 @@ module pp_hip_mod
        pp_correct_vel_device, pp_update_pressure, pp_update_pressure_device, pp_set_solve_ad, &
        pp_set_solve_ad_device, pp_solve_ad, pp_solve_ad_device, pp_calc_residual_ad, pp_geom_build_surfaces, &
        pp_geom_canonical_surface, pp_geom_fresh, pp_set_fresh_cells, pp_face_vel, pp_face_vel_device, &
-       pp_solve_ad_cn2, pp_solve_ad_cn2_device
+       pp_solve_ad_cn2, pp_solve_ad_cn2_device, pp_geom_build_device, pp_geom_build_surfaces_device
 
   interface
      subroutine pp_default_opts(o) bind(c, name="pp_default_opts")
@@ -401,6 +401,32 @@ module pp_hip_mod
        type(c_ptr), value :: iblank_prev
        integer(c_int32_t) :: rc
      end function pp_geom_build_surfaces
+
+     ! pp_geom_build (triangulated) and pp_geom_build_surfaces with the per-cell and per-ghost work on GPU
+     ! g%device; the same handle and getters, the same results bit for bit
+     function pp_geom_build_device(geo, g, n_body, bodies, probe_norm, iblank_prev) &
+          bind(c, name="pp_geom_build_device") result(rc)
+       import :: c_ptr, c_int32_t, c_double, pp_grid_desc, pp_body_desc
+       type(c_ptr), intent(out) :: geo
+       type(pp_grid_desc), intent(in) :: g
+       integer(c_int32_t), value :: n_body
+       type(pp_body_desc), intent(in) :: bodies(*)
+       real(c_double), value :: probe_norm
+       type(c_ptr), value :: iblank_prev
+       integer(c_int32_t) :: rc
+     end function pp_geom_build_device
+
+     function pp_geom_build_surfaces_device(geo, g, n_surface, surfaces, probe_length_normalized, iblank_prev) &
+          bind(c, name="pp_geom_build_surfaces_device") result(rc)
+       import :: c_ptr, c_int32_t, c_double, pp_grid_desc, pp_surface_desc
+       type(c_ptr), intent(out) :: geo
+       type(pp_grid_desc), intent(in) :: g
+       integer(c_int32_t), value :: n_surface
+       type(pp_surface_desc), intent(in) :: surfaces(*)
+       real(c_double), value :: probe_length_normalized
+       type(c_ptr), value :: iblank_prev
+       integer(c_int32_t) :: rc
+     end function pp_geom_build_surfaces_device
 
      ! counts with xyz = tri = C_NULL_PTR, then the arrays
      function pp_geom_canonical_surface(body, n_markers, n_elems, xyz, tri) &

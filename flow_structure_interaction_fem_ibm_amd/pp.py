@@ -173,6 +173,9 @@ def lib():
     L.pp_set_fresh_cells.argtypes = [P, C.c_int32] + [P] * 7
     L.pp_geom_build_surfaces.argtypes = [C.POINTER(C.c_void_p), C.POINTER(GridDesc), C.c_int32,
                                          C.POINTER(SurfaceDesc), C.c_double, P]
+    L.pp_geom_build_device.argtypes = [C.POINTER(C.c_void_p), C.POINTER(GridDesc), C.c_int32,
+                                       C.POINTER(BodyDesc), C.c_double, P]
+    L.pp_geom_build_surfaces_device.argtypes = L.pp_geom_build_surfaces.argtypes
     L.pp_geom_canonical_surface.argtypes = [C.POINTER(BodyDesc), C.POINTER(C.c_int32), C.POINTER(C.c_int32), P, P]
     L.pp_set_vel_ghosts.argtypes = [P, P, C.c_double, C.c_double]
     L.pp_correct_vel.argtypes = [P, C.c_double] + [P] * 11
@@ -197,7 +200,8 @@ def lib():
                  "pp_set_solve_ad", "pp_set_solve_ad_device", "pp_solve_ad", "pp_solve_ad_device",
                  "pp_calc_residual_ad", "pp_geom_build_surfaces", "pp_geom_canonical_surface",
                  "pp_level_apply", "pp_level_data", "pp_divergence_noalloc", "pp_geom_fresh", "pp_set_fresh_cells",
-                 "pp_face_vel", "pp_face_vel_device", "pp_solve_ad_cn2", "pp_solve_ad_cn2_device"):
+                 "pp_face_vel", "pp_face_vel_device", "pp_solve_ad_cn2", "pp_solve_ad_cn2_device",
+                 "pp_geom_build_device", "pp_geom_build_surfaces_device"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -212,7 +216,8 @@ EXPORTED = ("pp_default_opts", "pp_last_error", "pp_create", "pp_destroy", "pp_s
             "pp_adjoint_device", "pp_set_solve_ad", "pp_set_solve_ad_device", "pp_solve_ad", "pp_solve_ad_device",
             "pp_calc_residual_ad", "pp_geom_build_surfaces", "pp_geom_canonical_surface", "pp_level_apply",
             "pp_level_data", "pp_divergence_noalloc", "pp_geom_fresh", "pp_set_fresh_cells", "pp_face_vel",
-            "pp_face_vel_device", "pp_solve_ad_cn2", "pp_solve_ad_cn2_device")
+            "pp_face_vel_device", "pp_solve_ad_cn2", "pp_solve_ad_cn2_device", "pp_geom_build_device",
+            "pp_geom_build_surfaces_device")
 
 
 def _check(rc, allow=(PP_OK,)):
@@ -602,14 +607,17 @@ def _grid_desc(xn, yn, zn, device=0):
     return g
 
 
-def build_geometry(xn, yn, zn, bodies, mode=PP_GEOM_ANALYTIC, probe_norm=2.0, iblank_prev=None):
-    """set_boundary for canonical ellipsoids through pp_geom_build.
+def build_geometry(xn, yn, zn, bodies, mode=PP_GEOM_ANALYTIC, probe_norm=2.0, iblank_prev=None, device=None):
+    """set_boundary for canonical ellipsoids through pp_geom_build, or, with
+    device = a GPU index (triangulated mode only), pp_geom_build_device.
 
     Returns (iblank, fresh, GhostSet, extras) with iblank/fresh in the
     reference layout and extras = dict(bi=..., ip=..., nrm=...)."""
     L = lib()
     xn, yn, zn = (np.ascontiguousarray(a, dtype=np.float64) for a in (xn, yn, zn))
-    g = _grid_desc(xn, yn, zn)
+    g = _grid_desc(xn, yn, zn, 0 if device is None else device)
+    if device is not None and mode != PP_GEOM_TRIANGULATED:
+        raise ValueError("the device build is the triangulated pipeline (mode=PP_GEOM_TRIANGULATED)")
     arr = (BodyDesc * max(len(bodies), 1))()
     for q, b in enumerate(bodies):
         arr[q].shape = PP_BODY_ELLIPSOID
@@ -620,18 +628,23 @@ def build_geometry(xn, yn, zn, bodies, mode=PP_GEOM_ANALYTIC, probe_norm=2.0, ib
         arr[q].n_theta = b.n_theta
     prev = None if iblank_prev is None else np.ascontiguousarray(iblank_prev, dtype=np.int8)
     h = C.c_void_p()
-    _check(L.pp_geom_build(C.byref(h), C.byref(g), len(bodies), arr, mode, probe_norm,
-                           None if prev is None else prev.ctypes.data))
+    if device is None:
+        _check(L.pp_geom_build(C.byref(h), C.byref(g), len(bodies), arr, mode, probe_norm,
+                               None if prev is None else prev.ctypes.data))
+    else:
+        _check(L.pp_geom_build_device(C.byref(h), C.byref(g), len(bodies), arr, probe_norm,
+                                      None if prev is None else prev.ctypes.data))
     return _geometry_result(h, xn, yn, zn)
 
 
-def build_geometry_surfaces(xn, yn, zn, surfaces, probe_norm=2.0, iblank_prev=None):
-    """set_boundary for UNSTRUCTURED_SURFACE bodies (pp_geom_build_surfaces):
-    surfaces = [(xyz (n_markers, 3), tri (n_elems, 3) 1-based, outside point)].
-    Returns what build_geometry returns."""
+def build_geometry_surfaces(xn, yn, zn, surfaces, probe_norm=2.0, iblank_prev=None, device=None):
+    """set_boundary for UNSTRUCTURED_SURFACE bodies (pp_geom_build_surfaces,
+    or pp_geom_build_surfaces_device on GPU `device`): surfaces = [(xyz
+    (n_markers, 3), tri (n_elems, 3) 1-based, outside point)].  Returns what
+    build_geometry returns."""
     L = lib()
     xn, yn, zn = (np.ascontiguousarray(a, dtype=np.float64) for a in (xn, yn, zn))
-    g = _grid_desc(xn, yn, zn)
+    g = _grid_desc(xn, yn, zn, 0 if device is None else device)
     keep = []
     arr = (SurfaceDesc * max(len(surfaces), 1))()
     for q, (xyz, tri, out) in enumerate(surfaces):
@@ -644,8 +657,8 @@ def build_geometry_surfaces(xn, yn, zn, surfaces, probe_norm=2.0, iblank_prev=No
             arr[q].outside[d] = out[d]
     prev = None if iblank_prev is None else np.ascontiguousarray(iblank_prev, dtype=np.int8)
     h = C.c_void_p()
-    _check(L.pp_geom_build_surfaces(C.byref(h), C.byref(g), len(surfaces), arr, probe_norm,
-                                    None if prev is None else prev.ctypes.data))
+    build = L.pp_geom_build_surfaces if device is None else L.pp_geom_build_surfaces_device
+    _check(build(C.byref(h), C.byref(g), len(surfaces), arr, probe_norm, None if prev is None else prev.ctypes.data))
     return _geometry_result(h, xn, yn, zn)
 
 

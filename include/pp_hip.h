@@ -507,6 +507,26 @@ int32_t pp_geom_build_surfaces(pp_geom **out, const pp_grid_desc *g,
                                int32_t n_surface, const pp_surface_desc *s,
                                double probe_norm, const int8_t *iblank_prev);
 
+/* pp_geom_build (PP_GEOM_TRIANGULATED) and pp_geom_build_surfaces with the
+ * per-cell, per-line and per-ghost work on GPU g->device (the band
+ * dot-normal classification of set_iblank_canonical_body_fast, the i/j/k ray
+ * fills, find_isolatedIblank, fill_cell, the ghost marking and list of
+ * GCM_set_internal_boundary, GCM_calc_bodyIntercept_Unstruc, the image points
+ * and GCM_Calc_VanMatrixDN's Vandermonde solves, GCM_set_freshcell;
+ * BOUNDARY_IBLANK_FAST.F90:2-501, GCM_SET_BOUNDARY.F90:2-494,
+ * GCM_SET_FRESHCELL.F90:1-262).  The result is the host build's, bit for bit,
+ * behind the same handle and getters.  Grids up to 2^31 - 1 cells including
+ * the reference's outer layers. */
+int32_t pp_geom_build_device(pp_geom **out, const pp_grid_desc *g,
+                             int32_t n_body, const pp_body_desc *bodies,
+                             double probe_length_normalized,
+                             const int8_t *iblank_prev);
+int32_t pp_geom_build_surfaces_device(pp_geom **out, const pp_grid_desc *g,
+                                      int32_t n_surface,
+                                      const pp_surface_desc *s,
+                                      double probe_norm,
+                                      const int8_t *iblank_prev);
+
 /* The marker surface PP_GEOM_TRIANGULATED builds for one canonical body
  * (BOUNDARY_MARKER_INIT.F90 ELLIPSOID branch, oriented against the origin),
  * e.g. to write it as unstruc_surface_in.dat.  Call with xyz = tri = NULL
