@@ -1330,11 +1330,18 @@ int compute_null_vector(pp_ctx *c) {
     pp_solve_stats st;
     int lam = 0;
     // The transposed cycles' coarse correction takes the null vector's
-    // shape: a warm start uses the previous geometry's; a cold start first
-    // runs a few cycles without it and takes the shape of that iterate.
+    // shape: a warm start takes it from its initial iterate (the previous
+    // geometry's null vector on this geometry's fluid cells, 1 on the fresh
+    // ones; the previous shape would be 0 there); a cold start first runs a
+    // few cycles without it and takes the shape of that iterate.
     double ref = warm ? c->null_ref : -1.0;
     int stage1 = 0;
-    c->have_nshape = warm && c->d_nshape32 != nullptr;
+    c->have_nshape = false;
+    if (warm) {
+      rc = null_shape_from_mu(c, vsum);
+      if (rc) return rc;
+      c->have_nshape = c->d_nshape32 != nullptr;
+    }
     if (!c->have_nshape) {
       pp_solve_opts o1 = o;
       o1.max_outer = 3;
@@ -1524,7 +1531,7 @@ int32_t pp_destroy(pp_ctx *c) {
   free_levels(c);
   free_ghosts(c);
   hipFree(c->d_p); hipFree(c->d_b); hipFree(c->d_w); hipFree(c->d_bwork); hipFree(c->d_tmp);
-  hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_mu); hipFree(c->d_sv); hipFree(c->d_mu_prev);
+  hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_count); hipFree(c->d_mu); hipFree(c->d_sv); hipFree(c->d_mu_prev);
   hipFree(c->d_ad_coef);
   hipFree(c->d_nshape32); hipFree(c->d_nshape64);
   free_fresh_rows(c);
@@ -1580,13 +1587,6 @@ int32_t pp_set_geometry(pp_ctx *c, const pp_geometry *geo) {
           if (i == 0 || j == 0 || k == 0 || i > c->nxc || j > c->nyc || k > c->nzc)
             c->h_iblank[(long)i + c->ref.ex * j + c->ref.exy * k] = 0;
   });
-  derive_ghost_mark(c);
-  long nmark = 0;
-  for (long r = 0; r < c->nref; r++) nmark += c->h_ghost[r];
-  if (nmark != geo->n_ghost) {
-    set_error("pp_set_geometry: iblank has %ld ghost cells, the ghost list has %d", nmark, geo->n_ghost);
-    return PP_BAD_GEOMETRY;
-  }
   free_ghosts(c);
   auto now = [] { return std::chrono::steady_clock::now(); };
   auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
@@ -1595,13 +1595,18 @@ int32_t pp_set_geometry(pp_ctx *c, const pp_geometry *geo) {
   if (c->lv.empty()) {  // the hierarchy depends on the grid only
     rc = build_levels(c);
     if (rc) return rc;
-    build_flags(c);
     rc = upload_levels(c);
-  } else {
-    build_flags(c);
-    rc = upload_flags(c);
+    if (rc) return rc;
   }
+  if (!c->d_iblank_ref) HIPCHK(hipMalloc(&c->d_iblank_ref, c->nref));
+  HIPCHK(hipMemcpyAsync(c->d_iblank_ref, c->h_iblank.data(), c->nref, hipMemcpyHostToDevice, c->stream));
+  long nmark = 0;
+  rc = build_flags(c, c->d_iblank_ref, &nmark);
   if (rc) return rc;
+  if (nmark != geo->n_ghost) {
+    set_error("pp_set_geometry: iblank has %ld ghost cells, the ghost list has %d", nmark, geo->n_ghost);
+    return PP_BAD_GEOMETRY;
+  }
   auto t1 = now();
   rc = build_ghosts(c, geo);
   if (rc) return rc;
@@ -1614,17 +1619,17 @@ int32_t pp_set_geometry(pp_ctx *c, const pp_geometry *geo) {
   hipFree(c->d_fresh);
   c->d_fresh = nullptr;
   if (geo->fresh_cell) {
-    bool any = false;
-    for (long r = 0; r < c->nref; r++) any |= geo->fresh_cell[r] == 1;
-    if (any) {
+    std::atomic<bool> any(false);
+    host_parallel(c->nref, [&](long a, long b) {
+      for (long r = a; r < b; r++)
+        if (geo->fresh_cell[r] == 1) { any.store(true); break; }
+    });
+    if (any.load()) {
       c->h_fresh.assign(geo->fresh_cell, geo->fresh_cell + c->nref);
       HIPCHK(hipMalloc(&c->d_fresh, c->nref));
       HIPCHK(hipMemcpy(c->d_fresh, c->h_fresh.data(), c->nref, hipMemcpyHostToDevice));
     }
   }
-  hipFree(c->d_iblank_ref);
-  HIPCHK(hipMalloc(&c->d_iblank_ref, c->nref));
-  HIPCHK(hipMemcpy(c->d_iblank_ref, c->h_iblank.data(), c->nref, hipMemcpyHostToDevice));
   c->have_geom = true;
   auto t4 = now();
   rc = compute_null_vector(c);
@@ -2008,11 +2013,16 @@ int32_t pp_level_data(pp_ctx *c, int32_t level, uint8_t *flags, double *w, doubl
   if (!c->have_geom) { set_error("pp_level_data: no geometry"); return PP_NO_GEOMETRY; }
   if (level < 0 || level >= (int)c->lv.size()) { set_error("pp_level_data: bad level %d", level); return PP_BAD_ARG; }
   const Level &L = c->lv[level];
-  if (flags)
+  if (flags) {
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<uint8_t> hf(L.total);
+    HIPCHK(hipMemcpyAsync(hf.data(), L.flags, L.total, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     for (int k = 0; k < L.cz; k++)
       for (int j = 0; j < L.cy; j++)
         for (int i = 0; i < L.cx; i++)
-          flags[i + (long)L.cx * (j + (long)L.cy * k)] = L.hflags[lidx(L.sx, L.sxy, i, j, k)];
+          flags[i + (long)L.cx * (j + (long)L.cy * k)] = hf[lidx(L.sx, L.sxy, i, j, k)];
+  }
   double *outs[4] = {w, xc, cm, cp};
   const std::vector<double> *src[4] = {L.w, L.xc, L.cm, L.cp};
   for (int q = 0; q < 4; q++) {

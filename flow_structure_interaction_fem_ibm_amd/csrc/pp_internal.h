@@ -169,7 +169,7 @@ struct pp_ctx {
   std::vector<double> rxc[3], rdx[3], rdxinv[3], rdxc[3], rdxcinv[3];
   std::vector<pp::Level> lv;
   bool have_geom = false;
-  std::vector<int8_t> h_iblank, h_ghost, h_fresh;  // reference layout
+  std::vector<int8_t> h_iblank, h_fresh;  // reference layout
   pp::GhostTables gh;
   pp::CoarseSolve cs;
   void *blas = nullptr;                // rocblas_handle of the coarse factorisation
@@ -180,6 +180,7 @@ struct pp_ctx {
   double *d_tmp = nullptr;
   int8_t *d_fresh = nullptr;
   int8_t *d_iblank_ref = nullptr;
+  unsigned long long *d_count = nullptr;  // set-up counter (ghost cells of the iblank)
   // reductions
   double *d_part = nullptr;            // partial sums
   double *d_part2 = nullptr;           // first-stage results of the two-stage reductions

@@ -19,12 +19,12 @@ long coarsest_cells(int nxc, int nyc, int nzc);
 constexpr long kCoarseDenseMax = 16384;
 // longest line the x-line (wavefront) and y/z-line (partitioned) solvers hold
 constexpr int kMaxLine = 1024;
-void build_flags(pp_ctx *c);
+int build_flags(pp_ctx *c, const int8_t *d_iblank, long *n_ghost);
 int upload_levels(pp_ctx *c);
-int upload_flags(pp_ctx *c);
 void free_levels(pp_ctx *c);
 void free_ghosts(pp_ctx *c);
-void derive_ghost_mark(pp_ctx *c);
+// whether reference offset r is a ghost cell of c->h_iblank
+bool ref_is_ghost(const pp_ctx *c, long r);
 int build_ghosts(pp_ctx *c, const pp_geometry *g);
 int build_coarse(pp_ctx *c);
 }  // namespace pp

@@ -239,90 +239,131 @@ int build_levels(pp_ctx *c) {
   return PP_OK;
 }
 
-// Flags of every level.  Level 0: blank/ghost from the geometry, closed
-// faces on the outer boundary only (set_outer_iup, BOUNDARY_SET.F90:54-92;
-// under GCM the fine internal faces stay open, so fluid cells couple to the
-// ghost values).  Coarse levels: blank if every child is blank, no ghosts,
-// a face closed where a fluid cell meets a blank neighbour
-// (MG_Prepare, CORE_MG.F90:1736-1781) or the outer boundary.
-void build_flags(pp_ctx *c) {
-  for (size_t l = 0; l < c->lv.size(); l++) {
-    Level &L = c->lv[l];
-    L.hflags.assign(L.total, (uint8_t)(FL_BLANK));
-    std::vector<uint8_t> blank((size_t)L.cx * L.cy * L.cz, 0), ghost((size_t)L.cx * L.cy * L.cz, 0);
-    auto cidx = [&](int i, int j, int k) { return (size_t)i + (size_t)L.cx * (j + (size_t)L.cy * k); };
-    if (l == 0) {
-      host_parallel(L.cz, [&](long k0, long k1) {
-        for (int k = (int)k0; k < (int)k1; k++)
-          for (int j = 0; j < L.cy; j++)
-            for (int i = 0; i < L.cx; i++) {
-              long r = ridx(c->ref, i, j, k);
-              blank[cidx(i, j, k)] = c->h_iblank[r] ? 1 : 0;
-              ghost[cidx(i, j, k)] = c->h_ghost[r] ? 1 : 0;
-            }
-      });
-    } else {
-      Level &F = c->lv[l - 1];
-      const std::vector<uint8_t> &fh = F.hflags;
-      host_parallel(L.cz, [&](long k0, long k1) {
-        for (int k = (int)k0; k < (int)k1; k++)
-          for (int j = 0; j < L.cy; j++)
-            for (int i = 0; i < L.cx; i++) {
-              // children: cf per direction, the last coarse cell of a
-              // direction also takes the fine cell an odd count leaves over
-              const int i0 = i * L.cf[0], i1 = i == L.cx - 1 ? F.cx : i0 + L.cf[0];
-              const int j0 = j * L.cf[1], j1 = j == L.cy - 1 ? F.cy : j0 + L.cf[1];
-              const int k0c = k * L.cf[2], k1c = k == L.cz - 1 ? F.cz : k0c + L.cf[2];
-              int all = 1;
-              for (int kk = k0c; kk < k1c; kk++)
-                for (int jj = j0; jj < j1; jj++)
-                  for (int ii = i0; ii < i1; ii++) all &= (fh[lidx(F.sx, F.sxy, ii, jj, kk)] & FL_BLANK) ? 1 : 0;
-              blank[cidx(i, j, k)] = (uint8_t)all;
-            }
-      });
+// Flags of every level, built on the device.  Level 0: blank/ghost from the
+// geometry (a ghost is a blank cell with a fluid face neighbour, neighbours
+// clamped to the interior, GCM_SET_BOUNDARY.F90:49-73), closed faces on the
+// outer boundary only (set_outer_iup, BOUNDARY_SET.F90:54-92; under GCM the
+// fine internal faces stay open, so fluid cells couple to the ghost values).
+// Coarse levels: blank if every child is blank, no ghosts, a face closed
+// where a fluid cell meets a blank neighbour (MG_Prepare,
+// CORE_MG.F90:1736-1781) or the outer boundary.  Padding and halo cells
+// are blank.
+__global__ void k_flags_level0(RefArgs R, int sx, long sxy, const int8_t *__restrict__ ib, uint8_t *__restrict__ fl,
+                               unsigned long long *__restrict__ nghost) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (i >= R.nxc) return;
+  // reference (1-based) indices of the cell and its clamped neighbours
+  const int I = i + 1, J = j + 1, K = k + 1;
+  auto at = [&](int a, int b, int c) { return ib[(long)a + R.ex * b + R.exy * c]; };
+  const bool bl = at(I, J, K) != 0;
+  uint8_t f = 0;
+  if (bl) {
+    f |= FL_BLANK;
+    const int iM = max(I - 1, 1), iP = min(I + 1, R.nxc), jM = max(J - 1, 1), jP = min(J + 1, R.nyc);
+    const int kM = max(K - 1, 1), kP = min(K + 1, R.nzc);
+    if (at(iM, J, K) == 0 || at(iP, J, K) == 0 || at(I, jM, K) == 0 || at(I, jP, K) == 0 || at(I, J, kM) == 0 ||
+        at(I, J, kP) == 0) {
+      f |= FL_GHOST;
+      atomicAdd(nghost, 1ull);
     }
-    host_parallel(L.cz, [&](long k0, long k1) {
-    for (int k = (int)k0; k < (int)k1; k++)
-      for (int j = 0; j < L.cy; j++)
-        for (int i = 0; i < L.cx; i++) {
-          uint8_t f = 0;
-          bool bl = blank[cidx(i, j, k)];
-          if (bl) f |= FL_BLANK;
-          if (ghost[cidx(i, j, k)]) f |= FL_GHOST;
-          if (i == 0) f |= FL_IM;
-          if (i == L.cx - 1) f |= FL_IP;
-          if (j == 0) f |= FL_JM;
-          if (j == L.cy - 1) f |= FL_JP;
-          if (k == 0) f |= FL_KM;
-          if (k == L.cz - 1) f |= FL_KP;
-          if (l > 0 && !bl) {
-            if (i > 0 && blank[cidx(i - 1, j, k)]) f |= FL_IM;
-            if (i < L.cx - 1 && blank[cidx(i + 1, j, k)]) f |= FL_IP;
-            if (j > 0 && blank[cidx(i, j - 1, k)]) f |= FL_JM;
-            if (j < L.cy - 1 && blank[cidx(i, j + 1, k)]) f |= FL_JP;
-            if (k > 0 && blank[cidx(i, j, k - 1)]) f |= FL_KM;
-            if (k < L.cz - 1 && blank[cidx(i, j, k + 1)]) f |= FL_KP;
-          }
-          L.hflags[lidx(L.sx, L.sxy, i, j, k)] = f;
-        }
-    });
   }
+  if (i == 0) f |= FL_IM;
+  if (i == R.nxc - 1) f |= FL_IP;
+  if (j == 0) f |= FL_JM;
+  if (j == R.nyc - 1) f |= FL_JP;
+  if (k == 0) f |= FL_KM;
+  if (k == R.nzc - 1) f |= FL_KP;
+  fl[lidx(sx, sxy, i, j, k)] = f;
 }
 
-// a new geometry on an existing hierarchy: only the flags change
-int upload_flags(pp_ctx *c) {
-  for (auto &L : c->lv)
-    if (hipMemcpy(L.flags, L.hflags.data(), L.hflags.size(), hipMemcpyHostToDevice) != hipSuccess) {
-      set_error("flag upload failed");
+// a coarse cell is blank when every child is (the last coarse cell of a
+// direction also takes the fine cell an odd count leaves over)
+__global__ void k_flags_coarse_blank(int fsx, long fsxy, int fcx, int fcy, int fcz, const uint8_t *__restrict__ ffl,
+                                     int cx, int cy, int cz, int cf0, int cf1, int cf2, uint8_t *__restrict__ blank) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (i >= cx) return;
+  const int i0 = i * cf0, i1 = i == cx - 1 ? fcx : i0 + cf0;
+  const int j0 = j * cf1, j1 = j == cy - 1 ? fcy : j0 + cf1;
+  const int k0 = k * cf2, k1 = k == cz - 1 ? fcz : k0 + cf2;
+  int all = 1;
+  for (int kk = k0; kk < k1; kk++)
+    for (int jj = j0; jj < j1; jj++)
+      for (int ii = i0; ii < i1; ii++) all &= (ffl[lidx(fsx, fsxy, ii, jj, kk)] & FL_BLANK) ? 1 : 0;
+  blank[i + (long)cx * (j + (long)cy * k)] = (uint8_t)all;
+}
+
+__global__ void k_flags_coarse(int sx, long sxy, int cx, int cy, int cz, const uint8_t *__restrict__ blank,
+                               uint8_t *__restrict__ fl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (i >= cx) return;
+  auto b = [&](int a, int c, int d) { return blank[a + (long)cx * (c + (long)cy * d)] != 0; };
+  const bool bl = b(i, j, k);
+  uint8_t f = bl ? FL_BLANK : 0;
+  if (i == 0 || (!bl && b(i - 1, j, k))) f |= FL_IM;
+  if (i == cx - 1 || (!bl && b(i + 1, j, k))) f |= FL_IP;
+  if (j == 0 || (!bl && b(i, j - 1, k))) f |= FL_JM;
+  if (j == cy - 1 || (!bl && b(i, j + 1, k))) f |= FL_JP;
+  if (k == 0 || (!bl && b(i, j, k - 1))) f |= FL_KM;
+  if (k == cz - 1 || (!bl && b(i, j, k + 1))) f |= FL_KP;
+  fl[lidx(sx, sxy, i, j, k)] = f;
+}
+
+// The flags of every level from the reference-layout iblank on the device
+// (d_iblank); the coarsest level's are copied back for build_coarse, and
+// *n_ghost is the number of ghost cells the iblank has.
+int build_flags(pp_ctx *c, const int8_t *d_iblank, long *n_ghost) {
+  hipStream_t s = c->stream;
+  if (!c->d_count && hipMalloc(&c->d_count, sizeof(unsigned long long)) != hipSuccess) {
+    set_error("counter allocation failed");
+    return PP_HIP_ERROR;
+  }
+  if (hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), s) != hipSuccess) {
+    set_error("counter reset failed");
+    return PP_HIP_ERROR;
+  }
+  for (size_t l = 0; l < c->lv.size(); l++) {
+    Level &L = c->lv[l];
+    if (hipMemsetAsync(L.flags, FL_BLANK, L.total, s) != hipSuccess) {
+      set_error("flag fill failed");
       return PP_HIP_ERROR;
     }
+    dim3 grid((L.cx + 63) / 64, L.cy, L.cz), block(64);
+    if (l == 0) {
+      k_flags_level0<<<grid, block, 0, s>>>(c->ref, L.sx, L.sxy, d_iblank, L.flags, c->d_count);
+    } else {
+      const Level &F = c->lv[l - 1];
+      // the coarse level's FP32 residual buffer holds the blank mask (free at set-up)
+      uint8_t *blank = (uint8_t *)L.res32;
+      k_flags_coarse_blank<<<grid, block, 0, s>>>(F.sx, F.sxy, F.cx, F.cy, F.cz, F.flags, L.cx, L.cy, L.cz, L.cf[0],
+                                                 L.cf[1], L.cf[2], blank);
+      k_flags_coarse<<<grid, block, 0, s>>>(L.sx, L.sxy, L.cx, L.cy, L.cz, blank, L.flags);
+      if (hipMemsetAsync(blank, 0, (size_t)L.cx * L.cy * L.cz, s) != hipSuccess) {
+        set_error("residual buffer reset failed");
+        return PP_HIP_ERROR;
+      }
+    }
+  }
+  if (hipGetLastError() != hipSuccess) {
+    set_error("flag kernels failed to launch");
+    return PP_HIP_ERROR;
+  }
+  Level &C = c->lv.back();
+  C.hflags.resize(C.total);
+  unsigned long long ng = 0;
+  if (hipMemcpyAsync(C.hflags.data(), C.flags, C.total, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&ng, c->d_count, sizeof(ng), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("flag copy failed");
+    return PP_HIP_ERROR;
+  }
+  *n_ghost = (long)ng;
   return PP_OK;
 }
 
 int upload_levels(pp_ctx *c) {
   for (size_t l = 0; l < c->lv.size(); l++) {
     Level &L = c->lv[l];
-    L.flags = dev_upload(L.hflags);
+    if (hipMalloc(&L.flags, L.total) != hipSuccess) L.flags = nullptr;
     for (int d = 0; d < 3; d++) {
       L.cm64[d] = dev_upload(L.cm[d]); L.cp64[d] = dev_upload(L.cp[d]); L.w64[d] = dev_upload(L.w[d]);
       L.cm32[d] = dev_upload(to_f32(L.cm[d])); L.cp32[d] = dev_upload(to_f32(L.cp[d]));
@@ -346,25 +387,46 @@ int upload_levels(pp_ctx *c) {
 }
 
 // Ghost marking exactly as GCM_set_internal_boundary (GCM_SET_BOUNDARY.F90:49-73):
-// a blank cell with a fluid 6-neighbour, neighbours clamped to [1,n-1].
-void derive_ghost_mark(pp_ctx *c) {
+// reference offset r is a ghost cell when it is an interior blank cell with
+// a fluid 6-neighbour, neighbours clamped to the interior (the rule of
+// k_flags_level0, for the cells the host set-up looks at)
+bool ref_is_ghost(const pp_ctx *c, long r) {
   const auto &ib = c->h_iblank;
-  c->h_ghost.assign(c->nref, 0);
-  int nx = c->nxc + 1, ny = c->nyc + 1, nz = c->nzc + 1;
+  const int K = (int)(r / c->ref.exy), J = (int)((r % c->ref.exy) / c->ref.ex), I = (int)(r % c->ref.ex);
+  if (I < 1 || I > c->nxc || J < 1 || J > c->nyc || K < 1 || K > c->nzc || ib[r] != 1) return false;
   auto R = [&](int i, int j, int k) { return (long)i + c->ref.ex * j + c->ref.exy * k; };
-  host_parallel(nz - 1, [&](long k0, long k1) {
-    for (int k = (int)k0 + 1; k <= (int)k1; k++)
-      for (int j = 1; j <= ny - 1; j++)
-        for (int i = 1; i <= nx - 1; i++) {
-          int iM = std::max(i - 1, 1), iP = std::min(i + 1, nx - 1);
-          int jM = std::max(j - 1, 1), jP = std::min(j + 1, ny - 1);
-          int kM = std::max(k - 1, 1), kP = std::min(k + 1, nz - 1);
-          int b = ib[R(i, j, k)];
-          if (b == 1 && (ib[R(iM, j, k)] == 0 || ib[R(iP, j, k)] == 0 || ib[R(i, jM, k)] == 0 ||
-                         ib[R(i, jP, k)] == 0 || ib[R(i, j, kM)] == 0 || ib[R(i, j, kP)] == 0))
-            c->h_ghost[R(i, j, k)] = 1;
-        }
-  });
+  const int iM = std::max(I - 1, 1), iP = std::min(I + 1, c->nxc), jM = std::max(J - 1, 1),
+            jP = std::min(J + 1, c->nyc), kM = std::max(K - 1, 1), kP = std::min(K + 1, c->nzc);
+  return ib[R(iM, J, K)] == 0 || ib[R(iP, J, K)] == 0 || ib[R(I, jM, K)] == 0 || ib[R(I, jP, K)] == 0 ||
+         ib[R(I, J, kM)] == 0 || ib[R(I, J, kP)] == 0;
+}
+
+// std::stable_sort of v by cmp on the host threads: contiguous chunks
+// sorted in parallel, then merged pairwise (std::merge keeps the first
+// range's element first among equals, so the result is the stable order)
+template <typename T, typename Cmp>
+static void par_stable_sort(std::vector<T> &v, Cmp cmp) {
+  const long n = (long)v.size();
+  const long parts = n < 8192 ? 1 : 16;
+  if (parts == 1) { std::stable_sort(v.begin(), v.end(), cmp); return; }
+  std::vector<long> cut(parts + 1);
+  for (long p = 0; p <= parts; p++) cut[p] = n * p / parts;
+  auto each = [](long m, const std::function<void(long)> &fn) {  // one thread per item
+    std::vector<std::thread> th;
+    for (long p = 0; p < m; p++) th.emplace_back([&fn, p] { fn(p); });
+    for (auto &t : th) t.join();
+  };
+  each(parts, [&](long p) { std::stable_sort(v.begin() + cut[p], v.begin() + cut[p + 1], cmp); });
+  std::vector<T> buf(n);
+  for (long width = 1; width < parts; width *= 2) {
+    const long pairs = (parts + 2 * width - 1) / (2 * width);
+    each(pairs, [&](long p) {
+      const long lo = cut[std::min(parts, 2 * width * p)], mid = cut[std::min(parts, 2 * width * p + width)];
+      const long hi = cut[std::min(parts, 2 * width * (p + 1))];
+      std::merge(v.begin() + lo, v.begin() + mid, v.begin() + mid, v.begin() + hi, buf.begin() + lo, cmp);
+    });
+    v.swap(buf);
+  }
 }
 
 static const int incI[8] = {0, 0, 1, 1, 0, 0, 1, 1};
@@ -388,7 +450,7 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     int iG = g->i_ghost[q], jG = g->j_ghost[q], kG = g->k_ghost[q];
     if (!inb(iG, jG, kG)) { set_error("ghost %d index (%d,%d,%d) outside the grid", q, iG, jG, kG); return PP_BAD_GEOMETRY; }
     long rg = ridx(c->ref, iG - 1, jG - 1, kG - 1);
-    if (c->h_ghost[rg] != 1) {
+    if (!ref_is_ghost(c, rg)) {
       set_error("ghost %d at (%d,%d,%d) is not a ghost cell of iblank", q, iG, jG, kG);
       return PP_BAD_GEOMETRY;
     }
@@ -416,7 +478,7 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
       int ci = std::min(std::max(ii, 1), c->nxc), cj = std::min(std::max(jj, 1), c->nyc),
           ck = std::min(std::max(kk, 1), c->nzc);
       long rs = ridx(c->ref, ci - 1, cj - 1, ck - 1);
-      if (c->h_iblank[rs] == 1 && c->h_ghost[rs] != 1) G.solid_in_stencil++;
+      if (c->h_iblank[rs] == 1 && !ref_is_ghost(c, rs)) G.solid_in_stencil++;
       G.h_coef[8L * q + r] = cf;
       G.h_s_ref[8L * q + r] = rs;
       G.h_s_al[8L * q + r] = lidx(L0.sx, L0.sxy, ci - 1, cj - 1, ck - 1);
@@ -511,13 +573,16 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     std::vector<Row> cur = cgf, nxt(n);
     bool coupled = !G.h_gg_idx.empty();
     for (int t = 1; coupled && t < G.kexact; t++) {
-      for (int q = 0; q < n; q++) {
-        Row r = cgf[q];
-        for (int e = G.h_gg_ptr[q]; e < G.h_gg_ptr[q + 1]; e++)
-          for (auto &x : cur[G.h_gg_idx[e]]) r.push_back({x.first, G.h_gg_w[e] * x.second});
-        merge(r);
-        nxt[q].swap(r);
-      }
+      // rows are independent within a step: ghosts in parallel
+      host_parallel(n, [&](long a, long b) {
+        for (long q = a; q < b; q++) {
+          Row r = cgf[q];
+          for (int e = G.h_gg_ptr[q]; e < G.h_gg_ptr[q + 1]; e++)
+            for (auto &x : cur[G.h_gg_idx[e]]) r.push_back({x.first, G.h_gg_w[e] * x.second});
+          merge(r);
+          nxt[q].swap(r);
+        }
+      });
       cur.swap(nxt);
     }
     G.h_c_ptr.assign(1, 0); G.h_c_ref.clear(); G.h_c_al.clear(); G.h_c_w.clear();
@@ -571,10 +636,10 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
         double cf = G.h_coef[8L * q + r];
         long rs = G.h_s_ref[8L * q + r];
         if (cf == 0.0 || rs == G.h_g_ref[q]) continue;
-        if (c->h_iblank[rs] == 1 && c->h_ghost[rs] != 1) continue;  // solid cells are not unknowns
+        if (c->h_iblank[rs] == 1 && !ref_is_ghost(c, rs)) continue;  // solid cells are not unknowns
         ent.push_back({rs, {q, cf}});
       }
-    std::stable_sort(ent.begin(), ent.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    par_stable_sort(ent, [](const auto &a, const auto &b) { return a.first < b.first; });
     G.h_t_ref.clear(); G.h_t_al.clear(); G.h_t_ptr.assign(1, 0); G.h_t_ghost.clear(); G.h_t_coef.clear();
     G.h_t_vinv.clear();
     G.h_g_tidx.assign(n, -1);
@@ -600,7 +665,7 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     std::vector<std::pair<long, std::pair<int, double>>> ct;
     for (int q = 0; q < n; q++)
       for (int e = G.h_c_ptr[q]; e < G.h_c_ptr[q + 1]; e++) ct.push_back({G.h_c_ref[e], {q, G.h_c_w[e]}});
-    std::stable_sort(ct.begin(), ct.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    par_stable_sort(ct, [](const auto &a, const auto &b) { return a.first < b.first; });
     G.h_ct_ref.clear(); G.h_ct_al.clear(); G.h_ct_ptr.assign(1, 0); G.h_ct_ghost.clear(); G.h_ct_w.clear();
     G.h_ct_vinv.clear();
     for (size_t e = 0; e < ct.size(); e++) {
