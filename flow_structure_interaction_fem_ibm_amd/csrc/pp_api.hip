@@ -517,7 +517,7 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
     // the dense FP64 solution operator in both precisions (the coarsest
     // solve is one of the FP64 parts of the mixed scheme)
     kbegin(c, PP_K_COARSE, l, 8.0 * n * n + 2.0 * sizeof(T) * n);
-    k_coarse_mv<T, double><<<(n + 3) / 4, 256, sizeof(double) * n, st>>>(n, cs.cell, cs.G, lv_rhs<T>(L),
+    k_coarse_mv<T, double><<<(n + kCoarseRows - 1) / kCoarseRows, 64 * kCoarseRows, sizeof(double) * n, st>>>(n, cs.cell, cs.G, lv_rhs<T>(L),
                                                                         lv_e<T>(L));
     kend(c);
     return;
@@ -1298,7 +1298,6 @@ int compute_null_vector(pp_ctx *c) {
   hipStream_t s = c->stream;
   LevelArgs<double> a0 = largs<double>(c->lv[0]);
   CellLaunch cl = ref_launch(c);
-  const bool warm = c->d_mu_prev != nullptr;
   // vsum = sum of fluid volumes
   k_fluid_sum_vol<256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, nullptr, c->d_part);
   int rc = reduce_sums_host(c, 2);
@@ -1306,11 +1305,13 @@ int compute_null_vector(pp_ctx *c) {
   const double vsum = c->h_res[1];
   HIPCHK(hipMemsetAsync(c->d_w, 0, sizeof(double) * c->nref, s));
   if (c->gh.n > 0) {
-    // mu: the previous geometry's null vector where it exists (moving
-    // bodies), the fluid indicator elsewhere; s = 0
+    // mu: the fluid indicator; s = 0.  Every geometry starts cold: a moving
+    // body's previous null vector as the start took 22-23 transposed
+    // iterations on the 384x256x256 moving sphere, a cold start 19, and the
+    // result then depends on this geometry alone
     HIPCHK(hipMemsetAsync(c->d_mu, 0, sizeof(double) * c->nref, s));
     HIPCHK(hipMemsetAsync(c->d_sv, 0, sizeof(double) * c->nref, s));
-    k_null_init<<<cl.grid, cl.block, 0, s>>>(c->ref, a0, warm ? c->d_mu_prev : nullptr, c->d_mu);
+    k_null_init<<<cl.grid, cl.block, 0, s>>>(c->ref, a0, nullptr, c->d_mu);
     pp_solve_opts o;
     pp_default_opts(&o);
     o.precision = PP_PREC_MIXED;
@@ -1330,44 +1331,29 @@ int compute_null_vector(pp_ctx *c) {
     pp_solve_stats st;
     int lam = 0;
     // The transposed cycles' coarse correction takes the null vector's
-    // shape: a warm start takes it from its initial iterate (the previous
-    // geometry's null vector on this geometry's fluid cells, 1 on the fresh
-    // ones; the previous shape would be 0 there); a cold start first runs a
-    // few cycles without it and takes the shape of that iterate.
-    double ref = warm ? c->null_ref : -1.0;
-    int stage1 = 0;
+    // shape: a first few cycles run without it, the shape of that iterate
+    // then shapes the correction of the rest.
     c->have_nshape = false;
-    if (warm) {
-      rc = null_shape_from_mu(c, vsum);
-      if (rc) return rc;
-      c->have_nshape = c->d_nshape32 != nullptr;
-    }
-    if (!c->have_nshape) {
-      pp_solve_opts o1 = o;
-      o1.max_outer = 3;
-      rc = adjoint_impl<float>(c, nullptr, &o1, -1.0, &st, &lam);
-      if (rc != PP_OK && rc != PP_NOT_CONVERGED) return rc;
-      ref = st.b_l2;
-      stage1 = st.outer_iterations;
-      rc = null_shape_from_mu(c, vsum);
-      if (rc) return rc;
-      c->have_nshape = c->d_nshape32 != nullptr;
-      o.max_outer -= stage1;  // the bound counts both stages
-    }
-    // a warm start converges relative to the cold start's initial residual
-    // (of the previous geometry), not to its own much smaller one
+    pp_solve_opts o1 = o;
+    o1.max_outer = 3;
+    rc = adjoint_impl<float>(c, nullptr, &o1, -1.0, &st, &lam);
+    if (rc != PP_OK && rc != PP_NOT_CONVERGED) return rc;
+    const double ref = st.b_l2;
+    const int stage1 = st.outer_iterations;
+    rc = null_shape_from_mu(c, vsum);
+    if (rc) return rc;
+    c->have_nshape = c->d_nshape32 != nullptr;
+    o.max_outer -= stage1;  // the bound counts both stages
+    // the second stage converges relative to the first one's initial residual
     rc = adjoint_impl<float>(c, nullptr, &o, ref, &st, &lam);
     if (rc != PP_OK && rc != PP_NOT_CONVERGED) return rc;
     st.outer_iterations += stage1;
-    if (!warm) st.b_l2 = ref;
+    st.b_l2 = ref;
     if (!(st.rel_l2 <= accept)) {
       set_error("left null vector did not converge (relative residual %g after %d iterations, %d ghost "
                 "stencil entries on solid cells)", st.rel_l2, st.outer_iterations, c->gh.solid_in_stencil);
       return PP_BAD_GEOMETRY;
     }
-    if (!c->d_mu_prev) HIPCHK(hipMalloc(&c->d_mu_prev, sizeof(double) * c->nref));
-    HIPCHK(hipMemcpyAsync(c->d_mu_prev, c->d_mu, sizeof(double) * c->nref, hipMemcpyDeviceToDevice, s));
-    if (!warm) c->null_ref = st.b_l2;
     // w = V mu normalised to sum_f w = sum_f V; ghost entries lambda_g likewise
     k_mu_vol_sums<256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, nullptr, c->d_part);
     rc = reduce_sums_host(c, 2);
@@ -1531,7 +1517,7 @@ int32_t pp_destroy(pp_ctx *c) {
   free_levels(c);
   free_ghosts(c);
   hipFree(c->d_p); hipFree(c->d_b); hipFree(c->d_w); hipFree(c->d_bwork); hipFree(c->d_tmp);
-  hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_count); hipFree(c->d_mu); hipFree(c->d_sv); hipFree(c->d_mu_prev);
+  hipFree(c->d_fresh); hipFree(c->d_iblank_ref); hipFree(c->d_count); hipFree(c->d_mu); hipFree(c->d_sv);
   hipFree(c->d_ad_coef);
   hipFree(c->d_nshape32); hipFree(c->d_nshape64);
   free_fresh_rows(c);

@@ -189,7 +189,6 @@ struct pp_ctx {
   int nblk_red = 0;
   bool adjoint_mode = false;
   double setup_ms[4] = {};             // last pp_set_geometry: flags, ghost tables, coarse operator, null vector
-  double *d_mu_prev = nullptr;         // last null vector in mu = w/V form (warm start of the next one)
   // w/V of the null vector in the level-0 layout (FP32, FP64): the adjoint
   // V-cycle scales its level-0 coarse-grid correction by it
   float *d_nshape32 = nullptr;
@@ -201,7 +200,6 @@ struct pp_ctx {
   double ad_dt = 0.0;
   bool have_ad = false;
   pp::FreshRows fr;                    // fresh-cell rows of the current geometry
-  double null_ref = 1.0;               // initial residual norm of the cold-started null vector
   int null_iters = 0;
   double null_res = 0.0;           // level-0 smoother uses the transposed ghost rows
   std::vector<double *> d_kry;         // flexible GMRES (FP64 path): m+1 basis + m preconditioned vectors
