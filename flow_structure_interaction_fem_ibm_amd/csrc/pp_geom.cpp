@@ -464,6 +464,7 @@ int32_t pp_geom_build(pp_geom **out, const pp_grid_desc *g, int32_t nbody, const
   return build_geometry(out, G, mode, bs, nullptr, probe_norm, iblank_prev);
 }
 
+#ifndef PP_HOST_ONLY  // the host-only sanitizer build has no device code
 // the triangulated bodies of pp_geom_build's arguments, or a status
 static int32_t tri_bodies_of(const char *who, const Grid &G, int32_t nbody, const pp_body_desc *bodies,
                              TriBodies &tri) {
@@ -529,6 +530,8 @@ int32_t pp_geom_build_surfaces_device(pp_geom **out, const pp_grid_desc *g, int3
   }
   return build_geometry_device(out, G, tri, probe_norm, iblank_prev, g->device);
 }
+
+#endif
 
 int32_t pp_geom_build_surfaces(pp_geom **out, const pp_grid_desc *g, int32_t n_surface, const pp_surface_desc *s,
                                double probe_norm, const int8_t *iblank_prev) {
