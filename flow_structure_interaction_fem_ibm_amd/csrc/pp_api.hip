@@ -1160,6 +1160,84 @@ struct AdjOps {
   }
 };
 
+// The same operator on the FP32 level-0 basis of the mixed forward solve
+// (four cells per thread): w = A z by k_kryl_apply4 (z = e, which is 0 on
+// blank cells in adjoint mode), then the transposed ghost coupling of z on
+// its touched cells with the matching dot corrections.  For FP32
+// V-cycles on levels whose rows split into fours.
+struct AdjOps32 {
+  using V = KryVecsF;
+  pp_ctx *c;
+  const pp_solve_opts *o;
+  pp_solve_stats *S;
+  double cells;
+  float *u(int q) { return c->d_kryl[q]; }
+  float *z(int q) { return c->d_kryl[c->kryl_m + 1 + q]; }
+  CellLaunch launch() const {
+    const Level &L0 = c->lv[0];
+    return cell_launch(L0.cx / 4, L0.cy, L0.cz);
+  }
+  long apply_blocks() const {
+    CellLaunch cl = launch();
+    return (long)cl.grid.x * cl.grid.y * cl.grid.z;
+  }
+  long coupling_blocks() const { return (c->gh.nct + 255) / 256; }
+  int blocks() const { return (int)(apply_blocks() + coupling_blocks()); }
+  // the dots of both launches fit the partial-sum buffer
+  static bool fits(pp_ctx *c) {
+    if (c->lv[0].cx % 4 != 0) return false;
+    AdjOps32 t{c, nullptr, nullptr, 0.0};
+    return t.blocks() <= c->nblk_red;
+  }
+  int precondition() {
+    adj_ghost_base<float>(c);
+    c->adjoint_mode = true;
+    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
+      vcycle<float>(c, 0, o);
+      S->vcycles++;
+    }
+    c->adjoint_mode = false;
+    return PP_OK;
+  }
+  void apply(int j, const V &vv) {
+    Level &L0 = c->lv[0];
+    GhostTables &g = c->gh;
+    CellLaunch cl = launch();
+    kbegin(c, PP_K_KRYLOV, 0, cells * (4.0 + 1.0 + 4.0 + 4.0 + 4.0 * (j + 1)));
+    k_kryl_apply4<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), lv_e<float>(L0), z(j), u(j + 1),
+                                                            vv, c->d_part);
+    kend(c);
+    // with no touched cells the coupling launch still writes its (zero) partial block
+    kbegin(c, PP_K_GHOST, 0, (double)g.n * 6.0 * 12.0 + (double)g.nct * (8.0 + 8.0 + 4.0 * (j + 2)));
+    if (g.n > 0)
+      k_adj_ghost_rhs<float><<<dim3((g.n + 255) / 256), 256, 0, c->stream>>>(g.n, g.nb_al, g.nb_w32, nullptr,
+                                                                             lv_e<float>(L0), g.rhsg32);
+    k_kryl_adj_coupling<256><<<dim3((unsigned)std::max<long>(coupling_blocks(), 1)), 256, 0, c->stream>>>(
+        g.nct, g.ct_al, g.ct_ptr, g.ct_ghost, g.ct_w32, g.ct_vinv32, g.rhsg32, u(j + 1), vv, c->d_part,
+        apply_blocks());
+    kend(c);
+  }
+  void orth(int j, const V &vv) {
+    Level &L0 = c->lv[0];
+    CellLaunch cl = launch();
+    kbegin(c, PP_K_KRYLOV, 0, cells * (1.0 + 8.0 + 4.0 * (j + 1) + 8.0));
+    k_kryl_orth4<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), u(j + 1), vv, lv_rhs<float>(L0),
+                                                           lv_e<float>(L0), c->d_part);
+    kend(c);
+    // the orthogonalisation's norm is reduced over the apply launch's blocks
+    // only: zero the coupling blocks' slots it does not write
+    const long nb = coupling_blocks();
+    if (nb > 0) k_fill<double><<<dim3((unsigned)((nb + 255) / 256)), 256, 0, c->stream>>>(
+        c->d_part + apply_blocks(), (int)nb, 0.0);
+  }
+  void update(const V &vv) {
+    CellLaunch cl = ref_launch(c);
+    kbegin(c, PP_K_KRYLOV, 0, cells * (17.0 + 4.0 * vv.n));
+    k_kryl_update<<<cl.grid, cl.block, 0, c->stream>>>(c->ref, largs<double>(c->lv[0]), c->d_mu, vv);
+    kend(c);
+  }
+};
+
 template <typename T>
 int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref_norm, pp_solve_stats *st,
                  int *lam_idx) {
@@ -1178,8 +1256,10 @@ int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref
   HIPCHK(hipEventCreate(&ev1));
   HIPCHK(hipEventRecord(ev0, s));
   const int m = o->krylov;
+  // FP32 V-cycles use the FP32 level-0 basis where the rows split into fours
+  const bool kry32 = m > 0 && sizeof(T) == 4 && AdjOps32::fits(c);
   if (m > 0) {
-    int rc = ensure_krylov(c, m);
+    int rc = kry32 ? ensure_krylov_level(c, m) : ensure_krylov(c, m);
     if (rc) return rc;
   }
   HIPCHK(hipMemcpyAsync(c->d_bwork, c->d_sv, sizeof(double) * c->nref, hipMemcpyDeviceToDevice, s));
@@ -1204,8 +1284,9 @@ int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref
     }
     kbegin(c, PP_K_OUTER_RESIDUAL, 0, cells * (17.0 + 2.0 * sizeof(T)));
     k_outer_residual<T, 256><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, c->d_bwork, lv_rhs<T>(L0),
-                                                          lv_e<T>(L0), c->d_part, m > 0 ? c->d_kry[0] : nullptr,
-                                                          (T *)nullptr);
+                                                          lv_e<T>(L0), c->d_part,
+                                                          (m > 0 && !kry32) ? c->d_kry[0] : nullptr,
+                                                          kry32 ? (T *)c->d_kryl[0] : (T *)nullptr);
     kend(c);
     int rc = reduce_summax(c, (int)nblk);
     if (rc) return rc;
@@ -1223,8 +1304,14 @@ int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref
     if ((o->rel_tol > 0.0 && rel <= o->rel_tol) || rl2 == 0.0) { status = PP_OK; break; }
     if (outer >= o->max_outer) break;
     if (m > 0) {
-      AdjOps<T> ops{c, o, &S, cells};
-      int krc = gmres_cycle(c, o, ops, rl2, ref_norm > 0.0 ? ref_norm : 1.0, outer, S);
+      int krc;
+      if (kry32) {
+        AdjOps32 ops{c, o, &S, cells};
+        krc = gmres_cycle(c, o, ops, rl2, ref_norm > 0.0 ? ref_norm : 1.0, outer, S);
+      } else {
+        AdjOps<T> ops{c, o, &S, cells};
+        krc = gmres_cycle(c, o, ops, rl2, ref_norm > 0.0 ? ref_norm : 1.0, outer, S);
+      }
       if (krc == PP_NOT_CONVERGED || krc == PP_NONFINITE) { status = krc; break; }
       if (krc) return krc;
       continue;

@@ -1998,6 +1998,40 @@ __global__ void __launch_bounds__(NT) k_kryl_apply4(LevelArgs<double> L, const f
   block_sums_to<NT>(acc, V.n, s, part, blk);
 }
 
+// The adjoint operator's transposed ghost coupling on the FP32 level-0
+// basis: after k_kryl_apply4 has written w = A z, each touched fluid cell
+// gets w -= vinv * sum_q ct_w[q] rhsg[ct_ghost[q]] (the composite rows'
+// transpose applied to the ghost sources rhsg of z), and the partial dots
+// <w, u_q> of that launch are corrected by (w_new - w_old) u_q, written as
+// blocks blk0, blk0 + 1, ... of part next to the apply launch's blocks.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_kryl_adj_coupling(int nct, const long *__restrict__ ct_al,
+                                                          const int *__restrict__ ct_ptr,
+                                                          const int *__restrict__ ct_ghost,
+                                                          const float *__restrict__ ct_w,
+                                                          const float *__restrict__ vinv,
+                                                          const float *__restrict__ rhsg, float *__restrict__ w,
+                                                          KryVecsF V, double *__restrict__ part, long blk0) {
+  __shared__ double s[(NT / 64) * (KRY_MAX + 1)];
+  const int t = blockIdx.x * NT + threadIdx.x;
+  double acc[KRY_MAX + 1];
+#pragma unroll
+  for (int q = 0; q < KRY_MAX + 1; q++) acc[q] = 0.0;
+  if (t < nct) {
+    float sum = 0.0f;
+    for (int q = ct_ptr[t]; q < ct_ptr[t + 1]; q++) sum += ct_w[q] * rhsg[ct_ghost[q]];
+    const long a = ct_al[t];
+    const float wo = w[a];
+    const float wn = wo - vinv[t] * sum;
+    w[a] = wn;
+    const double d = (double)wn - (double)wo;
+#pragma unroll
+    for (int q = 0; q < KRY_MAX + 1; q++)
+      if (q < V.n) acc[q] = d * (double)V.u[q][a];
+  }
+  block_sums_to<NT>(acc, V.n, s, part, blk0 + blockIdx.x);
+}
+
 // k_kryl_orth for four consecutive cells per thread (cx % 4 == 0)
 template <int NT>
 __global__ void __launch_bounds__(NT) k_kryl_orth4(LevelArgs<double> L, float *__restrict__ w, KryVecsF V,
