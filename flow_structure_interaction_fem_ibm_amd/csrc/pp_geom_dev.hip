@@ -241,18 +241,16 @@ __global__ void k_geo_isolated_clear(long size, const int8_t *clr, int8_t *ib, i
     }
 }
 
-// fill_cell along one axis: a fluid cell on one of the two outer faces
-// normal to the axis whose inward neighbour is blank becomes blank (cells
-// of the two faces are not each other's neighbours for n >= 3)
-__global__ void k_geo_fill_face(GridV G, int axis, int8_t *ib, int8_t *fresh) {
+// fill_cell on one outer face normal to an axis: a fluid cell whose inward
+// neighbour is blank becomes blank (the faces are launched in the host
+// build's order, first then last)
+__global__ void k_geo_fill_face(GridV G, int axis, int face, int8_t *ib, int8_t *fresh) {
   const int n[3] = {G.nxc, G.nyc, G.nzc};
   const long st[3] = {1, G.ex, G.exy};
   const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
   const long per_face = (long)n[a1] * n[a2];
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= 2 * per_face) return;
-  const int face = (int)(t / per_face);
-  const long uv = t - face * per_face;
+  const long uv = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (uv >= per_face) return;
   const int c = face == 0 ? 1 : n[axis];
   int idx[3];
   idx[axis] = c;
@@ -489,8 +487,10 @@ int32_t build_geometry_device(pp_geom **out, const Grid &Gh, const TriBodies &tr
   k_geo_iblank<<<gb, kThreads>>>(G, size, temp, prev, ib, fresh);
   k_geo_isolated_mark<<<gb, kThreads>>>(G, size, ib, mask);
   k_geo_isolated_clear<<<gb, kThreads>>>(size, mask, ib, bodynum);
-  const long nface[3] = {2L * Gh.nyc * Gh.nzc, 2L * Gh.nzc * Gh.nxc, 2L * Gh.nxc * Gh.nyc};
-  for (int axis = 0; axis < 3; axis++) k_geo_fill_face<<<blocks_for(nface[axis]), kThreads>>>(G, axis, ib, fresh);
+  const long nface[3] = {(long)Gh.nyc * Gh.nzc, (long)Gh.nzc * Gh.nxc, (long)Gh.nxc * Gh.nyc};
+  for (int axis = 0; axis < 3; axis++)
+    for (int face = 0; face < 2; face++)
+      k_geo_fill_face<<<blocks_for(nface[axis]), kThreads>>>(G, axis, face, ib, fresh);
   k_geo_ghosts<<<gb, kThreads>>>(G, size, ib, mask);
   GCHK(hipGetLastError());
   pp_geom *o = new pp_geom();
