@@ -195,17 +195,6 @@ double bytes_line(int s) { return 3.0 * s + 1.0; }
 // ---------------------------------------------------------------------
 // line passes
 // ---------------------------------------------------------------------
-// lane-group width G and rows per lane S for a line of n <= 1024 cells
-// (FP64 holds 8 rows per lane in registers up to 512 cells)
-void line_shape(int n, int &G, int &S, int elem) {
-  if (elem == 8) G = n <= 256 ? 32 : 64;
-  else G = n <= 128 ? 16 : (n <= 512 ? 32 : 64);  // FP32: 16-lane groups measured faster up to 128 cells
-  int need = (n + G - 1) / G;
-  static const int opts[] = {1, 2, 3, 4, 6, 8, 12, 16};
-  S = 16;
-  for (int o : opts)
-    if (o >= need) { S = o; break; }
-}
 
 template <typename Fn>
 void with_line_shape(int G, int S, Fn &&fn) {
@@ -231,19 +220,46 @@ void with_line_shape(int G, int S, Fn &&fn) {
 #undef PP_CASE_S
 }
 
+// the colour-split level 0 is in use (forward FP32 V-cycles)
+template <typename T>
+bool split_on(const pp_ctx *c, int lvl) {
+  return std::is_same<T, float>::value && lvl == 0 && c->sp.on;
+}
+// level arguments and arrays of the colour-split level 0
+template <typename T>
+LevelArgs<T> split_args(const pp_ctx *c, const Level &L) {
+  LevelArgs<T> a = largs<T>(L);
+  a.sx = c->sp.sx; a.sxy = c->sp.sxy; a.fl = c->sp.fl;
+  return a;
+}
+template <typename T>
+T *split_e(pp_ctx *c) { return std::is_same<T, float>::value ? reinterpret_cast<T *>(c->sp.e) : nullptr; }
+template <typename T>
+T *split_rhs(pp_ctx *c) { return std::is_same<T, float>::value ? reinterpret_cast<T *>(c->sp.rhs) : nullptr; }
+
 template <typename T>
 void launch_xline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
-  LevelArgs<T> a = largs<T>(L);
+  const bool spl = split_on<T>(c, lvl);
+  LevelArgs<T> a = spl ? split_args<T>(c, L) : largs<T>(L);
   int G, S;
   line_shape(L.cx, G, S, (int)sizeof(T));
   long lines = (long)((L.cy + 1) / 2) * L.cz;
   dim3 grid((unsigned)((lines * G + 255) / 256)), block(256);
-  T *e = lv_e<T>(L);
-  const T *rhs = lv_rhs<T>(L);
+  T *e = spl ? split_e<T>(c) : lv_e<T>(L);
+  const T *rhs = spl ? split_rhs<T>(c) : lv_rhs<T>(L);
+  const int hp = c->sp.hp;
   hipStream_t st = c->stream;
   kbegin(c, PP_K_XLINE, lvl, 0.5 * (double)L.cx * L.cy * L.cz * bytes_line(sizeof(T)));
   with_line_shape(G, S, [&](auto g, auto sz) {
-    k_xline<T, decltype(g)::value, decltype(sz)::value><<<grid, block, 0, st>>>(a, rhs, e, color, omega);
+    constexpr int GG = decltype(g)::value, SS = decltype(sz)::value;
+    // the split layout needs even chunks (Split0::ok guarantees them)
+    if constexpr (SS % 2 == 0 && std::is_same<T, float>::value) {
+      if (spl) {
+        k_xline<T, GG, SS, true><<<grid, block, 0, st>>>(a, rhs, e, color, omega, hp);
+        return;
+      }
+    }
+    k_xline<T, GG, SS><<<grid, block, 0, st>>>(a, rhs, e, color, omega, 0);
   });
   kend(c);
 }
@@ -284,6 +300,22 @@ void launch_yzline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
   const T *rhs = lv_rhs<T>(L);
   hipStream_t st = c->stream;
   kbegin(c, DIR == 1 ? PP_K_YLINE : PP_K_ZLINE, lvl, 0.5 * (double)L.cx * L.cy * L.cz * bytes_line(sizeof(T)));
+  if constexpr (std::is_same<T, float>::value) {
+    if (split_on<T>(c, lvl)) {
+      LevelArgs<float> as = split_args<float>(c, L);
+      float *es = c->sp.e;
+      const float *rs = c->sp.rhs;
+      const int hp = c->sp.hp;
+      switch (LS) {
+        case 1: k_yzline_split<1, DIR><<<grid, block, sh, st>>>(as, hp, rs, es, color, omega); break;
+        case 2: k_yzline_split<2, DIR><<<grid, block, sh, st>>>(as, hp, rs, es, color, omega); break;
+        case 4: k_yzline_split<4, DIR><<<grid, block, sh, st>>>(as, hp, rs, es, color, omega); break;
+        default: k_yzline_split<8, DIR><<<grid, block, sh, st>>>(as, hp, rs, es, color, omega); break;
+      }
+      kend(c);
+      return;
+    }
+  }
   // loads issued up front on the narrow-tile (latency-bound) FP32 levels
   const bool upf = sizeof(T) == 4 && TX == 8;
   switch (LS) {
@@ -302,14 +334,17 @@ void launch_yzline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
 }
 
 // solved homogeneous ghost rows on a level-0 correction (one gather)
+// (on the colour-split copy, whatever e, while the split level 0 is in use)
 template <typename T>
 void ghost_sweep_level0(pp_ctx *c, T *e) {
   GhostTables &g = c->gh;
   if (g.n == 0) return;
   dim3 grid((g.n + 31) / 32), block(256);
   double nnz = (double)g.h_c_ref.size();
+  const bool spl = split_on<T>(c, 0);
   kbegin(c, PP_K_GHOST, 0, g.n * (12.0 + sizeof(T)) + nnz * (8.0 + 2.0 * sizeof(T)));
-  k_ghost_composite<T, T><<<grid, block, 0, c->stream>>>(g.n, g.g_al, g.c_ptr, g.c_al, ghost_cw<T>(g), nullptr, e);
+  k_ghost_composite<T, T><<<grid, block, 0, c->stream>>>(g.n, spl ? g.g_as : g.g_al, g.c_ptr, spl ? g.c_as : g.c_al,
+                                                          ghost_cw<T>(g), nullptr, spl ? split_e<T>(c) : e);
   kend(c);
 }
 
@@ -461,9 +496,15 @@ void restrict_step(pp_ctx *c, int l) {
   const double cells = (double)L.cx * L.cy * L.cz;
   LevelArgs<T> a = largs<T>(L), ac = largs<T>(C);
   CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
-  if (halves_by2(L, C)) {
+  if (split_on<T>(c, l)) {
+    // Split0::ok guarantees the by-2 halving
     kbegin(c, PP_K_RESTRICT, l, cells * (2.0 * sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz);
-    k_resrestrict2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, lv_e<T>(L), lv_rhs<T>(L), lv_rhs<T>(C), lv_e<T>(C));
+    k_resrestrict2<T, true><<<cc.grid, cc.block, 0, st>>>(split_args<T>(c, L), ac, split_e<T>(c), split_rhs<T>(c),
+                                                          lv_rhs<T>(C), lv_e<T>(C), c->sp.hp);
+    kend(c);
+  } else if (halves_by2(L, C)) {
+    kbegin(c, PP_K_RESTRICT, l, cells * (2.0 * sizeof(T) + 1.0) + 2.0 * sizeof(T) * C.cx * C.cy * C.cz);
+    k_resrestrict2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, lv_e<T>(L), lv_rhs<T>(L), lv_rhs<T>(C), lv_e<T>(C), 0);
     kend(c);
   } else {
     kbegin(c, PP_K_RESIDUAL, l, cells * (3.0 * sizeof(T) + 1.0));
@@ -495,10 +536,16 @@ void prolong_step(pp_ctx *c, int l) {
   // (w/V), which the coarse levels, whose null space is the constants, miss
   const T *scale = (c->adjoint_mode && l == 0 && c->have_nshape) ? nshape<T>(c) : nullptr;
   kbegin(c, PP_K_PROLONG, l, cells * (2.0 * sizeof(T) + 1.0) + (sizeof(T) + 1.0) * C.cx * C.cy * C.cz);
-  if (halves_by2(L, C)) {
+  if (split_on<T>(c, l)) {
+    // forward cycles only: no null-space scale
+    CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
+    k_prolong2<T, true><<<cc.grid, cc.block, 0, st>>>(split_args<T>(c, L), ac, pdir<T>(L, 0).wpar,
+                                                      pdir<T>(L, 1).wpar, pdir<T>(L, 2).wpar, lv_e<T>(C),
+                                                      split_e<T>(c), nullptr, c->sp.hp);
+  } else if (halves_by2(L, C)) {
     CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
     k_prolong2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, pdir<T>(L, 0).wpar, pdir<T>(L, 1).wpar, pdir<T>(L, 2).wpar,
-                                                 lv_e<T>(C), lv_e<T>(L), scale);
+                                                 lv_e<T>(C), lv_e<T>(L), scale, 0);
   } else {
     CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
     k_prolong<T><<<cl.grid, cl.block, 0, st>>>(a, ac, pdir<T>(L, 0), pdir<T>(L, 1), pdir<T>(L, 2), lv_e<T>(C),
@@ -531,6 +578,31 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   smooth_steps<T>(c, l, o, npost);
 }
 
+// The cycles of one preconditioner application.  Forward FP32 line-smoothed
+// cycles run level 0 on its colour-split copy: the rhs goes in (and the
+// split correction starts at zero), the correction comes back out into e0.
+template <typename T>
+void precondition_cycles(pp_ctx *c, const pp_solve_opts *o) {
+  const bool spl = std::is_same<T, float>::value && c->sp.ok && !c->adjoint_mode && o->smoother == PP_SMOOTH_LINE;
+  Level &L0 = c->lv[0];
+  const LevelArgs<float> n0 = largs<float>(L0), s0 = split_args<float>(c, L0);
+  const CellLaunch q = cell_launch((L0.cx + 1) / 2, L0.cy, L0.cz);
+  const double cells = (double)L0.cx * L0.cy * L0.cz;
+  if (spl) {
+    kbegin(c, PP_K_OTHER, 0, cells * 12.0);
+    k_split_in<<<q.grid, q.block, 0, c->stream>>>(n0, s0, c->sp.hp, L0.rhs32, c->sp.rhs, c->sp.e);
+    kend(c);
+    c->sp.on = true;
+  }
+  for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) vcycle<T>(c, 0, o);
+  if (spl) {
+    c->sp.on = false;
+    kbegin(c, PP_K_OTHER, 0, cells * 8.0);
+    k_split_out<<<q.grid, q.block, 0, c->stream>>>(n0, s0, c->sp.hp, c->sp.e, L0.e32);
+    kend(c);
+  }
+}
+
 // The preconditioner: cycles_per_outer V-cycles on the level-0 rhs (e0 = 0
 // on entry).  With use_graph the launches of one application are captured
 // once into a hipGraph (per precision and cycle shape; dropped when the
@@ -539,7 +611,7 @@ template <typename T>
 int precondition(pp_ctx *c, const pp_solve_opts *o, pp_solve_stats &S) {
   bool graph = o->use_graph && !c->prof_on && !c->adjoint_mode;
   if (!graph) {
-    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) vcycle<T>(c, 0, o);
+    precondition_cycles<T>(c, o);
     S.vcycles += o->cycles_per_outer;
     return PP_OK;
   }
@@ -554,7 +626,7 @@ int precondition(pp_ctx *c, const pp_solve_opts *o, pp_solve_stats &S) {
     double b0 = c->bytes_acc;
     hipGraph_t gr = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) vcycle<T>(c, 0, o);
+    precondition_cycles<T>(c, o);
     HIPCHK(hipStreamEndCapture(c->stream, &gr));
     hipError_t ie = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
     hipGraphDestroy(gr);
@@ -675,7 +747,8 @@ bool opts_ok(const pp_solve_opts *o) {
   return o && (o->precision == PP_PREC_FP64 || o->precision == PP_PREC_MIXED) && o->max_outer >= 0 &&
          o->max_outer <= 10000 && o->cycles_per_outer >= 1 && o->nu_pre >= 0 && o->nu_post >= 0 &&
          o->omega > 0.0 && o->omega < 2.0 && o->krylov >= 0 && o->krylov <= KRY_MAX &&
-         (o->smoother == PP_SMOOTH_LINE || o->smoother == PP_SMOOTH_POINT_JACOBI);
+         (o->smoother == PP_SMOOTH_LINE || o->smoother == PP_SMOOTH_POINT_JACOBI ||
+          o->smoother == PP_SMOOTH_LINE_NATURAL);
 }
 
 // ---------------------------------------------------------------------

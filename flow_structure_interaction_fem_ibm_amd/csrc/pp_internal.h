@@ -32,6 +32,16 @@ __host__ __device__ inline long lidx(int sx, long sxy, int i, int j, int k) {
   return (long)OX + i + (long)sx * (j + 1) + sxy * (k + 1);
 }
 
+// Colour-split rows of the finest level's FP32 smoothing arrays: row (j,k)
+// holds its even cells contiguously from SPLIT_E0 + i/2 and its odd cells
+// from SPLIT_E0 + hp + (i-1)/2 (the halo cell i = -1 sits just below the
+// odd half).  A zebra colour of y- or z-lines then owns one half of every
+// row: whole cache lines read and written instead of every other element.
+constexpr int SPLIT_E0 = 32;
+__host__ __device__ inline long sidx(int sx, long sxy, int hp, int i, int j, int k) {
+  return (long)sx * (j + 1) + sxy * (k + 1) + SPLIT_E0 + ((i & 1) ? hp + ((i - 1) >> 1) : (i >> 1));
+}
+
 // Per-level device view passed by value to kernels.
 template <typename T>
 struct LevelArgs {
@@ -83,6 +93,7 @@ struct GhostTables {
   int kexact = 1;                 // Jacobi sweeps that solve the ghost rows
   // forward gather (device)
   long *g_ref = nullptr, *g_al = nullptr;       // ghost cell offsets
+  long *g_as = nullptr, *c_as = nullptr;        // ghost and fluid offsets in the split level 0
   double *bself = nullptr;                      // c_self + probeLength
   double *bnd = nullptr;                        // bself * uNormal (per solve)
   // composite ghost rows: the solved ghost value as a combination of fluid
@@ -151,6 +162,18 @@ struct FreshRows {
   double *cc = nullptr;     // 4 per cell: their coeffGCMFreshD
 };
 
+// the colour-split copy of level 0 used by the forward FP32 V-cycles
+// (flags, correction, right-hand side; level-0 FP32 arrays stay in the
+// natural layout for everything else)
+struct Split0 {
+  bool ok = false;    // level 0 qualifies (halves by 2, even x-line chunks)
+  bool on = false;    // set while a forward FP32 preconditioner runs
+  int hp = 0, sx = 0;
+  long sxy = 0, total = 0;
+  float *e = nullptr, *rhs = nullptr;
+  uint8_t *fl = nullptr;
+};
+
 struct CoarseSolve {
   int n = 0;                      // cells of the coarsest level (at most 4096)
   double *G = nullptr;            // n*n by output cell: G[q*n+p] = d e_q / d r_p (FP64 in both precisions)
@@ -172,6 +195,7 @@ struct pp_ctx {
   std::vector<int8_t> h_iblank, h_fresh;  // reference layout
   pp::GhostTables gh;
   pp::CoarseSolve cs;
+  pp::Split0 sp;
   void *blas = nullptr;                // rocblas_handle of the coarse factorisation
   double *d_rdx[3] = {}, *d_rdxinv[3] = {};  // make_grid dx, dxinv (0:n+1) per direction
   double *d_rdxcinv[3] = {}, *d_rfx[3] = {};  // dxcinv and fx (0:n+1) per direction

@@ -1177,10 +1177,11 @@ __global__ void __launch_bounds__(256) k_restrict2(LevelArgs<T> F, LevelArgs<T> 
 // the residuals of its 8 children (the k_residual row, same operation
 // order) and their fluid-volume-weighted mean directly, so the fine residual
 // is never stored.  Also zeroes the coarse correction.
-template <typename T>
+// SPL: the fine level is in the colour-split row layout (sidx, hp).
+template <typename T, bool SPL = false>
 __global__ void __launch_bounds__(256) k_resrestrict2(LevelArgs<T> F, LevelArgs<T> C, const T *__restrict__ e,
                                                       const T *__restrict__ rhs, T *__restrict__ rhs_c,
-                                                      T *__restrict__ e_c) {
+                                                      T *__restrict__ e_c, int hp) {
   using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
@@ -1197,15 +1198,31 @@ __global__ void __launch_bounds__(256) k_resrestrict2(LevelArgs<T> F, LevelArgs<
     for (int b = 0; b < 2; b++) {
       const int j = 2 * J + b;
       const T wyz = F.wy[j] * F.wz[k];
-      const long id = lidx(F.sx, F.sxy, i0, j, k);
-      const V2 ev = *reinterpret_cast<const V2 *>(e + id);
-      const T el = e[id - 1], er = e[id + 2];
-      const V2 ejm = *reinterpret_cast<const V2 *>(e + id - F.sx);
-      const V2 ejp = *reinterpret_cast<const V2 *>(e + id + F.sx);
-      const V2 ekm = *reinterpret_cast<const V2 *>(e + id - F.sxy);
-      const V2 ekp = *reinterpret_cast<const V2 *>(e + id + F.sxy);
-      const V2 rv = *reinterpret_cast<const V2 *>(rhs + id);
-      const uint16_t f2 = *reinterpret_cast<const uint16_t *>(F.fl + id);
+      V2 ev, ejm, ejp, ekm, ekp, rv;
+      T el, er;
+      uint16_t f2;
+      if constexpr (SPL) {
+        // children 2I (even half) and 2I+1 (odd half, hp further)
+        const long id = sidx(F.sx, F.sxy, hp, i0, j, k), io = id + hp;
+        ev.x = e[id]; ev.y = e[io];
+        el = e[io - 1]; er = e[id + 1];
+        ejm.x = e[id - F.sx]; ejm.y = e[io - F.sx];
+        ejp.x = e[id + F.sx]; ejp.y = e[io + F.sx];
+        ekm.x = e[id - F.sxy]; ekm.y = e[io - F.sxy];
+        ekp.x = e[id + F.sxy]; ekp.y = e[io + F.sxy];
+        rv.x = rhs[id]; rv.y = rhs[io];
+        f2 = (uint16_t)(F.fl[id] | (F.fl[io] << 8));
+      } else {
+        const long id = lidx(F.sx, F.sxy, i0, j, k);
+        ev = *reinterpret_cast<const V2 *>(e + id);
+        el = e[id - 1]; er = e[id + 2];
+        ejm = *reinterpret_cast<const V2 *>(e + id - F.sx);
+        ejp = *reinterpret_cast<const V2 *>(e + id + F.sx);
+        ekm = *reinterpret_cast<const V2 *>(e + id - F.sxy);
+        ekp = *reinterpret_cast<const V2 *>(e + id + F.sxy);
+        rv = *reinterpret_cast<const V2 *>(rhs + id);
+        f2 = *reinterpret_cast<const uint16_t *>(F.fl + id);
+      }
       const T my = F.cmy[j], py = F.cpy[j], mz = F.cmz[k], pz = F.cpz[k];
 #pragma unroll
       for (int t = 0; t < 2; t++) {
@@ -1276,11 +1293,12 @@ __global__ void __launch_bounds__(256) k_prolong(LevelArgs<T> F, LevelArgs<T> C,
 // are I-1 / I+1 etc.; a partner outside the level carries weight 0).
 // Same weights, masks and renormalisation as k_prolong.  scale (fine level
 // layout, may be NULL) multiplies each cell's correction.
-template <typename T>
+// SPL: the fine level is in the colour-split row layout (sidx, hp; no scale).
+template <typename T, bool SPL = false>
 __global__ void __launch_bounds__(256) k_prolong2(LevelArgs<T> F, LevelArgs<T> C, const T *__restrict__ wx,
                                                   const T *__restrict__ wy, const T *__restrict__ wz,
                                                   const T *__restrict__ e_c, T *__restrict__ e_f,
-                                                  const T *__restrict__ scale) {
+                                                  const T *__restrict__ scale, int hp) {
   using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
@@ -1304,9 +1322,17 @@ __global__ void __launch_bounds__(256) k_prolong2(LevelArgs<T> F, LevelArgs<T> C
   for (int kc = 0; kc < 2; kc++)
 #pragma unroll
     for (int jc = 0; jc < 2; jc++) {
-      long id = lidx(F.sx, F.sxy, 2 * I, 2 * J + jc, 2 * K + kc);
-      V2 ef = *reinterpret_cast<const V2 *>(e_f + id);
-      uint16_t f2 = *reinterpret_cast<const uint16_t *>(F.fl + id);
+      const long id = SPL ? sidx(F.sx, F.sxy, hp, 2 * I, 2 * J + jc, 2 * K + kc)
+                          : lidx(F.sx, F.sxy, 2 * I, 2 * J + jc, 2 * K + kc);
+      V2 ef;
+      uint16_t f2;
+      if constexpr (SPL) {
+        ef.x = e_f[id]; ef.y = e_f[id + hp];
+        f2 = (uint16_t)(F.fl[id] | (F.fl[id + hp] << 8));
+      } else {
+        ef = *reinterpret_cast<const V2 *>(e_f + id);
+        f2 = *reinterpret_cast<const uint16_t *>(F.fl + id);
+      }
       T out[2] = {ef.x, ef.y};
 #pragma unroll
       for (int ic = 0; ic < 2; ic++) {
@@ -1330,12 +1356,18 @@ __global__ void __launch_bounds__(256) k_prolong2(LevelArgs<T> F, LevelArgs<T> C
             }
         if (den > T(0)) {
           const T corr = num / den;
-          out[ic] = out[ic] + (scale ? scale[id + ic] * corr : corr);
+          if constexpr (SPL) out[ic] = out[ic] + corr;
+          else out[ic] = out[ic] + (scale ? scale[id + ic] * corr : corr);
         }
       }
-      ef.x = out[0];
-      ef.y = out[1];
-      *reinterpret_cast<V2 *>(e_f + id) = ef;
+      if constexpr (SPL) {
+        e_f[id] = out[0];
+        e_f[id + hp] = out[1];
+      } else {
+        ef.x = out[0];
+        ef.y = out[1];
+        *reinterpret_cast<V2 *>(e_f + id) = ef;
+      }
     }
 }
 
@@ -1481,9 +1513,13 @@ __device__ inline int xcd_block(int b, int nb) {
 // x-line zebra pass (lines along the contiguous index): lines (j,k) with
 // (j+k)%2 == color; a group of G lanes holds one line, S cells per lane.
 // ---------------------------------------------------------------------
-template <typename T, int G, int S>
+// SPL: the level is in the colour-split row layout (sidx, half-row offset
+// hp; S even): a lane's S cells are S/2 consecutive even cells and the S/2
+// odd cells beside them.
+template <typename T, int G, int S, bool SPL = false>
 __global__ void __launch_bounds__(256) k_xline(LevelArgs<T> L, const T *__restrict__ rhs, T *__restrict__ e,
-                                               int color, T omega) {
+                                               int color, T omega, int hp) {
+  static_assert(!SPL || S % 2 == 0, "split rows need an even chunk");
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int line = (blk * 256 + (int)threadIdx.x) / G;
   const int sub = threadIdx.x & (G - 1);
@@ -1493,18 +1529,37 @@ __global__ void __launch_bounds__(256) k_xline(LevelArgs<T> L, const T *__restri
   const bool valid = k < L.cz && j < L.cy;
   const int i0 = sub * S;
   const int nv = valid ? max(0, min(S, L.cx - i0)) : 0;
-  const long base = valid ? lidx(L.sx, L.sxy, i0, j, k) : 0;
+  const long base = valid ? (SPL ? sidx(L.sx, L.sxy, hp, i0, j, k) : lidx(L.sx, L.sxy, i0, j, k)) : 0;
   T eo[S], ejm[S], ejp[S], ekm[S], ekp[S], rv[S], cmx[S], cpx[S];
   uint8_t fl[S];
-  load_chunk<T, S>(e + base, nv, eo);
-  load_chunk<T, S>(e + base - L.sx, nv, ejm);
-  load_chunk<T, S>(e + base + L.sx, nv, ejp);
-  load_chunk<T, S>(e + base - L.sxy, nv, ekm);
-  load_chunk<T, S>(e + base + L.sxy, nv, ekp);
-  load_chunk<T, S>(rhs + base, nv, rv);
+  auto ld = [&](const T *p, T(&v)[S]) {
+    if constexpr (SPL) {
+      T a[S / 2], b[S / 2];
+      load_chunk<T, S / 2>(p, (nv + 1) >> 1, a);
+      load_chunk<T, S / 2>(p + hp, nv >> 1, b);
+#pragma unroll
+      for (int q = 0; q < S / 2; q++) { v[2 * q] = a[q]; v[2 * q + 1] = b[q]; }
+    } else {
+      load_chunk<T, S>(p, nv, v);
+    }
+  };
+  ld(e + base, eo);
+  ld(e + base - L.sx, ejm);
+  ld(e + base + L.sx, ejp);
+  ld(e + base - L.sxy, ekm);
+  ld(e + base + L.sxy, ekp);
+  ld(rhs + base, rv);
   load_chunk<T, S>(L.cmx + i0, nv, cmx);
   load_chunk<T, S>(L.cpx + i0, nv, cpx);
-  load_flags<S>(L.fl + base, nv, fl);
+  if constexpr (SPL) {
+    uint8_t a[S / 2], b[S / 2];
+    load_flags<S / 2>(L.fl + base, (nv + 1) >> 1, a);
+    load_flags<S / 2>(L.fl + base + hp, nv >> 1, b);
+#pragma unroll
+    for (int q = 0; q < S / 2; q++) { fl[2 * q] = a[q]; fl[2 * q + 1] = b[q]; }
+  } else {
+    load_flags<S>(L.fl + base, nv, fl);
+  }
   T my = valid ? L.cmy[j] : T(0), py = valid ? L.cpy[j] : T(0);
   T mz = valid ? L.cmz[k] : T(0), pz = valid ? L.cpz[k] : T(0);
   SegElim<T, S> se;
@@ -1533,7 +1588,15 @@ __global__ void __launch_bounds__(256) k_xline(LevelArgs<T> L, const T *__restri
   se.template solve<G>();
 #pragma unroll
   for (int s = 0; s < S; s++) eo[s] = (T(1) - omega) * eo[s] + omega * se.dp[s];
-  store_chunk<T, S>(e + base, nv, eo);
+  if constexpr (SPL) {
+    T a[S / 2], b[S / 2];
+#pragma unroll
+    for (int q = 0; q < S / 2; q++) { a[q] = eo[2 * q]; b[q] = eo[2 * q + 1]; }
+    store_chunk<T, S / 2>(e + base, (nv + 1) >> 1, a);
+    store_chunk<T, S / 2>(e + base + hp, nv >> 1, b);
+  } else {
+    store_chunk<T, S>(e + base, nv, eo);
+  }
 }
 
 // Reduced system of a line split into P segments held by the P threads
@@ -1694,6 +1757,136 @@ __global__ void __launch_bounds__(1024) k_yzline(LevelArgs<T> Lv, const T *__res
   for (int m = 0; m < LS; m++) {
     if (m0 + m < n) e[id0 + m * lstride] = (T(1) - omega) * eo[m] + omega * se.dp[m];
   }
+}
+
+// k_yzline on the colour-split FP32 level 0 (sidx, half-row offset hp):
+// a colour's cells are contiguous half rows, their x-neighbours the
+// other half.  The cells of a segment are read PF ahead of their
+// elimination (every load unconditional: cells past the line or the level
+// read a valid cell and are ignored), and the register budget is held at
+// six wavefronts per SIMD so that two 768-thread blocks share a CU.
+template <int LS, int DIR>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8)))
+k_yzline_split(LevelArgs<float> Lv, int hp, const float *__restrict__ rhs, float *__restrict__ e, int color,
+               float omega) {
+  constexpr int PF = LS < 4 ? LS : 4;
+  extern __shared__ unsigned char smem_raw[];
+  float *sh = reinterpret_cast<float *>(smem_raw);
+  const int P = blockDim.y;
+  const int tx = threadIdx.x, seg = threadIdx.y;
+  const int lb = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int tile = lb % gridDim.x, plane = lb / gridDim.x;
+  const int par = (plane + color) & 1;
+  const int i = 2 * (tile * blockDim.x + tx) + par;
+  const int n = DIR == 1 ? Lv.cy : Lv.cz;
+  const bool ok = ((DIR == 1) ? (plane < Lv.cz) : (plane < Lv.cy)) && i < Lv.cx;
+  const int m0 = seg * LS;
+  float mx = 0.f, px = 0.f, mo = 0.f, po = 0.f;
+  if (ok) {
+    mx = Lv.cmx[i]; px = Lv.cpx[i];
+    mo = DIR == 1 ? Lv.cmz[plane] : Lv.cmy[plane];
+    po = DIR == 1 ? Lv.cpz[plane] : Lv.cpy[plane];
+  }
+  const long ostride = DIR == 1 ? Lv.sxy : (long)Lv.sx;
+  const long lstride = DIR == 1 ? (long)Lv.sx : Lv.sxy;
+  const uint8_t FO_M = DIR == 1 ? FL_KM : FL_JM, FO_P = DIR == 1 ? FL_KP : FL_JP;
+  const uint8_t FL_M = DIR == 1 ? FL_JM : FL_KM, FL_P = DIR == 1 ? FL_JP : FL_KP;
+  const float *cml = DIR == 1 ? Lv.cmy : Lv.cmz;
+  const float *cpl = DIR == 1 ? Lv.cpy : Lv.cpz;
+  const long idsafe = sidx(Lv.sx, Lv.sxy, hp, 0, 0, 0);
+  long id0 = idsafe;
+  if (ok) id0 = DIR == 1 ? sidx(Lv.sx, Lv.sxy, hp, i, m0, plane) : sidx(Lv.sx, Lv.sxy, hp, i, plane, m0);
+  // offsets of the x-neighbours i-1 and i+1 (the other half of the row)
+  const long dlo = par ? -(long)hp : (long)hp - 1;
+  const long dhi = par ? 1 - (long)hp : (long)hp;
+  uint8_t fv[PF];
+  float evv[PF], elv[PF], erv[PF], ezm[PF], ezp[PF], rvv[PF], cqm[PF], cqp[PF];
+  auto load_cell = [&](int m, int slot) {
+    const bool live = ok && m0 + m < n;
+    const long id = live ? id0 + m * lstride : idsafe;
+    const int q = live ? m0 + m : 0;
+    fv[slot] = Lv.fl[id];
+    evv[slot] = e[id];
+    elv[slot] = e[id + dlo];
+    erv[slot] = e[id + dhi];
+    ezm[slot] = e[id - ostride];
+    ezp[slot] = e[id + ostride];
+    rvv[slot] = rhs[id];
+    cqm[slot] = cml[q];
+    cqp[slot] = cpl[q];
+  };
+#pragma unroll
+  for (int m = 0; m < PF; m++) load_cell(m, m);
+  float eo[LS];
+  SegElim<float, LS> se;
+#pragma unroll
+  for (int m = 0; m < LS; m++) {
+    const int sl = m % PF;
+    const uint8_t f = fv[sl];
+    const float ev = evv[sl], el = elv[sl], er = erv[sl], zm = ezm[sl], zp = ezp[sl], rv = rvv[sl];
+    const float cm = cqm[sl], cp = cqp[sl];
+    if (m + PF < LS) load_cell(m + PF, sl);
+    float a = 0.f, b = 1.f, c = 0.f, d = 0.f;
+    const bool live = ok && m0 + m < n;
+    eo[m] = live ? ev : 0.f;
+    if (live) {
+      if (f & FL_BLANK) {
+        d = (f & FL_GHOST) ? ev : 0.f;
+      } else {
+        const float amx = (f & FL_IM) ? 0.f : mx, apx = (f & FL_IP) ? 0.f : px;
+        const float amo = (f & FO_M) ? 0.f : mo, apo = (f & FO_P) ? 0.f : po;
+        const float am = (f & FL_M) ? 0.f : cm, ap = (f & FL_P) ? 0.f : cp;
+        // (acx + acy + acz) in the reference's order
+        const float diag =
+            DIR == 1 ? (-(amx + apx) + -(am + ap)) + -(amo + apo) : (-(amx + apx) + -(amo + apo)) + -(am + ap);
+        const float r = rv - el * amx - er * apx - zm * amo - zp * apo;
+        if (tabs(diag) < 1.0e-12f) {
+          d = ev;
+        } else {
+          a = am; b = diag; c = ap; d = r;
+        }
+      }
+    }
+    se.push(m, a, b, c, d);
+  }
+  float XL, XS;
+  seg_reduced_solve<float, LS>(sh, P, tx, seg, se, XL, XS);
+  if (!ok) return;
+  se.combine(XL, XS);
+#pragma unroll
+  for (int m = 0; m < LS; m++) {
+    if (m0 + m < n) e[id0 + m * lstride] = (1.f - omega) * eo[m] + omega * se.dp[m];
+  }
+}
+
+// The FP32 level-0 right-hand side into the colour-split copy, and the
+// split correction zeroed (interior cells; two cells per thread).
+__global__ void __launch_bounds__(256) k_split_in(LevelArgs<float> N, LevelArgs<float> S, int hp,
+                                                  const float *__restrict__ rhs, float *__restrict__ rhs_s,
+                                                  float *__restrict__ e_s) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  const int i = 2 * I;
+  if (i >= N.cx || j >= N.cy) return;
+  const long a = lidx(N.sx, N.sxy, i, j, k), se = sidx(S.sx, S.sxy, hp, i, j, k);
+  rhs_s[se] = rhs[a];
+  e_s[se] = 0.f;
+  if (i + 1 < N.cx) {
+    rhs_s[se + hp] = rhs[a + 1];
+    e_s[se + hp] = 0.f;
+  }
+}
+
+// the split correction back into the natural FP32 level-0 layout
+__global__ void __launch_bounds__(256) k_split_out(LevelArgs<float> N, LevelArgs<float> S, int hp,
+                                                   const float *__restrict__ e_s, float *__restrict__ e) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  const int i = 2 * I;
+  if (i >= N.cx || j >= N.cy) return;
+  const long a = lidx(N.sx, N.sxy, i, j, k), se = sidx(S.sx, S.sxy, hp, i, j, k);
+  e[a] = e_s[se];
+  if (i + 1 < N.cx) e[a + 1] = e_s[se + hp];
 }
 
 // ---------------------------------------------------------------------

@@ -20,6 +20,8 @@ constexpr long kCoarseDenseMax = 16384;
 // longest line the x-line (wavefront) and y/z-line (partitioned) solvers hold
 constexpr int kMaxLine = 1024;
 int build_flags(pp_ctx *c, const int8_t *d_iblank, long *n_ghost);
+// lane-group width G and rows per lane S of the x-line pass (elem: bytes)
+void line_shape(int n, int &G, int &S, int elem);
 int upload_levels(pp_ctx *c);
 void free_levels(pp_ctx *c);
 void free_ghosts(pp_ctx *c);

@@ -122,6 +122,8 @@ void free_levels(pp_ctx *c) {
     hipFree(L.e64); hipFree(L.rhs64); hipFree(L.res64);
   }
   c->lv.clear();
+  hipFree(c->sp.e); hipFree(c->sp.rhs); hipFree(c->sp.fl);
+  c->sp = Split0();
 }
 
 void free_ghosts(pp_ctx *c) {
@@ -133,7 +135,7 @@ void free_ghosts(pp_ctx *c) {
                   g.tbase64, g.lam32[0], g.lam32[1], g.rhsg32, g.tbase32, g.olam64[0], g.olam64[1],
                   g.orhsg64, g.otbase64, g.v_cD, g.v_buf[0], g.v_buf[1], g.v_kind, g.v_off,
                   g.ct_ref, g.ct_al, g.ct_ptr, g.ct_ghost, g.ct_w64, g.ct_vinv64, g.ctbase64,
-                  g.ct_w32, g.ct_vinv32, g.ctbase32};
+                  g.ct_w32, g.ct_vinv32, g.ctbase32, g.g_as, g.c_as};
   for (void *p : ptrs) hipFree(p);
   g = GhostTables();
   hipFree(c->cs.G); hipFree(c->cs.cell);
@@ -308,6 +310,62 @@ __global__ void k_flags_coarse(int sx, long sxy, int cx, int cy, int cz, const u
   fl[lidx(sx, sxy, i, j, k)] = f;
 }
 
+// lane-group width G and rows per lane S of the x-line pass for a line of
+// n <= 1024 cells (FP64 holds 8 rows per lane in registers up to 512 cells)
+void line_shape(int n, int &G, int &S, int elem) {
+  if (elem == 8) G = n <= 256 ? 32 : 64;
+  else G = n <= 128 ? 16 : (n <= 512 ? 32 : 64);  // FP32: 16-lane groups measured faster up to 128 cells
+  int need = (n + G - 1) / G;
+  static const int opts[] = {1, 2, 3, 4, 6, 8, 12, 16};
+  S = 16;
+  for (int o : opts)
+    if (o >= need) { S = o; break; }
+}
+
+// level-0 flags (halo included) into the colour-split copy
+__global__ void k_flags_split(int cx, int cy, int cz, int sx, long sxy, const uint8_t *__restrict__ fl, int ssx,
+                              long ssxy, int hp, uint8_t *__restrict__ fs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x - 1;
+  const int j = (int)blockIdx.y - 1, k = (int)blockIdx.z - 1;
+  if (i > cx) return;
+  fs[sidx(ssx, ssxy, hp, i, j, k)] = fl[lidx(sx, sxy, i, j, k)];
+}
+
+// The colour-split copy of level 0 (Split0) where it applies: the level
+// halves by 2 (the fused transfer kernels) and the FP32 x-line chunks are
+// even.  Rows: SPLIT_E0 + two halves of hp cells (hp a multiple of 32, at
+// least cx/2 + 2, so the halo cells fit and both halves start on a
+// 128-byte line).
+int setup_split0(pp_ctx *c) {
+  Split0 &sp = c->sp;
+  hipFree(sp.e); hipFree(sp.rhs); hipFree(sp.fl);
+  sp = Split0();
+  if (c->lv.size() < 2) return PP_OK;
+  const Level &L = c->lv[0], &C = c->lv[1];
+  const bool by2 = C.cf[0] == 2 && C.cf[1] == 2 && C.cf[2] == 2 && L.cx == 2 * C.cx && L.cy == 2 * C.cy &&
+                   L.cz == 2 * C.cz;
+  int G, S;
+  line_shape(L.cx, G, S, 4);
+  if (!by2 || S % 2 != 0) return PP_OK;
+  sp.hp = ((L.cx / 2 + 2 + 31) / 32) * 32;
+  sp.sx = SPLIT_E0 + 2 * sp.hp;
+  sp.sxy = (long)sp.sx * (L.cy + 2);
+  sp.total = sp.sxy * (L.cz + 2);
+  if (hipMalloc(&sp.e, sp.total * 4) != hipSuccess || hipMalloc(&sp.rhs, sp.total * 4) != hipSuccess ||
+      hipMalloc(&sp.fl, sp.total) != hipSuccess) {
+    set_error("hipMalloc failed for the colour-split level 0 (%ld cells)", sp.total);
+    return PP_HIP_ERROR;
+  }
+  // halo cells of the split correction stay zero (only interior cells are written)
+  if (hipMemset(sp.e, 0, sp.total * 4) != hipSuccess || hipMemset(sp.rhs, 0, sp.total * 4) != hipSuccess ||
+      hipMemset(sp.fl, FL_BLANK, sp.total) != hipSuccess) {
+    set_error("colour-split level 0 initialisation failed");
+    return PP_HIP_ERROR;
+  }
+  sp.ok = true;
+  return PP_OK;
+}
+
 // The flags of every level from the reference-layout iblank on the device
 // (d_iblank); the coarsest level's are copied back for build_coarse, and
 // *n_ghost is the number of ghost cells the iblank has.
@@ -342,6 +400,12 @@ int build_flags(pp_ctx *c, const int8_t *d_iblank, long *n_ghost) {
         return PP_HIP_ERROR;
       }
     }
+  }
+  if (c->sp.ok) {
+    const Level &L = c->lv[0];
+    dim3 grid((L.cx + 2 + 63) / 64, L.cy + 2, L.cz + 2), block(64);
+    k_flags_split<<<grid, block, 0, s>>>(L.cx, L.cy, L.cz, L.sx, L.sxy, L.flags, c->sp.sx, c->sp.sxy, c->sp.hp,
+                                          c->sp.fl);
   }
   if (hipGetLastError() != hipSuccess) {
     set_error("flag kernels failed to launch");
@@ -383,7 +447,7 @@ int upload_levels(pp_ctx *c) {
     hipMemset(L.e64, 0, n * 8); hipMemset(L.rhs64, 0, n * 8); hipMemset(L.res64, 0, n * 8);
     if (!L.flags) { set_error("hipMalloc failed for level flags"); return PP_HIP_ERROR; }
   }
-  return PP_OK;
+  return setup_split0(c);
 }
 
 // Ghost marking exactly as GCM_set_internal_boundary (GCM_SET_BOUNDARY.F90:49-73):
@@ -682,6 +746,18 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     }
     if (!ct.empty()) G.h_ct_ptr.push_back((int)G.h_ct_ghost.size());
     G.nct = (int)G.h_ct_ref.size();
+  }
+  if (c->sp.ok) {
+    // the gather's offsets in the colour-split level 0
+    const Split0 &sp = c->sp;
+    auto split_of = [&](long r) {
+      const long z = r / c->ref.exy, rem = r % c->ref.exy, y = rem / c->ref.ex, x = rem % c->ref.ex;
+      return sidx(sp.sx, sp.sxy, sp.hp, (int)x - 1, (int)y - 1, (int)z - 1);
+    };
+    std::vector<long> gs(G.h_g_ref.size()), cs(G.h_c_ref.size());
+    for (size_t q = 0; q < gs.size(); q++) gs[q] = split_of(G.h_g_ref[q]);
+    for (size_t q = 0; q < cs.size(); q++) cs[q] = split_of(G.h_c_ref[q]);
+    G.g_as = dev_upload(gs); G.c_as = dev_upload(cs);
   }
   G.g_ref = dev_upload(G.h_g_ref); G.g_al = dev_upload(G.h_g_al);
   G.c_ptr = dev_upload(G.h_c_ptr); G.c_ref = dev_upload(G.h_c_ref); G.c_al = dev_upload(G.h_c_al);

@@ -17,7 +17,8 @@ module pp_hip_mod
        PP_BAD_GEOMETRY = 3, PP_HIP_ERROR = 4, PP_NONFINITE = 5, PP_NO_GEOMETRY = 6
   integer(c_int32_t), parameter, public :: PP_BC_NEUMANN = 2
   integer(c_int32_t), parameter, public :: PP_PREC_FP64 = 0, PP_PREC_MIXED = 1
-  integer(c_int32_t), parameter, public :: PP_SMOOTH_LINE = 0, PP_SMOOTH_POINT_JACOBI = 1
+  integer(c_int32_t), parameter, public :: PP_SMOOTH_LINE = 0, PP_SMOOTH_POINT_JACOBI = 1, &
+                                            PP_SMOOTH_LINE_NATURAL = 2
   integer, parameter, public :: PP_K_CLASSES = 13
   integer(c_int32_t), parameter, public :: PP_BODY_ELLIPSOID = 3
   integer(c_int32_t), parameter, public :: PP_GEOM_ANALYTIC = 0, PP_GEOM_TRIANGULATED = 1

@@ -83,6 +83,7 @@ class SolveOpts(C.Structure):
 
 PP_SMOOTH_LINE = 0
 PP_SMOOTH_POINT_JACOBI = 1
+PP_SMOOTH_LINE_NATURAL = 2   # PP_SMOOTH_LINE without the colour-split finest-level copy
 PP_LOP_RESIDUAL, PP_LOP_RESTRICT, PP_LOP_RESRESTRICT, PP_LOP_PROLONG, PP_LOP_LINE_PASS = range(5)
 
 KERNEL_CLASSES = ("outer_residual", "update", "xline", "yline", "zline", "residual", "restrict",

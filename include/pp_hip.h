@@ -100,6 +100,9 @@ typedef struct pp_solve_opts {
                                      CORE_MG.F90:1883-2270, pp_solver_type 3) */
 #define PP_SMOOTH_POINT_JACOBI 1  /* weighted point Jacobi (MG_itsolv_Point_Jacobi,
                                      CORE_MG.F90:3716-3800, pp_solver_type 8) */
+#define PP_SMOOTH_LINE_NATURAL 2  /* PP_SMOOTH_LINE with the finest level's FP32 arrays
+                                     kept in the natural layout (the default colour-split
+                                     copy gives the same results; pp_solve_opts only) */
 
 /* kernel classes of pp_solve_stats.kern_* (profile = 1) */
 #define PP_K_OUTER_RESIDUAL 0  /* FP64 b - A p on the reference layout */
