@@ -334,14 +334,13 @@ void launch_yzline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
 }
 
 // solved homogeneous ghost rows on a level-0 correction (one gather)
-// (on the colour-split copy, whatever e, while the split level 0 is in use)
+// (spl: on the colour-split copy, whatever e)
 template <typename T>
-void ghost_sweep_level0(pp_ctx *c, T *e) {
+void ghost_sweep_level0(pp_ctx *c, T *e, bool spl) {
   GhostTables &g = c->gh;
   if (g.n == 0) return;
   dim3 grid((g.n + 31) / 32), block(256);
   double nnz = (double)g.h_c_ref.size();
-  const bool spl = split_on<T>(c, 0);
   kbegin(c, PP_K_GHOST, 0, g.n * (12.0 + sizeof(T)) + nnz * (8.0 + 2.0 * sizeof(T)));
   k_ghost_composite<T, T><<<grid, block, 0, c->stream>>>(g.n, spl ? g.g_as : g.g_al, g.c_ptr, spl ? g.c_as : g.c_al,
                                                           ghost_cw<T>(g), nullptr, spl ? split_e<T>(c) : e);
@@ -412,7 +411,7 @@ void adj_ghost_base(pp_ctx *c) {
 template <typename T>
 void ghost_step_level0(pp_ctx *c, T *e, T *rhs) {
   if (c->adjoint_mode) adj_ghost_level0<T>(c, e, rhs);
-  else ghost_sweep_level0<T>(c, e);
+  else ghost_sweep_level0<T>(c, e, split_on<T>(c, 0));
 }
 
 // one zebra line pass set: x, y, z lines, two colours each
@@ -578,25 +577,38 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   smooth_steps<T>(c, l, o, npost);
 }
 
-// The cycles of one preconditioner application.  Forward FP32 line-smoothed
-// cycles run level 0 on its colour-split copy: the rhs goes in (and the
-// split correction starts at zero), the correction comes back out into e0.
+// Forward FP32 line-smoothed cycles run level 0 on its colour-split copy.
 template <typename T>
-void precondition_cycles(pp_ctx *c, const pp_solve_opts *o) {
-  const bool spl = std::is_same<T, float>::value && c->sp.ok && !c->adjoint_mode && o->smoother == PP_SMOOTH_LINE;
+bool split_cycles(const pp_ctx *c, const pp_solve_opts *o) {
+  return std::is_same<T, float>::value && c->sp.ok && !c->adjoint_mode && o->smoother == PP_SMOOTH_LINE;
+}
+// the preconditioner is captured into a hipGraph
+bool graph_path(const pp_ctx *c, const pp_solve_opts *o) { return o->use_graph && !c->prof_on && !c->adjoint_mode; }
+
+// The cycles of one preconditioner application.  On the colour-split level
+// 0 the rhs goes in (and the split correction starts at zero) unless the
+// caller has written both split copies already (in_split), and the
+// correction comes back out into e0 unless the caller reads the split copy
+// (out_split).
+template <typename T>
+void precondition_cycles(pp_ctx *c, const pp_solve_opts *o, bool in_split = false, bool out_split = false) {
+  const bool spl = split_cycles<T>(c, o);
   Level &L0 = c->lv[0];
   const LevelArgs<float> n0 = largs<float>(L0), s0 = split_args<float>(c, L0);
   const CellLaunch q = cell_launch((L0.cx + 1) / 2, L0.cy, L0.cz);
   const double cells = (double)L0.cx * L0.cy * L0.cz;
   if (spl) {
-    kbegin(c, PP_K_OTHER, 0, cells * 12.0);
-    k_split_in<<<q.grid, q.block, 0, c->stream>>>(n0, s0, c->sp.hp, L0.rhs32, c->sp.rhs, c->sp.e);
-    kend(c);
+    if (!in_split) {
+      kbegin(c, PP_K_OTHER, 0, cells * 12.0);
+      k_split_in<<<q.grid, q.block, 0, c->stream>>>(n0, s0, c->sp.hp, L0.rhs32, c->sp.rhs, c->sp.e);
+      kend(c);
+    }
     c->sp.on = true;
   }
   for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) vcycle<T>(c, 0, o);
   if (spl) {
     c->sp.on = false;
+    if (out_split) return;
     kbegin(c, PP_K_OTHER, 0, cells * 8.0);
     k_split_out<<<q.grid, q.block, 0, c->stream>>>(n0, s0, c->sp.hp, c->sp.e, L0.e32);
     kend(c);
@@ -607,11 +619,13 @@ void precondition_cycles(pp_ctx *c, const pp_solve_opts *o) {
 // on entry).  With use_graph the launches of one application are captured
 // once into a hipGraph (per precision and cycle shape; dropped when the
 // geometry changes) and replayed.
+// in_split / out_split as for precondition_cycles (eager launches only).
 template <typename T>
-int precondition(pp_ctx *c, const pp_solve_opts *o, pp_solve_stats &S) {
-  bool graph = o->use_graph && !c->prof_on && !c->adjoint_mode;
+int precondition(pp_ctx *c, const pp_solve_opts *o, pp_solve_stats &S, bool in_split = false,
+                 bool out_split = false) {
+  bool graph = graph_path(c, o);
   if (!graph) {
-    precondition_cycles<T>(c, o);
+    precondition_cycles<T>(c, o, in_split, out_split);
     S.vcycles += o->cycles_per_outer;
     return PP_OK;
   }
@@ -943,11 +957,15 @@ struct FwdOps32 {
   double *p;
   bool have_bnd;
   double cells;
+  bool rhs_split = false;  // the last orth wrote the split rhs (and zeroed the split correction)
   float *u(int q) { return c->d_kryl[q]; }
   float *z(int q) { return c->d_kryl[c->kryl_m + 1 + q]; }
-  int precondition() { return ::precondition<float>(c, o, *S); }
   // four cells per thread where the level-0 rows allow it
   bool by4() const { return c->lv[0].cx % 4 == 0; }
+  // the preconditioner's input and output stay in the colour-split copy:
+  // orth writes it, apply reads it (eager cycles, four-cell kernels)
+  bool sio() const { return by4() && split_cycles<float>(c, o) && !graph_path(c, o); }
+  int precondition() { return ::precondition<float>(c, o, *S, sio() && rhs_split, sio()); }
   CellLaunch launch() const {
     const Level &L0 = c->lv[0];
     return by4() ? cell_launch(L0.cx / 4, L0.cy, L0.cz) : ref_launch(c);
@@ -959,11 +977,15 @@ struct FwdOps32 {
   void apply(int j, const V &vv) {
     Level &L0 = c->lv[0];
     CellLaunch cl = launch();
-    ghost_sweep_level0<float>(c, lv_e<float>(L0));
+    const bool spl = sio();
+    ghost_sweep_level0<float>(c, lv_e<float>(L0), spl);
     kbegin(c, PP_K_KRYLOV, 0, cells * (4.0 + 1.0 + 4.0 + 4.0 + 4.0 * (j + 1)));
-    if (by4())
+    if (spl)
+      k_kryl_apply4<256, true><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), c->sp.e, z(j), u(j + 1), vv,
+                                                                    c->d_part, c->sp.sx, c->sp.sxy, c->sp.hp);
+    else if (by4())
       k_kryl_apply4<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), lv_e<float>(L0), z(j), u(j + 1),
-                                                              vv, c->d_part);
+                                                              vv, c->d_part, 0, 0, 0);
     else
       k_kryl_apply<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), lv_e<float>(L0), z(j), u(j + 1), vv,
                                                              c->d_part);
@@ -973,9 +995,14 @@ struct FwdOps32 {
     Level &L0 = c->lv[0];
     CellLaunch cl = launch();
     kbegin(c, PP_K_KRYLOV, 0, cells * (1.0 + 8.0 + 4.0 * (j + 1) + 8.0));
-    if (by4())
+    rhs_split = sio();
+    if (rhs_split)
+      k_kryl_orth4<256, true><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), u(j + 1), vv, c->sp.rhs,
+                                                                   c->sp.e, c->d_part, c->sp.sx, c->sp.sxy,
+                                                                   c->sp.hp);
+    else if (by4())
       k_kryl_orth4<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), u(j + 1), vv, lv_rhs<float>(L0),
-                                                             lv_e<float>(L0), c->d_part);
+                                                             lv_e<float>(L0), c->d_part, 0, 0, 0);
     else
       k_kryl_orth<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), u(j + 1), vv, lv_rhs<float>(L0),
                                                             lv_e<float>(L0), c->d_part);
@@ -1278,7 +1305,7 @@ struct AdjOps32 {
     CellLaunch cl = launch();
     kbegin(c, PP_K_KRYLOV, 0, cells * (4.0 + 1.0 + 4.0 + 4.0 + 4.0 * (j + 1)));
     k_kryl_apply4<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), lv_e<float>(L0), z(j), u(j + 1),
-                                                            vv, c->d_part);
+                                                            vv, c->d_part, 0, 0, 0);
     kend(c);
     // with no touched cells the coupling launch still writes its (zero) partial block
     kbegin(c, PP_K_GHOST, 0, (double)g.n * 6.0 * 12.0 + (double)g.nct * (8.0 + 8.0 + 4.0 * (j + 2)));
@@ -1295,7 +1322,7 @@ struct AdjOps32 {
     CellLaunch cl = launch();
     kbegin(c, PP_K_KRYLOV, 0, cells * (1.0 + 8.0 + 4.0 * (j + 1) + 8.0));
     k_kryl_orth4<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), u(j + 1), vv, lv_rhs<float>(L0),
-                                                           lv_e<float>(L0), c->d_part);
+                                                           lv_e<float>(L0), c->d_part, 0, 0, 0);
     kend(c);
     // the orthogonalisation's norm is reduced over the apply launch's blocks
     // only: zero the coupling blocks' slots it does not write

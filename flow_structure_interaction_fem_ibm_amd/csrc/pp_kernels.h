@@ -2134,10 +2134,12 @@ __global__ void __launch_bounds__(NT) k_kryl_orth(LevelArgs<double> L, float *__
 // k_kryl_apply for four consecutive cells per thread (rows whose interior
 // starts 16-byte aligned, cx % 4 == 0): vector loads of every field and all
 // dots reduced after one barrier.
-template <int NT>
+// SPL: e0 is the colour-split copy (rows sp_sx, planes sp_sxy, half-row
+// offset hp); z and w stay in the natural layout.
+template <int NT, bool SPL = false>
 __global__ void __launch_bounds__(NT) k_kryl_apply4(LevelArgs<double> L, const float *__restrict__ e0,
                                                     float *__restrict__ z, float *__restrict__ w, KryVecsF V,
-                                                    double *__restrict__ part) {
+                                                    double *__restrict__ part, int sp_sx, long sp_sxy, int hp) {
   __shared__ double s[(NT / 64) * (KRY_MAX + 1)];
   const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
   const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
@@ -2150,15 +2152,34 @@ __global__ void __launch_bounds__(NT) k_kryl_apply4(LevelArgs<double> L, const f
     float zc[4], ejm[4], ejp[4], ekm[4], ekp[4], wf[4];
     uint8_t fl[4];
     double cmx[4], cpx[4], wd[4];
-    load_chunk<float, 4>(e0 + a, 4, zc);
-    load_chunk<float, 4>(e0 + a - L.sx, 4, ejm);
-    load_chunk<float, 4>(e0 + a + L.sx, 4, ejp);
-    load_chunk<float, 4>(e0 + a - L.sxy, 4, ekm);
-    load_chunk<float, 4>(e0 + a + L.sxy, 4, ekp);
+    float el, er;
+    if constexpr (SPL) {
+      // cells i0, i0+2 at even half index i0/2, i0/2+1; i0+1, i0+3 in the odd half
+      const long se = sidx(sp_sx, sp_sxy, hp, i0, j, k);
+      auto ld = [&](long o, float(&v)[4]) {
+        const float2 ev = *reinterpret_cast<const float2 *>(e0 + se + o);
+        const float2 od = *reinterpret_cast<const float2 *>(e0 + se + hp + o);
+        v[0] = ev.x; v[1] = od.x; v[2] = ev.y; v[3] = od.y;
+      };
+      ld(0, zc);
+      ld(-sp_sx, ejm);
+      ld(sp_sx, ejp);
+      ld(-sp_sxy, ekm);
+      ld(sp_sxy, ekp);
+      el = e0[se + hp - 1];
+      er = e0[se + 2];
+    } else {
+      load_chunk<float, 4>(e0 + a, 4, zc);
+      load_chunk<float, 4>(e0 + a - L.sx, 4, ejm);
+      load_chunk<float, 4>(e0 + a + L.sx, 4, ejp);
+      load_chunk<float, 4>(e0 + a - L.sxy, 4, ekm);
+      load_chunk<float, 4>(e0 + a + L.sxy, 4, ekp);
+      el = e0[a - 1];
+      er = e0[a + 4];
+    }
     load_flags<4>(L.fl + a, 4, fl);
     load_chunk<double, 4>(L.cmx + i0, 4, cmx);
     load_chunk<double, 4>(L.cpx + i0, 4, cpx);
-    const float el = e0[a - 1], er = e0[a + 4];
     const double my = L.cmy[j], py = L.cpy[j], mz = L.cmz[k], pz = L.cpz[k];
     store_chunk<float, 4>(z + a, 4, zc);
 #pragma unroll
@@ -2226,10 +2247,12 @@ __global__ void __launch_bounds__(NT) k_kryl_adj_coupling(int nct, const long *_
 }
 
 // k_kryl_orth for four consecutive cells per thread (cx % 4 == 0)
-template <int NT>
+// SPL: rhs0 and e0 are the colour-split copies (rows sp_sx, planes sp_sxy,
+// half-row offset hp); w stays in the natural layout.
+template <int NT, bool SPL = false>
 __global__ void __launch_bounds__(NT) k_kryl_orth4(LevelArgs<double> L, float *__restrict__ w, KryVecsF V,
                                                    float *__restrict__ rhs0, float *__restrict__ e0,
-                                                   double *__restrict__ part) {
+                                                   double *__restrict__ part, int sp_sx, long sp_sxy, int hp) {
   __shared__ double s[NT / 64];
   const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
   const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
@@ -2258,9 +2281,17 @@ __global__ void __launch_bounds__(NT) k_kryl_orth4(LevelArgs<double> L, float *_
       u2 += (double)uf[t] * (double)uf[t];
     }
     store_chunk<float, 4>(w + a, 4, uf);
-    store_chunk<float, 4>(rhs0 + a, 4, uf);
-    const float zero[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    store_chunk<float, 4>(e0 + a, 4, zero);
+    if constexpr (SPL) {
+      const long se = sidx(sp_sx, sp_sxy, hp, i0, j, k);
+      *reinterpret_cast<float2 *>(rhs0 + se) = make_float2(uf[0], uf[2]);
+      *reinterpret_cast<float2 *>(rhs0 + se + hp) = make_float2(uf[1], uf[3]);
+      *reinterpret_cast<float2 *>(e0 + se) = make_float2(0.0f, 0.0f);
+      *reinterpret_cast<float2 *>(e0 + se + hp) = make_float2(0.0f, 0.0f);
+    } else {
+      store_chunk<float, 4>(rhs0 + a, 4, uf);
+      const float zero[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      store_chunk<float, 4>(e0 + a, 4, zero);
+    }
   }
   const double d = block_sum<NT>(u2, s);
   const long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
