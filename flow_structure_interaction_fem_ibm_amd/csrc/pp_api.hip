@@ -2133,8 +2133,9 @@ int time_kernel(pp_ctx *c, int cls, int level, int reps, double *avg_us, double 
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  CellLaunch cl = cell_launch(L.cx, L.cy, L.cz);
-  LevelArgs<T> a = largs<T>(L);
+  // the line passes of the FP32 level 0 run on its colour-split copy, as
+  // in the mixed solve
+  c->sp.on = std::is_same<T, float>::value && level == 0 && c->sp.ok && cls != PP_K_RESIDUAL;
   HIPCHK(hipEventRecord(e0, st));
   for (int r = 0; r < reps; r++) {
     switch (cls) {
@@ -2148,6 +2149,7 @@ int time_kernel(pp_ctx *c, int cls, int level, int reps, double *avg_us, double 
         break;
     }
   }
+  c->sp.on = false;
   HIPCHK(hipEventRecord(e1, st));
   HIPCHK(hipEventSynchronize(e1));
   HIPCHK(hipGetLastError());

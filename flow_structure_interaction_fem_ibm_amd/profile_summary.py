@@ -110,9 +110,10 @@ def pmc(paths):
     return "\n".join(out), res
 
 
-# finest-level (256x192x192) kernels of the FP32 V-cycle: class -> (name prefix, grid threads)
-FINEST = {"yline": ("pp::k_yzline<float, 8, 1", 589824), "zline": ("pp::k_yzline<float, 8, 2", 589824),
-          "xline": ("pp::k_xline<float, 32, 8>", 589824), "residual": ("pp::k_residual4<float>", 2359296)}
+# finest-level (256x192x192) kernels of the FP32 V-cycle (the line passes
+# on the colour-split copy): class -> (name prefix, grid threads)
+FINEST = {"yline": ("pp::k_yzline_split<8, 1>", 589824), "zline": ("pp::k_yzline_split<8, 2>", 589824),
+          "xline": ("pp::k_xline<float, 32, 8, true>", 589824), "residual": ("pp::k_residual4<float>", 2359296)}
 ALGORITHMIC = {"yline": 6.5, "zline": 6.5, "xline": 6.5, "residual": 13.0}  # bytes per cell per launch
 
 

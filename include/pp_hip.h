@@ -460,8 +460,10 @@ int32_t pp_setup_info(pp_ctx *ctx, double *phase_ms, int32_t *null_iterations,
  * PP_K_YLINE, PP_K_ZLINE, PP_K_RESIDUAL) on one level in the given
  * precision, bracketed by two HIP events on the context's stream;
  * *avg_us = time per launch, *bytes = algorithmic bytes per launch.
- * Overwrites that level's correction/residual work arrays (no effect on
- * later solves, which start from a fresh correction). */
+ * Level 0 in PP_PREC_MIXED times the line passes on the colour-split copy
+ * the mixed solve smooths (where the grid has one).  Overwrites that
+ * level's correction/residual work arrays (no effect on later solves,
+ * which start from a fresh correction). */
 int32_t pp_time_kernel(pp_ctx *ctx, int32_t kernel_class, int32_t level,
                        int32_t precision, int32_t reps, double *avg_us,
                        double *bytes);
