@@ -236,6 +236,8 @@ template <typename T>
 T *split_e(pp_ctx *c) { return std::is_same<T, float>::value ? reinterpret_cast<T *>(c->sp.e) : nullptr; }
 template <typename T>
 T *split_rhs(pp_ctx *c) { return std::is_same<T, float>::value ? reinterpret_cast<T *>(c->sp.rhs) : nullptr; }
+template <typename T>
+T *split_nshape(pp_ctx *c) { return std::is_same<T, float>::value ? reinterpret_cast<T *>(c->sp.nshape) : nullptr; }
 
 template <typename T>
 void launch_xline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
@@ -384,26 +386,34 @@ void adj_ghost_level0(pp_ctx *c, T *e, T *rhs) {
   GhostTables &g = c->gh;
   if (g.n == 0) return;
   hipStream_t st = c->stream;
+  // on the colour-split copy while it is in use
+  const bool spl = split_on<T>(c, 0);
+  const long *nb = spl ? g.nb_as : g.nb_al, *ct = spl ? g.ct_as : g.ct_al;
+  if (spl) {
+    e = split_e<T>(c);
+    rhs = split_rhs<T>(c);
+  }
   dim3 grid((g.n + 255) / 256), block(256);
   const double nnz = (double)g.h_ct_ghost.size();
   kbegin(c, PP_K_GHOST, 0, g.n * 6.0 * (8 + sizeof(T)) + g.n * sizeof(T) +
                               (double)g.nct * (8 + 8 + 2 * sizeof(T)) + nnz * (4 + 2 * sizeof(T)));
-  k_adj_ghost_rhs<T><<<grid, block, 0, st>>>(g.n, g.nb_al, AdjPtrs<T>::nbw(g), nullptr, e, AdjPtrs<T>::rhsg(g));
+  k_adj_ghost_rhs<T><<<grid, block, 0, st>>>(g.n, nb, AdjPtrs<T>::nbw(g), nullptr, e, AdjPtrs<T>::rhsg(g));
   if (g.nct > 0)
-    k_adj_composite<T><<<dim3((g.nct + 255) / 256), block, 0, st>>>(g.nct, g.ct_al, g.ct_ptr, g.ct_ghost,
+    k_adj_composite<T><<<dim3((g.nct + 255) / 256), block, 0, st>>>(g.nct, ct, g.ct_ptr, g.ct_ghost,
                                                                     AdjPtrs<T>::ctw(g), AdjPtrs<T>::ctvinv(g),
                                                                     AdjPtrs<T>::rhsg(g), AdjPtrs<T>::ctbase(g), rhs);
   kend(c);
 }
 
 // the rhs of the composite rows' fluid cells before a transposed V-cycle
-// (the base the transposed ghost coupling adds to)
+// (the base the transposed ghost coupling adds to; spl: from the
+// colour-split copy)
 template <typename T>
-void adj_ghost_base(pp_ctx *c) {
+void adj_ghost_base(pp_ctx *c, bool spl = false) {
   GhostTables &g = c->gh;
   if (g.nct > 0)
-    k_gather_idx<T><<<dim3((g.nct + 255) / 256), 256, 0, c->stream>>>(g.nct, g.ct_al, lv_rhs<T>(c->lv[0]),
-                                                                     AdjPtrs<T>::ctbase(g));
+    k_gather_idx<T><<<dim3((g.nct + 255) / 256), 256, 0, c->stream>>>(
+        g.nct, spl ? g.ct_as : g.ct_al, spl ? split_rhs<T>(c) : lv_rhs<T>(c->lv[0]), AdjPtrs<T>::ctbase(g));
 }
 
 // ghost rows of the level-0 correction before a smoothing step (forward:
@@ -536,11 +546,11 @@ void prolong_step(pp_ctx *c, int l) {
   const T *scale = (c->adjoint_mode && l == 0 && c->have_nshape) ? nshape<T>(c) : nullptr;
   kbegin(c, PP_K_PROLONG, l, cells * (2.0 * sizeof(T) + 1.0) + (sizeof(T) + 1.0) * C.cx * C.cy * C.cz);
   if (split_on<T>(c, l)) {
-    // forward cycles only: no null-space scale
     CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
+    const T *sscale = scale ? (const T *)split_nshape<T>(c) : nullptr;
     k_prolong2<T, true><<<cc.grid, cc.block, 0, st>>>(split_args<T>(c, L), ac, pdir<T>(L, 0).wpar,
                                                       pdir<T>(L, 1).wpar, pdir<T>(L, 2).wpar, lv_e<T>(C),
-                                                      split_e<T>(c), nullptr, c->sp.hp);
+                                                      split_e<T>(c), sscale, c->sp.hp);
   } else if (halves_by2(L, C)) {
     CellLaunch cc = cell_launch(C.cx, C.cy, C.cz);
     k_prolong2<T><<<cc.grid, cc.block, 0, st>>>(a, ac, pdir<T>(L, 0).wpar, pdir<T>(L, 1).wpar, pdir<T>(L, 2).wpar,
@@ -577,10 +587,26 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   smooth_steps<T>(c, l, o, npost);
 }
 
-// Forward FP32 line-smoothed cycles run level 0 on its colour-split copy.
+// FP32 line-smoothed cycles run level 0 on its colour-split copy.
 template <typename T>
 bool split_cycles(const pp_ctx *c, const pp_solve_opts *o) {
-  return std::is_same<T, float>::value && c->sp.ok && !c->adjoint_mode && o->smoother == PP_SMOOTH_LINE;
+  return std::is_same<T, float>::value && c->sp.ok && o->smoother == PP_SMOOTH_LINE;
+}
+void split_in(pp_ctx *c) {
+  Level &L0 = c->lv[0];
+  const CellLaunch q = cell_launch((L0.cx + 1) / 2, L0.cy, L0.cz);
+  kbegin(c, PP_K_OTHER, 0, (double)L0.cx * L0.cy * L0.cz * 12.0);
+  k_split_in<<<q.grid, q.block, 0, c->stream>>>(largs<float>(L0), split_args<float>(c, L0), c->sp.hp, L0.rhs32,
+                                                c->sp.rhs, c->sp.e);
+  kend(c);
+}
+void split_out(pp_ctx *c) {
+  Level &L0 = c->lv[0];
+  const CellLaunch q = cell_launch((L0.cx + 1) / 2, L0.cy, L0.cz);
+  kbegin(c, PP_K_OTHER, 0, (double)L0.cx * L0.cy * L0.cz * 8.0);
+  k_split_out<<<q.grid, q.block, 0, c->stream>>>(largs<float>(L0), split_args<float>(c, L0), c->sp.hp, c->sp.e,
+                                                 L0.e32);
+  kend(c);
 }
 // the preconditioner is captured into a hipGraph
 bool graph_path(const pp_ctx *c, const pp_solve_opts *o) { return o->use_graph && !c->prof_on && !c->adjoint_mode; }
@@ -593,26 +619,35 @@ bool graph_path(const pp_ctx *c, const pp_solve_opts *o) { return o->use_graph &
 template <typename T>
 void precondition_cycles(pp_ctx *c, const pp_solve_opts *o, bool in_split = false, bool out_split = false) {
   const bool spl = split_cycles<T>(c, o);
-  Level &L0 = c->lv[0];
-  const LevelArgs<float> n0 = largs<float>(L0), s0 = split_args<float>(c, L0);
-  const CellLaunch q = cell_launch((L0.cx + 1) / 2, L0.cy, L0.cz);
-  const double cells = (double)L0.cx * L0.cy * L0.cz;
   if (spl) {
-    if (!in_split) {
-      kbegin(c, PP_K_OTHER, 0, cells * 12.0);
-      k_split_in<<<q.grid, q.block, 0, c->stream>>>(n0, s0, c->sp.hp, L0.rhs32, c->sp.rhs, c->sp.e);
-      kend(c);
-    }
+    if (!in_split) split_in(c);
     c->sp.on = true;
   }
   for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) vcycle<T>(c, 0, o);
   if (spl) {
     c->sp.on = false;
-    if (out_split) return;
-    kbegin(c, PP_K_OTHER, 0, cells * 8.0);
-    k_split_out<<<q.grid, q.block, 0, c->stream>>>(n0, s0, c->sp.hp, c->sp.e, L0.e32);
-    kend(c);
+    if (!out_split) split_out(c);
   }
+}
+
+// The transposed cycles of one adjoint preconditioner application, on the
+// colour-split level 0 where it applies (in_split / out_split as for
+// precondition_cycles).
+template <typename T>
+void adjoint_cycles(pp_ctx *c, const pp_solve_opts *o, pp_solve_stats &S, bool in_split = false,
+                    bool out_split = false) {
+  const bool spl = split_cycles<T>(c, o);
+  if (spl && !in_split) split_in(c);
+  adj_ghost_base<T>(c, spl);
+  c->adjoint_mode = true;
+  c->sp.on = spl;
+  for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
+    vcycle<T>(c, 0, o);
+    S.vcycles++;
+  }
+  c->sp.on = false;
+  c->adjoint_mode = false;
+  if (spl && !out_split) split_out(c);
 }
 
 // The preconditioner: cycles_per_outer V-cycles on the level-0 rhs (e0 = 0
@@ -1223,13 +1258,7 @@ struct AdjOps {
   double *z(int q) { return c->d_kry[c->kry_m + 1 + q]; }
   int blocks() const { return c->nblk_red; }
   int precondition() {
-    adj_ghost_base<T>(c);
-    c->adjoint_mode = true;
-    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
-      vcycle<T>(c, 0, o);
-      S->vcycles++;
-    }
-    c->adjoint_mode = false;
+    adjoint_cycles<T>(c, o, *S);
     return PP_OK;
   }
   void apply(int j, const V &vv) {
@@ -1289,29 +1318,32 @@ struct AdjOps32 {
     AdjOps32 t{c, nullptr, nullptr, 0.0};
     return t.blocks() <= c->nblk_red;
   }
+  // the preconditioner's input and output stay in the colour-split copy:
+  // orth writes it, apply reads it
+  bool rhs_split = false;
+  bool sio() const { return split_cycles<float>(c, o); }
   int precondition() {
-    adj_ghost_base<float>(c);
-    c->adjoint_mode = true;
-    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
-      vcycle<float>(c, 0, o);
-      S->vcycles++;
-    }
-    c->adjoint_mode = false;
+    adjoint_cycles<float>(c, o, *S, sio() && rhs_split, sio());
     return PP_OK;
   }
   void apply(int j, const V &vv) {
     Level &L0 = c->lv[0];
     GhostTables &g = c->gh;
     CellLaunch cl = launch();
+    const bool spl = sio();
     kbegin(c, PP_K_KRYLOV, 0, cells * (4.0 + 1.0 + 4.0 + 4.0 + 4.0 * (j + 1)));
-    k_kryl_apply4<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), lv_e<float>(L0), z(j), u(j + 1),
-                                                            vv, c->d_part, 0, 0, 0);
+    if (spl)
+      k_kryl_apply4<256, true><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), c->sp.e, z(j), u(j + 1), vv,
+                                                                    c->d_part, c->sp.sx, c->sp.sxy, c->sp.hp);
+    else
+      k_kryl_apply4<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), lv_e<float>(L0), z(j), u(j + 1),
+                                                              vv, c->d_part, 0, 0, 0);
     kend(c);
     // with no touched cells the coupling launch still writes its (zero) partial block
     kbegin(c, PP_K_GHOST, 0, (double)g.n * 6.0 * 12.0 + (double)g.nct * (8.0 + 8.0 + 4.0 * (j + 2)));
     if (g.n > 0)
-      k_adj_ghost_rhs<float><<<dim3((g.n + 255) / 256), 256, 0, c->stream>>>(g.n, g.nb_al, g.nb_w32, nullptr,
-                                                                             lv_e<float>(L0), g.rhsg32);
+      k_adj_ghost_rhs<float><<<dim3((g.n + 255) / 256), 256, 0, c->stream>>>(
+          g.n, spl ? g.nb_as : g.nb_al, g.nb_w32, nullptr, spl ? c->sp.e : lv_e<float>(L0), g.rhsg32);
     k_kryl_adj_coupling<256><<<dim3((unsigned)std::max<long>(coupling_blocks(), 1)), 256, 0, c->stream>>>(
         g.nct, g.ct_al, g.ct_ptr, g.ct_ghost, g.ct_w32, g.ct_vinv32, g.rhsg32, u(j + 1), vv, c->d_part,
         apply_blocks());
@@ -1321,8 +1353,14 @@ struct AdjOps32 {
     Level &L0 = c->lv[0];
     CellLaunch cl = launch();
     kbegin(c, PP_K_KRYLOV, 0, cells * (1.0 + 8.0 + 4.0 * (j + 1) + 8.0));
-    k_kryl_orth4<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), u(j + 1), vv, lv_rhs<float>(L0),
-                                                           lv_e<float>(L0), c->d_part, 0, 0, 0);
+    rhs_split = sio();
+    if (rhs_split)
+      k_kryl_orth4<256, true><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), u(j + 1), vv, c->sp.rhs,
+                                                                   c->sp.e, c->d_part, c->sp.sx, c->sp.sxy,
+                                                                   c->sp.hp);
+    else
+      k_kryl_orth4<256><<<cl.grid, cl.block, 0, c->stream>>>(largs<double>(L0), u(j + 1), vv, lv_rhs<float>(L0),
+                                                             lv_e<float>(L0), c->d_part, 0, 0, 0);
     kend(c);
     // the orthogonalisation's norm is reduced over the apply launch's blocks
     // only: zero the coupling blocks' slots it does not write
@@ -1416,13 +1454,7 @@ int adjoint_impl(pp_ctx *c, const double *sg, const pp_solve_opts *o, double ref
       if (krc) return krc;
       continue;
     }
-    adj_ghost_base<T>(c);
-    c->adjoint_mode = true;
-    for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
-      vcycle<T>(c, 0, o);
-      S.vcycles++;
-    }
-    c->adjoint_mode = false;
+    adjoint_cycles<T>(c, o, S);
     kbegin(c, PP_K_UPDATE, 0, cells * (17.0 + sizeof(T)));
     k_update<T><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_mu, lv_e<T>(L0));
     kend(c);
@@ -1461,8 +1493,9 @@ int null_shape_arrays(pp_ctx *c) {
     HIPCHK(hipMemsetAsync(c->d_nshape32, 0, sizeof(float) * L0.total, s));
     HIPCHK(hipMemsetAsync(c->d_nshape64, 0, sizeof(double) * L0.total, s));
   }
-  k_null_shape<float><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_w, c->d_nshape32);
-  k_null_shape<double><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_w, c->d_nshape64);
+  k_null_shape<float><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_w, c->d_nshape32, c->sp.ok ? c->sp.nshape : nullptr,
+                                                   c->sp.sx, c->sp.sxy, c->sp.hp);
+  k_null_shape<double><<<cl.grid, cl.block, 0, s>>>(c->ref, a0, c->d_w, c->d_nshape64, nullptr, 0, 0, 0);
   HIPCHK(hipGetLastError());
   return PP_OK;
 }

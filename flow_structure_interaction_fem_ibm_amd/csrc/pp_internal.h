@@ -94,6 +94,7 @@ struct GhostTables {
   // forward gather (device)
   long *g_ref = nullptr, *g_al = nullptr;       // ghost cell offsets
   long *g_as = nullptr, *c_as = nullptr;        // ghost and fluid offsets in the split level 0
+  long *nb_as = nullptr, *ct_as = nullptr;      // the adjoint tables' offsets in the split level 0
   double *bself = nullptr;                      // c_self + probeLength
   double *bnd = nullptr;                        // bself * uNormal (per solve)
   // composite ghost rows: the solved ghost value as a combination of fluid
@@ -171,6 +172,7 @@ struct Split0 {
   int hp = 0, sx = 0;
   long sxy = 0, total = 0;
   float *e = nullptr, *rhs = nullptr;
+  float *nshape = nullptr;  // w/V of the null vector (the adjoint cycles' level-0 correction scale)
   uint8_t *fl = nullptr;
 };
 

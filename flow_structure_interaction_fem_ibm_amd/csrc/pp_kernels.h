@@ -755,12 +755,16 @@ __global__ void __launch_bounds__(256) k_null_init(RefArgs R, LevelArgs<double> 
 // open domain far from bodies), 1 on blank cells, in the level layout
 template <typename T>
 __global__ void __launch_bounds__(256) k_null_shape(RefArgs R, LevelArgs<double> L, const double *__restrict__ w,
-                                                    T *__restrict__ s) {
+                                                    T *__restrict__ s, T *__restrict__ s_split, int sp_sx,
+                                                    long sp_sxy, int hp) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
   if (i >= R.nxc || j >= R.nyc) return;
   const long a = lidx(L.sx, L.sxy, i, j, k);
-  s[a] = (L.fl[a] & FL_BLANK) ? T(1) : (T)(w[ridx(R, i, j, k)] / (L.wx[i] * L.wy[j] * L.wz[k]));
+  const T v = (L.fl[a] & FL_BLANK) ? T(1) : (T)(w[ridx(R, i, j, k)] / (L.wx[i] * L.wy[j] * L.wz[k]));
+  s[a] = v;
+  // the same values in the colour-split copy of level 0 (when given)
+  if (s_split) s_split[sidx(sp_sx, sp_sxy, hp, i, j, k)] = v;
 }
 
 __global__ void __launch_bounds__(256) k_null_scale(RefArgs R, LevelArgs<double> L, const double *__restrict__ mu,
@@ -1293,7 +1297,7 @@ __global__ void __launch_bounds__(256) k_prolong(LevelArgs<T> F, LevelArgs<T> C,
 // are I-1 / I+1 etc.; a partner outside the level carries weight 0).
 // Same weights, masks and renormalisation as k_prolong.  scale (fine level
 // layout, may be NULL) multiplies each cell's correction.
-// SPL: the fine level is in the colour-split row layout (sidx, hp; no scale).
+// SPL: the fine level (and scale) in the colour-split row layout (sidx, hp).
 template <typename T, bool SPL = false>
 __global__ void __launch_bounds__(256) k_prolong2(LevelArgs<T> F, LevelArgs<T> C, const T *__restrict__ wx,
                                                   const T *__restrict__ wy, const T *__restrict__ wz,
@@ -1356,8 +1360,8 @@ __global__ void __launch_bounds__(256) k_prolong2(LevelArgs<T> F, LevelArgs<T> C
             }
         if (den > T(0)) {
           const T corr = num / den;
-          if constexpr (SPL) out[ic] = out[ic] + corr;
-          else out[ic] = out[ic] + (scale ? scale[id + ic] * corr : corr);
+          const long is = SPL ? (ic ? id + hp : id) : id + ic;
+          out[ic] = out[ic] + (scale ? scale[is] * corr : corr);
         }
       }
       if constexpr (SPL) {

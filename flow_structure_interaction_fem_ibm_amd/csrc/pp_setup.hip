@@ -122,7 +122,7 @@ void free_levels(pp_ctx *c) {
     hipFree(L.e64); hipFree(L.rhs64); hipFree(L.res64);
   }
   c->lv.clear();
-  hipFree(c->sp.e); hipFree(c->sp.rhs); hipFree(c->sp.fl);
+  hipFree(c->sp.e); hipFree(c->sp.rhs); hipFree(c->sp.fl); hipFree(c->sp.nshape);
   c->sp = Split0();
 }
 
@@ -135,7 +135,7 @@ void free_ghosts(pp_ctx *c) {
                   g.tbase64, g.lam32[0], g.lam32[1], g.rhsg32, g.tbase32, g.olam64[0], g.olam64[1],
                   g.orhsg64, g.otbase64, g.v_cD, g.v_buf[0], g.v_buf[1], g.v_kind, g.v_off,
                   g.ct_ref, g.ct_al, g.ct_ptr, g.ct_ghost, g.ct_w64, g.ct_vinv64, g.ctbase64,
-                  g.ct_w32, g.ct_vinv32, g.ctbase32, g.g_as, g.c_as};
+                  g.ct_w32, g.ct_vinv32, g.ctbase32, g.g_as, g.c_as, g.nb_as, g.ct_as};
   for (void *p : ptrs) hipFree(p);
   g = GhostTables();
   hipFree(c->cs.G); hipFree(c->cs.cell);
@@ -338,7 +338,7 @@ __global__ void k_flags_split(int cx, int cy, int cz, int sx, long sxy, const ui
 // 128-byte line).
 int setup_split0(pp_ctx *c) {
   Split0 &sp = c->sp;
-  hipFree(sp.e); hipFree(sp.rhs); hipFree(sp.fl);
+  hipFree(sp.e); hipFree(sp.rhs); hipFree(sp.fl); hipFree(sp.nshape);
   sp = Split0();
   if (c->lv.size() < 2) return PP_OK;
   const Level &L = c->lv[0], &C = c->lv[1];
@@ -352,13 +352,13 @@ int setup_split0(pp_ctx *c) {
   sp.sxy = (long)sp.sx * (L.cy + 2);
   sp.total = sp.sxy * (L.cz + 2);
   if (hipMalloc(&sp.e, sp.total * 4) != hipSuccess || hipMalloc(&sp.rhs, sp.total * 4) != hipSuccess ||
-      hipMalloc(&sp.fl, sp.total) != hipSuccess) {
+      hipMalloc(&sp.nshape, sp.total * 4) != hipSuccess || hipMalloc(&sp.fl, sp.total) != hipSuccess) {
     set_error("hipMalloc failed for the colour-split level 0 (%ld cells)", sp.total);
     return PP_HIP_ERROR;
   }
   // halo cells of the split correction stay zero (only interior cells are written)
   if (hipMemset(sp.e, 0, sp.total * 4) != hipSuccess || hipMemset(sp.rhs, 0, sp.total * 4) != hipSuccess ||
-      hipMemset(sp.fl, FL_BLANK, sp.total) != hipSuccess) {
+      hipMemset(sp.nshape, 0, sp.total * 4) != hipSuccess || hipMemset(sp.fl, FL_BLANK, sp.total) != hipSuccess) {
     set_error("colour-split level 0 initialisation failed");
     return PP_HIP_ERROR;
   }
@@ -754,10 +754,13 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
       const long z = r / c->ref.exy, rem = r % c->ref.exy, y = rem / c->ref.ex, x = rem % c->ref.ex;
       return sidx(sp.sx, sp.sxy, sp.hp, (int)x - 1, (int)y - 1, (int)z - 1);
     };
-    std::vector<long> gs(G.h_g_ref.size()), cs(G.h_c_ref.size());
+    std::vector<long> gs(G.h_g_ref.size()), cs(G.h_c_ref.size()), nb(G.h_nb_ref.size()), ct(G.h_ct_ref.size());
     for (size_t q = 0; q < gs.size(); q++) gs[q] = split_of(G.h_g_ref[q]);
     for (size_t q = 0; q < cs.size(); q++) cs[q] = split_of(G.h_c_ref[q]);
+    for (size_t q = 0; q < nb.size(); q++) nb[q] = G.h_nb_ref[q] >= 0 ? split_of(G.h_nb_ref[q]) : -1;
+    for (size_t q = 0; q < ct.size(); q++) ct[q] = split_of(G.h_ct_ref[q]);
     G.g_as = dev_upload(gs); G.c_as = dev_upload(cs);
+    G.nb_as = dev_upload(nb); G.ct_as = dev_upload(ct);
   }
   G.g_ref = dev_upload(G.h_g_ref); G.g_al = dev_upload(G.h_g_al);
   G.c_ptr = dev_upload(G.h_c_ptr); G.c_ref = dev_upload(G.h_c_ref); G.c_al = dev_upload(G.h_c_al);

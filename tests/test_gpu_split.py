@@ -85,3 +85,21 @@ def test_natural_only_grid_still_solves():
     s.set_geometry(ib, gs)
     b = cases.random_field(ib.shape, 5, ib)
     assert_same(both_layouts(s, b, rel_tol=1e-10, max_outer=100))
+
+
+@pytest.mark.parametrize("label,kw", CASES[:2], ids=[c[0].split()[0] for c in CASES[:2]])
+def test_split_and_natural_adjoints_agree_bitwise(label, kw):
+    announce(f"adjoint: split vs natural finest level, {label}")
+    xn, yn, zn, ib, gs, _ = cases.sphere_case(**kw)
+    s = pp.PressureSolver(xn, yn, zn)
+    s.set_geometry(ib, gs)
+    w = cases.random_field(ib.shape, 21, ib)
+    out = {}
+    for sm in (pp.PP_SMOOTH_LINE, pp.PP_SMOOTH_LINE_NATURAL):
+        lam, st = s.adjoint(w, precision=pp.PP_PREC_MIXED, smoother=sm, rel_tol=1e-10, max_outer=100)
+        assert st["status"] == "PP_OK", (sm, st["status"], st["rel_l2"])
+        out[sm] = (lam, st)
+    (l0, s0), (l1, s1) = out[pp.PP_SMOOTH_LINE], out[pp.PP_SMOOTH_LINE_NATURAL]
+    assert s0["outer_iterations"] == s1["outer_iterations"] and s0["vcycles"] == s1["vcycles"]
+    assert s0["history"] == s1["history"]
+    assert np.array_equal(l0, l1)
