@@ -29,10 +29,22 @@ __device__ inline double shfl_down_d(double v, int d) { return __shfl_down(v, d,
 __device__ inline float shfl_down_d(float v, int d) { return __shfl_down(v, d, 64); }
 
 // chunk loads/stores of S consecutive elements (vectorised when the chunk
-// is whole and 16-byte aligned)
+// is whole: 16-byte vectors when S is a multiple of their length and the
+// chunk 16-byte aligned; FP32 chunks of an even length that is not get
+// 8-byte pairs, the chunk then 8-byte aligned)
 template <typename T, int S>
 __device__ inline void load_chunk(const T *__restrict__ p, int nvalid, T (&v)[S]) {
   constexpr int VE = 16 / sizeof(T);
+  if constexpr (sizeof(T) == 4 && S % VE != 0 && S % 2 == 0) {
+    if (nvalid == S) {
+#pragma unroll
+      for (int q = 0; q < S / 2; q++) {
+        const float2 w = reinterpret_cast<const float2 *>(p)[q];
+        v[2 * q] = w.x; v[2 * q + 1] = w.y;
+      }
+      return;
+    }
+  }
   if (S % VE == 0 && nvalid == S) {
     using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
 #pragma unroll
@@ -52,6 +64,13 @@ __device__ inline void load_chunk(const T *__restrict__ p, int nvalid, T (&v)[S]
 template <typename T, int S>
 __device__ inline void store_chunk(T *__restrict__ p, int nvalid, const T (&v)[S]) {
   constexpr int VE = 16 / sizeof(T);
+  if constexpr (sizeof(T) == 4 && S % VE != 0 && S % 2 == 0) {
+    if (nvalid == S) {
+#pragma unroll
+      for (int q = 0; q < S / 2; q++) reinterpret_cast<float2 *>(p)[q] = make_float2(v[2 * q], v[2 * q + 1]);
+      return;
+    }
+  }
   if (S % VE == 0 && nvalid == S) {
     using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
 #pragma unroll
@@ -72,6 +91,17 @@ __device__ inline void store_chunk(T *__restrict__ p, int nvalid, const T (&v)[S
 }
 template <int S>
 __device__ inline void load_flags(const uint8_t *__restrict__ p, int nvalid, uint8_t (&f)[S]) {
+  if constexpr (S % 4 != 0 && S % 2 == 0) {
+    // even chunks of other lengths: 2-byte pairs (the chunk 2-byte aligned)
+    if (nvalid == S) {
+#pragma unroll
+      for (int q = 0; q < S / 2; q++) {
+        const uint16_t w = reinterpret_cast<const uint16_t *>(p)[q];
+        f[2 * q] = w & 0xff; f[2 * q + 1] = w >> 8;
+      }
+      return;
+    }
+  }
   if (S % 4 == 0 && nvalid == S) {
 #pragma unroll
     for (int q = 0; q < S / 4; q++) {
