@@ -304,11 +304,12 @@ void launch_yzline(pp_ctx *c, Level &L, int lvl, int color, T omega) {
   kbegin(c, DIR == 1 ? PP_K_YLINE : PP_K_ZLINE, lvl, 0.5 * (double)L.cx * L.cy * L.cz * bytes_line(sizeof(T)));
   if constexpr (std::is_same<T, float>::value) {
     if (split_on<T>(c, lvl)) {
-      // at most 768 threads per block: with the split kernel's register
-      // budget (six wavefronts per SIMD) two blocks then share a CU
+      // at most 768 threads per block where tiles of 16 lines allow it: with
+      // the split kernel's register budget (six wavefronts per SIMD) two
+      // blocks then share a CU (narrower tiles cost more in coalescing)
       if (TX > 8) {
         TX = 64;
-        while (TX > 4 && TX * P > 768) TX >>= 1;
+        while (TX > 16 && TX * P > 768) TX >>= 1;
         grid = dim3((half + TX - 1) / TX, planes);
         block = dim3(TX, P);
         sh = sizeof(T) * 8 * TX * P;
