@@ -220,7 +220,7 @@ void with_line_shape(int G, int S, Fn &&fn) {
 #undef PP_CASE_S
 }
 
-// the colour-split level 0 is in use (forward FP32 V-cycles)
+// the colour-split level 0 is in use (FP32 V-cycles, forward or transposed)
 template <typename T>
 bool split_on(const pp_ctx *c, int lvl) {
   return std::is_same<T, float>::value && lvl == 0 && c->sp.on;

@@ -163,12 +163,12 @@ struct FreshRows {
   double *cc = nullptr;     // 4 per cell: their coeffGCMFreshD
 };
 
-// the colour-split copy of level 0 used by the forward FP32 V-cycles
-// (flags, correction, right-hand side; level-0 FP32 arrays stay in the
-// natural layout for everything else)
+// the colour-split copy of level 0 used by the FP32 V-cycles, forward and
+// transposed (flags, correction, right-hand side, null-space shape; the
+// level-0 FP32 arrays stay in the natural layout for everything else)
 struct Split0 {
   bool ok = false;    // level 0 qualifies (halves by 2, even x-line chunks)
-  bool on = false;    // set while a forward FP32 preconditioner runs
+  bool on = false;    // set while FP32 cycles run on the copy
   int hp = 0, sx = 0;
   long sxy = 0, total = 0;
   float *e = nullptr, *rhs = nullptr;
