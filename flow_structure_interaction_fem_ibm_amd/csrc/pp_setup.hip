@@ -803,6 +803,37 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
   return PP_OK;
 }
 
+// set-up kernels of the coarsest level's dense operator: the bordered
+// matrix's nonzeros into a zeroed dense matrix (unique offsets), the unit
+// right-hand sides of the fluid cells (column p holds a 1 at row p), and
+// the solution columns transposed into the by-output-cell layout
+// G[q*n + p] = X[p*m + q] (32x32 tiles through LDS)
+__global__ void k_coarse_scatter(long nn, const long *__restrict__ idx, const double *__restrict__ val,
+                                 double *__restrict__ a) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t < nn) a[idx[t]] = val[t];
+}
+__global__ void k_coarse_unit_rhs(long nf, const int *__restrict__ cells, int m, double *__restrict__ b) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t < nf) {
+    const long p = cells[t];
+    b[p * m + p] = 1.0;
+  }
+}
+__global__ void k_coarse_transpose(int n, int m, const double *__restrict__ x, double *__restrict__ g) {
+  __shared__ double tile[32][33];
+  const int p0 = blockIdx.x * 32, q0 = blockIdx.y * 32;
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int p = p0 + r, q = q0 + threadIdx.x;
+    if (p < n && q < n) tile[r][threadIdx.x] = x[(long)p * m + q];
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int q = q0 + r, p = p0 + threadIdx.x;
+    if (p < n && q < n) g[(long)q * n + p] = tile[threadIdx.x][r];
+  }
+}
+
 // Coarsest level: G maps the coarse rhs to the correction that solves the
 // coarse equations after removing the volume-weighted mean of the rhs on
 // every connected fluid component, with zero volume-weighted mean of the
@@ -843,30 +874,44 @@ int build_coarse(pp_ctx *c) {
     ncomp++;
   }
   int m = n + ncomp;
-  std::vector<double> A((long)m * m, 0.0);
+  // the bordered matrix's nonzeros (row-major offsets q*m + column), summed
+  // per offset; the dense matrix is formed on the device
+  std::vector<std::pair<long, double>> nz;
+  nz.reserve((size_t)8 * n + 8);
   for (int q = 0; q < n; q++) {
-    if (!fluid[q]) { A[(long)q * m + q] = 1.0; continue; }
+    if (!fluid[q]) { nz.push_back({(long)q * m + q, 1.0}); continue; }
     int i = q % L.cx, t = q / L.cx, j = t % L.cy, k = t / L.cy;
     uint8_t f = L.hflags[lidx(L.sx, L.sxy, i, j, k)];
     double amx = (f & FL_IM) ? 0 : L.cm[0][i], apx = (f & FL_IP) ? 0 : L.cp[0][i];
     double amy = (f & FL_JM) ? 0 : L.cm[1][j], apy = (f & FL_JP) ? 0 : L.cp[1][j];
     double amz = (f & FL_KM) ? 0 : L.cm[2][k], apz = (f & FL_KP) ? 0 : L.cp[2][k];
-    A[(long)q * m + q] = -(amx + apx) - (amy + apy) - (amz + apz);
-    if (amx != 0) A[(long)q * m + cid(i - 1, j, k)] += amx;
-    if (apx != 0) A[(long)q * m + cid(i + 1, j, k)] += apx;
-    if (amy != 0) A[(long)q * m + cid(i, j - 1, k)] += amy;
-    if (apy != 0) A[(long)q * m + cid(i, j + 1, k)] += apy;
-    if (amz != 0) A[(long)q * m + cid(i, j, k - 1)] += amz;
-    if (apz != 0) A[(long)q * m + cid(i, j, k + 1)] += apz;
-    A[(long)q * m + n + comp[q]] = 1.0;
+    nz.push_back({(long)q * m + q, -(amx + apx) - (amy + apy) - (amz + apz)});
+    if (amx != 0) nz.push_back({(long)q * m + cid(i - 1, j, k), amx});
+    if (apx != 0) nz.push_back({(long)q * m + cid(i + 1, j, k), apx});
+    if (amy != 0) nz.push_back({(long)q * m + cid(i, j - 1, k), amy});
+    if (apy != 0) nz.push_back({(long)q * m + cid(i, j + 1, k), apy});
+    if (amz != 0) nz.push_back({(long)q * m + cid(i, j, k - 1), amz});
+    if (apz != 0) nz.push_back({(long)q * m + cid(i, j, k + 1), apz});
+    nz.push_back({(long)q * m + n + comp[q], 1.0});
     double vol = L.w[0][i] * L.w[1][j] * L.w[2][k];
-    A[(long)(n + comp[q]) * m + q] = vol;
+    nz.push_back({(long)(n + comp[q]) * m + q, vol});
   }
+  std::stable_sort(nz.begin(), nz.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+  std::vector<long> nzi;
+  std::vector<double> nzv;
+  for (auto &e : nz) {
+    if (!nzi.empty() && nzi.back() == e.first) nzv.back() += e.second;
+    else { nzi.push_back(e.first); nzv.push_back(e.second); }
+  }
+  std::vector<int> fluid_cells;
+  for (int p = 0; p < n; p++)
+    if (fluid[p]) fluid_cells.push_back(p);
   // Explicit solution operator on the GPU: LU (rocSOLVER getrf) of the
   // bordered matrix and solves for the unit right-hand sides of the fluid
   // cells (getrs).  A is assembled row-major, i.e. column-major A^T, hence
   // the transposed solve.  Column p of the solution holds the response of
-  // every cell q to a unit rhs at p: G[p*n + q].
+  // every cell q to a unit rhs at p; the operator is kept by output cell,
+  // G[q*n + p], transposed on the device.
   if (!c->blas) {
     rocblas_handle h = nullptr;
     if (rocblas_create_handle(&h) != rocblas_status_success) {
@@ -877,57 +922,70 @@ int build_coarse(pp_ctx *c) {
   }
   rocblas_handle h = (rocblas_handle)c->blas;
   rocblas_set_stream(h, c->stream);
-  std::vector<double> B((long)m * n, 0.0);
-  for (int p = 0; p < n; p++)
-    if (fluid[p]) B[(long)p * m + p] = 1.0;
-  double *dA = dev_upload(A), *dB = dev_upload(B);
+  hipStream_t st = c->stream;
+  double *dA = nullptr, *dB = nullptr, *dG = nullptr;
+  long *dnzi = dev_upload(nzi);
+  double *dnzv = dev_upload(nzv);
+  int *dfl = dev_upload(fluid_cells);
   rocblas_int *dpiv = nullptr, *dinfo = nullptr;
+  hipMalloc(&dA, sizeof(double) * m * m);
+  hipMalloc(&dB, sizeof(double) * m * n);
+  hipMalloc(&dG, sizeof(double) * n * n);
   hipMalloc(&dpiv, sizeof(rocblas_int) * m);
   hipMalloc(&dinfo, sizeof(rocblas_int));
   int rc = PP_OK;
-  if (!dA || !dB || !dpiv || !dinfo) {
+  if (!dA || !dB || !dG || !dpiv || !dinfo || !dnzi || !dnzv || (!dfl && !fluid_cells.empty())) {
     set_error("coarse solve: device allocation failed");
     rc = PP_HIP_ERROR;
-  } else if (rocsolver_dgetrf(h, m, m, dA, m, dpiv, dinfo) != rocblas_status_success) {
-    set_error("rocsolver_dgetrf failed");
+  } else if (hipMemsetAsync(dA, 0, sizeof(double) * m * m, st) != hipSuccess ||
+             hipMemsetAsync(dB, 0, sizeof(double) * m * n, st) != hipSuccess) {
+    set_error("coarse solve: initialisation failed");
     rc = PP_HIP_ERROR;
   } else {
-    rocblas_int info = 0;
-    hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, c->stream);
-    hipStreamSynchronize(c->stream);
-    if (info != 0) {
-      set_error("coarsest-level bordered matrix is singular (getrf info %d)", (int)info);
-      rc = PP_BAD_GEOMETRY;
-    } else if (rocsolver_dgetrs(h, rocblas_operation_transpose, m, n, dA, m, dpiv, dB, m) !=
-               rocblas_status_success) {
-      set_error("rocsolver_dgetrs failed");
+    const long nn = (long)nzi.size(), nf = (long)fluid_cells.size();
+    k_coarse_scatter<<<(unsigned)((nn + 255) / 256), 256, 0, st>>>(nn, dnzi, dnzv, dA);
+    if (nf > 0) k_coarse_unit_rhs<<<(unsigned)((nf + 255) / 256), 256, 0, st>>>(nf, dfl, m, dB);
+    if (hipGetLastError() != hipSuccess) {
+      set_error("coarse solve: assembly kernels failed to launch");
       rc = PP_HIP_ERROR;
-    }
-  }
-  if (rc == PP_OK) {
-    // solution column p (response to a unit rhs at p) -> G[q*n + p]
-    // the solves run on the context's (non-blocking) stream: copy on it
-    std::vector<double> X((long)m * n), Gt((long)n * n);
-    if (hipMemcpyAsync(X.data(), dB, sizeof(double) * X.size(), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess) {
-      set_error("coarse solve: copy failed");
+    } else if (rocsolver_dgetrf(h, m, m, dA, m, dpiv, dinfo) != rocblas_status_success) {
+      set_error("rocsolver_dgetrf failed");
       rc = PP_HIP_ERROR;
     } else {
-      for (int p = 0; p < n; p++)
-        for (int q = 0; q < n; q++) Gt[(long)q * n + p] = X[(long)p * m + q];
-      hipFree(c->cs.G); hipFree(c->cs.cell);
-      c->cs = CoarseSolve();
-      c->cs.n = n;
-      c->cs.G = dev_upload(Gt);
-      std::vector<int> off(n);
-      for (int q = 0; q < n; q++) off[q] = (int)lidx(L.sx, L.sxy, q % L.cx, (q / L.cx) % L.cy, q / (L.cx * L.cy));
-      c->cs.cell = dev_upload(off);
-      if (!c->cs.G || !c->cs.cell) {
-        set_error("coarse solve: device allocation failed");
+      rocblas_int info = 0;
+      hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st);
+      hipStreamSynchronize(st);
+      if (info != 0) {
+        set_error("coarsest-level bordered matrix is singular (getrf info %d)", (int)info);
+        rc = PP_BAD_GEOMETRY;
+      } else if (rocsolver_dgetrs(h, rocblas_operation_transpose, m, n, dA, m, dpiv, dB, m) !=
+                 rocblas_status_success) {
+        set_error("rocsolver_dgetrs failed");
         rc = PP_HIP_ERROR;
       }
     }
   }
+  if (rc == PP_OK) {
+    // solution column p (response to a unit rhs at p) -> G[q*n + p]
+    dim3 tg((n + 31) / 32, (n + 31) / 32), tb(32, 8);
+    k_coarse_transpose<<<tg, tb, 0, st>>>(n, m, dB, dG);
+    std::vector<int> off(n);
+    for (int q = 0; q < n; q++) off[q] = (int)lidx(L.sx, L.sxy, q % L.cx, (q / L.cx) % L.cy, q / (L.cx * L.cy));
+    int *dcell = dev_upload(off);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess || !dcell) {
+      set_error("coarse solve: transpose failed");
+      hipFree(dcell);
+      rc = PP_HIP_ERROR;
+    } else {
+      hipFree(c->cs.G); hipFree(c->cs.cell);
+      c->cs = CoarseSolve();
+      c->cs.n = n;
+      c->cs.G = dG;
+      c->cs.cell = dcell;
+      dG = nullptr;
+    }
+  }
+  hipFree(dnzi); hipFree(dnzv); hipFree(dfl); hipFree(dG);
   hipFree(dA); hipFree(dB); hipFree(dpiv); hipFree(dinfo);
   return rc;
 }
