@@ -460,15 +460,17 @@ void jacobi_pass(pp_ctx *c, int l, const T *src, T *dst, T omega) {
 }
 
 // `steps` smoothing steps on level l (the ghost rows first on level 0 when
-// ghosts is set).  Point Jacobi alternates between e and the level's
-// residual buffer (free while smoothing) and copies back after an odd count.
+// ghosts is set; zero_first: the correction is zero on entry, so the first
+// step's ghost rows, homogeneous and linear in it, would change nothing).
+// Point Jacobi alternates between e and the level's residual buffer (free
+// while smoothing) and copies back after an odd count.
 template <typename T>
-void smooth_steps(pp_ctx *c, int l, const pp_solve_opts *o, int steps, bool ghosts = true) {
+void smooth_steps(pp_ctx *c, int l, const pp_solve_opts *o, int steps, bool ghosts = true, bool zero_first = false) {
   Level &L = c->lv[l];
   const T om = (T)o->omega;
   if (o->smoother != PP_SMOOTH_POINT_JACOBI) {
     for (int s = 0; s < steps; s++) {
-      if (l == 0 && ghosts) ghost_step_level0<T>(c, lv_e<T>(L), lv_rhs<T>(L));
+      if (l == 0 && ghosts && !(s == 0 && zero_first)) ghost_step_level0<T>(c, lv_e<T>(L), lv_rhs<T>(L));
       line_passes<T>(c, l, om);
     }
     return;
@@ -476,7 +478,7 @@ void smooth_steps(pp_ctx *c, int l, const pp_solve_opts *o, int steps, bool ghos
   T *buf[2] = {lv_e<T>(L), lv_res<T>(L)};
   int cur = 0;
   for (int s = 0; s < steps; s++) {
-    if (l == 0 && ghosts) ghost_step_level0<T>(c, buf[cur], lv_rhs<T>(L));
+    if (l == 0 && ghosts && !(s == 0 && zero_first)) ghost_step_level0<T>(c, buf[cur], lv_rhs<T>(L));
     jacobi_pass<T>(c, l, buf[cur], buf[1 - cur], om);
     cur ^= 1;
   }
@@ -590,7 +592,11 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
   }
   const int npre = (l > 0 && o->nu_coarse > 0) ? o->nu_coarse : o->nu_pre;
   const int npost = (l > 0 && o->nu_coarse > 0) ? o->nu_coarse : o->nu_post;
-  smooth_steps<T>(c, l, o, npre);
+  // the first cycle of a preconditioner application starts from a zero
+  // level-0 correction (every caller zeroes it)
+  const bool zero = l == 0 && c->e0_zero;
+  if (l == 0) c->e0_zero = false;
+  smooth_steps<T>(c, l, o, npre, true, zero);
   restrict_step<T>(c, l);
   vcycle<T>(c, l + 1, o);
   prolong_step<T>(c, l);
@@ -633,7 +639,9 @@ void precondition_cycles(pp_ctx *c, const pp_solve_opts *o, bool in_split = fals
     if (!in_split) split_in(c);
     c->sp.on = true;
   }
+  c->e0_zero = true;
   for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) vcycle<T>(c, 0, o);
+  c->e0_zero = false;
   if (spl) {
     c->sp.on = false;
     if (!out_split) split_out(c);
@@ -651,10 +659,12 @@ void adjoint_cycles(pp_ctx *c, const pp_solve_opts *o, pp_solve_stats &S, bool i
   adj_ghost_base<T>(c, spl);
   c->adjoint_mode = true;
   c->sp.on = spl;
+  c->e0_zero = true;
   for (int cyc = 0; cyc < o->cycles_per_outer; cyc++) {
     vcycle<T>(c, 0, o);
     S.vcycles++;
   }
+  c->e0_zero = false;
   c->sp.on = false;
   c->adjoint_mode = false;
   if (spl && !out_split) split_out(c);

@@ -214,6 +214,7 @@ struct pp_ctx {
   double *h_res = nullptr;             // pinned host mirror
   int nblk_red = 0;
   bool adjoint_mode = false;
+  bool e0_zero = false;                // the level-0 correction is zero (start of a preconditioner application)
   double setup_ms[4] = {};             // last pp_set_geometry: flags, ghost tables, coarse operator, null vector
   // w/V of the null vector in the level-0 layout (FP32, FP64): the adjoint
   // V-cycle scales its level-0 coarse-grid correction by it
