@@ -634,20 +634,38 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
       merge(cgf[q]);
       G.h_gg_ptr.push_back((int)G.h_gg_idx.size());
     }
-    std::vector<Row> cur = cgf, nxt(n);
+    // M = S C_gf with S = sum_{t < kexact} C_gg^t: S's rows (over ghosts,
+    // a few hops of the ghost layer) by the same recursion, S_t = I + C_gg
+    // S_{t-1}, then every row of M once from the fluid rows of C_gf
+    std::vector<Row> cur(n);
     bool coupled = !G.h_gg_idx.empty();
-    for (int t = 1; coupled && t < G.kexact; t++) {
-      // rows are independent within a step: ghosts in parallel
+    if (coupled) {
+      std::vector<Row> gcur(n), gnxt(n);
+      for (int q = 0; q < n; q++) gcur[q] = {{(long)q, 1.0}};
+      for (int t = 1; t < G.kexact; t++) {
+        // rows are independent within a step: ghosts in parallel
+        host_parallel(n, [&](long a, long b) {
+          for (long q = a; q < b; q++) {
+            Row r{{q, 1.0}};
+            for (int e = G.h_gg_ptr[q]; e < G.h_gg_ptr[q + 1]; e++)
+              for (auto &x : gcur[G.h_gg_idx[e]]) r.push_back({x.first, G.h_gg_w[e] * x.second});
+            merge(r);
+            gnxt[q].swap(r);
+          }
+        });
+        gcur.swap(gnxt);
+      }
       host_parallel(n, [&](long a, long b) {
         for (long q = a; q < b; q++) {
-          Row r = cgf[q];
-          for (int e = G.h_gg_ptr[q]; e < G.h_gg_ptr[q + 1]; e++)
-            for (auto &x : cur[G.h_gg_idx[e]]) r.push_back({x.first, G.h_gg_w[e] * x.second});
+          Row r;
+          for (auto &gx : gcur[q])
+            for (auto &x : cgf[gx.first]) r.push_back({x.first, gx.second * x.second});
           merge(r);
-          nxt[q].swap(r);
+          cur[q].swap(r);
         }
       });
-      cur.swap(nxt);
+    } else {
+      cur = cgf;
     }
     G.h_c_ptr.assign(1, 0); G.h_c_ref.clear(); G.h_c_al.clear(); G.h_c_w.clear();
     for (int q = 0; q < n; q++) {
