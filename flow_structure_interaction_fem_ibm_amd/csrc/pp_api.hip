@@ -2186,10 +2186,10 @@ int time_kernel(pp_ctx *c, int cls, int level, int reps, double *avg_us, double 
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
   // the line passes of the FP32 level 0 run on its colour-split copy, as
   // in the mixed solve
   c->sp.on = std::is_same<T, float>::value && level == 0 && c->sp.ok && cls != PP_K_RESIDUAL;
-  HIPCHK(hipEventRecord(e0, st));
   for (int r = 0; r < reps; r++) {
     switch (cls) {
       case PP_K_XLINE: launch_xline<T>(c, L, level, r & 1, T(1)); break;
