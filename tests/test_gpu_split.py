@@ -36,12 +36,16 @@ def assert_same(out):
     assert np.array_equal(p0, p1)
 
 
-# (label, sphere_case arguments): x-line chunks of 6 (96 and 130 cells) and
-# 8 (256 cells) cells per lane; 130x98x82 halves to odd counts below
+# (label, sphere_case arguments): x-line chunks of 2 (32 cells), 4 (64), 6
+# (96 and 130) and 8 (256) cells per lane; y/z segments of 1, 2, 4 and 8
+# cells (16-, 24-, 48- and 192-cell lines) and 64 segments (512-cell lines,
+# 16-line tiles of 1024 threads); 130x98x82 halves to odd counts below
 CASES = [
     ("96x64x48 stretched sphere", dict(n=96, ny=64, nz=48, r=0.2, stretch=1.5)),
     ("130x98x82 stretched sphere", dict(n=130, ny=98, nz=82, r=0.23, c=(0.45, 0.5, 0.55), stretch=1.2)),
     ("256x192x192 stretched sphere", dict(n=256, ny=192, nz=192, r=0.2, c=(0.4, 0.5, 0.5), stretch=2.0)),
+    ("64x24x16 stretched sphere", dict(n=64, ny=24, nz=16, r=0.3, stretch=1.2)),
+    ("32x512x64 stretched sphere", dict(n=32, ny=512, nz=64, r=0.25, stretch=1.5)),
 ]
 
 
