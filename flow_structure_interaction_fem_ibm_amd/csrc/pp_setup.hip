@@ -559,13 +559,15 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     // values (a hash of the offset), solid cells 0
     std::vector<int> sg(8L * n);
     std::vector<double> sv(8L * n), gv(n, 0.0), nw(n);
-    for (long e = 0; e < 8L * n; e++) {
-      long r = G.h_s_ref[e];
-      sg[e] = ghost_of(r);
-      uint64_t h = (uint64_t)r * 0x9E3779B97F4A7C15ull + 12345u;
-      h ^= h >> 31; h *= 0xBF58476D1CE4E5B9ull; h ^= h >> 27; h *= 0x94D049BB133111EBull; h ^= h >> 33;
-      sv[e] = c->h_iblank[r] ? 0.0 : (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-    }
+    host_parallel(8L * n, [&](long e0, long e1) {
+      for (long e = e0; e < e1; e++) {
+        long r = G.h_s_ref[e];
+        sg[e] = ghost_of(r);
+        uint64_t h = (uint64_t)r * 0x9E3779B97F4A7C15ull + 12345u;
+        h ^= h >> 31; h *= 0xBF58476D1CE4E5B9ull; h ^= h >> 27; h *= 0x94D049BB133111EBull; h ^= h >> 33;
+        sv[e] = c->h_iblank[r] ? 0.0 : (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+      }
+    });
     int it = 0;
     double dmax = 1.0;
     while (it < 200 && dmax > 1e-15) {
@@ -632,28 +634,40 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     std::vector<Row> cur(n);
     bool coupled = !G.h_gg_idx.empty();
     if (coupled) {
+      // each thread gathers into one reused buffer, sorts it by offset and
+      // sums equal offsets in place (merge's order), then copies the row
+      // into its own storage (the capacity the step before last left)
+      auto merge_into = [](Row &buf, Row &out) {
+        std::sort(buf.begin(), buf.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        size_t m = 0;
+        for (size_t i = 0; i < buf.size(); i++) {
+          if (m > 0 && buf[m - 1].first == buf[i].first) buf[m - 1].second += buf[i].second;
+          else buf[m++] = buf[i];
+        }
+        out.assign(buf.begin(), buf.begin() + m);
+      };
       std::vector<Row> gcur(n), gnxt(n);
       for (int q = 0; q < n; q++) gcur[q] = {{(long)q, 1.0}};
       for (int t = 1; t < G.kexact; t++) {
         // rows are independent within a step: ghosts in parallel
         host_parallel(n, [&](long a, long b) {
+          Row buf;
           for (long q = a; q < b; q++) {
-            Row r{{q, 1.0}};
+            buf.assign(1, {q, 1.0});
             for (int e = G.h_gg_ptr[q]; e < G.h_gg_ptr[q + 1]; e++)
-              for (auto &x : gcur[G.h_gg_idx[e]]) r.push_back({x.first, G.h_gg_w[e] * x.second});
-            merge(r);
-            gnxt[q].swap(r);
+              for (auto &x : gcur[G.h_gg_idx[e]]) buf.push_back({x.first, G.h_gg_w[e] * x.second});
+            merge_into(buf, gnxt[q]);
           }
         });
         gcur.swap(gnxt);
       }
       host_parallel(n, [&](long a, long b) {
+        Row buf;
         for (long q = a; q < b; q++) {
-          Row r;
+          buf.clear();
           for (auto &gx : gcur[q])
-            for (auto &x : cgf[gx.first]) r.push_back({x.first, gx.second * x.second});
-          merge(r);
-          cur[q].swap(r);
+            for (auto &x : cgf[gx.first]) buf.push_back({x.first, gx.second * x.second});
+          merge_into(buf, cur[q]);
         }
       });
     } else {
