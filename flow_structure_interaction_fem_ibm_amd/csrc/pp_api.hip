@@ -585,8 +585,12 @@ void vcycle(pp_ctx *c, int l, const pp_solve_opts *o) {
     // the dense FP64 solution operator in both precisions (the coarsest
     // solve is one of the FP64 parts of the mixed scheme)
     kbegin(c, PP_K_COARSE, l, 8.0 * n * n + 2.0 * sizeof(T) * n);
-    k_coarse_mv<T, double><<<(n + kCoarseRows - 1) / kCoarseRows, 64 * kCoarseRows, sizeof(double) * n, st>>>(n, cs.cell, cs.G, lv_rhs<T>(L),
-                                                                        lv_e<T>(L));
+    // rows round-robin over kCoarseBlocks blocks of enough wavefronts for
+    // one row each (up to kCoarseMaxWaves; beyond that each loops)
+    const int nb = n < kCoarseBlocks ? n : kCoarseBlocks;
+    int nw = (n + nb - 1) / nb;
+    if (nw > kCoarseMaxWaves) nw = kCoarseMaxWaves;
+    k_coarse_mv<T, double><<<nb, 64 * nw, sizeof(double) * n, st>>>(n, cs.cell, cs.G, lv_rhs<T>(L), lv_e<T>(L));
     kend(c);
     return;
   }

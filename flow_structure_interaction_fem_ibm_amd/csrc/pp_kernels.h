@@ -1932,35 +1932,40 @@ __global__ void __launch_bounds__(256) k_split_out(LevelArgs<float> N, LevelArgs
 // accumulation); cell[q] = offset of coarse cell q (i fastest)
 // in the level layout.  One wavefront per output cell streams
 // its row of GT (coalesced) against the rhs staged in LDS, then reduces
-// across the wavefront in a fixed order.
+// across the wavefront in a fixed order.  Rows go round-robin over the
+// blocks (row q to block q % gridDim.x), so a grid of one block per CU
+// gives every CU the same number of rows.
 // ---------------------------------------------------------------------
-constexpr int kCoarseRows = 8;  // output cells (wavefronts) per block
+constexpr int kCoarseBlocks = 256;   // blocks of the launch (one per CU)
+constexpr int kCoarseMaxWaves = 16;  // wavefronts per block at most
 template <typename T, typename TG>
-__global__ void __launch_bounds__(64 * kCoarseRows) k_coarse_mv(int n, const int *__restrict__ cell,
-                                                                const TG *__restrict__ GT,
-                                                                const T *__restrict__ rhs, T *__restrict__ e) {
+__global__ void __launch_bounds__(64 * kCoarseMaxWaves) k_coarse_mv(int n, const int *__restrict__ cell,
+                                                                    const TG *__restrict__ GT,
+                                                                    const T *__restrict__ rhs, T *__restrict__ e) {
   extern __shared__ unsigned char smem_raw[];
   double *r = reinterpret_cast<double *>(smem_raw);
-  for (int p = threadIdx.x; p < n; p += 64 * kCoarseRows) r[p] = (double)rhs[cell[p]];
+  for (int p = threadIdx.x; p < n; p += blockDim.x) r[p] = (double)rhs[cell[p]];
   __syncthreads();
-  const int q = blockIdx.x * kCoarseRows + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (q >= n) return;  // uniform per wavefront
-  const TG *row = GT + (long)q * n;
-  // four independent row loads in flight per lane, four partial sums added
-  // in a fixed order
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int p = lane;
-  for (; p + 192 < n; p += 256) {
-    const double a0 = (double)row[p], a1 = (double)row[p + 64], a2 = (double)row[p + 128], a3 = (double)row[p + 192];
-    s0 += a0 * r[p];
-    s1 += a1 * r[p + 64];
-    s2 += a2 * r[p + 128];
-    s3 += a3 * r[p + 192];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  // q is uniform per wavefront
+  for (int q = wv * gridDim.x + blockIdx.x; q < n; q += nwv * gridDim.x) {
+    const TG *row = GT + (long)q * n;
+    // four independent row loads in flight per lane, four partial sums added
+    // in a fixed order
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int p = lane;
+    for (; p + 192 < n; p += 256) {
+      const double a0 = (double)row[p], a1 = (double)row[p + 64], a2 = (double)row[p + 128], a3 = (double)row[p + 192];
+      s0 += a0 * r[p];
+      s1 += a1 * r[p + 64];
+      s2 += a2 * r[p + 128];
+      s3 += a3 * r[p + 192];
+    }
+    for (; p < n; p += 64) s0 += (double)row[p] * r[p];
+    double s = (s0 + s1) + (s2 + s3);
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_down(s, d, 64);
+    if (lane == 0) e[cell[q]] = (T)s;
   }
-  for (; p < n; p += 64) s0 += (double)row[p] * r[p];
-  double s = (s0 + s1) + (s2 + s3);
-  for (int d = 32; d >= 1; d >>= 1) s += __shfl_down(s, d, 64);
-  if (lane == 0) e[cell[q]] = (T)s;
 }
 
 // ---------------------------------------------------------------------
