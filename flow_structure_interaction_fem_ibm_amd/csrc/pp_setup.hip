@@ -692,41 +692,62 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
   {
     auto V = [&](int i, int j, int k) { return c->rdx[0][i] * c->rdx[1][j] * c->rdx[2][k]; };
     G.h_nb_ref.assign(6L * n, -1); G.h_nb_al.assign(6L * n, -1); G.h_nb_w.assign(6L * n, 0.0);
-    for (int q = 0; q < n; q++) {
-      int gi = G.h_ig[q], gj = G.h_jg[q], gk = G.h_kg[q];   // 1-based
-      int nbs[6][3] = {{gi - 1, gj, gk}, {gi + 1, gj, gk}, {gi, gj - 1, gk},
-                       {gi, gj + 1, gk}, {gi, gj, gk - 1}, {gi, gj, gk + 1}};
-      for (int m = 0; m < 6; m++) {
-        int i = nbs[m][0], j = nbs[m][1], k = nbs[m][2];
-        if (!inb(i, j, k)) continue;
-        long r = ridx(c->ref, i - 1, j - 1, k - 1);
-        if (c->h_iblank[r]) continue;
-        // the neighbour sits on side m of the ghost; the ghost is on the
-        // opposite side of the neighbour
-        double coef;
-        switch (m) {
-          case 0: coef = c->rdxcinv[0][i + 1] * c->rdxinv[0][i]; break;   // ghost is i+1 of f
-          case 1: coef = c->rdxcinv[0][i] * c->rdxinv[0][i]; break;       // ghost is i-1 of f
-          case 2: coef = c->rdxcinv[1][j + 1] * c->rdxinv[1][j]; break;
-          case 3: coef = c->rdxcinv[1][j] * c->rdxinv[1][j]; break;
-          case 4: coef = c->rdxcinv[2][k + 1] * c->rdxinv[2][k]; break;
-          default: coef = c->rdxcinv[2][k] * c->rdxinv[2][k]; break;
+    // each ghost fills its own six slots: ghosts in parallel
+    host_parallel(n, [&](long qa, long qb) {
+      for (int q = (int)qa; q < (int)qb; q++) {
+        int gi = G.h_ig[q], gj = G.h_jg[q], gk = G.h_kg[q];   // 1-based
+        int nbs[6][3] = {{gi - 1, gj, gk}, {gi + 1, gj, gk}, {gi, gj - 1, gk},
+                         {gi, gj + 1, gk}, {gi, gj, gk - 1}, {gi, gj, gk + 1}};
+        for (int m = 0; m < 6; m++) {
+          int i = nbs[m][0], j = nbs[m][1], k = nbs[m][2];
+          if (!inb(i, j, k)) continue;
+          long r = ridx(c->ref, i - 1, j - 1, k - 1);
+          if (c->h_iblank[r]) continue;
+          // the neighbour sits on side m of the ghost; the ghost is on the
+          // opposite side of the neighbour
+          double coef;
+          switch (m) {
+            case 0: coef = c->rdxcinv[0][i + 1] * c->rdxinv[0][i]; break;   // ghost is i+1 of f
+            case 1: coef = c->rdxcinv[0][i] * c->rdxinv[0][i]; break;       // ghost is i-1 of f
+            case 2: coef = c->rdxcinv[1][j + 1] * c->rdxinv[1][j]; break;
+            case 3: coef = c->rdxcinv[1][j] * c->rdxinv[1][j]; break;
+            case 4: coef = c->rdxcinv[2][k + 1] * c->rdxinv[2][k]; break;
+            default: coef = c->rdxcinv[2][k] * c->rdxinv[2][k]; break;
+          }
+          G.h_nb_ref[6L * q + m] = r;
+          G.h_nb_al[6L * q + m] = lidx(L0.sx, L0.sxy, i - 1, j - 1, k - 1);
+          G.h_nb_w[6L * q + m] = coef * V(i, j, k);
         }
-        G.h_nb_ref[6L * q + m] = r;
-        G.h_nb_al[6L * q + m] = lidx(L0.sx, L0.sxy, i - 1, j - 1, k - 1);
-        G.h_nb_w[6L * q + m] = coef * V(i, j, k);
       }
+    });
+    // touched cells: CSR of (ghost, coefficient) by stencil cell; the
+    // entries in ghost-major order (which stencil entries a ghost keeps is
+    // marked in parallel, then each ghost writes from its prefix offset)
+    std::vector<uint8_t> keep(8L * n, 0);
+    host_parallel(n, [&](long qa, long qb) {
+      for (long q = qa; q < qb; q++)
+        for (int r = 0; r < 8; r++) {
+          const double cf = G.h_coef[8L * q + r];
+          const long rs = G.h_s_ref[8L * q + r];
+          if (cf == 0.0 || rs == G.h_g_ref[q]) continue;
+          if (c->h_iblank[rs] == 1 && !ref_is_ghost(c, rs)) continue;  // solid cells are not unknowns
+          keep[8L * q + r] = 1;
+        }
+    });
+    std::vector<long> eoff(n + 1, 0);
+    for (int q = 0; q < n; q++) {
+      int kq = 0;
+      for (int r = 0; r < 8; r++) kq += keep[8L * q + r];
+      eoff[q + 1] = eoff[q] + kq;
     }
-    // touched cells: CSR of (ghost, coefficient) by stencil cell
-    std::vector<std::pair<long, std::pair<int, double>>> ent;
-    for (int q = 0; q < n; q++)
-      for (int r = 0; r < 8; r++) {
-        double cf = G.h_coef[8L * q + r];
-        long rs = G.h_s_ref[8L * q + r];
-        if (cf == 0.0 || rs == G.h_g_ref[q]) continue;
-        if (c->h_iblank[rs] == 1 && !ref_is_ghost(c, rs)) continue;  // solid cells are not unknowns
-        ent.push_back({rs, {q, cf}});
+    std::vector<std::pair<long, std::pair<int, double>>> ent(eoff[n]);
+    host_parallel(n, [&](long qa, long qb) {
+      for (long q = qa; q < qb; q++) {
+        long o = eoff[q];
+        for (int r = 0; r < 8; r++)
+          if (keep[8L * q + r]) ent[o++] = {G.h_s_ref[8L * q + r], {(int)q, G.h_coef[8L * q + r]}};
       }
+    });
     par_stable_sort(ent, [](const auto &a, const auto &b) { return a.first < b.first; });
     G.h_t_ref.clear(); G.h_t_al.clear(); G.h_t_ptr.assign(1, 0); G.h_t_ghost.clear(); G.h_t_coef.clear();
     G.h_t_vinv.clear();
@@ -750,9 +771,12 @@ int build_ghosts(pp_ctx *c, const pp_geometry *g) {
     if (!ent.empty()) G.h_t_ptr.push_back((int)G.h_t_ghost.size());
     G.nt = (int)G.h_t_ref.size();
     // transpose of the composite rows by fluid cell
-    std::vector<std::pair<long, std::pair<int, double>>> ct;
-    for (int q = 0; q < n; q++)
-      for (int e = G.h_c_ptr[q]; e < G.h_c_ptr[q + 1]; e++) ct.push_back({G.h_c_ref[e], {q, G.h_c_w[e]}});
+    // (entry e of ghost q's composite row goes to slot e: ghosts in parallel)
+    std::vector<std::pair<long, std::pair<int, double>>> ct(G.h_c_ptr.empty() ? 0 : G.h_c_ptr[n]);
+    host_parallel(n, [&](long qa, long qb) {
+      for (long q = qa; q < qb; q++)
+        for (int e = G.h_c_ptr[q]; e < G.h_c_ptr[q + 1]; e++) ct[e] = {G.h_c_ref[e], {(int)q, G.h_c_w[e]}};
+    });
     par_stable_sort(ct, [](const auto &a, const auto &b) { return a.first < b.first; });
     G.h_ct_ref.clear(); G.h_ct_al.clear(); G.h_ct_ptr.assign(1, 0); G.h_ct_ghost.clear(); G.h_ct_w.clear();
     G.h_ct_vinv.clear();
